@@ -1,0 +1,929 @@
+/*
+ * ORACLE — test infrastructure only (see oracle.h).
+ *
+ * CPU restatement of the reference's gate-sweep path.  Every function names
+ * the reference file:line range whose algorithm (and, for parity, whose
+ * summation order) it follows.  Built with plain gcc -O3 (no -ffast-math, no
+ * FMA), like the reference itself.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+static void* xalloc(size_t n)
+{
+	void* p = malloc(n > 0 ? n : 1);
+	if (p == NULL) {
+		fprintf(stderr, "oracle: allocation of %zu bytes failed\n", n);
+		abort();
+	}
+	return p;
+}
+
+/* exchange the roles of the two wires of a gate (rows and columns of |q_i q_j>):
+   reference src/gate.c:17-27 */
+static void swap_gate_wires(const ocplx* g, ocplx* gp)
+{
+	static const int p[4] = { 0, 2, 1, 3 };
+	for (int r = 0; r < 4; r++) {
+		for (int c = 0; c < 4; c++) {
+			gp[4*r + c] = g[4*p[r] + p[c]];
+		}
+	}
+}
+
+/* base index of quad number q for qubits i < j: zero bits inserted at
+   positions L-1-i (high) and L-1-j (low) */
+static inline long quad_base(long q, int L, int i, int j)
+{
+	const int lo = L - 1 - j, hi = L - 1 - i;
+	long x = ((q >> lo) << (lo + 1)) | (q & ((1L << lo) - 1));
+	x = ((x >> hi) << (hi + 1)) | (x & ((1L << hi) - 1));
+	return x;
+}
+
+/* ---------------------------------------------------------------------------
+ * Gate application, reference src/gate.c:10-76.  For j < i the gate is
+ * re-indexed and applied with the wires exchanged (:17-30).  Quads are visited
+ * in increasing base index, matching the (a, b, c) loop nest (:31-75).
+ */
+void orc_apply_gate(const ocplx* gate, int i, int j, int L, const ocplx* psi, ocplx* psi_out)
+{
+	if (j < i) {
+		ocplx gp[16];
+		swap_gate_wires(gate, gp);
+		orc_apply_gate(gp, j, i, L, psi, psi_out);
+		return;
+	}
+	const long nq = 1L << (L - 2);
+	const long si = 1L << (L - 1 - i), sj = 1L << (L - 1 - j);
+	for (long q = 0; q < nq; q++) {
+		const long k = quad_base(q, L, i, j);
+		const ocplx x = psi[k], y = psi[k + sj], z = psi[k + si], w = psi[k + si + sj];
+		psi_out[k]           = gate[ 0]*x + gate[ 1]*y + gate[ 2]*z + gate[ 3]*w;
+		psi_out[k + sj]      = gate[ 4]*x + gate[ 5]*y + gate[ 6]*z + gate[ 7]*w;
+		psi_out[k + si]      = gate[ 8]*x + gate[ 9]*y + gate[10]*z + gate[11]*w;
+		psi_out[k + si + sj] = gate[12]*x + gate[13]*y + gate[14]*z + gate[15]*w;
+	}
+}
+
+/* ---------------------------------------------------------------------------
+ * Backward pass of one gate, reference src/gate.c:83-188:
+ *   dpsi = gate^T dpsi_out;  dgate = sum over quads of dpsi_out (x) psi
+ * (holomorphic convention, no conjugation).  dgate is zeroed and then
+ * accumulated quad by quad; for j < i it is un-permuted at the end (:110-113).
+ */
+void orc_apply_gate_backward(const ocplx* gate, int i, int j, int L, const ocplx* psi,
+	const ocplx* dpsi_out, ocplx* dpsi, ocplx* dgate)
+{
+	if (j < i) {
+		ocplx gp[16], dgp[16];
+		swap_gate_wires(gate, gp);
+		orc_apply_gate_backward(gp, j, i, L, psi, dpsi_out, dpsi, dgp);
+		swap_gate_wires(dgp, dgate);
+		return;
+	}
+	for (int e = 0; e < 16; e++) { dgate[e] = 0; }
+	const long nq = 1L << (L - 2);
+	const long si = 1L << (L - 1 - i), sj = 1L << (L - 1 - j);
+	for (long q = 0; q < nq; q++) {
+		const long k = quad_base(q, L, i, j);
+		const long idx[4] = { k, k + sj, k + si, k + si + sj };
+		ocplx v[4], d[4];
+		for (int t = 0; t < 4; t++) { v[t] = psi[idx[t]]; d[t] = dpsi_out[idx[t]]; }
+		for (int r = 0; r < 4; r++) {
+			for (int c = 0; c < 4; c++) {
+				dgate[4*r + c] += d[r] * v[c];
+			}
+		}
+		for (int c = 0; c < 4; c++) {
+			dpsi[idx[c]] = gate[c]*d[0] + gate[4 + c]*d[1] + gate[8 + c]*d[2] + gate[12 + c]*d[3];
+		}
+	}
+}
+
+/* Gate on an interleaved statevector array, reference src/gate.c:195-262:
+   the trailing stride is multiplied by nstates. */
+void orc_apply_gate_to_array(const ocplx* gate, int i, int j, int L, int nstates, const ocplx* psi, ocplx* psi_out)
+{
+	if (j < i) {
+		ocplx gp[16];
+		swap_gate_wires(gate, gp);
+		orc_apply_gate_to_array(gp, j, i, L, nstates, psi, psi_out);
+		return;
+	}
+	const long nq = 1L << (L - 2);
+	const long si = (1L << (L - 1 - i)) * nstates, sj = (1L << (L - 1 - j)) * nstates;
+	for (long q = 0; q < nq; q++) {
+		const long kb = quad_base(q, L, i, j) * nstates;
+		for (int s = 0; s < nstates; s++) {
+			const long k = kb + s;
+			const ocplx x = psi[k], y = psi[k + sj], z = psi[k + si], w = psi[k + si + sj];
+			psi_out[k]           = gate[ 0]*x + gate[ 1]*y + gate[ 2]*z + gate[ 3]*w;
+			psi_out[k + sj]      = gate[ 4]*x + gate[ 5]*y + gate[ 6]*z + gate[ 7]*w;
+			psi_out[k + si]      = gate[ 8]*x + gate[ 9]*y + gate[10]*z + gate[11]*w;
+			psi_out[k + si + sj] = gate[12]*x + gate[13]*y + gate[14]*z + gate[15]*w;
+		}
+	}
+}
+
+/* ---------------------------------------------------------------------------
+ * Circuits of two-qubit gates on explicit wires.
+ */
+
+/* reference src/quantum_circuit.c:13-35 (sequential application) */
+int orc_apply_quantum_circuit(const ocplx* gates, int ngates, const int* wires, int L, const ocplx* psi, ocplx* psi_out)
+{
+	const long n = 1L << L;
+	ocplx* a = xalloc(n * sizeof(ocplx));
+	ocplx* b = xalloc(n * sizeof(ocplx));
+	memcpy(a, psi, n * sizeof(ocplx));
+	for (int g = 0; g < ngates; g++) {
+		orc_apply_gate(&gates[16*g], wires[2*g], wires[2*g + 1], L, a, b);
+		ocplx* t = a; a = b; b = t;
+	}
+	memcpy(psi_out, a, n * sizeof(ocplx));
+	free(a); free(b);
+	return 0;
+}
+
+/* reference src/quantum_circuit.c:90-108: cache[g] = state before gate g */
+int orc_quantum_circuit_forward(const ocplx* gates, int ngates, const int* wires, int L, const ocplx* psi,
+	ocplx* cache, ocplx* psi_out)
+{
+	const long n = 1L << L;
+	memcpy(cache, psi, n * sizeof(ocplx));
+	for (int g = 0; g < ngates; g++) {
+		orc_apply_gate(&gates[16*g], wires[2*g], wires[2*g + 1], L, &cache[g*n],
+			g + 1 < ngates ? &cache[(g + 1)*n] : psi_out);
+	}
+	return 0;
+}
+
+/* reference src/quantum_circuit.c:115-140: reverse sweep, dgates[g] per gate */
+int orc_quantum_circuit_backward(const ocplx* gates, int ngates, const int* wires, int L, const ocplx* cache,
+	const ocplx* dpsi_out, ocplx* dpsi, ocplx* dgates)
+{
+	const long n = 1L << L;
+	ocplx* cur = xalloc(n * sizeof(ocplx));
+	ocplx* nxt = xalloc(n * sizeof(ocplx));
+	memcpy(cur, dpsi_out, n * sizeof(ocplx));
+	for (int g = ngates - 1; g >= 0; g--) {
+		orc_apply_gate_backward(&gates[16*g], wires[2*g], wires[2*g + 1], L, &cache[g*n], cur, nxt, &dgates[16*g]);
+		ocplx* t = cur; cur = nxt; nxt = t;
+	}
+	memcpy(dpsi, cur, n * sizeof(ocplx));
+	free(cur); free(nxt);
+	return 0;
+}
+
+/* reference src/quantum_circuit.c:147-236.  Steps:
+   (1) forward cache psi_g (state before gate g);
+   (2) reverse sweep from phi storing phi_g (adjoint after gate g) and dgates;
+   (3) tangent-forward  dpsi_{g+1} = G_g dpsi_g + Z_g psi_g, hess_g  = phi_g (x) dpsi_g;
+   (4) tangent-backward dphi_{g-1} = G_g^T dphi_g + Z_g^T phi_g, hess_g += dphi_g (x) psi_g.
+   Gate-level calls and their operand order follow the reference exactly. */
+int orc_quantum_circuit_gates_hessian_vector_product(const ocplx* gates, const ocplx* gatedirs, int ngates,
+	const int* wires, int L, const ocplx* psi, const ocplx* phi, ocplx* psi_out, ocplx* dgates, ocplx* hess)
+{
+	const long n = 1L << L;
+	ocplx* cache = xalloc((size_t)ngates * n * sizeof(ocplx));
+	ocplx* cache_rev = xalloc((size_t)ngates * n * sizeof(ocplx));
+	ocplx* tmp1 = xalloc(n * sizeof(ocplx));
+	ocplx* tmp2 = xalloc(n * sizeof(ocplx));
+	ocplx* dv = xalloc(n * sizeof(ocplx));
+
+	orc_quantum_circuit_forward(gates, ngates, wires, L, psi, cache, psi_out);
+
+	memcpy(&cache_rev[(ngates - 1)*n], phi, n * sizeof(ocplx));
+	for (int g = ngates - 1; g >= 0; g--) {
+		ocplx* out = g > 0 ? &cache_rev[(g - 1)*n] : tmp1;  /* phi_out of the last step is discarded */
+		orc_apply_gate_backward(&gates[16*g], wires[2*g], wires[2*g + 1], L, &cache[g*n], &cache_rev[g*n], out, &dgates[16*g]);
+	}
+
+	/* tangent forward, :183-204 */
+	for (int e = 0; e < 16; e++) { hess[e] = 0; }
+	orc_apply_gate(&gatedirs[0], wires[0], wires[1], L, &cache[0], dv);
+	for (int g = 1; g < ngates; g++) {
+		ocplx gt[16], dg[16];
+		orc_transpose(&gates[16*g], gt);
+		orc_apply_gate_backward(gt, wires[2*g], wires[2*g + 1], L, &cache_rev[g*n], dv, tmp1, dg);
+		orc_transpose(dg, &hess[16*g]);
+		orc_apply_gate(&gatedirs[16*g], wires[2*g], wires[2*g + 1], L, &cache[g*n], tmp2);
+		for (long a = 0; a < n; a++) { dv[a] = tmp1[a] + tmp2[a]; }
+	}
+
+	/* tangent backward, :207-226 */
+	{
+		const int g = ngates - 1;
+		ocplx zt[16];
+		orc_transpose(&gatedirs[16*g], zt);
+		orc_apply_gate(zt, wires[2*g], wires[2*g + 1], L, &cache_rev[g*n], dv);
+	}
+	for (int g = ngates - 2; g >= 0; g--) {
+		ocplx dg[16], zt[16];
+		orc_apply_gate_backward(&gates[16*g], wires[2*g], wires[2*g + 1], L, &cache[g*n], dv, tmp1, dg);
+		for (int e = 0; e < 16; e++) { hess[16*g + e] += dg[e]; }
+		orc_transpose(&gatedirs[16*g], zt);
+		orc_apply_gate(zt, wires[2*g], wires[2*g + 1], L, &cache_rev[g*n], tmp2);
+		for (long a = 0; a < n; a++) { dv[a] = tmp1[a] + tmp2[a]; }
+	}
+
+	free(dv); free(tmp2); free(tmp1); free(cache_rev); free(cache);
+	return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Brickwall circuits: layer i applies vlist[i] to the pairs
+ * (perms[i][2k], perms[i][2k+1]), k = 0 .. L/2-1.
+ */
+
+/* expansion into a sequential gate list, reference src/target.c:443-458 */
+static void brickwall_to_sequential(int L, int nlayers, const ocplx* vlist, const int* perms, ocplx* gates, int* wires)
+{
+	const int per = L / 2;
+	for (int i = 0; i < nlayers; i++) {
+		for (int k = 0; k < per; k++) {
+			memcpy(&gates[16*(i*per + k)], &vlist[16*i], 16 * sizeof(ocplx));
+			wires[2*(i*per + k)]     = perms[i*L + 2*k];
+			wires[2*(i*per + k) + 1] = perms[i*L + 2*k + 1];
+		}
+	}
+}
+
+/* reference src/brickwall_circuit.c:14-44 */
+int orc_apply_brickwall_unitary(const ocplx* vlist, int nlayers, const int* perms, int L, const ocplx* psi, ocplx* psi_out)
+{
+	const int ng = nlayers * (L / 2);
+	ocplx* gates = xalloc((size_t)ng * 16 * sizeof(ocplx));
+	int* wires = xalloc((size_t)ng * 2 * sizeof(int));
+	brickwall_to_sequential(L, nlayers, vlist, perms, gates, wires);
+	int ret = orc_apply_quantum_circuit(gates, ng, wires, L, psi, psi_out);
+	free(wires); free(gates);
+	return ret;
+}
+
+/* reference src/brickwall_circuit.c:51-78 */
+int orc_brickwall_unitary_forward(const ocplx* vlist, int nlayers, const int* perms, int L, const ocplx* psi,
+	ocplx* cache, ocplx* psi_out)
+{
+	const int ng = nlayers * (L / 2);
+	ocplx* gates = xalloc((size_t)ng * 16 * sizeof(ocplx));
+	int* wires = xalloc((size_t)ng * 2 * sizeof(int));
+	brickwall_to_sequential(L, nlayers, vlist, perms, gates, wires);
+	int ret = orc_quantum_circuit_forward(gates, ng, wires, L, psi, cache, psi_out);
+	free(wires); free(gates);
+	return ret;
+}
+
+/* reference src/brickwall_circuit.c:85-124: per layer, gradients of its gates
+   are summed in reverse pair order (k = L/2-1 down to 0) */
+int orc_brickwall_unitary_backward(const ocplx* vlist, int nlayers, const int* perms, int L, const ocplx* cache,
+	const ocplx* dpsi_out, ocplx* dpsi, ocplx* dvlist)
+{
+	const long n = 1L << L;
+	const int per = L / 2;
+	ocplx* cur = xalloc(n * sizeof(ocplx));
+	ocplx* nxt = xalloc(n * sizeof(ocplx));
+	memcpy(cur, dpsi_out, n * sizeof(ocplx));
+	int g = nlayers * per - 1;
+	for (int i = nlayers - 1; i >= 0; i--) {
+		for (int e = 0; e < 16; e++) { dvlist[16*i + e] = 0; }
+		for (int k = per - 1; k >= 0; k--) {
+			ocplx dV[16];
+			orc_apply_gate_backward(&vlist[16*i], perms[i*L + 2*k], perms[i*L + 2*k + 1], L, &cache[g*n], cur, nxt, dV);
+			for (int e = 0; e < 16; e++) { dvlist[16*i + e] += dV[e]; }
+			ocplx* t = cur; cur = nxt; nxt = t;
+			g--;
+		}
+	}
+	memcpy(dpsi, cur, n * sizeof(ocplx));
+	free(cur); free(nxt);
+	return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * Targets f = -Tr(U^dagger C) via a trace over basis columns.
+ */
+
+int orc_ufunc_dense(int nqubits, const ocplx* psi, void* udata, ocplx* psi_out)
+{
+	const long n = 1L << nqubits;
+	const ocplx* U = udata;
+	for (long a = 0; a < n; a++) {
+		ocplx s = 0;
+		for (long b = 0; b < n; b++) { s += U[a*n + b] * psi[b]; }
+		psi_out[a] = s;
+	}
+	return 0;
+}
+
+/* the index-scrambling linear map of the reference tests, test/test_target.c:16-34 */
+int orc_ufunc_scramble(int nqubits, const ocplx* psi, void* udata, ocplx* psi_out)
+{
+	(void)udata;
+	const long n = 1L << nqubits;
+	for (long i = 0; i < n; i++) {
+		psi_out[i] = (-1.1 + 0.8*I) * psi[((i + 3) * 113) % n] + (0.4 - 0.7*I) * psi[((i + 9) * 173) % n]
+		           + (0.5 + 0.1*I) * psi[i] + (-0.3 + 0.2*I) * psi[((i + 4) * 199) % n];
+	}
+	return 0;
+}
+
+/* evaluates -conj(U e_b) into upsi, reference src/target.c:37-54 */
+static int adjoint_seed(orc_linear_func ufunc, void* udata, int L, long b, ocplx* psi, ocplx* upsi)
+{
+	const long n = 1L << L;
+	memset(psi, 0, n * sizeof(ocplx));
+	psi[b] = 1;
+	int ret = ufunc(L, psi, udata, upsi);
+	if (ret < 0) { return -2; }
+	for (long a = 0; a < n; a++) { upsi[a] = -conj(upsi[a]); }
+	return 0;
+}
+
+/* reference src/target.c:15-75 */
+int orc_circuit_unitary_target(orc_linear_func ufunc, void* udata, const ocplx* gates, int ngates, const int* wires,
+	int L, ocplx* fval)
+{
+	const long n = 1L << L;
+	ocplx* psi = xalloc(n * sizeof(ocplx));
+	ocplx* upsi = xalloc(n * sizeof(ocplx));
+	ocplx* cpsi = xalloc(n * sizeof(ocplx));
+	ocplx f = 0;
+	for (long b = 0; b < n; b++) {
+		if (adjoint_seed(ufunc, udata, L, b, psi, upsi) < 0) { return -2; }
+		orc_apply_quantum_circuit(gates, ngates, wires, L, psi, cpsi);
+		for (long a = 0; a < n; a++) { f += upsi[a] * cpsi[a]; }
+	}
+	free(cpsi); free(upsi); free(psi);
+	*fval = f;
+	return 0;
+}
+
+/* reference src/target.c:158-247 */
+int orc_circuit_unitary_target_and_gradient(orc_linear_func ufunc, void* udata, const ocplx* gates, int ngates,
+	const int* wires, int L, ocplx* fval, ocplx* dgates)
+{
+	const long n = 1L << L;
+	ocplx* psi = xalloc(n * sizeof(ocplx));
+	ocplx* upsi = xalloc(n * sizeof(ocplx));
+	ocplx* cpsi = xalloc(n * sizeof(ocplx));
+	ocplx* cache = xalloc((size_t)ngates * n * sizeof(ocplx));
+	ocplx* dunit = xalloc((size_t)ngates * 16 * sizeof(ocplx));
+	for (int e = 0; e < 16*ngates; e++) { dgates[e] = 0; }
+	ocplx f = 0;
+	for (long b = 0; b < n; b++) {
+		if (adjoint_seed(ufunc, udata, L, b, psi, upsi) < 0) { return -2; }
+		orc_quantum_circuit_forward(gates, ngates, wires, L, psi, cache, cpsi);
+		for (long a = 0; a < n; a++) { f += upsi[a] * cpsi[a]; }
+		orc_quantum_circuit_backward(gates, ngates, wires, L, cache, upsi, psi, dunit);
+		for (int e = 0; e < 16*ngates; e++) { dgates[e] += dunit[e]; }
+	}
+	free(dunit); free(cache); free(cpsi); free(upsi); free(psi);
+	*fval = f;
+	return 0;
+}
+
+/* reference src/target.c:256-280 */
+int orc_circuit_unitary_target_and_projected_gradient(orc_linear_func ufunc, void* udata, const ocplx* gates,
+	int ngates, const int* wires, int L, ocplx* fval, double* grad_vec)
+{
+	ocplx* dgates = xalloc((size_t)ngates * 16 * sizeof(ocplx));
+	int ret = orc_circuit_unitary_target_and_gradient(ufunc, udata, gates, ngates, wires, L, fval, dgates);
+	if (ret < 0) { free(dgates); return ret; }
+	for (int g = 0; g < ngates; g++) {
+		for (int e = 0; e < 16; e++) { dgates[16*g + e] = conj(dgates[16*g + e]); }
+		orc_tangent_to_real(&gates[16*g], &dgates[16*g], &grad_vec[16*g]);
+	}
+	free(dgates);
+	return 0;
+}
+
+/* column range [b0, b1) of the HVP trace, reference src/target.c:335-369 */
+static int circuit_hvp_columns(orc_linear_func ufunc, void* udata, const ocplx* gates, const ocplx* gatedirs,
+	int ngates, const int* wires, int L, long b0, long b1, ocplx* fval, ocplx* dgates, ocplx* hess)
+{
+	const long n = 1L << L;
+	ocplx* psi = xalloc(n * sizeof(ocplx));
+	ocplx* upsi = xalloc(n * sizeof(ocplx));
+	ocplx* cpsi = xalloc(n * sizeof(ocplx));
+	ocplx* dunit = xalloc((size_t)ngates * 16 * sizeof(ocplx));
+	ocplx* hunit = xalloc((size_t)ngates * 16 * sizeof(ocplx));
+	for (int e = 0; e < 16*ngates; e++) { dgates[e] = 0; hess[e] = 0; }
+	ocplx f = 0;
+	for (long b = b0; b < b1; b++) {
+		if (adjoint_seed(ufunc, udata, L, b, psi, upsi) < 0) { return -2; }
+		orc_quantum_circuit_gates_hessian_vector_product(gates, gatedirs, ngates, wires, L, psi, upsi, cpsi, dunit, hunit);
+		for (long a = 0; a < n; a++) { f += upsi[a] * cpsi[a]; }
+		for (int e = 0; e < 16*ngates; e++) {
+			dgates[e] += dunit[e];
+			hess[e] += hunit[e];
+		}
+	}
+	free(hunit); free(dunit); free(cpsi); free(upsi); free(psi);
+	*fval = f;
+	return 0;
+}
+
+/* reference src/target.c:289-380 */
+int orc_circuit_unitary_target_hessian_vector_product(orc_linear_func ufunc, void* udata, const ocplx* gates,
+	const ocplx* gatedirs, int ngates, const int* wires, int L, ocplx* fval, ocplx* dgates, ocplx* hess)
+{
+	return circuit_hvp_columns(ufunc, udata, gates, gatedirs, ngates, wires, L, 0, 1L << L, fval, dgates, hess);
+}
+
+/* Riemannian projection of gradient and HVP for one gate,
+   reference src/target.c:404-430 and :650-671 */
+static void project_grad_hvp(const ocplx* v, const ocplx* z, ocplx* dv, ocplx* hz, double* grad, double* hvp)
+{
+	for (int e = 0; e < 16; e++) { dv[e] = conj(dv[e]); }
+	orc_tangent_to_real(v, dv, grad);
+	for (int e = 0; e < 16; e++) { hz[e] = conj(hz[e]); }
+	ocplx gradh[16], pderiv[16];
+	orc_adjoint(dv, gradh);
+	orc_symmetric_triple_matrix_product(z, gradh, v, pderiv);
+	for (int e = 0; e < 16; e++) { hz[e] -= pderiv[e]; }
+	orc_tangent_to_real(v, hz, hvp);
+}
+
+/* reference src/target.c:389-436 */
+int orc_circuit_unitary_target_projected_hessian_vector_product(orc_linear_func ufunc, void* udata,
+	const ocplx* gates, const ocplx* gatedirs, int ngates, const int* wires, int L, ocplx* fval,
+	double* grad_vec, double* hvp_vec)
+{
+	ocplx* dgates = xalloc((size_t)ngates * 16 * sizeof(ocplx));
+	ocplx* hess = xalloc((size_t)ngates * 16 * sizeof(ocplx));
+	int ret = orc_circuit_unitary_target_hessian_vector_product(ufunc, udata, gates, gatedirs, ngates, wires, L, fval, dgates, hess);
+	if (ret < 0) { free(hess); free(dgates); return ret; }
+	for (int g = 0; g < ngates; g++) {
+		project_grad_hvp(&gates[16*g], &gatedirs[16*g], &dgates[16*g], &hess[16*g], &grad_vec[16*g], &hvp_vec[16*g]);
+	}
+	free(hess); free(dgates);
+	return 0;
+}
+
+/* reference src/target.c:467-481 */
+int orc_brickwall_unitary_target(orc_linear_func ufunc, void* udata, const ocplx* vlist, int nlayers, int L,
+	const int* perms, ocplx* fval)
+{
+	const int ng = nlayers * (L / 2);
+	ocplx* gates = xalloc((size_t)ng * 16 * sizeof(ocplx));
+	int* wires = xalloc((size_t)ng * 2 * sizeof(int));
+	brickwall_to_sequential(L, nlayers, vlist, perms, gates, wires);
+	int ret = orc_circuit_unitary_target(ufunc, udata, gates, ng, wires, L, fval);
+	free(wires); free(gates);
+	return ret;
+}
+
+/* per-layer sum of the per-gate matrices in pair order, reference src/target.c:526-535 */
+static void sum_layers(const ocplx* per_gate, int nlayers, int per, ocplx* per_layer)
+{
+	for (int i = 0; i < nlayers; i++) {
+		for (int e = 0; e < 16; e++) { per_layer[16*i + e] = 0; }
+		for (int k = 0; k < per; k++) {
+			for (int e = 0; e < 16; e++) { per_layer[16*i + e] += per_gate[16*(i*per + k) + e]; }
+		}
+	}
+}
+
+/* reference src/target.c:514-542 */
+int orc_brickwall_unitary_target_and_gradient(orc_linear_func ufunc, void* udata, const ocplx* vlist, int nlayers,
+	int L, const int* perms, ocplx* fval, ocplx* dvlist)
+{
+	const int ng = nlayers * (L / 2);
+	ocplx* gates = xalloc((size_t)ng * 16 * sizeof(ocplx));
+	ocplx* dgates = xalloc((size_t)ng * 16 * sizeof(ocplx));
+	int* wires = xalloc((size_t)ng * 2 * sizeof(int));
+	brickwall_to_sequential(L, nlayers, vlist, perms, gates, wires);
+	int ret = orc_circuit_unitary_target_and_gradient(ufunc, udata, gates, ng, wires, L, fval, dgates);
+	sum_layers(dgates, nlayers, L / 2, dvlist);
+	free(wires); free(dgates); free(gates);
+	return ret;
+}
+
+/* reference src/target.c:551-575 */
+int orc_brickwall_unitary_target_and_projected_gradient(orc_linear_func ufunc, void* udata, const ocplx* vlist,
+	int nlayers, int L, const int* perms, ocplx* fval, double* grad_vec)
+{
+	ocplx* dvlist = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
+	int ret = orc_brickwall_unitary_target_and_gradient(ufunc, udata, vlist, nlayers, L, perms, fval, dvlist);
+	if (ret < 0) { free(dvlist); return ret; }
+	for (int i = 0; i < nlayers; i++) {
+		for (int e = 0; e < 16; e++) { dvlist[16*i + e] = conj(dvlist[16*i + e]); }
+		orc_tangent_to_real(&vlist[16*i], &dvlist[16*i], &grad_vec[16*i]);
+	}
+	free(dvlist);
+	return 0;
+}
+
+int orc_brickwall_hvp_columns(orc_linear_func ufunc, void* udata, const ocplx* vlist, const ocplx* vdirs,
+	int nlayers, const int* perms, int L, long b0, long b1, ocplx* fval, ocplx* dvlist, ocplx* hess_vdirs)
+{
+	const int ng = nlayers * (L / 2);
+	ocplx* gates = xalloc((size_t)ng * 16 * sizeof(ocplx));
+	ocplx* gdirs = xalloc((size_t)ng * 16 * sizeof(ocplx));
+	ocplx* dgates = xalloc((size_t)ng * 16 * sizeof(ocplx));
+	ocplx* hess = xalloc((size_t)ng * 16 * sizeof(ocplx));
+	int* wires = xalloc((size_t)ng * 2 * sizeof(int));
+	brickwall_to_sequential(L, nlayers, vlist, perms, gates, wires);
+	brickwall_to_sequential(L, nlayers, vdirs, perms, gdirs, wires);
+	int ret = circuit_hvp_columns(ufunc, udata, gates, gdirs, ng, wires, L, b0, b1, fval, dgates, hess);
+	sum_layers(dgates, nlayers, L / 2, dvlist);
+	sum_layers(hess, nlayers, L / 2, hess_vdirs);
+	free(wires); free(hess); free(dgates); free(gdirs); free(gates);
+	return ret;
+}
+
+/* reference src/target.c:584-622 */
+int orc_brickwall_unitary_target_hessian_vector_product(orc_linear_func ufunc, void* udata, const ocplx* vlist,
+	const ocplx* vdirs, int nlayers, const int* perms, int L, ocplx* fval, ocplx* dvlist, ocplx* hess_vdirs)
+{
+	return orc_brickwall_hvp_columns(ufunc, udata, vlist, vdirs, nlayers, perms, L, 0, 1L << L, fval, dvlist, hess_vdirs);
+}
+
+/* reference src/target.c:631-678 */
+int orc_brickwall_unitary_target_projected_hessian_vector_product(orc_linear_func ufunc, void* udata,
+	const ocplx* vlist, const ocplx* vdirs, int nlayers, const int* perms, int L, ocplx* fval,
+	double* grad_vec, double* hvp_vec)
+{
+	ocplx* dvlist = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
+	ocplx* hess = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
+	int ret = orc_brickwall_unitary_target_hessian_vector_product(ufunc, udata, vlist, vdirs, nlayers, perms, L, fval, dvlist, hess);
+	if (ret < 0) { free(hess); free(dvlist); return ret; }
+	for (int i = 0; i < nlayers; i++) {
+		project_grad_hvp(&vlist[16*i], &vdirs[16*i], &dvlist[16*i], &hess[16*i], &grad_vec[16*i], &hvp_vec[16*i]);
+	}
+	free(hess); free(dvlist);
+	return 0;
+}
+
+/* ---------------------------------------------------------------------------
+ * 4x4 algebra, reference src/matrix.c
+ */
+
+/* reference src/matrix.c:356-369 (i, j, k loop order) */
+void orc_multiply(const ocplx* a, const ocplx* b, ocplx* c)
+{
+	for (int e = 0; e < 16; e++) { c[e] = 0; }
+	for (int i = 0; i < 4; i++) {
+		for (int j = 0; j < 4; j++) {
+			for (int k = 0; k < 4; k++) {
+				c[4*i + k] += a[4*i + j] * b[4*j + k];
+			}
+		}
+	}
+}
+
+/* reference src/matrix.c:119-144 */
+void orc_transpose(const ocplx* a, ocplx* at)
+{
+	for (int i = 0; i < 4; i++) { for (int j = 0; j < 4; j++) { at[4*i + j] = a[4*j + i]; } }
+}
+void orc_adjoint(const ocplx* a, ocplx* ah)
+{
+	for (int i = 0; i < 4; i++) { for (int j = 0; j < 4; j++) { ah[4*i + j] = conj(a[4*j + i]); } }
+}
+
+static const int upper_idx[6][2] = { {0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3} };
+
+/* reference src/matrix.c:150-174 */
+void orc_symm(const ocplx* w, ocplx* z)
+{
+	for (int k = 0; k < 4; k++) { z[5*k] = creal(w[5*k]); }
+	for (int t = 0; t < 6; t++) {
+		const int r = upper_idx[t][0], c = upper_idx[t][1];
+		z[4*r + c] = 0.5 * (w[4*r + c] + conj(w[4*c + r]));
+		z[4*c + r] = conj(z[4*r + c]);
+	}
+}
+
+/* reference src/matrix.c:180-211 */
+void orc_antisymm(const ocplx* w, ocplx* z)
+{
+	for (int k = 0; k < 4; k++) { z[5*k] = I * cimag(w[5*k]); }
+	for (int t = 0; t < 6; t++) {
+		const int r = upper_idx[t][0], c = upper_idx[t][1];
+		z[4*r + c] = 0.5 * (w[4*r + c] - conj(w[4*c + r]));
+		z[4*c + r] = -conj(z[4*r + c]);
+	}
+}
+
+/* reference src/matrix.c:218-244 (complex case) */
+void orc_real_to_antisymm(const double* r, ocplx* w)
+{
+	for (int k = 0; k < 4; k++) { w[5*k] = I * r[5*k]; }
+	for (int t = 0; t < 6; t++) {
+		const int a = upper_idx[t][0], b = upper_idx[t][1];
+		const double p = r[4*a + b], q = r[4*b + a];
+		w[4*a + b] = 0.5*(p - q) + 0.5*I*(p + q);
+		w[4*b + a] = -conj(w[4*a + b]);
+	}
+}
+
+/* reference src/matrix.c:279-288 (complex case) */
+void orc_antisymm_to_real(const ocplx* w, double* r)
+{
+	for (int e = 0; e < 16; e++) { r[e] = creal(w[e]) + cimag(w[e]); }
+}
+
+/* reference src/matrix.c:309-314: z = v A(r) */
+void orc_real_to_tangent(const double* r, const ocplx* v, ocplx* z)
+{
+	ocplx a[16];
+	orc_real_to_antisymm(r, a);
+	orc_multiply(v, a, z);
+}
+
+/* reference src/matrix.c:322-329: r = antisymm_to_real(antisymm(v^dagger z)) */
+void orc_tangent_to_real(const ocplx* v, const ocplx* z, double* r)
+{
+	ocplx w[16], t[16];
+	orc_adjoint(v, w);
+	orc_multiply(w, z, t);
+	orc_antisymm(t, w);
+	orc_antisymm_to_real(w, r);
+}
+
+/* reference src/matrix.c:336-347: p = z - u symm(u^dagger z) */
+void orc_project_tangent(const ocplx* u, const ocplx* z, ocplx* p)
+{
+	ocplx v[16], w[16];
+	orc_adjoint(u, v);
+	orc_multiply(v, z, w);
+	orc_symm(w, v);
+	orc_multiply(u, v, w);
+	for (int e = 0; e < 16; e++) { p[e] = z[e] - w[e]; }
+}
+
+/* reference src/matrix.c:376-388: (abc + cba) / 2 */
+void orc_symmetric_triple_matrix_product(const ocplx* a, const ocplx* b, const ocplx* c, ocplx* ret)
+{
+	ocplx u[16], v[16];
+	orc_multiply(a, b, u);
+	orc_multiply(u, c, ret);
+	orc_multiply(c, b, u);
+	orc_multiply(u, a, v);
+	for (int e = 0; e < 16; e++) { ret[e] += v[e]; ret[e] *= 0.5; }
+}
+
+/* Gauss-Jordan elimination with partial pivoting, reference src/matrix.c:410-475 */
+int orc_inverse_matrix(const ocplx* a, ocplx* ainv)
+{
+	ocplx m[16];
+	memcpy(m, a, sizeof(m));
+	for (int e = 0; e < 16; e++) { ainv[e] = (e % 5 == 0) ? 1 : 0; }
+	for (int k = 0; k < 4; k++) {
+		int piv = k;
+		double p = cabs(m[5*k]);
+		for (int i = k + 1; i < 4; i++) {
+			if (cabs(m[4*i + k]) > p) { piv = i; p = cabs(m[4*i + k]); }
+		}
+		if (p == 0) { return -1; }
+		if (piv != k) {
+			for (int c = 0; c < 4; c++) {
+				ocplx t = m[4*k + c]; m[4*k + c] = m[4*piv + c]; m[4*piv + c] = t;
+				t = ainv[4*k + c]; ainv[4*k + c] = ainv[4*piv + c]; ainv[4*piv + c] = t;
+			}
+		}
+		for (int i = 0; i < 4; i++) {
+			if (i == k) { continue; }
+			const ocplx s = m[4*i + k] / m[5*k];
+			for (int c = k + 1; c < 4; c++) { m[4*i + c] -= s * m[4*k + c]; }
+			m[4*i + k] = 0;
+			for (int c = 0; c < 4; c++) { ainv[4*i + c] -= s * ainv[4*k + c]; }
+		}
+		for (int c = k + 1; c < 4; c++) { m[4*k + c] /= m[5*k]; }
+		for (int c = 0; c < 4; c++) { ainv[4*k + c] /= m[5*k]; }
+		m[5*k] = 1;
+	}
+	return 0;
+}
+
+/* Newton iteration u <- (u + u^{-dagger}) / 2 (Higham 1986), reference src/matrix.c:488-510 */
+void orc_polar_factor(const ocplx* a, ocplx* u)
+{
+	memcpy(u, a, 16 * sizeof(ocplx));
+	for (int k = 0; k < 14; k++) {
+		ocplx v[16], w[16];
+		orc_inverse_matrix(u, v);
+		orc_adjoint(v, w);
+		for (int e = 0; e < 16; e++) { u[e] += w[e]; u[e] *= 0.5; }
+		if (k >= 4) {
+			double d = 0;
+			for (int e = 0; e < 16; e++) { d = fmax(d, cabs(u[e] - w[e])); }
+			if (d < 1e-14) { break; }
+		}
+	}
+}
+
+/* reference src/matrix.c:517-531: v = polar(u (I + A(eta))) */
+void orc_retract(const ocplx* u, const double* eta, ocplx* v)
+{
+	ocplx z[16], w[16];
+	orc_real_to_antisymm(eta, z);
+	for (int k = 0; k < 4; k++) { z[5*k] += 1; }
+	orc_multiply(u, z, w);
+	orc_polar_factor(w, v);
+}
+
+/* ---------------------------------------------------------------------------
+ * Trust region, reference src/trust_region.c (sequential dot products in
+ * place of CBLAS ddot/daxpy/dscal).
+ */
+static double ddot(int n, const double* x, const double* y)
+{
+	double s = 0;
+	for (int i = 0; i < n; i++) { s += x[i] * y[i]; }
+	return s;
+}
+
+/* roots of x^2 + 2 p x + q = 0 in ascending order, reference src/trust_region.c:14-40 */
+static void solve_quadratic(double p, double q, double x[2])
+{
+	if (p == 0) {
+		const double a = sqrt(-q);
+		x[0] = -a; x[1] = a;
+		return;
+	}
+	const double a = -(p + copysign(1.0, p) * sqrt(p * p - q));
+	const double b = q / a;
+	if (a <= b) { x[0] = a; x[1] = b; } else { x[0] = b; x[1] = a; }
+}
+
+/* t > 0 with ||b + t d|| = radius, reference src/trust_region.c:49-65 */
+static double move_to_boundary(const double* b, const double* d, int n, double radius)
+{
+	const double dsq = ddot(n, d, d);
+	if (dsq == 0) { return 0; }
+	const double p = ddot(n, b, d) / dsq;
+	const double q = (ddot(n, b, b) - radius * radius) / dsq;
+	double t[2];
+	solve_quadratic(p, q, t);
+	return t[1];
+}
+
+/* Steihaug-Toint truncated CG, reference src/trust_region.c:159-217 */
+bool orc_truncated_cg_hvp(const double* x, const double* grad, orc_hvp_func f_hvp, void* fdata, int n,
+	double radius, int maxiter, double abstol, double reltol, double* z)
+{
+	double* r = xalloc(n * sizeof(double));
+	double* d = xalloc(n * sizeof(double));
+	double* hd = xalloc(n * sizeof(double));
+	memcpy(r, grad, n * sizeof(double));
+	double rsq = ddot(n, r, r);
+	const double stoptol = fmax(abstol, reltol * sqrt(rsq));
+	for (int i = 0; i < n; i++) { z[i] = 0; d[i] = -r[i]; }
+	bool boundary = false;
+	for (int it = 0; it < maxiter; it++) {
+		f_hvp(x, fdata, d, hd);
+		const double dhd = ddot(n, d, hd);
+		const double t = move_to_boundary(z, d, n, radius);
+		const double alpha = rsq / dhd;
+		if (dhd <= 0 || alpha > t) {
+			for (int i = 0; i < n; i++) { z[i] += t * d[i]; }
+			boundary = true;
+			break;
+		}
+		for (int i = 0; i < n; i++) { r[i] += alpha * hd[i]; }
+		for (int i = 0; i < n; i++) { z[i] += alpha * d[i]; }
+		const double rsq_next = ddot(n, r, r);
+		if (sqrt(rsq_next) <= stoptol) { break; }
+		const double beta = rsq_next / rsq;
+		for (int i = 0; i < n; i++) { d[i] = beta * d[i] - r[i]; }
+		rsq = rsq_next;
+	}
+	free(hd); free(d); free(r);
+	return boundary;
+}
+
+struct bw_data { orc_linear_func ufunc; void* udata; const int* perms; int nlayers; int L; };
+
+/* wrappers, reference src/brickwall_opt.c:22-79 */
+static double bw_f(const double* x, struct bw_data* dd)
+{
+	ocplx f;
+	if (orc_brickwall_unitary_target(dd->ufunc, dd->udata, (const ocplx*)x, dd->nlayers, dd->L, dd->perms, &f) < 0) { return -1; }
+	return creal(f);
+}
+static double bw_f_deriv(const double* x, struct bw_data* dd, double* grad)
+{
+	ocplx f;
+	if (orc_brickwall_unitary_target_and_projected_gradient(dd->ufunc, dd->udata, (const ocplx*)x, dd->nlayers, dd->L, dd->perms, &f, grad) < 0) { return -1; }
+	return creal(f);
+}
+static void bw_f_hvp(const double* x, void* fdata, const double* vec, double* hvp)
+{
+	struct bw_data* dd = fdata;
+	const ocplx* vlist = (const ocplx*)x;
+	ocplx* vdirs = xalloc((size_t)dd->nlayers * 16 * sizeof(ocplx));
+	double* grad = xalloc((size_t)dd->nlayers * 16 * sizeof(double));
+	for (int i = 0; i < dd->nlayers; i++) { orc_real_to_tangent(&vec[16*i], &vlist[16*i], &vdirs[16*i]); }
+	ocplx f;
+	orc_brickwall_unitary_target_projected_hessian_vector_product(dd->ufunc, dd->udata, vlist, vdirs, dd->nlayers, dd->perms, dd->L, &f, grad, hvp);
+	free(grad); free(vdirs);
+}
+
+/* Riemannian trust region (Absil et al. Alg. 10) with default parameters,
+   reference src/trust_region.c:306-368 and src/brickwall_opt.c:107-150 */
+void orc_optimize_brickwall_circuit_hvp(orc_linear_func ufunc, void* udata, const ocplx* vlist_start, int nlayers,
+	int L, const int* perms, int niter, double* f_iter, ocplx* vlist_opt)
+{
+	struct bw_data dd = { ufunc, udata, perms, nlayers, L };
+	const int n = 16 * nlayers, m = 32 * nlayers;
+	const int maxiter = 2 * n;
+	const double abstol = 1e-8, reltol = 1e-6, rho_trust = 0.125, maxradius = 0.1;
+	double radius = 0.01;
+	double* x = xalloc(m * sizeof(double));
+	double* xn = xalloc(m * sizeof(double));
+	double* grad = xalloc(n * sizeof(double));
+	double* eta = xalloc(n * sizeof(double));
+	double* heta = xalloc(n * sizeof(double));
+	memcpy(x, vlist_start, m * sizeof(double));
+	for (int k = 0; k < niter; k++) {
+		const double fx = bw_f_deriv(x, &dd, grad);
+		const bool on_boundary = orc_truncated_cg_hvp(x, grad, bw_f_hvp, &dd, n, radius, maxiter, abstol, reltol, eta);
+		for (int i = 0; i < nlayers; i++) { orc_retract((const ocplx*)&x[32*i], &eta[16*i], (ocplx*)&xn[32*i]); }
+		f_iter[k] = fx;
+		bw_f_hvp(x, &dd, eta, heta);
+		const double rho = (bw_f(xn, &dd) - fx) / (ddot(n, grad, eta) + 0.5 * ddot(n, eta, heta));
+		if (rho < 0.25) { radius *= 0.25; }
+		else if (rho > 0.75 && on_boundary) { radius = fmin(2 * radius, maxradius); }
+		if (rho > rho_trust) { double* t = x; x = xn; xn = t; }
+	}
+	memcpy(vlist_opt, x, m * sizeof(double));
+	f_iter[niter] = bw_f(x, &dd);
+	free(heta); free(eta); free(grad); free(xn); free(x);
+}
+
+/* ---------------------------------------------------------------------------
+ * RNG: PCG32 (O'Neill 2014, XSH-RR output), two generators combined to 64 bits,
+ * reference src/pcg_basic.c:30-51,112-132 and src/rng.c:13-25,91-123,152-157.
+ */
+static uint32_t pcg32_next(struct orc_pcg32* g)
+{
+	const uint64_t old = g->state;
+	g->state = old * 6364136223846793005ULL + g->inc;
+	const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+	const uint32_t rot = (uint32_t)(old >> 59u);
+	return (xs >> rot) | (xs << ((-rot) & 31));
+}
+
+static void pcg32_seed(struct orc_pcg32* g, uint64_t initstate, uint64_t initseq)
+{
+	g->state = 0;
+	g->inc = (initseq << 1u) | 1u;
+	pcg32_next(g);
+	g->state += initstate;
+	pcg32_next(g);
+}
+
+static uint64_t pcg32_pair(struct orc_pcg32* g)
+{
+	const uint64_t hi = pcg32_next(g);
+	const uint64_t lo = pcg32_next(g);
+	return (hi << 32) | lo;
+}
+
+void orc_seed_rng_state(uint64_t seed, struct orc_rng* rng)
+{
+	struct orc_pcg32 tmp;
+	pcg32_seed(&tmp, seed, 6364136223846793005ULL * seed + 1442695040888963407ULL);
+	const uint64_t s1 = pcg32_pair(&tmp);
+	const uint64_t s2 = pcg32_pair(&tmp);
+	const uint64_t q1 = pcg32_pair(&tmp);
+	uint64_t q2 = pcg32_pair(&tmp);
+	const uint64_t mask = ~0ULL >> 1;
+	if ((q1 & mask) == (q2 & mask)) { q2 = ~q2; }
+	pcg32_seed(&rng->gen[0], s1, q1);
+	pcg32_seed(&rng->gen[1], s2, q2);
+}
+
+uint32_t orc_rand_uint32(struct orc_rng* rng) { return pcg32_next(&rng->gen[0]); }
+
+uint64_t orc_rand_uint64(struct orc_rng* rng)
+{
+	const uint64_t hi = pcg32_next(&rng->gen[0]);
+	const uint64_t lo = pcg32_next(&rng->gen[1]);
+	return (hi << 32) | lo;
+}
+
+double orc_randu(struct orc_rng* rng) { return ldexp((double)orc_rand_uint64(rng), -64); }
+
+/* Box-Muller (1958) */
+double orc_randn(struct orc_rng* rng)
+{
+	double u1 = orc_randu(rng);
+	const double u2 = orc_randu(rng);
+	if (u1 == 0) { u1 = 1; }
+	return sqrt(-2 * log(u1)) * sin(6.2831853071795864769 * u2);
+}
+
+ocplx orc_crandn(struct orc_rng* rng)
+{
+	const double x = orc_randn(rng);
+	const double y = orc_randn(rng);
+	return (x + I * y) / sqrt(2.0);
+}

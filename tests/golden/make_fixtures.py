@@ -1,0 +1,90 @@
+"""Convert the reference's own HDF5 test fixtures into committed .npz files.
+
+Runs only in the build container (it reads test/data/*.hdf5 of the reference
+checkout and links the image's libhdf5 from /opt/conda).  The GPU box never
+runs it: tests load the committed tests/golden/fixtures/*.npz instead.
+
+Each .npz holds the datasets of one fixture under their original names
+(complex arrays keep the trailing re/im axis the fixture uses, see
+test/io_util.py:4-8 of the reference).
+
+    python tests/golden/make_fixtures.py
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+REF_DATA = "/root/reference/test/data"
+HERE = os.path.dirname(os.path.abspath(__file__))
+OUT = os.path.join(HERE, "fixtures")
+
+# fixtures that pin the gate-sweep path (complex variants) plus the tCG one
+FIXTURES = [
+    "test_apply_gate_cplx",
+    "test_apply_gate_backward_cplx",
+    "test_apply_gate_to_array_cplx",
+    "test_apply_quantum_circuit_cplx",
+    "test_quantum_circuit_backward_cplx",
+    "test_quantum_circuit_gates_hessian_vector_product_cplx",
+    "test_apply_brickwall_unitary_cplx",
+    "test_brickwall_unitary_backward_cplx",
+    "test_circuit_unitary_target_cplx",
+    "test_circuit_unitary_target_and_gradient_cplx",
+    "test_circuit_unitary_target_hessian_vector_product_cplx",
+    "test_circuit_unitary_target_projected_hessian_vector_product_cplx",
+    "test_brickwall_unitary_target_cplx",
+    "test_brickwall_unitary_target_and_gradient_cplx",
+    "test_brickwall_unitary_target_and_projected_gradient_cplx",
+    "test_brickwall_unitary_target_hessian_vector_product_cplx",
+    "test_brickwall_unitary_target_projected_hessian_vector_product_cplx",
+    "test_brickwall_unitary_target_gradient_hessian_cplx",
+    "test_brickwall_unitary_target_gradient_vector_hessian_matrix_cplx",
+    "test_truncated_cg",
+    "test_polar_factor_cplx",
+    "test_project_tangent_cplx",
+    "test_real_to_tangent_cplx",
+    "test_multiply_cplx",
+    "test_transpose_statevector_cplx",
+]
+
+
+def build_converter(workdir):
+    exe = os.path.join(workdir, "h5_to_raw")
+    subprocess.check_call([
+        "gcc", "-O2", os.path.join(HERE, "h5_to_raw.c"),
+        "-I/opt/conda/include", "-L/opt/conda/lib", "-lhdf5",
+        "-Wl,-rpath,/opt/conda/lib", "-o", exe])
+    return exe
+
+
+def convert(exe, name, workdir):
+    src = os.path.join(REF_DATA, name + ".hdf5")
+    d = os.path.join(workdir, name)
+    os.makedirs(d, exist_ok=True)
+    out = subprocess.check_output([exe, src, d], text=True)
+    arrays = {}
+    for line in out.strip().splitlines():
+        parts = line.split()
+        key, dt, shape = parts[0], parts[1], tuple(int(x) for x in parts[2:])
+        raw = np.fromfile(os.path.join(d, key + ".bin"), dtype="<" + dt)
+        arrays[key] = raw.reshape(shape) if shape else raw.reshape(())
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrays)
+    return arrays
+
+
+def main():
+    if not os.path.isdir(REF_DATA):
+        sys.exit("reference fixtures not present; nothing to convert")
+    os.makedirs(OUT, exist_ok=True)
+    with tempfile.TemporaryDirectory() as work:
+        exe = build_converter(work)
+        for name in FIXTURES:
+            arrays = convert(exe, name, work)
+            print(name, {k: v.shape for k, v in arrays.items()})
+
+
+if __name__ == "__main__":
+    main()

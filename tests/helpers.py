@@ -1,0 +1,123 @@
+"""Shared test helpers: fixture loading, seeded inputs, dense-matrix models."""
+import os
+
+import numpy as np
+
+FIXDIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "fixtures")
+
+
+def fixture(name):
+    """Load one converted reference fixture (tests/golden/fixtures/<name>.npz)."""
+    with np.load(os.path.join(FIXDIR, name + ".npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def cplx(a):
+    """Trailing re/im axis -> complex (reference test/io_util.py layout)."""
+    a = np.asarray(a, dtype=np.float64)
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def stack(d, prefix, n):
+    return np.stack([cplx(d[f"{prefix}{i}"]) for i in range(n)])
+
+
+def perms_of(d, n):
+    return np.stack([np.asarray(d[f"perm{i}"], dtype=np.int32) for i in range(n)])
+
+
+def crandn(rng, shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) / np.sqrt(2)
+
+
+def haar_unitary(rng, n=4):
+    """Haar-random unitary via QR of a complex Ginibre matrix with phase fix (Mezzadri 2007)."""
+    z = crandn(rng, (n, n))
+    q, r = np.linalg.qr(z)
+    d = np.diag(r)
+    return q * (d / np.abs(d))[None, :]
+
+
+def brickwall_perms(L, nlayers):
+    """Periodic brickwall: even layers identity, odd layers roll by one."""
+    ident = np.arange(L, dtype=np.int32)
+    roll = np.roll(ident, 1).astype(np.int32)
+    return np.stack([ident if i % 2 == 0 else roll for i in range(nlayers)])
+
+
+def random_perms(rng, L, nlayers):
+    return np.stack([rng.permutation(L).astype(np.int32) for _ in range(nlayers)])
+
+
+def random_wires(rng, L, ngates):
+    return np.stack([rng.choice(L, 2, replace=False).astype(np.int32) for _ in range(ngates)])
+
+
+# ----------------------------------------------------------- dense models
+def gate_matrix(G, i, j, L):
+    """Dense 2^L x 2^L matrix of gate G on wires (i, j); qubit 0 is the most significant bit."""
+    n = 1 << L
+    M = np.zeros((n, n), np.complex128)
+    bi, bj = L - 1 - i, L - 1 - j
+    for a in range(n):
+        qi, qj = (a >> bi) & 1, (a >> bj) & 1
+        r = 2 * qi + qj
+        base = a & ~(1 << bi) & ~(1 << bj)
+        for c in range(4):
+            ci, cj = c >> 1, c & 1
+            b = base | (ci << bi) | (cj << bj)
+            M[a, b] = G[r, c]
+    return M
+
+
+def circuit_matrix(gates, wires, L):
+    n = 1 << L
+    W = np.eye(n, dtype=np.complex128)
+    for G, (i, j) in zip(gates, wires):
+        W = gate_matrix(G, int(i), int(j), L) @ W
+    return W
+
+
+def brickwall_sequential(vlist, perms):
+    L = perms.shape[1]
+    gates, wires = [], []
+    for i, V in enumerate(vlist):
+        for k in range(L // 2):
+            gates.append(V)
+            wires.append((perms[i][2 * k], perms[i][2 * k + 1]))
+    return np.array(gates), np.array(wires, dtype=np.int32)
+
+
+def brickwall_matrix(vlist, perms):
+    g, w = brickwall_sequential(vlist, perms)
+    return circuit_matrix(g, w, perms.shape[1])
+
+
+def scramble_matrix(L):
+    """Dense matrix of the reference tests' index-scrambling map (test/test_target.c:16-34)."""
+    n = 1 << L
+    U = np.zeros((n, n), np.complex128)
+    for i in range(n):
+        U[i, ((i + 3) * 113) % n] += -1.1 + 0.8j
+        U[i, ((i + 9) * 173) % n] += 0.4 - 0.7j
+        U[i, i] += 0.5 + 0.1j
+        U[i, ((i + 4) * 199) % n] += -0.3 + 0.2j
+    return U
+
+
+def real_to_antisymm(r):
+    r = np.asarray(r, np.float64).reshape(4, 4)
+    W = np.zeros((4, 4), np.complex128)
+    for k in range(4):
+        W[k, k] = 1j * r[k, k]
+    for a in range(4):
+        for b in range(a + 1, 4):
+            W[a, b] = 0.5 * (r[a, b] - r[b, a]) + 0.5j * (r[a, b] + r[b, a])
+            W[b, a] = -np.conj(W[a, b])
+    return W
+
+
+def reldiff(x, y):
+    x = np.asarray(x)
+    y = np.asarray(y)
+    return float(np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300))

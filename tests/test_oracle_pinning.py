@@ -1,0 +1,487 @@
+"""Pin the oracle (oracle/oracle.c) before it is trusted as the checker.
+
+Three kinds of evidence, each chosen so that a plausible slip (a dropped term,
+a wrong sign or conjugation, a transposed operand, a swapped wire) fails:
+  1. the reference's own golden fixtures (test/data/*.hdf5, converted to
+     tests/golden/fixtures/*.npz) with the reference tests' tolerances;
+  2. the compiled reference subset in oracle/_ref (gate.c, matrix.c, rng.c,
+     pcg_basic.c), when it has been built in this container;
+  3. independent mathematics: dense Kronecker-product matrices, finite
+     differences, second-order retraction expansions, symmetry, closed forms.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import (brickwall_matrix, brickwall_perms, circuit_matrix, cplx, crandn, fixture, gate_matrix,
+                     haar_unitary, perms_of, random_perms, random_wires, scramble_matrix, stack)
+
+SCR = O.Ufunc.scramble()
+
+
+# ================================================================ fixtures
+def test_fixture_apply_gate():
+    d = fixture("test_apply_gate_cplx")
+    V, psi = cplx(d["V"]), cplx(d["psi"])
+    for k, (i, j) in zip((1, 2, 3), ((2, 5), (4, 1), (3, 4))):
+        chi = O.apply_gate(V, i, j, psi)
+        assert np.max(np.abs(chi - cplx(d[f"chi{k}"]))) < 1e-12
+
+
+def test_fixture_apply_gate_to_array():
+    d = fixture("test_apply_gate_to_array_cplx")
+    V, psi = cplx(d["V"]), cplx(d["psi"])
+    for k, (i, j) in zip((1, 2, 3), ((2, 5), (4, 1), (3, 4))):
+        chi = O.apply_gate_to_array(V, i, j, psi)
+        assert np.max(np.abs(chi - cplx(d[f"chi{k}"]))) < 1e-12
+
+
+def test_fixture_apply_quantum_circuit():
+    d = fixture("test_apply_quantum_circuit_cplx")
+    G = stack(d, "G", 4)
+    out = O.apply_quantum_circuit(G, d["wires"], cplx(d["psi"]))
+    assert np.max(np.abs(out - cplx(d["psi_out"]))) < 1e-12
+
+
+def test_fixture_quantum_circuit_backward_forward_part():
+    d = fixture("test_quantum_circuit_backward_cplx")
+    G = stack(d, "G", 5)
+    out, cache = O.quantum_circuit_forward(G, d["wires"], cplx(d["psi"]))
+    assert np.max(np.abs(out - cplx(d["psi_out"]))) < 1e-12
+    assert np.array_equal(cache[0], cplx(d["psi"]))
+
+
+def test_fixture_apply_brickwall_unitary():
+    d = fixture("test_apply_brickwall_unitary_cplx")
+    V, P = stack(d, "V", 4), perms_of(d, 4)
+    for nl in (3, 4):
+        chi = O.apply_brickwall_unitary(V[:nl], P[:nl], cplx(d["psi"]))
+        assert np.max(np.abs(chi - cplx(d[f"chi{nl - 3}"]))) < 1e-12
+
+
+def test_fixture_brickwall_backward_forward_part():
+    d = fixture("test_brickwall_unitary_backward_cplx")
+    V, P = stack(d, "V", 4), perms_of(d, 4)
+    for nl in (3, 4):
+        out, _ = O.brickwall_unitary_forward(V[:nl], P[:nl], cplx(d["psi"]))
+        assert np.max(np.abs(out - cplx(d[f"psi_out{nl - 3}"]))) < 1e-12
+
+
+def test_fixture_qc_hvp_output_state():
+    d = fixture("test_quantum_circuit_gates_hessian_vector_product_cplx")
+    G, Z = stack(d, "G", 6), stack(d, "Z", 6)
+    out, _, _ = O.quantum_circuit_gates_hvp(G, Z, d["wires"], cplx(d["psi"]), cplx(d["phi"]))
+    assert np.max(np.abs(out - cplx(d["psi_out"]))) < 1e-12
+
+
+def test_fixture_circuit_targets():
+    d = fixture("test_circuit_unitary_target_cplx")
+    f = O.circuit_unitary_target(SCR, stack(d, "G", 5), d["wires"], 7)
+    assert abs(f - cplx(d["f"])) < 1e-12
+    d = fixture("test_circuit_unitary_target_and_gradient_cplx")
+    f, _ = O.circuit_unitary_target_and_gradient(SCR, stack(d, "G", 5), d["wires"], 8)
+    assert abs(f - cplx(d["f"])) < 1e-12
+    d = fixture("test_circuit_unitary_target_hessian_vector_product_cplx")
+    f, _, _ = O.circuit_unitary_target_hvp(SCR, stack(d, "G", 6), stack(d, "Z", 6), d["wires"], 7)
+    assert abs(f - cplx(d["f"])) < 1e-12
+    d = fixture("test_circuit_unitary_target_projected_hessian_vector_product_cplx")
+    f, _, _ = O.circuit_unitary_target_projected_hvp(SCR, stack(d, "G", 6), stack(d, "Z", 6), d["wires"], 7)
+    assert abs(f - cplx(d["f"])) < 1e-12
+
+
+def test_fixture_brickwall_target_and_gradient():
+    d = fixture("test_brickwall_unitary_target_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    for k, nl in enumerate((4, 5)):
+        f = O.brickwall_unitary_target(SCR, V[:nl], P[:nl], 8)
+        assert abs(f - cplx(d[f"f{k}"])) < 1e-12
+    d = fixture("test_brickwall_unitary_target_and_gradient_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    for k, nl in enumerate((4, 5)):
+        f, dV = O.brickwall_unitary_target_and_gradient(SCR, V[:nl], P[:nl], 8)
+        assert abs(f - cplx(d[f"f{k}"])) < 1e-12
+        assert np.max(np.abs(dV - cplx(d[f"dVlist{k}"]))) < 1e-12
+
+
+def test_fixture_brickwall_projected_gradient():
+    d = fixture("test_brickwall_unitary_target_and_projected_gradient_cplx")
+    V, P = stack(d, "V", 3), perms_of(d, 3)
+    f, g = O.brickwall_unitary_target_and_projected_gradient(SCR, V, P, 6)
+    assert abs(f - cplx(d["f"])) < 1e-12
+    assert np.max(np.abs(g - d["grad"])) < 1e-14
+
+
+def test_fixture_brickwall_hvp_values():
+    d = fixture("test_brickwall_unitary_target_hessian_vector_product_cplx")
+    V, Z, P = stack(d, "V", 5), stack(d, "Z", 5), perms_of(d, 5)
+    f, _, _ = O.brickwall_unitary_target_hvp(SCR, V, Z, P, 6)
+    assert abs(f - cplx(d["f"])) < 1e-12
+    d = fixture("test_brickwall_unitary_target_projected_hessian_vector_product_cplx")
+    V, Z, P = stack(d, "V", 4), stack(d, "Z", 4), perms_of(d, 4)
+    f, _, _ = O.brickwall_unitary_target_projected_hvp(SCR, V, Z, P, 6)
+    assert abs(f - cplx(d["f"])) < 1e-12
+
+
+def test_fixture_hessian_pins_hvp():
+    """hess (stored full Hessian) applied to Z equals the matrix-free HVP (pins a17)."""
+    d = fixture("test_brickwall_unitary_target_gradient_hessian_cplx")
+    V, Z, P = stack(d, "V", 5), stack(d, "Z", 5), perms_of(d, 5)
+    for k, nl in enumerate((4, 5)):
+        f, dV, HZ = O.brickwall_unitary_target_hvp(SCR, V[:nl], Z[:nl], P[:nl], 6)
+        assert abs(f - cplx(d[f"f{k}"])) < 1e-12
+        assert np.max(np.abs(dV - cplx(d[f"dVlist{k}"]))) < 1e-12
+        hess = cplx(d[f"hess{k}"])
+        ref = (hess @ Z[:nl].reshape(-1)).reshape(nl, 4, 4)
+        assert np.max(np.abs(HZ - ref)) < 1e-12
+
+
+def test_fixture_hessian_matrix_pins_projected_hvp():
+    """H (stored projected Hessian) times vec equals the projected HVP (pins a18)."""
+    d = fixture("test_brickwall_unitary_target_gradient_vector_hessian_matrix_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    rng = np.random.default_rng(11)
+    for _ in range(3):
+        vec = rng.standard_normal(80)
+        Zs = np.stack([O.real_to_tangent(vec[16 * i:16 * i + 16], V[i]) for i in range(5)])
+        f, g, h = O.brickwall_unitary_target_projected_hvp(SCR, V, Zs, P, 6)
+        assert abs(f - cplx(d["f"])) < 1e-12
+        assert np.max(np.abs(g - d["grad"])) < 1e-14
+        assert np.max(np.abs(h - d["H"] @ vec)) < 1e-12
+
+
+def test_fixture_truncated_cg():
+    d = fixture("test_truncated_cg")
+    H = d["hess"]
+    z, on = O.truncated_cg_hvp(d["grad"], lambda v: H @ v, 1.5)
+    assert np.max(np.abs(z - d["z"])) < 1e-14
+    assert on == bool(d["on_boundary"])
+
+
+def test_fixture_matrix_functions():
+    d = fixture("test_polar_factor_cplx")
+    assert np.max(np.abs(O.polar_factor(cplx(d["a"])) - cplx(d["u"]))) < 1e-12
+    d = fixture("test_project_tangent_cplx")
+    assert np.max(np.abs(O.project_tangent(cplx(d["u"]), cplx(d["z"])) - cplx(d["p"]))) < 1e-12
+    d = fixture("test_multiply_cplx")
+    assert np.max(np.abs(O.multiply(cplx(d["a"]), cplx(d["b"])) - cplx(d["c"]))) < 1e-12
+    d = fixture("test_real_to_tangent_cplx")
+    r, v = d["r"].reshape(-1), cplx(d["v"])
+    z = O.real_to_tangent(r, v)
+    assert np.max(np.abs(O.tangent_to_real(v, z) - r)) < 1e-14
+
+
+# ====================================================== dense / closed forms
+def test_dense_model_circuit_and_brickwall():
+    rng = np.random.default_rng(5)
+    for L in (4, 5, 6):
+        gates = crandn(rng, (6, 4, 4))
+        wires = random_wires(rng, L, 6)
+        psi = crandn(rng, 1 << L)
+        W = circuit_matrix(gates, wires, L)
+        assert np.allclose(O.apply_quantum_circuit(gates, wires, psi), W @ psi, atol=1e-12, rtol=0)
+        for (i, j), G in zip(wires, gates):
+            assert np.allclose(O.apply_gate(G, i, j, psi), gate_matrix(G, i, j, L) @ psi, atol=1e-13, rtol=0)
+    L = 6
+    V = np.stack([haar_unitary(rng) for _ in range(3)])
+    P = random_perms(rng, L, 3)
+    psi = crandn(rng, 1 << L)
+    assert np.allclose(O.apply_brickwall_unitary(V, P, psi), brickwall_matrix(V, P) @ psi, atol=1e-12, rtol=0)
+
+
+def test_identity_and_swap_gates():
+    rng = np.random.default_rng(6)
+    L = 5
+    psi = crandn(rng, 1 << L)
+    assert np.array_equal(O.apply_gate(np.eye(4), 1, 3, psi), psi)
+    swap = np.eye(4)[[0, 2, 1, 3]]
+    out = O.apply_gate(swap, 0, 4, psi)
+    t = psi.reshape([2] * L).transpose(4, 1, 2, 3, 0).reshape(-1)
+    assert np.array_equal(out, t)
+
+
+def test_target_dense_trace_and_maximum():
+    rng = np.random.default_rng(7)
+    L = 6
+    V = np.stack([haar_unitary(rng) for _ in range(4)])
+    P = brickwall_perms(L, 4)
+    W = brickwall_matrix(V, P)
+    U = scramble_matrix(L)
+    f = O.brickwall_unitary_target(O.Ufunc.dense(U), V, P, L)
+    assert abs(f - (-np.trace(U.conj().T @ W))) < 1e-12
+    # f(W = U) = -N for unitary circuits
+    f = O.brickwall_unitary_target(O.Ufunc.dense(W), V, P, L)
+    assert abs(f + (1 << L)) < 1e-11
+
+
+def _fd_wirtinger(fun, x, h=1e-6):
+    """Holomorphic derivative d f / d x_k of a function holomorphic in x, by complex-step central differences."""
+    g = np.zeros(x.size, np.complex128)
+    xf = x.reshape(-1)
+    for k in range(xf.size):
+        e = np.zeros_like(xf)
+        e[k] = h
+        dr = (fun((xf + e).reshape(x.shape)) - fun((xf - e).reshape(x.shape))) / (2 * h)
+        di = (fun((xf + 1j * e).reshape(x.shape)) - fun((xf - 1j * e).reshape(x.shape))) / (2 * h)
+        g[k] = 0.5 * (dr - 1j * di)
+    return g.reshape(x.shape)
+
+
+def test_gradient_finite_differences():
+    rng = np.random.default_rng(8)
+    L = 4
+    V = crandn(rng, (3, 4, 4))
+    P = random_perms(rng, L, 3)
+    uf = O.Ufunc.dense(crandn(rng, (1 << L, 1 << L)))
+    f, dV = O.brickwall_unitary_target_and_gradient(uf, V, P, L)
+    num = _fd_wirtinger(lambda x: O.brickwall_unitary_target(uf, x, P, L), V)
+    assert np.max(np.abs(dV - num)) < 1e-8 * np.max(np.abs(num))
+    # the backward sweep gives the same per-layer gradient
+    psi = crandn(rng, 1 << L)
+    dpsi_out = crandn(rng, 1 << L)
+    _, cache = O.brickwall_unitary_forward(V, P, psi)
+    _, dV2 = O.brickwall_unitary_backward(V, P, cache, dpsi_out)
+    num2 = _fd_wirtinger(lambda x: np.dot(dpsi_out, O.apply_brickwall_unitary(x, P, psi)), V)
+    assert np.max(np.abs(dV2 - num2)) < 1e-8 * np.max(np.abs(num2))
+
+
+def test_backward_bilinear_identity():
+    rng = np.random.default_rng(9)
+    L = 6
+    G = crandn(rng, (4, 4))
+    psi, dout = crandn(rng, 1 << L), crandn(rng, 1 << L)
+    for (i, j) in ((1, 4), (4, 1), (2, 3), (5, 0)):
+        dpsi, dG = O.apply_gate_backward(G, i, j, psi, dout)
+        lhs = np.dot(dout, O.apply_gate(G, i, j, psi))
+        assert abs(lhs - np.sum(dG * G)) < 1e-11
+        assert abs(lhs - np.dot(dpsi, psi)) < 1e-11
+
+
+def test_hvp_finite_differences():
+    rng = np.random.default_rng(10)
+    L = 4
+    V = crandn(rng, (3, 4, 4))
+    Z = crandn(rng, (3, 4, 4))
+    P = random_perms(rng, L, 3)
+    uf = O.Ufunc.dense(crandn(rng, (1 << L, 1 << L)))
+    f, dV, HZ = O.brickwall_unitary_target_hvp(uf, V, Z, P, L)
+    h = 1e-5
+    _, gp = O.brickwall_unitary_target_and_gradient(uf, V + h * Z, P, L)
+    _, gm = O.brickwall_unitary_target_and_gradient(uf, V - h * Z, P, L)
+    assert np.max(np.abs(HZ - (gp - gm) / (2 * h))) < 1e-7 * np.max(np.abs(HZ))
+    # circuit-level HVP against the same finite differences, random wires
+    gates = crandn(rng, (5, 4, 4))
+    dirs = crandn(rng, (5, 4, 4))
+    wires = random_wires(rng, L, 5)
+    f, dg, hg = O.circuit_unitary_target_hvp(uf, gates, dirs, wires, L)
+    _, gp = O.circuit_unitary_target_and_gradient(uf, gates + h * dirs, wires, L)
+    _, gm = O.circuit_unitary_target_and_gradient(uf, gates - h * dirs, wires, L)
+    assert np.max(np.abs(hg - (gp - gm) / (2 * h))) < 1e-7 * np.max(np.abs(hg))
+
+
+def test_projected_gradient_and_hvp_second_order_expansion():
+    """With the (second-order) polar retraction R:
+       d/dt Re f(R(t eta))|0 = <grad, eta>,  d2/dt2 Re f(R(t eta))|0 = <eta, Hess eta>."""
+    rng = np.random.default_rng(12)
+    L = 6
+    nl = 3
+    V = np.stack([haar_unitary(rng) for _ in range(nl)])
+    P = brickwall_perms(L, nl)
+    uf = O.Ufunc.dense(np.linalg.qr(crandn(rng, (1 << L, 1 << L)))[0])
+    eta = rng.standard_normal(16 * nl)
+    Zs = np.stack([O.real_to_tangent(eta[16 * i:16 * i + 16], V[i]) for i in range(nl)])
+    f0, g, hv = O.brickwall_unitary_target_projected_hvp(uf, V, Zs, P, L)
+
+    def fr(t):
+        Vt = np.stack([O.retract(V[i], t * eta[16 * i:16 * i + 16]) for i in range(nl)])
+        return O.brickwall_unitary_target(uf, Vt, P, L).real
+
+    h = 1e-4
+    d1 = (fr(h) - fr(-h)) / (2 * h)
+    d2 = (fr(h) - 2 * f0.real + fr(-h)) / h ** 2
+    assert abs(d1 - g @ eta) < 1e-6 * max(1, abs(d1))
+    assert abs(d2 - eta @ hv) < 1e-4 * max(1, abs(d2))
+    # symmetry of the projected Hessian
+    eta2 = rng.standard_normal(16 * nl)
+    Z2 = np.stack([O.real_to_tangent(eta2[16 * i:16 * i + 16], V[i]) for i in range(nl)])
+    _, _, hv2 = O.brickwall_unitary_target_projected_hvp(uf, V, Z2, P, L)
+    assert abs(eta2 @ hv - eta @ hv2) < 1e-10 * max(1, abs(eta @ hv2))
+
+
+def test_matrix_closed_forms():
+    rng = np.random.default_rng(13)
+    U = haar_unitary(rng)
+    assert np.max(np.abs(O.polar_factor(U) - U)) < 1e-14
+    A = crandn(rng, (4, 4))
+    u = O.polar_factor(A)
+    W, _, Vh = np.linalg.svd(A)
+    assert np.max(np.abs(u - W @ Vh)) < 1e-12
+    r, Ainv = O.inverse_matrix(A)
+    assert r == 0 and np.max(np.abs(A @ Ainv - np.eye(4))) < 1e-12
+    r, _ = O.inverse_matrix(np.zeros((4, 4)))
+    assert r == -1
+    x = rng.standard_normal(16)
+    assert np.max(np.abs(O.antisymm_to_real(O.real_to_antisymm(x)) - x)) < 1e-15
+    Wm = O.real_to_antisymm(x)
+    assert np.max(np.abs(Wm + Wm.conj().T)) < 1e-15
+    # tangent projection: idempotent, result is tangent (u^dagger p skew-Hermitian)
+    Z = crandn(rng, (4, 4))
+    p = O.project_tangent(U, Z)
+    assert np.max(np.abs(O.project_tangent(U, p) - p)) < 1e-14
+    s = U.conj().T @ p
+    assert np.max(np.abs(s + s.conj().T)) < 1e-14
+    # retraction lands on U(4) and is first-order V (I + A(eta))
+    eta = 1e-3 * rng.standard_normal(16)
+    R = O.retract(U, eta)
+    assert np.max(np.abs(R.conj().T @ R - np.eye(4))) < 1e-14
+    assert np.max(np.abs(R - U @ (np.eye(4) + O.real_to_antisymm(eta)))) < 1e-5
+
+
+def test_truncated_cg_interior_solution():
+    rng = np.random.default_rng(14)
+    n = 10
+    B = rng.standard_normal((n, n))
+    H = B @ B.T + n * np.eye(n)
+    g = rng.standard_normal(n)
+    z, on = O.truncated_cg_hvp(g, lambda v: H @ v, 1e6, abstol=1e-14, reltol=1e-14)
+    assert not on
+    assert np.max(np.abs(z + np.linalg.solve(H, g))) < 1e-10
+    z, on = O.truncated_cg_hvp(g, lambda v: H @ v, 1e-3)
+    assert on and abs(np.linalg.norm(z) - 1e-3) < 1e-15
+
+
+def test_pcg32_published_vectors():
+    """PCG32 (XSH-RR) reference output for state seed 42, stream 54 (pcg-random.org pcg32-demo)."""
+    class P(ct.Structure):
+        _fields_ = [("state", ct.c_uint64), ("inc", ct.c_uint64)]
+    # drive the oracle's generator 0 directly by seeding through the public struct layout
+    rng = O.OrcRng()
+    # replicate pcg32_srandom_r(42, 54) on generator 0
+    MULT = 6364136223846793005
+    M64 = (1 << 64) - 1
+
+    def step(s, inc):
+        return (s * MULT + inc) & M64
+    inc = ((54 << 1) | 1) & M64
+    s = step(0, inc)
+    s = (s + 42) & M64
+    s = step(s, inc)
+    rng.s0, rng.i0 = s, inc
+    out = [O.lib().orc_rand_uint32(ct.byref(rng)) for _ in range(6)]
+    assert out == [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
+
+
+def test_rng_distribution():
+    r = O.Rng(42)
+    x = np.array([r.randn() for _ in range(20000)])
+    assert abs(x.mean()) < 0.03 and abs(x.std() - 1) < 0.03
+    c = np.array([r.crandn() for _ in range(20000)])
+    assert abs(np.mean(np.abs(c) ** 2) - 1) < 0.03
+
+
+def test_rtr_optimiser_decreases_target():
+    rng = np.random.default_rng(15)
+    L, nl = 4, 3
+    P = brickwall_perms(L, nl)
+    Vt = np.stack([haar_unitary(rng) for _ in range(nl)])
+    U = brickwall_matrix(Vt, P)
+    V0 = np.stack([haar_unitary(rng) for _ in range(nl)])
+    f_iter, vopt = O.optimize_brickwall_circuit_hvp(O.Ufunc.dense(U), V0, P, L, 6)
+    assert f_iter[-1] <= f_iter[0] + 1e-12
+    assert np.all(np.diff(f_iter) <= 1e-9)  # rejected steps keep f, accepted ones decrease it
+    for V in vopt:
+        assert np.max(np.abs(V.conj().T @ V - np.eye(4))) < 1e-13
+
+
+# ==================================================== compiled reference
+REF = os.path.join(os.path.dirname(O.__file__), "_ref", "libref_subset.so")
+needs_ref = pytest.mark.skipif(not os.path.exists(REF), reason="oracle/_ref not built (container-only)")
+
+
+class SV(ct.Structure):
+    _fields_ = [("data", ct.c_void_p), ("nqubits", ct.c_int)]
+
+
+class SVA(ct.Structure):
+    _fields_ = [("data", ct.c_void_p), ("nqubits", ct.c_int), ("nstates", ct.c_int)]
+
+
+class RefRng(ct.Structure):
+    _fields_ = [("s0", ct.c_uint64), ("i0", ct.c_uint64), ("s1", ct.c_uint64), ("i1", ct.c_uint64)]
+
+
+def _ref():
+    R = ct.CDLL(REF)
+    R.randu.restype = ct.c_double
+    R.randn.restype = ct.c_double
+    R.crandn.restype = O.CDouble2
+    R.rand_uint64.restype = ct.c_uint64
+    R.seed_rng_state.argtypes = [ct.c_uint64, ct.POINTER(RefRng)]
+    return R
+
+
+@needs_ref
+def test_ref_gate_kernels_bitwise():
+    R = _ref()
+    rng = np.random.default_rng(16)
+    L = 7
+    G = crandn(rng, (4, 4))
+    psi, dout = crandn(rng, 1 << L), crandn(rng, 1 << L)
+    for (i, j) in ((0, 1), (1, 0), (2, 6), (6, 2), (3, 4), (6, 0)):
+        out = np.empty_like(psi)
+        R.apply_gate(G.ctypes.data_as(ct.c_void_p), i, j, ct.byref(SV(psi.ctypes.data, L)), ct.byref(SV(out.ctypes.data, L)))
+        assert np.array_equal(out, O.apply_gate(G, i, j, psi))
+        dpsi = np.empty_like(psi)
+        dG = np.empty((4, 4), np.complex128)
+        R.apply_gate_backward(G.ctypes.data_as(ct.c_void_p), i, j, ct.byref(SV(psi.ctypes.data, L)),
+                              ct.byref(SV(dout.ctypes.data, L)), ct.byref(SV(dpsi.ctypes.data, L)),
+                              dG.ctypes.data_as(ct.c_void_p))
+        d2, dG2 = O.apply_gate_backward(G, i, j, psi, dout)
+        assert np.array_equal(dpsi, d2) and np.array_equal(dG, dG2)
+        arr = crandn(rng, ((1 << L), 3))
+        outa = np.empty_like(arr)
+        R.apply_gate_to_array(G.ctypes.data_as(ct.c_void_p), i, j, ct.byref(SVA(arr.ctypes.data, L, 3)),
+                              ct.byref(SVA(outa.ctypes.data, L, 3)))
+        assert np.array_equal(outa, O.apply_gate_to_array(G, i, j, arr))
+
+
+@needs_ref
+def test_ref_matrix_functions_bitwise():
+    R = _ref()
+    rng = np.random.default_rng(17)
+    p = lambda a: a.ctypes.data_as(ct.c_void_p)
+    for _ in range(5):
+        a, b, c = crandn(rng, (4, 4)), crandn(rng, (4, 4)), crandn(rng, (4, 4))
+        r = rng.standard_normal(16)
+        out = np.empty((4, 4), np.complex128)
+        R.multiply_matrices(p(a), p(b), p(out)); assert np.array_equal(out, O.multiply(a, b))
+        R.project_tangent(p(a), p(b), p(out)); assert np.array_equal(out, O.project_tangent(a, b))
+        R.symmetric_triple_matrix_product(p(a), p(b), p(c), p(out))
+        assert np.array_equal(out, O.symmetric_triple_matrix_product(a, b, c))
+        R.polar_factor(p(a), p(out)); assert np.array_equal(out, O.polar_factor(a))
+        R.real_to_tangent(p(r), p(a), p(out)); assert np.array_equal(out, O.real_to_tangent(r, a))
+        rr = np.empty(16)
+        R.tangent_to_real(p(a), p(b), p(rr)); assert np.array_equal(rr, O.tangent_to_real(a, b))
+        R.retract(p(haar_unitary(rng)), p(0.1 * r), p(out))
+        u = haar_unitary(rng)
+        R.retract(p(u), p(0.1 * r), p(out)); assert np.array_equal(out, O.retract(u, 0.1 * r))
+        inv = np.empty((4, 4), np.complex128)
+        R.inverse_matrix(p(a), p(inv)); assert np.array_equal(inv, O.inverse_matrix(a)[1])
+
+
+@needs_ref
+def test_ref_rng_sequences_bitwise():
+    R = _ref()
+    for seed in (0, 1, 42, 2 ** 40 + 7):
+        rs = RefRng()
+        R.seed_rng_state(ct.c_uint64(seed), ct.byref(rs))
+        mine = O.Rng(seed)
+        assert (rs.s0, rs.i0, rs.s1, rs.i1) == (mine.s.s0, mine.s.i0, mine.s.s1, mine.s.i1)
+        for _ in range(50):
+            assert R.rand_uint64(ct.byref(rs)) == mine.uint64()
+            assert R.randu(ct.byref(rs)) == mine.randu()
+            assert R.randn(ct.byref(rs)) == mine.randn()
+            c = R.crandn(ct.byref(rs))
+            assert complex(c.re, c.im) == mine.crandn()
