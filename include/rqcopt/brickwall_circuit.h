@@ -1,0 +1,26 @@
+/*
+ * Brickwall circuits: layer i applies the shared gate vlist[i] to the pairs
+ * (perms[i][2k], perms[i][2k+1]), k = 0 .. nqubits/2-1; nqubits must be even.
+ * Replaces reference src/brickwall_circuit.h:8-15 (the three functions below).
+ */
+#pragma once
+
+#include "matrix.h"
+#include "statevector.h"
+#include "quantum_circuit.h"
+
+RQC_EXTERN_C_BEGIN
+
+/* reference src/brickwall_circuit.c:14-44 */
+int apply_brickwall_unitary(const struct mat4x4 vlist[], const int nlayers, const int* perms[],
+	const struct statevector* RQC_RESTRICT psi, struct statevector* RQC_RESTRICT psi_out);
+
+/* reference src/brickwall_circuit.c:51-78; cache holds nlayers*nqubits/2 states */
+int brickwall_unitary_forward(const struct mat4x4 vlist[], int nlayers, const int* perms[],
+	const struct statevector* RQC_RESTRICT psi, struct quantum_circuit_cache* cache, struct statevector* RQC_RESTRICT psi_out);
+
+/* reference src/brickwall_circuit.c:85-124; dvlist[i] sums the gradients of the pairs of layer i */
+int brickwall_unitary_backward(const struct mat4x4 vlist[], int nlayers, const int* perms[], const struct quantum_circuit_cache* cache,
+	const struct statevector* RQC_RESTRICT dpsi_out, struct statevector* RQC_RESTRICT dpsi, struct mat4x4 dvlist[]);
+
+RQC_EXTERN_C_END

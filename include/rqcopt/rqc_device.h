@@ -1,0 +1,78 @@
+/*
+ * GPU-specific additions to the C ABI (MI355X / gfx950).  None of these
+ * replaces a reference function: they are additive and do not change the
+ * signatures of gate.h, statevector.h, quantum_circuit.h, brickwall_circuit.h
+ * or target.h.
+ *
+ * Device-resident targets: the reference passes U as a host callback that is
+ * invoked once per basis column (reference src/target.h:7).  A callback cannot
+ * run inside a GPU sweep, so this library exports a sentinel callback,
+ * rqc_device_ufunc.  Passing it as `ufunc` with an rqc_target* as `udata`
+ * keeps U resident in HBM across calls; the trust-region driver and any
+ * other caller of target.h use it unchanged.
+ */
+#pragma once
+
+#include <stddef.h>
+#include "config.h"
+#include "matrix.h"
+#include "statevector.h"
+#include "target.h"
+
+struct rqc_target;  /* opaque: a dense 2^L x 2^L complex128 U in device memory */
+
+enum rqc_precision
+{
+	RQC_PRECISION_FP64  = 0,  /* complex FP64 sweeps and reductions */
+	RQC_PRECISION_MIXED = 1,  /* complex FP32 sweeps, FP64 reductions, FP64 projection */
+};
+
+enum rqc_kernel_kind
+{
+	RQC_KERNEL_FORWARD = 0,   /* forward sweep of one gate over the column block */
+	RQC_KERNEL_REVERSE = 1,   /* reverse (adjoint) sweep of one gate with gradient / HVP reductions */
+	RQC_KERNEL_TARGET  = 2,   /* -conj(U) seed and trace reduction */
+	RQC_KERNEL_OTHER   = 3,   /* fills, partial-sum reductions, projection */
+	RQC_KERNEL_KINDS   = 4,
+};
+
+RQC_EXTERN_C_BEGIN
+
+/* Sentinel linear_func.  Called directly it computes psi_out = U psi with U
+   taken from the rqc_target in fdata (on the GPU). */
+int rqc_device_ufunc(const struct statevector* RQC_RESTRICT psi, void* fdata, struct statevector* RQC_RESTRICT psi_out);
+
+/* U given densely (row-major, 2^nqubits x 2^nqubits; host or device pointer): copied to device memory */
+int rqc_target_create_dense(int nqubits, const numeric* U, struct rqc_target** target);
+/* U assembled once from a host callback (2^nqubits calls) and kept on the device */
+int rqc_target_create_from_ufunc(linear_func ufunc, void* udata, int nqubits, struct rqc_target** target);
+/* U = brickwall circuit (e.g. a Trotter product of exact bond layers), formed on the GPU */
+int rqc_target_create_brickwall(int nqubits, const struct mat4x4 vlist[], int nlayers, const int* perms[], struct rqc_target** target);
+void rqc_target_destroy(struct rqc_target* target);
+int rqc_target_nqubits(const struct rqc_target* target);
+/* copy U back to a host buffer of 4^nqubits numerics */
+int rqc_target_download(const struct rqc_target* target, numeric* U);
+
+/* precision of the target / gradient / HVP sweeps (process-wide; default FP64) */
+int rqc_set_precision(int mode);
+int rqc_get_precision(void);
+
+/* device workspace: upper bound in bytes (0 = automatic, about 85% of free HBM);
+   the workspace is kept between calls and released here */
+int rqc_set_workspace_limit(size_t bytes);
+void rqc_release_workspace(void);
+/* column-block width used by the last target evaluation */
+long rqc_last_column_block(void);
+
+/* kernel timing with HIP events on the library's stream: when enabled, every
+   launch is bracketed by events and its duration is added per kind */
+void rqc_profile_enable(int on);
+void rqc_profile_reset(void);
+int rqc_profile_query(int kind, double* total_ms, long* launches);
+
+/* blocks until all work of the library's stream has finished */
+int rqc_synchronize(void);
+/* "gfx950 <name> <CUs> <HBM bytes>" into buf; returns 0 when a device is present */
+int rqc_device_info(char* buf, int len);
+
+RQC_EXTERN_C_END

@@ -1,0 +1,80 @@
+/*
+ * Targets f = -Tr(U^dagger C) of a circuit C, their gate gradients and
+ * Hessian-vector products, evaluated on the GPU over all 2^nqubits basis
+ * columns at once.  Replaces reference src/target.h:7-67 (the ten functions
+ * below; the sampling, full-Hessian and block-encoding variants are not part
+ * of this library).
+ *
+ * `ufunc` applies U to one state (reference src/target.h:7).  Two kinds are
+ * accepted:
+ *   - rqc_device_ufunc (see rqc_device.h) with an rqc_target descriptor as
+ *     udata: U stays resident in GPU memory, no host callbacks;
+ *   - any other host callback: it is called once per basis column (as in the
+ *     reference) to assemble U, which is then uploaded for the GPU sweeps.
+ * Return value: 0 on success; -1 allocation/device failure, -2 ufunc failure,
+ * -3 invalid arguments.  Messages go to stderr.
+ *
+ * Note the argument order: the Hessian-vector functions take
+ * (vlist, vdirs, nlayers, perms, nqubits, ...), the others
+ * (vlist, nlayers, nqubits, perms, ...), as in the reference.
+ */
+#pragma once
+
+#include "matrix.h"
+#include "statevector.h"
+
+typedef int (*linear_func)(const struct statevector* RQC_RESTRICT psi, void* fdata, struct statevector* RQC_RESTRICT psi_out);
+
+RQC_EXTERN_C_BEGIN
+
+/* reference src/target.c:15-75 */
+int circuit_unitary_target(linear_func ufunc, void* udata,
+	const struct mat4x4 gates[], const int ngates, const int wires[], const int nqubits,
+	numeric* fval);
+
+/* reference src/target.c:158-247 */
+int circuit_unitary_target_and_gradient(linear_func ufunc, void* udata,
+	const struct mat4x4 gates[], const int ngates, const int wires[], const int nqubits,
+	numeric* fval, struct mat4x4 dgates[]);
+
+/* reference src/target.c:256-280 */
+int circuit_unitary_target_and_projected_gradient(linear_func ufunc, void* udata,
+	const struct mat4x4 gates[], const int ngates, const int wires[], const int nqubits,
+	numeric* fval, double* grad_vec);
+
+/* reference src/target.c:289-380 */
+int circuit_unitary_target_hessian_vector_product(linear_func ufunc, void* udata,
+	const struct mat4x4 gates[], const struct mat4x4 gatedirs[], const int ngates, const int wires[], const int nqubits,
+	numeric* fval, struct mat4x4 dgates[], struct mat4x4 hess_gatedirs[]);
+
+/* reference src/target.c:389-436 */
+int circuit_unitary_target_projected_hessian_vector_product(linear_func ufunc, void* udata,
+	const struct mat4x4 gates[], const struct mat4x4 gatedirs[], const int ngates, const int wires[], const int nqubits,
+	numeric* fval, double* RQC_RESTRICT grad_vec, double* RQC_RESTRICT hvp_vec);
+
+/* reference src/target.c:467-481 */
+int brickwall_unitary_target(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	numeric* fval);
+
+/* reference src/target.c:514-542 */
+int brickwall_unitary_target_and_gradient(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	numeric* fval, struct mat4x4 dvlist[]);
+
+/* reference src/target.c:551-575 */
+int brickwall_unitary_target_and_projected_gradient(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	numeric* fval, double* grad_vec);
+
+/* reference src/target.c:584-622 */
+int brickwall_unitary_target_hessian_vector_product(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const struct mat4x4 vdirs[], const int nlayers, const int* perms[], const int nqubits,
+	numeric* fval, struct mat4x4 dvlist[], struct mat4x4 hess_vdirs[]);
+
+/* reference src/target.c:631-678 — one "grad+HVP eval" */
+int brickwall_unitary_target_projected_hessian_vector_product(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const struct mat4x4 vdirs[], const int nlayers, const int* perms[], const int nqubits,
+	numeric* fval, double* RQC_RESTRICT grad_vec, double* RQC_RESTRICT hvp_vec);
+
+RQC_EXTERN_C_END

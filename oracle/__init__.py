@@ -29,6 +29,10 @@ class CDouble2(ct.Structure):
     _fields_ = [("re", ct.c_double), ("im", ct.c_double)]
 
 
+class BrickwallOp(ct.Structure):
+    _fields_ = [("vlist", ct.c_void_p), ("perms", ct.c_void_p), ("nlayers", ct.c_int)]
+
+
 class OrcRng(ct.Structure):
     _fields_ = [("s0", ct.c_uint64), ("i0", ct.c_uint64), ("s1", ct.c_uint64), ("i1", ct.c_uint64)]
 
@@ -82,6 +86,14 @@ class Ufunc:
     def scramble():
         return Ufunc("scramble")
 
+    @staticmethod
+    def brickwall(vlist, perms):
+        """U applied matrix-free as a brickwall circuit (cheap columns for large L)."""
+        vl, pm = _cplx(vlist), _i32(perms)
+        u = Ufunc("brickwall", (vl, pm))
+        u._keep = BrickwallOp(vl.ctypes.data, pm.ctypes.data, len(vl))
+        return u
+
     def args(self):
         L = lib()
         if self.kind == "dense":
@@ -89,6 +101,8 @@ class Ufunc:
             return fp, _p(self.payload)
         if self.kind == "scramble":
             return ct.cast(L.orc_ufunc_scramble, ct.c_void_p), None
+        if self.kind == "brickwall":
+            return ct.cast(L.orc_ufunc_brickwall, ct.c_void_p), ct.cast(ct.byref(self._keep), ct.c_void_p)
         raise ValueError(self.kind)
 
 

@@ -332,6 +332,12 @@ int orc_ufunc_scramble(int nqubits, const ocplx* psi, void* udata, ocplx* psi_ou
 	return 0;
 }
 
+int orc_ufunc_brickwall(int nqubits, const ocplx* psi, void* udata, ocplx* psi_out)
+{
+	const struct orc_brickwall_op* op = udata;
+	return orc_apply_brickwall_unitary(op->vlist, op->nlayers, op->perms, nqubits, psi, psi_out);
+}
+
 /* evaluates -conj(U e_b) into upsi, reference src/target.c:37-54 */
 static int adjoint_seed(orc_linear_func ufunc, void* udata, int L, long b, ocplx* psi, ocplx* upsi)
 {

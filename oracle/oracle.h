@@ -83,6 +83,9 @@ int orc_brickwall_unitary_target_projected_hessian_vector_product(orc_linear_fun
 /* built-in ufuncs */
 int orc_ufunc_dense(int nqubits, const ocplx* psi, void* udata, ocplx* psi_out);     /* udata = row-major N x N U */
 int orc_ufunc_scramble(int nqubits, const ocplx* psi, void* udata, ocplx* psi_out);  /* reference test map */
+/* U given matrix-free as a brickwall circuit (udata = struct orc_brickwall_op*) */
+struct orc_brickwall_op { const ocplx* vlist; const int* perms; int nlayers; };
+int orc_ufunc_brickwall(int nqubits, const ocplx* psi, void* udata, ocplx* psi_out);
 
 /* column-range partial evaluation used for the multi-core CPU baseline:
    accumulates f, dvlist, hess_vdirs over basis columns [b0, b1) only */

@@ -1,0 +1,669 @@
+"""rqcopt_hpc_amd — MI355X-native (gfx950) gate-sweep engine for Riemannian
+quantum-circuit optimisation.
+
+The product is the C-ABI shared library rqcopt_hpc_amd/librqcopt_hip.so
+(HIP kernels + host driver), declared in include/rqcopt/*.h with the
+reference's function names and argument orders (gate.h, statevector.h,
+quantum_circuit.h, brickwall_circuit.h, target.h, matrix.h, rng.h,
+trust_region.h, brickwall_opt.h) plus rqc_device.h.  This module binds it
+with ctypes and offers a numpy-level mirror of the same API:
+    apply_gate, apply_gate_backward, apply_gate_to_array,
+    apply_quantum_circuit, quantum_circuit_forward, quantum_circuit_backward,
+    quantum_circuit_gates_hessian_vector_product,
+    apply_brickwall_unitary, brickwall_unitary_forward, brickwall_unitary_backward,
+    circuit_unitary_target[_and_gradient|_and_projected_gradient|
+        _hessian_vector_product|_projected_hessian_vector_product],
+    brickwall_unitary_target[...same...],
+    optimize_brickwall_circuit_hvp, matrix and RNG helpers.
+Errors: a negative return code of the C ABI raises RqcError(code).  There is
+no CPU fallback: without a HIP GPU the compute entry points raise.
+"""
+import ctypes as ct
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librqcopt_hip.so")
+_LIB = None
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+KERNEL_FORWARD, KERNEL_REVERSE, KERNEL_TARGET, KERNEL_OTHER = range(4)
+
+
+class RqcError(RuntimeError):
+    def __init__(self, where, code):
+        super().__init__(f"{where} failed with code {code}")
+        self.code = code
+
+
+# --------------------------------------------------------------- ABI structs
+class Statevector(ct.Structure):
+    _fields_ = [("data", ct.c_void_p), ("nqubits", ct.c_int)]
+
+
+class StatevectorArray(ct.Structure):
+    _fields_ = [("data", ct.c_void_p), ("nqubits", ct.c_int), ("nstates", ct.c_int)]
+
+
+class QuantumCircuitCache(ct.Structure):
+    _fields_ = [("psi_list", ct.POINTER(Statevector)), ("nqubits", ct.c_int), ("ngates", ct.c_int)]
+
+
+class Pcg32(ct.Structure):
+    _fields_ = [("state", ct.c_uint64), ("inc", ct.c_uint64)]
+
+
+class RngState(ct.Structure):
+    _fields_ = [("gen", Pcg32 * 2)]
+
+
+class TruncatedCgParams(ct.Structure):
+    _fields_ = [("maxiter", ct.c_int), ("abstol", ct.c_double), ("reltol", ct.c_double)]
+
+
+TARGET_FUNC = ct.CFUNCTYPE(ct.c_double, ct.c_void_p, ct.c_void_p)
+
+
+class RtrParams(ct.Structure):
+    _fields_ = [("tcg_params", TruncatedCgParams), ("rho_trust", ct.c_double), ("radius_init", ct.c_double),
+                ("maxradius", ct.c_double), ("g_func", ct.c_void_p), ("g_data", ct.c_void_p),
+                ("g_iter", ct.c_void_p)]
+
+
+class CDouble2(ct.Structure):
+    _fields_ = [("re", ct.c_double), ("im", ct.c_double)]
+
+
+LINEAR_FUNC = ct.CFUNCTYPE(ct.c_int, ct.POINTER(Statevector), ct.c_void_p, ct.POINTER(Statevector))
+HVP_FUNC = ct.CFUNCTYPE(None, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double))
+
+# every function declared in include/rqcopt/*.h
+EXPORTED = [
+    # statevector.h
+    "allocate_statevector", "free_statevector", "add_statevectors", "allocate_statevector_array",
+    "free_statevector_array",
+    # gate.h
+    "apply_gate", "apply_gate_backward", "apply_gate_to_array",
+    # quantum_circuit.h
+    "apply_quantum_circuit", "allocate_quantum_circuit_cache", "free_quantum_circuit_cache",
+    "quantum_circuit_forward", "quantum_circuit_backward", "quantum_circuit_gates_hessian_vector_product",
+    # brickwall_circuit.h
+    "apply_brickwall_unitary", "brickwall_unitary_forward", "brickwall_unitary_backward",
+    # target.h
+    "circuit_unitary_target", "circuit_unitary_target_and_gradient", "circuit_unitary_target_and_projected_gradient",
+    "circuit_unitary_target_hessian_vector_product", "circuit_unitary_target_projected_hessian_vector_product",
+    "brickwall_unitary_target", "brickwall_unitary_target_and_gradient",
+    "brickwall_unitary_target_and_projected_gradient", "brickwall_unitary_target_hessian_vector_product",
+    "brickwall_unitary_target_projected_hessian_vector_product",
+    # matrix.h
+    "zero_matrix", "identity_matrix", "scale_matrix", "conjugate_matrix", "add_matrix", "sub_matrix",
+    "add_matrices", "sub_matrices", "transpose", "adjoint", "symm", "antisymm", "real_to_antisymm",
+    "antisymm_to_real", "real_to_tangent", "tangent_to_real", "project_tangent", "multiply_matrices",
+    "symmetric_triple_matrix_product", "inverse_matrix", "polar_factor", "retract",
+    # rng.h
+    "pcg32_srandom_r", "pcg32_random_r", "pcg32x2_srandom_r", "pcg32x2_random_r", "seed_rng_state",
+    "rand_uint32", "rand_uint64", "randu", "randn", "crandn",
+    # trust_region.h, brickwall_opt.h
+    "truncated_cg_hvp", "riemannian_trust_region_optimize_hvp", "optimize_brickwall_circuit_hvp",
+    # rqc_device.h
+    "rqc_device_ufunc", "rqc_target_create_dense", "rqc_target_create_from_ufunc", "rqc_target_create_brickwall",
+    "rqc_target_destroy", "rqc_target_nqubits", "rqc_target_download", "rqc_set_precision", "rqc_get_precision",
+    "rqc_set_workspace_limit", "rqc_release_workspace", "rqc_last_column_block", "rqc_profile_enable",
+    "rqc_profile_reset", "rqc_profile_query", "rqc_synchronize", "rqc_device_info",
+]
+
+
+def lib():
+    """Load librqcopt_hip.so (built in-tree); raises if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m rqcopt_hpc_amd.build`")
+        L = ct.CDLL(LIB_PATH)
+        L.randu.restype = ct.c_double
+        L.randn.restype = ct.c_double
+        L.crandn.restype = CDouble2
+        L.rand_uint64.restype = ct.c_uint64
+        L.rand_uint32.restype = ct.c_uint32
+        L.pcg32_random_r.restype = ct.c_uint32
+        L.pcg32x2_random_r.restype = ct.c_uint64
+        L.seed_rng_state.argtypes = [ct.c_uint64, ct.POINTER(RngState)]
+        L.truncated_cg_hvp.restype = ct.c_bool
+        L.rqc_last_column_block.restype = ct.c_long
+        L.rqc_set_workspace_limit.argtypes = [ct.c_size_t]
+        L.rqc_profile_query.argtypes = [ct.c_int, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long)]
+        _LIB = L
+    return _LIB
+
+
+def device_info():
+    buf = ct.create_string_buffer(256)
+    r = lib().rqc_device_info(buf, 256)
+    return r, buf.value.decode()
+
+
+def require_gpu():
+    r, info = device_info()
+    if r != 0:
+        raise RuntimeError(f"librqcopt_hip needs a HIP GPU: {info}")
+    return info
+
+
+def _check(where, r):
+    if r < 0:
+        raise RqcError(where, r)
+    return r
+
+
+def _cplx(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+def _p(a):
+    return ct.c_void_p(a.ctypes.data)
+
+
+def _nq(n):
+    L = int(n).bit_length() - 1
+    if (1 << L) != n:
+        raise ValueError(f"length {n} is not a power of two")
+    return L
+
+
+def _sv(a):
+    a = _cplx(a)
+    return Statevector(a.ctypes.data, _nq(a.size))
+
+
+def _gates(g):
+    g = _cplx(g).reshape(-1, 4, 4)
+    return np.ascontiguousarray(g)
+
+
+def _perms(perms):
+    perms = np.ascontiguousarray(perms, dtype=np.int32)
+    rows = (ct.POINTER(ct.c_int) * perms.shape[0])(*[perms[i].ctypes.data_as(ct.POINTER(ct.c_int))
+                                                     for i in range(perms.shape[0])])
+    return perms, rows
+
+
+def _wires(w):
+    return np.ascontiguousarray(w, dtype=np.int32).reshape(-1)
+
+
+# --------------------------------------------------------------- targets
+class DeviceTarget:
+    """U resident in device memory; pass it wherever a target is expected.
+
+    Built from a dense matrix (DeviceTarget.dense), from a host callable
+    assembled once (DeviceTarget.from_ufunc), or as a brickwall circuit formed
+    on the GPU (DeviceTarget.brickwall, e.g. a Trotter product)."""
+
+    def __init__(self, handle, nqubits):
+        self.handle = ct.c_void_p(handle)
+        self.nqubits = nqubits
+
+    @staticmethod
+    def dense(U):
+        U = _cplx(U)
+        L = _nq(U.shape[0])
+        h = ct.c_void_p()
+        _check("rqc_target_create_dense", lib().rqc_target_create_dense(L, _p(U), ct.byref(h)))
+        return DeviceTarget(h.value, L)
+
+    @staticmethod
+    def from_ufunc(fn, nqubits):
+        cb = HostUfunc(fn)
+        h = ct.c_void_p()
+        _check("rqc_target_create_from_ufunc",
+               lib().rqc_target_create_from_ufunc(cb.cfunc, None, nqubits, ct.byref(h)))
+        return DeviceTarget(h.value, nqubits)
+
+    @staticmethod
+    def brickwall(vlist, perms):
+        vlist = _gates(vlist)
+        perms, rows = _perms(perms)
+        h = ct.c_void_p()
+        _check("rqc_target_create_brickwall",
+               lib().rqc_target_create_brickwall(perms.shape[1], _p(vlist), len(vlist), rows, ct.byref(h)))
+        return DeviceTarget(h.value, perms.shape[1])
+
+    def download(self):
+        n = 1 << self.nqubits
+        U = np.empty((n, n), np.complex128)
+        _check("rqc_target_download", lib().rqc_target_download(self.handle, _p(U)))
+        return U
+
+    def ufunc_args(self):
+        return ct.cast(lib().rqc_device_ufunc, ct.c_void_p), self.handle
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            lib().rqc_target_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HostUfunc:
+    """A Python callable psi -> U psi exposed as a reference-style linear_func."""
+
+    def __init__(self, fn):
+        self.fn = fn
+
+        def cb(psi, udata, out):
+            n = 1 << psi.contents.nqubits
+            x = np.ctypeslib.as_array(ct.cast(psi.contents.data, ct.POINTER(ct.c_double)), shape=(2 * n,))
+            y = np.ctypeslib.as_array(ct.cast(out.contents.data, ct.POINTER(ct.c_double)), shape=(2 * n,))
+            try:
+                r = np.asarray(self.fn(x.view(np.complex128).copy()), dtype=np.complex128)
+                y.view(np.complex128)[:] = r
+                return 0
+            except Exception:
+                return -1
+
+        self.cfunc = LINEAR_FUNC(cb)
+
+    def ufunc_args(self):
+        return ct.cast(self.cfunc, ct.c_void_p), None
+
+
+def _target_args(target):
+    if isinstance(target, (DeviceTarget, HostUfunc)):
+        return target.ufunc_args()
+    if callable(target):
+        raise TypeError("wrap Python callables in HostUfunc (and keep a reference to it)")
+    raise TypeError("target must be a DeviceTarget or a HostUfunc")
+
+
+# --------------------------------------------------------------- gate level
+def apply_gate(V, i, j, psi):
+    psi = _cplx(psi)
+    out = np.empty_like(psi)
+    lib().apply_gate(_p(_cplx(V)), i, j, ct.byref(_sv(psi)), ct.byref(_sv(out)))
+    return out
+
+
+def apply_gate_backward(V, i, j, psi, dpsi_out):
+    psi, dpsi_out = _cplx(psi), _cplx(dpsi_out)
+    dpsi = np.empty_like(psi)
+    dV = np.empty((4, 4), np.complex128)
+    lib().apply_gate_backward(_p(_cplx(V)), i, j, ct.byref(_sv(psi)), ct.byref(_sv(dpsi_out)), ct.byref(_sv(dpsi)), _p(dV))
+    return dpsi, dV
+
+
+def apply_gate_to_array(V, i, j, psi):
+    """psi: (2^L, nstates) interleaved statevector array."""
+    psi = _cplx(psi)
+    out = np.empty_like(psi)
+    L = _nq(psi.shape[0])
+    a = StatevectorArray(psi.ctypes.data, L, psi.shape[1])
+    b = StatevectorArray(out.ctypes.data, L, psi.shape[1])
+    lib().apply_gate_to_array(_p(_cplx(V)), i, j, ct.byref(a), ct.byref(b))
+    return out
+
+
+def add_statevectors(a, b):
+    a, b = _cplx(a), _cplx(b)
+    out = np.empty_like(a)
+    lib().add_statevectors(ct.byref(_sv(a)), ct.byref(_sv(b)), ct.byref(_sv(out)))
+    return out
+
+
+# --------------------------------------------------------------- circuit level
+def apply_quantum_circuit(gates, wires, psi):
+    gates, w, psi = _gates(gates), _wires(wires), _cplx(psi)
+    out = np.empty_like(psi)
+    _check("apply_quantum_circuit", lib().apply_quantum_circuit(_p(gates), len(gates), _p(w), ct.byref(_sv(psi)),
+                                                                ct.byref(_sv(out))))
+    return out
+
+
+class _Cache:
+    """quantum_circuit_cache whose statevectors are numpy arrays."""
+
+    def __init__(self, L, ngates, states=None):
+        n = 1 << L
+        self.states = np.zeros((ngates, n), np.complex128) if states is None else _cplx(states)
+        self.svs = (Statevector * ngates)(*[Statevector(self.states[k].ctypes.data, L) for k in range(ngates)])
+        self.c = QuantumCircuitCache(self.svs, L, ngates)
+
+
+def quantum_circuit_forward(gates, wires, psi):
+    gates, w, psi = _gates(gates), _wires(wires), _cplx(psi)
+    cache = _Cache(_nq(psi.size), len(gates))
+    out = np.empty_like(psi)
+    _check("quantum_circuit_forward", lib().quantum_circuit_forward(_p(gates), len(gates), _p(w), ct.byref(_sv(psi)),
+                                                                    ct.byref(cache.c), ct.byref(_sv(out))))
+    return out, cache.states
+
+
+def quantum_circuit_backward(gates, wires, cache_states, dpsi_out):
+    gates, w, dpsi_out = _gates(gates), _wires(wires), _cplx(dpsi_out)
+    cache = _Cache(_nq(dpsi_out.size), len(gates), cache_states)
+    dpsi = np.empty_like(dpsi_out)
+    dgates = np.empty((len(gates), 4, 4), np.complex128)
+    _check("quantum_circuit_backward", lib().quantum_circuit_backward(_p(gates), len(gates), _p(w), ct.byref(cache.c),
+                                                                      ct.byref(_sv(dpsi_out)), ct.byref(_sv(dpsi)),
+                                                                      _p(dgates)))
+    return dpsi, dgates
+
+
+def quantum_circuit_gates_hessian_vector_product(gates, gatedirs, wires, psi, phi):
+    gates, dirs, w = _gates(gates), _gates(gatedirs), _wires(wires)
+    psi, phi = _cplx(psi), _cplx(phi)
+    out = np.empty_like(psi)
+    dg = np.empty((len(gates), 4, 4), np.complex128)
+    hg = np.empty((len(gates), 4, 4), np.complex128)
+    _check("quantum_circuit_gates_hessian_vector_product",
+           lib().quantum_circuit_gates_hessian_vector_product(_p(gates), _p(dirs), len(gates), _p(w),
+                                                              ct.byref(_sv(psi)), ct.byref(_sv(phi)),
+                                                              ct.byref(_sv(out)), _p(dg), _p(hg)))
+    return out, dg, hg
+
+
+def apply_brickwall_unitary(vlist, perms, psi):
+    vlist, psi = _gates(vlist), _cplx(psi)
+    perms, rows = _perms(perms)
+    out = np.empty_like(psi)
+    _check("apply_brickwall_unitary", lib().apply_brickwall_unitary(_p(vlist), len(vlist), rows, ct.byref(_sv(psi)),
+                                                                    ct.byref(_sv(out))))
+    return out
+
+
+def brickwall_unitary_forward(vlist, perms, psi):
+    vlist, psi = _gates(vlist), _cplx(psi)
+    perms, rows = _perms(perms)
+    L = _nq(psi.size)
+    cache = _Cache(L, len(vlist) * (L // 2))
+    out = np.empty_like(psi)
+    _check("brickwall_unitary_forward", lib().brickwall_unitary_forward(_p(vlist), len(vlist), rows,
+                                                                        ct.byref(_sv(psi)), ct.byref(cache.c),
+                                                                        ct.byref(_sv(out))))
+    return out, cache.states
+
+
+def brickwall_unitary_backward(vlist, perms, cache_states, dpsi_out):
+    vlist, dpsi_out = _gates(vlist), _cplx(dpsi_out)
+    perms, rows = _perms(perms)
+    L = _nq(dpsi_out.size)
+    cache = _Cache(L, len(vlist) * (L // 2), cache_states)
+    dpsi = np.empty_like(dpsi_out)
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    _check("brickwall_unitary_backward", lib().brickwall_unitary_backward(_p(vlist), len(vlist), rows, ct.byref(cache.c),
+                                                                          ct.byref(_sv(dpsi_out)), ct.byref(_sv(dpsi)),
+                                                                          _p(dv)))
+    return dpsi, dv
+
+
+# --------------------------------------------------------------- targets
+def _f():
+    return np.zeros(1, np.complex128)
+
+
+def circuit_unitary_target(target, gates, wires, nqubits):
+    gates, w, f = _gates(gates), _wires(wires), _f()
+    fp, ud = _target_args(target)
+    _check("circuit_unitary_target", lib().circuit_unitary_target(fp, ud, _p(gates), len(gates), _p(w), nqubits, _p(f)))
+    return f[0]
+
+
+def circuit_unitary_target_and_gradient(target, gates, wires, nqubits):
+    gates, w, f = _gates(gates), _wires(wires), _f()
+    dg = np.empty((len(gates), 4, 4), np.complex128)
+    fp, ud = _target_args(target)
+    _check("circuit_unitary_target_and_gradient",
+           lib().circuit_unitary_target_and_gradient(fp, ud, _p(gates), len(gates), _p(w), nqubits, _p(f), _p(dg)))
+    return f[0], dg
+
+
+def circuit_unitary_target_and_projected_gradient(target, gates, wires, nqubits):
+    gates, w, f = _gates(gates), _wires(wires), _f()
+    g = np.empty(16 * len(gates))
+    fp, ud = _target_args(target)
+    _check("circuit_unitary_target_and_projected_gradient",
+           lib().circuit_unitary_target_and_projected_gradient(fp, ud, _p(gates), len(gates), _p(w), nqubits, _p(f), _p(g)))
+    return f[0], g
+
+
+def circuit_unitary_target_hessian_vector_product(target, gates, gatedirs, wires, nqubits):
+    gates, dirs, w, f = _gates(gates), _gates(gatedirs), _wires(wires), _f()
+    dg = np.empty((len(gates), 4, 4), np.complex128)
+    hg = np.empty((len(gates), 4, 4), np.complex128)
+    fp, ud = _target_args(target)
+    _check("circuit_unitary_target_hessian_vector_product",
+           lib().circuit_unitary_target_hessian_vector_product(fp, ud, _p(gates), _p(dirs), len(gates), _p(w), nqubits,
+                                                               _p(f), _p(dg), _p(hg)))
+    return f[0], dg, hg
+
+
+def circuit_unitary_target_projected_hessian_vector_product(target, gates, gatedirs, wires, nqubits):
+    gates, dirs, w, f = _gates(gates), _gates(gatedirs), _wires(wires), _f()
+    g = np.empty(16 * len(gates))
+    h = np.empty(16 * len(gates))
+    fp, ud = _target_args(target)
+    _check("circuit_unitary_target_projected_hessian_vector_product",
+           lib().circuit_unitary_target_projected_hessian_vector_product(fp, ud, _p(gates), _p(dirs), len(gates), _p(w),
+                                                                         nqubits, _p(f), _p(g), _p(h)))
+    return f[0], g, h
+
+
+def brickwall_unitary_target(target, vlist, perms, nqubits=None):
+    vlist, f = _gates(vlist), _f()
+    perms, rows = _perms(perms)
+    L = perms.shape[1] if nqubits is None else nqubits
+    fp, ud = _target_args(target)
+    _check("brickwall_unitary_target", lib().brickwall_unitary_target(fp, ud, _p(vlist), len(vlist), L, rows, _p(f)))
+    return f[0]
+
+
+def brickwall_unitary_target_and_gradient(target, vlist, perms, nqubits=None):
+    vlist, f = _gates(vlist), _f()
+    perms, rows = _perms(perms)
+    L = perms.shape[1] if nqubits is None else nqubits
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    fp, ud = _target_args(target)
+    _check("brickwall_unitary_target_and_gradient",
+           lib().brickwall_unitary_target_and_gradient(fp, ud, _p(vlist), len(vlist), L, rows, _p(f), _p(dv)))
+    return f[0], dv
+
+
+def brickwall_unitary_target_and_projected_gradient(target, vlist, perms, nqubits=None):
+    vlist, f = _gates(vlist), _f()
+    perms, rows = _perms(perms)
+    L = perms.shape[1] if nqubits is None else nqubits
+    g = np.empty(16 * len(vlist))
+    fp, ud = _target_args(target)
+    _check("brickwall_unitary_target_and_projected_gradient",
+           lib().brickwall_unitary_target_and_projected_gradient(fp, ud, _p(vlist), len(vlist), L, rows, _p(f), _p(g)))
+    return f[0], g
+
+
+def brickwall_unitary_target_hessian_vector_product(target, vlist, vdirs, perms, nqubits=None):
+    vlist, vdirs, f = _gates(vlist), _gates(vdirs), _f()
+    perms, rows = _perms(perms)
+    L = perms.shape[1] if nqubits is None else nqubits
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    hz = np.empty((len(vlist), 4, 4), np.complex128)
+    fp, ud = _target_args(target)
+    _check("brickwall_unitary_target_hessian_vector_product",
+           lib().brickwall_unitary_target_hessian_vector_product(fp, ud, _p(vlist), _p(vdirs), len(vlist), rows, L,
+                                                                 _p(f), _p(dv), _p(hz)))
+    return f[0], dv, hz
+
+
+def brickwall_unitary_target_projected_hessian_vector_product(target, vlist, vdirs, perms, nqubits=None):
+    vlist, vdirs, f = _gates(vlist), _gates(vdirs), _f()
+    perms, rows = _perms(perms)
+    L = perms.shape[1] if nqubits is None else nqubits
+    g = np.empty(16 * len(vlist))
+    h = np.empty(16 * len(vlist))
+    fp, ud = _target_args(target)
+    _check("brickwall_unitary_target_projected_hessian_vector_product",
+           lib().brickwall_unitary_target_projected_hessian_vector_product(fp, ud, _p(vlist), _p(vdirs), len(vlist),
+                                                                           rows, L, _p(f), _p(g), _p(h)))
+    return f[0], g, h
+
+
+# --------------------------------------------------------------- optimiser
+def rtr_default_params(n):
+    """set_rtr_default_params (reference src/trust_region.h:68-76)."""
+    return RtrParams(TruncatedCgParams(2 * n, 1e-8, 1e-6), 0.125, 0.01, 0.1, None, None, None)
+
+
+def optimize_brickwall_circuit_hvp(target, vlist_start, perms, niter, params=None):
+    vlist_start = _gates(vlist_start)
+    perms, rows = _perms(perms)
+    nl = len(vlist_start)
+    params = rtr_default_params(16 * nl) if params is None else params
+    f_iter = np.empty(niter + 1)
+    vopt = np.empty_like(vlist_start)
+    fp, ud = _target_args(target)
+    lib().optimize_brickwall_circuit_hvp(fp, ud, _p(vlist_start), nl, perms.shape[1], rows, ct.byref(params), niter,
+                                         _p(f_iter), _p(vopt))
+    return f_iter, vopt
+
+
+def truncated_cg_hvp(grad, hvp_fn, radius, params=None):
+    grad = np.ascontiguousarray(grad, np.float64)
+    n = grad.size
+    params = TruncatedCgParams(2 * n, 1e-8, 1e-6) if params is None else params
+
+    def cb(x, fdata, vec, out):
+        v = np.ctypeslib.as_array(vec, shape=(n,)).copy()
+        np.ctypeslib.as_array(out, shape=(n,))[:] = hvp_fn(v)
+
+    cfun = HVP_FUNC(cb)
+    z = np.empty(n)
+    L = lib()
+    L.truncated_cg_hvp.argtypes = [ct.c_void_p, ct.c_void_p, HVP_FUNC, ct.c_void_p, ct.c_int, ct.c_double,
+                                   ct.POINTER(TruncatedCgParams), ct.c_void_p]
+    on = L.truncated_cg_hvp(None, _p(grad), cfun, None, n, radius, ct.byref(params), _p(z))
+    return z, bool(on)
+
+
+# --------------------------------------------------------------- algebra, RNG
+def _m1(name, a):
+    out = np.empty((4, 4), np.complex128)
+    getattr(lib(), name)(_p(_cplx(a)), _p(out))
+    return out
+
+
+def multiply_matrices(a, b):
+    out = np.empty((4, 4), np.complex128)
+    lib().multiply_matrices(_p(_cplx(a)), _p(_cplx(b)), _p(out))
+    return out
+
+
+def project_tangent(u, z):
+    out = np.empty((4, 4), np.complex128)
+    lib().project_tangent(_p(_cplx(u)), _p(_cplx(z)), _p(out))
+    return out
+
+
+def symmetric_triple_matrix_product(a, b, c):
+    out = np.empty((4, 4), np.complex128)
+    lib().symmetric_triple_matrix_product(_p(_cplx(a)), _p(_cplx(b)), _p(_cplx(c)), _p(out))
+    return out
+
+
+def polar_factor(a):
+    return _m1("polar_factor", a)
+
+
+def inverse_matrix(a):
+    out = np.empty((4, 4), np.complex128)
+    r = lib().inverse_matrix(_p(_cplx(a)), _p(out))
+    return r, out
+
+
+def real_to_tangent(r, v):
+    out = np.empty((4, 4), np.complex128)
+    lib().real_to_tangent(_p(np.ascontiguousarray(r, np.float64)), _p(_cplx(v)), _p(out))
+    return out
+
+
+def tangent_to_real(v, z):
+    out = np.empty(16)
+    lib().tangent_to_real(_p(_cplx(v)), _p(_cplx(z)), _p(out))
+    return out
+
+
+def retract(u, eta):
+    out = np.empty((4, 4), np.complex128)
+    lib().retract(_p(_cplx(u)), _p(np.ascontiguousarray(eta, np.float64)), _p(out))
+    return out
+
+
+class Rng:
+    """seed_rng_state / rand_uint64 / randu / randn / crandn (reference src/rng.c)."""
+
+    def __init__(self, seed):
+        self.s = RngState()
+        lib().seed_rng_state(ct.c_uint64(seed), ct.byref(self.s))
+
+    def uint64(self):
+        return lib().rand_uint64(ct.byref(self.s))
+
+    def randu(self):
+        return lib().randu(ct.byref(self.s))
+
+    def randn(self):
+        return lib().randn(ct.byref(self.s))
+
+    def crandn(self):
+        r = lib().crandn(ct.byref(self.s))
+        return complex(r.re, r.im)
+
+    def haar_unitary(self, n=4):
+        """Haar-random unitary: QR of a complex Ginibre matrix drawn with crandn, phases of R's diagonal
+        moved into Q (Mezzadri 2007)."""
+        z = np.array([[self.crandn() for _ in range(n)] for _ in range(n)])
+        q, r = np.linalg.qr(z)
+        d = np.diag(r)
+        return q * (d / np.abs(d))[None, :]
+
+
+# --------------------------------------------------------------- device controls
+def set_precision(mode):
+    _check("rqc_set_precision", lib().rqc_set_precision(mode))
+
+
+def get_precision():
+    return lib().rqc_get_precision()
+
+
+def set_workspace_limit(nbytes):
+    _check("rqc_set_workspace_limit", lib().rqc_set_workspace_limit(nbytes))
+
+
+def release_workspace():
+    lib().rqc_release_workspace()
+
+
+def last_column_block():
+    return lib().rqc_last_column_block()
+
+
+def profile_enable(on=True):
+    lib().rqc_profile_enable(1 if on else 0)
+
+
+def profile_reset():
+    lib().rqc_profile_reset()
+
+
+def profile_query(kind):
+    ms, n = ct.c_double(), ct.c_long()
+    _check("rqc_profile_query", lib().rqc_profile_query(kind, ct.byref(ms), ct.byref(n)))
+    return ms.value, n.value
+
+
+def synchronize():
+    _check("rqc_synchronize", lib().rqc_synchronize())

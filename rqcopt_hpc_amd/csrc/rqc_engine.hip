@@ -1,0 +1,1280 @@
+// Gate-sweep engine of librqcopt_hip and the GPU entry points of its C ABI
+// (gate.h, statevector.h, quantum_circuit.h, brickwall_circuit.h, target.h,
+// rqc_device.h).
+//
+// Target evaluation (reference src/target.c:158-436) is restructured for the
+// GPU: instead of one basis column at a time, a block of B columns is swept
+// at once as an N x B array (N = 2^L), gate by gate:
+//   forward  psi_{k+1} = G_k psi_k                     (+ dpsi_{k+1} = G_k dpsi_k + Z_k psi_k)
+//   trace    f += sum -conj(U) . psi_G,  phi = -conj(U e_b)
+//   reverse  grad_k += phi_k (x) psi_k, phi_{k-1} = G_k^T phi_k
+//            (+ hess_k += phi_k (x) dpsi_k + dphi_k (x) psi_k, dphi_{k-1} = G_k^T dphi_k + Z_k^T phi_k)
+// which is the reference's per-column forward / backward / tangent sweeps
+// (src/quantum_circuit.c:147-236) evaluated for all columns of the block, with
+// the tangent-forward folded into the forward sweep (dual numbers).
+// The column blocks cover all N columns; B is the largest power of two whose
+// checkpoints fit the device workspace.
+
+#include <algorithm>
+#include <cstring>
+#include <functional>
+#include <vector>
+
+#include "rqc_internal.hpp"
+#include "rqc_kernels.hpp"
+#include "mat4.hpp"
+#include "rqcopt/gate.h"
+#include "rqcopt/statevector.h"
+#include "rqcopt/quantum_circuit.h"
+#include "rqcopt/brickwall_circuit.h"
+#include "rqcopt/target.h"
+#include "rqcopt/rqc_device.h"
+
+namespace rqc {
+
+// ============================================================== context
+void DeviceBuffer::reserve(size_t n)
+{
+	if (n <= bytes) { return; }
+	if (ptr != nullptr) {
+		RQC_HIP(hipDeviceSynchronize());
+		RQC_HIP(hipFree(ptr));
+		ptr = nullptr;
+		bytes = 0;
+	}
+	const size_t r = (n + 4095) & ~(size_t)4095;
+	RQC_HIP(hipMalloc(&ptr, r));
+	bytes = r;
+}
+
+void DeviceBuffer::release()
+{
+	if (ptr != nullptr) {
+		(void)hipFree(ptr);
+		ptr = nullptr;
+		bytes = 0;
+	}
+}
+
+Context& ctx()
+{
+	static Context* c = nullptr;
+	static std::mutex init_mu;
+	std::lock_guard<std::mutex> lk(init_mu);
+	if (c == nullptr) {
+		int n = 0;
+		if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+			throw Error(-1, "librqcopt_hip: no HIP GPU visible (this library has no CPU path)");
+		}
+		auto* cc = new Context();
+		RQC_HIP(hipGetDevice(&cc->device));
+		hipDeviceProp_t p;
+		RQC_HIP(hipGetDeviceProperties(&p, cc->device));
+		cc->cu_count = p.multiProcessorCount;
+		cc->hbm_bytes = p.totalGlobalMem;
+		cc->name = p.gcnArchName;
+		RQC_HIP(hipStreamCreateWithFlags(&cc->stream, hipStreamNonBlocking));
+		c = cc;
+	}
+	return *c;
+}
+
+LaunchTimer::LaunchTimer(Context& cx, int kind) : c(cx)
+{
+	if (!c.profile) { return; }
+	if (c.events_used == c.events.size()) {
+		Context::Ev e;
+		RQC_HIP(hipEventCreate(&e.a));
+		RQC_HIP(hipEventCreate(&e.b));
+		c.events.push_back(e);
+	}
+	ev = &c.events[c.events_used++];
+	ev->kind = kind;
+	RQC_HIP(hipEventRecord(ev->a, c.stream));
+}
+
+LaunchTimer::~LaunchTimer()
+{
+	if (ev != nullptr) { (void)hipEventRecord(ev->b, c.stream); }
+}
+
+void profile_flush(Context& c)
+{
+	if (c.events_used == 0) { return; }
+	RQC_HIP(hipStreamSynchronize(c.stream));
+	for (size_t k = 0; k < c.events_used; k++) {
+		float ms = 0;
+		RQC_HIP(hipEventElapsedTime(&ms, c.events[k].a, c.events[k].b));
+		c.prof_ms[c.events[k].kind] += ms;
+		c.prof_n[c.events[k].kind] += 1;
+	}
+	c.events_used = 0;
+}
+
+bool is_device_pointer(const void* p)
+{
+	if (p == nullptr) { return false; }
+	hipPointerAttribute_t a;
+	if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+		(void)hipGetLastError();
+		return false;
+	}
+	return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged;
+}
+
+void fatal(const char* where, const std::exception& e)
+{
+	fprintf(stderr, "librqcopt_hip: %s failed: %s\n", where, e.what());
+	fflush(stderr);
+	abort();
+}
+
+// ============================================================== helpers
+static inline unsigned nblocks(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+static inline int ilog2_or_neg(long b)
+{
+	if (b <= 0 || (b & (b - 1)) != 0) { return -1; }
+	int l = 0;
+	while ((1L << l) < b) { l++; }
+	return l;
+}
+
+template <typename T>
+static Gate4<T> to_gate(const zc* g)
+{
+	Gate4<T> r;
+	for (int e = 0; e < 16; e++) { r.g[e] = cx<T>{(T)g[e].x, (T)g[e].y}; }
+	return r;
+}
+
+static Wires wires_of(int i, int j, int L) { return Wires{L - 1 - i, L - 1 - j}; }
+
+static void check_wires(int i, int j, int L)
+{
+	if (L < 2 || i < 0 || j < 0 || i >= L || j >= L || i == j) {
+		throw Error(-3, "invalid gate wires (" + std::to_string(i) + ", " + std::to_string(j) + ") for " +
+			std::to_string(L) + " qubits");
+	}
+}
+
+// grid of the grid-stride reduction kernels: fixed for given sizes (bitwise reproducibility)
+static inline unsigned reduce_grid(long nitems)
+{
+	const long want = (nitems + 4L * kBlock - 1) / (4L * kBlock);
+	return (unsigned)std::max(1L, std::min(want, 4096L));
+}
+
+// a sequential circuit of two-qubit gates
+struct Circuit
+{
+	int L = 0;
+	int ngates = 0;
+	std::vector<zc> gates;  // ngates * 16
+	std::vector<zc> dirs;   // ngates * 16 or empty
+	std::vector<int> wi, wj;
+};
+
+static Circuit make_circuit(const struct mat4x4* gates, const struct mat4x4* dirs, int ngates, const int* wires, int L)
+{
+	if (ngates < 1) { throw Error(-3, "a circuit needs at least one gate"); }
+	Circuit C;
+	C.L = L;
+	C.ngates = ngates;
+	C.gates.resize((size_t)ngates * 16);
+	std::memcpy(C.gates.data(), gates, (size_t)ngates * 16 * sizeof(zc));
+	if (dirs != nullptr) {
+		C.dirs.resize((size_t)ngates * 16);
+		std::memcpy(C.dirs.data(), dirs, (size_t)ngates * 16 * sizeof(zc));
+	}
+	for (int k = 0; k < ngates; k++) {
+		check_wires(wires[2*k], wires[2*k + 1], L);
+		C.wi.push_back(wires[2*k]);
+		C.wj.push_back(wires[2*k + 1]);
+	}
+	return C;
+}
+
+// expand a brickwall (reference src/target.c:443-458)
+static Circuit make_brickwall(const struct mat4x4* vlist, const struct mat4x4* vdirs, int nlayers, const int* const* perms, int L)
+{
+	if (L < 2 || L % 2 != 0) { throw Error(-3, "brickwall circuits need an even number of qubits >= 2"); }
+	if (nlayers < 1) { throw Error(-3, "a brickwall circuit needs at least one layer"); }
+	const int per = L / 2;
+	std::vector<struct mat4x4> g((size_t)nlayers * per), d;
+	std::vector<int> w((size_t)nlayers * per * 2);
+	if (vdirs != nullptr) { d.resize((size_t)nlayers * per); }
+	for (int i = 0; i < nlayers; i++) {
+		std::vector<int> seen(L, 0);
+		for (int q = 0; q < L; q++) {
+			const int p = perms[i][q];
+			if (p < 0 || p >= L || seen[p]) { throw Error(-3, "perms[" + std::to_string(i) + "] is not a permutation"); }
+			seen[p] = 1;
+		}
+		for (int k = 0; k < per; k++) {
+			g[(size_t)i * per + k] = vlist[i];
+			if (vdirs != nullptr) { d[(size_t)i * per + k] = vdirs[i]; }
+			w[2 * ((size_t)i * per + k)] = perms[i][2*k];
+			w[2 * ((size_t)i * per + k) + 1] = perms[i][2*k + 1];
+		}
+	}
+	return make_circuit(g.data(), vdirs != nullptr ? d.data() : nullptr, nlayers * per, w.data(), L);
+}
+
+// ============================================================== column-block sweeps
+enum class Mode { Target, Grad, Hvp };
+
+static int states_needed(Mode m, int G)
+{
+	switch (m) {
+	case Mode::Target: return 2;
+	case Mode::Grad:   return (G + 1) + 2;
+	default:           return (G + 1) + G + 4;
+	}
+}
+
+static long choose_block(Context& c, long N, int nstates, size_t elem)
+{
+	size_t budget = c.ws_limit;
+	if (budget == 0) {
+		size_t fr = 0, tot = 0;
+		RQC_HIP(hipMemGetInfo(&fr, &tot));
+		budget = (size_t)(0.85 * (double)(fr + c.states.bytes));
+	}
+	long B = N;
+	while (B > 1 && (size_t)nstates * N * B * elem > budget) { B /= 2; }
+	if ((size_t)nstates * N * B * elem > budget) {
+		throw Error(-1, "device memory too small for " + std::to_string(nstates) + " statevectors of " +
+			std::to_string(N) + " amplitudes");
+	}
+	return B;
+}
+
+// d_acc[0] = f, d_acc[1 + k*32 + e]: e < 16 gradient of gate k, e >= 16 its HVP
+template <typename T>
+static void sweep_targets(Context& c, const Circuit& C, const zc* dU, Mode mode, zc* d_acc, zc* d_partial)
+{
+	const int L = C.L, G = C.ngates;
+	const long N = 1L << L;
+	const int nst = states_needed(mode, G);
+	const long B = choose_block(c, N, nst, sizeof(cx<T>));
+	c.last_block = B;
+	const int logB = ilog2_or_neg(B);
+	const size_t sz = (size_t)N * B;
+	c.states.reserve((size_t)nst * sz * sizeof(cx<T>));
+	cx<T>* S = c.states.as<cx<T>>();
+	auto psi = [&](int k) { return S + (size_t)k * sz; };
+	auto dpsi = [&](int k) { return S + (size_t)(G + k) * sz; };            // k = 1 .. G
+	cx<T>* phi[2] = {S + (size_t)(mode == Mode::Hvp ? 2 * G + 1 : G + 1) * sz,
+	                 S + (size_t)(mode == Mode::Hvp ? 2 * G + 2 : G + 2) * sz};
+	cx<T>* dphi[2] = {S + (size_t)(2 * G + 3) * sz, S + (size_t)(2 * G + 4) * sz};
+
+	{
+		LaunchTimer lt(c, RQC_KERNEL_OTHER);
+		k_fill_zero<<<nblocks(1 + 32L * G), kBlock, 0, c.stream>>>(d_acc, 1 + 32L * G);
+	}
+	std::vector<Gate4<T>> Gt(G), Zt(G);
+	for (int k = 0; k < G; k++) {
+		Gt[k] = to_gate<T>(&C.gates[16 * (size_t)k]);
+		Zt[k] = (mode == Mode::Hvp) ? to_gate<T>(&C.dirs[16 * (size_t)k]) : Gate4<T>{};
+	}
+	const long nq = (N / 4) * B;
+	const unsigned grev = reduce_grid(nq);
+	const unsigned gtr = reduce_grid(N * B);
+
+	for (long b0 = 0; b0 < N; b0 += B) {
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_identity_block<T><<<nblocks(N * B), kBlock, 0, c.stream>>>(psi(0), B, logB, b0, N * B);
+		}
+		// forward sweep
+		for (int k = 0; k < G; k++) {
+			const Wires w = wires_of(C.wi[k], C.wj[k], L);
+			cx<T>* in = (mode == Mode::Target) ? psi(k % 2) : psi(k);
+			cx<T>* out = (mode == Mode::Target) ? psi((k + 1) % 2) : psi(k + 1);
+			LaunchTimer lt(c, RQC_KERNEL_FORWARD);
+			if (mode == Mode::Hvp) {
+				k_gate_forward<T, true><<<nblocks(nq), kBlock, 0, c.stream>>>(in, out, k > 0 ? dpsi(k) : nullptr, dpsi(k + 1),
+					Gt[k], Zt[k], w, B, logB, nq, k == 0 ? 1 : 0);
+			} else {
+				k_gate_forward<T, false><<<nblocks(nq), kBlock, 0, c.stream>>>(in, out, nullptr, nullptr,
+					Gt[k], Gt[k], w, B, logB, nq, 0);
+			}
+		}
+		cx<T>* psiG = (mode == Mode::Target) ? psi(G % 2) : psi(G);
+		{
+			LaunchTimer lt(c, RQC_KERNEL_TARGET);
+			k_target_trace<T><<<gtr, kBlock, 0, c.stream>>>(dU, N, b0, psiG, mode == Mode::Target ? nullptr : phi[0],
+				B, logB, N * B, d_partial);
+		}
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_reduce_partials<<<1, kBlock, 0, c.stream>>>(d_partial, (int)gtr, 1, d_acc);
+		}
+		if (mode == Mode::Target) { continue; }
+		// reverse sweep
+		int cur = 0;
+		for (int k = G - 1; k >= 0; k--) {
+			const Wires w = wires_of(C.wi[k], C.wj[k], L);
+			int flags = (k == 0) ? 4 : 0;
+			{
+				LaunchTimer lt(c, RQC_KERNEL_REVERSE);
+				if (mode == Mode::Hvp) {
+					if (k == 0) { flags |= 1; }
+					if (k == G - 1) { flags |= 2; }
+					k_gate_reverse<T, true><<<grev, kBlock, 0, c.stream>>>(psi(k), k > 0 ? dpsi(k) : nullptr,
+						phi[cur], dphi[cur], phi[1 - cur], dphi[1 - cur], Gt[k], Zt[k], w, B, logB, nq, flags, d_partial);
+				} else {
+					k_gate_reverse<T, false><<<grev, kBlock, 0, c.stream>>>(psi(k), nullptr,
+						phi[cur], nullptr, phi[1 - cur], nullptr, Gt[k], Gt[k], w, B, logB, nq, flags, d_partial);
+				}
+			}
+			{
+				LaunchTimer lt(c, RQC_KERNEL_OTHER);
+				k_reduce_partials<<<kWidth, kBlock, 0, c.stream>>>(d_partial, (int)grev, kWidth, d_acc + 1 + 32L * k);
+			}
+			cur = 1 - cur;
+		}
+	}
+}
+
+// per-group sums of per-gate gradients / HVPs and the Riemannian projection
+// (reference src/target.c:526-535 and :650-671), one thread per group
+__global__ void k_finalize(const zc* __restrict__ acc, int ngroups, int per, const zc* __restrict__ vlist,
+	const zc* __restrict__ vdirs, zc* __restrict__ dv_out, zc* __restrict__ hz_out, double* __restrict__ grad,
+	double* __restrict__ hvp, int project, int want_hvp)
+{
+	const int l = blockIdx.x * blockDim.x + threadIdx.x;
+	if (l >= ngroups) { return; }
+	zc dv[16], hz[16];
+	for (int e = 0; e < 16; e++) { dv[e] = {0, 0}; hz[e] = {0, 0}; }
+	for (int k = 0; k < per; k++) {
+		const zc* a = acc + 1 + 32L * ((long)l * per + k);
+		for (int e = 0; e < 16; e++) { dv[e] += a[e]; hz[e] += a[16 + e]; }
+	}
+	for (int e = 0; e < 16; e++) { dv_out[16 * l + e] = dv[e]; hz_out[16 * l + e] = hz[e]; }
+	if (project) {
+		m4_project_grad_hvp(vlist + 16 * l, want_hvp ? vdirs + 16 * l : nullptr, dv, hz, grad + 16 * l,
+			want_hvp ? hvp + 16 * l : nullptr);
+	}
+}
+
+// results of one target evaluation, on the host
+struct TargetResult
+{
+	zc f = {0, 0};
+	std::vector<zc> dv, hz;      // ngroups * 16
+	std::vector<double> grad, hvp;  // ngroups * 16
+};
+
+// dense U for a target evaluation: device-resident or assembled from the host callback
+static const zc* resolve_target(Context& c, linear_func ufunc, void* udata, int L);
+
+static TargetResult eval_targets(const Circuit& C, linear_func ufunc, void* udata, Mode mode, int per, bool project)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	const zc* dU = resolve_target(c, ufunc, udata, C.L);
+	const int G = C.ngates, ng = G / per;
+	// small buffer: acc | partial | vlist | vdirs | outputs
+	const size_t n_acc = 1 + 32 * (size_t)G;
+	const size_t n_part = 4096 * (size_t)kWidth;
+	const size_t n_in = 2 * 16 * (size_t)ng;
+	const size_t n_out = 2 * 16 * (size_t)ng;
+	const size_t n_real = 2 * 16 * (size_t)ng;
+	c.small.reserve((n_acc + n_part + n_in + n_out) * sizeof(zc) + n_real * sizeof(double));
+	zc* acc = c.small.as<zc>();
+	zc* part = acc + n_acc;
+	zc* vin = part + n_part;
+	zc* out = vin + n_in;
+	double* rout = reinterpret_cast<double*>(out + n_out);
+	if (c.precision == RQC_PRECISION_MIXED) {
+		sweep_targets<float>(c, C, dU, mode, acc, part);
+	} else {
+		sweep_targets<double>(c, C, dU, mode, acc, part);
+	}
+	// gates of each group (the first gate of the group) and directions
+	std::vector<zc> hv(n_in);
+	for (int l = 0; l < ng; l++) {
+		std::memcpy(&hv[16 * (size_t)l], &C.gates[16 * (size_t)l * per], 16 * sizeof(zc));
+		if (mode == Mode::Hvp) { std::memcpy(&hv[16 * ((size_t)ng + l)], &C.dirs[16 * (size_t)l * per], 16 * sizeof(zc)); }
+	}
+	RQC_HIP(hipMemcpyAsync(vin, hv.data(), n_in * sizeof(zc), hipMemcpyHostToDevice, c.stream));
+	if (mode != Mode::Target) {
+		LaunchTimer lt(c, RQC_KERNEL_OTHER);
+		k_finalize<<<(ng + 63) / 64, 64, 0, c.stream>>>(acc, ng, per, vin, vin + 16 * (size_t)ng, out, out + 16 * (size_t)ng,
+			rout, rout + 16 * (size_t)ng, project ? 1 : 0, mode == Mode::Hvp ? 1 : 0);
+	}
+	TargetResult R;
+	R.dv.resize(16 * (size_t)ng);
+	R.hz.resize(16 * (size_t)ng);
+	R.grad.resize(16 * (size_t)ng);
+	R.hvp.resize(16 * (size_t)ng);
+	RQC_HIP(hipMemcpyAsync(&R.f, acc, sizeof(zc), hipMemcpyDeviceToHost, c.stream));
+	if (mode != Mode::Target) {
+		RQC_HIP(hipMemcpyAsync(R.dv.data(), out, 16 * (size_t)ng * sizeof(zc), hipMemcpyDeviceToHost, c.stream));
+		RQC_HIP(hipMemcpyAsync(R.hz.data(), out + 16 * (size_t)ng, 16 * (size_t)ng * sizeof(zc), hipMemcpyDeviceToHost, c.stream));
+		RQC_HIP(hipMemcpyAsync(R.grad.data(), rout, 16 * (size_t)ng * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+		RQC_HIP(hipMemcpyAsync(R.hvp.data(), rout + 16 * (size_t)ng, 16 * (size_t)ng * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+	}
+	RQC_HIP(hipStreamSynchronize(c.stream));
+	RQC_HIP(hipGetLastError());
+	return R;
+}
+
+// ============================================================== device targets
+static constexpr uint64_t kTargetMagic = 0x5251435441524754ULL;  // "RQCTARGT"
+
+}  // namespace rqc
+
+struct rqc_target
+{
+	uint64_t magic;
+	int nqubits;
+	zc* d_U;
+};
+
+namespace rqc {
+
+static const rqc_target* as_target(void* udata, int L)
+{
+	const rqc_target* t = static_cast<const rqc_target*>(udata);
+	if (t == nullptr || t->magic != kTargetMagic) {
+		throw Error(-3, "rqc_device_ufunc needs an rqc_target* as udata");
+	}
+	if (L >= 0 && t->nqubits != L) {
+		throw Error(-3, "rqc_target has " + std::to_string(t->nqubits) + " qubits, circuit has " + std::to_string(L));
+	}
+	return t;
+}
+
+static const zc* resolve_target(Context& c, linear_func ufunc, void* udata, int L)
+{
+	if (ufunc == rqc_device_ufunc) { return as_target(udata, L)->d_U; }
+	if (ufunc == nullptr) { throw Error(-3, "ufunc is NULL"); }
+	// host callback: evaluate it on every basis column, as the reference does
+	// once per column of each evaluation (src/target.c:37-48)
+	const long N = 1L << L;
+	std::vector<zc> U((size_t)N * N), e((size_t)N), col((size_t)N);
+	struct statevector sin = {reinterpret_cast<numeric*>(e.data()), L};
+	struct statevector sout = {reinterpret_cast<numeric*>(col.data()), L};
+	for (long b = 0; b < N; b++) {
+		std::fill(e.begin(), e.end(), zc{0, 0});
+		e[b] = {1, 0};
+		const int r = ufunc(&sin, udata, &sout);
+		if (r < 0) { throw Error(-2, "call of 'ufunc' failed, return value: " + std::to_string(r)); }
+		for (long a = 0; a < N; a++) { U[(size_t)a * N + b] = col[a]; }
+	}
+	c.ubuf.reserve((size_t)N * N * sizeof(zc));
+	RQC_HIP(hipMemcpyAsync(c.ubuf.ptr, U.data(), (size_t)N * N * sizeof(zc), hipMemcpyHostToDevice, c.stream));
+	RQC_HIP(hipStreamSynchronize(c.stream));
+	return c.ubuf.as<zc>();
+}
+
+// ============================================================== statevector staging
+// Device views of the statevectors of one call: host buffers are copied into
+// the library's vector workspace, device buffers are used in place.
+struct Staging
+{
+	struct Item { const void* host; void* user; size_t bytes; bool in, out, dev; size_t off; };
+	std::vector<Item> items;
+	Context& c;
+	explicit Staging(Context& cx) : c(cx) {}
+
+	int add(const void* p, size_t bytes, bool in, bool out)
+	{
+		Item it{p, const_cast<void*>(p), bytes, in, out, is_device_pointer(p), 0};
+		items.push_back(it);
+		return (int)items.size() - 1;
+	}
+	int scratch(size_t bytes) { items.push_back(Item{nullptr, nullptr, bytes, false, false, false, 0}); return (int)items.size() - 1; }
+
+	void upload()
+	{
+		size_t tot = 0;
+		for (auto& it : items) {
+			if (!it.dev) { it.off = tot; tot += (it.bytes + 255) & ~(size_t)255; }
+		}
+		c.vecs.reserve(std::max<size_t>(tot, 256));
+		for (auto& it : items) {
+			if (!it.dev && it.in) {
+				RQC_HIP(hipMemcpyAsync(static_cast<char*>(c.vecs.ptr) + it.off, it.host, it.bytes, hipMemcpyHostToDevice, c.stream));
+			}
+		}
+	}
+	template <typename T> T* dev(int k)
+	{
+		auto& it = items[k];
+		return it.dev ? static_cast<T*>(it.user) : reinterpret_cast<T*>(static_cast<char*>(c.vecs.ptr) + it.off);
+	}
+	void download()
+	{
+		for (auto& it : items) {
+			if (!it.dev && it.out) {
+				RQC_HIP(hipMemcpyAsync(it.user, static_cast<char*>(c.vecs.ptr) + it.off, it.bytes, hipMemcpyDeviceToHost, c.stream));
+			}
+		}
+		RQC_HIP(hipStreamSynchronize(c.stream));
+		RQC_HIP(hipGetLastError());
+	}
+};
+
+// copy between two device buffers on the library stream
+static void dcopy(Context& c, void* dst, const void* src, size_t bytes)
+{
+	RQC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c.stream));
+}
+
+// copy a device buffer to a user buffer (host or device)
+static void copy_out(Context& c, void* user, const void* src, size_t bytes)
+{
+	RQC_HIP(hipMemcpyAsync(user, src, bytes, is_device_pointer(user) ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, c.stream));
+}
+
+static void copy_in(Context& c, void* dst, const void* user, size_t bytes)
+{
+	RQC_HIP(hipMemcpyAsync(dst, user, bytes, is_device_pointer(user) ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c.stream));
+}
+
+// one FP64 gate on a block of B columns
+static void launch_forward(Context& c, const zc* G, int i, int j, int L, long B, const zc* in, zc* out)
+{
+	const long nq = (1L << L) / 4 * B;
+	LaunchTimer lt(c, RQC_KERNEL_FORWARD);
+	k_gate_forward<double, false><<<nblocks(nq), kBlock, 0, c.stream>>>(in, out, nullptr, nullptr, to_gate<double>(G),
+		to_gate<double>(G), wires_of(i, j, L), B, ilog2_or_neg(B), nq, 0);
+}
+
+static void launch_forward_dual(Context& c, const zc* G, const zc* Z, int i, int j, int L, const zc* in, zc* out,
+	const zc* din, zc* dout)
+{
+	const long nq = (1L << L) / 4;
+	LaunchTimer lt(c, RQC_KERNEL_FORWARD);
+	k_gate_forward<double, true><<<nblocks(nq), kBlock, 0, c.stream>>>(in, out, din, dout, to_gate<double>(G),
+		to_gate<double>(Z), wires_of(i, j, L), 1, 0, nq, din == nullptr ? 1 : 0);
+}
+
+// reverse step of one FP64 gate on a single statevector; gradient (and HVP) summed into acc[0..32)
+static void launch_reverse(Context& c, const zc* G, const zc* Z, int i, int j, int L, const zc* psi, const zc* dpsi,
+	const zc* phi, const zc* dphi, zc* phi_out, zc* dphi_out, bool dual, zc* acc, zc* part)
+{
+	const long nq = (1L << L) / 4;
+	const unsigned g = reduce_grid(nq);
+	int flags = 0;
+	if (dpsi == nullptr) { flags |= 1; }
+	if (dphi == nullptr) { flags |= 2; }
+	if (phi_out == nullptr) { flags |= 4; }
+	{
+		LaunchTimer lt(c, RQC_KERNEL_REVERSE);
+		if (dual) {
+			k_gate_reverse<double, true><<<g, kBlock, 0, c.stream>>>(psi, dpsi, phi, dphi, phi_out, dphi_out,
+				to_gate<double>(G), to_gate<double>(Z), wires_of(i, j, L), 1, 0, nq, flags, part);
+		} else {
+			k_gate_reverse<double, false><<<g, kBlock, 0, c.stream>>>(psi, nullptr, phi, nullptr, phi_out, nullptr,
+				to_gate<double>(G), to_gate<double>(G), wires_of(i, j, L), 1, 0, nq, flags | 2, part);
+		}
+	}
+	LaunchTimer lt(c, RQC_KERNEL_OTHER);
+	k_reduce_partials<<<kWidth, kBlock, 0, c.stream>>>(part, (int)g, kWidth, acc);
+}
+
+static zc* small_acc(Context& c, size_t n_acc)
+{
+	c.small.reserve((n_acc + 4096 * (size_t)kWidth) * sizeof(zc));
+	zc* acc = c.small.as<zc>();
+	LaunchTimer lt(c, RQC_KERNEL_OTHER);
+	k_fill_zero<<<nblocks((long)n_acc), kBlock, 0, c.stream>>>(acc, (long)n_acc);
+	return acc;
+}
+
+}  // namespace rqc
+
+using namespace rqc;
+
+// ============================================================== C ABI: statevector.h
+extern "C" int allocate_statevector(int nqubits, struct statevector* psi)
+{
+	psi->nqubits = nqubits;
+	const size_t size = ((size_t)1 << nqubits) * sizeof(numeric);
+	psi->data = static_cast<numeric*>(aligned_malloc(size));
+	if (psi->data == nullptr) {
+		fprintf(stderr, "allocating statevector memory (%zu bytes) failed\n", size);
+		return -1;
+	}
+	return 0;
+}
+
+extern "C" void free_statevector(struct statevector* psi)
+{
+	if (psi->data != nullptr) {
+		aligned_free(psi->data);
+		psi->data = nullptr;
+	}
+	psi->nqubits = 0;
+}
+
+extern "C" int allocate_statevector_array(int nqubits, int nstates, struct statevector_array* psi_array)
+{
+	psi_array->nqubits = nqubits;
+	psi_array->nstates = nstates;
+	const size_t size = ((size_t)1 << nqubits) * nstates * sizeof(numeric);
+	psi_array->data = static_cast<numeric*>(aligned_malloc(size));
+	if (psi_array->data == nullptr) {
+		fprintf(stderr, "allocating statevector array memory (%zu bytes) failed\n", size);
+		return -1;
+	}
+	return 0;
+}
+
+extern "C" void free_statevector_array(struct statevector_array* psi_array)
+{
+	if (psi_array->data != nullptr) {
+		aligned_free(psi_array->data);
+		psi_array->data = nullptr;
+	}
+	psi_array->nqubits = 0;
+	psi_array->nstates = 0;
+}
+
+extern "C" void add_statevectors(const struct statevector* RQC_RESTRICT psi1, const struct statevector* RQC_RESTRICT psi2,
+	struct statevector* RQC_RESTRICT psi_sum)
+{
+	try {
+		if (psi1->nqubits != psi2->nqubits || psi1->nqubits != psi_sum->nqubits) { throw Error(-3, "qubit counts differ"); }
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const long n = 1L << psi1->nqubits;
+		Staging s(c);
+		const int a = s.add(psi1->data, n * sizeof(zc), true, false);
+		const int b = s.add(psi2->data, n * sizeof(zc), true, false);
+		const int o = s.add(psi_sum->data, n * sizeof(zc), false, true);
+		s.upload();
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_add<<<nblocks(n), kBlock, 0, c.stream>>>(s.dev<zc>(a), s.dev<zc>(b), s.dev<zc>(o), n);
+		}
+		s.download();
+	} catch (const std::exception& e) { fatal("add_statevectors", e); }
+}
+
+// ============================================================== C ABI: gate.h
+extern "C" void apply_gate(const struct mat4x4* gate, const int i, const int j, const struct statevector* RQC_RESTRICT psi,
+	struct statevector* RQC_RESTRICT psi_out)
+{
+	try {
+		if (psi->nqubits != psi_out->nqubits) { throw Error(-3, "qubit counts differ"); }
+		const int L = psi->nqubits;
+		check_wires(i, j, L);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t bytes = ((size_t)1 << L) * sizeof(zc);
+		Staging s(c);
+		const int a = s.add(psi->data, bytes, true, false);
+		const int o = s.add(psi_out->data, bytes, false, true);
+		s.upload();
+		launch_forward(c, reinterpret_cast<const zc*>(gate->data), i, j, L, 1, s.dev<zc>(a), s.dev<zc>(o));
+		s.download();
+	} catch (const std::exception& e) { fatal("apply_gate", e); }
+}
+
+extern "C" void apply_gate_backward(const struct mat4x4* gate, const int i, const int j, const struct statevector* RQC_RESTRICT psi,
+	const struct statevector* RQC_RESTRICT dpsi_out, struct statevector* RQC_RESTRICT dpsi, struct mat4x4* dgate)
+{
+	try {
+		if (psi->nqubits != dpsi_out->nqubits || psi->nqubits != dpsi->nqubits) { throw Error(-3, "qubit counts differ"); }
+		const int L = psi->nqubits;
+		check_wires(i, j, L);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t bytes = ((size_t)1 << L) * sizeof(zc);
+		zc* acc = small_acc(c, 32);
+		Staging s(c);
+		const int a = s.add(psi->data, bytes, true, false);
+		const int d = s.add(dpsi_out->data, bytes, true, false);
+		const int o = s.add(dpsi->data, bytes, false, true);
+		s.upload();
+		const zc* G = reinterpret_cast<const zc*>(gate->data);
+		launch_reverse(c, G, G, i, j, L, s.dev<zc>(a), nullptr, s.dev<zc>(d), nullptr, s.dev<zc>(o), nullptr, false, acc, acc + 32);
+		copy_out(c, dgate->data, acc, 16 * sizeof(zc));
+		s.download();
+	} catch (const std::exception& e) { fatal("apply_gate_backward", e); }
+}
+
+extern "C" void apply_gate_to_array(const struct mat4x4* gate, const int i, const int j, const struct statevector_array* RQC_RESTRICT psi,
+	struct statevector_array* RQC_RESTRICT psi_out)
+{
+	try {
+		if (psi->nqubits != psi_out->nqubits || psi->nstates != psi_out->nstates) { throw Error(-3, "array shapes differ"); }
+		const int L = psi->nqubits;
+		check_wires(i, j, L);
+		if (psi->nstates < 1) { return; }
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t bytes = ((size_t)1 << L) * psi->nstates * sizeof(zc);
+		Staging s(c);
+		const int a = s.add(psi->data, bytes, true, false);
+		const int o = s.add(psi_out->data, bytes, false, true);
+		s.upload();
+		launch_forward(c, reinterpret_cast<const zc*>(gate->data), i, j, L, psi->nstates, s.dev<zc>(a), s.dev<zc>(o));
+		s.download();
+	} catch (const std::exception& e) { fatal("apply_gate_to_array", e); }
+}
+
+// ============================================================== C ABI: quantum_circuit.h
+static int circuit_error(const char* where, const std::exception& e)
+{
+	fprintf(stderr, "librqcopt_hip: %s failed: %s\n", where, e.what());
+	if (auto* r = dynamic_cast<const Error*>(&e)) { return r->code; }
+	return -1;
+}
+
+extern "C" int apply_quantum_circuit(const struct mat4x4 gates[], const int ngates, const int wires[],
+	const struct statevector* RQC_RESTRICT psi, struct statevector* RQC_RESTRICT psi_out)
+{
+	try {
+		if (psi->nqubits != psi_out->nqubits) { throw Error(-3, "qubit counts differ"); }
+		const int L = psi->nqubits;
+		Circuit C = make_circuit(gates, nullptr, ngates, wires, L);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t bytes = ((size_t)1 << L) * sizeof(zc);
+		Staging s(c);
+		const int a = s.add(psi->data, bytes, true, false);
+		const int t0 = s.scratch(bytes), t1 = s.scratch(bytes);
+		const int o = s.add(psi_out->data, bytes, false, true);
+		s.upload();
+		const zc* in = s.dev<zc>(a);
+		zc* buf[2] = {s.dev<zc>(t0), s.dev<zc>(t1)};
+		for (int k = 0; k < C.ngates; k++) {
+			zc* out = (k == C.ngates - 1) ? s.dev<zc>(o) : buf[k % 2];
+			launch_forward(c, &C.gates[16 * (size_t)k], C.wi[k], C.wj[k], L, 1, in, out);
+			in = out;
+		}
+		s.download();
+		return 0;
+	} catch (const std::exception& e) { return circuit_error("apply_quantum_circuit", e); }
+}
+
+extern "C" int allocate_quantum_circuit_cache(const int nqubits, const int ngates, struct quantum_circuit_cache* cache)
+{
+	if (ngates < 1) {
+		fprintf(stderr, "a quantum circuit cache needs at least one gate\n");
+		return -3;
+	}
+	cache->nqubits = nqubits;
+	cache->ngates = ngates;
+	cache->psi_list = static_cast<struct statevector*>(aligned_malloc(ngates * sizeof(struct statevector)));
+	if (cache->psi_list == nullptr) {
+		fprintf(stderr, "allocating memory for statevector array failed\n");
+		return -1;
+	}
+	for (int i = 0; i < ngates; i++) { cache->psi_list[i].data = nullptr; cache->psi_list[i].nqubits = 0; }
+	for (int i = 0; i < ngates; i++) {
+		const int ret = allocate_statevector(nqubits, &cache->psi_list[i]);
+		if (ret < 0) { return ret; }
+	}
+	return 0;
+}
+
+extern "C" void free_quantum_circuit_cache(struct quantum_circuit_cache* cache)
+{
+	if (cache->psi_list != nullptr) {
+		for (int i = 0; i < cache->ngates; i++) { free_statevector(&cache->psi_list[i]); }
+		aligned_free(cache->psi_list);
+	}
+	cache->psi_list = nullptr;
+	cache->nqubits = 0;
+	cache->ngates = 0;
+}
+
+extern "C" int quantum_circuit_forward(const struct mat4x4 gates[], const int ngates, const int wires[],
+	const struct statevector* RQC_RESTRICT psi, struct quantum_circuit_cache* cache, struct statevector* RQC_RESTRICT psi_out)
+{
+	try {
+		if (psi->nqubits != psi_out->nqubits || cache->nqubits != psi->nqubits || cache->ngates != ngates) {
+			throw Error(-3, "shapes of psi, psi_out and cache differ");
+		}
+		const int L = psi->nqubits;
+		Circuit C = make_circuit(gates, nullptr, ngates, wires, L);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t bytes = ((size_t)1 << L) * sizeof(zc);
+		Staging s(c);
+		const int a = s.add(psi->data, bytes, true, false);
+		const int st = s.scratch(bytes * (ngates + 1));
+		s.upload();
+		zc* S = s.dev<zc>(st);
+		const size_t n = (size_t)1 << L;
+		dcopy(c, S, s.dev<zc>(a), bytes);
+		for (int k = 0; k < ngates; k++) {
+			launch_forward(c, &C.gates[16 * (size_t)k], C.wi[k], C.wj[k], L, 1, S + k * n, S + (k + 1) * n);
+		}
+		for (int k = 0; k < ngates; k++) { copy_out(c, cache->psi_list[k].data, S + k * n, bytes); }
+		copy_out(c, psi_out->data, S + ngates * n, bytes);
+		s.download();
+		return 0;
+	} catch (const std::exception& e) { return circuit_error("quantum_circuit_forward", e); }
+}
+
+extern "C" int quantum_circuit_backward(const struct mat4x4 gates[], const int ngates, const int wires[], const struct quantum_circuit_cache* cache,
+	const struct statevector* RQC_RESTRICT dpsi_out, struct statevector* RQC_RESTRICT dpsi, struct mat4x4 dgates[])
+{
+	try {
+		if (dpsi_out->nqubits != dpsi->nqubits || cache->nqubits != dpsi->nqubits || cache->ngates != ngates) {
+			throw Error(-3, "shapes of dpsi_out, dpsi and cache differ");
+		}
+		const int L = dpsi->nqubits;
+		Circuit C = make_circuit(gates, nullptr, ngates, wires, L);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t bytes = ((size_t)1 << L) * sizeof(zc);
+		const size_t n = (size_t)1 << L;
+		zc* acc = small_acc(c, 32 * (size_t)ngates);
+		zc* part = acc + 32 * (size_t)ngates;
+		Staging s(c);
+		const int st = s.scratch(bytes * ngates);
+		const int d = s.add(dpsi_out->data, bytes, true, false);
+		const int t0 = s.scratch(bytes), t1 = s.scratch(bytes);
+		const int o = s.add(dpsi->data, bytes, false, true);
+		s.upload();
+		zc* S = s.dev<zc>(st);
+		for (int k = 0; k < ngates; k++) { copy_in(c, S + k * n, cache->psi_list[k].data, bytes); }
+		const zc* cur = s.dev<zc>(d);
+		zc* buf[2] = {s.dev<zc>(t0), s.dev<zc>(t1)};
+		for (int k = ngates - 1; k >= 0; k--) {
+			zc* nxt = (k == 0) ? s.dev<zc>(o) : buf[k % 2];
+			const zc* G = &C.gates[16 * (size_t)k];
+			launch_reverse(c, G, G, C.wi[k], C.wj[k], L, S + k * n, nullptr, cur, nullptr, nxt, nullptr, false, acc + 32 * (size_t)k, part);
+			cur = nxt;
+		}
+		std::vector<zc> hacc(32 * (size_t)ngates);
+		RQC_HIP(hipMemcpyAsync(hacc.data(), acc, hacc.size() * sizeof(zc), hipMemcpyDeviceToHost, c.stream));
+		s.download();
+		for (int k = 0; k < ngates; k++) { std::memcpy(dgates[k].data, &hacc[32 * (size_t)k], 16 * sizeof(zc)); }
+		return 0;
+	} catch (const std::exception& e) { return circuit_error("quantum_circuit_backward", e); }
+}
+
+extern "C" int quantum_circuit_gates_hessian_vector_product(const struct mat4x4 gates[], const struct mat4x4 gatedirs[], const int ngates, const int wires[],
+	const struct statevector* RQC_RESTRICT psi, const struct statevector* RQC_RESTRICT phi,
+	struct statevector* RQC_RESTRICT psi_out, struct mat4x4 dgates[], struct mat4x4 hess_gatedirs[])
+{
+	try {
+		if (psi->nqubits != phi->nqubits || psi->nqubits != psi_out->nqubits) { throw Error(-3, "qubit counts differ"); }
+		const int L = psi->nqubits;
+		Circuit C = make_circuit(gates, gatedirs, ngates, wires, L);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t bytes = ((size_t)1 << L) * sizeof(zc);
+		const size_t n = (size_t)1 << L;
+		zc* acc = small_acc(c, 32 * (size_t)ngates);
+		zc* part = acc + 32 * (size_t)ngates;
+		Staging s(c);
+		const int a = s.add(psi->data, bytes, true, false);
+		const int f = s.add(phi->data, bytes, true, false);
+		const int st = s.scratch(bytes * (2 * (size_t)ngates + 5));
+		const int o = s.add(psi_out->data, bytes, false, true);
+		s.upload();
+		zc* S = s.dev<zc>(st);
+		auto P = [&](int k) { return S + k * n; };               // psi_k, k = 0 .. G
+		auto D = [&](int k) { return S + (ngates + k) * n; };    // dpsi_k, k = 1 .. G
+		zc* F[2] = {S + (2 * ngates + 1) * n, S + (2 * ngates + 2) * n};
+		zc* DF[2] = {S + (2 * ngates + 3) * n, S + (2 * ngates + 4) * n};
+		dcopy(c, P(0), s.dev<zc>(a), bytes);
+		for (int k = 0; k < ngates; k++) {
+			launch_forward_dual(c, &C.gates[16 * (size_t)k], &C.dirs[16 * (size_t)k], C.wi[k], C.wj[k], L, P(k), P(k + 1),
+				k > 0 ? D(k) : nullptr, D(k + 1));
+		}
+		dcopy(c, s.dev<zc>(o), P(ngates), bytes);
+		dcopy(c, F[0], s.dev<zc>(f), bytes);
+		int cur = 0;
+		for (int k = ngates - 1; k >= 0; k--) {
+			launch_reverse(c, &C.gates[16 * (size_t)k], &C.dirs[16 * (size_t)k], C.wi[k], C.wj[k], L, P(k),
+				k > 0 ? D(k) : nullptr, F[cur], k < ngates - 1 ? DF[cur] : nullptr,
+				k > 0 ? F[1 - cur] : nullptr, k > 0 ? DF[1 - cur] : nullptr, true, acc + 32 * (size_t)k, part);
+			cur = 1 - cur;
+		}
+		std::vector<zc> hacc(32 * (size_t)ngates);
+		RQC_HIP(hipMemcpyAsync(hacc.data(), acc, hacc.size() * sizeof(zc), hipMemcpyDeviceToHost, c.stream));
+		s.download();
+		for (int k = 0; k < ngates; k++) {
+			std::memcpy(dgates[k].data, &hacc[32 * (size_t)k], 16 * sizeof(zc));
+			std::memcpy(hess_gatedirs[k].data, &hacc[32 * (size_t)k + 16], 16 * sizeof(zc));
+		}
+		return 0;
+	} catch (const std::exception& e) { return circuit_error("quantum_circuit_gates_hessian_vector_product", e); }
+}
+
+// ============================================================== C ABI: brickwall_circuit.h
+extern "C" int apply_brickwall_unitary(const struct mat4x4 vlist[], const int nlayers, const int* perms[],
+	const struct statevector* RQC_RESTRICT psi, struct statevector* RQC_RESTRICT psi_out)
+{
+	try {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, psi->nqubits);
+		std::vector<int> w(2 * (size_t)C.ngates);
+		for (int k = 0; k < C.ngates; k++) { w[2*k] = C.wi[k]; w[2*k + 1] = C.wj[k]; }
+		return apply_quantum_circuit(reinterpret_cast<const struct mat4x4*>(C.gates.data()), C.ngates, w.data(), psi, psi_out);
+	} catch (const std::exception& e) { return circuit_error("apply_brickwall_unitary", e); }
+}
+
+extern "C" int brickwall_unitary_forward(const struct mat4x4 vlist[], int nlayers, const int* perms[],
+	const struct statevector* RQC_RESTRICT psi, struct quantum_circuit_cache* cache, struct statevector* RQC_RESTRICT psi_out)
+{
+	try {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, psi->nqubits);
+		std::vector<int> w(2 * (size_t)C.ngates);
+		for (int k = 0; k < C.ngates; k++) { w[2*k] = C.wi[k]; w[2*k + 1] = C.wj[k]; }
+		return quantum_circuit_forward(reinterpret_cast<const struct mat4x4*>(C.gates.data()), C.ngates, w.data(), psi, cache, psi_out);
+	} catch (const std::exception& e) { return circuit_error("brickwall_unitary_forward", e); }
+}
+
+extern "C" int brickwall_unitary_backward(const struct mat4x4 vlist[], int nlayers, const int* perms[], const struct quantum_circuit_cache* cache,
+	const struct statevector* RQC_RESTRICT dpsi_out, struct statevector* RQC_RESTRICT dpsi, struct mat4x4 dvlist[])
+{
+	try {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, dpsi->nqubits);
+		std::vector<int> w(2 * (size_t)C.ngates);
+		for (int k = 0; k < C.ngates; k++) { w[2*k] = C.wi[k]; w[2*k + 1] = C.wj[k]; }
+		std::vector<struct mat4x4> dg(C.ngates);
+		const int r = quantum_circuit_backward(reinterpret_cast<const struct mat4x4*>(C.gates.data()), C.ngates, w.data(), cache,
+			dpsi_out, dpsi, dg.data());
+		if (r < 0) { return r; }
+		// per layer, gradients of its pairs summed in reverse pair order (reference src/brickwall_circuit.c:100-120)
+		const int per = dpsi->nqubits / 2;
+		for (int i = 0; i < nlayers; i++) {
+			zc s[16];
+			for (int e = 0; e < 16; e++) { s[e] = {0, 0}; }
+			for (int k = per - 1; k >= 0; k--) {
+				const zc* g = reinterpret_cast<const zc*>(dg[(size_t)i * per + k].data);
+				for (int e = 0; e < 16; e++) { s[e] += g[e]; }
+			}
+			std::memcpy(dvlist[i].data, s, sizeof(s));
+		}
+		return 0;
+	} catch (const std::exception& e) { return circuit_error("brickwall_unitary_backward", e); }
+}
+
+// ============================================================== C ABI: target.h
+static int target_call(const char* where, const std::function<void()>& body)
+{
+	try {
+		body();
+		return 0;
+	} catch (const Error& e) {
+		fprintf(stderr, "librqcopt_hip: %s failed: %s\n", where, e.what());
+		return e.code;
+	} catch (const std::exception& e) {
+		fprintf(stderr, "librqcopt_hip: %s failed: %s\n", where, e.what());
+		return -1;
+	}
+}
+
+static void put_f(numeric* fval, const TargetResult& R) { std::memcpy(fval, &R.f, sizeof(zc)); }
+
+extern "C" int circuit_unitary_target(linear_func ufunc, void* udata,
+	const struct mat4x4 gates[], const int ngates, const int wires[], const int nqubits, numeric* fval)
+{
+	return target_call("circuit_unitary_target", [&] {
+		Circuit C = make_circuit(gates, nullptr, ngates, wires, nqubits);
+		put_f(fval, eval_targets(C, ufunc, udata, Mode::Target, 1, false));
+	});
+}
+
+extern "C" int circuit_unitary_target_and_gradient(linear_func ufunc, void* udata,
+	const struct mat4x4 gates[], const int ngates, const int wires[], const int nqubits, numeric* fval, struct mat4x4 dgates[])
+{
+	return target_call("circuit_unitary_target_and_gradient", [&] {
+		Circuit C = make_circuit(gates, nullptr, ngates, wires, nqubits);
+		TargetResult R = eval_targets(C, ufunc, udata, Mode::Grad, 1, false);
+		put_f(fval, R);
+		std::memcpy(dgates, R.dv.data(), R.dv.size() * sizeof(zc));
+	});
+}
+
+extern "C" int circuit_unitary_target_and_projected_gradient(linear_func ufunc, void* udata,
+	const struct mat4x4 gates[], const int ngates, const int wires[], const int nqubits, numeric* fval, double* grad_vec)
+{
+	return target_call("circuit_unitary_target_and_projected_gradient", [&] {
+		Circuit C = make_circuit(gates, nullptr, ngates, wires, nqubits);
+		TargetResult R = eval_targets(C, ufunc, udata, Mode::Grad, 1, true);
+		put_f(fval, R);
+		std::memcpy(grad_vec, R.grad.data(), R.grad.size() * sizeof(double));
+	});
+}
+
+extern "C" int circuit_unitary_target_hessian_vector_product(linear_func ufunc, void* udata,
+	const struct mat4x4 gates[], const struct mat4x4 gatedirs[], const int ngates, const int wires[], const int nqubits,
+	numeric* fval, struct mat4x4 dgates[], struct mat4x4 hess_gatedirs[])
+{
+	return target_call("circuit_unitary_target_hessian_vector_product", [&] {
+		Circuit C = make_circuit(gates, gatedirs, ngates, wires, nqubits);
+		TargetResult R = eval_targets(C, ufunc, udata, Mode::Hvp, 1, false);
+		put_f(fval, R);
+		std::memcpy(dgates, R.dv.data(), R.dv.size() * sizeof(zc));
+		std::memcpy(hess_gatedirs, R.hz.data(), R.hz.size() * sizeof(zc));
+	});
+}
+
+extern "C" int circuit_unitary_target_projected_hessian_vector_product(linear_func ufunc, void* udata,
+	const struct mat4x4 gates[], const struct mat4x4 gatedirs[], const int ngates, const int wires[], const int nqubits,
+	numeric* fval, double* RQC_RESTRICT grad_vec, double* RQC_RESTRICT hvp_vec)
+{
+	return target_call("circuit_unitary_target_projected_hessian_vector_product", [&] {
+		Circuit C = make_circuit(gates, gatedirs, ngates, wires, nqubits);
+		TargetResult R = eval_targets(C, ufunc, udata, Mode::Hvp, 1, true);
+		put_f(fval, R);
+		std::memcpy(grad_vec, R.grad.data(), R.grad.size() * sizeof(double));
+		std::memcpy(hvp_vec, R.hvp.data(), R.hvp.size() * sizeof(double));
+	});
+}
+
+extern "C" int brickwall_unitary_target(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[], numeric* fval)
+{
+	return target_call("brickwall_unitary_target", [&] {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
+		put_f(fval, eval_targets(C, ufunc, udata, Mode::Target, nqubits / 2, false));
+	});
+}
+
+extern "C" int brickwall_unitary_target_and_gradient(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[], numeric* fval, struct mat4x4 dvlist[])
+{
+	return target_call("brickwall_unitary_target_and_gradient", [&] {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
+		TargetResult R = eval_targets(C, ufunc, udata, Mode::Grad, nqubits / 2, false);
+		put_f(fval, R);
+		std::memcpy(dvlist, R.dv.data(), R.dv.size() * sizeof(zc));
+	});
+}
+
+extern "C" int brickwall_unitary_target_and_projected_gradient(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[], numeric* fval, double* grad_vec)
+{
+	return target_call("brickwall_unitary_target_and_projected_gradient", [&] {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
+		TargetResult R = eval_targets(C, ufunc, udata, Mode::Grad, nqubits / 2, true);
+		put_f(fval, R);
+		std::memcpy(grad_vec, R.grad.data(), R.grad.size() * sizeof(double));
+	});
+}
+
+extern "C" int brickwall_unitary_target_hessian_vector_product(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const struct mat4x4 vdirs[], const int nlayers, const int* perms[], const int nqubits,
+	numeric* fval, struct mat4x4 dvlist[], struct mat4x4 hess_vdirs[])
+{
+	return target_call("brickwall_unitary_target_hessian_vector_product", [&] {
+		Circuit C = make_brickwall(vlist, vdirs, nlayers, perms, nqubits);
+		TargetResult R = eval_targets(C, ufunc, udata, Mode::Hvp, nqubits / 2, false);
+		put_f(fval, R);
+		std::memcpy(dvlist, R.dv.data(), R.dv.size() * sizeof(zc));
+		std::memcpy(hess_vdirs, R.hz.data(), R.hz.size() * sizeof(zc));
+	});
+}
+
+extern "C" int brickwall_unitary_target_projected_hessian_vector_product(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const struct mat4x4 vdirs[], const int nlayers, const int* perms[], const int nqubits,
+	numeric* fval, double* RQC_RESTRICT grad_vec, double* RQC_RESTRICT hvp_vec)
+{
+	return target_call("brickwall_unitary_target_projected_hessian_vector_product", [&] {
+		Circuit C = make_brickwall(vlist, vdirs, nlayers, perms, nqubits);
+		TargetResult R = eval_targets(C, ufunc, udata, Mode::Hvp, nqubits / 2, true);
+		put_f(fval, R);
+		std::memcpy(grad_vec, R.grad.data(), R.grad.size() * sizeof(double));
+		std::memcpy(hvp_vec, R.hvp.data(), R.hvp.size() * sizeof(double));
+	});
+}
+
+// ============================================================== C ABI: rqc_device.h
+extern "C" int rqc_device_ufunc(const struct statevector* RQC_RESTRICT psi, void* fdata, struct statevector* RQC_RESTRICT psi_out)
+{
+	return target_call("rqc_device_ufunc", [&] {
+		if (psi->nqubits != psi_out->nqubits) { throw Error(-3, "qubit counts differ"); }
+		const rqc_target* t = as_target(fdata, psi->nqubits);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const long n = 1L << psi->nqubits;
+		Staging s(c);
+		const int a = s.add(psi->data, n * sizeof(zc), true, false);
+		const int o = s.add(psi_out->data, n * sizeof(zc), false, true);
+		s.upload();
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_matvec<<<nblocks(n), kBlock, 0, c.stream>>>(t->d_U, s.dev<zc>(a), s.dev<zc>(o), n);
+		}
+		s.download();
+	});
+}
+
+static rqc_target* new_target(int L)
+{
+	if (L < 1 || L > 15) { throw Error(-3, "dense targets support 1 .. 15 qubits"); }
+	Context& c = ctx();
+	(void)c;
+	auto* t = new rqc_target();
+	t->magic = kTargetMagic;
+	t->nqubits = L;
+	const size_t N = (size_t)1 << L;
+	hipError_t e = hipMalloc(&t->d_U, N * N * sizeof(zc));
+	if (e != hipSuccess) {
+		delete t;
+		throw Error(-1, std::string("allocating the device target failed: ") + hipGetErrorString(e));
+	}
+	return t;
+}
+
+extern "C" int rqc_target_create_dense(int nqubits, const numeric* U, struct rqc_target** target)
+{
+	return target_call("rqc_target_create_dense", [&] {
+		rqc_target* t = new_target(nqubits);
+		Context& c = ctx();
+		const size_t N = (size_t)1 << nqubits;
+		copy_in(c, t->d_U, U, N * N * sizeof(zc));
+		RQC_HIP(hipStreamSynchronize(c.stream));
+		*target = t;
+	});
+}
+
+extern "C" int rqc_target_create_from_ufunc(linear_func ufunc, void* udata, int nqubits, struct rqc_target** target)
+{
+	return target_call("rqc_target_create_from_ufunc", [&] {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const zc* dU = resolve_target(c, ufunc, udata, nqubits);
+		rqc_target* t = new_target(nqubits);
+		const size_t N = (size_t)1 << nqubits;
+		dcopy(c, t->d_U, dU, N * N * sizeof(zc));
+		RQC_HIP(hipStreamSynchronize(c.stream));
+		*target = t;
+	});
+}
+
+extern "C" int rqc_target_create_brickwall(int nqubits, const struct mat4x4 vlist[], int nlayers, const int* perms[], struct rqc_target** target)
+{
+	return target_call("rqc_target_create_brickwall", [&] {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		rqc_target* t = new_target(nqubits);
+		const long N = 1L << nqubits;
+		const long B = choose_block(c, N, 2, sizeof(zc));
+		c.states.reserve(2 * (size_t)N * B * sizeof(zc));
+		zc* S = c.states.as<zc>();
+		for (long b0 = 0; b0 < N; b0 += B) {
+			zc* buf[2] = {S, S + (size_t)N * B};
+			k_identity_block<double><<<nblocks(N * B), kBlock, 0, c.stream>>>(buf[0], B, ilog2_or_neg(B), b0, N * B);
+			for (int k = 0; k < C.ngates; k++) {
+				launch_forward(c, &C.gates[16 * (size_t)k], C.wi[k], C.wj[k], nqubits, B, buf[k % 2], buf[(k + 1) % 2]);
+			}
+			k_store_columns<<<nblocks(N * B), kBlock, 0, c.stream>>>(buf[C.ngates % 2], t->d_U, N, B, b0);
+		}
+		RQC_HIP(hipStreamSynchronize(c.stream));
+		RQC_HIP(hipGetLastError());
+		*target = t;
+	});
+}
+
+extern "C" void rqc_target_destroy(struct rqc_target* target)
+{
+	if (target == nullptr || target->magic != kTargetMagic) { return; }
+	(void)hipFree(target->d_U);
+	target->magic = 0;
+	delete target;
+}
+
+extern "C" int rqc_target_nqubits(const struct rqc_target* target) { return target != nullptr ? target->nqubits : -1; }
+
+extern "C" int rqc_target_download(const struct rqc_target* target, numeric* U)
+{
+	return target_call("rqc_target_download", [&] {
+		as_target(const_cast<rqc_target*>(target), -1);
+		Context& c = ctx();
+		const size_t N = (size_t)1 << target->nqubits;
+		copy_out(c, U, target->d_U, N * N * sizeof(zc));
+		RQC_HIP(hipStreamSynchronize(c.stream));
+	});
+}
+
+extern "C" int rqc_set_precision(int mode)
+{
+	if (mode != RQC_PRECISION_FP64 && mode != RQC_PRECISION_MIXED) { return -3; }
+	return target_call("rqc_set_precision", [&] { ctx().precision = mode; });
+}
+
+extern "C" int rqc_get_precision(void)
+{
+	try { return ctx().precision; } catch (...) { return -1; }
+}
+
+extern "C" int rqc_set_workspace_limit(size_t bytes)
+{
+	return target_call("rqc_set_workspace_limit", [&] { ctx().ws_limit = bytes; });
+}
+
+extern "C" void rqc_release_workspace(void)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		(void)hipStreamSynchronize(c.stream);
+		c.states.release();
+		c.vecs.release();
+		c.ubuf.release();
+	} catch (...) {}
+}
+
+extern "C" long rqc_last_column_block(void)
+{
+	try { return ctx().last_block; } catch (...) { return -1; }
+}
+
+extern "C" void rqc_profile_enable(int on)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		profile_flush(c);
+		c.profile = (on != 0);
+	} catch (const std::exception& e) { fatal("rqc_profile_enable", e); }
+}
+
+extern "C" void rqc_profile_reset(void)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		profile_flush(c);
+		for (int k = 0; k < RQC_KERNEL_KINDS; k++) { c.prof_ms[k] = 0; c.prof_n[k] = 0; }
+	} catch (const std::exception& e) { fatal("rqc_profile_reset", e); }
+}
+
+extern "C" int rqc_profile_query(int kind, double* total_ms, long* launches)
+{
+	if (kind < 0 || kind >= RQC_KERNEL_KINDS) { return -3; }
+	return target_call("rqc_profile_query", [&] {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		profile_flush(c);
+		*total_ms = c.prof_ms[kind];
+		*launches = c.prof_n[kind];
+	});
+}
+
+extern "C" int rqc_synchronize(void)
+{
+	return target_call("rqc_synchronize", [&] { RQC_HIP(hipStreamSynchronize(ctx().stream)); });
+}
+
+extern "C" int rqc_device_info(char* buf, int len)
+{
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+		(void)hipGetLastError();
+		if (len > 0) { snprintf(buf, len, "no HIP device"); }
+		return -1;
+	}
+	return target_call("rqc_device_info", [&] {
+		Context& c = ctx();
+		snprintf(buf, len, "%s %d %zu", c.name.c_str(), c.cu_count, c.hbm_bytes);
+	});
+}

@@ -1,0 +1,343 @@
+// Host entry points of librqcopt_hip: 4x4 algebra (matrix.h), seeded random
+// numbers (rng.h), the RTR-HVP driver (trust_region.h) and its brickwall glue
+// (brickwall_opt.h).  These are O(nlayers) host computations that the
+// reference also runs on the host; every target / gradient / HVP they request
+// runs on the GPU through target.h.  The 4x4 algebra is the same
+// __host__ __device__ code the GPU finalize kernel uses for the projection.
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "mat4.hpp"
+#include "rqcopt/matrix.h"
+#include "rqcopt/rng.h"
+#include "rqcopt/trust_region.h"
+#include "rqcopt/brickwall_opt.h"
+
+
+
+static inline zc* Z(struct mat4x4* a) { return reinterpret_cast<zc*>(a->data); }
+static inline const zc* Z(const struct mat4x4* a) { return reinterpret_cast<const zc*>(a->data); }
+
+// ============================================================== matrix.h
+extern "C" void zero_matrix(struct mat4x4* a) { rqc::m4_zero(Z(a)); }
+extern "C" void identity_matrix(struct mat4x4* a) { rqc::m4_identity(Z(a)); }
+
+extern "C" void scale_matrix(struct mat4x4* RQC_RESTRICT a, const double x)
+{
+	for (int e = 0; e < 16; e++) { Z(a)[e] = x * Z(a)[e]; }
+}
+
+extern "C" void conjugate_matrix(struct mat4x4* a)
+{
+	for (int e = 0; e < 16; e++) { Z(a)[e] = conj(Z(a)[e]); }
+}
+
+extern "C" void add_matrix(struct mat4x4* RQC_RESTRICT a, const struct mat4x4* RQC_RESTRICT b)
+{
+	for (int e = 0; e < 16; e++) { Z(a)[e] += Z(b)[e]; }
+}
+
+extern "C" void sub_matrix(struct mat4x4* RQC_RESTRICT a, const struct mat4x4* RQC_RESTRICT b)
+{
+	for (int e = 0; e < 16; e++) { Z(a)[e] -= Z(b)[e]; }
+}
+
+extern "C" void add_matrices(const struct mat4x4* RQC_RESTRICT a, const struct mat4x4* RQC_RESTRICT b, struct mat4x4* RQC_RESTRICT c)
+{
+	for (int e = 0; e < 16; e++) { Z(c)[e] = Z(a)[e] + Z(b)[e]; }
+}
+
+extern "C" void sub_matrices(const struct mat4x4* RQC_RESTRICT a, const struct mat4x4* RQC_RESTRICT b, struct mat4x4* RQC_RESTRICT c)
+{
+	for (int e = 0; e < 16; e++) { Z(c)[e] = Z(a)[e] - Z(b)[e]; }
+}
+
+extern "C" void transpose(const struct mat4x4* RQC_RESTRICT a, struct mat4x4* RQC_RESTRICT at) { rqc::m4_transpose(Z(a), Z(at)); }
+extern "C" void adjoint(const struct mat4x4* RQC_RESTRICT a, struct mat4x4* RQC_RESTRICT ah) { rqc::m4_adjoint(Z(a), Z(ah)); }
+extern "C" void symm(const struct mat4x4* RQC_RESTRICT w, struct mat4x4* RQC_RESTRICT z) { rqc::m4_symm(Z(w), Z(z)); }
+extern "C" void antisymm(const struct mat4x4* RQC_RESTRICT w, struct mat4x4* RQC_RESTRICT z) { rqc::m4_antisymm(Z(w), Z(z)); }
+extern "C" void real_to_antisymm(const double* r, struct mat4x4* w) { rqc::m4_real_to_antisymm(r, Z(w)); }
+extern "C" void antisymm_to_real(const struct mat4x4* w, double* r) { rqc::m4_antisymm_to_real(Z(w), r); }
+
+extern "C" void real_to_tangent(const double* r, const struct mat4x4* RQC_RESTRICT v, struct mat4x4* RQC_RESTRICT z)
+{
+	rqc::m4_real_to_tangent(r, Z(v), Z(z));
+}
+
+extern "C" void tangent_to_real(const struct mat4x4* RQC_RESTRICT v, const struct mat4x4* RQC_RESTRICT z, double* r)
+{
+	rqc::m4_tangent_to_real(Z(v), Z(z), r);
+}
+
+extern "C" void project_tangent(const struct mat4x4* RQC_RESTRICT u, const struct mat4x4* RQC_RESTRICT z, struct mat4x4* RQC_RESTRICT p)
+{
+	rqc::m4_project_tangent(Z(u), Z(z), Z(p));
+}
+
+extern "C" void multiply_matrices(const struct mat4x4* RQC_RESTRICT a, const struct mat4x4* RQC_RESTRICT b, struct mat4x4* RQC_RESTRICT c)
+{
+	rqc::m4_mul(Z(a), Z(b), Z(c));
+}
+
+extern "C" void symmetric_triple_matrix_product(const struct mat4x4* RQC_RESTRICT a, const struct mat4x4* RQC_RESTRICT b,
+	const struct mat4x4* RQC_RESTRICT c, struct mat4x4* RQC_RESTRICT ret)
+{
+	rqc::m4_symmetric_triple(Z(a), Z(b), Z(c), Z(ret));
+}
+
+extern "C" int inverse_matrix(const struct mat4x4* RQC_RESTRICT a, struct mat4x4* RQC_RESTRICT ainv)
+{
+	return rqc::m4_inverse(Z(a), Z(ainv));
+}
+
+extern "C" void polar_factor(const struct mat4x4* RQC_RESTRICT a, struct mat4x4* RQC_RESTRICT u) { rqc::m4_polar(Z(a), Z(u)); }
+
+extern "C" void retract(const struct mat4x4* RQC_RESTRICT u, const double* RQC_RESTRICT eta, struct mat4x4* RQC_RESTRICT v)
+{
+	rqc::m4_retract(Z(u), eta, Z(v));
+}
+
+// ============================================================== rng.h
+// PCG32 with XSH-RR output (O'Neill 2014); two streams give 64 bits
+extern "C" uint32_t pcg32_random_r(struct pcg32_random* rng)
+{
+	const uint64_t old = rng->state;
+	rng->state = old * 6364136223846793005ULL + rng->inc;
+	const uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+	const uint32_t rot = (uint32_t)(old >> 59u);
+	return (xs >> rot) | (xs << ((0u - rot) & 31u));
+}
+
+extern "C" void pcg32_srandom_r(const uint64_t initstate, const uint64_t initseq, struct pcg32_random* rng)
+{
+	rng->state = 0;
+	rng->inc = (initseq << 1u) | 1u;
+	pcg32_random_r(rng);
+	rng->state += initstate;
+	pcg32_random_r(rng);
+}
+
+extern "C" void pcg32x2_srandom_r(uint64_t seed1, uint64_t seed2, uint64_t seq1, uint64_t seq2, struct pcg32x2_random* rng)
+{
+	const uint64_t mask = ~0ULL >> 1;
+	if ((seq1 & mask) == (seq2 & mask)) { seq2 = ~seq2; }  // the two streams must differ
+	pcg32_srandom_r(seed1, seq1, &rng->gen[0]);
+	pcg32_srandom_r(seed2, seq2, &rng->gen[1]);
+}
+
+extern "C" uint64_t pcg32x2_random_r(struct pcg32x2_random* rng)
+{
+	const uint64_t hi = pcg32_random_r(&rng->gen[0]);
+	const uint64_t lo = pcg32_random_r(&rng->gen[1]);
+	return (hi << 32) | lo;
+}
+
+// seeding through a temporary generator (reference src/rng.c:13-25; the high
+// word of each 64-bit seed is drawn first)
+extern "C" void seed_rng_state(const uint64_t seed, struct rng_state* state)
+{
+	struct pcg32_random t;
+	pcg32_srandom_r(seed, 6364136223846793005ULL * seed + 1442695040888963407ULL, &t);
+	uint64_t v[4];
+	for (int k = 0; k < 4; k++) {
+		const uint64_t hi = pcg32_random_r(&t);
+		const uint64_t lo = pcg32_random_r(&t);
+		v[k] = (hi << 32) | lo;
+	}
+	pcg32x2_srandom_r(v[0], v[1], v[2], v[3], &state->pcgstate);
+}
+
+extern "C" uint32_t rand_uint32(struct rng_state* state) { return pcg32_random_r(&state->pcgstate.gen[0]); }
+extern "C" uint64_t rand_uint64(struct rng_state* state) { return pcg32x2_random_r(&state->pcgstate); }
+extern "C" double randu(struct rng_state* state) { return std::ldexp((double)rand_uint64(state), -64); }
+
+// Box-Muller (1958), using log(u1) with u1 = 0 mapped to 1
+extern "C" double randn(struct rng_state* state)
+{
+	double u1 = randu(state);
+	const double u2 = randu(state);
+	if (u1 == 0) { u1 = 1; }
+	return std::sqrt(-2 * std::log(u1)) * std::sin(6.2831853071795864769 * u2);
+}
+
+extern "C" numeric crandn(struct rng_state* state)
+{
+	const double x = randn(state);
+	const double y = randn(state);
+	const double s = std::sqrt(2.0);
+	return numeric(x / s, y / s);
+}
+
+// ============================================================== trust_region.h
+static double ddot(int n, const double* x, const double* y)
+{
+	double s = 0;
+	for (int i = 0; i < n; i++) { s += x[i] * y[i]; }
+	return s;
+}
+
+// roots of x^2 + 2 p x + q = 0, ascending (reference src/trust_region.c:14-40)
+static void solve_quadratic(double p, double q, double x[2])
+{
+	if (p == 0) {
+		const double a = std::sqrt(-q);
+		x[0] = -a; x[1] = a;
+		return;
+	}
+	const double a = -(p + std::copysign(1.0, p) * std::sqrt(p * p - q));
+	const double b = q / a;
+	if (a <= b) { x[0] = a; x[1] = b; } else { x[0] = b; x[1] = a; }
+}
+
+// t > 0 with ||b + t d|| = radius (reference src/trust_region.c:49-65)
+static double move_to_boundary(const double* b, const double* d, int n, double radius)
+{
+	const double dsq = ddot(n, d, d);
+	if (dsq == 0) {
+		fprintf(stderr, "input vector 'd' is zero\n");
+		return 0;
+	}
+	const double p = ddot(n, b, d) / dsq;
+	const double q = (ddot(n, b, b) - radius * radius) / dsq;
+	double t[2];
+	solve_quadratic(p, q, t);
+	if (t[1] < 0) { fprintf(stderr, "encountered t < 0: t = %g\n", t[1]); }
+	return t[1];
+}
+
+extern "C" bool truncated_cg_hvp(const double* RQC_RESTRICT x, const double* RQC_RESTRICT grad, const hessian_vector_product_func f_hvp, void* fdata,
+	const int n, const double radius, const struct truncated_cg_params* params, double* RQC_RESTRICT z)
+{
+	std::vector<double> r(grad, grad + n), d(n), hd(n);
+	double rsq = ddot(n, r.data(), r.data());
+	const double stoptol = std::fmax(params->abstol, params->reltol * std::sqrt(rsq));
+	for (int i = 0; i < n; i++) { z[i] = 0; d[i] = -r[i]; }
+	for (int j = 0; j < params->maxiter; j++) {
+		f_hvp(x, fdata, d.data(), hd.data());
+		const double dhd = ddot(n, d.data(), hd.data());
+		const double t = move_to_boundary(z, d.data(), n, radius);
+		const double alpha = rsq / dhd;
+		if (dhd <= 0 || alpha > t) {
+			for (int i = 0; i < n; i++) { z[i] += t * d[i]; }
+			return true;
+		}
+		for (int i = 0; i < n; i++) { r[i] += alpha * hd[i]; }
+		for (int i = 0; i < n; i++) { z[i] += alpha * d[i]; }
+		const double rsq_next = ddot(n, r.data(), r.data());
+		if (std::sqrt(rsq_next) <= stoptol) { break; }
+		const double beta = rsq_next / rsq;
+		for (int i = 0; i < n; i++) { d[i] = beta * d[i] - r[i]; }
+		rsq = rsq_next;
+	}
+	return false;
+}
+
+extern "C" void riemannian_trust_region_optimize_hvp(const target_func f, const target_gradient_func f_deriv, const hessian_vector_product_func f_hvp, void* fdata,
+	retract_func retract_fn, void* rdata,
+	const int n, const double* x_init, const int m, const struct rtr_params* params, const int niter, double* f_iter, double* x_final)
+{
+	if (!(0 <= params->rho_trust && params->rho_trust < 0.25)) {
+		fprintf(stderr, "rho_trust must lie in [0, 0.25)\n");
+		abort();
+	}
+	std::vector<double> xa(x_init, x_init + m), xb(m), grad(n), eta(n), heta(n);
+	double* x = xa.data();
+	double* xn = xb.data();
+	double radius = params->radius_init;
+	if (params->g_func != nullptr) { params->g_iter[0] = params->g_func(x, params->g_data); }
+	for (int k = 0; k < niter; k++) {
+		const double fx = f_deriv(x, fdata, grad.data());
+		const bool on_boundary = truncated_cg_hvp(x, grad.data(), f_hvp, fdata, n, radius, &params->tcg_params, eta.data());
+		retract_fn(x, eta.data(), rdata, xn);
+		f_iter[k] = fx;
+		// model decrease, Eq. (7.7) of Absil et al.
+		f_hvp(x, fdata, eta.data(), heta.data());
+		const double rho = (f(xn, fdata) - fx) / (ddot(n, grad.data(), eta.data()) + 0.5 * ddot(n, eta.data(), heta.data()));
+		if (rho < 0.25) {
+			radius *= 0.25;
+		} else if (rho > 0.75 && on_boundary) {
+			radius = std::fmin(2 * radius, params->maxradius);
+		}
+		if (rho > params->rho_trust) { std::swap(x, xn); }
+		if (params->g_func != nullptr) { params->g_iter[k + 1] = params->g_func(x, params->g_data); }
+	}
+	std::memcpy(x_final, x, m * sizeof(double));
+	f_iter[niter] = f(x_final, fdata);
+}
+
+// ============================================================== brickwall_opt.h
+namespace {
+
+struct BrickwallData
+{
+	linear_func ufunc;
+	void* udata;
+	const int** perms;
+	int nlayers;
+	int nqubits;
+};
+
+// wrappers of reference src/brickwall_opt.c:22-79
+double bw_f(const double* x, void* fdata)
+{
+	auto* d = static_cast<BrickwallData*>(fdata);
+	numeric f;
+	if (brickwall_unitary_target(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x), d->nlayers, d->nqubits, d->perms, &f) < 0) {
+		fprintf(stderr, "target function evaluation failed internally\n");
+		return -1;
+	}
+	return f.real();
+}
+
+double bw_f_deriv(const double* RQC_RESTRICT x, void* fdata, double* RQC_RESTRICT grad)
+{
+	auto* d = static_cast<BrickwallData*>(fdata);
+	numeric f;
+	if (brickwall_unitary_target_and_projected_gradient(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x), d->nlayers,
+			d->nqubits, d->perms, &f, grad) < 0) {
+		fprintf(stderr, "target function and derivative evaluation failed internally\n");
+		return -1;
+	}
+	return f.real();
+}
+
+void bw_f_hvp(const double* RQC_RESTRICT x, void* fdata, double* RQC_RESTRICT vec, double* RQC_RESTRICT hvp)
+{
+	auto* d = static_cast<BrickwallData*>(fdata);
+	const auto* vlist = reinterpret_cast<const struct mat4x4*>(x);
+	std::vector<struct mat4x4> vdirs(d->nlayers);
+	for (int i = 0; i < d->nlayers; i++) { real_to_tangent(&vec[16 * i], &vlist[i], &vdirs[i]); }
+	std::vector<double> grad(16 * (size_t)d->nlayers);
+	numeric f;
+	if (brickwall_unitary_target_projected_hessian_vector_product(d->ufunc, d->udata, vlist, vdirs.data(), d->nlayers, d->perms,
+			d->nqubits, &f, grad.data(), hvp) < 0) {
+		fprintf(stderr, "Hessian-vector product evaluation failed internally\n");
+	}
+}
+
+void bw_retract(const double* RQC_RESTRICT x, const double* RQC_RESTRICT eta, void* rdata, double* RQC_RESTRICT xs)
+{
+	const int nlayers = *static_cast<int*>(rdata);
+	const auto* v = reinterpret_cast<const struct mat4x4*>(x);
+	auto* w = reinterpret_cast<struct mat4x4*>(xs);
+	for (int i = 0; i < nlayers; i++) { retract(&v[i], &eta[16 * i], &w[i]); }
+}
+
+}  // namespace
+
+extern "C" void optimize_brickwall_circuit_hvp(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist_start[], const int nlayers, const int nqubits, const int* perms[],
+	struct rtr_params* params, const int niter, double* f_iter, struct mat4x4 vlist_opt[])
+{
+	BrickwallData fdata{ufunc, udata, perms, nlayers, nqubits};
+	params->g_func = nullptr;
+	params->g_data = nullptr;
+	params->g_iter = nullptr;
+	int rdata = nlayers;
+	riemannian_trust_region_optimize_hvp(bw_f, bw_f_deriv, bw_f_hvp, &fdata, bw_retract, &rdata,
+		nlayers * num_tangent_params, reinterpret_cast<const double*>(vlist_start), nlayers * 32, params, niter, f_iter,
+		reinterpret_cast<double*>(vlist_opt));
+}

@@ -1,0 +1,94 @@
+// Internal runtime of librqcopt_hip: device context, stream, workspace,
+// kernel timing and error reporting.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "zc.hpp"
+#include "rqcopt/rqc_device.h"
+
+namespace rqc {
+
+// error thrown inside the library and turned into a return code (or, for the
+// void entry points of the reference ABI, a message and abort) at the boundary
+struct Error : std::runtime_error
+{
+	int code;
+	Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define RQC_HIP(call)                                                                      \
+	do {                                                                                   \
+		hipError_t e_ = (call);                                                            \
+		if (e_ != hipSuccess) {                                                            \
+			throw ::rqc::Error(-1, std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_)); \
+		}                                                                                  \
+	} while (0)
+
+// grow-only device allocation
+struct DeviceBuffer
+{
+	void* ptr = nullptr;
+	size_t bytes = 0;
+	void reserve(size_t n);
+	void release();
+	template <typename T> T* as() const { return static_cast<T*>(ptr); }
+};
+
+struct Context
+{
+	int device = -1;
+	hipStream_t stream = nullptr;
+	int cu_count = 0;
+	size_t hbm_bytes = 0;
+	std::string name;
+	int precision = RQC_PRECISION_FP64;
+	size_t ws_limit = 0;  // 0 = automatic
+	long last_block = 0;
+
+	DeviceBuffer states;   // column-block statevectors (checkpoints, adjoints)
+	DeviceBuffer small;    // partial sums, accumulators, gates, outputs
+	DeviceBuffer vecs;     // uploaded statevectors of the statevector-level API
+	DeviceBuffer ubuf;     // U assembled from a host callback
+
+	// kernel timing
+	bool profile = false;
+	struct Ev { hipEvent_t a, b; int kind; };
+	std::vector<Ev> events;
+	size_t events_used = 0;
+	double prof_ms[RQC_KERNEL_KINDS] = {0, 0, 0, 0};
+	long prof_n[RQC_KERNEL_KINDS] = {0, 0, 0, 0};
+
+	std::recursive_mutex mu;
+};
+
+// the process-wide context on the current HIP device; throws if no GPU is present
+Context& ctx();
+
+// bracket a kernel launch with timing events when profiling is enabled
+struct LaunchTimer
+{
+	Context& c;
+	Context::Ev* ev = nullptr;
+	LaunchTimer(Context& cx, int kind);
+	~LaunchTimer();
+};
+
+// flush pending timing events into the per-kind totals
+void profile_flush(Context& c);
+
+// true if p is a HIP device allocation (or managed memory usable by kernels)
+bool is_device_pointer(const void* p);
+
+// report an error from a void entry point of the reference ABI and stop
+[[noreturn]] void fatal(const char* where, const std::exception& e);
+
+}  // namespace rqc

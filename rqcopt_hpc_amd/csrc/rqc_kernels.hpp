@@ -1,0 +1,318 @@
+// HIP kernels of the gate-sweep engine (gfx950).
+//
+// Data layout of a column block: X[a * B + c] — amplitude (row) index a over
+// the 2^L circuit qubits, column c over B consecutive basis columns, column
+// fastest.  A gate on wires (i, j) touches rows a with stride 2^(L-1-i) and
+// 2^(L-1-j), so every load and store of a wavefront covers 64 consecutive
+// columns: fully coalesced for any wire pair, including the wrap pair
+// (L-1, 0) of a periodic brickwall.  The same layout is the reference's
+// interleaved statevector_array (data[amp * nstates + k]).
+//
+// All reductions (trace, gate gradients, Hessian-vector products) accumulate
+// in FP64, first per thread, then by wavefront shuffles and an LDS tree in a
+// fixed order; per-workgroup partial sums are combined by a second kernel in
+// workgroup order.  No atomics: results are bitwise reproducible run to run.
+#pragma once
+
+#include "zc.hpp"
+
+namespace rqc {
+
+constexpr int kBlock = 256;    // threads per workgroup of the sweep kernels
+constexpr int kWidth = 32;     // partial-sum width of the reverse kernel (16 grad + 16 hess)
+
+template <typename T>
+struct Gate4
+{
+	cx<T> g[16];
+};
+
+// bit positions of the gate wires: wire i -> bit (L-1-i) carries the more
+// significant index of the 4x4 gate, wire j -> bit (L-1-j) the less significant
+struct Wires
+{
+	int bi, bj;
+};
+
+// index of quad q with zero bits inserted at positions lo < hi
+__device__ __forceinline__ long quad_base(long q, int lo, int hi)
+{
+	long x = ((q >> lo) << (lo + 1)) | (q & ((1L << lo) - 1));
+	x = ((x >> hi) << (hi + 1)) | (x & ((1L << hi) - 1));
+	return x;
+}
+
+// element offsets of the 4 amplitudes (|00>, |01>, |10>, |11> of (wire i, wire j))
+// of work item t = (quad, column)
+struct Quad
+{
+	long off[4];
+};
+
+__device__ __forceinline__ Quad quad_offsets(long t, long B, int logB, Wires w)
+{
+	long q, c;
+	if (logB >= 0) { q = t >> logB; c = t & (B - 1); }
+	else { q = t / B; c = t - q * B; }
+	const int lo = min(w.bi, w.bj), hi = max(w.bi, w.bj);
+	const long base = quad_base(q, lo, hi);
+	const long si = 1L << w.bi, sj = 1L << w.bj;
+	Quad r;
+	r.off[0] = base * B + c;
+	r.off[1] = (base + sj) * B + c;
+	r.off[2] = (base + si) * B + c;
+	r.off[3] = (base + si + sj) * B + c;
+	return r;
+}
+
+// y = G x for one quad
+template <typename T>
+__device__ __forceinline__ void gate_apply(const Gate4<T>& G, const cx<T>* x, cx<T>* y)
+{
+#pragma unroll
+	for (int r = 0; r < 4; r++) {
+		cx<T> s = {0, 0};
+#pragma unroll
+		for (int k = 0; k < 4; k++) { cmac(s, G.g[4*r + k], x[k]); }
+		y[r] = s;
+	}
+}
+
+// y += G^T x for one quad
+template <typename T>
+__device__ __forceinline__ void gate_apply_t_acc(const Gate4<T>& G, const cx<T>* x, cx<T>* y)
+{
+#pragma unroll
+	for (int c = 0; c < 4; c++) {
+#pragma unroll
+		for (int r = 0; r < 4; r++) { cmac(y[c], G.g[4*r + c], x[r]); }
+	}
+}
+
+// ---------------------------------------------------------------------------
+// forward sweep of one gate: out = G in; with DUAL also dout = G din + Z in
+// (din treated as zero when dzero is set)
+template <typename T, bool DUAL>
+__global__ __launch_bounds__(kBlock) void k_gate_forward(
+	const cx<T>* __restrict__ in, cx<T>* __restrict__ out,
+	const cx<T>* __restrict__ din, cx<T>* __restrict__ dout,
+	Gate4<T> G, Gate4<T> Z, Wires w, long B, int logB, long nitems, int dzero)
+{
+	const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+	if (t >= nitems) { return; }
+	const Quad o = quad_offsets(t, B, logB, w);
+	cx<T> x[4], y[4];
+#pragma unroll
+	for (int k = 0; k < 4; k++) { x[k] = in[o.off[k]]; }
+	gate_apply(G, x, y);
+#pragma unroll
+	for (int k = 0; k < 4; k++) { out[o.off[k]] = y[k]; }
+	if (DUAL) {
+		cx<T> dy[4];
+		if (dzero) {
+			gate_apply(Z, x, dy);
+		} else {
+			cx<T> dx[4];
+#pragma unroll
+			for (int k = 0; k < 4; k++) { dx[k] = din[o.off[k]]; }
+			gate_apply(G, dx, dy);
+#pragma unroll
+			for (int r = 0; r < 4; r++) {
+#pragma unroll
+				for (int k = 0; k < 4; k++) { cmac(dy[r], Z.g[4*r + k], x[k]); }
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < 4; k++) { dout[o.off[k]] = dy[k]; }
+	}
+}
+
+// ---------------------------------------------------------------------------
+// block-wide fixed-order sum of NV complex FP64 values per thread; thread 0..NV-1
+// of the block end up holding... the results are written to out[0..NV)
+template <int NV>
+__device__ __forceinline__ void block_sum_store(zc* v, zc* out)
+{
+	__shared__ double red[kBlock / 64][NV][2];
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+	for (int e = 0; e < NV; e++) {
+		double re = v[e].x, im = v[e].y;
+#pragma unroll
+		for (int s = 32; s >= 1; s >>= 1) {
+			re += __shfl_xor(re, s, 64);
+			im += __shfl_xor(im, s, 64);
+		}
+		if (lane == 0) { red[wave][e][0] = re; red[wave][e][1] = im; }
+	}
+	__syncthreads();
+	if (threadIdx.x < NV) {
+		double re = 0, im = 0;
+		for (int k = 0; k < kBlock / 64; k++) { re += red[k][threadIdx.x][0]; im += red[k][threadIdx.x][1]; }
+		out[threadIdx.x] = {re, im};
+	}
+}
+
+// ---------------------------------------------------------------------------
+// reverse sweep of one gate (adjoint state phi after the gate, psi before it):
+//   grad_rc += phi_r psi_c                                    (16 entries)
+//   hess_rc += phi_r dpsi_c + dphi_r psi_c           (DUAL)   (16 entries)
+//   phi_out  = G^T phi,  dphi_out = G^T dphi + Z^T phi (DUAL)
+// flags: 1 = dpsi is zero, 2 = dphi is zero, 4 = do not write phi_out/dphi_out
+template <typename T, bool DUAL>
+__global__ __launch_bounds__(kBlock) void k_gate_reverse(
+	const cx<T>* __restrict__ psi, const cx<T>* __restrict__ dpsi,
+	const cx<T>* __restrict__ phi, const cx<T>* __restrict__ dphi,
+	cx<T>* __restrict__ phi_out, cx<T>* __restrict__ dphi_out,
+	Gate4<T> G, Gate4<T> Z, Wires w, long B, int logB, long nitems, int flags, zc* __restrict__ partial)
+{
+	zc ga[16], ha[DUAL ? 16 : 1];
+#pragma unroll
+	for (int e = 0; e < 16; e++) { ga[e] = {0, 0}; }
+	if (DUAL) {
+#pragma unroll
+		for (int e = 0; e < 16; e++) { ha[e] = {0, 0}; }
+	}
+	const bool dpsi_zero = flags & 1, dphi_zero = flags & 2, no_write = flags & 4;
+	const long stride = (long)gridDim.x * kBlock;
+	for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < nitems; t += stride) {
+		const Quad o = quad_offsets(t, B, logB, w);
+		cx<T> x[4], f[4];
+#pragma unroll
+		for (int k = 0; k < 4; k++) { x[k] = psi[o.off[k]]; f[k] = phi[o.off[k]]; }
+		zc xz[4], fz[4];
+#pragma unroll
+		for (int k = 0; k < 4; k++) { xz[k] = to_z(x[k]); fz[k] = to_z(f[k]); }
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+#pragma unroll
+			for (int c = 0; c < 4; c++) { cmac(ga[4*r + c], fz[r], xz[c]); }
+		}
+		if (DUAL) {
+			cx<T> dx[4], df[4];
+#pragma unroll
+			for (int k = 0; k < 4; k++) {
+				dx[k] = dpsi_zero ? cx<T>{0, 0} : dpsi[o.off[k]];
+				df[k] = dphi_zero ? cx<T>{0, 0} : dphi[o.off[k]];
+			}
+			if (!dpsi_zero) {
+#pragma unroll
+				for (int r = 0; r < 4; r++) {
+#pragma unroll
+					for (int c = 0; c < 4; c++) { cmac(ha[4*r + c], fz[r], to_z(dx[c])); }
+				}
+			}
+			if (!dphi_zero) {
+#pragma unroll
+				for (int r = 0; r < 4; r++) {
+#pragma unroll
+					for (int c = 0; c < 4; c++) { cmac(ha[4*r + c], to_z(df[r]), xz[c]); }
+				}
+			}
+			if (!no_write) {
+				cx<T> dy[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+				if (!dphi_zero) { gate_apply_t_acc(G, df, dy); }
+				gate_apply_t_acc(Z, f, dy);
+#pragma unroll
+				for (int k = 0; k < 4; k++) { dphi_out[o.off[k]] = dy[k]; }
+			}
+		}
+		if (!no_write) {
+			cx<T> y[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+			gate_apply_t_acc(G, f, y);
+#pragma unroll
+			for (int k = 0; k < 4; k++) { phi_out[o.off[k]] = y[k]; }
+		}
+	}
+	zc* out = partial + (long)blockIdx.x * kWidth;
+	if (DUAL) {
+		zc all[32];
+#pragma unroll
+		for (int e = 0; e < 16; e++) { all[e] = ga[e]; all[16 + e] = ha[e]; }
+		block_sum_store<32>(all, out);
+	} else {
+		block_sum_store<16>(ga, out);
+		if (threadIdx.x < 16) { out[16 + threadIdx.x] = {0, 0}; }
+	}
+}
+
+// ---------------------------------------------------------------------------
+// trace of the column block: f += sum_{a,c} -conj(U[a, b0+c]) psi[a, c]; when
+// phi is given the adjoint seed -conj(U[:, b0+c]) is also stored there
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_target_trace(
+	const zc* __restrict__ U, long ldu, long b0, const cx<T>* __restrict__ psi, cx<T>* __restrict__ phi,
+	long B, int logB, long nitems, zc* __restrict__ partial)
+{
+	zc acc = {0, 0};
+	const long stride = (long)gridDim.x * kBlock;
+	for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < nitems; t += stride) {
+		long a, c;
+		if (logB >= 0) { a = t >> logB; c = t & (B - 1); }
+		else { a = t / B; c = t - a * B; }
+		const zc u = U[a * ldu + b0 + c];
+		const zc s = {-u.x, u.y};
+		cmac(acc, s, to_z(psi[t]));
+		if (phi != nullptr) { phi[t] = cx<T>{(T)s.x, (T)s.y}; }
+	}
+	block_sum_store<1>(&acc, partial + blockIdx.x);
+}
+
+// acc[e] += sum_p partial[p * width + e] in a fixed order; one workgroup per e
+__global__ __launch_bounds__(kBlock) void k_reduce_partials(const zc* __restrict__ partial, int nparts, int width,
+	zc* __restrict__ acc)
+{
+	const int e = blockIdx.x;
+	zc s = {0, 0};
+	for (int p = threadIdx.x; p < nparts; p += kBlock) { s += partial[(long)p * width + e]; }
+	zc r[1] = {s};
+	__shared__ zc res[1];
+	block_sum_store<1>(r, res);
+	__syncthreads();
+	if (threadIdx.x == 0) { acc[e] += res[0]; }
+}
+
+// X[a * B + c] = (a == b0 + c)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_identity_block(cx<T>* __restrict__ X, long B, int logB, long b0, long nitems)
+{
+	const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+	if (t >= nitems) { return; }
+	long a, c;
+	if (logB >= 0) { a = t >> logB; c = t & (B - 1); }
+	else { a = t / B; c = t - a * B; }
+	X[t] = cx<T>{(a == b0 + c) ? (T)1 : (T)0, (T)0};
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_zero(zc* __restrict__ p, long n)
+{
+	const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+	if (t < n) { p[t] = {0, 0}; }
+}
+
+__global__ __launch_bounds__(kBlock) void k_add(const zc* __restrict__ a, const zc* __restrict__ b, zc* __restrict__ c, long n)
+{
+	const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+	if (t < n) { c[t] = a[t] + b[t]; }
+}
+
+// y = U x for a dense row-major N x N matrix (one row per thread)
+__global__ __launch_bounds__(kBlock) void k_matvec(const zc* __restrict__ U, const zc* __restrict__ x, zc* __restrict__ y, long n)
+{
+	const long a = (long)blockIdx.x * kBlock + threadIdx.x;
+	if (a >= n) { return; }
+	zc s = {0, 0};
+	for (long b = 0; b < n; b++) { cmac(s, U[a * n + b], x[b]); }
+	y[a] = s;
+}
+
+// copy column block X (n x B, column fastest) into columns [b0, b0 + B) of the row-major n x n U
+__global__ __launch_bounds__(kBlock) void k_store_columns(const zc* __restrict__ X, zc* __restrict__ U, long n, long B, long b0)
+{
+	const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+	if (t >= n * B) { return; }
+	const long a = t / B, c = t - a * B;
+	U[a * n + b0 + c] = X[t];
+}
+
+}  // namespace rqc

@@ -1,4 +1,5 @@
 """Shared test helpers: fixture loading, seeded inputs, dense-matrix models."""
+import functools
 import os
 
 import numpy as np
@@ -93,6 +94,7 @@ def brickwall_matrix(vlist, perms):
     return circuit_matrix(g, w, perms.shape[1])
 
 
+@functools.lru_cache(maxsize=None)
 def scramble_matrix(L):
     """Dense matrix of the reference tests' index-scrambling map (test/test_target.c:16-34)."""
     n = 1 << L

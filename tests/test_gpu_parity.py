@@ -1,0 +1,363 @@
+"""GPU parity of librqcopt_hip against the oracle and the reference fixtures.
+
+Every call goes through the C ABI (ctypes on librqcopt_hip.so).  Tolerances:
+FP64 results match the oracle to 1e-12 relative unless a
+tighter reference-test tolerance applies; the mixed-precision path is checked
+against the FP64 path with the error bounds stated in each test.
+"""
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+import oracle as O
+import rqcopt_hpc_amd as R
+from helpers import (brickwall_matrix, brickwall_perms, cplx, crandn, fixture, haar_unitary, perms_of, random_perms,
+                     random_wires, reldiff, scramble_matrix, stack)
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    R.require_gpu()
+    R.set_precision(R.PRECISION_FP64)
+    yield
+    R.set_precision(R.PRECISION_FP64)
+    R.set_workspace_limit(0)
+
+
+SCR_H = R.HostUfunc(lambda x: scramble_matrix(int(np.log2(x.size))) @ x)
+SCR_O = O.Ufunc.scramble()
+
+
+def _max(a):
+    return float(np.max(np.abs(a)))
+
+
+# ------------------------------------------------------------------ fixtures
+def test_fixture_apply_gate_and_array():
+    d = fixture("test_apply_gate_cplx")
+    V, psi = cplx(d["V"]), cplx(d["psi"])
+    for k, (i, j) in zip((1, 2, 3), ((2, 5), (4, 1), (3, 4))):
+        assert _max(R.apply_gate(V, i, j, psi) - cplx(d[f"chi{k}"])) < 1e-12
+    d = fixture("test_apply_gate_to_array_cplx")
+    V, psi = cplx(d["V"]), cplx(d["psi"])
+    for k, (i, j) in zip((1, 2, 3), ((2, 5), (4, 1), (3, 4))):
+        assert _max(R.apply_gate_to_array(V, i, j, psi) - cplx(d[f"chi{k}"])) < 1e-12
+
+
+def test_fixture_circuits():
+    d = fixture("test_apply_quantum_circuit_cplx")
+    G = stack(d, "G", 4)
+    assert _max(R.apply_quantum_circuit(G, d["wires"], cplx(d["psi"])) - cplx(d["psi_out"])) < 1e-12
+    d = fixture("test_quantum_circuit_backward_cplx")
+    G = stack(d, "G", 5)
+    out, cache = R.quantum_circuit_forward(G, d["wires"], cplx(d["psi"]))
+    assert _max(out - cplx(d["psi_out"])) < 1e-12
+    _, cache_o = O.quantum_circuit_forward(G, d["wires"], cplx(d["psi"]))
+    assert _max(cache - cache_o) < 1e-13
+    dpsi, dg = R.quantum_circuit_backward(G, d["wires"], cache, cplx(d["dpsi_out"]))
+    dpsi_o, dg_o = O.quantum_circuit_backward(G, d["wires"], cache_o, cplx(d["dpsi_out"]))
+    assert reldiff(dpsi, dpsi_o) < TOL and reldiff(dg, dg_o) < TOL
+    d = fixture("test_quantum_circuit_gates_hessian_vector_product_cplx")
+    G, Z = stack(d, "G", 6), stack(d, "Z", 6)
+    out, dg, hg = R.quantum_circuit_gates_hessian_vector_product(G, Z, d["wires"], cplx(d["psi"]), cplx(d["phi"]))
+    out_o, dg_o, hg_o = O.quantum_circuit_gates_hvp(G, Z, d["wires"], cplx(d["psi"]), cplx(d["phi"]))
+    assert _max(out - cplx(d["psi_out"])) < 1e-12
+    assert reldiff(dg, dg_o) < TOL and reldiff(hg, hg_o) < TOL
+
+
+def test_fixture_brickwall_circuits():
+    d = fixture("test_apply_brickwall_unitary_cplx")
+    V, P = stack(d, "V", 4), perms_of(d, 4)
+    for nl in (3, 4):
+        assert _max(R.apply_brickwall_unitary(V[:nl], P[:nl], cplx(d["psi"])) - cplx(d[f"chi{nl - 3}"])) < 1e-12
+    d = fixture("test_brickwall_unitary_backward_cplx")
+    V, P = stack(d, "V", 4), perms_of(d, 4)
+    for nl in (3, 4):
+        out, cache = R.brickwall_unitary_forward(V[:nl], P[:nl], cplx(d["psi"]))
+        assert _max(out - cplx(d[f"psi_out{nl - 3}"])) < 1e-12
+        dpsi, dv = R.brickwall_unitary_backward(V[:nl], P[:nl], cache, cplx(d["dpsi_out"]))
+        _, cache_o = O.brickwall_unitary_forward(V[:nl], P[:nl], cplx(d["psi"]))
+        dpsi_o, dv_o = O.brickwall_unitary_backward(V[:nl], P[:nl], cache_o, cplx(d["dpsi_out"]))
+        assert reldiff(dpsi, dpsi_o) < TOL and reldiff(dv, dv_o) < TOL
+
+
+def test_fixture_circuit_targets():
+    d = fixture("test_circuit_unitary_target_cplx")
+    assert abs(R.circuit_unitary_target(SCR_H, stack(d, "G", 5), d["wires"], 7) - cplx(d["f"])) < 1e-12
+    d = fixture("test_circuit_unitary_target_and_gradient_cplx")
+    G = stack(d, "G", 5)
+    f, dg = R.circuit_unitary_target_and_gradient(SCR_H, G, d["wires"], 8)
+    f_o, dg_o = O.circuit_unitary_target_and_gradient(SCR_O, G, d["wires"], 8)
+    assert abs(f - cplx(d["f"])) < 1e-12 and reldiff(dg, dg_o) < TOL
+    f, g = R.circuit_unitary_target_and_projected_gradient(SCR_H, G, d["wires"], 8)
+    _, g_o = O.circuit_unitary_target_and_projected_gradient(SCR_O, G, d["wires"], 8)
+    assert reldiff(g, g_o) < TOL
+    for name in ("test_circuit_unitary_target_hessian_vector_product_cplx",
+                 "test_circuit_unitary_target_projected_hessian_vector_product_cplx"):
+        d = fixture(name)
+        G, Z = stack(d, "G", 6), stack(d, "Z", 6)
+        f, dg, hg = R.circuit_unitary_target_hessian_vector_product(SCR_H, G, Z, d["wires"], 7)
+        f_o, dg_o, hg_o = O.circuit_unitary_target_hvp(SCR_O, G, Z, d["wires"], 7)
+        assert abs(f - cplx(d["f"])) < 1e-12 and reldiff(dg, dg_o) < TOL and reldiff(hg, hg_o) < TOL
+        f, g, h = R.circuit_unitary_target_projected_hessian_vector_product(SCR_H, G, Z, d["wires"], 7)
+        _, g_o, h_o = O.circuit_unitary_target_projected_hvp(SCR_O, G, Z, d["wires"], 7)
+        assert reldiff(g, g_o) < TOL and reldiff(h, h_o) < TOL
+
+
+def test_fixture_brickwall_targets():
+    d = fixture("test_brickwall_unitary_target_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    for k, nl in enumerate((4, 5)):
+        assert abs(R.brickwall_unitary_target(SCR_H, V[:nl], P[:nl]) - cplx(d[f"f{k}"])) < 1e-12
+    d = fixture("test_brickwall_unitary_target_and_gradient_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    for k, nl in enumerate((4, 5)):
+        f, dv = R.brickwall_unitary_target_and_gradient(SCR_H, V[:nl], P[:nl])
+        assert abs(f - cplx(d[f"f{k}"])) < 1e-12 and _max(dv - cplx(d[f"dVlist{k}"])) < 1e-12
+    d = fixture("test_brickwall_unitary_target_and_projected_gradient_cplx")
+    f, g = R.brickwall_unitary_target_and_projected_gradient(SCR_H, stack(d, "V", 3), perms_of(d, 3))
+    assert abs(f - cplx(d["f"])) < 1e-12 and _max(g - d["grad"]) < 1e-14
+
+
+def test_fixture_hessians_pin_hvp():
+    d = fixture("test_brickwall_unitary_target_gradient_hessian_cplx")
+    V, Z, P = stack(d, "V", 5), stack(d, "Z", 5), perms_of(d, 5)
+    for k, nl in enumerate((4, 5)):
+        f, dv, hz = R.brickwall_unitary_target_hessian_vector_product(SCR_H, V[:nl], Z[:nl], P[:nl])
+        ref = (cplx(d[f"hess{k}"]) @ Z[:nl].reshape(-1)).reshape(nl, 4, 4)
+        assert abs(f - cplx(d[f"f{k}"])) < 1e-12
+        assert _max(dv - cplx(d[f"dVlist{k}"])) < 1e-12 and _max(hz - ref) < 1e-12
+    d = fixture("test_brickwall_unitary_target_gradient_vector_hessian_matrix_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    vec = np.random.default_rng(1).standard_normal(80)
+    Zs = np.stack([R.real_to_tangent(vec[16 * i:16 * i + 16], V[i]) for i in range(5)])
+    f, g, h = R.brickwall_unitary_target_projected_hessian_vector_product(SCR_H, V, Zs, P)
+    assert abs(f - cplx(d["f"])) < 1e-12 and _max(g - d["grad"]) < 1e-14 and _max(h - d["H"] @ vec) < 1e-12
+    d = fixture("test_brickwall_unitary_target_projected_hessian_vector_product_cplx")
+    V, Z, P = stack(d, "V", 4), stack(d, "Z", 4), perms_of(d, 4)
+    f, g, h = R.brickwall_unitary_target_projected_hessian_vector_product(SCR_H, V, Z, P)
+    f_o, g_o, h_o = O.brickwall_unitary_target_projected_hvp(SCR_O, V, Z, P, 6)
+    assert abs(f - cplx(d["f"])) < 1e-12 and reldiff(g, g_o) < TOL and reldiff(h, h_o) < TOL
+
+
+# ------------------------------------------------------------ random / edges
+@pytest.mark.parametrize("L", [2, 3, 5, 9])
+def test_apply_gate_all_wire_orders(L):
+    rng = np.random.default_rng(100 + L)
+    G = crandn(rng, (4, 4))
+    psi, dout = crandn(rng, 1 << L), crandn(rng, 1 << L)
+    for i in range(L):
+        for j in range(L):
+            if i == j:
+                continue
+            assert reldiff(R.apply_gate(G, i, j, psi), O.apply_gate(G, i, j, psi)) < 1e-14
+            dpsi, dG = R.apply_gate_backward(G, i, j, psi, dout)
+            dpsi_o, dG_o = O.apply_gate_backward(G, i, j, psi, dout)
+            assert reldiff(dpsi, dpsi_o) < 1e-14 and reldiff(dG, dG_o) < 1e-13
+
+
+@pytest.mark.parametrize("nstates", [1, 3, 8, 64])
+def test_apply_gate_to_array_widths(nstates):
+    rng = np.random.default_rng(nstates)
+    L = 6
+    G = crandn(rng, (4, 4))
+    psi = crandn(rng, (1 << L, nstates))
+    for (i, j) in ((0, 5), (5, 0), (2, 3), (4, 1)):
+        assert reldiff(R.apply_gate_to_array(G, i, j, psi), O.apply_gate_to_array(G, i, j, psi)) < 1e-14
+
+
+def test_add_statevectors():
+    rng = np.random.default_rng(5)
+    a, b = crandn(rng, 1 << 10), crandn(rng, 1 << 10)
+    assert np.array_equal(R.add_statevectors(a, b), a + b)
+
+
+def test_invalid_arguments_are_rejected():
+    rng = np.random.default_rng(6)
+    G = crandn(rng, (3, 4, 4))
+    with pytest.raises(R.RqcError):
+        R.apply_quantum_circuit(G, np.array([[0, 1], [1, 1], [2, 0]]), crandn(rng, 8))
+    with pytest.raises(R.RqcError):
+        R.apply_quantum_circuit(G, np.array([[0, 1], [1, 3], [2, 0]]), crandn(rng, 8))
+    with pytest.raises(R.RqcError):
+        R.apply_brickwall_unitary(G[:1], np.array([[0, 0, 1, 2]]), crandn(rng, 16))
+    bad = R.HostUfunc(lambda x: 1 / 0)
+    with pytest.raises(R.RqcError) as e:
+        R.brickwall_unitary_target(bad, G[:2], brickwall_perms(4, 2))
+    assert e.value.code == -2
+    T = R.DeviceTarget.dense(np.eye(16))
+    with pytest.raises(R.RqcError):
+        R.brickwall_unitary_target(T, G[:2], brickwall_perms(6, 2))
+
+
+def test_random_circuit_targets_vs_oracle():
+    rng = np.random.default_rng(7)
+    L, ng = 7, 9
+    U = crandn(rng, (1 << L, 1 << L))
+    T, Uo = R.DeviceTarget.dense(U), O.Ufunc.dense(U)
+    gates, dirs = crandn(rng, (ng, 4, 4)), crandn(rng, (ng, 4, 4))
+    wires = random_wires(rng, L, ng)
+    assert reldiff(R.circuit_unitary_target(T, gates, wires, L), O.circuit_unitary_target(Uo, gates, wires, L)) < TOL
+    f, dg, hg = R.circuit_unitary_target_hessian_vector_product(T, gates, dirs, wires, L)
+    f_o, dg_o, hg_o = O.circuit_unitary_target_hvp(Uo, gates, dirs, wires, L)
+    assert reldiff(f, f_o) < TOL and reldiff(dg, dg_o) < TOL and reldiff(hg, hg_o) < TOL
+
+
+@pytest.mark.parametrize("L,nl", [(4, 2), (6, 3), (8, 5), (10, 3)])
+def test_random_brickwall_targets_vs_oracle(L, nl):
+    rng = np.random.default_rng(L * 10 + nl)
+    U = np.linalg.qr(crandn(rng, (1 << L, 1 << L)))[0]
+    T, Uo = R.DeviceTarget.dense(U), O.Ufunc.dense(U)
+    V = np.stack([haar_unitary(rng) for _ in range(nl)])
+    P = random_perms(rng, L, nl)
+    vec = rng.standard_normal(16 * nl)
+    Z = np.stack([O.real_to_tangent(vec[16 * i:16 * i + 16], V[i]) for i in range(nl)])
+    f, g = R.brickwall_unitary_target_and_projected_gradient(T, V, P)
+    f_o, g_o = O.brickwall_unitary_target_and_projected_gradient(Uo, V, P, L)
+    assert abs(f - f_o) < TOL * max(1, abs(f_o)) and reldiff(g, g_o) < TOL
+    f, g, h = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
+    f_o, g_o, h_o = O.brickwall_unitary_target_projected_hvp(Uo, V, Z, P, L)
+    assert abs(f - f_o) < TOL * max(1, abs(f_o)) and reldiff(g, g_o) < TOL and reldiff(h, h_o) < TOL
+
+
+def test_ising_workload_small_vs_oracle():
+    """L=8 periodic brickwall d=5, Haar gates from the library RNG, U = expm(-iHt) of the Ising chain."""
+    from scipy.linalg import expm
+    from rqcopt_hpc_amd.workloads import ising_hamiltonian
+    L, nl = 8, 5
+    H = ising_hamiltonian(L, 1.0, 0.75, 0.6)
+    U = expm(-1j * 0.25 * H)
+    r = R.Rng(1)
+    V = np.stack([r.haar_unitary() for _ in range(nl)])
+    P = brickwall_perms(L, nl)
+    r2 = R.Rng(2)
+    vec = np.array([r2.randn() for _ in range(16 * nl)])
+    Z = np.stack([R.real_to_tangent(vec[16 * i:16 * i + 16], V[i]) for i in range(nl)])
+    T, Uo = R.DeviceTarget.dense(U), O.Ufunc.dense(U)
+    f, g, h = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
+    f_o, g_o, h_o = O.brickwall_unitary_target_projected_hvp(Uo, V, Z, P, L)
+    assert abs(f - f_o) < TOL * max(1, abs(f_o)) and reldiff(g, g_o) < TOL and reldiff(h, h_o) < TOL
+
+
+def test_column_blocking_and_determinism():
+    rng = np.random.default_rng(8)
+    L, nl = 8, 4
+    U = np.linalg.qr(crandn(rng, (1 << L, 1 << L)))[0]
+    T = R.DeviceTarget.dense(U)
+    V = np.stack([haar_unitary(rng) for _ in range(nl)])
+    Z = crandn(rng, (nl, 4, 4))
+    P = brickwall_perms(L, nl)
+    a = R.brickwall_unitary_target_hessian_vector_product(T, V, Z, P)
+    b = R.brickwall_unitary_target_hessian_vector_product(T, V, Z, P)
+    assert R.last_column_block() == 1 << L
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)  # bitwise reproducible
+    N = 1 << L
+    ng = nl * L // 2
+    R.set_workspace_limit((2 * ng + 5) * N * 16 * 16)  # room for a block of 16 columns
+    try:
+        c = R.brickwall_unitary_target_hessian_vector_product(T, V, Z, P)
+        assert R.last_column_block() == 16
+    finally:
+        R.set_workspace_limit(0)
+    for x, y in zip(a, c):
+        assert reldiff(x, y) < 1e-13
+
+
+def test_mixed_precision_error_bounds():
+    rng = np.random.default_rng(9)
+    L, nl = 10, 5
+    U = np.linalg.qr(crandn(rng, (1 << L, 1 << L)))[0]
+    T = R.DeviceTarget.dense(U)
+    V = np.stack([haar_unitary(rng) for _ in range(nl)])
+    vec = rng.standard_normal(16 * nl)
+    Z = np.stack([R.real_to_tangent(vec[16 * i:16 * i + 16], V[i]) for i in range(nl)])
+    P = brickwall_perms(L, nl)
+    f64 = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
+    R.set_precision(R.PRECISION_MIXED)
+    try:
+        f32 = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
+    finally:
+        R.set_precision(R.PRECISION_FP64)
+    N = 1 << L
+    # stated bounds: |df| / N <= 1e-6, gradient and HVP within 1e-4 relative (FP32 sweeps, FP64 sums)
+    assert abs(f32[0] - f64[0]) / N < 1e-6
+    assert reldiff(f32[1], f64[1]) < 1e-4 and reldiff(f32[2], f64[2]) < 1e-4
+
+
+def test_device_target_brickwall_and_ufunc():
+    rng = np.random.default_rng(10)
+    L, nl = 6, 3
+    Vt = np.stack([haar_unitary(rng) for _ in range(nl)])
+    P = brickwall_perms(L, nl)
+    T = R.DeviceTarget.brickwall(Vt, P)
+    W = brickwall_matrix(Vt, P)
+    assert _max(T.download() - W) < 1e-13
+    # f(W = U) = -N and the Riemannian gradient vanishes at the optimum
+    f, g = R.brickwall_unitary_target_and_projected_gradient(T, Vt, P)
+    assert abs(f + (1 << L)) < 1e-11 and _max(g) < 1e-11
+    T2 = R.DeviceTarget.from_ufunc(lambda x: W @ x, L)
+    assert _max(T2.download() - W) < 1e-14
+    psi = crandn(rng, 1 << L)
+    out = np.empty_like(psi)
+    r = R.lib().rqc_device_ufunc(ct.byref(R._sv(psi)), T.handle, ct.byref(R._sv(out)))
+    assert r == 0 and reldiff(out, W @ psi) < 1e-13
+
+
+def test_device_resident_statevectors():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(11)
+    L = 12
+    G = crandn(rng, (4, 4))
+    psi = crandn(rng, 1 << L)
+    d_in = torch.from_numpy(psi).to("cuda")
+    d_out = torch.empty_like(d_in)
+    torch.cuda.synchronize()
+    a = R.Statevector(d_in.data_ptr(), L)
+    b = R.Statevector(d_out.data_ptr(), L)
+    R.lib().apply_gate(R._p(R._cplx(G)), 3, 11, ct.byref(a), ct.byref(b))
+    R.synchronize()
+    assert reldiff(d_out.cpu().numpy(), O.apply_gate(G, 3, 11, psi)) < 1e-14
+
+
+def test_rtr_optimiser_vs_oracle():
+    rng = np.random.default_rng(12)
+    L, nl = 6, 3
+    P = brickwall_perms(L, nl)
+    Vt = np.stack([haar_unitary(rng) for _ in range(nl)])
+    U = brickwall_matrix(Vt, P)
+    V0 = np.stack([haar_unitary(rng) for _ in range(nl)])
+    f_iter, vopt = R.optimize_brickwall_circuit_hvp(R.DeviceTarget.dense(U), V0, P, 5)
+    f_o, vopt_o = O.optimize_brickwall_circuit_hvp(O.Ufunc.dense(U), V0, P, L, 5)
+    assert np.max(np.abs(f_iter - f_o) / np.maximum(1, np.abs(f_o))) < 1e-9
+    assert _max(vopt - vopt_o) < 1e-8
+    assert f_iter[-1] < f_iter[0]
+
+
+# ------------------------------------------------------- full-size properties
+@pytest.mark.parametrize("L,nl", [(12, 5), (14, 3)])
+def test_large_size_invariants(L, nl):
+    """At sizes beyond the oracle: f(W=U) = -N, zero Riemannian gradient at the optimum,
+    HVP linear in the direction and symmetric."""
+    rng = np.random.default_rng(L)
+    P = brickwall_perms(L, nl)
+    V = np.stack([haar_unitary(rng) for _ in range(nl)])
+    T = R.DeviceTarget.brickwall(V, P)
+    N = 1 << L
+    e1, e2 = rng.standard_normal(16 * nl), rng.standard_normal(16 * nl)
+    Z1 = np.stack([R.real_to_tangent(e1[16 * i:16 * i + 16], V[i]) for i in range(nl)])
+    Z2 = np.stack([R.real_to_tangent(e2[16 * i:16 * i + 16], V[i]) for i in range(nl)])
+    f, g, h1 = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z1, P)
+    assert abs(f + N) < 1e-12 * N
+    assert _max(g) < 1e-12 * N
+    _, _, h2 = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z2, P)
+    _, _, h12 = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, 2.0 * Z1 - 0.5 * Z2, P)
+    assert reldiff(h12, 2.0 * h1 - 0.5 * h2) < 1e-12
+    assert abs(e2 @ h1 - e1 @ h2) < 1e-10 * max(1.0, abs(e1 @ h2))
+    # at the optimum the projected Hessian is positive semidefinite
+    assert e1 @ h1 >= -1e-9 * N
