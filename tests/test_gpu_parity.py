@@ -309,20 +309,45 @@ def test_device_target_brickwall_and_ufunc():
     assert r == 0 and reldiff(out, W @ psi) < 1e-13
 
 
+def _hip():
+    """The HIP runtime librqcopt_hip is linked against (already loaded in this process)."""
+    h = ct.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [ct.POINTER(ct.c_void_p), ct.c_size_t]
+    h.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+    h.hipFree.argtypes = [ct.c_void_p]
+    return h
+
+
 def test_device_resident_statevectors():
-    torch = pytest.importorskip("torch")
+    """Device pointers in struct statevector are used in place (no host staging)."""
+    hip = _hip()
     rng = np.random.default_rng(11)
     L = 12
     G = crandn(rng, (4, 4))
     psi = crandn(rng, 1 << L)
-    d_in = torch.from_numpy(psi).to("cuda")
-    d_out = torch.empty_like(d_in)
-    torch.cuda.synchronize()
-    a = R.Statevector(d_in.data_ptr(), L)
-    b = R.Statevector(d_out.data_ptr(), L)
-    R.lib().apply_gate(R._p(R._cplx(G)), 3, 11, ct.byref(a), ct.byref(b))
-    R.synchronize()
-    assert reldiff(d_out.cpu().numpy(), O.apply_gate(G, 3, 11, psi)) < 1e-14
+    nbytes = psi.nbytes
+    d_in, d_out = ct.c_void_p(), ct.c_void_p()
+    assert hip.hipMalloc(ct.byref(d_in), nbytes) == 0 and hip.hipMalloc(ct.byref(d_out), nbytes) == 0
+    try:
+        assert hip.hipMemcpy(d_in, psi.ctypes.data, nbytes, 1) == 0  # host to device
+        a = R.Statevector(d_in.value, L)
+        b = R.Statevector(d_out.value, L)
+        R.lib().apply_gate(R._p(R._cplx(G)), 3, 11, ct.byref(a), ct.byref(b))
+        R.synchronize()
+        out = np.empty_like(psi)
+        assert hip.hipMemcpy(out.ctypes.data, d_out, nbytes, 2) == 0  # device to host
+        assert reldiff(out, O.apply_gate(G, 3, 11, psi)) < 1e-14
+        # a whole circuit with device-resident input and output
+        gates = crandn(rng, (6, 4, 4))
+        wires = random_wires(rng, L, 6).reshape(-1)
+        g = R._gates(gates)
+        r = R.lib().apply_quantum_circuit(R._p(g), 6, R._p(wires), ct.byref(a), ct.byref(b))
+        assert r == 0
+        assert hip.hipMemcpy(out.ctypes.data, d_out, nbytes, 2) == 0
+        assert reldiff(out, O.apply_quantum_circuit(gates, wires.reshape(-1, 2), psi)) < 1e-13
+    finally:
+        hip.hipFree(d_in)
+        hip.hipFree(d_out)
 
 
 def test_rtr_optimiser_vs_oracle():
