@@ -90,8 +90,14 @@ def run(args):
     def one():
         return R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
 
+    copy_gbs = R.measure_copy_bandwidth(1 << 30, 5)
+    print(f"[bench] device copy bandwidth {copy_gbs:.0f} GB/s (1 GiB streaming copy, read + write)",
+          file=sys.stderr, flush=True)
+    precs = [(R.PRECISION_FP64, "fp64"), (R.PRECISION_MIXED, "mixed")]
+    if args.precision != "both":
+        precs = [p for p in precs if p[1] == args.precision]
     results = {}
-    for prec, name in ((R.PRECISION_FP64, "fp64"), (R.PRECISION_MIXED, "mixed")):
+    for prec, name in precs:
         R.set_precision(prec)
         for _ in range(args.warmup):
             out = one()
@@ -103,14 +109,18 @@ def run(args):
         dt = time.perf_counter() - t0
         # kernel timing pass (HIP events on the library's stream)
         R.profile_reset()
-        R.profile_enable(True)
-        one()
-        R.profile_enable(False)
+        if not args.no_profile_pass:
+            R.profile_enable(True)
+            one()
+            R.profile_enable(False)
         prof = {k: R.profile_query(k) for k in range(4)}
         results[name] = {"eval_s": dt / args.steps, "out": out, "prof": prof, "block": R.last_column_block()}
         print(f"[bench] {name}: {dt / args.steps * 1e3:.2f} ms/eval, column block {R.last_column_block()}, "
               f"kernel ms {[round(prof[k][0], 2) for k in range(4)]}", file=sys.stderr, flush=True)
     R.set_precision(R.PRECISION_FP64)
+    if args.precision != "both" or args.no_profile_pass:
+        print(json.dumps({k: {"ms_per_eval": v["eval_s"] * 1e3, "column_block": v["block"]} for k, v in results.items()}))
+        return
 
     r64, r32 = results["fp64"], results["mixed"]
     s = 16
@@ -139,6 +149,7 @@ def run(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded PCG Haar gates, Ising-Trotter target formed on the GPU)",
+        "copy_bandwidth_gbs": copy_gbs,
         "config": {"workload": f"brickwall grad+HVP eval, L={L}, depth {d}, periodic, Ising-Trotter U (t=0.5, 4 Strang steps)",
                    "L": L, "depth": d, "gates": G, "column_block": B, "device": info},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -170,6 +181,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-cols", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
+                    help="profiling runs: time one precision only and print a short summary")
+    ap.add_argument("--no-profile-pass", action="store_true", help="skip the HIP-event kernel timing pass")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU column sharding is a "

@@ -27,6 +27,12 @@ enum rqc_precision
 	RQC_PRECISION_MIXED = 1,  /* complex FP32 sweeps, FP64 reductions, FP64 projection */
 };
 
+enum rqc_sweep_engine
+{
+	RQC_ENGINE_TILE_PASS = 0,  /* several gates per HBM pass through LDS tiles (default) */
+	RQC_ENGINE_PER_GATE  = 1,  /* one gate per pass over the column block */
+};
+
 enum rqc_kernel_kind
 {
 	RQC_KERNEL_FORWARD = 0,   /* forward sweep of one gate over the column block */
@@ -57,6 +63,10 @@ int rqc_target_download(const struct rqc_target* target, numeric* U);
 int rqc_set_precision(int mode);
 int rqc_get_precision(void);
 
+/* sweep engine of the target / gradient / HVP functions (process-wide); also
+   selectable with the environment variable RQC_SWEEP_ENGINE=pergate */
+int rqc_set_sweep_engine(int engine);
+
 /* device workspace: upper bound in bytes (0 = automatic, about 85% of free HBM);
    the workspace is kept between calls and released here */
 int rqc_set_workspace_limit(size_t bytes);
@@ -69,6 +79,12 @@ long rqc_last_column_block(void);
 void rqc_profile_enable(int on);
 void rqc_profile_reset(void);
 int rqc_profile_query(int kind, double* total_ms, long* launches);
+
+/* device-to-device copy of `bytes` bytes by a plain streaming kernel (reads
+   `bytes`, writes `bytes`), timed with HIP events over `reps` launches; gives the
+   measured copy bandwidth in GB/s (read + write bytes per second).  Used to
+   calibrate HBM byte counters and the roofline peak on the running device. */
+int rqc_measure_copy_bandwidth(size_t bytes, int reps, double* gbps);
 
 /* blocks until all work of the library's stream has finished */
 int rqc_synchronize(void);

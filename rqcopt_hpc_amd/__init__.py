@@ -30,6 +30,7 @@ _LIB = None
 PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 KERNEL_FORWARD, KERNEL_REVERSE, KERNEL_TARGET, KERNEL_OTHER = range(4)
+ENGINE_TILE_PASS, ENGINE_PER_GATE = 0, 1
 
 
 class RqcError(RuntimeError):
@@ -111,7 +112,8 @@ EXPORTED = [
     "rqc_device_ufunc", "rqc_target_create_dense", "rqc_target_create_from_ufunc", "rqc_target_create_brickwall",
     "rqc_target_destroy", "rqc_target_nqubits", "rqc_target_download", "rqc_set_precision", "rqc_get_precision",
     "rqc_set_workspace_limit", "rqc_release_workspace", "rqc_last_column_block", "rqc_profile_enable",
-    "rqc_profile_reset", "rqc_profile_query", "rqc_synchronize", "rqc_device_info",
+    "rqc_profile_reset", "rqc_profile_query", "rqc_synchronize", "rqc_device_info", "rqc_measure_copy_bandwidth",
+    "rqc_set_sweep_engine",
 ]
 
 
@@ -635,6 +637,10 @@ def set_precision(mode):
     _check("rqc_set_precision", lib().rqc_set_precision(mode))
 
 
+def set_sweep_engine(engine):
+    _check("rqc_set_sweep_engine", lib().rqc_set_sweep_engine(engine))
+
+
 def get_precision():
     return lib().rqc_get_precision()
 
@@ -663,6 +669,15 @@ def profile_query(kind):
     ms, n = ct.c_double(), ct.c_long()
     _check("rqc_profile_query", lib().rqc_profile_query(kind, ct.byref(ms), ct.byref(n)))
     return ms.value, n.value
+
+
+def measure_copy_bandwidth(nbytes=1 << 30, reps=10):
+    """GB/s of a plain device-to-device streaming copy (read + write bytes)."""
+    L = lib()
+    L.rqc_measure_copy_bandwidth.argtypes = [ct.c_size_t, ct.c_int, ct.POINTER(ct.c_double)]
+    g = ct.c_double()
+    _check("rqc_measure_copy_bandwidth", L.rqc_measure_copy_bandwidth(nbytes, reps, ct.byref(g)))
+    return g.value
 
 
 def synchronize():

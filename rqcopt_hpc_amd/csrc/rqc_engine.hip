@@ -22,6 +22,7 @@
 
 #include "rqc_internal.hpp"
 #include "rqc_kernels.hpp"
+#include "rqc_pass_kernels.hpp"
 #include "mat4.hpp"
 #include "rqcopt/gate.h"
 #include "rqcopt/statevector.h"
@@ -73,6 +74,9 @@ Context& ctx()
 		cc->cu_count = p.multiProcessorCount;
 		cc->hbm_bytes = p.totalGlobalMem;
 		cc->name = p.gcnArchName;
+		cc->lds_per_block = p.sharedMemPerBlock;
+		const char* eng = getenv("RQC_SWEEP_ENGINE");
+		if (eng != nullptr && std::string(eng) == "pergate") { cc->engine = RQC_ENGINE_PER_GATE; }
 		RQC_HIP(hipStreamCreateWithFlags(&cc->stream, hipStreamNonBlocking));
 		c = cc;
 	}
@@ -338,6 +342,203 @@ static void sweep_targets(Context& c, const Circuit& C, const zc* dU, Mode mode,
 	}
 }
 
+// ---------------------------------------------------------------- tile passes
+struct Pass
+{
+	int g0 = 0, ng = 0;
+	std::vector<int> rb;  // tile row bits, ascending
+};
+
+// greedy split of the gate sequence into runs whose wires fit K tile qubits,
+// at most gmax gates each; tiles are padded with the lowest unused row bits
+static std::vector<Pass> plan_passes(const Circuit& C, int K, int gmax)
+{
+	const int L = C.L;
+	std::vector<Pass> out;
+	int k = 0;
+	while (k < C.ngates) {
+		Pass p;
+		p.g0 = k;
+		std::vector<int> Q;
+		while (k < C.ngates && p.ng < gmax) {
+			std::vector<int> Q2 = Q;
+			for (int b : {L - 1 - C.wi[k], L - 1 - C.wj[k]}) {
+				if (std::find(Q2.begin(), Q2.end(), b) == Q2.end()) { Q2.push_back(b); }
+			}
+			if ((int)Q2.size() > K) { break; }
+			Q = Q2;
+			p.ng++;
+			k++;
+		}
+		for (int b = 0; b < L && (int)Q.size() < K; b++) {
+			if (std::find(Q.begin(), Q.end(), b) == Q.end()) { Q.push_back(b); }
+		}
+		std::sort(Q.begin(), Q.end());
+		p.rb = Q;
+		out.push_back(p);
+	}
+	return out;
+}
+
+template <typename T>
+static PassArgs<T> make_pass_args(const Circuit& C, const Pass& p, int K, int logC, bool dual)
+{
+	PassArgs<T> a;
+	std::memset(&a, 0, sizeof(a));
+	a.K = K;
+	a.logC = logC;
+	a.nrest = C.L - K;
+	a.ng = p.ng;
+	std::vector<int> tile_of(C.L, -1);
+	for (int t = 0; t < K; t++) { a.rb[t] = p.rb[t]; tile_of[p.rb[t]] = t; }
+	int u = 0;
+	for (int b = 0; b < C.L; b++) {
+		if (tile_of[b] < 0) { a.restb[u++] = b; }
+	}
+	for (int j = 0; j < p.ng; j++) {
+		const int k = p.g0 + j;
+		a.lp[j] = tile_of[C.L - 1 - C.wi[k]];
+		a.lq[j] = tile_of[C.L - 1 - C.wj[k]];
+		a.G[j] = to_gate<T>(&C.gates[16 * (size_t)k]);
+		a.Z[j] = dual ? to_gate<T>(&C.dirs[16 * (size_t)k]) : a.G[j];
+	}
+	return a;
+}
+
+template <typename KernelT>
+static void allow_lds(KernelT* fn, size_t bytes)
+{
+	if (bytes > 65536) {
+		RQC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+	}
+}
+
+// tile shape for element type T: 16 KiB per LDS state, 128-byte tile rows
+struct TileShape
+{
+	int K, logC, gmax;
+	size_t state_bytes;
+};
+
+template <typename T>
+static TileShape tile_shape(Context& c, int L, long B, Mode mode)
+{
+	const int elem = (int)sizeof(cx<T>);
+	const int logE = (elem == 16) ? 10 : 11;
+	const int logCdef = (elem == 16) ? 3 : 4;
+	TileShape s;
+	s.K = std::min(L, logE - logCdef);
+	int logB = 0;
+	while ((1L << (logB + 1)) <= B) { logB++; }
+	s.logC = std::min(logE - s.K, logB);
+	s.state_bytes = (size_t)elem << (s.K + s.logC);
+	const long nslots = (long)(c.lds_per_block / s.state_bytes);
+	if (mode == Mode::Hvp) { s.gmax = (int)std::min<long>(kMaxPassGates, (nslots - 2) / 2); }
+	else { s.gmax = (int)std::min<long>(kMaxPassGates, nslots - 1); }
+	if (s.gmax < 1) { throw Error(-1, "LDS per workgroup too small for one tile pass"); }
+	return s;
+}
+
+template <typename T>
+static void sweep_passes(Context& c, const Circuit& C, const zc* dU, Mode mode, zc* d_acc, zc* d_partial)
+{
+	const int L = C.L, G = C.ngates;
+	const long N = 1L << L;
+	const bool dual = (mode == Mode::Hvp);
+	// the pass plan does not depend on B: plan with the widest tile shape first
+	TileShape ts = tile_shape<T>(c, L, N, mode);
+	std::vector<Pass> passes = plan_passes(C, ts.K, ts.gmax);
+	const int P = (int)passes.size();
+	const int nst = (mode == Mode::Target) ? 2 : (mode == Mode::Grad) ? P + 2 : 2 * P + 3;
+	const long B = choose_block(c, N, nst, sizeof(cx<T>));
+	c.last_block = B;
+	ts = tile_shape<T>(c, L, B, mode);
+	const int logB = ilog2_or_neg(B);
+	const size_t sz = (size_t)N * B;
+	c.states.reserve((size_t)nst * sz * sizeof(cx<T>));
+	cx<T>* S = c.states.as<cx<T>>();
+	auto PS = [&](int p) { return S + (size_t)(p - 1) * sz; };        // psi after pass p-1, p = 1 .. P
+	auto DPS = [&](int p) { return S + (size_t)(P + p - 1) * sz; };   // dpsi after pass p-1, p = 1 .. P-1
+	cx<T>* buf[2] = {S, S + sz};
+	cx<T>* phi[2] = {S + (size_t)(dual ? 2 * P - 1 : P) * sz, S + (size_t)(dual ? 2 * P : P + 1) * sz};
+	cx<T>* dphi[2] = {S + (size_t)(2 * P + 1) * sz, S + (size_t)(2 * P + 2) * sz};
+
+	const long ntiles = (N >> ts.K) * (B >> ts.logC);
+	const unsigned gfwd = (unsigned)std::min<long>(ntiles, 4L * c.cu_count);
+	const unsigned grev = (unsigned)std::min<long>(ntiles, 2L * c.cu_count);
+	const size_t lds_fwd = (dual ? 2 : 1) * ts.state_bytes;
+	const size_t lds_rev_max = (size_t)(dual ? 2 * ts.gmax + 2 : ts.gmax + 1) * ts.state_bytes;
+	allow_lds(k_pass_forward<T, true>, lds_fwd);
+	allow_lds(k_pass_forward<T, false>, lds_fwd);
+	allow_lds(k_pass_reverse<T, true>, lds_rev_max);
+	allow_lds(k_pass_reverse<T, false>, lds_rev_max);
+	double* mpart = reinterpret_cast<double*>(d_partial);     // grev * waves * gmax * 256 doubles
+	zc* tpart = d_partial;                                      // trace partials (reused, sequential)
+
+	{
+		LaunchTimer lt(c, RQC_KERNEL_OTHER);
+		k_fill_zero<<<nblocks(1 + 32L * G), kBlock, 0, c.stream>>>(d_acc, 1 + 32L * G);
+	}
+	std::vector<PassArgs<T>> args;
+	for (const Pass& p : passes) { args.push_back(make_pass_args<T>(C, p, ts.K, ts.logC, dual)); }
+	const unsigned gtr = reduce_grid(N * B);
+
+	for (long b0 = 0; b0 < N; b0 += B) {
+		// forward passes
+		for (int p = 0; p < P; p++) {
+			const int flags = (p == 0) ? 3 : 0;
+			LaunchTimer lt(c, RQC_KERNEL_FORWARD);
+			if (mode == Mode::Target) {
+				k_pass_forward<T, false><<<gfwd, kPassThreads, ts.state_bytes, c.stream>>>(
+					p == 0 ? nullptr : buf[(p + 1) % 2], buf[p % 2], nullptr, nullptr, args[p], B, b0, ntiles, flags);
+			} else if (dual && p < P - 1) {
+				k_pass_forward<T, true><<<gfwd, kPassThreads, 2 * ts.state_bytes, c.stream>>>(
+					p == 0 ? nullptr : PS(p), PS(p + 1), p == 0 ? nullptr : DPS(p), DPS(p + 1), args[p], B, b0, ntiles, flags);
+			} else {
+				k_pass_forward<T, false><<<gfwd, kPassThreads, ts.state_bytes, c.stream>>>(
+					p == 0 ? nullptr : PS(p), PS(p + 1), nullptr, nullptr, args[p], B, b0, ntiles, flags);
+			}
+		}
+		cx<T>* psiG = (mode == Mode::Target) ? buf[(P - 1) % 2] : PS(P);
+		{
+			LaunchTimer lt(c, RQC_KERNEL_TARGET);
+			k_target_trace<T><<<gtr, kBlock, 0, c.stream>>>(dU, N, b0, psiG, mode == Mode::Target ? nullptr : phi[0],
+				B, logB, N * B, tpart);
+		}
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_reduce_partials<<<1, kBlock, 0, c.stream>>>(tpart, (int)gtr, 1, d_acc);
+		}
+		if (mode == Mode::Target) { continue; }
+		// reverse passes
+		int cur = 0;
+		for (int p = P - 1; p >= 0; p--) {
+			int flags = 0;
+			if (p == 0) { flags |= 1 | 2 | 8; }
+			if (p == P - 1) { flags |= 4; }
+			const int ng = passes[p].ng;
+			const size_t lds = (size_t)(dual ? 2 * ng + 2 : ng + 1) * ts.state_bytes;
+			{
+				LaunchTimer lt(c, RQC_KERNEL_REVERSE);
+				if (dual) {
+					k_pass_reverse<T, true><<<grev, kPassThreads, lds, c.stream>>>(
+						p == 0 ? nullptr : PS(p), p == 0 ? nullptr : DPS(p), phi[cur], dphi[cur], phi[1 - cur], dphi[1 - cur],
+						args[p], B, b0, ntiles, flags, mpart);
+				} else {
+					k_pass_reverse<T, false><<<grev, kPassThreads, lds, c.stream>>>(
+						p == 0 ? nullptr : PS(p), nullptr, phi[cur], nullptr, phi[1 - cur], nullptr,
+						args[p], B, b0, ntiles, flags, mpart);
+				}
+			}
+			{
+				LaunchTimer lt(c, RQC_KERNEL_OTHER);
+				k_pass_reduce<<<ng, 256, 0, c.stream>>>(mpart, (long)grev * kPassWaves, ng, passes[p].g0, d_acc);
+			}
+			cur = 1 - cur;
+		}
+	}
+}
+
 // per-group sums of per-gate gradients / HVPs and the Riemannian projection
 // (reference src/target.c:526-535 and :650-671), one thread per group
 __global__ void k_finalize(const zc* __restrict__ acc, int ngroups, int per, const zc* __restrict__ vlist,
@@ -378,7 +579,10 @@ static TargetResult eval_targets(const Circuit& C, linear_func ufunc, void* udat
 	const int G = C.ngates, ng = G / per;
 	// small buffer: acc | partial | vlist | vdirs | outputs
 	const size_t n_acc = 1 + 32 * (size_t)G;
-	const size_t n_part = 4096 * (size_t)kWidth;
+	// partial sums: per-gate reverse (4096 x 32 complex) or tile-pass matrix-core partials
+	// (2 x CUs workgroups x waves x gates x 256 doubles), whichever is larger
+	const size_t n_part = std::max<size_t>(4096 * (size_t)kWidth,
+		(size_t)2 * c.cu_count * kPassWaves * kMaxPassGates * 256 / 2 + 4096);
 	const size_t n_in = 2 * 16 * (size_t)ng;
 	const size_t n_out = 2 * 16 * (size_t)ng;
 	const size_t n_real = 2 * 16 * (size_t)ng;
@@ -388,10 +592,13 @@ static TargetResult eval_targets(const Circuit& C, linear_func ufunc, void* udat
 	zc* vin = part + n_part;
 	zc* out = vin + n_in;
 	double* rout = reinterpret_cast<double*>(out + n_out);
-	if (c.precision == RQC_PRECISION_MIXED) {
-		sweep_targets<float>(c, C, dU, mode, acc, part);
+	// tiles need at least 16 quads (one matrix-core group per wave): L >= 3
+	if (c.engine == RQC_ENGINE_PER_GATE || C.L < 3) {
+		if (c.precision == RQC_PRECISION_MIXED) { sweep_targets<float>(c, C, dU, mode, acc, part); }
+		else { sweep_targets<double>(c, C, dU, mode, acc, part); }
 	} else {
-		sweep_targets<double>(c, C, dU, mode, acc, part);
+		if (c.precision == RQC_PRECISION_MIXED) { sweep_passes<float>(c, C, dU, mode, acc, part); }
+		else { sweep_passes<double>(c, C, dU, mode, acc, part); }
 	}
 	// gates of each group (the first gate of the group) and directions
 	std::vector<zc> hv(n_in);
@@ -1201,6 +1408,12 @@ extern "C" int rqc_set_precision(int mode)
 	return target_call("rqc_set_precision", [&] { ctx().precision = mode; });
 }
 
+extern "C" int rqc_set_sweep_engine(int engine)
+{
+	if (engine != RQC_ENGINE_TILE_PASS && engine != RQC_ENGINE_PER_GATE) { return -3; }
+	return target_call("rqc_set_sweep_engine", [&] { ctx().engine = engine; });
+}
+
 extern "C" int rqc_get_precision(void)
 {
 	try { return ctx().precision; } catch (...) { return -1; }
@@ -1260,6 +1473,34 @@ extern "C" int rqc_profile_query(int kind, double* total_ms, long* launches)
 	});
 }
 
+extern "C" int rqc_measure_copy_bandwidth(size_t bytes, int reps, double* gbps)
+{
+	return target_call("rqc_measure_copy_bandwidth", [&] {
+		if (reps < 1 || bytes < 16) { throw Error(-3, "need reps >= 1 and bytes >= 16"); }
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const long n = (long)(bytes / 16);
+		c.states.reserve(2 * (size_t)n * 16);
+		zc* a = c.states.as<zc>();
+		zc* b = a + n;
+		const unsigned grid = (unsigned)std::min<long>((n + kBlock - 1) / kBlock, 8L * c.cu_count * 8);
+		k_fill_zero<<<nblocks(n), kBlock, 0, c.stream>>>(a, n);
+		hipEvent_t e0, e1;
+		RQC_HIP(hipEventCreate(&e0));
+		RQC_HIP(hipEventCreate(&e1));
+		k_copy16<<<grid, kBlock, 0, c.stream>>>(a, b, n);  // warm-up
+		RQC_HIP(hipEventRecord(e0, c.stream));
+		for (int r = 0; r < reps; r++) { k_copy16<<<grid, kBlock, 0, c.stream>>>(a, b, n); }
+		RQC_HIP(hipEventRecord(e1, c.stream));
+		RQC_HIP(hipEventSynchronize(e1));
+		float ms = 0;
+		RQC_HIP(hipEventElapsedTime(&ms, e0, e1));
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		*gbps = 2.0 * (double)n * 16 * reps / (ms * 1e-3) / 1e9;
+	});
+}
+
 extern "C" int rqc_synchronize(void)
 {
 	return target_call("rqc_synchronize", [&] { RQC_HIP(hipStreamSynchronize(ctx().stream)); });
@@ -1275,6 +1516,6 @@ extern "C" int rqc_device_info(char* buf, int len)
 	}
 	return target_call("rqc_device_info", [&] {
 		Context& c = ctx();
-		snprintf(buf, len, "%s %d %zu", c.name.c_str(), c.cu_count, c.hbm_bytes);
+		snprintf(buf, len, "%s %d %zu lds/wg=%zu", c.name.c_str(), c.cu_count, c.hbm_bytes, c.lds_per_block);
 	});
 }

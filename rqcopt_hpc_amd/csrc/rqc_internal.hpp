@@ -51,6 +51,8 @@ struct Context
 	size_t hbm_bytes = 0;
 	std::string name;
 	int precision = RQC_PRECISION_FP64;
+	int engine = RQC_ENGINE_TILE_PASS;
+	size_t lds_per_block = 65536;
 	size_t ws_limit = 0;  // 0 = automatic
 	long last_block = 0;
 

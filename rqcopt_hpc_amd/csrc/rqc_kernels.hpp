@@ -306,6 +306,13 @@ __global__ __launch_bounds__(kBlock) void k_matvec(const zc* __restrict__ U, con
 	y[a] = s;
 }
 
+// plain streaming copy (16-byte elements), used to calibrate HBM counters
+__global__ __launch_bounds__(kBlock) void k_copy16(const zc* __restrict__ src, zc* __restrict__ dst, long n)
+{
+	const long stride = (long)gridDim.x * kBlock;
+	for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < n; t += stride) { dst[t] = src[t]; }
+}
+
 // copy column block X (n x B, column fastest) into columns [b0, b0 + B) of the row-major n x n U
 __global__ __launch_bounds__(kBlock) void k_store_columns(const zc* __restrict__ X, zc* __restrict__ U, long n, long B, long b0)
 {

@@ -364,6 +364,67 @@ def test_rtr_optimiser_vs_oracle():
     assert f_iter[-1] < f_iter[0]
 
 
+@pytest.mark.parametrize("L", [2, 3, 5, 7, 8, 11])
+def test_sweep_engines_agree(L):
+    """Tile-pass engine (several gates per HBM pass) vs per-gate engine vs oracle, random wires."""
+    rng = np.random.default_rng(200 + L)
+    ng = 3 * L
+    gates, dirs = crandn(rng, (ng, 4, 4)), crandn(rng, (ng, 4, 4))
+    wires = random_wires(rng, L, ng)
+    U = crandn(rng, (1 << L, 1 << L))
+    T = R.DeviceTarget.dense(U)
+    res = {}
+    for eng in (R.ENGINE_TILE_PASS, R.ENGINE_PER_GATE):
+        R.set_sweep_engine(eng)
+        try:
+            res[eng] = (R.circuit_unitary_target(T, gates, wires, L),
+                        R.circuit_unitary_target_and_gradient(T, gates, wires, L),
+                        R.circuit_unitary_target_hessian_vector_product(T, gates, dirs, wires, L))
+        finally:
+            R.set_sweep_engine(R.ENGINE_TILE_PASS)
+    a, b = res[R.ENGINE_TILE_PASS], res[R.ENGINE_PER_GATE]
+    assert reldiff(a[0], b[0]) < TOL
+    assert reldiff(a[1][1], b[1][1]) < TOL
+    for x, y in zip(a[2], b[2]):
+        assert reldiff(x, y) < TOL
+    if L <= 8:
+        f_o, dg_o, hg_o = O.circuit_unitary_target_hvp(O.Ufunc.dense(U), gates, dirs, wires, L)
+        assert reldiff(a[2][0], f_o) < TOL and reldiff(a[2][1], dg_o) < TOL and reldiff(a[2][2], hg_o) < TOL
+
+
+def test_tile_pass_engine_column_blocks_and_mixed():
+    rng = np.random.default_rng(300)
+    L, nl = 12, 4
+    P = brickwall_perms(L, nl)
+    V = np.stack([haar_unitary(rng) for _ in range(nl)])
+    Z = crandn(rng, (nl, 4, 4))
+    T = R.DeviceTarget.brickwall(np.stack([haar_unitary(rng) for _ in range(3)]), brickwall_perms(L, 3))
+    full = R.brickwall_unitary_target_hessian_vector_product(T, V, Z, P)
+    assert R.last_column_block() == 1 << L
+    R.set_workspace_limit(64 << 20)
+    try:
+        blk = R.brickwall_unitary_target_hessian_vector_product(T, V, Z, P)
+        assert R.last_column_block() < 1 << L
+    finally:
+        R.set_workspace_limit(0)
+    for x, y in zip(full, blk):
+        assert reldiff(x, y) < 1e-13
+    R.set_sweep_engine(R.ENGINE_PER_GATE)
+    try:
+        ref = R.brickwall_unitary_target_hessian_vector_product(T, V, Z, P)
+    finally:
+        R.set_sweep_engine(R.ENGINE_TILE_PASS)
+    for x, y in zip(full, ref):
+        assert reldiff(x, y) < TOL
+    R.set_precision(R.PRECISION_MIXED)
+    try:
+        mixed = R.brickwall_unitary_target_hessian_vector_product(T, V, Z, P)
+    finally:
+        R.set_precision(R.PRECISION_FP64)
+    assert abs(mixed[0] - full[0]) / (1 << L) < 1e-6
+    assert reldiff(mixed[1], full[1]) < 1e-4 and reldiff(mixed[2], full[2]) < 1e-4
+
+
 # ------------------------------------------------------- full-size properties
 @pytest.mark.parametrize("L,nl", [(12, 5), (14, 3)])
 def test_large_size_invariants(L, nl):
