@@ -389,6 +389,7 @@ static PassArgs<T> make_pass_args(const Circuit& C, const Pass& p, int K, int lo
 	a.logC = logC;
 	a.nrest = C.L - K;
 	a.ng = p.ng;
+	a.g0 = p.g0;
 	std::vector<int> tile_of(C.L, -1);
 	for (int t = 0; t < K; t++) { a.rb[t] = p.rb[t]; tile_of[p.rb[t]] = t; }
 	int u = 0;
@@ -399,9 +400,8 @@ static PassArgs<T> make_pass_args(const Circuit& C, const Pass& p, int K, int lo
 		const int k = p.g0 + j;
 		a.lp[j] = tile_of[C.L - 1 - C.wi[k]];
 		a.lq[j] = tile_of[C.L - 1 - C.wj[k]];
-		a.G[j] = to_gate<T>(&C.gates[16 * (size_t)k]);
-		a.Z[j] = dual ? to_gate<T>(&C.dirs[16 * (size_t)k]) : a.G[j];
 	}
+	(void)dual;
 	return a;
 }
 
@@ -424,7 +424,8 @@ template <typename T>
 static TileShape tile_shape(Context& c, int L, long B, Mode mode)
 {
 	const int elem = (int)sizeof(cx<T>);
-	const int logE = (elem == 16) ? 10 : 11;
+	int logE = (elem == 16) ? 10 : 11;
+	if (const char* v = getenv(elem == 16 ? "RQC_TILE_LOGE_FP64" : "RQC_TILE_LOGE_FP32")) { logE = atoi(v); }
 	const int logCdef = (elem == 16) ? 3 : 4;
 	TileShape s;
 	s.K = std::min(L, logE - logCdef);
@@ -432,9 +433,11 @@ static TileShape tile_shape(Context& c, int L, long B, Mode mode)
 	while ((1L << (logB + 1)) <= B) { logB++; }
 	s.logC = std::min(logE - s.K, logB);
 	s.state_bytes = (size_t)elem << (s.K + s.logC);
-	const long nslots = (long)(c.lds_per_block / s.state_bytes);
+	const size_t table = 32 * kMaxPassGates * (size_t)elem;   // gate table in LDS
+	const long nslots = (long)((c.lds_per_block - table) / s.state_bytes);
 	if (mode == Mode::Hvp) { s.gmax = (int)std::min<long>(kMaxPassGates, (nslots - 2) / 2); }
 	else { s.gmax = (int)std::min<long>(kMaxPassGates, nslots - 1); }
+	if (const char* v = getenv("RQC_PASS_GMAX")) { s.gmax = std::max(1, std::min(s.gmax, atoi(v))); }
 	if (s.gmax < 1) { throw Error(-1, "LDS per workgroup too small for one tile pass"); }
 	return s;
 }
@@ -464,10 +467,13 @@ static void sweep_passes(Context& c, const Circuit& C, const zc* dU, Mode mode, 
 	cx<T>* dphi[2] = {S + (size_t)(2 * P + 1) * sz, S + (size_t)(2 * P + 2) * sz};
 
 	const long ntiles = (N >> ts.K) * (B >> ts.logC);
-	const unsigned gfwd = (unsigned)std::min<long>(ntiles, 4L * c.cu_count);
-	const unsigned grev = (unsigned)std::min<long>(ntiles, 2L * c.cu_count);
-	const size_t lds_fwd = (dual ? 2 : 1) * ts.state_bytes;
-	const size_t lds_rev_max = (size_t)(dual ? 2 * ts.gmax + 2 : ts.gmax + 1) * ts.state_bytes;
+	const long fit_fwd = std::max<long>(1, std::min<long>(8, (long)(c.lds_per_block / ((dual ? 2 : 1) * ts.state_bytes + 32 * kMaxPassGates * sizeof(cx<T>)))));
+	const long fit_rev = std::max<long>(1, std::min<long>(2, (long)(c.lds_per_block / ((size_t)(dual ? 2 * ts.gmax + 2 : ts.gmax + 1) * ts.state_bytes + 32 * kMaxPassGates * sizeof(cx<T>)))));
+	const unsigned gfwd = (unsigned)std::min<long>(ntiles, fit_fwd * c.cu_count);
+	const unsigned grev = (unsigned)std::min<long>(ntiles, fit_rev * c.cu_count);
+	const size_t table = 32 * kMaxPassGates * sizeof(cx<T>);
+	const size_t lds_fwd = (dual ? 2 : 1) * ts.state_bytes + table;
+	const size_t lds_rev_max = (size_t)(dual ? 2 * ts.gmax + 2 : ts.gmax + 1) * ts.state_bytes + table;
 	allow_lds(k_pass_forward<T, true>, lds_fwd);
 	allow_lds(k_pass_forward<T, false>, lds_fwd);
 	allow_lds(k_pass_reverse<T, true>, lds_rev_max);
@@ -481,6 +487,19 @@ static void sweep_passes(Context& c, const Circuit& C, const zc* dU, Mode mode, 
 	}
 	std::vector<PassArgs<T>> args;
 	for (const Pass& p : passes) { args.push_back(make_pass_args<T>(C, p, ts.K, ts.logC, dual)); }
+	// device gate table: G_k at [32 k], Z_k at [32 k + 16]
+	std::vector<cx<T>> hg(32 * (size_t)G);
+	for (int k = 0; k < G; k++) {
+		for (int e = 0; e < 16; e++) {
+			const zc g = C.gates[16 * (size_t)k + e];
+			const zc z = dual ? C.dirs[16 * (size_t)k + e] : g;
+			hg[32 * (size_t)k + e] = cx<T>{(T)g.x, (T)g.y};
+			hg[32 * (size_t)k + 16 + e] = cx<T>{(T)z.x, (T)z.y};
+		}
+	}
+	c.gtab.reserve(hg.size() * sizeof(cx<T>));
+	cx<T>* gtab = c.gtab.as<cx<T>>();
+	RQC_HIP(hipMemcpyAsync(gtab, hg.data(), hg.size() * sizeof(cx<T>), hipMemcpyHostToDevice, c.stream));
 	const unsigned gtr = reduce_grid(N * B);
 
 	for (long b0 = 0; b0 < N; b0 += B) {
@@ -489,14 +508,14 @@ static void sweep_passes(Context& c, const Circuit& C, const zc* dU, Mode mode, 
 			const int flags = (p == 0) ? 3 : 0;
 			LaunchTimer lt(c, RQC_KERNEL_FORWARD);
 			if (mode == Mode::Target) {
-				k_pass_forward<T, false><<<gfwd, kPassThreads, ts.state_bytes, c.stream>>>(
-					p == 0 ? nullptr : buf[(p + 1) % 2], buf[p % 2], nullptr, nullptr, args[p], B, b0, ntiles, flags);
+				k_pass_forward<T, false><<<gfwd, kPassThreads, ts.state_bytes + table, c.stream>>>(
+					p == 0 ? nullptr : buf[(p + 1) % 2], buf[p % 2], nullptr, nullptr, args[p], gtab, B, b0, ntiles, flags);
 			} else if (dual && p < P - 1) {
-				k_pass_forward<T, true><<<gfwd, kPassThreads, 2 * ts.state_bytes, c.stream>>>(
-					p == 0 ? nullptr : PS(p), PS(p + 1), p == 0 ? nullptr : DPS(p), DPS(p + 1), args[p], B, b0, ntiles, flags);
+				k_pass_forward<T, true><<<gfwd, kPassThreads, 2 * ts.state_bytes + table, c.stream>>>(
+					p == 0 ? nullptr : PS(p), PS(p + 1), p == 0 ? nullptr : DPS(p), DPS(p + 1), args[p], gtab, B, b0, ntiles, flags);
 			} else {
-				k_pass_forward<T, false><<<gfwd, kPassThreads, ts.state_bytes, c.stream>>>(
-					p == 0 ? nullptr : PS(p), PS(p + 1), nullptr, nullptr, args[p], B, b0, ntiles, flags);
+				k_pass_forward<T, false><<<gfwd, kPassThreads, ts.state_bytes + table, c.stream>>>(
+					p == 0 ? nullptr : PS(p), PS(p + 1), nullptr, nullptr, args[p], gtab, B, b0, ntiles, flags);
 			}
 		}
 		cx<T>* psiG = (mode == Mode::Target) ? buf[(P - 1) % 2] : PS(P);
@@ -517,17 +536,17 @@ static void sweep_passes(Context& c, const Circuit& C, const zc* dU, Mode mode, 
 			if (p == 0) { flags |= 1 | 2 | 8; }
 			if (p == P - 1) { flags |= 4; }
 			const int ng = passes[p].ng;
-			const size_t lds = (size_t)(dual ? 2 * ng + 2 : ng + 1) * ts.state_bytes;
+			const size_t lds = (size_t)(dual ? 2 * ng + 2 : ng + 1) * ts.state_bytes + table;
 			{
 				LaunchTimer lt(c, RQC_KERNEL_REVERSE);
 				if (dual) {
 					k_pass_reverse<T, true><<<grev, kPassThreads, lds, c.stream>>>(
 						p == 0 ? nullptr : PS(p), p == 0 ? nullptr : DPS(p), phi[cur], dphi[cur], phi[1 - cur], dphi[1 - cur],
-						args[p], B, b0, ntiles, flags, mpart);
+						args[p], gtab, B, b0, ntiles, flags, mpart);
 				} else {
 					k_pass_reverse<T, false><<<grev, kPassThreads, lds, c.stream>>>(
 						p == 0 ? nullptr : PS(p), nullptr, phi[cur], nullptr, phi[1 - cur], nullptr,
-						args[p], B, b0, ntiles, flags, mpart);
+						args[p], gtab, B, b0, ntiles, flags, mpart);
 				}
 			}
 			{

@@ -60,6 +60,7 @@ struct Context
 	DeviceBuffer small;    // partial sums, accumulators, gates, outputs
 	DeviceBuffer vecs;     // uploaded statevectors of the statevector-level API
 	DeviceBuffer ubuf;     // U assembled from a host callback
+	DeviceBuffer gtab;     // gate matrices of the current sweep (tile-pass engine)
 
 	// kernel timing
 	bool profile = false;

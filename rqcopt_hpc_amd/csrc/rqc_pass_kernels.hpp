@@ -49,9 +49,18 @@ struct PassArgs
 	int rb[16];        // row bit of tile bit t, t < K
 	int restb[24];     // row bit of rest bit u, u < L - K
 	int lp[kMaxPassGates], lq[kMaxPassGates];  // tile bits of wire i and wire j of each gate
-	Gate4<T> G[kMaxPassGates];
-	Gate4<T> Z[kMaxPassGates];
+	int g0;            // first gate of the pass in the device gate table
 };
+
+// gate table in LDS: G_j at [32 j], Z_j at [32 j + 16]; filled from the device
+// table (G_k at [32 k], Z_k at [32 k + 16]) at kernel start.  Gate entries are
+// read from LDS at the point of use (a broadcast read per entry) instead of
+// being held in scalar registers, which cannot hold 8 complex FP64 4x4 matrices.
+template <typename T>
+__device__ __forceinline__ void load_gate_table(const cx<T>* __restrict__ gtab, int g0, int ng, cx<T>* lgt)
+{
+	for (int i = threadIdx.x; i < 32 * ng; i += blockDim.x) { lgt[i] = gtab[32L * g0 + i]; }
+}
 
 // zero bits inserted at tile bit positions lo < hi
 __device__ __forceinline__ int tile_quad_row(int q, int lo, int hi)
@@ -98,7 +107,7 @@ __device__ __forceinline__ long rest_row_bits(const int* restb, int nrest, long 
 // apply gate j of the pass to one quad held in LDS (in place): x <- G x,
 // with DUAL also dx <- G dx + Z x (dx read before x is overwritten)
 template <typename T, bool DUAL>
-__device__ __forceinline__ void lds_gate_forward(cx<T>* S, cx<T>* D, const TileQuad& o, const Gate4<T>& G, const Gate4<T>& Z)
+__device__ __forceinline__ void lds_gate_forward(cx<T>* S, cx<T>* D, const TileQuad& o, const cx<T>* G, const cx<T>* Z)
 {
 	cx<T> x[4], y[4];
 #pragma unroll
@@ -107,7 +116,7 @@ __device__ __forceinline__ void lds_gate_forward(cx<T>* S, cx<T>* D, const TileQ
 	for (int r = 0; r < 4; r++) {
 		cx<T> s = {0, 0};
 #pragma unroll
-		for (int k = 0; k < 4; k++) { cmac(s, G.g[4*r + k], x[k]); }
+		for (int k = 0; k < 4; k++) { cmac(s, G[4*r + k], x[k]); }
 		y[r] = s;
 	}
 	if (DUAL) {
@@ -118,7 +127,7 @@ __device__ __forceinline__ void lds_gate_forward(cx<T>* S, cx<T>* D, const TileQ
 		for (int r = 0; r < 4; r++) {
 			cx<T> s = {0, 0};
 #pragma unroll
-			for (int k = 0; k < 4; k++) { cmac(s, G.g[4*r + k], dx[k]); cmac(s, Z.g[4*r + k], x[k]); }
+			for (int k = 0; k < 4; k++) { cmac(s, G[4*r + k], dx[k]); cmac(s, Z[4*r + k], x[k]); }
 			D[o.e[r]] = s;
 		}
 	}
@@ -129,7 +138,7 @@ __device__ __forceinline__ void lds_gate_forward(cx<T>* S, cx<T>* D, const TileQ
 // out-of-place variant (per-gate states of the reverse recompute)
 template <typename T, bool DUAL>
 __device__ __forceinline__ void lds_gate_forward_oop(const cx<T>* S0, const cx<T>* D0, cx<T>* S1, cx<T>* D1,
-	const TileQuad& o, const Gate4<T>& G, const Gate4<T>& Z)
+	const TileQuad& o, const cx<T>* G, const cx<T>* Z)
 {
 	cx<T> x[4];
 #pragma unroll
@@ -138,7 +147,7 @@ __device__ __forceinline__ void lds_gate_forward_oop(const cx<T>* S0, const cx<T
 	for (int r = 0; r < 4; r++) {
 		cx<T> s = {0, 0};
 #pragma unroll
-		for (int k = 0; k < 4; k++) { cmac(s, G.g[4*r + k], x[k]); }
+		for (int k = 0; k < 4; k++) { cmac(s, G[4*r + k], x[k]); }
 		S1[o.e[r]] = s;
 	}
 	if (DUAL) {
@@ -149,7 +158,7 @@ __device__ __forceinline__ void lds_gate_forward_oop(const cx<T>* S0, const cx<T
 		for (int r = 0; r < 4; r++) {
 			cx<T> s = {0, 0};
 #pragma unroll
-			for (int k = 0; k < 4; k++) { cmac(s, G.g[4*r + k], dx[k]); cmac(s, Z.g[4*r + k], x[k]); }
+			for (int k = 0; k < 4; k++) { cmac(s, G[4*r + k], dx[k]); cmac(s, Z[4*r + k], x[k]); }
 			D1[o.e[r]] = s;
 		}
 	}
@@ -161,13 +170,15 @@ __device__ __forceinline__ void lds_gate_forward_oop(const cx<T>* S0, const cx<T
 template <typename T, bool DUAL>
 __global__ __launch_bounds__(kPassThreads) void k_pass_forward(
 	const cx<T>* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __restrict__ din, cx<T>* __restrict__ dout,
-	const PassArgs<T> P, long B, long b0, long ntiles, int flags)
+	const PassArgs<T> P, const cx<T>* __restrict__ gtab, long B, long b0, long ntiles, int flags)
 {
 	extern __shared__ char lds_raw[];
 	const int E = 1 << (P.K + P.logC);
 	const int C = 1 << P.logC;
 	cx<T>* S = reinterpret_cast<cx<T>*>(lds_raw);
 	cx<T>* D = S + E;
+	cx<T>* lgt = S + (DUAL ? 2 : 1) * E;
+	load_gate_table(gtab, P.g0, P.ng, lgt);
 	const long nchunks = B >> P.logC;
 	const int tid = threadIdx.x;
 	for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
@@ -188,7 +199,7 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_forward(
 		for (int j = 0; j < P.ng; j++) {
 			for (int qd = tid; qd < E / 4; qd += kPassThreads) {
 				const TileQuad o = tile_quad(qd, P.logC, P.lp[j], P.lq[j]);
-				lds_gate_forward<T, DUAL>(S, D, o, P.G[j], P.Z[j]);
+				lds_gate_forward<T, DUAL>(S, D, o, lgt + 32 * j, lgt + 32 * j + 16);
 			}
 			__syncthreads();
 		}
@@ -222,7 +233,7 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_reverse(
 	const cx<T>* __restrict__ psi_in, const cx<T>* __restrict__ dpsi_in,
 	const cx<T>* __restrict__ phi_out, const cx<T>* __restrict__ dphi_out,
 	cx<T>* __restrict__ phi_in, cx<T>* __restrict__ dphi_in,
-	const PassArgs<T> P, long B, long b0, long ntiles, int flags, double* __restrict__ partial)
+	const PassArgs<T> P, const cx<T>* __restrict__ gtab, long B, long b0, long ntiles, int flags, double* __restrict__ partial)
 {
 	extern __shared__ char lds_raw[];
 	const int E = 1 << (P.K + P.logC);
@@ -234,14 +245,17 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_reverse(
 	auto Dj = [&](int j) { return base + (long)(ng + j) * E; };
 	cx<T>* F = base + (long)(DUAL ? 2 * ng : ng) * E;
 	cx<T>* DF = F + E;
+	cx<T>* lgt = base + (long)(DUAL ? 2 * ng + 2 : ng + 1) * E;
+	load_gate_table(gtab, P.g0, ng, lgt);
 	const long nchunks = B >> P.logC;
 	const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
 	const int qpw = (E / 4) / kPassWaves;          // quads owned by a wave
 	const int q0 = wave * qpw;
 
-	double4_t acc[kMaxPassGates];
+	// two accumulators per gate (even / odd matrix-core steps) halve the dependent chain
+	double4_t acc[kMaxPassGates][2];
 #pragma unroll
-	for (int j = 0; j < kMaxPassGates; j++) { acc[j] = double4_t{0, 0, 0, 0}; }
+	for (int j = 0; j < kMaxPassGates; j++) { acc[j][0] = double4_t{0, 0, 0, 0}; acc[j][1] = double4_t{0, 0, 0, 0}; }
 
 	for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 		const long chunk = tile % nchunks, rest = tile / nchunks;
@@ -263,7 +277,7 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_reverse(
 		for (int j = 0; j + 1 < ng; j++) {
 			for (int qd = tid; qd < E / 4; qd += kPassThreads) {
 				const TileQuad o = tile_quad(qd, P.logC, P.lp[j], P.lq[j]);
-				lds_gate_forward_oop<T, DUAL>(Sj(j), Dj(j), Sj(j + 1), Dj(j + 1), o, P.G[j], P.Z[j]);
+				lds_gate_forward_oop<T, DUAL>(Sj(j), Dj(j), Sj(j + 1), Dj(j + 1), o, lgt + 32 * j, lgt + 32 * j + 16);
 			}
 			__syncthreads();
 		}
@@ -274,11 +288,15 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_reverse(
 			if (j >= ng) { continue; }
 			const int p = P.lp[j], q = P.lq[j];
 			// matrix-core sums over the wave's quads, 4 quads per instruction
-			for (int it = 0; it < qpw / 4; it++) {
-				const TileQuad o = tile_quad(q0 + 4 * it + (lane >> 4), P.logC, p, q);
-				const double a = mfma_operand<T, DUAL>(F, DF, o, lane & 15);
-				const double b = mfma_operand<T, DUAL>(Sj(j), Dj(j), o, lane & 15);
-				acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+			for (int it = 0; it < qpw / 4; it += 2) {
+				const TileQuad o0 = tile_quad(q0 + 4 * it + (lane >> 4), P.logC, p, q);
+				const TileQuad o1 = tile_quad(q0 + 4 * it + 4 + (lane >> 4), P.logC, p, q);
+				const double a0 = mfma_operand<T, DUAL>(F, DF, o0, lane & 15);
+				const double b0v = mfma_operand<T, DUAL>(Sj(j), Dj(j), o0, lane & 15);
+				const double a1 = mfma_operand<T, DUAL>(F, DF, o1, lane & 15);
+				const double b1v = mfma_operand<T, DUAL>(Sj(j), Dj(j), o1, lane & 15);
+				acc[j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0v, acc[j][0], 0, 0, 0);
+				acc[j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1v, acc[j][1], 0, 0, 0);
 			}
 			// adjoint update of the same quads (vector ALUs)
 			if (j > 0 || !(flags & 8)) {
@@ -287,21 +305,21 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_reverse(
 					cx<T> f[4], y[4];
 #pragma unroll
 					for (int k = 0; k < 4; k++) { f[k] = F[o.e[k]]; y[k] = cx<T>{0, 0}; }
-					const Gate4<T>& G = P.G[j];
+					const cx<T>* G = lgt + 32 * j;
 #pragma unroll
 					for (int c = 0; c < 4; c++) {
 #pragma unroll
-						for (int r = 0; r < 4; r++) { cmac(y[c], G.g[4*r + c], f[r]); }
+						for (int r = 0; r < 4; r++) { cmac(y[c], G[4*r + c], f[r]); }
 					}
 					if (DUAL) {
-						const Gate4<T>& Z = P.Z[j];
+						const cx<T>* Z = lgt + 32 * j + 16;
 						cx<T> df[4], dy[4];
 #pragma unroll
 						for (int k = 0; k < 4; k++) { df[k] = DF[o.e[k]]; dy[k] = cx<T>{0, 0}; }
 #pragma unroll
 						for (int c = 0; c < 4; c++) {
 #pragma unroll
-							for (int r = 0; r < 4; r++) { cmac(dy[c], G.g[4*r + c], df[r]); cmac(dy[c], Z.g[4*r + c], f[r]); }
+							for (int r = 0; r < 4; r++) { cmac(dy[c], G[4*r + c], df[r]); cmac(dy[c], Z[4*r + c], f[r]); }
 						}
 #pragma unroll
 						for (int k = 0; k < 4; k++) { DF[o.e[k]] = dy[k]; }
@@ -328,7 +346,8 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_reverse(
 	for (int j = 0; j < kMaxPassGates; j++) {
 		if (j < ng) {
 			double* d = partial + ((part * ng + j) * 256) + lane * 4;
-			d[0] = acc[j][0]; d[1] = acc[j][1]; d[2] = acc[j][2]; d[3] = acc[j][3];
+			const double4_t a = acc[j][0] + acc[j][1];
+			d[0] = a[0]; d[1] = a[1]; d[2] = a[2]; d[3] = a[3];
 		}
 	}
 }
