@@ -426,6 +426,8 @@ static TileShape tile_shape(Context& c, int L, long B, Mode mode)
 	const int elem = (int)sizeof(cx<T>);
 	int logE = (elem == 16) ? 10 : 11;
 	if (const char* v = getenv(elem == 16 ? "RQC_TILE_LOGE_FP64" : "RQC_TILE_LOGE_FP32")) { logE = atoi(v); }
+	// a tile state holds at most kMaxEpt elements per thread and at least 16 quads per wave group
+	logE = std::max(8, std::min(logE, 11));
 	const int logCdef = (elem == 16) ? 3 : 4;
 	TileShape s;
 	s.K = std::min(L, logE - logCdef);

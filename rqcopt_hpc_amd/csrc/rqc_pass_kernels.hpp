@@ -104,6 +104,20 @@ __device__ __forceinline__ long rest_row_bits(const int* restb, int nrest, long 
 	return r;
 }
 
+// elements a thread loads/stores per tile state (E / threads, E <= 2048)
+constexpr int kMaxEpt = 8;
+
+// row-index contributions of the tile bits for the elements e = tid + i * threads
+// a thread moves between HBM and LDS; fixed for all tiles of a pass
+__device__ __forceinline__ void thread_tile_rows(const int* rb, int K, int logC, int E, long* trow)
+{
+#pragma unroll
+	for (int i = 0; i < kMaxEpt; i++) {
+		const int e = threadIdx.x + i * kPassThreads;
+		trow[i] = (e < E) ? tile_row_bits(rb, K, e >> logC) : 0;
+	}
+}
+
 // apply gate j of the pass to one quad held in LDS (in place): x <- G x,
 // with DUAL also dx <- G dx + Z x (dx read before x is overwritten)
 template <typename T, bool DUAL>
@@ -181,12 +195,17 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_forward(
 	load_gate_table(gtab, P.g0, P.ng, lgt);
 	const long nchunks = B >> P.logC;
 	const int tid = threadIdx.x;
+	long trow[kMaxEpt];
+	thread_tile_rows(P.rb, P.K, P.logC, E, trow);
 	for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 		const long chunk = tile % nchunks, rest = tile / nchunks;
 		const long rbase = rest_row_bits(P.restb, P.nrest, rest);
 		const long c0 = chunk << P.logC;
-		for (int e = tid; e < E; e += kPassThreads) {
-			const long row = rbase | tile_row_bits(P.rb, P.K, e >> P.logC);
+#pragma unroll
+		for (int i = 0; i < kMaxEpt; i++) {
+			const int e = tid + i * kPassThreads;
+			if (e >= E) { break; }
+			const long row = rbase | trow[i];
 			const long col = c0 + (e & (C - 1));
 			if (flags & 1) {
 				S[e] = cx<T>{(row == b0 + col) ? (T)1 : (T)0, (T)0};
@@ -203,8 +222,11 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_forward(
 			}
 			__syncthreads();
 		}
-		for (int e = tid; e < E; e += kPassThreads) {
-			const long row = rbase | tile_row_bits(P.rb, P.K, e >> P.logC);
+#pragma unroll
+		for (int i = 0; i < kMaxEpt; i++) {
+			const int e = tid + i * kPassThreads;
+			if (e >= E) { break; }
+			const long row = rbase | trow[i];
 			const long col = c0 + (e & (C - 1));
 			out[row * B + col] = S[e];
 			if (DUAL) { dout[row * B + col] = D[e]; }
@@ -257,12 +279,17 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_reverse(
 #pragma unroll
 	for (int j = 0; j < kMaxPassGates; j++) { acc[j][0] = double4_t{0, 0, 0, 0}; acc[j][1] = double4_t{0, 0, 0, 0}; }
 
+	long trow[kMaxEpt];
+	thread_tile_rows(P.rb, P.K, P.logC, E, trow);
 	for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
 		const long chunk = tile % nchunks, rest = tile / nchunks;
 		const long rbase = rest_row_bits(P.restb, P.nrest, rest);
 		const long c0 = chunk << P.logC;
-		for (int e = tid; e < E; e += kPassThreads) {
-			const long row = rbase | tile_row_bits(P.rb, P.K, e >> P.logC);
+#pragma unroll
+		for (int i = 0; i < kMaxEpt; i++) {
+			const int e = tid + i * kPassThreads;
+			if (e >= E) { break; }
+			const long row = rbase | trow[i];
 			const long col = c0 + (e & (C - 1));
 			const long g = row * B + col;
 			Sj(0)[e] = (flags & 1) ? cx<T>{(row == b0 + col) ? (T)1 : (T)0, (T)0} : psi_in[g];
@@ -331,8 +358,11 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_reverse(
 			__syncthreads();
 		}
 		if (!(flags & 8)) {
-			for (int e = tid; e < E; e += kPassThreads) {
-				const long row = rbase | tile_row_bits(P.rb, P.K, e >> P.logC);
+#pragma unroll
+			for (int i = 0; i < kMaxEpt; i++) {
+				const int e = tid + i * kPassThreads;
+				if (e >= E) { break; }
+				const long row = rbase | trow[i];
 				const long col = c0 + (e & (C - 1));
 				phi_in[row * B + col] = F[e];
 				if (DUAL) { dphi_in[row * B + col] = DF[e]; }
