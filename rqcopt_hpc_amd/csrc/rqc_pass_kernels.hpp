@@ -315,7 +315,9 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_reverse(
 			if (j >= ng) { continue; }
 			const int p = P.lp[j], q = P.lq[j];
 			// matrix-core sums over the wave's quads, 4 quads per instruction
-			for (int it = 0; it < qpw / 4; it += 2) {
+			const int ngroups = qpw / 4;   // 4-quad groups owned by this wave
+			int it = 0;
+			for (; it + 1 < ngroups; it += 2) {
 				const TileQuad o0 = tile_quad(q0 + 4 * it + (lane >> 4), P.logC, p, q);
 				const TileQuad o1 = tile_quad(q0 + 4 * it + 4 + (lane >> 4), P.logC, p, q);
 				const double a0 = mfma_operand<T, DUAL>(F, DF, o0, lane & 15);
@@ -324,6 +326,12 @@ __global__ __launch_bounds__(kPassThreads) void k_pass_reverse(
 				const double b1v = mfma_operand<T, DUAL>(Sj(j), Dj(j), o1, lane & 15);
 				acc[j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0v, acc[j][0], 0, 0, 0);
 				acc[j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1v, acc[j][1], 0, 0, 0);
+			}
+			if (it < ngroups) {
+				const TileQuad o0 = tile_quad(q0 + 4 * it + (lane >> 4), P.logC, p, q);
+				const double a0 = mfma_operand<T, DUAL>(F, DF, o0, lane & 15);
+				const double b0v = mfma_operand<T, DUAL>(Sj(j), Dj(j), o0, lane & 15);
+				acc[j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0v, acc[j][0], 0, 0, 0);
 			}
 			// adjoint update of the same quads (vector ALUs)
 			if (j > 0 || !(flags & 8)) {
