@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "grad+HVP evals/s, L=14 brickwall d9, FP64 vs mixed; served p99 latency"
 HBM_PEAK_GBS = 8000.0
+REVERSE_KERNEL = {
+    "tile_pass": "k_pass_reverse<double,dual> (reverse tile pass: up to 4 gates per HBM pass, gradient/HVP sums on matrix cores)",
+    "per_gate": "k_gate_reverse<double,dual> (per-gate reverse sweep with gradient/HVP sums)",
+}
 FP64_PEAK_TFLOPS = 78.6
 
 
@@ -38,13 +42,6 @@ def canonical_bytes(L, d, s):
 def canonical_flops(L, d):
     """~144 L d N^2: 36 complex MACs per amplitude per gate, L/2 gates per layer."""
     return 144.0 * L * d * (1 << L) ** 2
-
-
-def reverse_bytes_per_block(G, N, B, s):
-    """Algorithmic bytes of the per-gate reverse (grad+HVP) launches of one column block:
-    gates 1..G-2 read psi, dpsi, phi, dphi and write phi, dphi (6 arrays); the last gate has no
-    dphi to read (5); gate 0 reads psi, phi, dphi only (3)."""
-    return (6 * (G - 2) + 5 + 3) * N * B * s
 
 
 def _cpu_worker(args):
@@ -80,6 +77,8 @@ def run(args):
     from rqcopt_hpc_amd.workloads import config_inputs, ising_trotter_brickwall
 
     info = R.require_gpu()
+    engine = args.engine
+    R.set_sweep_engine(R.ENGINE_TILE_PASS if engine == "tile_pass" else R.ENGINE_PER_GATE)
     L, d = args.L, args.depth
     N = 1 << L
     G = d * (L // 2)
@@ -116,7 +115,9 @@ def run(args):
         prof = {k: R.profile_query(k) for k in range(4)}
         results[name] = {"eval_s": dt / args.steps, "out": out, "prof": prof, "block": R.last_column_block()}
         print(f"[bench] {name}: {dt / args.steps * 1e3:.2f} ms/eval, column block {R.last_column_block()}, "
-              f"kernel ms {[round(prof[k][0], 2) for k in range(4)]}", file=sys.stderr, flush=True)
+              f"kernel ms {[round(prof[k][0], 2) for k in range(4)]}, launches {[prof[k][1] for k in range(4)]}, "
+              f"GB/s {[round(prof[k][2] / max(prof[k][0], 1e-9) / 1e6) for k in range(4)]}",
+              file=sys.stderr, flush=True)
     R.set_precision(R.PRECISION_FP64)
     if args.precision != "both" or args.no_profile_pass:
         print(json.dumps({k: {"ms_per_eval": v["eval_s"] * 1e3, "column_block": v["block"]} for k, v in results.items()}))
@@ -125,9 +126,9 @@ def run(args):
     r64, r32 = results["fp64"], results["mixed"]
     s = 16
     B = r64["block"]
-    rev_ms, rev_n = r64["prof"][R.KERNEL_REVERSE]
-    nblk = N // B
-    rev_bytes = reverse_bytes_per_block(G, N, B, s) * nblk
+    # algorithmic bytes of every reverse launch (state arrays read + written once, summed by the
+    # library per launch) over the launches' HIP-event durations
+    rev_ms, rev_n, rev_bytes = r64["prof"][R.KERNEL_REVERSE]
     achieved = rev_bytes / (rev_ms * 1e-3) / 1e9
     f64, g64, h64 = r64["out"]
     f32, g32, h32 = r32["out"]
@@ -151,11 +152,12 @@ def run(args):
         "data": "synthetic (seeded PCG Haar gates, Ising-Trotter target formed on the GPU)",
         "copy_bandwidth_gbs": copy_gbs,
         "config": {"workload": f"brickwall grad+HVP eval, L={L}, depth {d}, periodic, Ising-Trotter U (t=0.5, 4 Strang steps)",
-                   "L": L, "depth": d, "gates": G, "column_block": B, "device": info},
+                   "L": L, "depth": d, "gates": G, "column_block": B, "sweep_engine": engine, "device": info},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_gate_reverse<double,dual> (per-gate reverse sweep with gradient/HVP sums)",
-                     "launches": rev_n, "avg_launch_ms": rev_ms / max(rev_n, 1)},
+                     "kernel": REVERSE_KERNEL[engine],
+                     "launches": rev_n, "avg_launch_ms": rev_ms / max(rev_n, 1),
+                     "alg_bytes_per_launch": rev_bytes / max(rev_n, 1)},
         "eval_canonical": {"bytes": canonical_bytes(L, d, s), "flops": canonical_flops(L, d),
                            "hbm_frac": canonical_bytes(L, d, s) / r64["eval_s"] / 1e9 / HBM_PEAK_GBS,
                            "fp64_frac": canonical_flops(L, d) / r64["eval_s"] / 1e12 / FP64_PEAK_TFLOPS},
@@ -181,6 +183,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--cpu-cols", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--engine", choices=["tile_pass", "per_gate"], default="tile_pass",
+                    help="sweep engine: several gates per HBM pass through LDS, or one launch per gate")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
                     help="profiling runs: time one precision only and print a short summary")
     ap.add_argument("--no-profile-pass", action="store_true", help="skip the HIP-event kernel timing pass")

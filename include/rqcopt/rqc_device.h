@@ -75,10 +75,12 @@ void rqc_release_workspace(void);
 long rqc_last_column_block(void);
 
 /* kernel timing with HIP events on the library's stream: when enabled, every
-   launch is bracketed by events and its duration is added per kind */
+   launch is bracketed by events and its duration is added per kind, together
+   with the launch's algorithmic HBM bytes (each state array it must read or
+   write counted once; any output pointer may be NULL) */
 void rqc_profile_enable(int on);
 void rqc_profile_reset(void);
-int rqc_profile_query(int kind, double* total_ms, long* launches);
+int rqc_profile_query(int kind, double* total_ms, long* launches, double* alg_bytes);
 
 /* device-to-device copy of `bytes` bytes by a plain streaming kernel (reads
    `bytes`, writes `bytes`), timed with HIP events over `reps` launches; gives the

@@ -135,7 +135,7 @@ def lib():
         L.truncated_cg_hvp.restype = ct.c_bool
         L.rqc_last_column_block.restype = ct.c_long
         L.rqc_set_workspace_limit.argtypes = [ct.c_size_t]
-        L.rqc_profile_query.argtypes = [ct.c_int, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long)]
+        L.rqc_profile_query.argtypes = [ct.c_int, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long), ct.POINTER(ct.c_double)]
         _LIB = L
     return _LIB
 
@@ -666,9 +666,10 @@ def profile_reset():
 
 
 def profile_query(kind):
-    ms, n = ct.c_double(), ct.c_long()
-    _check("rqc_profile_query", lib().rqc_profile_query(kind, ct.byref(ms), ct.byref(n)))
-    return ms.value, n.value
+    """(total ms, launches, algorithmic HBM bytes) of the timed launches of one kernel kind."""
+    ms, n, b = ct.c_double(), ct.c_long(), ct.c_double()
+    _check("rqc_profile_query", lib().rqc_profile_query(kind, ct.byref(ms), ct.byref(n), ct.byref(b)))
+    return ms.value, n.value, b.value
 
 
 def measure_copy_bandwidth(nbytes=1 << 30, reps=10):

@@ -83,7 +83,7 @@ Context& ctx()
 	return *c;
 }
 
-LaunchTimer::LaunchTimer(Context& cx, int kind) : c(cx)
+LaunchTimer::LaunchTimer(Context& cx, int kind, double bytes) : c(cx)
 {
 	if (!c.profile) { return; }
 	if (c.events_used == c.events.size()) {
@@ -94,6 +94,7 @@ LaunchTimer::LaunchTimer(Context& cx, int kind) : c(cx)
 	}
 	ev = &c.events[c.events_used++];
 	ev->kind = kind;
+	ev->bytes = bytes;
 	RQC_HIP(hipEventRecord(ev->a, c.stream));
 }
 
@@ -111,6 +112,7 @@ void profile_flush(Context& c)
 		RQC_HIP(hipEventElapsedTime(&ms, c.events[k].a, c.events[k].b));
 		c.prof_ms[c.events[k].kind] += ms;
 		c.prof_n[c.events[k].kind] += 1;
+		c.prof_bytes[c.events[k].kind] += c.events[k].bytes;
 	}
 	c.events_used = 0;
 }
@@ -285,10 +287,12 @@ static void sweep_targets(Context& c, const Circuit& C, const zc* dU, Mode mode,
 	const long nq = (N / 4) * B;
 	const unsigned grev = reduce_grid(nq);
 	const unsigned gtr = reduce_grid(N * B);
+	const double ab = (double)sz * sizeof(cx<T>);   // bytes of one state block
+	const bool hvp = (mode == Mode::Hvp);
 
 	for (long b0 = 0; b0 < N; b0 += B) {
 		{
-			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			LaunchTimer lt(c, RQC_KERNEL_OTHER, ab);
 			k_identity_block<T><<<nblocks(N * B), kBlock, 0, c.stream>>>(psi(0), B, logB, b0, N * B);
 		}
 		// forward sweep
@@ -296,7 +300,7 @@ static void sweep_targets(Context& c, const Circuit& C, const zc* dU, Mode mode,
 			const Wires w = wires_of(C.wi[k], C.wj[k], L);
 			cx<T>* in = (mode == Mode::Target) ? psi(k % 2) : psi(k);
 			cx<T>* out = (mode == Mode::Target) ? psi((k + 1) % 2) : psi(k + 1);
-			LaunchTimer lt(c, RQC_KERNEL_FORWARD);
+			LaunchTimer lt(c, RQC_KERNEL_FORWARD, hvp ? (3 + (k > 0)) * ab : 2 * ab);
 			if (mode == Mode::Hvp) {
 				k_gate_forward<T, true><<<nblocks(nq), kBlock, 0, c.stream>>>(in, out, k > 0 ? dpsi(k) : nullptr, dpsi(k + 1),
 					Gt[k], Zt[k], w, B, logB, nq, k == 0 ? 1 : 0);
@@ -307,7 +311,7 @@ static void sweep_targets(Context& c, const Circuit& C, const zc* dU, Mode mode,
 		}
 		cx<T>* psiG = (mode == Mode::Target) ? psi(G % 2) : psi(G);
 		{
-			LaunchTimer lt(c, RQC_KERNEL_TARGET);
+			LaunchTimer lt(c, RQC_KERNEL_TARGET, (double)sz * sizeof(zc) + (mode == Mode::Target ? 1 : 2) * ab);
 			k_target_trace<T><<<gtr, kBlock, 0, c.stream>>>(dU, N, b0, psiG, mode == Mode::Target ? nullptr : phi[0],
 				B, logB, N * B, d_partial);
 		}
@@ -322,7 +326,8 @@ static void sweep_targets(Context& c, const Circuit& C, const zc* dU, Mode mode,
 			const Wires w = wires_of(C.wi[k], C.wj[k], L);
 			int flags = (k == 0) ? 4 : 0;
 			{
-				LaunchTimer lt(c, RQC_KERNEL_REVERSE);
+				LaunchTimer lt(c, RQC_KERNEL_REVERSE, hvp ? (2 + (k > 0) + (k < G - 1) + (k > 0 ? 2 : 0)) * ab
+				                                          : (2 + (k > 0)) * ab);
 				if (mode == Mode::Hvp) {
 					if (k == 0) { flags |= 1; }
 					if (k == G - 1) { flags |= 2; }
@@ -503,12 +508,14 @@ static void sweep_passes(Context& c, const Circuit& C, const zc* dU, Mode mode, 
 	cx<T>* gtab = c.gtab.as<cx<T>>();
 	RQC_HIP(hipMemcpyAsync(gtab, hg.data(), hg.size() * sizeof(cx<T>), hipMemcpyHostToDevice, c.stream));
 	const unsigned gtr = reduce_grid(N * B);
+	const double ab = (double)sz * sizeof(cx<T>);   // bytes of one state block
 
 	for (long b0 = 0; b0 < N; b0 += B) {
 		// forward passes
 		for (int p = 0; p < P; p++) {
 			const int flags = (p == 0) ? 3 : 0;
-			LaunchTimer lt(c, RQC_KERNEL_FORWARD);
+			const bool dpass = dual && p < P - 1;
+			LaunchTimer lt(c, RQC_KERNEL_FORWARD, (dpass ? 2 * (p > 0) + 2 : (p > 0) + 1) * ab);
 			if (mode == Mode::Target) {
 				k_pass_forward<T, false><<<gfwd, kPassThreads, ts.state_bytes + table, c.stream>>>(
 					p == 0 ? nullptr : buf[(p + 1) % 2], buf[p % 2], nullptr, nullptr, args[p], gtab, B, b0, ntiles, flags);
@@ -522,7 +529,7 @@ static void sweep_passes(Context& c, const Circuit& C, const zc* dU, Mode mode, 
 		}
 		cx<T>* psiG = (mode == Mode::Target) ? buf[(P - 1) % 2] : PS(P);
 		{
-			LaunchTimer lt(c, RQC_KERNEL_TARGET);
+			LaunchTimer lt(c, RQC_KERNEL_TARGET, (double)sz * sizeof(zc) + (mode == Mode::Target ? 1 : 2) * ab);
 			k_target_trace<T><<<gtr, kBlock, 0, c.stream>>>(dU, N, b0, psiG, mode == Mode::Target ? nullptr : phi[0],
 				B, logB, N * B, tpart);
 		}
@@ -540,7 +547,8 @@ static void sweep_passes(Context& c, const Circuit& C, const zc* dU, Mode mode, 
 			const int ng = passes[p].ng;
 			const size_t lds = (size_t)(dual ? 2 * ng + 2 : ng + 1) * ts.state_bytes + table;
 			{
-				LaunchTimer lt(c, RQC_KERNEL_REVERSE);
+				LaunchTimer lt(c, RQC_KERNEL_REVERSE, dual ? (2 * (p > 0) + 1 + (p < P - 1) + 2 * (p > 0)) * ab
+				                                           : (2 * (p > 0) + 1) * ab);
 				if (dual) {
 					k_pass_reverse<T, true><<<grev, kPassThreads, lds, c.stream>>>(
 						p == 0 ? nullptr : PS(p), p == 0 ? nullptr : DPS(p), phi[cur], dphi[cur], phi[1 - cur], dphi[1 - cur],
@@ -1478,19 +1486,20 @@ extern "C" void rqc_profile_reset(void)
 		Context& c = ctx();
 		std::lock_guard<std::recursive_mutex> lk(c.mu);
 		profile_flush(c);
-		for (int k = 0; k < RQC_KERNEL_KINDS; k++) { c.prof_ms[k] = 0; c.prof_n[k] = 0; }
+		for (int k = 0; k < RQC_KERNEL_KINDS; k++) { c.prof_ms[k] = 0; c.prof_n[k] = 0; c.prof_bytes[k] = 0; }
 	} catch (const std::exception& e) { fatal("rqc_profile_reset", e); }
 }
 
-extern "C" int rqc_profile_query(int kind, double* total_ms, long* launches)
+extern "C" int rqc_profile_query(int kind, double* total_ms, long* launches, double* alg_bytes)
 {
 	if (kind < 0 || kind >= RQC_KERNEL_KINDS) { return -3; }
 	return target_call("rqc_profile_query", [&] {
 		Context& c = ctx();
 		std::lock_guard<std::recursive_mutex> lk(c.mu);
 		profile_flush(c);
-		*total_ms = c.prof_ms[kind];
-		*launches = c.prof_n[kind];
+		if (total_ms != nullptr) { *total_ms = c.prof_ms[kind]; }
+		if (launches != nullptr) { *launches = c.prof_n[kind]; }
+		if (alg_bytes != nullptr) { *alg_bytes = c.prof_bytes[kind]; }
 	});
 }
 

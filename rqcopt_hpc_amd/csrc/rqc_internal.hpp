@@ -64,11 +64,12 @@ struct Context
 
 	// kernel timing
 	bool profile = false;
-	struct Ev { hipEvent_t a, b; int kind; };
+	struct Ev { hipEvent_t a, b; int kind; double bytes; };
 	std::vector<Ev> events;
 	size_t events_used = 0;
 	double prof_ms[RQC_KERNEL_KINDS] = {0, 0, 0, 0};
 	long prof_n[RQC_KERNEL_KINDS] = {0, 0, 0, 0};
+	double prof_bytes[RQC_KERNEL_KINDS] = {0, 0, 0, 0};   // algorithmic HBM bytes of the timed launches
 
 	std::recursive_mutex mu;
 };
@@ -76,12 +77,13 @@ struct Context
 // the process-wide context on the current HIP device; throws if no GPU is present
 Context& ctx();
 
-// bracket a kernel launch with timing events when profiling is enabled
+// bracket a kernel launch with timing events when profiling is enabled; `bytes` is the
+// launch's algorithmic HBM traffic (arrays it must read and write once), summed per kind
 struct LaunchTimer
 {
 	Context& c;
 	Context::Ev* ev = nullptr;
-	LaunchTimer(Context& cx, int kind);
+	LaunchTimer(Context& cx, int kind, double bytes = 0);
 	~LaunchTimer();
 };
 
