@@ -73,6 +73,11 @@ int rqc_set_workspace_limit(size_t bytes);
 void rqc_release_workspace(void);
 /* column-block width used by the last target evaluation */
 long rqc_last_column_block(void);
+/* sweep engine (RQC_ENGINE_*) of the last target evaluation and its number of
+   HBM passes per sweep direction (the gate count for the per-gate engine);
+   the tile-pass engine hands gradient / HVP sweeps whose gates are not unitary
+   to 1e-6, and circuits too small for one tile, to the per-gate engine */
+int rqc_last_sweep(int* engine, int* passes);
 
 /* kernel timing with HIP events on the library's stream: when enabled, every
    launch is bracketed by events and its duration is added per kind, together

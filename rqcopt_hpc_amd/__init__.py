@@ -113,7 +113,7 @@ EXPORTED = [
     "rqc_target_destroy", "rqc_target_nqubits", "rqc_target_download", "rqc_set_precision", "rqc_get_precision",
     "rqc_set_workspace_limit", "rqc_release_workspace", "rqc_last_column_block", "rqc_profile_enable",
     "rqc_profile_reset", "rqc_profile_query", "rqc_synchronize", "rqc_device_info", "rqc_measure_copy_bandwidth",
-    "rqc_set_sweep_engine",
+    "rqc_set_sweep_engine", "rqc_last_sweep",
 ]
 
 
@@ -639,6 +639,14 @@ def set_precision(mode):
 
 def set_sweep_engine(engine):
     _check("rqc_set_sweep_engine", lib().rqc_set_sweep_engine(engine))
+
+
+def last_sweep():
+    """(engine, passes) of the last target evaluation: ENGINE_TILE_PASS or ENGINE_PER_GATE and its
+    number of HBM passes per sweep direction."""
+    e, p = ct.c_int(), ct.c_int()
+    _check("rqc_last_sweep", lib().rqc_last_sweep(ct.byref(e), ct.byref(p)))
+    return e.value, p.value
 
 
 def get_precision():

@@ -22,7 +22,7 @@
 
 #include "rqc_internal.hpp"
 #include "rqc_kernels.hpp"
-#include "rqc_pass_kernels.hpp"
+#include "rqc_tile_kernels.hpp"
 #include "mat4.hpp"
 #include "rqcopt/gate.h"
 #include "rqcopt/statevector.h"
@@ -265,6 +265,8 @@ static void sweep_targets(Context& c, const Circuit& C, const zc* dU, Mode mode,
 	const int nst = states_needed(mode, G);
 	const long B = choose_block(c, N, nst, sizeof(cx<T>));
 	c.last_block = B;
+	c.last_engine = RQC_ENGINE_PER_GATE;
+	c.last_passes = G;
 	const int logB = ilog2_or_neg(B);
 	const size_t sz = (size_t)N * B;
 	c.states.reserve((size_t)nst * sz * sizeof(cx<T>));
@@ -350,63 +352,104 @@ static void sweep_targets(Context& c, const Circuit& C, const zc* dU, Mode mode,
 // ---------------------------------------------------------------- tile passes
 struct Pass
 {
-	int g0 = 0, ng = 0;
-	std::vector<int> rb;  // tile row bits, ascending
+	std::vector<int> gates;  // circuit gate indices, in execution order
+	std::vector<int> rb;     // tile row bits, ascending
 };
 
-// greedy split of the gate sequence into runs whose wires fit K tile qubits,
-// at most gmax gates each; tiles are padded with the lowest unused row bits
+// Greedy pass planning over the circuit's dependency order: gates on disjoint
+// wires commute, so a pass may take any gate whose predecessors on its two
+// wires are done, as long as its wires lie in the pass's K tile qubits.
+// Candidate tile sets: every window of K consecutive wires (cyclic) and the
+// wires of the ready gates in circuit order; the candidate that takes the
+// most gates (at most gmax) wins, ties to the earlier candidate.  Tiles are
+// padded with the lowest unused row bits.  Any topological order of the gates
+// gives the same states at each gate up to rounding, so the sweeps of the
+// passes compute the reference's per-gate quantities.
 static std::vector<Pass> plan_passes(const Circuit& C, int K, int gmax)
 {
-	const int L = C.L;
+	const int L = C.L, G = C.ngates;
+	auto bitmask = [&](int k) { return (1u << (L - 1 - C.wi[k])) | (1u << (L - 1 - C.wj[k])); };
+	std::vector<int> prev_i(G, -1), prev_j(G, -1), last(L, -1);
+	for (int k = 0; k < G; k++) {
+		prev_i[k] = last[C.wi[k]];
+		prev_j[k] = last[C.wj[k]];
+		last[C.wi[k]] = k;
+		last[C.wj[k]] = k;
+	}
+	std::vector<char> done(G, 0), taken(G, 0);
+	int ndone = 0;
+	auto ok = [&](int k) { return (prev_i[k] < 0 || done[prev_i[k]] || taken[prev_i[k]]) &&
+	                              (prev_j[k] < 0 || done[prev_j[k]] || taken[prev_j[k]]); };
+	auto take = [&](unsigned mask, std::vector<int>* out) {
+		std::fill(taken.begin(), taken.end(), 0);
+		int n = 0;
+		for (int k = 0; k < G && n < gmax; k++) {
+			if (done[k] || (bitmask(k) & ~mask) != 0 || !ok(k)) { continue; }
+			taken[k] = 1;
+			n++;
+			if (out != nullptr) { out->push_back(k); }
+		}
+		return n;
+	};
 	std::vector<Pass> out;
-	int k = 0;
-	while (k < C.ngates) {
-		Pass p;
-		p.g0 = k;
-		std::vector<int> Q;
-		while (k < C.ngates && p.ng < gmax) {
-			std::vector<int> Q2 = Q;
-			for (int b : {L - 1 - C.wi[k], L - 1 - C.wj[k]}) {
-				if (std::find(Q2.begin(), Q2.end(), b) == Q2.end()) { Q2.push_back(b); }
+	while (ndone < G) {
+		std::vector<unsigned> cands;
+		if (K >= L) {
+			cands.push_back((L >= 32) ? ~0u : ((1u << L) - 1));
+		} else {
+			for (int s = 0; s < L; s++) {
+				unsigned m = 0;
+				for (int t = 0; t < K; t++) { m |= 1u << (L - 1 - (s + t) % L); }
+				cands.push_back(m);
 			}
-			if ((int)Q2.size() > K) { break; }
-			Q = Q2;
-			p.ng++;
-			k++;
+			std::fill(taken.begin(), taken.end(), 0);
+			unsigned m = 0;
+			for (int k = 0; k < G; k++) {
+				if (done[k] || !ok(k)) { continue; }
+				const unsigned m2 = m | bitmask(k);
+				if (__builtin_popcount(m2) <= K) { m = m2; }
+			}
+			cands.push_back(m);
 		}
-		for (int b = 0; b < L && (int)Q.size() < K; b++) {
-			if (std::find(Q.begin(), Q.end(), b) == Q.end()) { Q.push_back(b); }
+		int best = -1, bestn = 0;
+		for (size_t ci = 0; ci < cands.size(); ci++) {
+			const int n = take(cands[ci], nullptr);
+			if (n > bestn) { bestn = n; best = (int)ci; }
 		}
-		std::sort(Q.begin(), Q.end());
-		p.rb = Q;
+		if (best < 0) { throw Error(-4, "pass planning made no progress"); }
+		Pass p;
+		take(cands[best], &p.gates);
+		unsigned used = 0;
+		for (int k : p.gates) { done[k] = 1; ndone++; used |= bitmask(k); }
+		std::fill(taken.begin(), taken.end(), 0);
+		for (int b = 0; b < L; b++) {
+			if (used & (1u << b)) { p.rb.push_back(b); }
+		}
+		for (int b = 0; b < L && (int)p.rb.size() < K; b++) {
+			if (!(used & (1u << b))) { p.rb.push_back(b); }
+		}
+		std::sort(p.rb.begin(), p.rb.end());
 		out.push_back(p);
 	}
 	return out;
 }
 
-template <typename T>
-static PassArgs<T> make_pass_args(const Circuit& C, const Pass& p, int K, int logC, bool dual)
+static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC, int x0)
 {
-	PassArgs<T> a;
+	TileArgs a;
 	std::memset(&a, 0, sizeof(a));
 	a.K = K;
 	a.logC = logC;
-	a.nrest = C.L - K;
-	a.ng = p.ng;
-	a.g0 = p.g0;
+	a.ng = (int)p.gates.size();
+	a.x0 = x0;
 	std::vector<int> tile_of(C.L, -1);
-	for (int t = 0; t < K; t++) { a.rb[t] = p.rb[t]; tile_of[p.rb[t]] = t; }
-	int u = 0;
-	for (int b = 0; b < C.L; b++) {
-		if (tile_of[b] < 0) { a.restb[u++] = b; }
-	}
-	for (int j = 0; j < p.ng; j++) {
-		const int k = p.g0 + j;
+	for (int t = 0; t < K; t++) { tile_of[p.rb[t]] = t; a.tmask |= 1u << p.rb[t]; }
+	a.rmask = (unsigned)(((1UL << C.L) - 1) & ~(unsigned long)a.tmask);
+	for (int j = 0; j < a.ng; j++) {
+		const int k = p.gates[j];
 		a.lp[j] = tile_of[C.L - 1 - C.wi[k]];
 		a.lq[j] = tile_of[C.L - 1 - C.wj[k]];
 	}
-	(void)dual;
 	return a;
 }
 
@@ -418,114 +461,193 @@ static void allow_lds(KernelT* fn, size_t bytes)
 	}
 }
 
-// tile shape for element type T: 16 KiB per LDS state, 128-byte tile rows
+// real 16 x 16 matrix of the complex block matrix [[b00, b01], [b10, b11]]
+// (4 x 4 blocks, nullptr = 0) in the component order m = 4 v + r, v = 2 s + p,
+// laid out as the matrix-core A operands: out[kk * 64 + l] = M[l % 16][4 kk + l / 16]
+template <typename T>
+static void a_operands(const zc* b00, const zc* b01, const zc* b10, const zc* b11, T* out)
+{
+	const zc* blk[2][2] = {{b00, b01}, {b10, b11}};
+	double M[16][16];
+	for (int so = 0; so < 2; so++) for (int si = 0; si < 2; si++)
+	for (int po = 0; po < 2; po++) for (int pi = 0; pi < 2; pi++)
+	for (int ro = 0; ro < 4; ro++) for (int ri = 0; ri < 4; ri++) {
+		double v = 0;
+		if (blk[so][si] != nullptr) {
+			const zc z = blk[so][si][4 * ro + ri];
+			v = (po == pi) ? z.x : (po == 0 ? -z.y : z.y);
+		}
+		M[4 * (2 * so + po) + ro][4 * (2 * si + pi) + ri] = v;
+	}
+	for (int kk = 0; kk < 4; kk++) {
+		for (int l = 0; l < 64; l++) { out[kk * 64 + l] = (T)M[l % 16][4 * kk + l / 16]; }
+	}
+}
+
+// gates whose inverse is their adjoint up to 1e-6 (the reverse tile pass
+// recovers earlier states with G^-1, which then has norm 1)
+static bool gates_near_unitary(const Circuit& C)
+{
+	for (int k = 0; k < C.ngates; k++) {
+		const zc* g = &C.gates[16 * (size_t)k];
+		zc gh[16], p[16];
+		m4_adjoint(g, gh);
+		m4_mul(gh, g, p);
+		for (int e = 0; e < 16; e++) {
+			const zc d = p[e] - zc{(e % 5 == 0) ? 1.0 : 0.0, 0.0};
+			if (std::fabs(d.x) > 1e-6 || std::fabs(d.y) > 1e-6) { return false; }
+		}
+	}
+	return true;
+}
+
+// tile shape for element type T: 2^K rows x 2^logC columns, 128-byte rows
 struct TileShape
 {
-	int K, logC, gmax;
-	size_t state_bytes;
+	int K, logC;
+	bool usable() const { return K + logC >= 9; }   // >= 512 elements: 4 waves x 32 quads
 };
 
 template <typename T>
-static TileShape tile_shape(Context& c, int L, long B, Mode mode)
+static TileShape tile_shape(int L, long B)
 {
 	const int elem = (int)sizeof(cx<T>);
-	int logE = (elem == 16) ? 10 : 11;
+	int logE = 10;
 	if (const char* v = getenv(elem == 16 ? "RQC_TILE_LOGE_FP64" : "RQC_TILE_LOGE_FP32")) { logE = atoi(v); }
-	// a tile state holds at most kMaxEpt elements per thread and at least 16 quads per wave group
-	logE = std::max(8, std::min(logE, 11));
+	logE = std::max(9, std::min(logE, 11));
 	const int logCdef = (elem == 16) ? 3 : 4;
-	TileShape s;
-	s.K = std::min(L, logE - logCdef);
 	int logB = 0;
 	while ((1L << (logB + 1)) <= B) { logB++; }
-	s.logC = std::min(logE - s.K, logB);
-	s.state_bytes = (size_t)elem << (s.K + s.logC);
-	const size_t table = 32 * kMaxPassGates * (size_t)elem;   // gate table in LDS
-	const long nslots = (long)((c.lds_per_block - table) / s.state_bytes);
-	if (mode == Mode::Hvp) { s.gmax = (int)std::min<long>(kMaxPassGates, (nslots - 2) / 2); }
-	else { s.gmax = (int)std::min<long>(kMaxPassGates, nslots - 1); }
-	if (const char* v = getenv("RQC_PASS_GMAX")) { s.gmax = std::max(1, std::min(s.gmax, atoi(v))); }
-	if (s.gmax < 1) { throw Error(-1, "LDS per workgroup too small for one tile pass"); }
+	TileShape s;
+	s.K = std::min(L, logE - logCdef);
+	s.logC = std::min(logB, logE - s.K);
 	return s;
 }
 
+static int tile_gmax()
+{
+	int g = kMaxTileGates;
+	if (const char* v = getenv("RQC_PASS_GMAX")) { g = std::max(1, std::min(g, atoi(v))); }
+	return g;
+}
+
+// true if the tile-pass engine handles this sweep (else the per-gate engine does)
 template <typename T>
-static void sweep_passes(Context& c, const Circuit& C, const zc* dU, Mode mode, zc* d_acc, zc* d_partial)
+static bool tiles_apply(const Context& c, const Circuit& C, Mode mode)
+{
+	if (c.engine == RQC_ENGINE_PER_GATE || C.L > 30) { return false; }
+	if (!tile_shape<T>(C.L, 1L << C.L).usable()) { return false; }
+	return mode == Mode::Target || gates_near_unitary(C);
+}
+
+template <typename T>
+static void sweep_tiles(Context& c, const Circuit& C, const zc* dU, Mode mode, zc* d_acc, zc* d_partial, size_t partial_doubles)
 {
 	const int L = C.L, G = C.ngates;
 	const long N = 1L << L;
 	const bool dual = (mode == Mode::Hvp);
-	// the pass plan does not depend on B: plan with the widest tile shape first
-	TileShape ts = tile_shape<T>(c, L, N, mode);
-	std::vector<Pass> passes = plan_passes(C, ts.K, ts.gmax);
+	TileShape ts = tile_shape<T>(L, N);
+	const std::vector<Pass> passes = plan_passes(C, ts.K, tile_gmax());
 	const int P = (int)passes.size();
-	const int nst = (mode == Mode::Target) ? 2 : (mode == Mode::Grad) ? P + 2 : 2 * P + 3;
+	const int nst = (mode == Mode::Target) ? 2 : (mode == Mode::Grad) ? P + 2 : 2 * P + 4;
 	const long B = choose_block(c, N, nst, sizeof(cx<T>));
+	ts = tile_shape<T>(L, B);
+	if (!ts.usable()) {
+		sweep_targets<T>(c, C, dU, mode, d_acc, d_partial);
+		return;
+	}
 	c.last_block = B;
-	ts = tile_shape<T>(c, L, B, mode);
-	const int logB = ilog2_or_neg(B);
+	c.last_engine = RQC_ENGINE_TILE_PASS;
+	c.last_passes = P;
 	const size_t sz = (size_t)N * B;
 	c.states.reserve((size_t)nst * sz * sizeof(cx<T>));
 	cx<T>* S = c.states.as<cx<T>>();
-	auto PS = [&](int p) { return S + (size_t)(p - 1) * sz; };        // psi after pass p-1, p = 1 .. P
-	auto DPS = [&](int p) { return S + (size_t)(P + p - 1) * sz; };   // dpsi after pass p-1, p = 1 .. P-1
+	// checkpoints after pass p - 1, p = 1 .. P: psi at PS(p), dpsi at DPS(p)
+	auto PS = [&](int p) { return S + (size_t)(p - 1) * sz; };
+	auto DPS = [&](int p) { return S + (size_t)(P + p - 1) * sz; };
 	cx<T>* buf[2] = {S, S + sz};
-	cx<T>* phi[2] = {S + (size_t)(dual ? 2 * P - 1 : P) * sz, S + (size_t)(dual ? 2 * P : P + 1) * sz};
-	cx<T>* dphi[2] = {S + (size_t)(2 * P + 1) * sz, S + (size_t)(2 * P + 2) * sz};
+	const size_t nck = dual ? 2 * (size_t)P : (mode == Mode::Grad ? (size_t)P : 0);
+	cx<T>* phi[2] = {S + nck * sz, S + (nck + 1) * sz};
+	cx<T>* dphi[2] = {S + (nck + 2) * sz, S + (nck + 3) * sz};
 
+	// operand tables of the gates in execution order and the pass arguments
+	std::vector<T> ha((size_t)G * 3 * 256);
+	std::vector<TileArgs> args;
+	std::vector<TileGates> gks;
+	int x = 0;
+	for (const Pass& p : passes) {
+		args.push_back(make_tile_args(C, p, ts.K, ts.logC, x));
+		TileGates tg;
+		std::memset(&tg, 0, sizeof(tg));
+		for (size_t j = 0; j < p.gates.size(); j++) {
+			const int k = p.gates[j];
+			tg.k[j] = k;
+			const zc* g = &C.gates[16 * (size_t)k];
+			T* o = &ha[(size_t)x * 3 * 256];
+			if (mode == Mode::Target) {
+				a_operands<T>(g, nullptr, nullptr, g, o);
+			} else {
+				zc gi[16], gt[16];
+				if (m4_inverse(g, gi) != 0) { throw Error(-4, "singular gate in the tile-pass sweep"); }
+				m4_transpose(g, gt);
+				if (dual) {
+					const zc* z = &C.dirs[16 * (size_t)k];
+					zc t1[16], w[16], zt[16];
+					m4_mul(gi, z, t1);
+					m4_mul(t1, gi, w);
+					for (int e = 0; e < 16; e++) { w[e] = zc{-w[e].x, -w[e].y}; }
+					m4_transpose(z, zt);
+					a_operands<T>(g, nullptr, z, g, o);
+					a_operands<T>(gi, nullptr, w, gi, o + 256);
+					a_operands<T>(gt, nullptr, zt, gt, o + 512);
+				} else {
+					a_operands<T>(g, nullptr, nullptr, g, o);
+					a_operands<T>(gi, nullptr, nullptr, gi, o + 256);
+					a_operands<T>(gt, nullptr, nullptr, gt, o + 512);
+				}
+			}
+			x++;
+		}
+		gks.push_back(tg);
+	}
+	c.gtab.reserve(ha.size() * sizeof(T));
+	T* amat = c.gtab.as<T>();
+	RQC_HIP(hipMemcpyAsync(amat, ha.data(), ha.size() * sizeof(T), hipMemcpyHostToDevice, c.stream));
+
+	const int E = 1 << (ts.K + ts.logC);
 	const long ntiles = (N >> ts.K) * (B >> ts.logC);
-	const long fit_fwd = std::max<long>(1, std::min<long>(8, (long)(c.lds_per_block / ((dual ? 2 : 1) * ts.state_bytes + 32 * kMaxPassGates * sizeof(cx<T>)))));
-	const long fit_rev = std::max<long>(1, std::min<long>(2, (long)(c.lds_per_block / ((size_t)(dual ? 2 * ts.gmax + 2 : ts.gmax + 1) * ts.state_bytes + 32 * kMaxPassGates * sizeof(cx<T>)))));
-	const unsigned gfwd = (unsigned)std::min<long>(ntiles, fit_fwd * c.cu_count);
-	const unsigned grev = (unsigned)std::min<long>(ntiles, fit_rev * c.cu_count);
-	const size_t table = 32 * kMaxPassGates * sizeof(cx<T>);
-	const size_t lds_fwd = (dual ? 2 : 1) * ts.state_bytes + table;
-	const size_t lds_rev_max = (size_t)(dual ? 2 * ts.gmax + 2 : ts.gmax + 1) * ts.state_bytes + table;
-	allow_lds(k_pass_forward<T, true>, lds_fwd);
-	allow_lds(k_pass_forward<T, false>, lds_fwd);
-	allow_lds(k_pass_reverse<T, true>, lds_rev_max);
-	allow_lds(k_pass_reverse<T, false>, lds_rev_max);
-	double* mpart = reinterpret_cast<double*>(d_partial);     // grev * waves * gmax * 256 doubles
-	zc* tpart = d_partial;                                      // trace partials (reused, sequential)
+	const size_t lds_fwd = (size_t)(dual ? 2 : 1) * E * sizeof(cx<T>);
+	const size_t lds_rev = (size_t)(dual ? 4 : 2) * E * sizeof(cx<T>);
+	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
+	auto* kr = dual ? k_tile_reverse<T, true> : k_tile_reverse<T, false>;
+	allow_lds(kf, lds_fwd);
+	allow_lds(kr, lds_rev);
+	int occ_f = 0, occ_r = 0;
+	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, reinterpret_cast<const void*>(kf), kTileThreads, lds_fwd));
+	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, reinterpret_cast<const void*>(kr), kTileThreads, lds_rev));
+	const unsigned gfwd = (unsigned)std::min<long>(ntiles, (long)std::max(1, occ_f) * c.cu_count);
+	const long rev_cap = (long)(partial_doubles / ((size_t)kTileWaves * kMaxTileGates * 256));
+	const unsigned grev = (unsigned)std::max<long>(1, std::min<long>({ntiles, (long)std::max(1, occ_r) * c.cu_count, rev_cap}));
+	double* mpart = reinterpret_cast<double*>(d_partial);   // grev * waves * ng * 256 doubles
+	zc* tpart = d_partial;                                  // trace partials (reused, sequential)
 
 	{
 		LaunchTimer lt(c, RQC_KERNEL_OTHER);
 		k_fill_zero<<<nblocks(1 + 32L * G), kBlock, 0, c.stream>>>(d_acc, 1 + 32L * G);
 	}
-	std::vector<PassArgs<T>> args;
-	for (const Pass& p : passes) { args.push_back(make_pass_args<T>(C, p, ts.K, ts.logC, dual)); }
-	// device gate table: G_k at [32 k], Z_k at [32 k + 16]
-	std::vector<cx<T>> hg(32 * (size_t)G);
-	for (int k = 0; k < G; k++) {
-		for (int e = 0; e < 16; e++) {
-			const zc g = C.gates[16 * (size_t)k + e];
-			const zc z = dual ? C.dirs[16 * (size_t)k + e] : g;
-			hg[32 * (size_t)k + e] = cx<T>{(T)g.x, (T)g.y};
-			hg[32 * (size_t)k + 16 + e] = cx<T>{(T)z.x, (T)z.y};
-		}
-	}
-	c.gtab.reserve(hg.size() * sizeof(cx<T>));
-	cx<T>* gtab = c.gtab.as<cx<T>>();
-	RQC_HIP(hipMemcpyAsync(gtab, hg.data(), hg.size() * sizeof(cx<T>), hipMemcpyHostToDevice, c.stream));
 	const unsigned gtr = reduce_grid(N * B);
+	const int logB = ilog2_or_neg(B);
 	const double ab = (double)sz * sizeof(cx<T>);   // bytes of one state block
+	const int ns = dual ? 2 : 1;                     // state arrays per side
 
 	for (long b0 = 0; b0 < N; b0 += B) {
-		// forward passes
 		for (int p = 0; p < P; p++) {
 			const int flags = (p == 0) ? 3 : 0;
-			const bool dpass = dual && p < P - 1;
-			LaunchTimer lt(c, RQC_KERNEL_FORWARD, (dpass ? 2 * (p > 0) + 2 : (p > 0) + 1) * ab);
-			if (mode == Mode::Target) {
-				k_pass_forward<T, false><<<gfwd, kPassThreads, ts.state_bytes + table, c.stream>>>(
-					p == 0 ? nullptr : buf[(p + 1) % 2], buf[p % 2], nullptr, nullptr, args[p], gtab, B, b0, ntiles, flags);
-			} else if (dual && p < P - 1) {
-				k_pass_forward<T, true><<<gfwd, kPassThreads, 2 * ts.state_bytes + table, c.stream>>>(
-					p == 0 ? nullptr : PS(p), PS(p + 1), p == 0 ? nullptr : DPS(p), DPS(p + 1), args[p], gtab, B, b0, ntiles, flags);
-			} else {
-				k_pass_forward<T, false><<<gfwd, kPassThreads, ts.state_bytes + table, c.stream>>>(
-					p == 0 ? nullptr : PS(p), PS(p + 1), nullptr, nullptr, args[p], gtab, B, b0, ntiles, flags);
-			}
+			const cx<T>* in = (p == 0) ? nullptr : (mode == Mode::Target ? buf[(p + 1) % 2] : PS(p));
+			cx<T>* out = (mode == Mode::Target) ? buf[p % 2] : PS(p + 1);
+			LaunchTimer lt(c, RQC_KERNEL_FORWARD, (double)(ns * (p > 0) + ns) * ab);
+			kf<<<gfwd, kTileThreads, lds_fwd, c.stream>>>(in, out, (dual && p > 0) ? DPS(p) : nullptr,
+				dual ? DPS(p + 1) : nullptr, args[p], amat, B, b0, ntiles, flags);
 		}
 		cx<T>* psiG = (mode == Mode::Target) ? buf[(P - 1) % 2] : PS(P);
 		{
@@ -538,30 +660,22 @@ static void sweep_passes(Context& c, const Circuit& C, const zc* dU, Mode mode, 
 			k_reduce_partials<<<1, kBlock, 0, c.stream>>>(tpart, (int)gtr, 1, d_acc);
 		}
 		if (mode == Mode::Target) { continue; }
-		// reverse passes
 		int cur = 0;
 		for (int p = P - 1; p >= 0; p--) {
 			int flags = 0;
-			if (p == 0) { flags |= 1 | 2 | 8; }
 			if (p == P - 1) { flags |= 4; }
-			const int ng = passes[p].ng;
-			const size_t lds = (size_t)(dual ? 2 * ng + 2 : ng + 1) * ts.state_bytes + table;
+			if (p == 0) { flags |= 8; }
+			const int ng = (int)passes[p].gates.size();
 			{
-				LaunchTimer lt(c, RQC_KERNEL_REVERSE, dual ? (2 * (p > 0) + 1 + (p < P - 1) + 2 * (p > 0)) * ab
-				                                           : (2 * (p > 0) + 1) * ab);
-				if (dual) {
-					k_pass_reverse<T, true><<<grev, kPassThreads, lds, c.stream>>>(
-						p == 0 ? nullptr : PS(p), p == 0 ? nullptr : DPS(p), phi[cur], dphi[cur], phi[1 - cur], dphi[1 - cur],
-						args[p], gtab, B, b0, ntiles, flags, mpart);
-				} else {
-					k_pass_reverse<T, false><<<grev, kPassThreads, lds, c.stream>>>(
-						p == 0 ? nullptr : PS(p), nullptr, phi[cur], nullptr, phi[1 - cur], nullptr,
-						args[p], gtab, B, b0, ntiles, flags, mpart);
-				}
+				const int reads = dual ? 3 + (p < P - 1) : 2;
+				LaunchTimer lt(c, RQC_KERNEL_REVERSE, (double)(reads + ns * (p > 0)) * ab);
+				kr<<<grev, kTileThreads, lds_rev, c.stream>>>(PS(p + 1), dual ? DPS(p + 1) : nullptr, phi[cur],
+					dual ? dphi[cur] : nullptr, phi[1 - cur], dual ? dphi[1 - cur] : nullptr, args[p], amat, B, ntiles,
+					flags, mpart);
 			}
 			{
 				LaunchTimer lt(c, RQC_KERNEL_OTHER);
-				k_pass_reduce<<<ng, 256, 0, c.stream>>>(mpart, (long)grev * kPassWaves, ng, passes[p].g0, d_acc);
+				k_tile_reduce<<<ng, 256, 0, c.stream>>>(mpart, (long)grev * kTileWaves, ng, gks[p], dual ? 1 : 0, d_acc);
 			}
 			cur = 1 - cur;
 		}
@@ -609,9 +723,9 @@ static TargetResult eval_targets(const Circuit& C, linear_func ufunc, void* udat
 	// small buffer: acc | partial | vlist | vdirs | outputs
 	const size_t n_acc = 1 + 32 * (size_t)G;
 	// partial sums: per-gate reverse (4096 x 32 complex) or tile-pass matrix-core partials
-	// (2 x CUs workgroups x waves x gates x 256 doubles), whichever is larger
+	// (up to 4 x CUs workgroups x waves x gates x 256 doubles), whichever is larger
 	const size_t n_part = std::max<size_t>(4096 * (size_t)kWidth,
-		(size_t)2 * c.cu_count * kPassWaves * kMaxPassGates * 256 / 2 + 4096);
+		(size_t)4 * c.cu_count * kTileWaves * kMaxTileGates * 256 / 2);
 	const size_t n_in = 2 * 16 * (size_t)ng;
 	const size_t n_out = 2 * 16 * (size_t)ng;
 	const size_t n_real = 2 * 16 * (size_t)ng;
@@ -621,13 +735,12 @@ static TargetResult eval_targets(const Circuit& C, linear_func ufunc, void* udat
 	zc* vin = part + n_part;
 	zc* out = vin + n_in;
 	double* rout = reinterpret_cast<double*>(out + n_out);
-	// tiles need at least 16 quads (one matrix-core group per wave): L >= 3
-	if (c.engine == RQC_ENGINE_PER_GATE || C.L < 3) {
-		if (c.precision == RQC_PRECISION_MIXED) { sweep_targets<float>(c, C, dU, mode, acc, part); }
-		else { sweep_targets<double>(c, C, dU, mode, acc, part); }
+	if (c.precision == RQC_PRECISION_MIXED) {
+		if (tiles_apply<float>(c, C, mode)) { sweep_tiles<float>(c, C, dU, mode, acc, part, 2 * n_part); }
+		else { sweep_targets<float>(c, C, dU, mode, acc, part); }
 	} else {
-		if (c.precision == RQC_PRECISION_MIXED) { sweep_passes<float>(c, C, dU, mode, acc, part); }
-		else { sweep_passes<double>(c, C, dU, mode, acc, part); }
+		if (tiles_apply<double>(c, C, mode)) { sweep_tiles<double>(c, C, dU, mode, acc, part, 2 * n_part); }
+		else { sweep_targets<double>(c, C, dU, mode, acc, part); }
 	}
 	// gates of each group (the first gate of the group) and directions
 	std::vector<zc> hv(n_in);
@@ -1468,6 +1581,16 @@ extern "C" void rqc_release_workspace(void)
 extern "C" long rqc_last_column_block(void)
 {
 	try { return ctx().last_block; } catch (...) { return -1; }
+}
+
+extern "C" int rqc_last_sweep(int* engine, int* passes)
+{
+	return target_call("rqc_last_sweep", [&] {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		if (engine != nullptr) { *engine = c.last_engine; }
+		if (passes != nullptr) { *passes = c.last_passes; }
+	});
 }
 
 extern "C" void rqc_profile_enable(int on)

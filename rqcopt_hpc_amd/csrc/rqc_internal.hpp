@@ -60,7 +60,9 @@ struct Context
 	DeviceBuffer small;    // partial sums, accumulators, gates, outputs
 	DeviceBuffer vecs;     // uploaded statevectors of the statevector-level API
 	DeviceBuffer ubuf;     // U assembled from a host callback
-	DeviceBuffer gtab;     // gate matrices of the current sweep (tile-pass engine)
+	DeviceBuffer gtab;     // gate operand tables of the current sweep (tile-pass engine)
+	int last_engine = -1;  // sweep engine of the last target evaluation
+	int last_passes = 0;   // HBM passes per sweep direction of the last target evaluation
 
 	// kernel timing
 	bool profile = false;

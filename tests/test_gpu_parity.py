@@ -380,24 +380,40 @@ def test_rtr_optimiser_vs_oracle():
     assert f_iter[-1] < f_iter[0]
 
 
+@pytest.mark.parametrize("unitary", [True, False])
 @pytest.mark.parametrize("L", [2, 3, 5, 7, 8, 11])
-def test_sweep_engines_agree(L):
-    """Tile-pass engine (several gates per HBM pass) vs per-gate engine vs oracle, random wires."""
+def test_sweep_engines_agree(L, unitary):
+    """Tile-pass engine (several gates per HBM pass) vs per-gate engine vs oracle, random wires.
+    Unitary gates run the tile-pass reverse sweep (uncompute with G^-1) from L = 5 on; general
+    gates are handed to the per-gate engine for gradient and HVP."""
     rng = np.random.default_rng(200 + L)
     ng = 3 * L
-    gates, dirs = crandn(rng, (ng, 4, 4)), crandn(rng, (ng, 4, 4))
+    if unitary:
+        gates = np.stack([haar_unitary(rng) for _ in range(ng)])
+    else:
+        gates = crandn(rng, (ng, 4, 4))
+    dirs = crandn(rng, (ng, 4, 4))
     wires = random_wires(rng, L, ng)
     U = crandn(rng, (1 << L, 1 << L))
     T = R.DeviceTarget.dense(U)
-    res = {}
+    res, used = {}, {}
     for eng in (R.ENGINE_TILE_PASS, R.ENGINE_PER_GATE):
         R.set_sweep_engine(eng)
         try:
-            res[eng] = (R.circuit_unitary_target(T, gates, wires, L),
-                        R.circuit_unitary_target_and_gradient(T, gates, wires, L),
-                        R.circuit_unitary_target_hessian_vector_product(T, gates, dirs, wires, L))
+            r0 = R.circuit_unitary_target(T, gates, wires, L)
+            u0 = R.last_sweep()
+            r1 = R.circuit_unitary_target_and_gradient(T, gates, wires, L)
+            u1 = R.last_sweep()
+            r2 = R.circuit_unitary_target_hessian_vector_product(T, gates, dirs, wires, L)
+            u2 = R.last_sweep()
+            res[eng], used[eng] = (r0, r1, r2), (u0, u1, u2)
         finally:
             R.set_sweep_engine(R.ENGINE_TILE_PASS)
+    tile_ok = L >= 5
+    assert used[R.ENGINE_TILE_PASS][0][0] == (R.ENGINE_TILE_PASS if tile_ok else R.ENGINE_PER_GATE)
+    for u in used[R.ENGINE_TILE_PASS][1:]:
+        assert u[0] == (R.ENGINE_TILE_PASS if tile_ok and unitary else R.ENGINE_PER_GATE)
+    assert all(u[0] == R.ENGINE_PER_GATE for u in used[R.ENGINE_PER_GATE])
     a, b = res[R.ENGINE_TILE_PASS], res[R.ENGINE_PER_GATE]
     assert reldiff(a[0], b[0]) < TOL
     assert reldiff(a[1][1], b[1][1]) < TOL
