@@ -434,7 +434,16 @@ static std::vector<Pass> plan_passes(const Circuit& C, int K, int gmax)
 	return out;
 }
 
-static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC, int x0)
+// Kernel arguments of a pass.  Tile-bit order: the two least used bits of the
+// pass go to the top tile positions (wave ownership of quads changes, and a
+// barrier is needed, only between consecutive gates whose two highest non-gate
+// tile bits differ) and the third least used to position 0 (the two rows of a
+// 256-byte LDS bank window differ in tile bit 0, so gates away from it read
+// their matrix-core groups without bank conflicts).  Column swizzle: a greedy
+// colouring of the pass's gate graph (tile bits as vertices) with three colours
+// h in {1, 2, 3} (FP64: column bits 0-1; FP32: bits 1-2) gives the four
+// amplitude rows of every gate distinct bank slots in the outer-product reads.
+static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC, int x0, int elem_bytes)
 {
 	TileArgs a;
 	std::memset(&a, 0, sizeof(a));
@@ -442,13 +451,60 @@ static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC,
 	a.logC = logC;
 	a.ng = (int)p.gates.size();
 	a.x0 = x0;
+	// usage of the tile's row bits by the pass's gates
+	std::vector<int> use(C.L, 0);
+	for (int k : p.gates) { use[C.L - 1 - C.wi[k]]++; use[C.L - 1 - C.wj[k]]++; }
+	std::vector<int> bits = p.rb;
+	std::stable_sort(bits.begin(), bits.end(), [&](int x, int y) { return use[x] < use[y]; });
+	std::vector<int> order(K);
+	int lo = 0, hi = K - 1;
+	for (int i = 0; i < K; i++) {
+		// least used -> K-1, second -> K-2, third -> 0, then upwards from 1
+		if (i < 2 && K >= 3) { order[hi--] = bits[i]; }
+		else { order[lo++] = bits[i]; }
+	}
 	std::vector<int> tile_of(C.L, -1);
-	for (int t = 0; t < K; t++) { tile_of[p.rb[t]] = t; a.tmask |= 1u << p.rb[t]; }
-	a.rmask = (unsigned)(((1UL << C.L) - 1) & ~(unsigned long)a.tmask);
+	unsigned tmask = 0;
+	for (int t = 0; t < K; t++) { a.rb[t] = order[t]; tile_of[order[t]] = t; tmask |= 1u << order[t]; }
+	a.rmask = (unsigned)(((1UL << C.L) - 1) & ~(unsigned long)tmask);
 	for (int j = 0; j < a.ng; j++) {
 		const int k = p.gates[j];
 		a.lp[j] = tile_of[C.L - 1 - C.wi[k]];
 		a.lq[j] = tile_of[C.L - 1 - C.wj[k]];
+	}
+	// three-colouring of the gate graph, highest degree first
+	std::vector<std::vector<int>> nb(K);
+	for (int j = 0; j < a.ng; j++) { nb[a.lp[j]].push_back(a.lq[j]); nb[a.lq[j]].push_back(a.lp[j]); }
+	std::vector<int> vs(K), col(K, 0);
+	for (int t = 0; t < K; t++) { vs[t] = t; }
+	std::stable_sort(vs.begin(), vs.end(), [&](int x, int y) { return nb[x].size() > nb[y].size(); });
+	for (int t : vs) {
+		int c = 1;
+		for (; c <= 3; c++) {
+			bool used = false;
+			for (int u : nb[t]) { used |= (col[u] == c); }
+			if (!used) { break; }
+		}
+		col[t] = (c <= 3) ? c : 1;
+	}
+	const int shift = (elem_bytes == 16) ? 0 : 1;
+	for (int t = 0; t < K; t++) {
+		const int h = (col[t] << shift) & ((1 << logC) - 1);
+		for (int i = 0; i < 3; i++) {
+			if ((h >> i) & 1) { a.swz[i] |= 1u << t; }
+		}
+	}
+	// barriers between gates whose quads are owned differently
+	auto top2 = [&](int j) {
+		int b1 = -1, b2 = -1;
+		for (int t = K - 1; t >= 0 && b2 < 0; t--) {
+			if (t == a.lp[j] || t == a.lq[j]) { continue; }
+			if (b1 < 0) { b1 = t; } else { b2 = t; }
+		}
+		return std::make_pair(b1, b2);
+	};
+	for (int j = 1; j < a.ng; j++) {
+		if (top2(j) != top2(j - 1)) { a.barrier |= 1u << j; }
 	}
 	return a;
 }
@@ -505,7 +561,9 @@ static bool gates_near_unitary(const Circuit& C)
 struct TileShape
 {
 	int K, logC;
-	bool usable() const { return K + logC >= 9; }   // >= 512 elements: 4 waves x 32 quads
+	// >= 1024 elements: 4 waves x kGroupStep matrix-core groups of 32 quads (plain
+	// sweeps); >= 4 columns: each wave owns a quarter of them
+	bool usable() const { return K + logC >= 10 && logC >= 2; }
 };
 
 template <typename T>
@@ -514,7 +572,7 @@ static TileShape tile_shape(int L, long B)
 	const int elem = (int)sizeof(cx<T>);
 	int logE = 10;
 	if (const char* v = getenv(elem == 16 ? "RQC_TILE_LOGE_FP64" : "RQC_TILE_LOGE_FP32")) { logE = atoi(v); }
-	logE = std::max(9, std::min(logE, 11));
+	logE = std::max(10, std::min(logE, 11));
 	const int logCdef = (elem == 16) ? 3 : 4;
 	int logB = 0;
 	while ((1L << (logB + 1)) <= B) { logB++; }
@@ -576,7 +634,7 @@ static void sweep_tiles(Context& c, const Circuit& C, const zc* dU, Mode mode, z
 	std::vector<TileGates> gks;
 	int x = 0;
 	for (const Pass& p : passes) {
-		args.push_back(make_tile_args(C, p, ts.K, ts.logC, x));
+		args.push_back(make_tile_args(C, p, ts.K, ts.logC, x, (int)sizeof(cx<T>)));
 		TileGates tg;
 		std::memset(&tg, 0, sizeof(tg));
 		for (size_t j = 0; j < p.gates.size(); j++) {
@@ -616,8 +674,8 @@ static void sweep_tiles(Context& c, const Circuit& C, const zc* dU, Mode mode, z
 
 	const int E = 1 << (ts.K + ts.logC);
 	const long ntiles = (N >> ts.K) * (B >> ts.logC);
-	const size_t lds_fwd = (size_t)(dual ? 2 : 1) * E * sizeof(cx<T>);
-	const size_t lds_rev = (size_t)(dual ? 4 : 2) * E * sizeof(cx<T>);
+	const size_t lds_fwd = (size_t)(dual ? 2 : 1) * (E * sizeof(cx<T>) + kLdsPadBytes);
+	const size_t lds_rev = (size_t)(dual ? 4 : 2) * (E * sizeof(cx<T>) + kLdsPadBytes);
 	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
 	auto* kr = dual ? k_tile_reverse<T, true> : k_tile_reverse<T, false>;
 	allow_lds(kf, lds_fwd);
