@@ -443,14 +443,17 @@ static std::vector<Pass> plan_passes(const Circuit& C, int K, int gmax)
 // colouring of the pass's gate graph (tile bits as vertices) with three colours
 // h in {1, 2, 3} (FP64: column bits 0-1; FP32: bits 1-2) gives the four
 // amplitude rows of every gate distinct bank slots in the outer-product reads.
-static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC, int x0, int elem_bytes)
+static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC, int lognc, int x0, int elem_bytes,
+	GateTerms* gterms)
 {
 	TileArgs a;
 	std::memset(&a, 0, sizeof(a));
 	a.K = K;
 	a.logC = logC;
+	a.lognc = lognc;
 	a.ng = (int)p.gates.size();
 	a.x0 = x0;
+	int lp[kMaxTileGates], lq[kMaxTileGates];
 	// usage of the tile's row bits by the pass's gates
 	std::vector<int> use(C.L, 0);
 	for (int k : p.gates) { use[C.L - 1 - C.wi[k]]++; use[C.L - 1 - C.wj[k]]++; }
@@ -469,12 +472,12 @@ static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC,
 	a.rmask = (unsigned)(((1UL << C.L) - 1) & ~(unsigned long)tmask);
 	for (int j = 0; j < a.ng; j++) {
 		const int k = p.gates[j];
-		a.lp[j] = tile_of[C.L - 1 - C.wi[k]];
-		a.lq[j] = tile_of[C.L - 1 - C.wj[k]];
+		lp[j] = tile_of[C.L - 1 - C.wi[k]];
+		lq[j] = tile_of[C.L - 1 - C.wj[k]];
 	}
 	// three-colouring of the gate graph, highest degree first
 	std::vector<std::vector<int>> nb(K);
-	for (int j = 0; j < a.ng; j++) { nb[a.lp[j]].push_back(a.lq[j]); nb[a.lq[j]].push_back(a.lp[j]); }
+	for (int j = 0; j < a.ng; j++) { nb[lp[j]].push_back(lq[j]); nb[lq[j]].push_back(lp[j]); }
 	std::vector<int> vs(K), col(K, 0);
 	for (int t = 0; t < K; t++) { vs[t] = t; }
 	std::stable_sort(vs.begin(), vs.end(), [&](int x, int y) { return nb[x].size() > nb[y].size(); });
@@ -498,13 +501,25 @@ static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC,
 	auto top2 = [&](int j) {
 		int b1 = -1, b2 = -1;
 		for (int t = K - 1; t >= 0 && b2 < 0; t--) {
-			if (t == a.lp[j] || t == a.lq[j]) { continue; }
+			if (t == lp[j] || t == lq[j]) { continue; }
 			if (b1 < 0) { b1 = t; } else { b2 = t; }
 		}
 		return std::make_pair(b1, b2);
 	};
 	for (int j = 1; j < a.ng; j++) {
 		if (top2(j) != top2(j - 1)) { a.barrier |= 1u << j; }
+	}
+	// uniform LDS position terms of each gate (after the swizzle is fixed)
+	const int qpw = (1 << (K + logC)) / 4 / kTileWaves;
+	for (int j = 0; j < a.ng; j++) {
+		GateTerms& g = gterms[x0 + j];
+		for (int i = 0; i < 4; i++) {
+			g.u[i] = quad_pos(0, i, lp[j], lq[j], a);
+			g.w[i] = quad_pos(4 * i, 0, lp[j], lq[j], a);
+			g.q4[i] = quad_pos(i, 0, lp[j], lq[j], a);
+		}
+		for (int w = 0; w < kTileWaves; w++) { g.wv[w] = quad_pos(w * qpw, 0, lp[j], lq[j], a); }
+		for (int i = 0; i < 3; i++) { g.gbit[i] = (16 << i < qpw) ? quad_pos(16 << i, 0, lp[j], lq[j], a) : 0; }
 	}
 	return a;
 }
@@ -514,29 +529,6 @@ static void allow_lds(KernelT* fn, size_t bytes)
 {
 	if (bytes > 65536) {
 		RQC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-	}
-}
-
-// real 16 x 16 matrix of the complex block matrix [[b00, b01], [b10, b11]]
-// (4 x 4 blocks, nullptr = 0) in the component order m = 4 v + r, v = 2 s + p,
-// laid out as the matrix-core A operands: out[kk * 64 + l] = M[l % 16][4 kk + l / 16]
-template <typename T>
-static void a_operands(const zc* b00, const zc* b01, const zc* b10, const zc* b11, T* out)
-{
-	const zc* blk[2][2] = {{b00, b01}, {b10, b11}};
-	double M[16][16];
-	for (int so = 0; so < 2; so++) for (int si = 0; si < 2; si++)
-	for (int po = 0; po < 2; po++) for (int pi = 0; pi < 2; pi++)
-	for (int ro = 0; ro < 4; ro++) for (int ri = 0; ri < 4; ri++) {
-		double v = 0;
-		if (blk[so][si] != nullptr) {
-			const zc z = blk[so][si][4 * ro + ri];
-			v = (po == pi) ? z.x : (po == 0 ? -z.y : z.y);
-		}
-		M[4 * (2 * so + po) + ro][4 * (2 * si + pi) + ri] = v;
-	}
-	for (int kk = 0; kk < 4; kk++) {
-		for (int l = 0; l < 64; l++) { out[kk * 64 + l] = (T)M[l % 16][4 * kk + l / 16]; }
 	}
 }
 
@@ -561,18 +553,20 @@ static bool gates_near_unitary(const Circuit& C)
 struct TileShape
 {
 	int K, logC;
-	// >= 1024 elements: 4 waves x kGroupStep matrix-core groups of 32 quads (plain
-	// sweeps); >= 4 columns: each wave owns a quarter of them
-	bool usable() const { return K + logC >= 10 && logC >= 2; }
+	// >= 512 elements: 4 waves x one matrix-core group of 32 quads (plain sweeps);
+	// >= 2 columns: the two FP32 elements of a 16-byte slot are adjacent columns
+	bool usable() const { return K + logC >= 9 && logC >= 1; }
 };
 
 template <typename T>
 static TileShape tile_shape(int L, long B)
 {
 	const int elem = (int)sizeof(cx<T>);
-	int logE = 10;
+	// FP64 512 and FP32 1024 elements (8 KiB per state): the reverse pass double-
+	// buffers four states, 64 KiB per workgroup, two workgroups per CU
+	int logE = (elem == 16) ? 9 : 10;
 	if (const char* v = getenv(elem == 16 ? "RQC_TILE_LOGE_FP64" : "RQC_TILE_LOGE_FP32")) { logE = atoi(v); }
-	logE = std::max(10, std::min(logE, 11));
+	logE = std::max(9, std::min(logE, elem == 16 ? 10 : 11));   // double-buffered reverse states fit 160 KiB
 	const int logCdef = (elem == 16) ? 3 : 4;
 	int logB = 0;
 	while ((1L << (logB + 1)) <= B) { logB++; }
@@ -629,53 +623,59 @@ static void sweep_tiles(Context& c, const Circuit& C, const zc* dU, Mode mode, z
 	cx<T>* dphi[2] = {S + (nck + 2) * sz, S + (nck + 3) * sz};
 
 	// operand tables of the gates in execution order and the pass arguments
-	std::vector<T> ha((size_t)G * 3 * 256);
+	std::vector<cx<T>> hm((size_t)G * kMats * 16);
+	std::vector<GateTerms> hterms((size_t)G);
 	std::vector<TileArgs> args;
 	std::vector<TileGates> gks;
 	int x = 0;
 	for (const Pass& p : passes) {
-		args.push_back(make_tile_args(C, p, ts.K, ts.logC, x, (int)sizeof(cx<T>)));
+		args.push_back(make_tile_args(C, p, ts.K, ts.logC, ilog2_or_neg(B >> ts.logC), x, (int)sizeof(cx<T>), hterms.data()));
 		TileGates tg;
 		std::memset(&tg, 0, sizeof(tg));
 		for (size_t j = 0; j < p.gates.size(); j++) {
 			const int k = p.gates[j];
 			tg.k[j] = k;
+			// G, Z, G^-1, -G^-1 Z G^-1, G^T, Z^T (kMats matrices, zero where unused)
 			const zc* g = &C.gates[16 * (size_t)k];
-			T* o = &ha[(size_t)x * 3 * 256];
-			if (mode == Mode::Target) {
-				a_operands<T>(g, nullptr, nullptr, g, o);
-			} else {
-				zc gi[16], gt[16];
-				if (m4_inverse(g, gi) != 0) { throw Error(-4, "singular gate in the tile-pass sweep"); }
-				m4_transpose(g, gt);
+			zc m6[kMats][16];
+			std::memset(m6, 0, sizeof(m6));
+			std::memcpy(m6[kMatG], g, sizeof(m6[0]));
+			if (dual) { std::memcpy(m6[kMatZ], &C.dirs[16 * (size_t)k], sizeof(m6[0])); }
+			if (mode != Mode::Target) {
+				if (m4_inverse(g, m6[kMatGinv]) != 0) { throw Error(-4, "singular gate in the tile-pass sweep"); }
+				m4_transpose(g, m6[kMatGT]);
 				if (dual) {
-					const zc* z = &C.dirs[16 * (size_t)k];
-					zc t1[16], w[16], zt[16];
-					m4_mul(gi, z, t1);
-					m4_mul(t1, gi, w);
-					for (int e = 0; e < 16; e++) { w[e] = zc{-w[e].x, -w[e].y}; }
-					m4_transpose(z, zt);
-					a_operands<T>(g, nullptr, z, g, o);
-					a_operands<T>(gi, nullptr, w, gi, o + 256);
-					a_operands<T>(gt, nullptr, zt, gt, o + 512);
-				} else {
-					a_operands<T>(g, nullptr, nullptr, g, o);
-					a_operands<T>(gi, nullptr, nullptr, gi, o + 256);
-					a_operands<T>(gt, nullptr, nullptr, gt, o + 512);
+					zc t1[16];
+					m4_mul(m6[kMatGinv], m6[kMatZ], t1);
+					m4_mul(t1, m6[kMatGinv], m6[kMatW]);
+					for (int e = 0; e < 16; e++) { m6[kMatW][e] = zc{-m6[kMatW][e].x, -m6[kMatW][e].y}; }
+					m4_transpose(m6[kMatZ], m6[kMatZT]);
+				}
+			}
+			for (int w = 0; w < kMats; w++) {
+				for (int e = 0; e < 16; e++) {
+					hm[((size_t)x * kMats + w) * 16 + e] = cx<T>{(T)m6[w][e].x, (T)m6[w][e].y};
 				}
 			}
 			x++;
 		}
 		gks.push_back(tg);
 	}
-	c.gtab.reserve(ha.size() * sizeof(T));
-	T* amat = c.gtab.as<T>();
-	RQC_HIP(hipMemcpyAsync(amat, ha.data(), ha.size() * sizeof(T), hipMemcpyHostToDevice, c.stream));
+	// device table: gate matrices, then the gates' position terms
+	const size_t mbytes = (hm.size() * sizeof(cx<T>) + 255) & ~(size_t)255;
+	c.gtab.reserve(mbytes + hterms.size() * sizeof(GateTerms));
+	cx<T>* mats = c.gtab.as<cx<T>>();
+	GateTerms* gterms = reinterpret_cast<GateTerms*>(reinterpret_cast<char*>(mats) + mbytes);
+	RQC_HIP(hipMemcpyAsync(mats, hm.data(), hm.size() * sizeof(cx<T>), hipMemcpyHostToDevice, c.stream));
+	RQC_HIP(hipMemcpyAsync(gterms, hterms.data(), hterms.size() * sizeof(GateTerms), hipMemcpyHostToDevice, c.stream));
 
 	const int E = 1 << (ts.K + ts.logC);
 	const long ntiles = (N >> ts.K) * (B >> ts.logC);
-	const size_t lds_fwd = (size_t)(dual ? 2 : 1) * (E * sizeof(cx<T>) + kLdsPadBytes);
-	const size_t lds_rev = (size_t)(dual ? 4 : 2) * (E * sizeof(cx<T>) + kLdsPadBytes);
+	// two buffers of (psi[, dpsi]) forward and (psi, [dpsi,] phi[, dphi]) reverse, then
+	// the pass's gate matrices (forward G[, Z]; reverse G^-1, [W,] G^T[, Z^T])
+	const size_t mat_bytes = (size_t)kMaxTileGates * 16 * sizeof(cx<T>);
+	const size_t lds_fwd = 2 * (size_t)(dual ? 2 : 1) * (E * sizeof(cx<T>) + kLdsPadBytes) + (dual ? 2 : 1) * mat_bytes;
+	const size_t lds_rev = 2 * (size_t)(dual ? 4 : 2) * (E * sizeof(cx<T>) + kLdsPadBytes) + (dual ? 4 : 2) * mat_bytes;
 	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
 	auto* kr = dual ? k_tile_reverse<T, true> : k_tile_reverse<T, false>;
 	allow_lds(kf, lds_fwd);
@@ -705,7 +705,7 @@ static void sweep_tiles(Context& c, const Circuit& C, const zc* dU, Mode mode, z
 			cx<T>* out = (mode == Mode::Target) ? buf[p % 2] : PS(p + 1);
 			LaunchTimer lt(c, RQC_KERNEL_FORWARD, (double)(ns * (p > 0) + ns) * ab);
 			kf<<<gfwd, kTileThreads, lds_fwd, c.stream>>>(in, out, (dual && p > 0) ? DPS(p) : nullptr,
-				dual ? DPS(p + 1) : nullptr, args[p], amat, B, b0, ntiles, flags);
+				dual ? DPS(p + 1) : nullptr, args[p], mats, gterms, B, b0, ntiles, flags);
 		}
 		cx<T>* psiG = (mode == Mode::Target) ? buf[(P - 1) % 2] : PS(P);
 		{
@@ -728,7 +728,7 @@ static void sweep_tiles(Context& c, const Circuit& C, const zc* dU, Mode mode, z
 				const int reads = dual ? 3 + (p < P - 1) : 2;
 				LaunchTimer lt(c, RQC_KERNEL_REVERSE, (double)(reads + ns * (p > 0)) * ab);
 				kr<<<grev, kTileThreads, lds_rev, c.stream>>>(PS(p + 1), dual ? DPS(p + 1) : nullptr, phi[cur],
-					dual ? dphi[cur] : nullptr, phi[1 - cur], dual ? dphi[1 - cur] : nullptr, args[p], amat, B, ntiles,
+					dual ? dphi[cur] : nullptr, phi[1 - cur], dual ? dphi[1 - cur] : nullptr, args[p], mats, gterms, B, ntiles,
 					flags, mpart);
 			}
 			{

@@ -38,6 +38,8 @@
 // k_tile_reduce: no atomics, bitwise reproducible.
 #pragma once
 
+#include <cstdint>
+
 #include "zc.hpp"
 
 namespace rqc {
@@ -45,10 +47,6 @@ namespace rqc {
 constexpr int kTileThreads = 256;
 constexpr int kTileWaves = kTileThreads / 64;
 constexpr int kMaxTileGates = 8;
-// matrix-core groups a wave processes together (independent chains interleave)
-constexpr int kGroupStep = 2;
-// elements a thread moves per tile state (E / threads, E <= 2048)
-constexpr int kMaxEpt = 8;
 // LDS states of a tile are E elements + 64 bytes apart, so that equal positions of
 // two states fall on different banks
 constexpr int kLdsPadBytes = 64;
@@ -61,40 +59,84 @@ typedef float float4_t __attribute__((ext_vector_type(4)));
 // (bit i of f = parity(row & swz[i])).  The host picks h per pass (a colouring
 // of the pass's gate graph) so that the four amplitude rows of every gate's
 // quads fall on different banks.
+// Wave-uniform LDS position terms of a gate (see quad_pos): pos(0, r) of the
+// amplitudes, pos(4 t, 0) and pos(k, 0) (t, k < 4) of the quads within a
+// 16-quad group, pos(w qpw, 0) of the waves' quad ranges and pos(16 * 2^b, 0)
+// of the groups within a range (pos(16 i, 0) is the XOR over the set bits b of i).  Computed by the host per gate in execution order
+// (a read-only device table read with scalar loads), so that the gate loop
+// forms every position with a few selects and XORs.
+struct GateTerms
+{
+	int u[4], w[4], q4[4], wv[kTileWaves], gbit[3];
+};
+
 struct TileArgs
 {
 	int K;             // tile row bits
 	int logC;          // log2 of tile columns
 	int ng;            // gates in this pass
-	int x0;            // position of the pass's first gate in the operand table
+	int x0;            // position of the pass's first gate in the matrix table
+	int lognc;         // log2 of the column chunks of the block (B >> logC)
 	unsigned rmask;    // row bits outside the tile (rest), deposited lowest first
 	unsigned swz[3];   // column swizzle masks over tile rows
 	unsigned barrier;  // bit j: the waves own different quads for gates j - 1 and j
 	int rb[16];        // row bit of tile bit t, t < K
-	int lp[kMaxTileGates], lq[kMaxTileGates];  // tile bits of wire i and wire j of each gate
 };
 
-// operand table: A values of gate x (execution order), kind (0 forward, 1 inverse,
-// 2 adjoint), step kk, lane l at amat[((x * 3 + kind) * 4 + kk) * 64 + l]
-__device__ __forceinline__ long amat_index(int x, int kind, int kk, int lane)
+// Gate matrices of the sweep in execution order: gate x has kMats complex 4x4
+// matrices at mats[(x * kMats + i) * 16], i = G, Z, G^-1, -G^-1 Z G^-1, G^T, Z^T.
+// A kernel copies its pass's matrices into LDS once; each lane then forms its
+// matrix-core A operands from them (no global loads inside the pipelined tile
+// loop, whose asynchronous tile loads complete in issue order with any other
+// vector-memory access).
+constexpr int kMats = 6;
+enum { kMatG = 0, kMatZ = 1, kMatGinv = 2, kMatW = 3, kMatGT = 4, kMatZT = 5 };
+
+// A operand of step kk for this lane of the real 16 x 16 matrix with complex
+// blocks [[X, 0], [Y, X]] (Y = 0 without the tangent state), components
+// m = 4 v + r, v = 2 s + p: M[lane % 16][4 kk + lane / 16]
+template <typename T, bool DUAL>
+__device__ __forceinline__ T a_operand(const cx<T>* X, const cx<T>* Y, int kk)
 {
-	return (((long)x * 3 + kind) * 4 + kk) * 64 + lane;
+	const int lane = threadIdx.x & 63;
+	const int i = lane & 15, vo = i >> 2, ro = i & 3, ri = lane >> 4;
+	const int so = vo >> 1, po = vo & 1, si = kk >> 1, pi = kk & 1;
+	// (branch-free: both blocks are read; sign and zero block by exact factors)
+	const cx<T> zx = X[4 * ro + ri];
+	const cx<T> zy = DUAL ? Y[4 * ro + ri] : zx;
+	const cx<T> z = (so == si) ? zx : zy;
+	const T v = (po == pi) ? z.x : z.y;
+	const T sign = (T)(1 - 2 * (po < pi));                     // -Im above the diagonal
+	const T keep = (T)((so == si) | (DUAL & (so == 1)));       // [[X, 0], [Y, X]]
+	return v * sign * keep;
+}
+
+// copy the pass's gate matrices (those listed in `which`) into LDS
+template <typename T>
+__device__ __forceinline__ void load_pass_mats(const cx<T>* __restrict__ mats, int x0, int ng, const int* which, int nw,
+	cx<T>* lm)
+{
+	for (int e = threadIdx.x; e < ng * nw * 16; e += kTileThreads) {
+		const int j = e / (nw * 16), w = (e / 16) % nw, t = e & 15;
+		lm[e] = mats[((long)(x0 + j) * kMats + which[w]) * 16 + t];
+	}
 }
 
 // zero bits inserted at tile bit positions lo < hi
-__device__ __forceinline__ int tile_quad_row(int q, int lo, int hi)
+__host__ __device__ __forceinline__ int tile_quad_row(int q, int lo, int hi)
 {
 	int x = ((q >> lo) << (lo + 1)) | (q & ((1 << lo) - 1));
 	x = ((x >> hi) << (hi + 1)) | (x & ((1 << hi) - 1));
 	return x;
 }
 
-__device__ __forceinline__ int swz_f(int row, const TileArgs& P)
+__host__ __device__ __forceinline__ int swz_f(int row, const TileArgs& P)
 {
-	return (__popc(row & P.swz[0]) & 1) | ((__popc(row & P.swz[1]) & 1) << 1) | ((__popc(row & P.swz[2]) & 1) << 2);
+	return (__builtin_popcount(row & P.swz[0]) & 1) | ((__builtin_popcount(row & P.swz[1]) & 1) << 1) |
+	       ((__builtin_popcount(row & P.swz[2]) & 1) << 2);
 }
 
-__device__ __forceinline__ int lds_elem(int row, int col, const TileArgs& P)
+__host__ __device__ __forceinline__ int lds_elem(int row, int col, const TileArgs& P)
 {
 	return (row << P.logC) | (col ^ swz_f(row, P));
 }
@@ -104,25 +146,10 @@ __device__ __forceinline__ int lds_elem(int row, int col, const TileArgs& P)
 // r = 2 qi + qj (bit p = qi, bit q = qj).  The position is XOR-linear in qd
 // for disjoint bit parts, so pos(gbase | j, r) = pos(gbase, 0) ^ pos(j, r): a
 // wave-uniform group term and a per-lane term fixed for the gate.
-__device__ __forceinline__ int quad_pos(int qd, int r, int p, int q, const TileArgs& P)
+__host__ __device__ __forceinline__ int quad_pos(int qd, int r, int p, int q, const TileArgs& P)
 {
 	const int row = tile_quad_row(qd >> P.logC, min(p, q), max(p, q)) | ((r >> 1) << p) | ((r & 1) << q);
 	return lds_elem(row, qd & ((1 << P.logC) - 1), P);
-}
-
-// row-index contributions of the tile bits and LDS positions of the elements
-// e = tid + i * threads a thread moves between HBM and LDS; fixed for a pass
-__device__ __forceinline__ void thread_tile_elems(const TileArgs& P, int E, int* trow, int* lidx)
-{
-#pragma unroll
-	for (int i = 0; i < kMaxEpt; i++) {
-		const int e = threadIdx.x + i * kTileThreads;
-		int r = 0;
-		const int tr = e >> P.logC;
-		for (int t = 0; t < P.K; t++) { r |= ((tr >> t) & 1) << P.rb[t]; }
-		trow[i] = (e < E) ? r : 0;
-		lidx[i] = (e < E) ? lds_elem(tr, e & ((1 << P.logC) - 1), P) : 0;
-	}
 }
 
 // scatter the low bits of x to the set bits of mask, lowest first
@@ -164,130 +191,217 @@ __device__ __forceinline__ int mfma_row(int lane, int v)
 // (step t: quad 4 t + lane >> 4, amplitude of component lane & 15)
 struct GateLanes
 {
-	int p, q;
 	int ea;          // gate application, amplitude lane >> 4
 	int ew[4];       // amplitudes 0..3 of quad lane & 15
 	int eo[4];       // outer-product step t
 };
 
-__device__ __forceinline__ GateLanes gate_lanes(int p, int q, const TileArgs& P)
+// (by linearity: pos(j, r) = pos(j, 0) ^ pos(0, r), pos(4 t + k, 0) =
+// pos(4 t, 0) ^ pos(k, 0))
+// v[i] for a per-lane i < 4 of four wave-uniform values, as selects (the values
+// are passed by value so that they stay in scalar registers)
+__device__ __forceinline__ int sel4(int v0, int v1, int v2, int v3, int i)
+{
+	const int lo = (i & 1) ? v1 : v0;
+	const int hi = (i & 1) ? v3 : v2;
+	return (i & 2) ? hi : lo;
+}
+
+__device__ __forceinline__ int sel4(const int (&v)[4], int i) { return sel4(v[0], v[1], v[2], v[3], i); }
+
+// position term of the groups starting at quad 16 i of a wave's range
+__device__ __forceinline__ int group_term(const GateTerms& t, int i)
+{
+	return ((i & 1) ? t.gbit[0] : 0) ^ ((i & 2) ? t.gbit[1] : 0) ^ ((i & 4) ? t.gbit[2] : 0);
+}
+
+__device__ __forceinline__ GateLanes gate_lanes(const GateTerms& t)
 {
 	const int lane = threadIdx.x & 63;
 	GateLanes g;
-	g.p = p;
-	g.q = q;
+	const int base = sel4(t.q4, lane & 3) ^ sel4(t.w, (lane >> 2) & 3);
 #pragma unroll
-	for (int v = 0; v < 4; v++) { g.ew[v] = quad_pos(lane & 15, v, p, q, P); }
-	g.ea = quad_pos(lane & 15, lane >> 4, p, q, P);
-	const int ra = ((lane & 15) >> 1) & 3;
+	for (int v = 0; v < 4; v++) { g.ew[v] = base ^ t.u[v]; }
+	g.ea = base ^ sel4(t.u, lane >> 4);
+	const int bk = sel4(t.q4, lane >> 4) ^ sel4(t.u, ((lane & 15) >> 1) & 3);
 #pragma unroll
-	for (int t = 0; t < 4; t++) { g.eo[t] = quad_pos(4 * t + (lane >> 4), ra, p, q, P); }
+	for (int k = 0; k < 4; k++) { g.eo[k] = bk ^ t.w[k]; }
 	return g;
 }
 
-// apply a gate to kGroupStep groups of 16 quads in LDS (in place).  Group n
-// starts at quad gb[n] (a multiple of 16; group term eg[n] = pos(gb[n], 0)); its
-// second operand quads are the same quads of Y (dual: tangent state) or the 16
-// quads after them in X (plain, group terms eg2[n]).  a[kk] = M[lane % 16][4 kk + lane / 16].
-// All loads are issued before the matrix-core chains and all stores after, so
-// the groups' chains overlap.
-template <typename T>
-__device__ __forceinline__ void mfma_gate_apply(cx<T>* X, cx<T>* Y, const int* eg, const int* eg2, const GateLanes& gl,
-	const T* a)
+// The three parts of a gate application on GS groups, so that independent
+// applications (the state and the adjoint of the reverse pass) can interleave.
+template <typename T, int GS>
+struct GateOperands
 {
-	const int lane = threadIdx.x & 63;
-	const int r = lane >> 4;
-	int ex[kGroupStep], ey[kGroupStep];
-	cx<T> x[kGroupStep], y[kGroupStep];
-	typename Vec4<T>::type acc[kGroupStep];
+	cx<T> x[GS], y[GS];
+	typename Vec4<T>::type acc[GS];
+};
+
+template <typename T, int GS>
+__device__ __forceinline__ void gate_load(const cx<T>* X, const cx<T>* Y, const int* eg, const int* eg2,
+	const GateLanes& gl, GateOperands<T, GS>& o)
+{
 #pragma unroll
-	for (int n = 0; n < kGroupStep; n++) {
-		ex[n] = eg[n] ^ gl.ea;
-		ey[n] = eg2[n] ^ gl.ea;
-		x[n] = X[ex[n]];
-		y[n] = Y[ey[n]];
-		acc[n] = typename Vec4<T>::type{0, 0, 0, 0};
+	for (int n = 0; n < GS; n++) {
+		o.x[n] = X[eg[n] ^ gl.ea];
+		o.y[n] = Y[eg2[n] ^ gl.ea];
 	}
+}
+
+template <typename T, int GS>
+__device__ __forceinline__ void gate_mfma(const T* a, GateOperands<T, GS>& o)
+{
 #pragma unroll
-	for (int n = 0; n < kGroupStep; n++) { acc[n] = mfma_step(a[0], x[n].x, acc[n]); }
+	for (int n = 0; n < GS; n++) { o.acc[n] = mfma_step(a[0], o.x[n].x, typename Vec4<T>::type{0, 0, 0, 0}); }
 #pragma unroll
-	for (int n = 0; n < kGroupStep; n++) { acc[n] = mfma_step(a[1], x[n].y, acc[n]); }
+	for (int n = 0; n < GS; n++) { o.acc[n] = mfma_step(a[1], o.x[n].y, o.acc[n]); }
 #pragma unroll
-	for (int n = 0; n < kGroupStep; n++) { acc[n] = mfma_step(a[2], y[n].x, acc[n]); }
+	for (int n = 0; n < GS; n++) { o.acc[n] = mfma_step(a[2], o.y[n].x, o.acc[n]); }
 #pragma unroll
-	for (int n = 0; n < kGroupStep; n++) { acc[n] = mfma_step(a[3], y[n].y, acc[n]); }
+	for (int n = 0; n < GS; n++) { o.acc[n] = mfma_step(a[3], o.y[n].y, o.acc[n]); }
+}
+
+template <typename T, int GS>
+__device__ __forceinline__ void gate_store(cx<T>* X, cx<T>* Y, const int* eg, const int* eg2, const GateLanes& gl,
+	const GateOperands<T, GS>& o)
+{
+	const int r = (threadIdx.x & 63) >> 4;
 #pragma unroll
-	for (int n = 0; n < kGroupStep; n++) {
+	for (int n = 0; n < GS; n++) {
 		if constexpr (sizeof(T) == 8) {
 			// register v = component v (x.re, x.im, y.re, y.im) of amplitude r
-			X[ex[n]] = cx<T>{acc[n][0], acc[n][1]};
-			Y[ey[n]] = cx<T>{acc[n][2], acc[n][3]};
+			X[eg[n] ^ gl.ea] = cx<T>{o.acc[n][0], o.acc[n][1]};
+			Y[eg2[n] ^ gl.ea] = cx<T>{o.acc[n][2], o.acc[n][3]};
 		} else {
 			// register v = component r (x.re, x.im, y.re, y.im) of amplitude v
 			T* base = reinterpret_cast<T*>(r < 2 ? X : Y);
 			const int e0 = (r < 2) ? eg[n] : eg2[n];
 #pragma unroll
-			for (int v = 0; v < 4; v++) { base[2 * (e0 ^ gl.ew[v]) + (r & 1)] = acc[n][v]; }
+			for (int v = 0; v < 4; v++) { base[2 * (e0 ^ gl.ew[v]) + (r & 1)] = o.acc[n][v]; }
 		}
 	}
 }
 
-// load / store one tile state between HBM and LDS
-template <typename T>
-__device__ __forceinline__ void tile_load(cx<T>* S, const cx<T>* __restrict__ src, const int* trow, const int* lidx,
-	long rbase, long c0, int E, int logC, long B)
+// all loads are issued before the matrix-core chains and all stores after, so
+// the groups' chains overlap
+template <typename T, int GS>
+__device__ __forceinline__ void mfma_gate_apply(cx<T>* X, cx<T>* Y, const int* eg, const int* eg2, const GateLanes& gl,
+	const T* a)
 {
-	const int C = 1 << logC;
+	GateOperands<T, GS> o;
+	gate_load<T, GS>(X, Y, eg, eg2, gl, o);
+	gate_mfma<T, GS>(a, o);
+	gate_store<T, GS>(X, Y, eg, eg2, gl, o);
+}
+
+// ---------------------------------------------------------------------------
+// HBM <-> LDS.  A tile state is E * sizeof(cx<T>) / 16 slots of 16 bytes (one
+// FP64 or two FP32 elements; the FP32 swizzle keeps the two elements of a slot
+// in adjacent columns).  Thread tid handles slots tid + k * 256.  Loads go
+// straight from HBM into LDS (global_load_lds_dwordx4: lane-contiguous slots
+// from the wave's base), so the next tile streams in while the current one is
+// computed; the lane picks the global element whose LDS position is its slot.
+constexpr int kMaxSlots = 8;   // slots per thread and state (E * sizeof(cx<T>) <= 32 KiB)
+
+template <typename T>
+struct TileIO
+{
+	static constexpr int epl = 16 / (int)sizeof(cx<T>);   // elements per slot
+	long goff[kMaxSlots];   // global element offset (from rbase * B + c0) of the slot's first element
+	int trow[kMaxSlots];    // row-index contribution of the tile bits
+	int col[kMaxSlots];     // column of the slot's first element
+	int nk;                 // slots per thread
+};
+
+template <typename T>
+__device__ __forceinline__ void tile_io_init(const TileArgs& P, int E, long B, TileIO<T>& io)
+{
+	constexpr int epl = TileIO<T>::epl;
+	io.nk = E / epl / kTileThreads;
 #pragma unroll
-	for (int i = 0; i < kMaxEpt; i++) {
-		const int e = threadIdx.x + i * kTileThreads;
-		if (e >= E) { break; }
-		S[lidx[i]] = src[(rbase | (long)trow[i]) * B + c0 + (e & (C - 1))];
+	for (int k = 0; k < kMaxSlots; k++) {
+		const int pe = (threadIdx.x + k * kTileThreads) * epl;   // LDS position of the slot
+		const int row = pe >> P.logC;
+		const int col = (pe & ((1 << P.logC) - 1)) ^ swz_f(row, P);
+		int tr = 0;
+		for (int t = 0; t < P.K; t++) { tr |= ((row >> t) & 1) << P.rb[t]; }
+		io.trow[k] = tr;
+		io.col[k] = col;
+		io.goff[k] = (long)tr * B + col;
 	}
 }
 
-// (a uniform fill needs no swizzle)
+// start the asynchronous load of a tile state (completion: s_waitcnt vmcnt)
 template <typename T>
-__device__ __forceinline__ void tile_fill(cx<T>* S, int E, cx<T> v)
+__device__ __forceinline__ void tile_issue(cx<T>* S, const cx<T>* __restrict__ src, long base, const TileIO<T>& io)
 {
+#if defined(__HIP_DEVICE_COMPILE__)
+	// (the builtin exists in the device compilation only)
+	constexpr int epl = TileIO<T>::epl;
+	const int w0 = threadIdx.x & ~63;
 #pragma unroll
-	for (int i = 0; i < kMaxEpt; i++) {
-		const int e = threadIdx.x + i * kTileThreads;
-		if (e >= E) { break; }
-		S[e] = v;
+	for (int k = 0; k < kMaxSlots; k++) {
+		if (k < io.nk) {
+			__builtin_amdgcn_global_load_lds(src + base + io.goff[k],
+				reinterpret_cast<__attribute__((address_space(3))) void*>(
+					reinterpret_cast<uintptr_t>(S + (k * kTileThreads + w0) * epl)), 16, 0, 0);
+		}
+	}
+#endif
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_fill(cx<T>* S, const TileIO<T>& io, cx<T> v)
+{
+	constexpr int epl = TileIO<T>::epl;
+#pragma unroll
+	for (int k = 0; k < kMaxSlots; k++) {
+		if (k < io.nk) {
+#pragma unroll
+			for (int i = 0; i < epl; i++) { S[(threadIdx.x + k * kTileThreads) * epl + i] = v; }
+		}
+	}
+}
+
+// column block b0 of the identity: element (row, col) = [row == b0 + c0 + col]
+template <typename T>
+__device__ __forceinline__ void tile_identity(cx<T>* S, const TileIO<T>& io, long rbase, long c0, long b0)
+{
+	constexpr int epl = TileIO<T>::epl;
+#pragma unroll
+	for (int k = 0; k < kMaxSlots; k++) {
+		if (k < io.nk) {
+			const long row = rbase | (long)io.trow[k];
+#pragma unroll
+			for (int i = 0; i < epl; i++) {
+				S[(threadIdx.x + k * kTileThreads) * epl + i] = cx<T>{(row == b0 + c0 + io.col[k] + i) ? (T)1 : (T)0, (T)0};
+			}
+		}
 	}
 }
 
 template <typename T>
-__device__ __forceinline__ void tile_identity(cx<T>* S, const int* trow, const int* lidx, long rbase, long c0, long b0,
-	int E, int logC)
+__device__ __forceinline__ void tile_store(cx<T>* __restrict__ dst, const cx<T>* S, long base, const TileIO<T>& io)
 {
-	const int C = 1 << logC;
+	constexpr int epl = TileIO<T>::epl;
+	typedef double slot_t __attribute__((ext_vector_type(2)));
 #pragma unroll
-	for (int i = 0; i < kMaxEpt; i++) {
-		const int e = threadIdx.x + i * kTileThreads;
-		if (e >= E) { break; }
-		const long row = rbase | (long)trow[i];
-		S[lidx[i]] = cx<T>{(row == b0 + c0 + (e & (C - 1))) ? (T)1 : (T)0, (T)0};
-	}
-}
-
-template <typename T>
-__device__ __forceinline__ void tile_store(cx<T>* __restrict__ dst, const cx<T>* S, const int* trow, const int* lidx,
-	long rbase, long c0, int E, int logC, long B)
-{
-	const int C = 1 << logC;
-#pragma unroll
-	for (int i = 0; i < kMaxEpt; i++) {
-		const int e = threadIdx.x + i * kTileThreads;
-		if (e >= E) { break; }
-		dst[(rbase | (long)trow[i]) * B + c0 + (e & (C - 1))] = S[lidx[i]];
+	for (int k = 0; k < kMaxSlots; k++) {
+		if (k < io.nk) {
+			const slot_t v = *reinterpret_cast<const slot_t*>(S + (threadIdx.x + k * kTileThreads) * epl);
+			*reinterpret_cast<slot_t*>(dst + base + io.goff[k]) = v;
+		}
 	}
 }
 
 // Wave w owns the quads [w qpw, (w + 1) qpw) of every gate, i.e. those whose two
 // highest tile bits outside the gate equal w; consecutive gates with the same
 // two bits need no barrier between them (TileArgs::barrier).
+//
+// Both kernels are software-pipelined over the tiles of a workgroup: LDS holds
+// two buffers of the tile's states; while the gates of tile t run in one, tile
+// t + grid streams into the other.
 
 // ---------------------------------------------------------------------------
 // forward pass: psi -> (gates) psi, with DUAL also dpsi.  The input is
@@ -296,54 +410,66 @@ __device__ __forceinline__ void tile_store(cx<T>* __restrict__ dst, const cx<T>*
 template <typename T, bool DUAL>
 __global__ __launch_bounds__(kTileThreads) void k_tile_forward(
 	const cx<T>* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __restrict__ din, cx<T>* __restrict__ dout,
-	const TileArgs P, const T* __restrict__ amat, long B, long b0, long ntiles, int flags)
+	const TileArgs P, const cx<T>* __restrict__ mats, const GateTerms* __restrict__ gterms, long B, long b0, long ntiles,
+	int flags)
 {
 	extern __shared__ __align__(16) char lds_raw[];
-	const int E = 1 << (P.K + P.logC);
-	const int pad = kLdsPadBytes / (int)sizeof(cx<T>);
-	cx<T>* S = reinterpret_cast<cx<T>*>(lds_raw);
-	cx<T>* D = S + E + pad;
-	const int lane = threadIdx.x & 63;
-	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+	constexpr int NS = DUAL ? 2 : 1;              // states per buffer
+	constexpr int GS = DUAL ? 2 : 1;              // matrix-core groups per step
 	constexpr int gq = DUAL ? 16 : 32;            // quads per matrix-core group
+	const int E = 1 << (P.K + P.logC);
+	const int sbytes = E * (int)sizeof(cx<T>) + kLdsPadBytes;
+	auto state = [&](int b, int i) { return reinterpret_cast<cx<T>*>(lds_raw + (b * NS + i) * sbytes); };
+	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int qpw = (E / 4) / kTileWaves;         // quads owned by a wave
-	const int q0 = wave * qpw;
-
-	const long nchunks = B >> P.logC;
-	int trow[kMaxEpt], lidx[kMaxEpt];
-	thread_tile_elems(P, E, trow, lidx);
-	for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-		const long chunk = tile % nchunks, rest = tile / nchunks;
-		const long rbase = deposit_bits(rest, P.rmask);
-		const long c0 = chunk << P.logC;
-		if (flags & 1) { tile_identity(S, trow, lidx, rbase, c0, b0, E, P.logC); }
-		else { tile_load(S, in, trow, lidx, rbase, c0, E, P.logC, B); }
+	const long cmask = (1L << P.lognc) - 1;
+	TileIO<T> io;
+	tile_io_init(P, E, B, io);
+	// gate matrices G, Z of the pass after the two buffers
+	cx<T>* lm = reinterpret_cast<cx<T>*>(lds_raw + 2 * NS * sbytes);
+	const int which[2] = {kMatG, kMatZ};
+	load_pass_mats(mats, P.x0, P.ng, which, DUAL ? 2 : 1, lm);
+	auto prefetch = [&](long tile, int b) {
+		const long rbase = deposit_bits(tile >> P.lognc, P.rmask);
+		const long c0 = (tile & cmask) << P.logC;
+		if (flags & 1) { tile_identity(state(b, 0), io, rbase, c0, b0); }
+		else { tile_issue(state(b, 0), in, rbase * B + c0, io); }
 		if (DUAL) {
-			if (flags & 2) { tile_fill(D, E, cx<T>{0, 0}); }
-			else { tile_load(D, din, trow, lidx, rbase, c0, E, P.logC, B); }
+			if (flags & 2) { tile_fill(state(b, 1), io, cx<T>{0, 0}); }
+			else { tile_issue(state(b, 1), din, rbase * B + c0, io); }
 		}
+	};
+	long tile = blockIdx.x;
+	if (tile < ntiles) { prefetch(tile, 0); }
+	for (int b = 0; tile < ntiles; tile += gridDim.x, b ^= 1) {
+		__builtin_amdgcn_s_waitcnt(0);
 		__syncthreads();
+		if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, b ^ 1); }
+		cx<T>* S = state(b, 0);
+		cx<T>* D = state(b, DUAL ? 1 : 0);
 		for (int j = 0; j < P.ng; j++) {
 			if (j > 0 && ((P.barrier >> j) & 1)) { __syncthreads(); }
-			const int p = P.lp[j], q = P.lq[j];
-			const GateLanes gl = gate_lanes(p, q, P);
+			const GateTerms gt = gterms[P.x0 + j];
+			const GateLanes gl = gate_lanes(gt);
+			const int ewv = sel4(gt.wv, wave);
+			const cx<T>* mj = lm + j * (DUAL ? 2 : 1) * 16;
 			T a[4];
 #pragma unroll
-			for (int kk = 0; kk < 4; kk++) { a[kk] = amat[amat_index(P.x0 + j, 0, kk, lane)]; }
-			for (int g = q0; g < q0 + qpw; g += kGroupStep * gq) {
-				int eg[kGroupStep], eg2[kGroupStep];
+			for (int kk = 0; kk < 4; kk++) { a[kk] = a_operand<T, DUAL>(mj, mj + 16, kk); }
+			for (int g = 0; g < qpw; g += GS * gq) {
+				int eg[GS], eg2[GS];
 #pragma unroll
-				for (int n = 0; n < kGroupStep; n++) {
-					eg[n] = quad_pos(g + n * gq, 0, p, q, P);
-					eg2[n] = DUAL ? eg[n] : quad_pos(g + n * gq + 16, 0, p, q, P);
+				for (int n = 0; n < GS; n++) {
+					eg[n] = ewv ^ group_term(gt, (g + n * gq) >> 4);
+					eg2[n] = DUAL ? eg[n] : ewv ^ group_term(gt, ((g + n * gq) >> 4) + 1);
 				}
-				mfma_gate_apply<T>(S, DUAL ? D : S, eg, eg2, gl, a);
+				mfma_gate_apply<T, GS>(S, D, eg, eg2, gl, a);
 			}
 		}
 		__syncthreads();
-		tile_store(out, S, trow, lidx, rbase, c0, E, P.logC, B);
-		if (DUAL) { tile_store(dout, D, trow, lidx, rbase, c0, E, P.logC, B); }
-		__syncthreads();
+		const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
+		tile_store(out, S, base, io);
+		if (DUAL) { tile_store(dout, D, base, io); }
 	}
 }
 
@@ -359,115 +485,141 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 	const cx<T>* __restrict__ psi_out, const cx<T>* __restrict__ dpsi_out,
 	const cx<T>* __restrict__ phi_out, const cx<T>* __restrict__ dphi_out,
 	cx<T>* __restrict__ phi_in, cx<T>* __restrict__ dphi_in,
-	const TileArgs P, const T* __restrict__ amat, long B, long ntiles, int flags, double* __restrict__ partial)
+	const TileArgs P, const cx<T>* __restrict__ mats, const GateTerms* __restrict__ gterms, long B, long ntiles, int flags,
+	double* __restrict__ partial)
 {
 	extern __shared__ __align__(16) char lds_raw[];
+	constexpr int NS = DUAL ? 4 : 2;              // states per buffer: psi, [dpsi], phi, [dphi]
+	constexpr int GS = DUAL ? 2 : 1;
+	constexpr int gq = DUAL ? 16 : 32;
 	const int E = 1 << (P.K + P.logC);
-	const int pad = kLdsPadBytes / (int)sizeof(cx<T>);
-	cx<T>* S = reinterpret_cast<cx<T>*>(lds_raw);
-	cx<T>* D = S + E + pad;
-	cx<T>* F = S + (DUAL ? 2 : 1) * (E + pad);
-	cx<T>* DF = F + E + pad;
+	const int sbytes = E * (int)sizeof(cx<T>) + kLdsPadBytes;
+	auto state = [&](int b, int i) { return reinterpret_cast<cx<T>*>(lds_raw + (b * NS + i) * sbytes); };
 	const int lane = threadIdx.x & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-	constexpr int gq = DUAL ? 16 : 32;
 	const int qpw = (E / 4) / kTileWaves;
-	const int q0 = wave * qpw;
 	const int m = lane & 15;
 	// outer-product operands: component m = 8 s + 2 r + p of the quad (dual: s
 	// selects the primary / tangent array; plain: the quad or the one 16 after it)
-	const T* Fa = reinterpret_cast<const T*>((DUAL && (m >> 3)) ? DF : F);
-	const T* Sa = reinterpret_cast<const T*>((DUAL && (m >> 3)) ? D : S);
 	const int osel = DUAL ? 0 : (m >> 3);
+	const int oarr = (DUAL && (m >> 3)) ? 1 : 0;
 
 	double4_t acc[kMaxTileGates];
 #pragma unroll
 	for (int j = 0; j < kMaxTileGates; j++) { acc[j] = double4_t{0, 0, 0, 0}; }
 
-	const long nchunks = B >> P.logC;
-	int trow[kMaxEpt], lidx[kMaxEpt];
-	thread_tile_elems(P, E, trow, lidx);
-	for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-		const long chunk = tile % nchunks, rest = tile / nchunks;
-		const long rbase = deposit_bits(rest, P.rmask);
-		const long c0 = chunk << P.logC;
-		tile_load(S, psi_out, trow, lidx, rbase, c0, E, P.logC, B);
-		tile_load(F, phi_out, trow, lidx, rbase, c0, E, P.logC, B);
+	const long cmask = (1L << P.lognc) - 1;
+	TileIO<T> io;
+	tile_io_init(P, E, B, io);
+	// gate matrices G^-1, -G^-1 Z G^-1, G^T, Z^T of the pass after the two buffers
+	constexpr int NM = DUAL ? 4 : 2;
+	cx<T>* lm = reinterpret_cast<cx<T>*>(lds_raw + 2 * NS * sbytes);
+	const int which[4] = {kMatGinv, DUAL ? kMatW : kMatGT, kMatGT, kMatZT};
+	const int which_plain[2] = {kMatGinv, kMatGT};
+	load_pass_mats(mats, P.x0, P.ng, DUAL ? which : which_plain, NM, lm);
+	auto prefetch = [&](long tile, int b) {
+		const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
+		tile_issue(state(b, 0), psi_out, base, io);
+		tile_issue(state(b, NS / 2), phi_out, base, io);
 		if (DUAL) {
-			tile_load(D, dpsi_out, trow, lidx, rbase, c0, E, P.logC, B);
-			if (flags & 4) { tile_fill(DF, E, cx<T>{0, 0}); }
-			else { tile_load(DF, dphi_out, trow, lidx, rbase, c0, E, P.logC, B); }
+			tile_issue(state(b, 1), dpsi_out, base, io);
+			if (flags & 4) { tile_fill(state(b, 3), io, cx<T>{0, 0}); }
+			else { tile_issue(state(b, 3), dphi_out, base, io); }
 		}
+	};
+	long tile = blockIdx.x;
+	if (tile < ntiles) { prefetch(tile, 0); }
+	for (int b = 0; tile < ntiles; tile += gridDim.x, b ^= 1) {
+		__builtin_amdgcn_s_waitcnt(0);
 		__syncthreads();
+		if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, b ^ 1); }
+		cx<T>* S = state(b, 0);
+		cx<T>* D = state(b, DUAL ? 1 : 0);
+		cx<T>* F = state(b, NS / 2);
+		cx<T>* DF = state(b, DUAL ? 3 : NS / 2);
+		const T* Fa = reinterpret_cast<const T*>(oarr ? DF : F);
+		const T* Sa = reinterpret_cast<const T*>(oarr ? D : S);
 		for (int j = P.ng - 1; j >= 0; j--) {
-			{
-				if (j + 1 < P.ng && ((P.barrier >> (j + 1)) & 1)) { __syncthreads(); }
-				const int p = P.lp[j], q = P.lq[j];
-				const GateLanes gl = gate_lanes(p, q, P);
-				T au[4], aa[4];
+			if (j + 1 < P.ng && ((P.barrier >> (j + 1)) & 1)) { __syncthreads(); }
+			const GateTerms gt = gterms[P.x0 + j];
+			const GateLanes gl = gate_lanes(gt);
+			const int ewv = sel4(gt.wv, wave);
+			// recovery: [[G^-1, 0], [-G^-1 Z G^-1, G^-1]]; adjoint: [[G^T, 0], [Z^T, G^T]]
+			const cx<T>* mj = lm + j * NM * 16;
+			T au[4], aa[4];
 #pragma unroll
-				for (int kk = 0; kk < 4; kk++) {
-					au[kk] = amat[amat_index(P.x0 + j, 1, kk, lane)];
-					aa[kk] = amat[amat_index(P.x0 + j, 2, kk, lane)];
+			for (int kk = 0; kk < 4; kk++) {
+				au[kk] = a_operand<T, DUAL>(mj, mj + 16, kk);
+				aa[kk] = a_operand<T, DUAL>(mj + (DUAL ? 32 : 16), mj + 48, kk);
+			}
+			const bool adjoint = (j > 0) || !(flags & 8);
+			double4_t gsum = {0, 0, 0, 0};
+			for (int g = 0; g < qpw; g += GS * gq) {
+				int eg[GS], eg2[GS];
+#pragma unroll
+				for (int n = 0; n < GS; n++) {
+					eg[n] = ewv ^ group_term(gt, (g + n * gq) >> 4);
+					eg2[n] = DUAL ? eg[n] : ewv ^ group_term(gt, ((g + n * gq) >> 4) + 1);
 				}
-				const bool adjoint = (j > 0) || !(flags & 8);
-				double4_t gsum = {0, 0, 0, 0};
-				for (int g = q0; g < q0 + qpw; g += kGroupStep * gq) {
-					int eg[kGroupStep], eg2[kGroupStep];
+				// The state's recovery (uncompute) and the adjoint's update are
+				// independent: load both, run both chains, store the state; the
+				// outer products need the recovered state and the adjoint before
+				// its update (read before the adjoint's store).
+				GateOperands<T, GS> os, oa;
+				gate_load<T, GS>(S, D, eg, eg2, gl, os);
+				gate_load<T, GS>(F, DF, eg, eg2, gl, oa);
+				T av[GS][4], bv[GS][4];
 #pragma unroll
-					for (int n = 0; n < kGroupStep; n++) {
-						eg[n] = quad_pos(g + n * gq, 0, p, q, P);
-						eg2[n] = DUAL ? eg[n] : quad_pos(g + n * gq + 16, 0, p, q, P);
-					}
-					// states before gate j
-					mfma_gate_apply<T>(S, DUAL ? D : S, eg, eg2, gl, au);
-					// outer products over the groups' quads, 4 quads per step
-					typename Vec4<T>::type grp[kGroupStep];
-					T av[kGroupStep][4], bv[kGroupStep][4];
+				for (int n = 0; n < GS; n++) {
+					const int eb = osel ? eg2[n] : eg[n];
 #pragma unroll
-					for (int n = 0; n < kGroupStep; n++) {
-						grp[n] = typename Vec4<T>::type{0, 0, 0, 0};
-						const int eb = osel ? eg2[n] : eg[n];
-#pragma unroll
-						for (int t = 0; t < 4; t++) {
-							const int el = 2 * (eb ^ gl.eo[t]) + (m & 1);
-							av[n][t] = Fa[el];
-							bv[n][t] = Sa[el];
-						}
-					}
-#pragma unroll
-					for (int t = 0; t < 4; t++) {
-#pragma unroll
-						for (int n = 0; n < kGroupStep; n++) { grp[n] = mfma_step(av[n][t], bv[n][t], grp[n]); }
-					}
-#pragma unroll
-					for (int n = 0; n < kGroupStep; n++) {
-#pragma unroll
-						for (int v = 0; v < 4; v++) { gsum[v] += (double)grp[n][v]; }
-					}
-					// adjoints at the gate input
-					if (adjoint) { mfma_gate_apply<T>(F, DUAL ? DF : F, eg, eg2, gl, aa); }
+					for (int t = 0; t < 4; t++) { av[n][t] = Fa[2 * (eb ^ gl.eo[t]) + (m & 1)]; }
 				}
-				// accumulator of gate j (constant register indices in every case)
-				static_assert(kMaxTileGates == 8, "one case per gate of a pass");
-				switch (j) {
-				case 0: acc[0] += gsum; break;
-				case 1: acc[1] += gsum; break;
-				case 2: acc[2] += gsum; break;
-				case 3: acc[3] += gsum; break;
-				case 4: acc[4] += gsum; break;
-				case 5: acc[5] += gsum; break;
-				case 6: acc[6] += gsum; break;
-				default: acc[7] += gsum; break;
+				gate_mfma<T, GS>(au, os);
+				gate_mfma<T, GS>(aa, oa);
+				gate_store<T, GS>(S, D, eg, eg2, gl, os);
+				// outer products over the groups' quads, 4 quads per step
+#pragma unroll
+				for (int n = 0; n < GS; n++) {
+					const int eb = osel ? eg2[n] : eg[n];
+#pragma unroll
+					for (int t = 0; t < 4; t++) { bv[n][t] = Sa[2 * (eb ^ gl.eo[t]) + (m & 1)]; }
 				}
+				typename Vec4<T>::type grp[GS];
+#pragma unroll
+				for (int n = 0; n < GS; n++) { grp[n] = typename Vec4<T>::type{0, 0, 0, 0}; }
+#pragma unroll
+				for (int t = 0; t < 4; t++) {
+#pragma unroll
+					for (int n = 0; n < GS; n++) { grp[n] = mfma_step(av[n][t], bv[n][t], grp[n]); }
+				}
+#pragma unroll
+				for (int n = 0; n < GS; n++) {
+#pragma unroll
+					for (int v = 0; v < 4; v++) { gsum[v] += (double)grp[n][v]; }
+				}
+				// adjoints at the gate input
+				if (adjoint) { gate_store<T, GS>(F, DF, eg, eg2, gl, oa); }
+			}
+			// accumulator of gate j (constant register indices in every case)
+			static_assert(kMaxTileGates == 8, "one case per gate of a pass");
+			switch (j) {
+			case 0: acc[0] += gsum; break;
+			case 1: acc[1] += gsum; break;
+			case 2: acc[2] += gsum; break;
+			case 3: acc[3] += gsum; break;
+			case 4: acc[4] += gsum; break;
+			case 5: acc[5] += gsum; break;
+			case 6: acc[6] += gsum; break;
+			default: acc[7] += gsum; break;
 			}
 		}
 		__syncthreads();
 		if (!(flags & 8)) {
-			tile_store(phi_in, F, trow, lidx, rbase, c0, E, P.logC, B);
-			if (DUAL) { tile_store(dphi_in, DF, trow, lidx, rbase, c0, E, P.logC, B); }
+			const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
+			tile_store(phi_in, F, base, io);
+			if (DUAL) { tile_store(dphi_in, DF, base, io); }
 		}
-		__syncthreads();
 	}
 	const long part = (long)blockIdx.x * kTileWaves + (threadIdx.x >> 6);
 #pragma unroll
