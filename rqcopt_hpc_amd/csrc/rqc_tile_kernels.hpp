@@ -39,6 +39,7 @@
 #pragma once
 
 #include <cstdint>
+#include <type_traits>
 
 #include "zc.hpp"
 
@@ -539,21 +540,23 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 		cx<T>* DF = state(b, DUAL ? 3 : NS / 2);
 		const T* Fa = reinterpret_cast<const T*>(oarr ? DF : F);
 		const T* Sa = reinterpret_cast<const T*>(oarr ? D : S);
-		for (int j = P.ng - 1; j >= 0; j--) {
-			if (j + 1 < P.ng && ((P.barrier >> (j + 1)) & 1)) { __syncthreads(); }
-			const GateTerms gt = gterms[P.x0 + j];
+		// body of gate J (compile-time, so that its outer products accumulate in
+		// the registers acc[J] directly); dispatched once per gate on the
+		// wave-uniform gate index
+		auto gate_body = [&](auto jc) {
+			constexpr int J = decltype(jc)::value;
+			const GateTerms gt = gterms[P.x0 + J];
 			const GateLanes gl = gate_lanes(gt);
 			const int ewv = sel4(gt.wv, wave);
 			// recovery: [[G^-1, 0], [-G^-1 Z G^-1, G^-1]]; adjoint: [[G^T, 0], [Z^T, G^T]]
-			const cx<T>* mj = lm + j * NM * 16;
+			const cx<T>* mj = lm + J * NM * 16;
 			T au[4], aa[4];
 #pragma unroll
 			for (int kk = 0; kk < 4; kk++) {
 				au[kk] = a_operand<T, DUAL>(mj, mj + 16, kk);
 				aa[kk] = a_operand<T, DUAL>(mj + (DUAL ? 32 : 16), mj + 48, kk);
 			}
-			const bool adjoint = (j > 0) || !(flags & 8);
-			double4_t gsum = {0, 0, 0, 0};
+			const bool adjoint = (J > 0) || !(flags & 8);
 			for (int g = 0; g < qpw; g += GS * gq) {
 				int eg[GS], eg2[GS];
 #pragma unroll
@@ -585,33 +588,51 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 #pragma unroll
 					for (int t = 0; t < 4; t++) { bv[n][t] = Sa[2 * (eb ^ gl.eo[t]) + (m & 1)]; }
 				}
-				typename Vec4<T>::type grp[GS];
+				if constexpr (sizeof(T) == 8) {
+					// FP64: group 0 accumulates onto acc[J], the others from zero
+					double4_t grp[GS];
 #pragma unroll
-				for (int n = 0; n < GS; n++) { grp[n] = typename Vec4<T>::type{0, 0, 0, 0}; }
+					for (int n = 0; n < GS; n++) { grp[n] = (n == 0) ? acc[J] : double4_t{0, 0, 0, 0}; }
 #pragma unroll
-				for (int t = 0; t < 4; t++) {
+					for (int t = 0; t < 4; t++) {
 #pragma unroll
-					for (int n = 0; n < GS; n++) { grp[n] = mfma_step(av[n][t], bv[n][t], grp[n]); }
-				}
+						for (int n = 0; n < GS; n++) { grp[n] = mfma_step(av[n][t], bv[n][t], grp[n]); }
+					}
 #pragma unroll
-				for (int n = 0; n < GS; n++) {
+					for (int n = 1; n < GS; n++) { grp[0] += grp[n]; }
+					acc[J] = grp[0];
+				} else {
+					// FP32 steps per 16-quad group, added into the FP64 accumulator
+					typename Vec4<T>::type grp[GS];
 #pragma unroll
-					for (int v = 0; v < 4; v++) { gsum[v] += (double)grp[n][v]; }
+					for (int n = 0; n < GS; n++) { grp[n] = typename Vec4<T>::type{0, 0, 0, 0}; }
+#pragma unroll
+					for (int t = 0; t < 4; t++) {
+#pragma unroll
+						for (int n = 0; n < GS; n++) { grp[n] = mfma_step(av[n][t], bv[n][t], grp[n]); }
+					}
+#pragma unroll
+					for (int n = 0; n < GS; n++) {
+#pragma unroll
+						for (int v = 0; v < 4; v++) { acc[J][v] += (double)grp[n][v]; }
+					}
 				}
 				// adjoints at the gate input
 				if (adjoint) { gate_store<T, GS>(F, DF, eg, eg2, gl, oa); }
 			}
-			// accumulator of gate j (constant register indices in every case)
-			static_assert(kMaxTileGates == 8, "one case per gate of a pass");
+		};
+		static_assert(kMaxTileGates == 8, "one case per gate of a pass");
+		for (int j = P.ng - 1; j >= 0; j--) {
+			if (j + 1 < P.ng && ((P.barrier >> (j + 1)) & 1)) { __syncthreads(); }
 			switch (j) {
-			case 0: acc[0] += gsum; break;
-			case 1: acc[1] += gsum; break;
-			case 2: acc[2] += gsum; break;
-			case 3: acc[3] += gsum; break;
-			case 4: acc[4] += gsum; break;
-			case 5: acc[5] += gsum; break;
-			case 6: acc[6] += gsum; break;
-			default: acc[7] += gsum; break;
+			case 0: gate_body(std::integral_constant<int, 0>{}); break;
+			case 1: gate_body(std::integral_constant<int, 1>{}); break;
+			case 2: gate_body(std::integral_constant<int, 2>{}); break;
+			case 3: gate_body(std::integral_constant<int, 3>{}); break;
+			case 4: gate_body(std::integral_constant<int, 4>{}); break;
+			case 5: gate_body(std::integral_constant<int, 5>{}); break;
+			case 6: gate_body(std::integral_constant<int, 6>{}); break;
+			default: gate_body(std::integral_constant<int, 7>{}); break;
 			}
 		}
 		__syncthreads();
