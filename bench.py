@@ -27,11 +27,21 @@ sys.path.insert(0, ROOT)
 
 METRIC = "grad+HVP evals/s, L=14 brickwall d9, FP64 vs mixed; served p99 latency"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 REVERSE_KERNEL = {
-    "tile_pass": "k_pass_reverse<double,dual> (reverse tile pass: up to 4 gates per HBM pass, gradient/HVP sums on matrix cores)",
+    "tile_pass": "k_tile_reverse<double,dual> (reverse tile pass: up to 8 gates per HBM pass; state recovery, "
+                 "adjoint update and gradient/HVP outer products on the FP64 matrix cores)",
     "per_gate": "k_gate_reverse<double,dual> (per-gate reverse sweep with gradient/HVP sums)",
 }
-FP64_PEAK_TFLOPS = 78.6
+# measured HBM traffic of the reverse kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_k_tile_reverse_L14_d9.json")
+
+
+def reverse_alg_flops(L, d):
+    """Algorithmic FP64 flops of the reverse tile sweep of one eval: per gate and amplitude, the state
+    recovery (psi, dpsi: 12 complex MACs), the adjoint update (phi, dphi: 12) and the gradient / HVP
+    outer products (phi x psi, phi x dpsi + dphi x psi: 12), 8 flops per complex MAC: 288 flops."""
+    return 288.0 * (d * (L // 2)) * (1 << L) ** 2
 
 
 def canonical_bytes(L, d, s):
@@ -126,10 +136,18 @@ def run(args):
     r64, r32 = results["fp64"], results["mixed"]
     s = 16
     B = r64["block"]
-    # algorithmic bytes of every reverse launch (state arrays read + written once, summed by the
-    # library per launch) over the launches' HIP-event durations
+    # dominant kernel: the reverse sweep, compute-bound on the FP64 matrix cores.  achieved =
+    # algorithmic flops of all reverse launches over their HIP-event durations; its HBM rate (state
+    # arrays read + written once, summed by the library per launch) is reported beside it
     rev_ms, rev_n, rev_bytes = r64["prof"][R.KERNEL_REVERSE]
-    achieved = rev_bytes / (rev_ms * 1e-3) / 1e9
+    engine_used, passes = R.last_sweep()
+    rev_flops = reverse_alg_flops(L, d) if engine == "tile_pass" else None
+    achieved_tf = rev_flops / (rev_ms * 1e-3) / 1e12 if rev_flops else None
+    hbm_gbs = rev_bytes / (rev_ms * 1e-3) / 1e9
+    traffic = None
+    if engine == "tile_pass" and os.path.exists(TRAFFIC_FILE) and (L, d) == (14, 9):
+        with open(TRAFFIC_FILE) as fh:
+            traffic = json.load(fh)["k_tile_reverse_fp64"]["hbm_bytes_per_launch"]
     f64, g64, h64 = r64["out"]
     f32, g32, h32 = r32["out"]
     mixed_err = {"f_abs": float(abs(f32 - f64)), "f_abs_over_N": float(abs(f32 - f64) / N),
@@ -153,11 +171,15 @@ def run(args):
         "copy_bandwidth_gbs": copy_gbs,
         "config": {"workload": f"brickwall grad+HVP eval, L={L}, depth {d}, periodic, Ising-Trotter U (t=0.5, 4 Strang steps)",
                    "L": L, "depth": d, "gates": G, "column_block": B, "sweep_engine": engine, "device": info},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": REVERSE_KERNEL[engine],
-                     "launches": rev_n, "avg_launch_ms": rev_ms / max(rev_n, 1),
-                     "alg_bytes_per_launch": rev_bytes / max(rev_n, 1)},
+        "roofline": ({"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                      "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,
+                      "alg_flops_per_launch": rev_flops / max(rev_n, 1)}
+                     if achieved_tf else
+                     {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": None}) | {
+                     "kernel": REVERSE_KERNEL[engine], "launches": rev_n, "avg_launch_ms": rev_ms / max(rev_n, 1),
+                     "alg_bytes_per_launch": rev_bytes / max(rev_n, 1), "hbm_gbs": hbm_gbs,
+                     "hbm_frac": hbm_gbs / HBM_PEAK_GBS, "passes_per_sweep": passes},
         "eval_canonical": {"bytes": canonical_bytes(L, d, s), "flops": canonical_flops(L, d),
                            "hbm_frac": canonical_bytes(L, d, s) / r64["eval_s"] / 1e9 / HBM_PEAK_GBS,
                            "fp64_frac": canonical_flops(L, d) / r64["eval_s"] / 1e12 / FP64_PEAK_TFLOPS},
