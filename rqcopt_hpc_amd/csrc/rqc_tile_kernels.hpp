@@ -440,11 +440,21 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_forward(
 			else { tile_issue(state(b, 1), din, rbase * B + c0, io); }
 		}
 	};
+	// store tile t - 1 from buffer b ^ 1 (after the barrier that ends its gates); each
+	// thread stores and then prefetches the same slots, so its own LDS reads precede
+	// its asynchronous LDS writes and no further barrier is needed
+	auto store = [&](long tile, int b) {
+		const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
+		tile_store(out, state(b, 0), base, io);
+		if (DUAL) { tile_store(dout, state(b, 1), base, io); }
+	};
 	long tile = blockIdx.x;
 	if (tile < ntiles) { prefetch(tile, 0); }
-	for (int b = 0; tile < ntiles; tile += gridDim.x, b ^= 1) {
+	int b = 0;
+	for (; tile < ntiles; tile += gridDim.x, b ^= 1) {
 		__builtin_amdgcn_s_waitcnt(0);
 		__syncthreads();
+		if (tile != (long)blockIdx.x) { store(tile - gridDim.x, b ^ 1); }
 		if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, b ^ 1); }
 		cx<T>* S = state(b, 0);
 		cx<T>* D = state(b, DUAL ? 1 : 0);
@@ -467,10 +477,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_forward(
 				mfma_gate_apply<T, GS>(S, D, eg, eg2, gl, a);
 			}
 		}
+	}
+	if ((long)blockIdx.x < ntiles) {
 		__syncthreads();
-		const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
-		tile_store(out, S, base, io);
-		if (DUAL) { tile_store(dout, D, base, io); }
+		store(tile - gridDim.x, b ^ 1);
 	}
 }
 
@@ -528,11 +538,21 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 			else { tile_issue(state(b, 3), dphi_out, base, io); }
 		}
 	};
+	// phi (and dphi) of tile t - 1 from buffer b ^ 1, stored after the barrier that ends
+	// its gates, by the threads that then prefetch the same slots
+	auto store = [&](long tile, int b) {
+		if (flags & 8) { return; }
+		const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
+		tile_store(phi_in, state(b, NS / 2), base, io);
+		if (DUAL) { tile_store(dphi_in, state(b, 3), base, io); }
+	};
 	long tile = blockIdx.x;
 	if (tile < ntiles) { prefetch(tile, 0); }
-	for (int b = 0; tile < ntiles; tile += gridDim.x, b ^= 1) {
+	int b = 0;
+	for (; tile < ntiles; tile += gridDim.x, b ^= 1) {
 		__builtin_amdgcn_s_waitcnt(0);
 		__syncthreads();
+		if (tile != (long)blockIdx.x) { store(tile - gridDim.x, b ^ 1); }
 		if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, b ^ 1); }
 		cx<T>* S = state(b, 0);
 		cx<T>* D = state(b, DUAL ? 1 : 0);
@@ -635,12 +655,10 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 			default: gate_body(std::integral_constant<int, 7>{}); break;
 			}
 		}
+	}
+	if ((long)blockIdx.x < ntiles) {
 		__syncthreads();
-		if (!(flags & 8)) {
-			const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
-			tile_store(phi_in, F, base, io);
-			if (DUAL) { tile_store(dphi_in, DF, base, io); }
-		}
+		store(tile - gridDim.x, b ^ 1);
 	}
 	const long part = (long)blockIdx.x * kTileWaves + (threadIdx.x >> 6);
 #pragma unroll
