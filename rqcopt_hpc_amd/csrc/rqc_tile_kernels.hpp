@@ -518,6 +518,14 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 	double4_t acc[kMaxTileGates];
 #pragma unroll
 	for (int j = 0; j < kMaxTileGates; j++) { acc[j] = double4_t{0, 0, 0, 0}; }
+	// FP32 states: the last processed gate's tile sums (FP32 matrix-core chain over the
+	// wave's quads of the tile), added into its FP64 accumulator by the next gate
+	typename Vec4<T>::type pend = {0, 0, 0, 0};
+	auto flush32 = [&](double4_t& a) {
+#pragma unroll
+		for (int v = 0; v < 4; v++) { a[v] += (double)pend[v]; }
+		pend = typename Vec4<T>::type{0, 0, 0, 0};
+	};
 
 	const long cmask = (1L << P.lognc) - 1;
 	TileIO<T> io;
@@ -565,6 +573,10 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 		// wave-uniform gate index
 		auto gate_body = [&](auto jc) {
 			constexpr int J = decltype(jc)::value;
+			if constexpr (sizeof(T) == 4 && J + 1 < kMaxTileGates) {
+				// FP32: the previous gate (J + 1) left its tile sums in pend
+				if (J + 1 < P.ng) { flush32(acc[J + 1]); }
+			}
 			const GateTerms gt = gterms[P.x0 + J];
 			const GateLanes gl = gate_lanes(gt);
 			const int ewv = sel4(gt.wv, wave);
@@ -601,6 +613,8 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 				gate_mfma<T, GS>(au, os);
 				gate_mfma<T, GS>(aa, oa);
 				gate_store<T, GS>(S, D, eg, eg2, gl, os);
+				// adjoints at the gate input (their pre-update values are in av)
+				if (adjoint) { gate_store<T, GS>(F, DF, eg, eg2, gl, oa); }
 				// outer products over the groups' quads, 4 quads per step
 #pragma unroll
 				for (int n = 0; n < GS; n++) {
@@ -622,23 +636,14 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 					for (int n = 1; n < GS; n++) { grp[0] += grp[n]; }
 					acc[J] = grp[0];
 				} else {
-					// FP32 steps per 16-quad group, added into the FP64 accumulator
-					typename Vec4<T>::type grp[GS];
-#pragma unroll
-					for (int n = 0; n < GS; n++) { grp[n] = typename Vec4<T>::type{0, 0, 0, 0}; }
+					// FP32: one chain over the gate's quads of this tile into pend,
+					// added into the FP64 accumulator by the next gate (no wait here)
 #pragma unroll
 					for (int t = 0; t < 4; t++) {
 #pragma unroll
-						for (int n = 0; n < GS; n++) { grp[n] = mfma_step(av[n][t], bv[n][t], grp[n]); }
-					}
-#pragma unroll
-					for (int n = 0; n < GS; n++) {
-#pragma unroll
-						for (int v = 0; v < 4; v++) { acc[J][v] += (double)grp[n][v]; }
+						for (int n = 0; n < GS; n++) { pend = mfma_step(av[n][t], bv[n][t], pend); }
 					}
 				}
-				// adjoints at the gate input
-				if (adjoint) { gate_store<T, GS>(F, DF, eg, eg2, gl, oa); }
 			}
 		};
 		static_assert(kMaxTileGates == 8, "one case per gate of a pass");
@@ -655,6 +660,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 			default: gate_body(std::integral_constant<int, 7>{}); break;
 			}
 		}
+		if constexpr (sizeof(T) == 4) { flush32(acc[0]); }
 	}
 	if ((long)blockIdx.x < ntiles) {
 		__syncthreads();
