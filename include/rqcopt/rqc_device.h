@@ -54,6 +54,8 @@ int rqc_target_create_dense(int nqubits, const numeric* U, struct rqc_target** t
 int rqc_target_create_from_ufunc(linear_func ufunc, void* udata, int nqubits, struct rqc_target** target);
 /* U = brickwall circuit (e.g. a Trotter product of exact bond layers), formed on the GPU */
 int rqc_target_create_brickwall(int nqubits, const struct mat4x4 vlist[], int nlayers, const int* perms[], struct rqc_target** target);
+/* U = a general circuit of two-qubit gates (gates[k] on wires[2k], wires[2k+1]), formed on the GPU */
+int rqc_target_create_circuit(int nqubits, const struct mat4x4 gates[], int ngates, const int wires[], struct rqc_target** target);
 void rqc_target_destroy(struct rqc_target* target);
 int rqc_target_nqubits(const struct rqc_target* target);
 /* copy U back to a host buffer of 4^nqubits numerics */

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "rqc_internal.hpp"
+#include "rqc_eval.hpp"
 #include "rqc_kernels.hpp"
 #include "rqc_tile_kernels.hpp"
 #include "mat4.hpp"
@@ -83,24 +84,24 @@ Context& ctx()
 	return *c;
 }
 
-LaunchTimer::LaunchTimer(Context& cx, int kind, double bytes) : c(cx)
+LaunchTimer::LaunchTimer(Slot& sl, int kind, double bytes) : s(sl)
 {
-	if (!c.profile) { return; }
-	if (c.events_used == c.events.size()) {
-		Context::Ev e;
+	if (!s.profile) { return; }
+	if (s.events_used == s.events.size()) {
+		Slot::Ev e;
 		RQC_HIP(hipEventCreate(&e.a));
 		RQC_HIP(hipEventCreate(&e.b));
-		c.events.push_back(e);
+		s.events.push_back(e);
 	}
-	ev = &c.events[c.events_used++];
+	ev = &s.events[s.events_used++];
 	ev->kind = kind;
 	ev->bytes = bytes;
-	RQC_HIP(hipEventRecord(ev->a, c.stream));
+	RQC_HIP(hipEventRecord(ev->a, s.stream));
 }
 
 LaunchTimer::~LaunchTimer()
 {
-	if (ev != nullptr) { (void)hipEventRecord(ev->b, c.stream); }
+	if (ev != nullptr) { (void)hipEventRecord(ev->b, s.stream); }
 }
 
 void profile_flush(Context& c)
@@ -171,17 +172,7 @@ static inline unsigned reduce_grid(long nitems)
 	return (unsigned)std::max(1L, std::min(want, 4096L));
 }
 
-// a sequential circuit of two-qubit gates
-struct Circuit
-{
-	int L = 0;
-	int ngates = 0;
-	std::vector<zc> gates;  // ngates * 16
-	std::vector<zc> dirs;   // ngates * 16 or empty
-	std::vector<int> wi, wj;
-};
-
-static Circuit make_circuit(const struct mat4x4* gates, const struct mat4x4* dirs, int ngates, const int* wires, int L)
+Circuit make_circuit(const struct mat4x4* gates, const struct mat4x4* dirs, int ngates, const int* wires, int L)
 {
 	if (ngates < 1) { throw Error(-3, "a circuit needs at least one gate"); }
 	Circuit C;
@@ -202,7 +193,7 @@ static Circuit make_circuit(const struct mat4x4* gates, const struct mat4x4* dir
 }
 
 // expand a brickwall (reference src/target.c:443-458)
-static Circuit make_brickwall(const struct mat4x4* vlist, const struct mat4x4* vdirs, int nlayers, const int* const* perms, int L)
+Circuit make_brickwall(const struct mat4x4* vlist, const struct mat4x4* vdirs, int nlayers, const int* const* perms, int L)
 {
 	if (L < 2 || L % 2 != 0) { throw Error(-3, "brickwall circuits need an even number of qubits >= 2"); }
 	if (nlayers < 1) { throw Error(-3, "a brickwall circuit needs at least one layer"); }
@@ -228,8 +219,6 @@ static Circuit make_brickwall(const struct mat4x4* vlist, const struct mat4x4* v
 }
 
 // ============================================================== column-block sweeps
-enum class Mode { Target, Grad, Hvp };
-
 static int states_needed(Mode m, int G)
 {
 	switch (m) {
@@ -239,13 +228,14 @@ static int states_needed(Mode m, int G)
 	}
 }
 
-static long choose_block(Context& c, long N, int nstates, size_t elem)
+// largest power-of-two column block whose nstates state arrays fit the budget
+// (0 = 85% of the free device memory plus the `held` bytes the caller would release)
+static long choose_block(long N, int nstates, size_t elem, size_t budget, size_t held)
 {
-	size_t budget = c.ws_limit;
 	if (budget == 0) {
 		size_t fr = 0, tot = 0;
 		RQC_HIP(hipMemGetInfo(&fr, &tot));
-		budget = (size_t)(0.85 * (double)(fr + c.states.bytes));
+		budget = (size_t)(0.85 * (double)(fr + held));
 	}
 	long B = N;
 	while (B > 1 && (size_t)nstates * N * B * elem > budget) { B /= 2; }
@@ -256,70 +246,62 @@ static long choose_block(Context& c, long N, int nstates, size_t elem)
 	return B;
 }
 
-// d_acc[0] = f, d_acc[1 + k*32 + e]: e < 16 gradient of gate k, e >= 16 its HVP
+// per-gate engine: one gate per pass over the column block.
+// acc[0] = f, acc[1 + k*32 + e]: e < 16 gradient of gate k, e >= 16 its HVP
 template <typename T>
-static void sweep_targets(Context& c, const Circuit& C, const zc* dU, Mode mode, zc* d_acc, zc* d_partial)
+static void run_gates(Slot& s, const EvalPlan& pl, const Circuit& C, const zc* dU, zc* d_acc, zc* d_partial, cx<T>* S,
+	const int* skip)
 {
+	const Mode mode = pl.mode;
 	const int L = C.L, G = C.ngates;
-	const long N = 1L << L;
-	const int nst = states_needed(mode, G);
-	const long B = choose_block(c, N, nst, sizeof(cx<T>));
-	c.last_block = B;
-	c.last_engine = RQC_ENGINE_PER_GATE;
-	c.last_passes = G;
+	const long N = pl.N, B = pl.B;
 	const int logB = ilog2_or_neg(B);
 	const size_t sz = (size_t)N * B;
-	c.states.reserve((size_t)nst * sz * sizeof(cx<T>));
-	cx<T>* S = c.states.as<cx<T>>();
 	auto psi = [&](int k) { return S + (size_t)k * sz; };
 	auto dpsi = [&](int k) { return S + (size_t)(G + k) * sz; };            // k = 1 .. G
 	cx<T>* phi[2] = {S + (size_t)(mode == Mode::Hvp ? 2 * G + 1 : G + 1) * sz,
 	                 S + (size_t)(mode == Mode::Hvp ? 2 * G + 2 : G + 2) * sz};
 	cx<T>* dphi[2] = {S + (size_t)(2 * G + 3) * sz, S + (size_t)(2 * G + 4) * sz};
 
-	{
-		LaunchTimer lt(c, RQC_KERNEL_OTHER);
-		k_fill_zero<<<nblocks(1 + 32L * G), kBlock, 0, c.stream>>>(d_acc, 1 + 32L * G);
-	}
 	std::vector<Gate4<T>> Gt(G), Zt(G);
 	for (int k = 0; k < G; k++) {
 		Gt[k] = to_gate<T>(&C.gates[16 * (size_t)k]);
 		Zt[k] = (mode == Mode::Hvp) ? to_gate<T>(&C.dirs[16 * (size_t)k]) : Gate4<T>{};
 	}
 	const long nq = (N / 4) * B;
-	const unsigned grev = reduce_grid(nq);
-	const unsigned gtr = reduce_grid(N * B);
+	const unsigned grev = pl.grev_pg;
+	const unsigned gtr = pl.gtr;
 	const double ab = (double)sz * sizeof(cx<T>);   // bytes of one state block
 	const bool hvp = (mode == Mode::Hvp);
 
 	for (long b0 = 0; b0 < N; b0 += B) {
 		{
-			LaunchTimer lt(c, RQC_KERNEL_OTHER, ab);
-			k_identity_block<T><<<nblocks(N * B), kBlock, 0, c.stream>>>(psi(0), B, logB, b0, N * B);
+			LaunchTimer lt(s, RQC_KERNEL_OTHER, ab);
+			k_identity_block<T><<<nblocks(N * B), kBlock, 0, s.stream>>>(psi(0), B, logB, b0, N * B);
 		}
 		// forward sweep
 		for (int k = 0; k < G; k++) {
 			const Wires w = wires_of(C.wi[k], C.wj[k], L);
 			cx<T>* in = (mode == Mode::Target) ? psi(k % 2) : psi(k);
 			cx<T>* out = (mode == Mode::Target) ? psi((k + 1) % 2) : psi(k + 1);
-			LaunchTimer lt(c, RQC_KERNEL_FORWARD, hvp ? (3 + (k > 0)) * ab : 2 * ab);
+			LaunchTimer lt(s, RQC_KERNEL_FORWARD, hvp ? (3 + (k > 0)) * ab : 2 * ab);
 			if (mode == Mode::Hvp) {
-				k_gate_forward<T, true><<<nblocks(nq), kBlock, 0, c.stream>>>(in, out, k > 0 ? dpsi(k) : nullptr, dpsi(k + 1),
+				k_gate_forward<T, true><<<nblocks(nq), kBlock, 0, s.stream>>>(in, out, k > 0 ? dpsi(k) : nullptr, dpsi(k + 1),
 					Gt[k], Zt[k], w, B, logB, nq, k == 0 ? 1 : 0);
 			} else {
-				k_gate_forward<T, false><<<nblocks(nq), kBlock, 0, c.stream>>>(in, out, nullptr, nullptr,
+				k_gate_forward<T, false><<<nblocks(nq), kBlock, 0, s.stream>>>(in, out, nullptr, nullptr,
 					Gt[k], Gt[k], w, B, logB, nq, 0);
 			}
 		}
 		cx<T>* psiG = (mode == Mode::Target) ? psi(G % 2) : psi(G);
 		{
-			LaunchTimer lt(c, RQC_KERNEL_TARGET, (double)sz * sizeof(zc) + (mode == Mode::Target ? 1 : 2) * ab);
-			k_target_trace<T><<<gtr, kBlock, 0, c.stream>>>(dU, N, b0, psiG, mode == Mode::Target ? nullptr : phi[0],
-				B, logB, N * B, d_partial);
+			LaunchTimer lt(s, RQC_KERNEL_TARGET, (double)sz * sizeof(zc) + (mode == Mode::Target ? 1 : 2) * ab);
+			k_target_trace<T><<<gtr, kBlock, 0, s.stream>>>(dU, N, b0, psiG, mode == Mode::Target ? nullptr : phi[0],
+				B, logB, N * B, d_partial, skip);
 		}
 		{
-			LaunchTimer lt(c, RQC_KERNEL_OTHER);
-			k_reduce_partials<<<1, kBlock, 0, c.stream>>>(d_partial, (int)gtr, 1, d_acc);
+			LaunchTimer lt(s, RQC_KERNEL_OTHER);
+			k_reduce_partials<<<1, kBlock, 0, s.stream>>>(d_partial, (int)gtr, 1, d_acc, skip);
 		}
 		if (mode == Mode::Target) { continue; }
 		// reverse sweep
@@ -328,21 +310,21 @@ static void sweep_targets(Context& c, const Circuit& C, const zc* dU, Mode mode,
 			const Wires w = wires_of(C.wi[k], C.wj[k], L);
 			int flags = (k == 0) ? 4 : 0;
 			{
-				LaunchTimer lt(c, RQC_KERNEL_REVERSE, hvp ? (2 + (k > 0) + (k < G - 1) + (k > 0 ? 2 : 0)) * ab
+				LaunchTimer lt(s, RQC_KERNEL_REVERSE, hvp ? (2 + (k > 0) + (k < G - 1) + (k > 0 ? 2 : 0)) * ab
 				                                          : (2 + (k > 0)) * ab);
 				if (mode == Mode::Hvp) {
 					if (k == 0) { flags |= 1; }
 					if (k == G - 1) { flags |= 2; }
-					k_gate_reverse<T, true><<<grev, kBlock, 0, c.stream>>>(psi(k), k > 0 ? dpsi(k) : nullptr,
+					k_gate_reverse<T, true><<<grev, kBlock, 0, s.stream>>>(psi(k), k > 0 ? dpsi(k) : nullptr,
 						phi[cur], dphi[cur], phi[1 - cur], dphi[1 - cur], Gt[k], Zt[k], w, B, logB, nq, flags, d_partial);
 				} else {
-					k_gate_reverse<T, false><<<grev, kBlock, 0, c.stream>>>(psi(k), nullptr,
+					k_gate_reverse<T, false><<<grev, kBlock, 0, s.stream>>>(psi(k), nullptr,
 						phi[cur], nullptr, phi[1 - cur], nullptr, Gt[k], Gt[k], w, B, logB, nq, flags, d_partial);
 				}
 			}
 			{
-				LaunchTimer lt(c, RQC_KERNEL_OTHER);
-				k_reduce_partials<<<kWidth, kBlock, 0, c.stream>>>(d_partial, (int)grev, kWidth, d_acc + 1 + 32L * k);
+				LaunchTimer lt(s, RQC_KERNEL_OTHER);
+				k_reduce_partials<<<kWidth, kBlock, 0, s.stream>>>(d_partial, (int)grev, kWidth, d_acc + 1 + 32L * k, skip);
 			}
 			cur = 1 - cur;
 		}
@@ -534,7 +516,7 @@ static void allow_lds(KernelT* fn, size_t bytes)
 
 // gates whose inverse is their adjoint up to 1e-6 (the reverse tile pass
 // recovers earlier states with G^-1, which then has norm 1)
-static bool gates_near_unitary(const Circuit& C)
+bool gates_near_unitary(const Circuit& C)
 {
 	for (int k = 0; k < C.ngates; k++) {
 		const zc* g = &C.gates[16 * (size_t)k];
@@ -583,65 +565,143 @@ static int tile_gmax()
 	return g;
 }
 
-// true if the tile-pass engine handles this sweep (else the per-gate engine does)
-template <typename T>
-static bool tiles_apply(const Context& c, const Circuit& C, Mode mode)
+// tile-pass part of a plan: passes, their kernel arguments and the gate
+// position terms (all fixed by the circuit structure and the column block)
+struct TilePlan
 {
-	if (c.engine == RQC_ENGINE_PER_GATE || C.L > 30) { return false; }
-	if (!tile_shape<T>(C.L, 1L << C.L).usable()) { return false; }
-	return mode == Mode::Target || gates_near_unitary(C);
-}
-
-template <typename T>
-static void sweep_tiles(Context& c, const Circuit& C, const zc* dU, Mode mode, zc* d_acc, zc* d_partial, size_t partial_doubles)
-{
-	const int L = C.L, G = C.ngates;
-	const long N = 1L << L;
-	const bool dual = (mode == Mode::Hvp);
-	TileShape ts = tile_shape<T>(L, N);
-	const std::vector<Pass> passes = plan_passes(C, ts.K, tile_gmax());
-	const int P = (int)passes.size();
-	const int nst = (mode == Mode::Target) ? 2 : (mode == Mode::Grad) ? P + 2 : 2 * P + 4;
-	const long B = choose_block(c, N, nst, sizeof(cx<T>));
-	ts = tile_shape<T>(L, B);
-	if (!ts.usable()) {
-		sweep_targets<T>(c, C, dU, mode, d_acc, d_partial);
-		return;
-	}
-	c.last_block = B;
-	c.last_engine = RQC_ENGINE_TILE_PASS;
-	c.last_passes = P;
-	const size_t sz = (size_t)N * B;
-	c.states.reserve((size_t)nst * sz * sizeof(cx<T>));
-	cx<T>* S = c.states.as<cx<T>>();
-	// checkpoints after pass p - 1, p = 1 .. P: psi at PS(p), dpsi at DPS(p)
-	auto PS = [&](int p) { return S + (size_t)(p - 1) * sz; };
-	auto DPS = [&](int p) { return S + (size_t)(P + p - 1) * sz; };
-	cx<T>* buf[2] = {S, S + sz};
-	const size_t nck = dual ? 2 * (size_t)P : (mode == Mode::Grad ? (size_t)P : 0);
-	cx<T>* phi[2] = {S + nck * sz, S + (nck + 1) * sz};
-	cx<T>* dphi[2] = {S + (nck + 2) * sz, S + (nck + 3) * sz};
-
-	// operand tables of the gates in execution order and the pass arguments
-	std::vector<cx<T>> hm((size_t)G * kMats * 16);
-	std::vector<GateTerms> hterms((size_t)G);
+	TileShape ts;
+	std::vector<Pass> passes;
 	std::vector<TileArgs> args;
 	std::vector<TileGates> gks;
-	int x = 0;
+	std::vector<GateTerms> terms;   // per gate, in execution order
+	std::vector<int> order;         // circuit gate index of each execution slot
+	unsigned gfwd = 0, grev = 0;
+	size_t lds_fwd = 0, lds_rev = 0;
+	size_t partial_doubles = 0;
+};
+
+template <typename T>
+static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C, EvalPlan& pl, const std::vector<Pass>& passes,
+	const TileShape& ts)
+{
+	const bool dual = (pl.mode == Mode::Hvp);
+	auto tp = std::make_shared<TilePlan>();
+	tp->ts = ts;
+	tp->passes = passes;
+	tp->terms.resize((size_t)C.ngates);
+	int x = 0, maxng = 1;
 	for (const Pass& p : passes) {
-		args.push_back(make_tile_args(C, p, ts.K, ts.logC, ilog2_or_neg(B >> ts.logC), x, (int)sizeof(cx<T>), hterms.data()));
+		tp->args.push_back(make_tile_args(C, p, ts.K, ts.logC, ilog2_or_neg(pl.B >> ts.logC), x, (int)sizeof(cx<T>),
+			tp->terms.data()));
 		TileGates tg;
 		std::memset(&tg, 0, sizeof(tg));
 		for (size_t j = 0; j < p.gates.size(); j++) {
-			const int k = p.gates[j];
-			tg.k[j] = k;
+			tg.k[j] = p.gates[j];
+			tp->order.push_back(p.gates[j]);
+			x++;
+		}
+		tp->gks.push_back(tg);
+		maxng = std::max(maxng, (int)p.gates.size());
+	}
+	const int E = 1 << (ts.K + ts.logC);
+	const long ntiles = (pl.N >> ts.K) * (pl.B >> ts.logC);
+	// two buffers of (psi[, dpsi]) forward and (psi, [dpsi,] phi[, dphi]) reverse, then
+	// the pass's gate matrices (forward G[, Z]; reverse G^-1, [W,] G^T[, Z^T])
+	const size_t mat_bytes = (size_t)kMaxTileGates * 16 * sizeof(cx<T>);
+	tp->lds_fwd = 2 * (size_t)(dual ? 2 : 1) * (E * sizeof(cx<T>) + kLdsPadBytes) + (dual ? 2 : 1) * mat_bytes;
+	tp->lds_rev = 2 * (size_t)(dual ? 4 : 2) * (E * sizeof(cx<T>) + kLdsPadBytes) + (dual ? 4 : 2) * mat_bytes;
+	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
+	auto* kr = dual ? k_tile_reverse<T, true> : k_tile_reverse<T, false>;
+	allow_lds(kf, tp->lds_fwd);
+	allow_lds(kr, tp->lds_rev);
+	int occ_f = 0, occ_r = 0;
+	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, reinterpret_cast<const void*>(kf), kTileThreads, tp->lds_fwd));
+	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, reinterpret_cast<const void*>(kr), kTileThreads, tp->lds_rev));
+	tp->gfwd = (unsigned)std::min<long>(ntiles, (long)std::max(1, occ_f) * dev.cu_count);
+	tp->grev = (unsigned)std::max<long>(1, std::min<long>(ntiles, (long)std::max(1, occ_r) * dev.cu_count));
+	tp->partial_doubles = (pl.mode == Mode::Target) ? 0 : (size_t)tp->grev * kTileWaves * maxng * 256;
+	return tp;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, bool fp32, bool project, bool unitary,
+	size_t budget, size_t held)
+{
+	if (per < 1 || C.ngates % per != 0) { throw Error(-3, "gate groups must divide the gate count"); }
+	if (mode == Mode::Hvp && C.dirs.size() != C.gates.size()) { throw Error(-3, "an HVP evaluation needs gate directions"); }
+	EvalPlan pl;
+	pl.mode = mode;
+	pl.fp32 = fp32;
+	pl.project = project;
+	pl.L = C.L;
+	pl.G = C.ngates;
+	pl.per = per;
+	pl.ng = C.ngates / per;
+	pl.N = 1L << C.L;
+	const size_t elem = fp32 ? sizeof(cx<float>) : sizeof(cx<double>);
+	// tile-pass engine when a tile fits the circuit (and, for reverse sweeps, the gates are unitary)
+	bool tiles = dev.engine == RQC_ENGINE_TILE_PASS && C.L <= 30 && (mode == Mode::Target || unitary);
+	TileShape ts{0, 0};
+	std::vector<Pass> passes;
+	if (tiles) {
+		ts = fp32 ? tile_shape<float>(C.L, pl.N) : tile_shape<double>(C.L, pl.N);
+		tiles = ts.usable();
+	}
+	if (tiles) {
+		passes = plan_passes(C, ts.K, tile_gmax());
+		const int P = (int)passes.size();
+		pl.nst = (mode == Mode::Target) ? 2 : (mode == Mode::Grad) ? P + 2 : 2 * P + 4;
+		pl.B = choose_block(pl.N, pl.nst, elem, budget, held);
+		ts = fp32 ? tile_shape<float>(C.L, pl.B) : tile_shape<double>(C.L, pl.B);
+		tiles = ts.usable();
+	}
+	if (tiles) {
+		pl.tiles = true;
+		pl.passes = (int)passes.size();
+		pl.tp = fp32 ? plan_tiles<float>(dev, C, pl, passes, ts) : plan_tiles<double>(dev, C, pl, passes, ts);
+	} else {
+		pl.nst = states_needed(mode, C.ngates);
+		pl.B = choose_block(pl.N, pl.nst, elem, budget, held);
+		pl.passes = C.ngates;
+		pl.grev_pg = reduce_grid((pl.N / 4) * pl.B);
+	}
+	pl.gtr = reduce_grid(pl.N * pl.B);
+	pl.n_acc = 1 + 32 * (size_t)pl.G;
+	size_t npart = pl.gtr;
+	if (pl.tiles) { npart = std::max(npart, (pl.tp->partial_doubles + 1) / 2); }
+	else { npart = std::max(npart, (size_t)pl.grev_pg * kWidth); }
+	pl.n_part = npart;
+	if (pl.tiles) {
+		pl.mats_bytes = align256((size_t)pl.G * kMats * 16 * elem);
+		pl.terms_bytes = align256((size_t)pl.G * sizeof(GateTerms));
+	}
+	pl.tab_bytes = pl.mats_bytes + pl.terms_bytes + 2 * 16 * (size_t)pl.ng * sizeof(zc);
+	pl.off_acc = 0;
+	pl.off_part = align256(pl.n_acc * sizeof(zc));
+	pl.off_tab = pl.off_part + align256(pl.n_part * sizeof(zc));
+	pl.off_res = pl.off_tab + align256(pl.tab_bytes);
+	pl.off_states = pl.off_res + align256(pl.res_bytes());
+	pl.bytes = pl.off_states + (size_t)pl.nst * pl.N * pl.B * elem;
+	return pl;
+}
+
+template <typename T>
+static void fill_tables(const EvalPlan& pl, const Circuit& C, char* host)
+{
+	const bool dual = (pl.mode == Mode::Hvp);
+	if (pl.tiles) {
+		const TilePlan& tp = *pl.tp;
+		cx<T>* hm = reinterpret_cast<cx<T>*>(host);
+		for (size_t x = 0; x < tp.order.size(); x++) {
+			const int k = tp.order[x];
 			// G, Z, G^-1, -G^-1 Z G^-1, G^T, Z^T (kMats matrices, zero where unused)
 			const zc* g = &C.gates[16 * (size_t)k];
 			zc m6[kMats][16];
 			std::memset(m6, 0, sizeof(m6));
 			std::memcpy(m6[kMatG], g, sizeof(m6[0]));
 			if (dual) { std::memcpy(m6[kMatZ], &C.dirs[16 * (size_t)k], sizeof(m6[0])); }
-			if (mode != Mode::Target) {
+			if (pl.mode != Mode::Target) {
 				if (m4_inverse(g, m6[kMatGinv]) != 0) { throw Error(-4, "singular gate in the tile-pass sweep"); }
 				m4_transpose(g, m6[kMatGT]);
 				if (dual) {
@@ -654,46 +714,52 @@ static void sweep_tiles(Context& c, const Circuit& C, const zc* dU, Mode mode, z
 			}
 			for (int w = 0; w < kMats; w++) {
 				for (int e = 0; e < 16; e++) {
-					hm[((size_t)x * kMats + w) * 16 + e] = cx<T>{(T)m6[w][e].x, (T)m6[w][e].y};
+					hm[(x * kMats + w) * 16 + e] = cx<T>{(T)m6[w][e].x, (T)m6[w][e].y};
 				}
 			}
-			x++;
 		}
-		gks.push_back(tg);
+		std::memcpy(host + pl.mats_bytes, tp.terms.data(), tp.terms.size() * sizeof(GateTerms));
 	}
-	// device table: gate matrices, then the gates' position terms
-	const size_t mbytes = (hm.size() * sizeof(cx<T>) + 255) & ~(size_t)255;
-	c.gtab.reserve(mbytes + hterms.size() * sizeof(GateTerms));
-	cx<T>* mats = c.gtab.as<cx<T>>();
-	GateTerms* gterms = reinterpret_cast<GateTerms*>(reinterpret_cast<char*>(mats) + mbytes);
-	RQC_HIP(hipMemcpyAsync(mats, hm.data(), hm.size() * sizeof(cx<T>), hipMemcpyHostToDevice, c.stream));
-	RQC_HIP(hipMemcpyAsync(gterms, hterms.data(), hterms.size() * sizeof(GateTerms), hipMemcpyHostToDevice, c.stream));
+	// gates of each group (its first gate) and directions, for the projection
+	zc* vin = reinterpret_cast<zc*>(host + pl.mats_bytes + pl.terms_bytes);
+	for (int l = 0; l < pl.ng; l++) {
+		std::memcpy(&vin[16 * (size_t)l], &C.gates[16 * (size_t)l * pl.per], 16 * sizeof(zc));
+		if (dual) { std::memcpy(&vin[16 * ((size_t)pl.ng + l)], &C.dirs[16 * (size_t)l * pl.per], 16 * sizeof(zc)); }
+		else { std::memset(&vin[16 * ((size_t)pl.ng + l)], 0, 16 * sizeof(zc)); }
+	}
+}
 
-	const int E = 1 << (ts.K + ts.logC);
-	const long ntiles = (N >> ts.K) * (B >> ts.logC);
-	// two buffers of (psi[, dpsi]) forward and (psi, [dpsi,] phi[, dphi]) reverse, then
-	// the pass's gate matrices (forward G[, Z]; reverse G^-1, [W,] G^T[, Z^T])
-	const size_t mat_bytes = (size_t)kMaxTileGates * 16 * sizeof(cx<T>);
-	const size_t lds_fwd = 2 * (size_t)(dual ? 2 : 1) * (E * sizeof(cx<T>) + kLdsPadBytes) + (dual ? 2 : 1) * mat_bytes;
-	const size_t lds_rev = 2 * (size_t)(dual ? 4 : 2) * (E * sizeof(cx<T>) + kLdsPadBytes) + (dual ? 4 : 2) * mat_bytes;
+void fill_eval_tables(const EvalPlan& pl, const Circuit& C, char* host)
+{
+	std::memset(host, 0, pl.tab_bytes);
+	if (pl.fp32) { fill_tables<float>(pl, C, host); }
+	else { fill_tables<double>(pl, C, host); }
+}
+
+// tile-pass engine: several gates per HBM pass through LDS tiles
+template <typename T>
+static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* d_partial, const cx<T>* mats,
+	const GateTerms* gterms, cx<T>* S, const int* skip)
+{
+	const Mode mode = pl.mode;
+	const TilePlan& tp = *pl.tp;
+	const long N = pl.N, B = pl.B;
+	const bool dual = (mode == Mode::Hvp);
+	const int P = pl.passes;
+	const size_t sz = (size_t)N * B;
+	// checkpoints after pass p - 1, p = 1 .. P: psi at PS(p), dpsi at DPS(p)
+	auto PS = [&](int p) { return S + (size_t)(p - 1) * sz; };
+	auto DPS = [&](int p) { return S + (size_t)(P + p - 1) * sz; };
+	cx<T>* buf[2] = {S, S + sz};
+	const size_t nck = dual ? 2 * (size_t)P : (mode == Mode::Grad ? (size_t)P : 0);
+	cx<T>* phi[2] = {S + nck * sz, S + (nck + 1) * sz};
+	cx<T>* dphi[2] = {S + (nck + 2) * sz, S + (nck + 3) * sz};
+	const long ntiles = (N >> tp.ts.K) * (B >> tp.ts.logC);
 	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
 	auto* kr = dual ? k_tile_reverse<T, true> : k_tile_reverse<T, false>;
-	allow_lds(kf, lds_fwd);
-	allow_lds(kr, lds_rev);
-	int occ_f = 0, occ_r = 0;
-	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, reinterpret_cast<const void*>(kf), kTileThreads, lds_fwd));
-	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, reinterpret_cast<const void*>(kr), kTileThreads, lds_rev));
-	const unsigned gfwd = (unsigned)std::min<long>(ntiles, (long)std::max(1, occ_f) * c.cu_count);
-	const long rev_cap = (long)(partial_doubles / ((size_t)kTileWaves * kMaxTileGates * 256));
-	const unsigned grev = (unsigned)std::max<long>(1, std::min<long>({ntiles, (long)std::max(1, occ_r) * c.cu_count, rev_cap}));
 	double* mpart = reinterpret_cast<double*>(d_partial);   // grev * waves * ng * 256 doubles
 	zc* tpart = d_partial;                                  // trace partials (reused, sequential)
-
-	{
-		LaunchTimer lt(c, RQC_KERNEL_OTHER);
-		k_fill_zero<<<nblocks(1 + 32L * G), kBlock, 0, c.stream>>>(d_acc, 1 + 32L * G);
-	}
-	const unsigned gtr = reduce_grid(N * B);
+	const unsigned gtr = pl.gtr;
 	const int logB = ilog2_or_neg(B);
 	const double ab = (double)sz * sizeof(cx<T>);   // bytes of one state block
 	const int ns = dual ? 2 : 1;                     // state arrays per side
@@ -703,19 +769,19 @@ static void sweep_tiles(Context& c, const Circuit& C, const zc* dU, Mode mode, z
 			const int flags = (p == 0) ? 3 : 0;
 			const cx<T>* in = (p == 0) ? nullptr : (mode == Mode::Target ? buf[(p + 1) % 2] : PS(p));
 			cx<T>* out = (mode == Mode::Target) ? buf[p % 2] : PS(p + 1);
-			LaunchTimer lt(c, RQC_KERNEL_FORWARD, (double)(ns * (p > 0) + ns) * ab);
-			kf<<<gfwd, kTileThreads, lds_fwd, c.stream>>>(in, out, (dual && p > 0) ? DPS(p) : nullptr,
-				dual ? DPS(p + 1) : nullptr, args[p], mats, gterms, B, b0, ntiles, flags);
+			LaunchTimer lt(s, RQC_KERNEL_FORWARD, (double)(ns * (p > 0) + ns) * ab);
+			kf<<<tp.gfwd, kTileThreads, tp.lds_fwd, s.stream>>>(in, out, (dual && p > 0) ? DPS(p) : nullptr,
+				dual ? DPS(p + 1) : nullptr, tp.args[p], mats, gterms, B, b0, ntiles, flags, skip);
 		}
 		cx<T>* psiG = (mode == Mode::Target) ? buf[(P - 1) % 2] : PS(P);
 		{
-			LaunchTimer lt(c, RQC_KERNEL_TARGET, (double)sz * sizeof(zc) + (mode == Mode::Target ? 1 : 2) * ab);
-			k_target_trace<T><<<gtr, kBlock, 0, c.stream>>>(dU, N, b0, psiG, mode == Mode::Target ? nullptr : phi[0],
-				B, logB, N * B, tpart);
+			LaunchTimer lt(s, RQC_KERNEL_TARGET, (double)sz * sizeof(zc) + (mode == Mode::Target ? 1 : 2) * ab);
+			k_target_trace<T><<<gtr, kBlock, 0, s.stream>>>(dU, N, b0, psiG, mode == Mode::Target ? nullptr : phi[0],
+				B, logB, N * B, tpart, skip);
 		}
 		{
-			LaunchTimer lt(c, RQC_KERNEL_OTHER);
-			k_reduce_partials<<<1, kBlock, 0, c.stream>>>(tpart, (int)gtr, 1, d_acc);
+			LaunchTimer lt(s, RQC_KERNEL_OTHER);
+			k_reduce_partials<<<1, kBlock, 0, s.stream>>>(tpart, (int)gtr, 1, d_acc, skip);
 		}
 		if (mode == Mode::Target) { continue; }
 		int cur = 0;
@@ -723,17 +789,18 @@ static void sweep_tiles(Context& c, const Circuit& C, const zc* dU, Mode mode, z
 			int flags = 0;
 			if (p == P - 1) { flags |= 4; }
 			if (p == 0) { flags |= 8; }
-			const int ng = (int)passes[p].gates.size();
+			const int ng = (int)tp.passes[p].gates.size();
 			{
 				const int reads = dual ? 3 + (p < P - 1) : 2;
-				LaunchTimer lt(c, RQC_KERNEL_REVERSE, (double)(reads + ns * (p > 0)) * ab);
-				kr<<<grev, kTileThreads, lds_rev, c.stream>>>(PS(p + 1), dual ? DPS(p + 1) : nullptr, phi[cur],
-					dual ? dphi[cur] : nullptr, phi[1 - cur], dual ? dphi[1 - cur] : nullptr, args[p], mats, gterms, B, ntiles,
-					flags, mpart);
+				LaunchTimer lt(s, RQC_KERNEL_REVERSE, (double)(reads + ns * (p > 0)) * ab);
+				kr<<<tp.grev, kTileThreads, tp.lds_rev, s.stream>>>(PS(p + 1), dual ? DPS(p + 1) : nullptr, phi[cur],
+					dual ? dphi[cur] : nullptr, phi[1 - cur], dual ? dphi[1 - cur] : nullptr, tp.args[p], mats, gterms, B,
+					ntiles, flags, mpart, skip);
 			}
 			{
-				LaunchTimer lt(c, RQC_KERNEL_OTHER);
-				k_tile_reduce<<<ng, 256, 0, c.stream>>>(mpart, (long)grev * kTileWaves, ng, gks[p], dual ? 1 : 0, d_acc);
+				LaunchTimer lt(s, RQC_KERNEL_OTHER);
+				k_tile_reduce<<<ng, 256, 0, s.stream>>>(mpart, (long)tp.grev * kTileWaves, ng, tp.gks[p], dual ? 1 : 0, d_acc,
+					skip);
 			}
 			cur = 1 - cur;
 		}
@@ -741,13 +808,19 @@ static void sweep_tiles(Context& c, const Circuit& C, const zc* dU, Mode mode, z
 }
 
 // per-group sums of per-gate gradients / HVPs and the Riemannian projection
-// (reference src/target.c:526-535 and :650-671), one thread per group
+// (reference src/target.c:526-535 and :650-671), one thread per group.
+// res: f, dv[16 ng], hz[16 ng] (complex), grad[16 ng], hvp[16 ng] (double)
 __global__ void k_finalize(const zc* __restrict__ acc, int ngroups, int per, const zc* __restrict__ vlist,
-	const zc* __restrict__ vdirs, zc* __restrict__ dv_out, zc* __restrict__ hz_out, double* __restrict__ grad,
-	double* __restrict__ hvp, int project, int want_hvp)
+	const zc* __restrict__ vdirs, zc* __restrict__ res, int want_grad, int project, int want_hvp, const int* skip)
 {
+	if (skipped(skip)) { return; }
 	const int l = blockIdx.x * blockDim.x + threadIdx.x;
-	if (l >= ngroups) { return; }
+	if (l == 0) { res[0] = acc[0]; }
+	if (!want_grad || l >= ngroups) { return; }
+	zc* dv_out = res + 1;
+	zc* hz_out = dv_out + 16L * ngroups;
+	double* grad = reinterpret_cast<double*>(hz_out + 16L * ngroups);
+	double* hvp = grad + 16L * ngroups;
 	zc dv[16], hz[16];
 	for (int e = 0; e < 16; e++) { dv[e] = {0, 0}; hz[e] = {0, 0}; }
 	for (int k = 0; k < per; k++) {
@@ -761,71 +834,77 @@ __global__ void k_finalize(const zc* __restrict__ acc, int ngroups, int per, con
 	}
 }
 
-// results of one target evaluation, on the host
-struct TargetResult
+void enqueue_eval(const Context& dev, Slot& s, const EvalPlan& pl, const Circuit& C, const zc* dU, char* work,
+	char* tab, char* res_dev, const char* host_tab, const int* skip)
 {
-	zc f = {0, 0};
-	std::vector<zc> dv, hz;      // ngroups * 16
-	std::vector<double> grad, hvp;  // ngroups * 16
-};
+	(void)dev;
+	if (skip != nullptr && !pl.tiles) { throw Error(-3, "skip flags need the tile-pass engine"); }
+	zc* acc = reinterpret_cast<zc*>(work + pl.off_acc);
+	zc* part = reinterpret_cast<zc*>(work + pl.off_part);
+	if (tab == nullptr) { tab = work + pl.off_tab; }
+	const zc* vin = reinterpret_cast<const zc*>(tab + pl.mats_bytes + pl.terms_bytes);
+	zc* res = reinterpret_cast<zc*>(res_dev != nullptr ? res_dev : work + pl.off_res);
+	if (host_tab != nullptr) {
+		RQC_HIP(hipMemcpyAsync(tab, host_tab, pl.tab_bytes, hipMemcpyHostToDevice, s.stream));
+	}
+	s.last_block = pl.B;
+	s.last_engine = pl.tiles ? RQC_ENGINE_TILE_PASS : RQC_ENGINE_PER_GATE;
+	s.last_passes = pl.passes;
+	{
+		LaunchTimer lt(s, RQC_KERNEL_OTHER);
+		k_fill_zero<<<nblocks((long)pl.n_acc), kBlock, 0, s.stream>>>(acc, (long)pl.n_acc, skip);
+	}
+	const GateTerms* gterms = reinterpret_cast<const GateTerms*>(tab + pl.mats_bytes);
+	if (pl.fp32) {
+		cx<float>* S = reinterpret_cast<cx<float>*>(work + pl.off_states);
+		if (pl.tiles) { run_tiles<float>(s, pl, dU, acc, part, reinterpret_cast<const cx<float>*>(tab), gterms, S, skip); }
+		else { run_gates<float>(s, pl, C, dU, acc, part, S, skip); }
+	} else {
+		cx<double>* S = reinterpret_cast<cx<double>*>(work + pl.off_states);
+		if (pl.tiles) { run_tiles<double>(s, pl, dU, acc, part, reinterpret_cast<const cx<double>*>(tab), gterms, S, skip); }
+		else { run_gates<double>(s, pl, C, dU, acc, part, S, skip); }
+	}
+	LaunchTimer lt(s, RQC_KERNEL_OTHER);
+	k_finalize<<<std::max(1, (pl.ng + 63) / 64), 64, 0, s.stream>>>(acc, pl.ng, pl.per, vin, vin + 16 * (size_t)pl.ng, res,
+		pl.mode != Mode::Target ? 1 : 0, pl.project ? 1 : 0, pl.mode == Mode::Hvp ? 1 : 0, skip);
+}
+
+void unpack_result(const EvalPlan& pl, const char* res, EvalResult& R)
+{
+	const size_t n = 16 * (size_t)pl.ng;
+	const zc* z = reinterpret_cast<const zc*>(res);
+	R.f = z[0];
+	R.dv.assign(z + 1, z + 1 + n);
+	R.hz.assign(z + 1 + n, z + 1 + 2 * n);
+	const double* d = reinterpret_cast<const double*>(z + 1 + 2 * n);
+	R.grad.assign(d, d + n);
+	R.hvp.assign(d + n, d + 2 * n);
+}
 
 // dense U for a target evaluation: device-resident or assembled from the host callback
 static const zc* resolve_target(Context& c, linear_func ufunc, void* udata, int L);
 
-static TargetResult eval_targets(const Circuit& C, linear_func ufunc, void* udata, Mode mode, int per, bool project)
+// one evaluation on the main slot (the reference ABI's target functions)
+static EvalResult eval_targets(const Circuit& C, linear_func ufunc, void* udata, Mode mode, int per, bool project)
 {
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
 	const zc* dU = resolve_target(c, ufunc, udata, C.L);
-	const int G = C.ngates, ng = G / per;
-	// small buffer: acc | partial | vlist | vdirs | outputs
-	const size_t n_acc = 1 + 32 * (size_t)G;
-	// partial sums: per-gate reverse (4096 x 32 complex) or tile-pass matrix-core partials
-	// (up to 4 x CUs workgroups x waves x gates x 256 doubles), whichever is larger
-	const size_t n_part = std::max<size_t>(4096 * (size_t)kWidth,
-		(size_t)4 * c.cu_count * kTileWaves * kMaxTileGates * 256 / 2);
-	const size_t n_in = 2 * 16 * (size_t)ng;
-	const size_t n_out = 2 * 16 * (size_t)ng;
-	const size_t n_real = 2 * 16 * (size_t)ng;
-	c.small.reserve((n_acc + n_part + n_in + n_out) * sizeof(zc) + n_real * sizeof(double));
-	zc* acc = c.small.as<zc>();
-	zc* part = acc + n_acc;
-	zc* vin = part + n_part;
-	zc* out = vin + n_in;
-	double* rout = reinterpret_cast<double*>(out + n_out);
-	if (c.precision == RQC_PRECISION_MIXED) {
-		if (tiles_apply<float>(c, C, mode)) { sweep_tiles<float>(c, C, dU, mode, acc, part, 2 * n_part); }
-		else { sweep_targets<float>(c, C, dU, mode, acc, part); }
-	} else {
-		if (tiles_apply<double>(c, C, mode)) { sweep_tiles<double>(c, C, dU, mode, acc, part, 2 * n_part); }
-		else { sweep_targets<double>(c, C, dU, mode, acc, part); }
-	}
-	// gates of each group (the first gate of the group) and directions
-	std::vector<zc> hv(n_in);
-	for (int l = 0; l < ng; l++) {
-		std::memcpy(&hv[16 * (size_t)l], &C.gates[16 * (size_t)l * per], 16 * sizeof(zc));
-		if (mode == Mode::Hvp) { std::memcpy(&hv[16 * ((size_t)ng + l)], &C.dirs[16 * (size_t)l * per], 16 * sizeof(zc)); }
-	}
-	RQC_HIP(hipMemcpyAsync(vin, hv.data(), n_in * sizeof(zc), hipMemcpyHostToDevice, c.stream));
-	if (mode != Mode::Target) {
-		LaunchTimer lt(c, RQC_KERNEL_OTHER);
-		k_finalize<<<(ng + 63) / 64, 64, 0, c.stream>>>(acc, ng, per, vin, vin + 16 * (size_t)ng, out, out + 16 * (size_t)ng,
-			rout, rout + 16 * (size_t)ng, project ? 1 : 0, mode == Mode::Hvp ? 1 : 0);
-	}
-	TargetResult R;
-	R.dv.resize(16 * (size_t)ng);
-	R.hz.resize(16 * (size_t)ng);
-	R.grad.resize(16 * (size_t)ng);
-	R.hvp.resize(16 * (size_t)ng);
-	RQC_HIP(hipMemcpyAsync(&R.f, acc, sizeof(zc), hipMemcpyDeviceToHost, c.stream));
-	if (mode != Mode::Target) {
-		RQC_HIP(hipMemcpyAsync(R.dv.data(), out, 16 * (size_t)ng * sizeof(zc), hipMemcpyDeviceToHost, c.stream));
-		RQC_HIP(hipMemcpyAsync(R.hz.data(), out + 16 * (size_t)ng, 16 * (size_t)ng * sizeof(zc), hipMemcpyDeviceToHost, c.stream));
-		RQC_HIP(hipMemcpyAsync(R.grad.data(), rout, 16 * (size_t)ng * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-		RQC_HIP(hipMemcpyAsync(R.hvp.data(), rout + 16 * (size_t)ng, 16 * (size_t)ng * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-	}
+	const bool unitary = (mode == Mode::Target) || gates_near_unitary(C);
+	const EvalPlan pl = plan_eval(c, C, mode, per, c.precision == RQC_PRECISION_MIXED, project, unitary, c.ws_limit,
+		c.work.bytes);
+	std::vector<char> tab(pl.tab_bytes);
+	fill_eval_tables(pl, C, tab.data());
+	c.work.reserve(pl.bytes);
+	char* work = c.work.as<char>();
+	enqueue_eval(c, c, pl, C, dU, work, nullptr, nullptr, tab.data(), nullptr);
+	std::vector<char> res(pl.res_bytes());
+	const size_t nres = (mode == Mode::Target) ? sizeof(zc) : res.size();
+	RQC_HIP(hipMemcpyAsync(res.data(), work + pl.off_res, nres, hipMemcpyDeviceToHost, c.stream));
 	RQC_HIP(hipStreamSynchronize(c.stream));
 	RQC_HIP(hipGetLastError());
+	EvalResult R;
+	unpack_result(pl, res.data(), R);
 	return R;
 }
 
@@ -854,6 +933,8 @@ static const rqc_target* as_target(void* udata, int L)
 	}
 	return t;
 }
+
+const zc* device_target_matrix(void* udata, int L) { return as_target(udata, L)->d_U; }
 
 static const zc* resolve_target(Context& c, linear_func ufunc, void* udata, int L)
 {
@@ -982,7 +1063,7 @@ static void launch_reverse(Context& c, const zc* G, const zc* Z, int i, int j, i
 		}
 	}
 	LaunchTimer lt(c, RQC_KERNEL_OTHER);
-	k_reduce_partials<<<kWidth, kBlock, 0, c.stream>>>(part, (int)g, kWidth, acc);
+	k_reduce_partials<<<kWidth, kBlock, 0, c.stream>>>(part, (int)g, kWidth, acc, nullptr);
 }
 
 static zc* small_acc(Context& c, size_t n_acc)
@@ -990,7 +1071,7 @@ static zc* small_acc(Context& c, size_t n_acc)
 	c.small.reserve((n_acc + 4096 * (size_t)kWidth) * sizeof(zc));
 	zc* acc = c.small.as<zc>();
 	LaunchTimer lt(c, RQC_KERNEL_OTHER);
-	k_fill_zero<<<nblocks((long)n_acc), kBlock, 0, c.stream>>>(acc, (long)n_acc);
+	k_fill_zero<<<nblocks((long)n_acc), kBlock, 0, c.stream>>>(acc, (long)n_acc, nullptr);
 	return acc;
 }
 
@@ -1128,7 +1209,7 @@ extern "C" void apply_gate_to_array(const struct mat4x4* gate, const int i, cons
 }
 
 // ============================================================== C ABI: quantum_circuit.h
-static int circuit_error(const char* where, const std::exception& e)
+int rqc::report_error(const char* where, const std::exception& e)
 {
 	fprintf(stderr, "librqcopt_hip: %s failed: %s\n", where, e.what());
 	if (auto* r = dynamic_cast<const Error*>(&e)) { return r->code; }
@@ -1159,7 +1240,7 @@ extern "C" int apply_quantum_circuit(const struct mat4x4 gates[], const int ngat
 		}
 		s.download();
 		return 0;
-	} catch (const std::exception& e) { return circuit_error("apply_quantum_circuit", e); }
+	} catch (const std::exception& e) { return report_error("apply_quantum_circuit", e); }
 }
 
 extern "C" int allocate_quantum_circuit_cache(const int nqubits, const int ngates, struct quantum_circuit_cache* cache)
@@ -1220,7 +1301,7 @@ extern "C" int quantum_circuit_forward(const struct mat4x4 gates[], const int ng
 		copy_out(c, psi_out->data, S + ngates * n, bytes);
 		s.download();
 		return 0;
-	} catch (const std::exception& e) { return circuit_error("quantum_circuit_forward", e); }
+	} catch (const std::exception& e) { return report_error("quantum_circuit_forward", e); }
 }
 
 extern "C" int quantum_circuit_backward(const struct mat4x4 gates[], const int ngates, const int wires[], const struct quantum_circuit_cache* cache,
@@ -1259,7 +1340,7 @@ extern "C" int quantum_circuit_backward(const struct mat4x4 gates[], const int n
 		s.download();
 		for (int k = 0; k < ngates; k++) { std::memcpy(dgates[k].data, &hacc[32 * (size_t)k], 16 * sizeof(zc)); }
 		return 0;
-	} catch (const std::exception& e) { return circuit_error("quantum_circuit_backward", e); }
+	} catch (const std::exception& e) { return report_error("quantum_circuit_backward", e); }
 }
 
 extern "C" int quantum_circuit_gates_hessian_vector_product(const struct mat4x4 gates[], const struct mat4x4 gatedirs[], const int ngates, const int wires[],
@@ -1309,7 +1390,7 @@ extern "C" int quantum_circuit_gates_hessian_vector_product(const struct mat4x4 
 			std::memcpy(hess_gatedirs[k].data, &hacc[32 * (size_t)k + 16], 16 * sizeof(zc));
 		}
 		return 0;
-	} catch (const std::exception& e) { return circuit_error("quantum_circuit_gates_hessian_vector_product", e); }
+	} catch (const std::exception& e) { return report_error("quantum_circuit_gates_hessian_vector_product", e); }
 }
 
 // ============================================================== C ABI: brickwall_circuit.h
@@ -1321,7 +1402,7 @@ extern "C" int apply_brickwall_unitary(const struct mat4x4 vlist[], const int nl
 		std::vector<int> w(2 * (size_t)C.ngates);
 		for (int k = 0; k < C.ngates; k++) { w[2*k] = C.wi[k]; w[2*k + 1] = C.wj[k]; }
 		return apply_quantum_circuit(reinterpret_cast<const struct mat4x4*>(C.gates.data()), C.ngates, w.data(), psi, psi_out);
-	} catch (const std::exception& e) { return circuit_error("apply_brickwall_unitary", e); }
+	} catch (const std::exception& e) { return report_error("apply_brickwall_unitary", e); }
 }
 
 extern "C" int brickwall_unitary_forward(const struct mat4x4 vlist[], int nlayers, const int* perms[],
@@ -1332,7 +1413,7 @@ extern "C" int brickwall_unitary_forward(const struct mat4x4 vlist[], int nlayer
 		std::vector<int> w(2 * (size_t)C.ngates);
 		for (int k = 0; k < C.ngates; k++) { w[2*k] = C.wi[k]; w[2*k + 1] = C.wj[k]; }
 		return quantum_circuit_forward(reinterpret_cast<const struct mat4x4*>(C.gates.data()), C.ngates, w.data(), psi, cache, psi_out);
-	} catch (const std::exception& e) { return circuit_error("brickwall_unitary_forward", e); }
+	} catch (const std::exception& e) { return report_error("brickwall_unitary_forward", e); }
 }
 
 extern "C" int brickwall_unitary_backward(const struct mat4x4 vlist[], int nlayers, const int* perms[], const struct quantum_circuit_cache* cache,
@@ -1358,7 +1439,7 @@ extern "C" int brickwall_unitary_backward(const struct mat4x4 vlist[], int nlaye
 			std::memcpy(dvlist[i].data, s, sizeof(s));
 		}
 		return 0;
-	} catch (const std::exception& e) { return circuit_error("brickwall_unitary_backward", e); }
+	} catch (const std::exception& e) { return report_error("brickwall_unitary_backward", e); }
 }
 
 // ============================================================== C ABI: target.h
@@ -1376,7 +1457,7 @@ static int target_call(const char* where, const std::function<void()>& body)
 	}
 }
 
-static void put_f(numeric* fval, const TargetResult& R) { std::memcpy(fval, &R.f, sizeof(zc)); }
+static void put_f(numeric* fval, const EvalResult& R) { std::memcpy(fval, &R.f, sizeof(zc)); }
 
 extern "C" int circuit_unitary_target(linear_func ufunc, void* udata,
 	const struct mat4x4 gates[], const int ngates, const int wires[], const int nqubits, numeric* fval)
@@ -1392,7 +1473,7 @@ extern "C" int circuit_unitary_target_and_gradient(linear_func ufunc, void* udat
 {
 	return target_call("circuit_unitary_target_and_gradient", [&] {
 		Circuit C = make_circuit(gates, nullptr, ngates, wires, nqubits);
-		TargetResult R = eval_targets(C, ufunc, udata, Mode::Grad, 1, false);
+		EvalResult R = eval_targets(C, ufunc, udata, Mode::Grad, 1, false);
 		put_f(fval, R);
 		std::memcpy(dgates, R.dv.data(), R.dv.size() * sizeof(zc));
 	});
@@ -1403,7 +1484,7 @@ extern "C" int circuit_unitary_target_and_projected_gradient(linear_func ufunc, 
 {
 	return target_call("circuit_unitary_target_and_projected_gradient", [&] {
 		Circuit C = make_circuit(gates, nullptr, ngates, wires, nqubits);
-		TargetResult R = eval_targets(C, ufunc, udata, Mode::Grad, 1, true);
+		EvalResult R = eval_targets(C, ufunc, udata, Mode::Grad, 1, true);
 		put_f(fval, R);
 		std::memcpy(grad_vec, R.grad.data(), R.grad.size() * sizeof(double));
 	});
@@ -1415,7 +1496,7 @@ extern "C" int circuit_unitary_target_hessian_vector_product(linear_func ufunc, 
 {
 	return target_call("circuit_unitary_target_hessian_vector_product", [&] {
 		Circuit C = make_circuit(gates, gatedirs, ngates, wires, nqubits);
-		TargetResult R = eval_targets(C, ufunc, udata, Mode::Hvp, 1, false);
+		EvalResult R = eval_targets(C, ufunc, udata, Mode::Hvp, 1, false);
 		put_f(fval, R);
 		std::memcpy(dgates, R.dv.data(), R.dv.size() * sizeof(zc));
 		std::memcpy(hess_gatedirs, R.hz.data(), R.hz.size() * sizeof(zc));
@@ -1428,7 +1509,7 @@ extern "C" int circuit_unitary_target_projected_hessian_vector_product(linear_fu
 {
 	return target_call("circuit_unitary_target_projected_hessian_vector_product", [&] {
 		Circuit C = make_circuit(gates, gatedirs, ngates, wires, nqubits);
-		TargetResult R = eval_targets(C, ufunc, udata, Mode::Hvp, 1, true);
+		EvalResult R = eval_targets(C, ufunc, udata, Mode::Hvp, 1, true);
 		put_f(fval, R);
 		std::memcpy(grad_vec, R.grad.data(), R.grad.size() * sizeof(double));
 		std::memcpy(hvp_vec, R.hvp.data(), R.hvp.size() * sizeof(double));
@@ -1449,7 +1530,7 @@ extern "C" int brickwall_unitary_target_and_gradient(linear_func ufunc, void* ud
 {
 	return target_call("brickwall_unitary_target_and_gradient", [&] {
 		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
-		TargetResult R = eval_targets(C, ufunc, udata, Mode::Grad, nqubits / 2, false);
+		EvalResult R = eval_targets(C, ufunc, udata, Mode::Grad, nqubits / 2, false);
 		put_f(fval, R);
 		std::memcpy(dvlist, R.dv.data(), R.dv.size() * sizeof(zc));
 	});
@@ -1460,7 +1541,7 @@ extern "C" int brickwall_unitary_target_and_projected_gradient(linear_func ufunc
 {
 	return target_call("brickwall_unitary_target_and_projected_gradient", [&] {
 		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
-		TargetResult R = eval_targets(C, ufunc, udata, Mode::Grad, nqubits / 2, true);
+		EvalResult R = eval_targets(C, ufunc, udata, Mode::Grad, nqubits / 2, true);
 		put_f(fval, R);
 		std::memcpy(grad_vec, R.grad.data(), R.grad.size() * sizeof(double));
 	});
@@ -1472,7 +1553,7 @@ extern "C" int brickwall_unitary_target_hessian_vector_product(linear_func ufunc
 {
 	return target_call("brickwall_unitary_target_hessian_vector_product", [&] {
 		Circuit C = make_brickwall(vlist, vdirs, nlayers, perms, nqubits);
-		TargetResult R = eval_targets(C, ufunc, udata, Mode::Hvp, nqubits / 2, false);
+		EvalResult R = eval_targets(C, ufunc, udata, Mode::Hvp, nqubits / 2, false);
 		put_f(fval, R);
 		std::memcpy(dvlist, R.dv.data(), R.dv.size() * sizeof(zc));
 		std::memcpy(hess_vdirs, R.hz.data(), R.hz.size() * sizeof(zc));
@@ -1485,7 +1566,7 @@ extern "C" int brickwall_unitary_target_projected_hessian_vector_product(linear_
 {
 	return target_call("brickwall_unitary_target_projected_hessian_vector_product", [&] {
 		Circuit C = make_brickwall(vlist, vdirs, nlayers, perms, nqubits);
-		TargetResult R = eval_targets(C, ufunc, udata, Mode::Hvp, nqubits / 2, true);
+		EvalResult R = eval_targets(C, ufunc, udata, Mode::Hvp, nqubits / 2, true);
 		put_f(fval, R);
 		std::memcpy(grad_vec, R.grad.data(), R.grad.size() * sizeof(double));
 		std::memcpy(hvp_vec, R.hvp.data(), R.hvp.size() * sizeof(double));
@@ -1512,6 +1593,8 @@ extern "C" int rqc_device_ufunc(const struct statevector* RQC_RESTRICT psi, void
 		s.download();
 	});
 }
+
+extern "C" void rqc_target_destroy(struct rqc_target* target);
 
 static rqc_target* new_target(int L)
 {
@@ -1556,28 +1639,45 @@ extern "C" int rqc_target_create_from_ufunc(linear_func ufunc, void* udata, int 
 	});
 }
 
-extern "C" int rqc_target_create_brickwall(int nqubits, const struct mat4x4 vlist[], int nlayers, const int* perms[], struct rqc_target** target)
+// U = C, formed column block by column block with the per-gate forward kernel
+static rqc_target* target_from_circuit(const Circuit& C)
 {
-	return target_call("rqc_target_create_brickwall", [&] {
-		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
-		Context& c = ctx();
-		std::lock_guard<std::recursive_mutex> lk(c.mu);
-		rqc_target* t = new_target(nqubits);
-		const long N = 1L << nqubits;
-		const long B = choose_block(c, N, 2, sizeof(zc));
-		c.states.reserve(2 * (size_t)N * B * sizeof(zc));
-		zc* S = c.states.as<zc>();
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	rqc_target* t = new_target(C.L);
+	try {
+		const long N = 1L << C.L;
+		const long B = choose_block(N, 2, sizeof(zc), c.ws_limit, c.work.bytes);
+		c.work.reserve(2 * (size_t)N * B * sizeof(zc));
+		zc* S = c.work.as<zc>();
 		for (long b0 = 0; b0 < N; b0 += B) {
 			zc* buf[2] = {S, S + (size_t)N * B};
 			k_identity_block<double><<<nblocks(N * B), kBlock, 0, c.stream>>>(buf[0], B, ilog2_or_neg(B), b0, N * B);
 			for (int k = 0; k < C.ngates; k++) {
-				launch_forward(c, &C.gates[16 * (size_t)k], C.wi[k], C.wj[k], nqubits, B, buf[k % 2], buf[(k + 1) % 2]);
+				launch_forward(c, &C.gates[16 * (size_t)k], C.wi[k], C.wj[k], C.L, B, buf[k % 2], buf[(k + 1) % 2]);
 			}
 			k_store_columns<<<nblocks(N * B), kBlock, 0, c.stream>>>(buf[C.ngates % 2], t->d_U, N, B, b0);
 		}
 		RQC_HIP(hipStreamSynchronize(c.stream));
 		RQC_HIP(hipGetLastError());
-		*target = t;
+	} catch (...) {
+		rqc_target_destroy(t);
+		throw;
+	}
+	return t;
+}
+
+extern "C" int rqc_target_create_brickwall(int nqubits, const struct mat4x4 vlist[], int nlayers, const int* perms[], struct rqc_target** target)
+{
+	return target_call("rqc_target_create_brickwall", [&] {
+		*target = target_from_circuit(make_brickwall(vlist, nullptr, nlayers, perms, nqubits));
+	});
+}
+
+extern "C" int rqc_target_create_circuit(int nqubits, const struct mat4x4 gates[], int ngates, const int wires[], struct rqc_target** target)
+{
+	return target_call("rqc_target_create_circuit", [&] {
+		*target = target_from_circuit(make_circuit(gates, nullptr, ngates, wires, nqubits));
 	});
 }
 
@@ -1630,7 +1730,8 @@ extern "C" void rqc_release_workspace(void)
 		Context& c = ctx();
 		std::lock_guard<std::recursive_mutex> lk(c.mu);
 		(void)hipStreamSynchronize(c.stream);
-		c.states.release();
+		c.work.release();
+		c.small.release();
 		c.vecs.release();
 		c.ubuf.release();
 	} catch (...) {}
@@ -1691,11 +1792,11 @@ extern "C" int rqc_measure_copy_bandwidth(size_t bytes, int reps, double* gbps)
 		Context& c = ctx();
 		std::lock_guard<std::recursive_mutex> lk(c.mu);
 		const long n = (long)(bytes / 16);
-		c.states.reserve(2 * (size_t)n * 16);
-		zc* a = c.states.as<zc>();
+		c.work.reserve(2 * (size_t)n * 16);
+		zc* a = c.work.as<zc>();
 		zc* b = a + n;
 		const unsigned grid = (unsigned)std::min<long>((n + kBlock - 1) / kBlock, 8L * c.cu_count * 8);
-		k_fill_zero<<<nblocks(n), kBlock, 0, c.stream>>>(a, n);
+		k_fill_zero<<<nblocks(n), kBlock, 0, c.stream>>>(a, n, nullptr);
 		hipEvent_t e0, e1;
 		RQC_HIP(hipEventCreate(&e0));
 		RQC_HIP(hipEventCreate(&e1));

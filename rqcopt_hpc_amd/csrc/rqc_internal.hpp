@@ -43,10 +43,27 @@ struct DeviceBuffer
 	template <typename T> T* as() const { return static_cast<T*>(ptr); }
 };
 
-struct Context
+// An execution slot: one HIP stream and the bookkeeping of the evaluations
+// enqueued on it.  Evaluations on different slots share nothing on the device
+// and may run concurrently.
+struct Slot
+{
+	hipStream_t stream = nullptr;
+	long last_block = 0;   // column-block width of the last target evaluation
+	int last_engine = -1;  // sweep engine of the last target evaluation
+	int last_passes = 0;   // HBM passes per sweep direction of the last target evaluation
+
+	// kernel timing (HIP events on this slot's stream)
+	bool profile = false;
+	struct Ev { hipEvent_t a, b; int kind; double bytes; };
+	std::vector<Ev> events;
+	size_t events_used = 0;
+};
+
+// the device, its settings and the main slot, which serves the reference ABI
+struct Context : Slot
 {
 	int device = -1;
-	hipStream_t stream = nullptr;
 	int cu_count = 0;
 	size_t hbm_bytes = 0;
 	std::string name;
@@ -54,21 +71,12 @@ struct Context
 	int engine = RQC_ENGINE_TILE_PASS;
 	size_t lds_per_block = 65536;
 	size_t ws_limit = 0;  // 0 = automatic
-	long last_block = 0;
 
-	DeviceBuffer states;   // column-block statevectors (checkpoints, adjoints)
-	DeviceBuffer small;    // partial sums, accumulators, gates, outputs
+	DeviceBuffer work;     // workspace of target evaluations (states, partials, tables, results)
+	DeviceBuffer small;    // accumulators of the statevector-level API
 	DeviceBuffer vecs;     // uploaded statevectors of the statevector-level API
 	DeviceBuffer ubuf;     // U assembled from a host callback
-	DeviceBuffer gtab;     // gate operand tables of the current sweep (tile-pass engine)
-	int last_engine = -1;  // sweep engine of the last target evaluation
-	int last_passes = 0;   // HBM passes per sweep direction of the last target evaluation
 
-	// kernel timing
-	bool profile = false;
-	struct Ev { hipEvent_t a, b; int kind; double bytes; };
-	std::vector<Ev> events;
-	size_t events_used = 0;
 	double prof_ms[RQC_KERNEL_KINDS] = {0, 0, 0, 0};
 	long prof_n[RQC_KERNEL_KINDS] = {0, 0, 0, 0};
 	double prof_bytes[RQC_KERNEL_KINDS] = {0, 0, 0, 0};   // algorithmic HBM bytes of the timed launches
@@ -79,17 +87,17 @@ struct Context
 // the process-wide context on the current HIP device; throws if no GPU is present
 Context& ctx();
 
-// bracket a kernel launch with timing events when profiling is enabled; `bytes` is the
+// bracket a kernel launch with timing events when the slot profiles; `bytes` is the
 // launch's algorithmic HBM traffic (arrays it must read and write once), summed per kind
 struct LaunchTimer
 {
-	Context& c;
-	Context::Ev* ev = nullptr;
-	LaunchTimer(Context& cx, int kind, double bytes = 0);
+	Slot& s;
+	Slot::Ev* ev = nullptr;
+	LaunchTimer(Slot& sl, int kind, double bytes = 0);
 	~LaunchTimer();
 };
 
-// flush pending timing events into the per-kind totals
+// flush the main slot's pending timing events into the per-kind totals
 void profile_flush(Context& c);
 
 // true if p is a HIP device allocation (or managed memory usable by kernels)

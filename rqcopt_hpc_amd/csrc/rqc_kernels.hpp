@@ -19,6 +19,10 @@
 namespace rqc {
 
 constexpr int kBlock = 256;    // threads per workgroup of the sweep kernels
+
+// skip flag of a batched launch (a device word read when the kernel starts):
+// the whole grid returns at once when it is set
+__device__ __forceinline__ bool skipped(const int* skip) { return skip != nullptr && *skip != 0; }
 constexpr int kWidth = 32;     // partial-sum width of the reverse kernel (16 grad + 16 hess)
 
 template <typename T>
@@ -242,8 +246,9 @@ __global__ __launch_bounds__(kBlock) void k_gate_reverse(
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_target_trace(
 	const zc* __restrict__ U, long ldu, long b0, const cx<T>* __restrict__ psi, cx<T>* __restrict__ phi,
-	long B, int logB, long nitems, zc* __restrict__ partial)
+	long B, int logB, long nitems, zc* __restrict__ partial, const int* skip)
 {
+	if (skipped(skip)) { return; }
 	zc acc = {0, 0};
 	const long stride = (long)gridDim.x * kBlock;
 	for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < nitems; t += stride) {
@@ -260,8 +265,9 @@ __global__ __launch_bounds__(kBlock) void k_target_trace(
 
 // acc[e] += sum_p partial[p * width + e] in a fixed order; one workgroup per e
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const zc* __restrict__ partial, int nparts, int width,
-	zc* __restrict__ acc)
+	zc* __restrict__ acc, const int* skip)
 {
+	if (skipped(skip)) { return; }
 	const int e = blockIdx.x;
 	zc s = {0, 0};
 	for (int p = threadIdx.x; p < nparts; p += kBlock) { s += partial[(long)p * width + e]; }
@@ -284,8 +290,9 @@ __global__ __launch_bounds__(kBlock) void k_identity_block(cx<T>* __restrict__ X
 	X[t] = cx<T>{(a == b0 + c) ? (T)1 : (T)0, (T)0};
 }
 
-__global__ __launch_bounds__(kBlock) void k_fill_zero(zc* __restrict__ p, long n)
+__global__ __launch_bounds__(kBlock) void k_fill_zero(zc* __restrict__ p, long n, const int* skip)
 {
+	if (skipped(skip)) { return; }
 	const long t = (long)blockIdx.x * kBlock + threadIdx.x;
 	if (t < n) { p[t] = {0, 0}; }
 }

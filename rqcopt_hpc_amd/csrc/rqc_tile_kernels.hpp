@@ -412,8 +412,9 @@ template <typename T, bool DUAL>
 __global__ __launch_bounds__(kTileThreads) void k_tile_forward(
 	const cx<T>* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __restrict__ din, cx<T>* __restrict__ dout,
 	const TileArgs P, const cx<T>* __restrict__ mats, const GateTerms* __restrict__ gterms, long B, long b0, long ntiles,
-	int flags)
+	int flags, const int* skip)
 {
+	if (skipped(skip)) { return; }
 	extern __shared__ __align__(16) char lds_raw[];
 	constexpr int NS = DUAL ? 2 : 1;              // states per buffer
 	constexpr int GS = DUAL ? 2 : 1;              // matrix-core groups per step
@@ -497,8 +498,9 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 	const cx<T>* __restrict__ phi_out, const cx<T>* __restrict__ dphi_out,
 	cx<T>* __restrict__ phi_in, cx<T>* __restrict__ dphi_in,
 	const TileArgs P, const cx<T>* __restrict__ mats, const GateTerms* __restrict__ gterms, long B, long ntiles, int flags,
-	double* __restrict__ partial)
+	double* __restrict__ partial, const int* skip)
 {
+	if (skipped(skip)) { return; }
 	extern __shared__ __align__(16) char lds_raw[];
 	constexpr int NS = DUAL ? 4 : 2;              // states per buffer: psi, [dpsi], phi, [dphi]
 	constexpr int GS = DUAL ? 2 : 1;
@@ -689,8 +691,9 @@ struct TileGates
 //   dual:  grad = phi (x) psi,  hess = phi (x) dpsi + dphi (x) psi
 //   plain: grad = phi (x) psi summed over the two quads of each operand pair
 __global__ __launch_bounds__(256) void k_tile_reduce(const double* __restrict__ partial, long nparts, int ng,
-	const TileGates gk, int dual, zc* __restrict__ acc)
+	const TileGates gk, int dual, zc* __restrict__ acc, const int* skip)
 {
+	if (skipped(skip)) { return; }
 	__shared__ double Dm[16][16];
 	const int j = blockIdx.x, i = threadIdx.x;
 	double s = 0;
