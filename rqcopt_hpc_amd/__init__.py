@@ -31,6 +31,8 @@ PRECISION_FP64 = 0
 PRECISION_MIXED = 1
 KERNEL_FORWARD, KERNEL_REVERSE, KERNEL_TARGET, KERNEL_OTHER = range(4)
 ENGINE_TILE_PASS, ENGINE_PER_GATE = 0, 1
+EVAL_NONE, EVAL_TARGET, EVAL_GRADIENT, EVAL_HVP = -1, 0, 1, 2
+BATCH_GRAPH, BATCH_EAGER = 0, 1
 
 
 class RqcError(RuntimeError):
@@ -113,7 +115,10 @@ EXPORTED = [
     "rqc_target_destroy", "rqc_target_nqubits", "rqc_target_download", "rqc_set_precision", "rqc_get_precision",
     "rqc_set_workspace_limit", "rqc_release_workspace", "rqc_last_column_block", "rqc_profile_enable",
     "rqc_profile_reset", "rqc_profile_query", "rqc_synchronize", "rqc_device_info", "rqc_measure_copy_bandwidth",
-    "rqc_set_sweep_engine", "rqc_last_sweep",
+    "rqc_set_sweep_engine", "rqc_last_sweep", "rqc_target_create_circuit",
+    # rqc_batch.h
+    "rqc_batch_create", "rqc_batch_destroy", "rqc_batch_set_exec", "rqc_batch_evaluate",
+    "rqc_batch_optimize_brickwall_hvp", "rqc_batch_get_stats", "rqc_batch_problem_plan",
 ]
 
 
@@ -231,6 +236,15 @@ class DeviceTarget:
         _check("rqc_target_create_brickwall",
                lib().rqc_target_create_brickwall(perms.shape[1], _p(vlist), len(vlist), rows, ct.byref(h)))
         return DeviceTarget(h.value, perms.shape[1])
+
+    @staticmethod
+    def circuit(gates, wires, nqubits):
+        """U = product of two-qubit gates (gates[k] on wires[k]) formed on the GPU."""
+        gates, wires = _gates(gates), _wires(wires)
+        h = ct.c_void_p()
+        _check("rqc_target_create_circuit",
+               lib().rqc_target_create_circuit(nqubits, _p(gates), len(gates), _p(wires), ct.byref(h)))
+        return DeviceTarget(h.value, nqubits)
 
     def download(self):
         n = 1 << self.nqubits
@@ -548,6 +562,96 @@ def truncated_cg_hvp(grad, hvp_fn, radius, params=None):
                                    ct.POINTER(TruncatedCgParams), ct.c_void_p]
     on = L.truncated_cg_hvp(None, _p(grad), cfun, None, n, radius, ct.byref(params), _p(z))
     return z, bool(on)
+
+
+# --------------------------------------------------------------- batched optimisation (rqc_batch.h)
+class _BatchProblem(ct.Structure):
+    _fields_ = [("nqubits", ct.c_int), ("nlayers", ct.c_int), ("perms", ct.POINTER(ct.POINTER(ct.c_int))),
+                ("target", ct.c_void_p)]
+
+
+class BatchStats(ct.Structure):
+    _fields_ = [("steps", ct.c_long), ("evaluations", ct.c_long), ("graph_launches", ct.c_long),
+                ("device_ms", ct.c_double), ("workspace_bytes", ct.c_size_t)]
+
+
+class Batch:
+    """Many independent brickwall problems evaluated and optimised together on the GPU.
+
+    problems: list of (perms [nlayers, L] int array, DeviceTarget).  Each step
+    evaluates every problem whose kind is not EVAL_NONE (one graph launch per
+    evaluation kind); optimize() runs one trust-region optimisation per problem
+    in lock-step."""
+
+    def __init__(self, problems, precision=PRECISION_FP64, nstreams=0, workspace_limit=0):
+        self._keep = []
+        arr = (_BatchProblem * len(problems))()
+        self.nlayers = []
+        for i, (perms, target) in enumerate(problems):
+            perms, rows = _perms(perms)
+            self._keep += [perms, rows, target]
+            arr[i] = _BatchProblem(perms.shape[1], perms.shape[0], rows, target.handle.value)
+            self.nlayers.append(perms.shape[0])
+        self._keep.append(arr)
+        self.n = len(problems)
+        h = ct.c_void_p()
+        L = lib()
+        L.rqc_batch_create.argtypes = [ct.c_int, ct.c_void_p, ct.c_int, ct.c_int, ct.c_size_t, ct.c_void_p]
+        _check("rqc_batch_create", L.rqc_batch_create(self.n, ct.cast(arr, ct.c_void_p), precision, nstreams,
+                                                      workspace_limit, ct.byref(h)))
+        self.handle = h
+
+    def set_exec(self, mode):
+        _check("rqc_batch_set_exec", lib().rqc_batch_set_exec(self.handle, mode))
+
+    @staticmethod
+    def _ptrs(arrs):
+        return (ct.c_void_p * len(arrs))(*[None if a is None else a.ctypes.data for a in arrs])
+
+    def evaluate(self, kinds, vlists, vdirs=None):
+        """Returns (f [n] complex, grads, hvps): lists of 16*nlayers arrays (None where not computed)."""
+        kinds = np.ascontiguousarray(kinds, np.int32)
+        vl = [_gates(v) for v in vlists]
+        vd = [None if (vdirs is None or vdirs[i] is None) else _gates(vdirs[i]) for i in range(self.n)]
+        f = np.zeros(self.n, np.complex128)
+        grads = [np.full(16 * self.nlayers[i], np.nan) if kinds[i] >= EVAL_GRADIENT else None for i in range(self.n)]
+        hvps = [np.full(16 * self.nlayers[i], np.nan) if kinds[i] == EVAL_HVP else None for i in range(self.n)]
+        _check("rqc_batch_evaluate", lib().rqc_batch_evaluate(self.handle, _p(kinds), self._ptrs(vl), self._ptrs(vd),
+                                                              _p(f), self._ptrs(grads), self._ptrs(hvps)))
+        return f, grads, hvps
+
+    def optimize(self, vlist_starts, niter, params=None):
+        """Lock-step optimize_brickwall_circuit_hvp of every problem: (f_iters list, vlist_opt list)."""
+        vl = [_gates(v) for v in vlist_starts]
+        params = rtr_default_params(16 * self.nlayers[0]) if params is None else params
+        f_iters = [np.empty(niter + 1) for _ in range(self.n)]
+        vopt = [np.empty_like(v) for v in vl]
+        _check("rqc_batch_optimize_brickwall_hvp",
+               lib().rqc_batch_optimize_brickwall_hvp(self.handle, self._ptrs(vl), ct.byref(params), niter,
+                                                      self._ptrs(f_iters), self._ptrs(vopt)))
+        return f_iters, vopt
+
+    def stats(self):
+        st = BatchStats()
+        _check("rqc_batch_get_stats", lib().rqc_batch_get_stats(self.handle, ct.byref(st)))
+        return {k: getattr(st, k) for k, _ in BatchStats._fields_}
+
+    def plan(self, p, kind):
+        b, e, n = ct.c_long(), ct.c_int(), ct.c_int()
+        _check("rqc_batch_problem_plan", lib().rqc_batch_problem_plan(self.handle, p, kind, ct.byref(b), ct.byref(e),
+                                                                      ct.byref(n)))
+        return b.value, e.value, n.value
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            lib().rqc_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # --------------------------------------------------------------- algebra, RNG

@@ -1,0 +1,557 @@
+// Batched brickwall optimisation (include/rqcopt/rqc_batch.h).
+//
+// Device memory of a batch is one arena:
+//   flags | tables | results | workspaces
+// flags[k * P + p] = 1 makes every kernel of problem p in the graph of
+// evaluation kind k return at once; tables and results of all problems are
+// contiguous so that a step uploads and downloads each in one copy.  Each
+// problem has one workspace, sized for the largest of its three plans (a
+// problem is evaluated in at most one kind per step).
+//
+// The graph of kind k forks into nbranch parallel branches; problem p's chain
+// (zero accumulators, forward passes, trace, reverse passes and reductions,
+// finalisation) runs on branch p.branch, problems being spread over the
+// branches largest first onto the least loaded branch.
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "rqc_eval.hpp"
+#include "rqcopt/matrix.h"
+#include "rqcopt/rqc_batch.h"
+#include "rqcopt/trust_region.h"
+
+namespace rqc {
+
+static constexpr int kKinds = 3;
+
+static Mode kind_mode(int k) { return k == RQC_EVAL_TARGET ? Mode::Target : (k == RQC_EVAL_GRADIENT ? Mode::Grad : Mode::Hvp); }
+
+struct BatchProblem
+{
+	int L = 0, nlayers = 0, per = 0;
+	Circuit C;                  // brickwall expansion; gate values are set per step
+	const zc* dU = nullptr;
+	EvalPlan plan[kKinds];
+	size_t budget = 0;          // state-array budget the plans were made with
+	size_t work_off = 0, work_bytes = 0;
+	size_t tab_off = 0, tab_bytes = 0;
+	size_t res_off = 0, res_bytes = 0;
+	int branch = 0;
+	double cost = 0;
+};
+
+struct Batch
+{
+	Context* dev = nullptr;
+	bool fp32 = false;
+	int exec = RQC_BATCH_GRAPH;
+	std::vector<BatchProblem> probs;
+	int nbranch = 1;
+	std::vector<Slot> branches;
+	Slot io;                     // uploads and downloads
+	Slot kstream[kKinds];        // streams the kind graphs are launched on
+	char* arena = nullptr;
+	size_t arena_bytes = 0;
+	size_t off_tabs = 0, off_res = 0, off_work = 0, up_bytes = 0, down_bytes = 0;
+	char* h_up = nullptr;        // pinned mirror of flags + tables
+	char* h_down = nullptr;      // pinned mirror of the results
+	hipGraphExec_t graph[kKinds] = {nullptr, nullptr, nullptr};
+	hipEvent_t ev_up = nullptr, ev_fork = nullptr;
+	hipEvent_t ev_kind[kKinds] = {nullptr, nullptr, nullptr};
+	std::vector<hipEvent_t> ev_join;
+	rqc_batch_stats stats{};
+	std::mutex mu;
+
+	int* flags_dev(int k, int p) const { return reinterpret_cast<int*>(arena) + (size_t)k * probs.size() + p; }
+	char* work(const BatchProblem& b) const { return arena + off_work + b.work_off; }
+	char* tab(const BatchProblem& b) const { return arena + off_tabs + b.tab_off; }
+	char* res(const BatchProblem& b) const { return arena + off_res + b.res_off; }
+};
+
+static void plan_problem(const Batch& bt, BatchProblem& b)
+{
+	for (int k = 0; k < kKinds; k++) {
+		b.plan[k] = plan_eval(*bt.dev, b.C, kind_mode(k), b.per, bt.fp32, true, true, b.budget, 0);
+		if (!b.plan[k].tiles) {
+			throw Error(-3, "batched problems need the tile-pass engine (even nqubits >= 6, tile-pass engine selected)");
+		}
+	}
+	b.work_bytes = 0;
+	b.tab_bytes = 0;
+	b.res_bytes = 0;
+	for (int k = 0; k < kKinds; k++) {
+		b.work_bytes = std::max(b.work_bytes, b.plan[k].off_states + (size_t)b.plan[k].state_bytes());
+		b.tab_bytes = std::max(b.tab_bytes, b.plan[k].tab_bytes);
+		b.res_bytes = std::max(b.res_bytes, b.plan[k].res_bytes());
+	}
+}
+
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+static void batch_release(Batch& bt)
+{
+	for (auto& g : bt.graph) { if (g != nullptr) { (void)hipGraphExecDestroy(g); g = nullptr; } }
+	if (bt.arena != nullptr) { (void)hipFree(bt.arena); }
+	if (bt.h_up != nullptr) { (void)hipHostFree(bt.h_up); }
+	if (bt.h_down != nullptr) { (void)hipHostFree(bt.h_down); }
+	for (auto& s : bt.branches) { if (s.stream != nullptr) { (void)hipStreamDestroy(s.stream); } }
+	if (bt.io.stream != nullptr) { (void)hipStreamDestroy(bt.io.stream); }
+	for (auto& s : bt.kstream) { if (s.stream != nullptr) { (void)hipStreamDestroy(s.stream); } }
+	if (bt.ev_up != nullptr) { (void)hipEventDestroy(bt.ev_up); }
+	if (bt.ev_fork != nullptr) { (void)hipEventDestroy(bt.ev_fork); }
+	for (auto& e : bt.ev_kind) { if (e != nullptr) { (void)hipEventDestroy(e); } }
+	for (auto& e : bt.ev_join) { (void)hipEventDestroy(e); }
+}
+
+static void batch_init(Batch& bt, int n, const rqc_batch_problem* pr, int precision, int nstreams, size_t limit)
+{
+	if (n < 1) { throw Error(-3, "a batch needs at least one problem"); }
+	if (precision != RQC_PRECISION_FP64 && precision != RQC_PRECISION_MIXED) { throw Error(-3, "unknown precision"); }
+	bt.dev = &ctx();
+	bt.fp32 = (precision == RQC_PRECISION_MIXED);
+	bt.probs.resize(n);
+	for (int p = 0; p < n; p++) {
+		BatchProblem& b = bt.probs[p];
+		b.L = pr[p].nqubits;
+		b.nlayers = pr[p].nlayers;
+		if (b.L < 6 || b.L % 2 != 0) { throw Error(-3, "batched problems need an even number of qubits >= 6"); }
+		b.per = b.L / 2;
+		b.dU = device_target_matrix(const_cast<rqc_target*>(pr[p].target), b.L);
+		std::vector<struct mat4x4> z((size_t)b.nlayers);
+		for (auto& m : z) { for (auto& e : m.data) { e = numeric(0.0, 0.0); } }
+		for (int i = 0; i < b.nlayers; i++) { z[i].data[0] = z[i].data[5] = z[i].data[10] = z[i].data[15] = numeric(1.0, 0.0); }
+		b.C = make_brickwall(z.data(), z.data(), b.nlayers, pr[p].perms, b.L);
+		b.budget = (size_t)1 << 62;
+		b.cost = std::ldexp((double)b.C.ngates, 2 * b.L);
+	}
+	// fit the workspaces: halve the state budget of the largest one until all fit
+	size_t avail = limit;
+	if (avail == 0) {
+		size_t fr = 0, tot = 0;
+		RQC_HIP(hipMemGetInfo(&fr, &tot));
+		avail = (size_t)(0.85 * (double)fr);
+	}
+	for (auto& b : bt.probs) { plan_problem(bt, b); }
+	for (;;) {
+		size_t tot = 0;
+		BatchProblem* big = nullptr;
+		for (auto& b : bt.probs) {
+			tot += al256(b.work_bytes) + al256(b.tab_bytes) + al256(b.res_bytes);
+			if (big == nullptr || b.work_bytes > big->work_bytes) { big = &b; }
+		}
+		if (tot + 3 * sizeof(int) * n + 4096 <= avail) { break; }
+		const size_t st = (size_t)big->plan[RQC_EVAL_HVP].state_bytes();
+		if (big->plan[RQC_EVAL_HVP].B <= 1 || st < 2) { throw Error(-1, "device memory too small for the batch"); }
+		big->budget = st / 2;
+		plan_problem(bt, *big);
+	}
+	// layout of the arena
+	size_t tabs = 0, ress = 0, works = 0;
+	for (auto& b : bt.probs) {
+		b.tab_off = tabs;
+		tabs += al256(b.tab_bytes);
+		b.res_off = ress;
+		ress += al256(b.res_bytes);
+		b.work_off = works;
+		works += al256(b.work_bytes);
+	}
+	bt.off_tabs = al256(3 * sizeof(int) * (size_t)n);
+	bt.off_res = bt.off_tabs + tabs;
+	bt.off_work = bt.off_res + ress;
+	bt.up_bytes = bt.off_res;
+	bt.down_bytes = ress;
+	bt.arena_bytes = bt.off_work + works;
+	RQC_HIP(hipMalloc(&bt.arena, bt.arena_bytes));
+	RQC_HIP(hipHostMalloc(reinterpret_cast<void**>(&bt.h_up), bt.up_bytes, hipHostMallocDefault));
+	RQC_HIP(hipHostMalloc(reinterpret_cast<void**>(&bt.h_down), std::max<size_t>(bt.down_bytes, 256), hipHostMallocDefault));
+	std::memset(bt.h_up, 0, bt.up_bytes);
+	bt.stats.workspace_bytes = bt.arena_bytes;
+	// branches: largest problems first onto the least loaded branch
+	bt.nbranch = (nstreams > 0) ? std::min(nstreams, n) : std::min(n, 8);
+	std::vector<int> order(n);
+	for (int p = 0; p < n; p++) { order[p] = p; }
+	std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bt.probs[a].cost > bt.probs[b].cost; });
+	std::vector<double> load(bt.nbranch, 0.0);
+	for (int p : order) {
+		const int br = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+		bt.probs[p].branch = br;
+		load[br] += bt.probs[p].cost;
+	}
+	bt.branches.resize(bt.nbranch);
+	for (auto& s : bt.branches) { RQC_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)); }
+	RQC_HIP(hipStreamCreateWithFlags(&bt.io.stream, hipStreamNonBlocking));
+	for (auto& s : bt.kstream) { RQC_HIP(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking)); }
+	RQC_HIP(hipEventCreateWithFlags(&bt.ev_up, hipEventDisableTiming));
+	RQC_HIP(hipEventCreateWithFlags(&bt.ev_fork, hipEventDisableTiming));
+	for (auto& e : bt.ev_kind) { RQC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+	bt.ev_join.resize(bt.nbranch);
+	for (auto& e : bt.ev_join) { RQC_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming)); }
+}
+
+// capture the graph of evaluation kind k: fork into the branches, every
+// problem's chain on its branch with its skip flag, join
+static void capture_kind(Batch& bt, int k)
+{
+	hipStream_t cs = bt.kstream[k].stream;
+	RQC_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+	try {
+		RQC_HIP(hipEventRecord(bt.ev_fork, cs));
+		for (auto& s : bt.branches) { RQC_HIP(hipStreamWaitEvent(s.stream, bt.ev_fork, 0)); }
+		for (size_t p = 0; p < bt.probs.size(); p++) {
+			BatchProblem& b = bt.probs[p];
+			enqueue_eval(*bt.dev, bt.branches[b.branch], b.plan[k], b.C, b.dU, bt.work(b), bt.tab(b), bt.res(b), nullptr,
+				bt.flags_dev(k, (int)p));
+		}
+		for (int br = 0; br < bt.nbranch; br++) {
+			RQC_HIP(hipEventRecord(bt.ev_join[br], bt.branches[br].stream));
+			RQC_HIP(hipStreamWaitEvent(cs, bt.ev_join[br], 0));
+		}
+	} catch (...) {
+		hipGraph_t g = nullptr;
+		(void)hipStreamEndCapture(cs, &g);
+		if (g != nullptr) { (void)hipGraphDestroy(g); }
+		throw;
+	}
+	hipGraph_t g = nullptr;
+	RQC_HIP(hipStreamEndCapture(cs, &g));
+	const hipError_t e = hipGraphInstantiate(&bt.graph[k], g, nullptr, nullptr, 0);
+	(void)hipGraphDestroy(g);
+	RQC_HIP(e);
+}
+
+// gates of problem b for one step: layer gates (and directions) expanded to its circuit
+static void set_gates(BatchProblem& b, const struct mat4x4* vlist, const struct mat4x4* vdirs)
+{
+	for (int i = 0; i < b.nlayers; i++) {
+		for (int q = 0; q < b.per; q++) {
+			const size_t g = (size_t)i * b.per + q;
+			std::memcpy(&b.C.gates[16 * g], vlist[i].data, 16 * sizeof(zc));
+			if (vdirs != nullptr) { std::memcpy(&b.C.dirs[16 * g], vdirs[i].data, 16 * sizeof(zc)); }
+			else { std::memset(&b.C.dirs[16 * g], 0, 16 * sizeof(zc)); }
+		}
+	}
+}
+
+// one step: every problem p with kinds[p] >= 0 is evaluated; results unpacked into R[p]
+static void batch_step(Batch& bt, const int* kinds, const struct mat4x4* const* vlists, const struct mat4x4* const* vdirs,
+	std::vector<EvalResult>& R)
+{
+	const auto t0 = std::chrono::steady_clock::now();
+	const int n = (int)bt.probs.size();
+	int* flags = reinterpret_cast<int*>(bt.h_up);
+	bool present[kKinds] = {false, false, false};
+	long nev = 0;
+	for (int p = 0; p < n; p++) {
+		const int k = kinds[p];
+		if (k < RQC_EVAL_NONE || k > RQC_EVAL_HVP) { throw Error(-3, "unknown evaluation kind"); }
+		for (int kk = 0; kk < kKinds; kk++) { flags[(size_t)kk * n + p] = (kk == k) ? 0 : 1; }
+		if (k < 0) { continue; }
+		BatchProblem& b = bt.probs[p];
+		set_gates(b, vlists[p], k == RQC_EVAL_HVP ? vdirs[p] : nullptr);
+		if (k != RQC_EVAL_TARGET && !gates_near_unitary(b.C)) {
+			throw Error(-3, "batched gradient and HVP evaluations need gates unitary to 1e-6 (problem " + std::to_string(p) + ")");
+		}
+		fill_eval_tables(b.plan[k], b.C, bt.h_up + bt.off_tabs + b.tab_off);
+		present[k] = true;
+		nev++;
+	}
+	if (nev == 0) { return; }
+	RQC_HIP(hipMemcpyAsync(bt.arena, bt.h_up, bt.up_bytes, hipMemcpyHostToDevice, bt.io.stream));
+	RQC_HIP(hipEventRecord(bt.ev_up, bt.io.stream));
+	if (bt.exec == RQC_BATCH_GRAPH) {
+		for (int k = 0; k < kKinds; k++) {
+			if (!present[k]) { continue; }
+			if (bt.graph[k] == nullptr) { capture_kind(bt, k); }
+			RQC_HIP(hipStreamWaitEvent(bt.kstream[k].stream, bt.ev_up, 0));
+			RQC_HIP(hipGraphLaunch(bt.graph[k], bt.kstream[k].stream));
+			RQC_HIP(hipEventRecord(bt.ev_kind[k], bt.kstream[k].stream));
+			RQC_HIP(hipStreamWaitEvent(bt.io.stream, bt.ev_kind[k], 0));
+			bt.stats.graph_launches++;
+		}
+	} else {
+		for (auto& s : bt.branches) { RQC_HIP(hipStreamWaitEvent(s.stream, bt.ev_up, 0)); }
+		for (int p = 0; p < n; p++) {
+			if (kinds[p] < 0) { continue; }
+			BatchProblem& b = bt.probs[p];
+			enqueue_eval(*bt.dev, bt.branches[b.branch], b.plan[kinds[p]], b.C, b.dU, bt.work(b), bt.tab(b), bt.res(b),
+				nullptr, nullptr);
+		}
+		for (int br = 0; br < bt.nbranch; br++) {
+			RQC_HIP(hipEventRecord(bt.ev_join[br], bt.branches[br].stream));
+			RQC_HIP(hipStreamWaitEvent(bt.io.stream, bt.ev_join[br], 0));
+		}
+	}
+	RQC_HIP(hipMemcpyAsync(bt.h_down, bt.arena + bt.off_res, bt.down_bytes, hipMemcpyDeviceToHost, bt.io.stream));
+	RQC_HIP(hipStreamSynchronize(bt.io.stream));
+	RQC_HIP(hipGetLastError());
+	for (int p = 0; p < n; p++) {
+		if (kinds[p] < 0) { continue; }
+		const BatchProblem& b = bt.probs[p];
+		unpack_result(b.plan[kinds[p]], bt.h_down + b.res_off, R[p]);
+	}
+	bt.stats.steps++;
+	bt.stats.evaluations += nev;
+	bt.stats.device_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// ---------------------------------------------------------------- lock-step trust region
+namespace {
+
+struct Lockstep;
+
+struct SolveCtx
+{
+	Lockstep* ls;
+	int p;
+};
+
+struct Request
+{
+	bool pending = false;
+	int kind = RQC_EVAL_NONE;
+	const double* x = nullptr;
+	std::vector<struct mat4x4> dirs;
+	double f = 0;
+	double* grad = nullptr;
+	double* hvp = nullptr;
+};
+
+struct Lockstep
+{
+	Batch* bt = nullptr;
+	std::mutex mu;
+	std::condition_variable cv;
+	int active = 0;
+	int posted = 0;
+	bool failed = false;
+	std::vector<Request> req;
+
+	// called from problem p's solver thread: wait until the step that includes it has run
+	double post(int p, int kind, const double* x, const double* vec, double* grad, double* hvp)
+	{
+		std::unique_lock<std::mutex> lk(mu);
+		Request& r = req[p];
+		r.kind = kind;
+		r.x = x;
+		r.grad = grad;
+		r.hvp = hvp;
+		if (kind == RQC_EVAL_HVP) {
+			const int nl = bt->probs[p].nlayers;
+			const auto* v = reinterpret_cast<const struct mat4x4*>(x);
+			r.dirs.resize(nl);
+			for (int i = 0; i < nl; i++) { real_to_tangent(&vec[16 * i], &v[i], &r.dirs[i]); }
+		}
+		r.pending = true;
+		posted++;
+		cv.notify_all();
+		cv.wait(lk, [&] { return !r.pending; });
+		return r.f;
+	}
+
+	void finish()
+	{
+		std::lock_guard<std::mutex> lk(mu);
+		active--;
+		cv.notify_all();
+	}
+
+	// the calling thread runs the steps until every solver has finished
+	void serve()
+	{
+		const int n = (int)req.size();
+		std::vector<int> kinds(n);
+		std::vector<const struct mat4x4*> vl(n), vd(n);
+		std::vector<EvalResult> R(n);
+		std::unique_lock<std::mutex> lk(mu);
+		for (;;) {
+			cv.wait(lk, [&] { return active == 0 || (posted > 0 && posted == active); });
+			if (active == 0) { break; }
+			for (int p = 0; p < n; p++) {
+				kinds[p] = req[p].pending ? req[p].kind : RQC_EVAL_NONE;
+				vl[p] = reinterpret_cast<const struct mat4x4*>(req[p].x);
+				vd[p] = req[p].dirs.empty() ? nullptr : req[p].dirs.data();
+			}
+			lk.unlock();
+			bool ok = true;
+			try {
+				batch_step(*bt, kinds.data(), vl.data(), vd.data(), R);
+			} catch (const std::exception& e) {
+				fprintf(stderr, "librqcopt_hip: batched step failed: %s\n", e.what());
+				ok = false;
+			}
+			lk.lock();
+			if (!ok) { failed = true; }
+			for (int p = 0; p < n; p++) {
+				Request& r = req[p];
+				if (!r.pending) { continue; }
+				const double nan = std::nan("");
+				r.f = ok ? R[p].f.x : nan;
+				const size_t m = 16 * (size_t)bt->probs[p].nlayers;
+				if (r.grad != nullptr) { for (size_t i = 0; i < m; i++) { r.grad[i] = ok ? R[p].grad[i] : nan; } }
+				if (r.hvp != nullptr) { for (size_t i = 0; i < m; i++) { r.hvp[i] = ok ? R[p].hvp[i] : nan; } }
+				r.pending = false;
+			}
+			posted = 0;
+			cv.notify_all();
+		}
+	}
+};
+
+double ls_f(const double* x, void* d)
+{
+	auto* s = static_cast<SolveCtx*>(d);
+	return s->ls->post(s->p, RQC_EVAL_TARGET, x, nullptr, nullptr, nullptr);
+}
+
+double ls_f_deriv(const double* RQC_RESTRICT x, void* d, double* RQC_RESTRICT grad)
+{
+	auto* s = static_cast<SolveCtx*>(d);
+	return s->ls->post(s->p, RQC_EVAL_GRADIENT, x, nullptr, grad, nullptr);
+}
+
+void ls_f_hvp(const double* RQC_RESTRICT x, void* d, double* RQC_RESTRICT vec, double* RQC_RESTRICT hvp)
+{
+	auto* s = static_cast<SolveCtx*>(d);
+	s->ls->post(s->p, RQC_EVAL_HVP, x, vec, nullptr, hvp);
+}
+
+void ls_retract(const double* RQC_RESTRICT x, const double* RQC_RESTRICT eta, void* rdata, double* RQC_RESTRICT xs)
+{
+	const int nlayers = *static_cast<int*>(rdata);
+	const auto* v = reinterpret_cast<const struct mat4x4*>(x);
+	auto* w = reinterpret_cast<struct mat4x4*>(xs);
+	for (int i = 0; i < nlayers; i++) { retract(&v[i], &eta[16 * i], &w[i]); }
+}
+
+}  // namespace
+
+}  // namespace rqc
+
+using namespace rqc;
+
+struct rqc_batch
+{
+	Batch b;
+};
+
+extern "C" int rqc_batch_create(int nproblems, const struct rqc_batch_problem problems[], int precision, int nstreams,
+	size_t workspace_limit, struct rqc_batch** batch)
+{
+	rqc_batch* h = nullptr;
+	try {
+		h = new rqc_batch();
+		batch_init(h->b, nproblems, problems, precision, nstreams, workspace_limit);
+		*batch = h;
+		return 0;
+	} catch (const std::exception& e) {
+		if (h != nullptr) { batch_release(h->b); delete h; }
+		*batch = nullptr;
+		return report_error("rqc_batch_create", e);
+	}
+}
+
+extern "C" void rqc_batch_destroy(struct rqc_batch* batch)
+{
+	if (batch == nullptr) { return; }
+	if (batch->b.io.stream != nullptr) { (void)hipStreamSynchronize(batch->b.io.stream); }
+	batch_release(batch->b);
+	delete batch;
+}
+
+extern "C" int rqc_batch_set_exec(struct rqc_batch* batch, int exec)
+{
+	if (batch == nullptr || (exec != RQC_BATCH_GRAPH && exec != RQC_BATCH_EAGER)) { return -3; }
+	std::lock_guard<std::mutex> lk(batch->b.mu);
+	batch->b.exec = exec;
+	return 0;
+}
+
+extern "C" int rqc_batch_evaluate(struct rqc_batch* batch, const int kinds[], const struct mat4x4* const vlists[],
+	const struct mat4x4* const vdirs[], numeric fvals[], double* const grads[], double* const hvps[])
+{
+	try {
+		if (batch == nullptr) { throw Error(-3, "batch is NULL"); }
+		Batch& bt = batch->b;
+		std::lock_guard<std::mutex> lk(bt.mu);
+		const int n = (int)bt.probs.size();
+		for (int p = 0; p < n; p++) {
+			if (kinds[p] == RQC_EVAL_HVP && (vdirs == nullptr || vdirs[p] == nullptr)) {
+				throw Error(-3, "HVP evaluation of problem " + std::to_string(p) + " without directions");
+			}
+		}
+		std::vector<EvalResult> R(n);
+		batch_step(bt, kinds, vlists, vdirs, R);
+		for (int p = 0; p < n; p++) {
+			if (kinds[p] < 0) { continue; }
+			if (fvals != nullptr) { std::memcpy(&fvals[p], &R[p].f, sizeof(zc)); }
+			if (kinds[p] >= RQC_EVAL_GRADIENT && grads != nullptr && grads[p] != nullptr) {
+				std::memcpy(grads[p], R[p].grad.data(), R[p].grad.size() * sizeof(double));
+			}
+			if (kinds[p] == RQC_EVAL_HVP && hvps != nullptr && hvps[p] != nullptr) {
+				std::memcpy(hvps[p], R[p].hvp.data(), R[p].hvp.size() * sizeof(double));
+			}
+		}
+		return 0;
+	} catch (const std::exception& e) { return report_error("rqc_batch_evaluate", e); }
+}
+
+extern "C" int rqc_batch_optimize_brickwall_hvp(struct rqc_batch* batch, const struct mat4x4* const vlist_start[],
+	const struct rtr_params* params, int niter, double* const f_iters[], struct mat4x4* const vlist_opt[])
+{
+	try {
+		if (batch == nullptr) { throw Error(-3, "batch is NULL"); }
+		if (niter < 0) { throw Error(-3, "niter must be >= 0"); }
+		Batch& bt = batch->b;
+		std::lock_guard<std::mutex> lk(bt.mu);
+		const int n = (int)bt.probs.size();
+		Lockstep ls;
+		ls.bt = &bt;
+		ls.req.resize(n);
+		ls.active = n;
+		std::vector<SolveCtx> sc(n);
+		std::vector<int> nl(n);
+		std::vector<std::thread> th;
+		th.reserve(n);
+		for (int p = 0; p < n; p++) {
+			sc[p] = SolveCtx{&ls, p};
+			nl[p] = bt.probs[p].nlayers;
+			th.emplace_back([&, p] {
+				struct rtr_params prm = *params;
+				prm.g_func = nullptr;
+				prm.g_data = nullptr;
+				prm.g_iter = nullptr;
+				riemannian_trust_region_optimize_hvp(ls_f, ls_f_deriv, ls_f_hvp, &sc[p], ls_retract, &nl[p],
+					nl[p] * num_tangent_params, reinterpret_cast<const double*>(vlist_start[p]), nl[p] * 32, &prm, niter,
+					f_iters[p], reinterpret_cast<double*>(vlist_opt[p]));
+				ls.finish();
+			});
+		}
+		ls.serve();
+		for (auto& t : th) { t.join(); }
+		if (ls.failed) { throw Error(-1, "a batched evaluation step failed"); }
+		return 0;
+	} catch (const std::exception& e) { return report_error("rqc_batch_optimize_brickwall_hvp", e); }
+}
+
+extern "C" int rqc_batch_get_stats(const struct rqc_batch* batch, struct rqc_batch_stats* stats)
+{
+	if (batch == nullptr || stats == nullptr) { return -3; }
+	*stats = batch->b.stats;
+	return 0;
+}
+
+extern "C" int rqc_batch_problem_plan(const struct rqc_batch* batch, int p, int kind, long* block, int* engine, int* passes)
+{
+	if (batch == nullptr || p < 0 || p >= (int)batch->b.probs.size() || kind < 0 || kind >= kKinds) { return -3; }
+	const EvalPlan& pl = batch->b.probs[p].plan[kind];
+	if (block != nullptr) { *block = pl.B; }
+	if (engine != nullptr) { *engine = pl.tiles ? RQC_ENGINE_TILE_PASS : RQC_ENGINE_PER_GATE; }
+	if (passes != nullptr) { *passes = pl.passes; }
+	return 0;
+}

@@ -9,7 +9,12 @@
   periodic, as dense matrices for small L;
 - the Strang-split Trotter circuit of the Ising chain as a brickwall of exact
   bond layers (one shared gate per layer), which the GPU turns into a dense
-  device-resident U (DeviceTarget.brickwall).
+  device-resident U (DeviceTarget.brickwall);
+- the same splitting of the XXZ chain with site-dependent fields as a general
+  circuit of exact bond gates (DeviceTarget.circuit);
+- the batched workload: independent problems with L in {8, 10, 12}, depth in
+  {3, 5, 7}, XXZ targets (Delta = 0.5, fields uniform in [-1, 1], t uniform
+  in [0.1, 1.0]) and Haar-random starting gates.
 Qubit 0 is the most significant bit of the amplitude index.
 """
 import numpy as np
@@ -107,3 +112,46 @@ def config_inputs(L, nlayers, seed):
     P = brickwall_perms(L, nlayers)
     vec, Z = tangent_direction(seed + 1, V)
     return V, P, vec, Z
+
+
+def xxz_bond(delta, hj, hk):
+    """Two-site term XX + YY + Delta ZZ + hj/2 ZI + hk/2 IZ (every site lies in one even and one odd bond)."""
+    return (np.kron(PAULI_X, PAULI_X) + np.kron(PAULI_Y, PAULI_Y) + delta * np.kron(PAULI_Z, PAULI_Z)
+            + 0.5 * hj * np.kron(PAULI_Z, ID2) + 0.5 * hk * np.kron(ID2, PAULI_Z))
+
+
+def xxz_trotter_circuit(L, delta, fields, t, steps):
+    """Strang splitting of the periodic XXZ chain (even bonds (2k, 2k+1), odd bonds (2k+1, 2k+2 mod L)),
+    adjacent even half steps merged; returns gates [ngates, 4, 4] and wires [ngates, 2]."""
+    assert L % 2 == 0
+    tau = t / steps
+    gates, wires = [], []
+
+    def layer(parity, dt):
+        for k in range(L // 2):
+            j = (2 * k + parity) % L
+            q = (j + 1) % L
+            gates.append(expm_hermitian(xxz_bond(delta, fields[j], fields[q]), dt))
+            wires.append((j, q))
+
+    for s in range(steps):
+        layer(0, tau / 2 if s == 0 else tau)
+        layer(1, tau)
+    layer(0, tau / 2)
+    return np.stack(gates), np.array(wires, dtype=np.int32)
+
+
+def batch_config(seed, nproblems=64, qubits=(8, 10, 12), depths=(3, 5, 7), trotter_steps=4):
+    """The batched workload's problems: dicts with L, depth, perms, t, fields, target (gates, wires), V0."""
+    r = Rng(seed)
+    probs = []
+    for p in range(nproblems):
+        L = qubits[int(r.randu() * len(qubits))]
+        d = depths[int(r.randu() * len(depths))]
+        t = 0.1 + 0.9 * r.randu()
+        fields = np.array([2 * r.randu() - 1 for _ in range(L)])
+        tg, tw = xxz_trotter_circuit(L, 0.5, fields, t, trotter_steps)
+        V0 = np.stack([r.haar_unitary() for _ in range(d)])
+        probs.append(dict(L=L, depth=d, perms=brickwall_perms(L, d), t=t, fields=fields, target_gates=tg,
+                          target_wires=tw, V0=V0))
+    return probs
