@@ -119,6 +119,10 @@ EXPORTED = [
     # rqc_batch.h
     "rqc_batch_create", "rqc_batch_destroy", "rqc_batch_set_exec", "rqc_batch_evaluate",
     "rqc_batch_optimize_brickwall_hvp", "rqc_batch_get_stats", "rqc_batch_problem_plan",
+    # rqc_server.h
+    "rqc_server_create", "rqc_server_destroy", "rqc_session_create_brickwall", "rqc_session_create_circuit",
+    "rqc_session_ngroups", "rqc_session_destroy", "rqc_server_submit", "rqc_server_wait", "rqc_server_poll",
+    "rqc_server_release", "rqc_server_get_stats",
 ]
 
 
@@ -645,6 +649,127 @@ class Batch:
     def close(self):
         if self.handle is not None and self.handle.value:
             lib().rqc_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# --------------------------------------------------------------- request server (rqc_server.h)
+class _ServerParams(ct.Structure):
+    _fields_ = [("arena_bytes", ct.c_size_t), ("max_request_bytes", ct.c_size_t), ("nslots", ct.c_int)]
+
+
+class _RequestInfo(ct.Structure):
+    _fields_ = [("status", ct.c_int), ("slot", ct.c_int), ("queued_ms", ct.c_double), ("run_ms", ct.c_double),
+                ("latency_ms", ct.c_double), ("start_ms", ct.c_double), ("end_ms", ct.c_double),
+                ("workspace_bytes", ct.c_size_t), ("block", ct.c_long)]
+
+
+class _ServerStats(ct.Structure):
+    _fields_ = [("submitted", ct.c_long), ("completed", ct.c_long), ("failed", ct.c_long),
+                ("peak_in_flight", ct.c_int), ("arena_bytes", ct.c_size_t), ("peak_arena_bytes", ct.c_size_t)]
+
+
+class Session:
+    """A circuit structure with its device target and precision, registered with a Server."""
+
+    def __init__(self, server, handle, ngroups, target):
+        self.server, self.handle, self.ngroups, self._target = server, handle, ngroups, target
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            lib().rqc_session_destroy(self.handle)
+            self.handle = None
+
+
+class Pending:
+    """A submitted request: its ticket and the output buffers the server writes on completion."""
+
+    def __init__(self, ticket, f, grad, hvp, keep):
+        self.ticket, self.f, self.grad, self.hvp, self._keep = ticket, f, grad, hvp, keep
+
+
+class Server:
+    """Continuous-batching request server: target / gradient / HVP requests of many sessions run
+    concurrently on the GPU, admitted in arrival order as execution slots and arena memory free up."""
+
+    def __init__(self, arena_bytes=0, max_request_bytes=0, nslots=0):
+        prm = _ServerParams(arena_bytes, max_request_bytes, nslots)
+        h = ct.c_void_p()
+        L = lib()
+        L.rqc_server_create.argtypes = [ct.POINTER(_ServerParams), ct.c_void_p]
+        _check("rqc_server_create", L.rqc_server_create(ct.byref(prm), ct.byref(h)))
+        self.handle = h
+
+    def session_brickwall(self, perms, target, precision=PRECISION_FP64):
+        perms, rows = _perms(perms)
+        h = ct.c_void_p()
+        _check("rqc_session_create_brickwall",
+               lib().rqc_session_create_brickwall(self.handle, perms.shape[1], perms.shape[0], rows, target.handle,
+                                                  precision, ct.byref(h)))
+        return Session(self, h, perms.shape[0], target)
+
+    def session_circuit(self, wires, nqubits, target, precision=PRECISION_FP64):
+        wires = _wires(wires)
+        h = ct.c_void_p()
+        _check("rqc_session_create_circuit",
+               lib().rqc_session_create_circuit(self.handle, nqubits, wires.size // 2, _p(wires), target.handle,
+                                                precision, ct.byref(h)))
+        return Session(self, h, wires.size // 2, target)
+
+    def submit(self, session, kind, gates=None, dirs=None):
+        g = None if gates is None else _gates(gates)
+        d = None if dirs is None else _gates(dirs)
+        f = np.zeros(1, np.complex128)
+        grad = np.full(16 * session.ngroups, np.nan) if kind >= EVAL_GRADIENT else None
+        hvp = np.full(16 * session.ngroups, np.nan) if kind == EVAL_HVP else None
+        t = ct.c_long()
+        L = lib()
+        L.rqc_server_submit.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p, ct.c_void_p,
+                                        ct.c_void_p, ct.c_void_p, ct.c_void_p]
+        _check("rqc_server_submit", L.rqc_server_submit(self.handle, session.handle, kind,
+                                                        None if g is None else g.ctypes.data,
+                                                        None if d is None else d.ctypes.data, f.ctypes.data,
+                                                        None if grad is None else grad.ctypes.data,
+                                                        None if hvp is None else hvp.ctypes.data, ct.byref(t)))
+        return Pending(t.value, f, grad, hvp, (g, d))
+
+    @staticmethod
+    def _info(i):
+        return {k: getattr(i, k) for k, _ in _RequestInfo._fields_}
+
+    def wait(self, pending):
+        """Block until done: (f, grad, hvp, info)."""
+        info = _RequestInfo()
+        L = lib()
+        L.rqc_server_wait.argtypes = [ct.c_void_p, ct.c_long, ct.POINTER(_RequestInfo)]
+        _check("rqc_server_wait", L.rqc_server_wait(self.handle, pending.ticket, ct.byref(info)))
+        return pending.f[0], pending.grad, pending.hvp, self._info(info)
+
+    def poll(self, pending):
+        info = _RequestInfo()
+        L = lib()
+        L.rqc_server_poll.argtypes = [ct.c_void_p, ct.c_long, ct.POINTER(_RequestInfo)]
+        r = _check("rqc_server_poll", L.rqc_server_poll(self.handle, pending.ticket, ct.byref(info)))
+        return self._info(info) if r == 1 else None
+
+    def release(self, pending):
+        L = lib()
+        L.rqc_server_release.argtypes = [ct.c_void_p, ct.c_long]
+        _check("rqc_server_release", L.rqc_server_release(self.handle, pending.ticket))
+
+    def stats(self):
+        st = _ServerStats()
+        _check("rqc_server_get_stats", lib().rqc_server_get_stats(self.handle, ct.byref(st)))
+        return {k: getattr(st, k) for k, _ in _ServerStats._fields_}
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            lib().rqc_server_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

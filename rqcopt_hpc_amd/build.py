@@ -18,7 +18,7 @@ INC = os.path.join(ROOT, "include")
 OUT = os.path.join(HERE, "librqcopt_hip.so")
 ARCH = os.environ.get("RQC_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["rqc_engine.hip", "rqc_batch.hip", "rqc_host.cpp"]
+SOURCES = ["rqc_engine.hip", "rqc_batch.hip", "rqc_server.hip", "rqc_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -64,6 +64,7 @@ C_CALLER = r"""
 #include "rqcopt/brickwall_opt.h"
 #include "rqcopt/rqc_device.h"
 #include "rqcopt/rqc_batch.h"
+#include "rqcopt/rqc_server.h"
 #include <stdio.h>
 int main(void)
 {
@@ -86,7 +87,8 @@ int main(void)
 		(void*)brickwall_unitary_forward, (void*)brickwall_unitary_backward,
 		(void*)brickwall_unitary_target_projected_hessian_vector_product, (void*)rqc_device_ufunc,
 		(void*)optimize_brickwall_circuit_hvp, (void*)riemannian_trust_region_optimize_hvp,
-		(void*)rqc_batch_create, (void*)rqc_batch_evaluate, (void*)rqc_batch_optimize_brickwall_hvp };
+		(void*)rqc_batch_create, (void*)rqc_batch_evaluate, (void*)rqc_batch_optimize_brickwall_hvp,
+		(void*)rqc_server_create, (void*)rqc_server_submit, (void*)rqc_server_wait };
 	return fns[0] == 0;
 }
 """
@@ -112,7 +114,7 @@ def test_headers_compile_as_cpp():
     with tempfile.TemporaryDirectory() as d:
         src = os.path.join(d, "caller.cpp")
         open(src, "w").write('#include "rqcopt/target.h"\n#include "rqcopt/brickwall_opt.h"\n'
-                             '#include "rqcopt/rqc_device.h"\n#include "rqcopt/rqc_batch.h"\nint main(){ struct mat4x4 a; zero_matrix(&a); return 0; }\n')
+                             '#include "rqcopt/rqc_device.h"\n#include "rqcopt/rqc_batch.h"\n#include "rqcopt/rqc_server.h"\nint main(){ struct mat4x4 a; zero_matrix(&a); return 0; }\n')
         subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), "-c", src,
                                "-o", os.path.join(d, "caller.o")])
 

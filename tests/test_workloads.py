@@ -64,3 +64,63 @@ def test_config_inputs_are_seeded_and_tangent():
         s = V.conj().T @ Z
         assert np.max(np.abs(s + s.conj().T)) < 1e-14           # Z in the tangent space at V
         assert np.max(np.abs(O.tangent_to_real(V, Z) - vec1[16 * k:16 * k + 16])) < 1e-14
+
+
+def _circuit_matrix(gates, wires, L):
+    from helpers import gate_matrix
+    U = np.eye(1 << L, dtype=np.complex128)
+    for G, (i, j) in zip(gates, wires):
+        U = gate_matrix(G, int(i), int(j), L) @ U
+    return U
+
+
+def test_xxz_trotter_circuit_second_order_convergence():
+    from rqcopt_hpc_amd.workloads import xxz_trotter_circuit
+    L, t = 6, 0.4
+    fields = np.linspace(-1, 1, L)
+    U = expm_hermitian(xxz_hamiltonian(L, 0.5, fields), t)
+    errs = [np.max(np.abs(_circuit_matrix(*xxz_trotter_circuit(L, 0.5, fields, t, s), L) - U)) for s in (2, 4, 8)]
+    # second-order splitting: the error drops by about 4x per doubling of the steps
+    assert errs[2] < errs[1] < errs[0] and errs[0] / errs[2] > 10 and errs[2] < 3e-3
+
+
+def test_serving_chain_targets_match_dense_hamiltonians():
+    from rqcopt_hpc_amd.serving import chain_bonds, chain_trotter_circuit, ising_term, xxz_term
+    for L in (6, 7):
+        bonds = chain_bonds(L)
+        assert len(bonds) == (L if L % 2 == 0 else L - 1)
+        fields = np.linspace(-0.8, 0.9, L)
+        # dense H of the same chains: periodic for even L, open for odd L
+        Hi = np.zeros((1 << L, 1 << L), np.complex128)
+        Hx = np.zeros((1 << L, 1 << L), np.complex128)
+        for j, q in bonds:
+            Hi += _kron_site(PAULI_Z, j, L) @ _kron_site(PAULI_Z, q, L)
+            for P, c in ((PAULI_X, 1.0), (np.array([[0, -1j], [1j, 0]]), 1.0), (PAULI_Z, 0.5)):
+                Hx += c * _kron_site(P, j, L) @ _kron_site(P, q, L)
+        for j in range(L):
+            Hi += 0.75 * _kron_site(PAULI_X, j, L) + 0.6 * _kron_site(PAULI_Z, j, L)
+            Hx += fields[j] * _kron_site(PAULI_Z, j, L)
+        for H, term in ((Hi, ising_term(1.0, 0.75, 0.6)), (Hx, xxz_term(0.5, fields))):
+            U = expm_hermitian(H, 0.3)
+            e4 = np.max(np.abs(_circuit_matrix(*chain_trotter_circuit(L, term, 0.3, 4), L) - U))
+            e16 = np.max(np.abs(_circuit_matrix(*chain_trotter_circuit(L, term, 0.3, 16), L) - U))
+            assert e16 < e4 / 8 and e16 < 1e-3, (L, e4, e16)
+        if L % 2 == 0:
+            assert np.allclose(Hi, ising_hamiltonian(L, 1.0, 0.75, 0.6))
+
+
+def test_serving_workload_layouts():
+    from rqcopt_hpc_amd.serving import open_brickwall_wires, serving_workload
+    solves, targets = serving_workload(5, nsolves=40)
+    for s in solves:
+        assert 8 <= s["L"] <= 14 and s["depth"] in (3, 5, 7, 9)
+        g, w = targets[s["target"]]
+        assert np.all(w < s["L"])
+        if s["L"] % 2:
+            assert not s["brickwall"] and len(s["gates"]) == len(s["wires"]) == s["depth"] * ((s["L"] - 1) // 2)
+        else:
+            assert s["brickwall"] and len(s["gates"]) == s["depth"]
+        for G in s["gates"]:
+            assert np.allclose(G.conj().T @ G, np.eye(4), atol=1e-12)
+    w = open_brickwall_wires(7, 2)
+    assert w.tolist() == [[0, 1], [2, 3], [4, 5], [1, 2], [3, 4], [5, 6]]
