@@ -9,6 +9,16 @@ circuit of the Ising chain (J=1, h=0.75, g=0.6, t=0.5, 4 steps) formed on the
 GPU.  The FP64 path is the headline value; the mixed-precision path (complex
 FP32 sweeps, FP64 reductions) is reported beside it with its error.
 
+Beside the headline line's single-circuit numbers it reports
+  "batched": 64 independent brickwall problems (L in {8, 10, 12}, depth in
+    {3, 5, 7}, XXZ Trotter targets with seeded fields and times) evaluated
+    together (one graph launch per evaluation kind and step): grad+HVP
+    evals/s in FP64 and mixed, and a lock-step trust-region run of all 64;
+  "serving": the request server under an open-loop Poisson stream of
+    target / gradient / HVP requests from 256 solves (L in {8..14}, depth in
+    {3, 5, 7, 9}, Ising or XXZ targets): p50 / p99 latency and completed
+    requests/s in FP64 and mixed, at the offered rate and at a quarter of it.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 14] [--depth 9]
 
 Prints one JSON line.  Multi-GPU runs are not part of this build (--gpus > 1 exits).
@@ -186,12 +196,88 @@ def run(args):
         "mixed": {"value": 1.0 / r32["eval_s"], "ms_per_step": r32["eval_s"] * 1e3, "speedup_vs_fp64": r64["eval_s"] / r32["eval_s"],
                   "error_vs_fp64": mixed_err, "column_block": r32["block"]},
     }
+    R.release_workspace()
+    line["batched"] = None if args.no_batch else run_batched(args, R)
+    line["serving"] = None if args.no_serve else run_serving(args, R)
+    if line["serving"] is not None:
+        line["served_p99_ms"] = {k: line["serving"][k]["offered"]["p99_ms"] for k in ("fp64", "mixed")}
     if not args.no_cpu:
         threads = min(args.cpu_threads, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(L, V, Z, P, Vt, Pt, args.cpu_cols, threads)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
+
+
+def run_batched(args, R):
+    """64 independent problems: all-HVP steps (evals/s) and a lock-step trust-region run."""
+    from rqcopt_hpc_amd import workloads as W
+
+    probs = W.batch_config(args.seed, args.batch_problems)
+    targets = [R.DeviceTarget.circuit(p["target_gates"], p["target_wires"], p["L"]) for p in probs]
+    V = [p["V0"] for p in probs]
+    Z = [W.tangent_direction(1000 + i, V[i])[1] for i in range(len(probs))]
+    kinds = [R.EVAL_HVP] * len(probs)
+    out = {"problems": len(probs), "qubits": sorted({p["L"] for p in probs}),
+           "depths": sorted({p["depth"] for p in probs}), "steps": args.steps}
+    for prec, name in ((R.PRECISION_FP64, "fp64"), (R.PRECISION_MIXED, "mixed")):
+        B = R.Batch([(p["perms"], t) for p, t in zip(probs, targets)], precision=prec)
+        for _ in range(max(1, args.warmup)):
+            B.evaluate(kinds, V, Z)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            B.evaluate(kinds, V, Z)
+        dt = (time.perf_counter() - t0) / args.steps
+        rec = {"evals_per_s": len(probs) / dt, "ms_per_step": dt * 1e3,
+               "workspace_gib": B.stats()["workspace_bytes"] / 2**30}
+        if args.batch_niter > 0:
+            st0 = B.stats()
+            t0 = time.perf_counter()
+            f_iters, _ = B.optimize(V, args.batch_niter)
+            tt = time.perf_counter() - t0
+            st1 = B.stats()
+            rec["trust_region"] = {"niter": args.batch_niter, "seconds": tt,
+                                   "steps": st1["steps"] - st0["steps"],
+                                   "evaluations": st1["evaluations"] - st0["evaluations"],
+                                   "graph_launches": st1["graph_launches"] - st0["graph_launches"],
+                                   "evals_per_s": (st1["evaluations"] - st0["evaluations"]) / tt,
+                                   "f_decreased": int(sum(f[-1] < f[0] for f in f_iters))}
+        out[name] = rec
+        print(f"[bench] batched {name}: {rec}", file=sys.stderr, flush=True)
+        B.close()
+    out["mixed_speedup"] = out["mixed"]["evals_per_s"] / out["fp64"]["evals_per_s"]
+    for t in targets:
+        t.close()
+    return out
+
+
+def run_serving(args, R):
+    """Open-loop Poisson load on the request server, FP64 and mixed."""
+    from rqcopt_hpc_amd import serving as S
+
+    solves, targets = S.serving_workload(args.seed, nsolves=256)
+    out = {"solves": 256, "offered_rate": args.serve_rate, "seconds": args.serve_seconds,
+           "kinds": "target / gradient / HVP, equal weights", "qubits": "8..14", "depths": [3, 5, 7, 9]}
+    for prec, name in ((R.PRECISION_FP64, "fp64"), (R.PRECISION_MIXED, "mixed")):
+        srv = R.Server(nslots=8)
+        sessions, dev = S.open_sessions(srv, solves, targets, prec)
+        # plans made and every kernel loaded before the timed load
+        S.run_poisson(srv, solves, sessions, rate=args.serve_rate, duration=0.25, seed=args.seed + 99)
+        rec = {}
+        for label, rate in (("offered", args.serve_rate), ("quarter", args.serve_rate / 4)):
+            res = S.run_poisson(srv, solves, sessions, rate=rate, duration=args.serve_seconds, seed=args.seed)
+            rec[label] = {k: res[k] for k in ("requests", "offered_rate", "achieved_rate", "p50_ms", "p99_ms",
+                                              "mean_ms", "p50_by_kind")}
+            print(f"[bench] serving {name} {label}: {rec[label]}", file=sys.stderr, flush=True)
+        rec["peak_in_flight"] = srv.stats()["peak_in_flight"]
+        rec["arena_gib"] = srv.stats()["arena_bytes"] / 2**30
+        for s_ in sessions:
+            s_.close()
+        srv.close()
+        for t in dev.values():
+            t.close()
+        out[name] = rec
+    return out
 
 
 def main():
@@ -210,6 +296,12 @@ def main():
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
                     help="profiling runs: time one precision only and print a short summary")
     ap.add_argument("--no-profile-pass", action="store_true", help="skip the HIP-event kernel timing pass")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-problems section")
+    ap.add_argument("--batch-problems", type=int, default=64)
+    ap.add_argument("--batch-niter", type=int, default=2, help="lock-step trust-region iterations (0: skip)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="offered requests/s of the Poisson stream")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the Poisson stream")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU column sharding is a "

@@ -18,6 +18,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -140,6 +141,8 @@ struct Server
 	rqc_server_stats stats{};
 	std::vector<::rqc_session*> sessions;   // released with the server
 	Clock::time_point t0 = Clock::now();
+	std::mutex plan_mu;
+	std::map<std::string, std::shared_ptr<const EvalPlan>> plans;   // by circuit structure
 };
 
 static void ensure_pinned(char** p, size_t* cap, size_t n)
@@ -212,21 +215,38 @@ static void worker_loop(Server& S, Worker& w)
 	}
 }
 
+// plans are shared by every session of the same structure (qubits, wires,
+// groups, precision) and made once per evaluation kind and engine
 static std::shared_ptr<const EvalPlan> session_plan(Session& s, int kind, bool unitary)
 {
-	std::lock_guard<std::mutex> lk(s.mu);
-	auto& p = s.plan[kind][unitary ? 1 : 0];
-	if (!p) {
-		const Mode m = kind == RQC_EVAL_TARGET ? Mode::Target : (kind == RQC_EVAL_GRADIENT ? Mode::Grad : Mode::Hvp);
-		Circuit C = s.C;
-		if (C.dirs.size() != C.gates.size()) { C.dirs.assign(C.gates.size(), zc{0, 0}); }
-		p = std::make_shared<EvalPlan>(plan_eval(*s.srv->dev, C, m, s.per, s.fp32, true, unitary, s.srv->max_req, 0));
-		if (p->bytes > s.srv->alloc.size) {
-			auto b = p->bytes;
-			p.reset();
-			throw Error(-1, "request workspace (" + std::to_string(b) + " bytes) exceeds the server arena");
+	{
+		std::lock_guard<std::mutex> lk(s.mu);
+		if (s.plan[kind][unitary ? 1 : 0]) { return s.plan[kind][unitary ? 1 : 0]; }
+	}
+	Server& S = *s.srv;
+	std::string key = std::to_string(s.C.L) + "/" + std::to_string(s.per) + "/" + std::to_string(s.fp32) + "/" +
+		std::to_string(kind) + "/" + std::to_string(unitary) + ":";
+	for (int k = 0; k < s.C.ngates; k++) { key += std::to_string(s.C.wi[k]) + "," + std::to_string(s.C.wj[k]) + ";"; }
+	std::shared_ptr<const EvalPlan> p;
+	{
+		std::lock_guard<std::mutex> lk(S.plan_mu);
+		auto it = S.plans.find(key);
+		if (it != S.plans.end()) {
+			p = it->second;
+		} else {
+			const Mode m = kind == RQC_EVAL_TARGET ? Mode::Target : (kind == RQC_EVAL_GRADIENT ? Mode::Grad : Mode::Hvp);
+			Circuit C = s.C;
+			C.dirs.assign(C.gates.size(), zc{0, 0});
+			auto np = std::make_shared<EvalPlan>(plan_eval(*S.dev, C, m, s.per, s.fp32, true, unitary, S.max_req, 0));
+			if (np->bytes > S.alloc.size) {
+				throw Error(-1, "request workspace (" + std::to_string(np->bytes) + " bytes) exceeds the server arena");
+			}
+			S.plans.emplace(key, np);
+			p = np;
 		}
 	}
+	std::lock_guard<std::mutex> lk(s.mu);
+	s.plan[kind][unitary ? 1 : 0] = p;
 	return p;
 }
 
