@@ -138,6 +138,20 @@ def run(args):
               f"kernel ms {[round(prof[k][0], 2) for k in range(4)]}, launches {[prof[k][1] for k in range(4)]}, "
               f"GB/s {[round(prof[k][2] / max(prof[k][0], 1e-9) / 1e6) for k in range(4)]}",
               file=sys.stderr, flush=True)
+    # the gradient eval (projected gradient only) of the same workload, reported on its own
+    grad_eval = {}
+    if args.precision == "both" and not args.no_profile_pass:
+        for prec, name in precs:
+            R.set_precision(prec)
+            R.brickwall_unitary_target_and_projected_gradient(T, V, P)
+            R.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                R.brickwall_unitary_target_and_projected_gradient(T, V, P)
+            R.synchronize()
+            dtg = (time.perf_counter() - t0) / args.steps
+            grad_eval[name] = {"evals_per_s": 1.0 / dtg, "ms_per_eval": dtg * 1e3}
+        print(f"[bench] grad eval: {grad_eval}", file=sys.stderr, flush=True)
     R.set_precision(R.PRECISION_FP64)
     if args.precision != "both" or args.no_profile_pass:
         print(json.dumps({k: {"ms_per_eval": v["eval_s"] * 1e3, "column_block": v["block"]} for k, v in results.items()}))
@@ -196,6 +210,8 @@ def run(args):
         "mixed": {"value": 1.0 / r32["eval_s"], "ms_per_step": r32["eval_s"] * 1e3, "speedup_vs_fp64": r64["eval_s"] / r32["eval_s"],
                   "error_vs_fp64": mixed_err, "column_block": r32["block"]},
     }
+    line["grad_eval"] = grad_eval
+    line["mixed_error_seeds"] = mixed_error_sweep(args, R, T, P) if args.mixed_seeds > 0 else None
     R.release_workspace()
     line["batched"] = None if args.no_batch else run_batched(args, R)
     line["serving"] = None if args.no_serve else run_serving(args, R)
@@ -207,6 +223,30 @@ def run(args):
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
+
+
+def mixed_error_sweep(args, R, T, P):
+    """Mixed-precision error of f, grad and HVP against the FP64 path for seeded gate sets
+    (the FP64 path matches the oracle to 1e-12, so it stands in for it at this size)."""
+    from rqcopt_hpc_amd.workloads import config_inputs
+
+    rows = []
+    for seed in range(1, args.mixed_seeds + 1):
+        V, _, _, Z = config_inputs(args.L, args.depth, seed)
+        R.set_precision(R.PRECISION_FP64)
+        f64, g64, h64 = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
+        R.set_precision(R.PRECISION_MIXED)
+        f32, g32, h32 = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
+        rows.append((abs(f32 - f64), abs(f32 - f64) / abs(f64), np.max(np.abs(g32 - g64)),
+                     np.max(np.abs(g32 - g64)) / np.max(np.abs(g64)), np.max(np.abs(h32 - h64)),
+                     np.max(np.abs(h32 - h64)) / np.max(np.abs(h64))))
+    R.set_precision(R.PRECISION_FP64)
+    a = np.array(rows)
+    names = ["f_abs", "f_rel", "grad_abs", "grad_rel", "hvp_abs", "hvp_rel"]
+    out = {"seeds": args.mixed_seeds, "max": dict(zip(names, map(float, a.max(0)))),
+           "median": dict(zip(names, map(float, np.median(a, 0))))}
+    print(f"[bench] mixed-precision errors over {args.mixed_seeds} seeds: {out}", file=sys.stderr, flush=True)
+    return out
 
 
 def run_batched(args, R):
@@ -229,7 +269,8 @@ def run_batched(args, R):
             B.evaluate(kinds, V, Z)
         dt = (time.perf_counter() - t0) / args.steps
         rec = {"evals_per_s": len(probs) / dt, "ms_per_step": dt * 1e3,
-               "workspace_gib": B.stats()["workspace_bytes"] / 2**30}
+               "workspace_gib": B.stats()["workspace_bytes"] / 2**30,
+               "tiles": {f"L{p['L']}_d{p['depth']}": B.tiles(i) for i, p in enumerate(probs)}}
         if args.batch_niter > 0:
             st0 = B.stats()
             t0 = time.perf_counter()
@@ -296,6 +337,7 @@ def main():
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
                     help="profiling runs: time one precision only and print a short summary")
     ap.add_argument("--no-profile-pass", action="store_true", help="skip the HIP-event kernel timing pass")
+    ap.add_argument("--mixed-seeds", type=int, default=20, help="seeded gate sets of the mixed-precision error sweep")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-problems section")
     ap.add_argument("--batch-problems", type=int, default=64)
     ap.add_argument("--batch-niter", type=int, default=2, help="lock-step trust-region iterations (0: skip)")

@@ -80,5 +80,10 @@ int rqc_batch_optimize_brickwall_hvp(struct rqc_batch* batch, const struct mat4x
 int rqc_batch_get_stats(const struct rqc_batch* batch, struct rqc_batch_stats* stats);
 /* column block, sweep engine and passes of problem p for an evaluation kind */
 int rqc_batch_problem_plan(const struct rqc_batch* batch, int p, int kind, long* block, int* engine, int* passes);
+/* tile shape (2^rows x 2^cols amplitudes) and the most gates per pass of problem p: chosen by the
+   autotuner at rqc_batch_create (rqc_set_autotune), which times whole HVP steps of the batch for
+   each candidate once per batch structure and precision */
+int rqc_batch_problem_tiles(const struct rqc_batch* batch, int p, int kind, int* tile_rows_log2, int* tile_cols_log2,
+	int* gates_per_pass);
 
 RQC_EXTERN_C_END

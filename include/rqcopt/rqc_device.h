@@ -69,6 +69,12 @@ int rqc_get_precision(void);
    selectable with the environment variable RQC_SWEEP_ENGINE=pergate */
 int rqc_set_sweep_engine(int engine);
 
+/* tile-shape autotuning of batches (process-wide; default on, or the
+   environment variable RQC_AUTOTUNE=0 switches it off): the tile height and
+   the gates per pass are timed on whole steps of the batch, once per batch
+   structure and precision */
+int rqc_set_autotune(int on);
+
 /* device workspace: upper bound in bytes (0 = automatic, about 85% of free HBM);
    the workspace is kept between calls and released here */
 int rqc_set_workspace_limit(size_t bytes);

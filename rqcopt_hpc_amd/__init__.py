@@ -118,7 +118,8 @@ EXPORTED = [
     "rqc_set_sweep_engine", "rqc_last_sweep", "rqc_target_create_circuit",
     # rqc_batch.h
     "rqc_batch_create", "rqc_batch_destroy", "rqc_batch_set_exec", "rqc_batch_evaluate",
-    "rqc_batch_optimize_brickwall_hvp", "rqc_batch_get_stats", "rqc_batch_problem_plan",
+    "rqc_batch_optimize_brickwall_hvp", "rqc_batch_get_stats", "rqc_batch_problem_plan", "rqc_batch_problem_tiles",
+    "rqc_set_autotune",
     # rqc_server.h
     "rqc_server_create", "rqc_server_destroy", "rqc_session_create_brickwall", "rqc_session_create_circuit",
     "rqc_session_ngroups", "rqc_session_destroy", "rqc_server_submit", "rqc_server_wait", "rqc_server_poll",
@@ -646,6 +647,13 @@ class Batch:
                                                                       ct.byref(n)))
         return b.value, e.value, n.value
 
+    def tiles(self, p, kind=EVAL_HVP):
+        """(log2 tile rows, log2 tile columns, most gates per pass) of problem p."""
+        k, c, g = ct.c_int(), ct.c_int(), ct.c_int()
+        _check("rqc_batch_problem_tiles", lib().rqc_batch_problem_tiles(self.handle, p, kind, ct.byref(k), ct.byref(c),
+                                                                        ct.byref(g)))
+        return k.value, c.value, g.value
+
     def close(self):
         if self.handle is not None and self.handle.value:
             lib().rqc_batch_destroy(self.handle)
@@ -864,6 +872,10 @@ class Rng:
 # --------------------------------------------------------------- device controls
 def set_precision(mode):
     _check("rqc_set_precision", lib().rqc_set_precision(mode))
+
+
+def set_autotune(on):
+    _check("rqc_set_autotune", lib().rqc_set_autotune(1 if on else 0))
 
 
 def set_sweep_engine(engine):

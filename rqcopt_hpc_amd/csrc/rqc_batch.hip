@@ -17,7 +17,9 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -39,6 +41,7 @@ struct BatchProblem
 	const zc* dU = nullptr;
 	EvalPlan plan[kKinds];
 	size_t budget = 0;          // state-array budget the plans were made with
+	TileChoice tc;              // tile height and gates per pass (autotuned or default)
 	size_t work_off = 0, work_bytes = 0;
 	size_t tab_off = 0, tab_bytes = 0;
 	size_t res_off = 0, res_bytes = 0;
@@ -77,7 +80,7 @@ struct Batch
 static void plan_problem(const Batch& bt, BatchProblem& b)
 {
 	for (int k = 0; k < kKinds; k++) {
-		b.plan[k] = plan_eval(*bt.dev, b.C, kind_mode(k), b.per, bt.fp32, true, true, b.budget, 0);
+		b.plan[k] = plan_eval(*bt.dev, b.C, kind_mode(k), b.per, bt.fp32, true, true, b.budget, 0, &b.tc);
 		if (!b.plan[k].tiles) {
 			throw Error(-3, "batched problems need the tile-pass engine (even nqubits >= 6, tile-pass engine selected)");
 		}
@@ -109,7 +112,8 @@ static void batch_release(Batch& bt)
 	for (auto& e : bt.ev_join) { (void)hipEventDestroy(e); }
 }
 
-static void batch_init(Batch& bt, int n, const rqc_batch_problem* pr, int precision, int nstreams, size_t limit)
+static void batch_init(Batch& bt, int n, const rqc_batch_problem* pr, int precision, int nstreams, size_t limit,
+	const TileChoice& tc)
 {
 	if (n < 1) { throw Error(-3, "a batch needs at least one problem"); }
 	if (precision != RQC_PRECISION_FP64 && precision != RQC_PRECISION_MIXED) { throw Error(-3, "unknown precision"); }
@@ -130,6 +134,7 @@ static void batch_init(Batch& bt, int n, const rqc_batch_problem* pr, int precis
 		b.budget = (size_t)1 << 62;
 		b.cost = std::ldexp((double)b.C.ngates, 2 * b.L);
 	}
+	for (auto& b : bt.probs) { b.tc = tc; }
 	// fit the workspaces: halve the state budget of the largest one until all fit
 	size_t avail = limit;
 	if (avail == 0) {
@@ -440,13 +445,74 @@ struct rqc_batch
 	Batch b;
 };
 
+// Tile autotuning of a batch: each candidate (tile height and gates per pass,
+// the same for every problem) is timed on whole HVP steps of the batch itself,
+// one warm-up and the best of two, since the problems' kernels overlap and a
+// choice that is faster for a problem alone can be slower beside the others.
+// Results per batch structure (problem shapes, precision, branches) are cached.
+static TileChoice tune_batch(int n, const rqc_batch_problem* pr, int precision, int nstreams, size_t limit)
+{
+	static std::mutex mu;
+	static std::map<std::string, TileChoice> cache;
+	std::string key = std::to_string(precision) + "/" + std::to_string(nstreams) + ":";
+	for (int p = 0; p < n; p++) {
+		key += std::to_string(pr[p].nqubits) + "x" + std::to_string(pr[p].nlayers) + "[";
+		for (int i = 0; i < pr[p].nlayers; i++) {
+			for (int q = 0; q < pr[p].nqubits; q++) { key += std::to_string(pr[p].perms[i][q]) + ","; }
+		}
+		key += "]";
+	}
+	std::lock_guard<std::mutex> lk(mu);
+	auto it = cache.find(key);
+	if (it != cache.end()) { return it->second; }
+	const int base = (precision == RQC_PRECISION_MIXED) ? 10 : 9;
+	const TileChoice cands[] = {{base, 8}, {base + 1, 8}, {base, 6}, {base, 4}};
+	TileChoice best{0, 0};
+	double best_ms = 0;
+	std::vector<int> kinds(n, RQC_EVAL_HVP);
+	for (const TileChoice& c : cands) {
+		Batch tmp;
+		try {
+			batch_init(tmp, n, pr, precision, nstreams, limit, c);
+			std::vector<const struct mat4x4*> vl(n), vd(n);
+			std::vector<std::vector<struct mat4x4>> gates(n);
+			for (int p = 0; p < n; p++) {
+				gates[p].resize(pr[p].nlayers);
+				for (auto& m : gates[p]) {
+					for (int e = 0; e < 16; e++) { m.data[e] = numeric((e % 5 == 0) ? 1.0 : 0.0, 0.0); }
+				}
+				vl[p] = gates[p].data();
+				vd[p] = gates[p].data();
+			}
+			std::vector<EvalResult> R(n);
+			batch_step(tmp, kinds.data(), vl.data(), vd.data(), R);
+			double ms = 0;
+			for (int r = 0; r < 2; r++) {
+				const double before = tmp.stats.device_ms;
+				batch_step(tmp, kinds.data(), vl.data(), vd.data(), R);
+				const double t = tmp.stats.device_ms - before;
+				ms = (r == 0) ? t : std::min(ms, t);
+			}
+			if (best.logE == 0 || ms < best_ms) { best = c; best_ms = ms; }
+		} catch (const Error&) {
+			// a candidate that does not fit (memory, tile shape) is skipped
+		}
+		if (tmp.io.stream != nullptr) { (void)hipStreamSynchronize(tmp.io.stream); }
+		batch_release(tmp);
+	}
+	cache[key] = best;
+	return best;
+}
+
 extern "C" int rqc_batch_create(int nproblems, const struct rqc_batch_problem problems[], int precision, int nstreams,
 	size_t workspace_limit, struct rqc_batch** batch)
 {
 	rqc_batch* h = nullptr;
 	try {
 		h = new rqc_batch();
-		batch_init(h->b, nproblems, problems, precision, nstreams, workspace_limit);
+		TileChoice tc{0, 0};
+		if (ctx().autotune && nproblems > 0) { tc = tune_batch(nproblems, problems, precision, nstreams, workspace_limit); }
+		batch_init(h->b, nproblems, problems, precision, nstreams, workspace_limit, tc);
 		*batch = h;
 		return 0;
 	} catch (const std::exception& e) {
@@ -553,5 +619,16 @@ extern "C" int rqc_batch_problem_plan(const struct rqc_batch* batch, int p, int 
 	if (block != nullptr) { *block = pl.B; }
 	if (engine != nullptr) { *engine = pl.tiles ? RQC_ENGINE_TILE_PASS : RQC_ENGINE_PER_GATE; }
 	if (passes != nullptr) { *passes = pl.passes; }
+	return 0;
+}
+
+extern "C" int rqc_batch_problem_tiles(const struct rqc_batch* batch, int p, int kind, int* tile_rows_log2, int* tile_cols_log2,
+	int* gates_per_pass)
+{
+	if (batch == nullptr || p < 0 || p >= (int)batch->b.probs.size() || kind < 0 || kind >= kKinds) { return -3; }
+	const EvalPlan& pl = batch->b.probs[p].plan[kind];
+	if (tile_rows_log2 != nullptr) { *tile_rows_log2 = pl.tile_K; }
+	if (tile_cols_log2 != nullptr) { *tile_cols_log2 = pl.tile_logC; }
+	if (gates_per_pass != nullptr) { *gates_per_pass = pl.gmax; }
 	return 0;
 }

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <cstring>
 #include <functional>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "rqc_internal.hpp"
@@ -78,6 +80,8 @@ Context& ctx()
 		cc->lds_per_block = p.sharedMemPerBlock;
 		const char* eng = getenv("RQC_SWEEP_ENGINE");
 		if (eng != nullptr && std::string(eng) == "pergate") { cc->engine = RQC_ENGINE_PER_GATE; }
+		const char* at = getenv("RQC_AUTOTUNE");
+		if (at != nullptr && std::string(at) == "0") { cc->autotune = false; }
 		RQC_HIP(hipStreamCreateWithFlags(&cc->stream, hipStreamNonBlocking));
 		c = cc;
 	}
@@ -541,13 +545,14 @@ struct TileShape
 };
 
 template <typename T>
-static TileShape tile_shape(int L, long B)
+static TileShape tile_shape(int L, long B, int logE_choice = 0)
 {
 	const int elem = (int)sizeof(cx<T>);
 	// FP64 512 and FP32 1024 elements (8 KiB per state): the reverse pass double-
 	// buffers four states, 64 KiB per workgroup, two workgroups per CU
 	int logE = (elem == 16) ? 9 : 10;
 	if (const char* v = getenv(elem == 16 ? "RQC_TILE_LOGE_FP64" : "RQC_TILE_LOGE_FP32")) { logE = atoi(v); }
+	if (logE_choice > 0) { logE = logE_choice; }
 	logE = std::max(9, std::min(logE, elem == 16 ? 10 : 11));   // double-buffered reverse states fit 160 KiB
 	const int logCdef = (elem == 16) ? 3 : 4;
 	int logB = 0;
@@ -626,8 +631,10 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, bool fp32, bool project, bool unitary,
-	size_t budget, size_t held)
+	size_t budget, size_t held, const TileChoice* choice)
 {
+	const int logE_c = (choice != nullptr) ? choice->logE : 0;
+	const int gmax = (choice != nullptr && choice->gmax > 0) ? std::min(choice->gmax, (int)kMaxTileGates) : tile_gmax();
 	if (per < 1 || C.ngates % per != 0) { throw Error(-3, "gate groups must divide the gate count"); }
 	if (mode == Mode::Hvp && C.dirs.size() != C.gates.size()) { throw Error(-3, "an HVP evaluation needs gate directions"); }
 	EvalPlan pl;
@@ -645,20 +652,23 @@ EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, boo
 	TileShape ts{0, 0};
 	std::vector<Pass> passes;
 	if (tiles) {
-		ts = fp32 ? tile_shape<float>(C.L, pl.N) : tile_shape<double>(C.L, pl.N);
+		ts = fp32 ? tile_shape<float>(C.L, pl.N, logE_c) : tile_shape<double>(C.L, pl.N, logE_c);
 		tiles = ts.usable();
 	}
 	if (tiles) {
-		passes = plan_passes(C, ts.K, tile_gmax());
+		passes = plan_passes(C, ts.K, gmax);
 		const int P = (int)passes.size();
 		pl.nst = (mode == Mode::Target) ? 2 : (mode == Mode::Grad) ? P + 2 : 2 * P + 4;
 		pl.B = choose_block(pl.N, pl.nst, elem, budget, held);
-		ts = fp32 ? tile_shape<float>(C.L, pl.B) : tile_shape<double>(C.L, pl.B);
+		ts = fp32 ? tile_shape<float>(C.L, pl.B, logE_c) : tile_shape<double>(C.L, pl.B, logE_c);
 		tiles = ts.usable();
 	}
 	if (tiles) {
 		pl.tiles = true;
 		pl.passes = (int)passes.size();
+		pl.tile_K = ts.K;
+		pl.tile_logC = ts.logC;
+		pl.gmax = gmax;
 		pl.tp = fp32 ? plan_tiles<float>(dev, C, pl, passes, ts) : plan_tiles<double>(dev, C, pl, passes, ts);
 	} else {
 		pl.nst = states_needed(mode, C.ngates);
@@ -1712,6 +1722,11 @@ extern "C" int rqc_set_sweep_engine(int engine)
 {
 	if (engine != RQC_ENGINE_TILE_PASS && engine != RQC_ENGINE_PER_GATE) { return -3; }
 	return target_call("rqc_set_sweep_engine", [&] { ctx().engine = engine; });
+}
+
+extern "C" int rqc_set_autotune(int on)
+{
+	return target_call("rqc_set_autotune", [&] { ctx().autotune = (on != 0); });
 }
 
 extern "C" int rqc_get_precision(void)

@@ -44,6 +44,13 @@ bool gates_near_unitary(const Circuit& C);
 
 struct TilePlan;  // tile-pass specific part of a plan (passes, pass arguments, grids)
 
+// tile height (log2 of the elements of a tile) and gates per pass; 0 = default
+struct TileChoice
+{
+	int logE = 0;
+	int gmax = 0;
+};
+
 struct EvalPlan
 {
 	Mode mode = Mode::Target;
@@ -60,6 +67,7 @@ struct EvalPlan
 	// workspace layout: byte offsets from a 256-byte aligned base
 	size_t off_acc = 0, off_part = 0, off_tab = 0, off_res = 0, off_states = 0, bytes = 0;
 	size_t mats_bytes = 0, terms_bytes = 0, tab_bytes = 0;
+	int tile_K = 0, tile_logC = 0, gmax = 0;   // tile shape and gates per pass (tile-pass plans)
 	std::shared_ptr<const TilePlan> tp;
 
 	size_t res_bytes() const { return sizeof(zc) * (1 + 32 * (size_t)ng) + sizeof(double) * 32 * (size_t)ng; }
@@ -70,7 +78,7 @@ struct EvalPlan
 // gradient and HVP sweeps only then); `budget`: bytes for the state arrays
 // (0 = automatic, 85% of the free device memory plus `held`)
 EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, bool fp32, bool project, bool unitary,
-	size_t budget, size_t held);
+	size_t budget, size_t held, const TileChoice* choice = nullptr);
 // host image of the evaluation's table region (pl.tab_bytes bytes at `host`)
 void fill_eval_tables(const EvalPlan& pl, const Circuit& C, char* host);
 // enqueue the evaluation on slot s into workspace `work` (pl.bytes).  The table

@@ -71,6 +71,7 @@ struct Context : Slot
 	int engine = RQC_ENGINE_TILE_PASS;
 	size_t lds_per_block = 65536;
 	size_t ws_limit = 0;  // 0 = automatic
+	bool autotune = true;  // tile autotuning of batched problems
 
 	DeviceBuffer work;     // workspace of target evaluations (states, partials, tables, results)
 	DeviceBuffer small;    // accumulators of the statevector-level API

@@ -160,3 +160,32 @@ def test_batch_rejects_invalid_problems():
     with pytest.raises(R.RqcError):
         B.evaluate([R.EVAL_GRADIENT], [2 * probs[0]["V"]])
     B.close()
+
+
+def test_batch_autotuned_tiles_keep_parity():
+    """Autotuned tile shapes and pass sizes change only the rounding: FP64 results stay within 1e-12
+    of the oracle and of the default-tile batch."""
+    probs = _problems(6, [(8, 3), (10, 5), (6, 2)])
+    kinds = [R.EVAL_HVP] * len(probs)
+    V = [p["V"] for p in probs]
+    Z = [p["Z"] for p in probs]
+    R.set_autotune(False)
+    try:
+        Bd = R.Batch([(p["P"], p["T"]) for p in probs])
+    finally:
+        R.set_autotune(True)
+    Ba = R.Batch([(p["P"], p["T"]) for p in probs])
+    fd, gd, hd = Bd.evaluate(kinds, V, Z)
+    fa, ga, ha = Ba.evaluate(kinds, V, Z)
+    for i, p in enumerate(probs):
+        K, logC, gmax = Ba.tiles(i)
+        print("problem", i, "L", p["L"], "tiles", (K, logC, gmax), "default", Bd.tiles(i), flush=True)
+        assert K + logC in (9, 10) and 1 <= gmax <= 8
+        assert abs(fa[i] - fd[i]) < TOL * max(1, abs(fd[i]))
+        assert reldiff(ga[i], gd[i]) < TOL and reldiff(ha[i], hd[i]) < TOL
+        if p["L"] <= 8:
+            Uo = O.Ufunc.dense(p["T"].download())
+            f_o, g_o, h_o = O.brickwall_unitary_target_projected_hvp(Uo, p["V"], p["Z"], p["P"], p["L"])
+            assert abs(fa[i] - f_o) < TOL * max(1, abs(f_o)) and reldiff(ga[i], g_o) < TOL and reldiff(ha[i], h_o) < TOL
+    Bd.close()
+    Ba.close()
