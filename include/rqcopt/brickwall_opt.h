@@ -15,4 +15,8 @@ void optimize_brickwall_circuit_hvp(linear_func ufunc, void* udata,
 	const struct mat4x4 vlist_start[], const int nlayers, const int nqubits, const int* perms[],
 	struct rtr_params* params, const int niter, double* f_iter, struct mat4x4 vlist_opt[]);
 
+void optimize_brickwall_circuit_hmat(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist_start[], const int nlayers, const int nqubits, const int* perms[],
+	struct rtr_params* params, const int niter, double* f_iter, struct mat4x4 vlist_opt[]);
+
 RQC_EXTERN_C_END

@@ -77,4 +77,16 @@ int brickwall_unitary_target_projected_hessian_vector_product(linear_func ufunc,
 	const struct mat4x4 vlist[], const struct mat4x4 vdirs[], const int nlayers, const int* perms[], const int nqubits,
 	numeric* fval, double* RQC_RESTRICT grad_vec, double* RQC_RESTRICT hvp_vec);
 
+/* reference src/target.c:687-797: f, the gate gradients and the complex Hessian
+   hess[((i*16 + x)*nlayers + j)*16 + y] ((16 nlayers)^2 numerics) */
+int brickwall_unitary_target_gradient_hessian(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	numeric* fval, struct mat4x4 dvlist[], numeric* hess);
+
+/* reference src/target.c:1003-1040 (projection :937-994): f, the projected gradient (16 nlayers reals)
+   and the Riemannian Hessian matrix H ((16 nlayers)^2 reals, row-major, symmetric) */
+int brickwall_unitary_target_gradient_vector_hessian_matrix(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	numeric* fval, double* grad_vec, double* H);
+
 RQC_EXTERN_C_END

@@ -337,6 +337,85 @@ def optimize_brickwall_circuit_hvp(uf, vlist_start, perms, L, niter):
     return f_iter, vopt
 
 
+# ------------------------------------------------------------------- full-Hessian route
+def apply_gate_placeholder(i, j, psi):
+    psi = _cplx(psi)
+    L = int(np.log2(psi.size))
+    out = np.empty((psi.size, 16), np.complex128)
+    lib().orc_apply_gate_placeholder(_i(i), _i(j), _i(L), _p(psi), _p(out))
+    return out
+
+
+def apply_gate_backward_array(i, j, psi, dpsi_out):
+    psi, dpsi_out = _cplx(psi), _cplx(dpsi_out)
+    L = int(np.log2(dpsi_out.size))
+    ns = psi.shape[1]
+    dg = np.empty((ns, 4, 4), np.complex128)
+    lib().orc_apply_gate_backward_array(_i(i), _i(j), _i(L), _i(ns), _p(psi), _p(dpsi_out), _p(dg))
+    return dg
+
+
+def brickwall_unitary_backward_hessian(vlist, perms, cache, dpsi_out):
+    vlist, perms, cache, dpsi_out = _cplx(vlist), _i32(perms), _cplx(cache), _cplx(dpsi_out)
+    L = perms.shape[1]
+    m = 16 * len(vlist)
+    dpsi = np.empty_like(dpsi_out)
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    hess = np.empty((m, m), np.complex128)
+    r = lib().orc_brickwall_unitary_backward_hessian(_p(vlist), _i(len(vlist)), _p(perms), _i(L), _p(cache),
+                                                      _p(dpsi_out), _p(dpsi), _p(dv), _p(hess))
+    assert r == 0
+    return dpsi, dv, hess
+
+
+def brickwall_unitary_target_gradient_hessian(uf, vlist, perms, L):
+    vlist, perms = _cplx(vlist), _i32(perms)
+    m = 16 * len(vlist)
+    f = _fval()
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    hess = np.empty((m, m), np.complex128)
+    fp, ud = uf.args()
+    r = lib().orc_brickwall_unitary_target_gradient_hessian(fp, ud, _p(vlist), _i(len(vlist)), _i(L), _p(perms), _p(f),
+                                                           _p(dv), _p(hess))
+    assert r == 0
+    return f[0], dv, hess
+
+
+def brickwall_unitary_target_gradient_vector_hessian_matrix(uf, vlist, perms, L):
+    vlist, perms = _cplx(vlist), _i32(perms)
+    m = 16 * len(vlist)
+    f = _fval()
+    g = np.empty(m)
+    H = np.empty((m, m))
+    fp, ud = uf.args()
+    r = lib().orc_brickwall_unitary_target_gradient_vector_hessian_matrix(fp, ud, _p(vlist), _i(len(vlist)), _i(L),
+                                                                         _p(perms), _p(f), _p(g), _p(H))
+    assert r == 0
+    return f[0], g, H
+
+
+def truncated_cg_hmat(grad, hess, radius, maxiter=None, abstol=1e-8, reltol=1e-6):
+    grad = np.ascontiguousarray(grad, np.float64)
+    hess = np.ascontiguousarray(hess, np.float64)
+    n = grad.size
+    z = np.empty(n)
+    L = lib()
+    L.orc_truncated_cg_hmat.restype = ct.c_bool
+    on = L.orc_truncated_cg_hmat(_p(grad), _p(hess), _i(n), ct.c_double(radius), _i(2 * n if maxiter is None else maxiter),
+                                 ct.c_double(abstol), ct.c_double(reltol), _p(z))
+    return z, bool(on)
+
+
+def optimize_brickwall_circuit_hmat(uf, vlist_start, perms, L, niter):
+    vlist_start, perms = _cplx(vlist_start), _i32(perms)
+    f_iter = np.empty(niter + 1)
+    vopt = np.empty_like(vlist_start)
+    fp, ud = uf.args()
+    lib().orc_optimize_brickwall_circuit_hmat(fp, ud, _p(vlist_start), _i(len(vlist_start)), _i(L), _p(perms),
+                                              _i(niter), _p(f_iter), _p(vopt))
+    return f_iter, vopt
+
+
 # ------------------------------------------------------------------- algebra
 def _mat_fn(name, nin):
     def f(*args):

@@ -865,6 +865,276 @@ void orc_optimize_brickwall_circuit_hvp(orc_linear_func ufunc, void* udata, cons
 }
 
 /* ---------------------------------------------------------------------------
+ * Full-Hessian route.
+ */
+
+/* the four amplitude offsets of the quad whose other bits are those of `base`,
+   in the gate basis |q_i q_j> (wire i the more significant index) */
+static void quad_offsets4(long base, int L, int i, int j, long off[4])
+{
+	const long si = 1L << (L - 1 - i), sj = 1L << (L - 1 - j);
+	off[0] = base; off[1] = base + sj; off[2] = base + si; off[3] = base + si + sj;
+}
+
+/* reference src/gate.c:374-460: state 4r + c of the output array is E_rc psi,
+   i.e. psi's (i, j) = c component moved to (i, j) = r */
+void orc_apply_gate_placeholder(int i, int j, int L, const ocplx* psi, ocplx* psi_out)
+{
+	const long n = 1L << L;
+	memset(psi_out, 0, (size_t)n * 16 * sizeof(ocplx));
+	const int lo = i < j ? i : j, hi = i < j ? j : i;
+	const long nq = 1L << (L - 2);
+	for (long q = 0; q < nq; q++) {
+		long off[4];
+		quad_offsets4(quad_base(q, L, lo, hi), L, i, j, off);
+		for (int r = 0; r < 4; r++) {
+			for (int c = 0; c < 4; c++) {
+				psi_out[off[r] * 16 + 4*r + c] = psi[off[c]];
+			}
+		}
+	}
+}
+
+/* reference src/gate.c:269-366 */
+void orc_apply_gate_backward_array(int i, int j, int L, int nstates, const ocplx* psi, const ocplx* dpsi_out, ocplx* dgates)
+{
+	for (int e = 0; e < 16 * nstates; e++) { dgates[e] = 0; }
+	const int lo = i < j ? i : j, hi = i < j ? j : i;
+	const long nq = 1L << (L - 2);
+	for (long q = 0; q < nq; q++) {
+		long off[4];
+		quad_offsets4(quad_base(q, L, lo, hi), L, i, j, off);
+		for (int k = 0; k < nstates; k++) {
+			for (int r = 0; r < 4; r++) {
+				for (int c = 0; c < 4; c++) {
+					dgates[16*k + 4*r + c] += dpsi_out[off[r]] * psi[off[c] * nstates + k];
+				}
+			}
+		}
+	}
+}
+
+/* reference src/brickwall_circuit.c:134-255.  Gradient sweep storing the adjoint
+   after every gate; then, for every gate position, the 16 placeholder states are
+   carried forward through the later gates and contracted with the adjoint at
+   each of them (pairs of one layer: only later pairs); diagonal blocks are
+   symmetrised and the lower blocks copied from the upper ones. */
+int orc_brickwall_unitary_backward_hessian(const ocplx* vlist, int nlayers, const int* perms, int L, const ocplx* cache,
+	const ocplx* dpsi_out, ocplx* dpsi, ocplx* dvlist, ocplx* hess)
+{
+	const long n = 1L << L;
+	const int per = L / 2, ng = nlayers * per, m = 16 * nlayers;
+	ocplx* adj = xalloc((size_t)ng * n * sizeof(ocplx));   /* adjoint after gate g */
+	memcpy(&adj[(size_t)(ng - 1) * n], dpsi_out, n * sizeof(ocplx));
+	for (int i = 0; i < nlayers; i++) { for (int e = 0; e < 16; e++) { dvlist[16*i + e] = 0; } }
+	for (int g = ng - 1; g >= 0; g--) {
+		const int i = g / per, k = g % per;
+		ocplx dV[16];
+		ocplx* out = (g > 0) ? &adj[(size_t)(g - 1) * n] : dpsi;
+		orc_apply_gate_backward(&vlist[16*i], perms[i*L + 2*k], perms[i*L + 2*k + 1], L, &cache[(size_t)g * n],
+			&adj[(size_t)g * n], out, dV);
+		for (int e = 0; e < 16; e++) { dvlist[16*i + e] += dV[e]; }
+	}
+	memset(hess, 0, (size_t)m * m * sizeof(ocplx));
+	ocplx* t0 = xalloc((size_t)n * 16 * sizeof(ocplx));
+	ocplx* t1 = xalloc((size_t)n * 16 * sizeof(ocplx));
+	ocplx h[256];
+	for (int g = 0; g < ng; g++) {
+		const int i = g / per, r = g % per;
+		orc_apply_gate_placeholder(perms[i*L + 2*r], perms[i*L + 2*r + 1], L, &cache[(size_t)g * n], t0);
+		ocplx* cur = t0;
+		ocplx* nxt = t1;
+		for (int l = g + 1; l < ng; l++) {
+			const int j = l / per, s2 = l % per;
+			const int wi = perms[j*L + 2*s2], wj = perms[j*L + 2*s2 + 1];
+			orc_apply_gate_backward_array(wi, wj, L, 16, cur, &adj[(size_t)l * n], h);
+			for (int x = 0; x < 16; x++) {
+				for (int y = 0; y < 16; y++) {
+					hess[(size_t)(i*16 + x) * m + j*16 + y] += h[16*x + y];
+				}
+			}
+			if (l < ng - 1) {
+				orc_apply_gate_to_array(&vlist[16*j], wi, wj, L, 16, cur, nxt);
+				ocplx* t = cur; cur = nxt; nxt = t;
+			}
+		}
+	}
+	for (int i = 0; i < nlayers; i++) {
+		for (int x = 0; x < 16; x++) {
+			for (int y = 0; y <= x; y++) {
+				const size_t a = (size_t)(i*16 + x) * m + i*16 + y, b = (size_t)(i*16 + y) * m + i*16 + x;
+				const ocplx sum = hess[a] + hess[b];
+				hess[a] = sum;
+				hess[b] = sum;
+			}
+		}
+		for (int j = 0; j < i; j++) {
+			for (int x = 0; x < 16; x++) {
+				for (int y = 0; y < 16; y++) {
+					hess[(size_t)(i*16 + x) * m + j*16 + y] = hess[(size_t)(j*16 + y) * m + i*16 + x];
+				}
+			}
+		}
+	}
+	free(t1); free(t0); free(adj);
+	return 0;
+}
+
+/* reference src/target.c:687-797: per basis column, forward with cache, trace
+   against -conj(U e_b), backward Hessian; all summed over the columns */
+int orc_brickwall_unitary_target_gradient_hessian(orc_linear_func ufunc, void* udata, const ocplx* vlist, int nlayers,
+	int L, const int* perms, ocplx* fval, ocplx* dvlist, ocplx* hess)
+{
+	const long n = 1L << L;
+	const int ng = nlayers * (L / 2), m = 16 * nlayers;
+	ocplx* psi = xalloc(n * sizeof(ocplx));
+	ocplx* upsi = xalloc(n * sizeof(ocplx));
+	ocplx* wpsi = xalloc(n * sizeof(ocplx));
+	ocplx* cache = xalloc((size_t)ng * n * sizeof(ocplx));
+	ocplx* dvu = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
+	ocplx* hu = xalloc((size_t)m * m * sizeof(ocplx));
+	for (int e = 0; e < 16 * nlayers; e++) { dvlist[e] = 0; }
+	memset(hess, 0, (size_t)m * m * sizeof(ocplx));
+	ocplx f = 0;
+	int ret = 0;
+	for (long b = 0; b < n && ret == 0; b++) {
+		if (adjoint_seed(ufunc, udata, L, b, psi, upsi) < 0) { ret = -2; break; }
+		orc_brickwall_unitary_forward(vlist, nlayers, perms, L, psi, cache, wpsi);
+		for (long a = 0; a < n; a++) { f += upsi[a] * wpsi[a]; }
+		orc_brickwall_unitary_backward_hessian(vlist, nlayers, perms, L, cache, upsi, psi, dvu, hu);
+		for (int e = 0; e < 16 * nlayers; e++) { dvlist[e] += dvu[e]; }
+		for (size_t e = 0; e < (size_t)m * m; e++) { hess[e] += hu[e]; }
+	}
+	*fval = f;
+	free(hu); free(dvu); free(cache); free(wpsi); free(upsi); free(psi);
+	return ret;
+}
+
+/* reference src/target.c:937-994: upper blocks (i <= j) of the Riemannian Hessian,
+   column k of block (i, j) from the tangent basis vector k at V_j; the diagonal
+   blocks also carry the curvature term of the gradient; lower half by symmetry */
+static void project_hessian_matrix(int nlayers, const ocplx* vlist, const ocplx* dvc, const ocplx* hess, double* H)
+{
+	const int m = 16 * nlayers;
+	for (int i = 0; i < nlayers; i++) {
+		for (int j = i; j < nlayers; j++) {
+			for (int k = 0; k < 16; k++) {
+				double r[16] = { 0 };
+				r[k] = 1;
+				ocplx Z[16], G[16];
+				orc_real_to_tangent(r, &vlist[16*j], Z);
+				for (int x = 0; x < 16; x++) {
+					ocplx s = 0;
+					for (int y = 0; y < 16; y++) { s += hess[(size_t)(i*16 + x) * m + j*16 + y] * Z[y]; }
+					G[x] = conj(s);
+				}
+				if (i == j) {
+					ocplx P[16], gh[16], T[16];
+					orc_project_tangent(&vlist[16*i], G, P);
+					orc_adjoint(&dvc[16*i], gh);
+					orc_symmetric_triple_matrix_product(Z, gh, &vlist[16*i], T);
+					for (int e = 0; e < 16; e++) { G[e] = P[e] - T[e]; }
+				}
+				orc_tangent_to_real(&vlist[16*i], G, r);
+				for (int x = 0; x < 16; x++) { H[(size_t)(i*16 + x) * m + j*16 + k] = r[x]; }
+			}
+		}
+	}
+	for (int a = 0; a < m; a++) {
+		for (int b = 0; b < a; b++) { H[(size_t)a * m + b] = H[(size_t)b * m + a]; }
+	}
+}
+
+/* reference src/target.c:1003-1040 */
+int orc_brickwall_unitary_target_gradient_vector_hessian_matrix(orc_linear_func ufunc, void* udata, const ocplx* vlist,
+	int nlayers, int L, const int* perms, ocplx* fval, double* grad_vec, double* H)
+{
+	const int m = 16 * nlayers;
+	ocplx* dv = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
+	ocplx* hess = xalloc((size_t)m * m * sizeof(ocplx));
+	int ret = orc_brickwall_unitary_target_gradient_hessian(ufunc, udata, vlist, nlayers, L, perms, fval, dv, hess);
+	if (ret == 0) {
+		for (int i = 0; i < nlayers; i++) {
+			for (int e = 0; e < 16; e++) { dv[16*i + e] = conj(dv[16*i + e]); }
+			orc_tangent_to_real(&vlist[16*i], &dv[16*i], &grad_vec[16*i]);
+		}
+		project_hessian_matrix(nlayers, vlist, dv, hess, H);
+	}
+	free(hess); free(dv);
+	return ret;
+}
+
+/* reference src/trust_region.c:83-141 (Steihaug tCG with an explicit Hessian) */
+bool orc_truncated_cg_hmat(const double* grad, const double* hess, int n, double radius, int maxiter, double abstol,
+	double reltol, double* z)
+{
+	double* r = xalloc(n * sizeof(double));
+	double* d = xalloc(n * sizeof(double));
+	double* hd = xalloc(n * sizeof(double));
+	memcpy(r, grad, n * sizeof(double));
+	double rsq = ddot(n, r, r);
+	const double stoptol = fmax(abstol, reltol * sqrt(rsq));
+	for (int i = 0; i < n; i++) { z[i] = 0; d[i] = -r[i]; }
+	bool boundary = false;
+	for (int it = 0; it < maxiter; it++) {
+		for (int a = 0; a < n; a++) { hd[a] = ddot(n, &hess[(size_t)a * n], d); }
+		const double dhd = ddot(n, d, hd);
+		const double t = move_to_boundary(z, d, n, radius);
+		const double alpha = rsq / dhd;
+		if (dhd <= 0 || alpha > t) {
+			for (int i = 0; i < n; i++) { z[i] += t * d[i]; }
+			boundary = true;
+			break;
+		}
+		for (int i = 0; i < n; i++) { r[i] += alpha * hd[i]; }
+		for (int i = 0; i < n; i++) { z[i] += alpha * d[i]; }
+		const double rsq_next = ddot(n, r, r);
+		if (sqrt(rsq_next) <= stoptol) { break; }
+		const double beta = rsq_next / rsq;
+		for (int i = 0; i < n; i++) { d[i] = beta * d[i] - r[i]; }
+		rsq = rsq_next;
+	}
+	free(hd); free(d); free(r);
+	return boundary;
+}
+
+/* RTR with the full Hessian (reference src/trust_region.c:230-293) on the
+   brickwall target (reference src/brickwall_opt.c f_deriv_hess and
+   optimize_brickwall_circuit_hmat), default parameters */
+void orc_optimize_brickwall_circuit_hmat(orc_linear_func ufunc, void* udata, const ocplx* vlist_start, int nlayers,
+	int L, const int* perms, int niter, double* f_iter, ocplx* vlist_opt)
+{
+	struct bw_data dd = { ufunc, udata, perms, nlayers, L };
+	const int n = 16 * nlayers, m = 32 * nlayers;
+	const int maxiter = 2 * n;
+	const double abstol = 1e-8, reltol = 1e-6, rho_trust = 0.125, maxradius = 0.1;
+	double radius = 0.01;
+	double* x = xalloc(m * sizeof(double));
+	double* xn = xalloc(m * sizeof(double));
+	double* grad = xalloc(n * sizeof(double));
+	double* H = xalloc((size_t)n * n * sizeof(double));
+	double* eta = xalloc(n * sizeof(double));
+	double* heta = xalloc(n * sizeof(double));
+	memcpy(x, vlist_start, m * sizeof(double));
+	for (int k = 0; k < niter; k++) {
+		ocplx fv;
+		orc_brickwall_unitary_target_gradient_vector_hessian_matrix(ufunc, udata, (const ocplx*)x, nlayers, L, perms, &fv, grad, H);
+		const double fx = creal(fv);
+		const bool on_boundary = orc_truncated_cg_hmat(grad, H, n, radius, maxiter, abstol, reltol, eta);
+		for (int i = 0; i < nlayers; i++) { orc_retract((const ocplx*)&x[32*i], &eta[16*i], (ocplx*)&xn[32*i]); }
+		f_iter[k] = fx;
+		for (int a = 0; a < n; a++) { heta[a] = ddot(n, &H[(size_t)a * n], eta); }
+		const double rho = (bw_f(xn, &dd) - fx) / (ddot(n, grad, eta) + 0.5 * ddot(n, eta, heta));
+		if (rho < 0.25) { radius *= 0.25; }
+		else if (rho > 0.75 && on_boundary) { radius = fmin(2 * radius, maxradius); }
+		if (rho > rho_trust) { double* t = x; x = xn; xn = t; }
+	}
+	memcpy(vlist_opt, x, m * sizeof(double));
+	f_iter[niter] = bw_f(x, &dd);
+	free(heta); free(eta); free(H); free(grad); free(xn); free(x);
+}
+
+/* ---------------------------------------------------------------------------
  * RNG: PCG32 (O'Neill 2014, XSH-RR output), two generators combined to 64 bits,
  * reference src/pcg_basic.c:30-51,112-132 and src/rng.c:13-25,91-123,152-157.
  */

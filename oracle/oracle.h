@@ -117,6 +117,25 @@ bool orc_truncated_cg_hvp(const double* x, const double* grad, orc_hvp_func f_hv
 void orc_optimize_brickwall_circuit_hvp(orc_linear_func ufunc, void* udata, const ocplx* vlist_start, int nlayers,
 	int L, const int* perms, int niter, double* f_iter, ocplx* vlist_opt);
 
+/* ---- full-Hessian route: reference src/gate.c:269-460, src/brickwall_circuit.c:134-255,
+        src/target.c:687-797,937-1040, src/trust_region.c:83-141,230-293, src/brickwall_opt.c ---- */
+/* psi_out: 2^L x 16 interleaved (state 4r + c = E_rc psi on wires (i, j)) */
+void orc_apply_gate_placeholder(int i, int j, int L, const ocplx* psi, ocplx* psi_out);
+/* dgates[16 k + 4 r + c] = sum dpsi_out[r-amplitude] psi[c-amplitude, state k] */
+void orc_apply_gate_backward_array(int i, int j, int L, int nstates, const ocplx* psi, const ocplx* dpsi_out, ocplx* dgates);
+/* cache: nlayers*L/2 states (before each gate); hess: (16 nlayers)^2 */
+int orc_brickwall_unitary_backward_hessian(const ocplx* vlist, int nlayers, const int* perms, int L, const ocplx* cache,
+	const ocplx* dpsi_out, ocplx* dpsi, ocplx* dvlist, ocplx* hess);
+int orc_brickwall_unitary_target_gradient_hessian(orc_linear_func ufunc, void* udata, const ocplx* vlist, int nlayers,
+	int L, const int* perms, ocplx* fval, ocplx* dvlist, ocplx* hess);
+int orc_brickwall_unitary_target_gradient_vector_hessian_matrix(orc_linear_func ufunc, void* udata, const ocplx* vlist,
+	int nlayers, int L, const int* perms, ocplx* fval, double* grad_vec, double* H);
+bool orc_truncated_cg_hmat(const double* grad, const double* hess, int n, double radius, int maxiter, double abstol,
+	double reltol, double* z);
+/* RTR with the full Hessian, default parameters; f_iter has niter + 1 entries */
+void orc_optimize_brickwall_circuit_hmat(orc_linear_func ufunc, void* udata, const ocplx* vlist_start, int nlayers,
+	int L, const int* perms, int niter, double* f_iter, ocplx* vlist_opt);
+
 /* ---- RNG: reference src/rng.c, src/pcg_basic.c ---- */
 struct orc_pcg32 { uint64_t state, inc; };
 struct orc_rng { struct orc_pcg32 gen[2]; };

@@ -88,18 +88,20 @@ EXPORTED = [
     "allocate_statevector", "free_statevector", "add_statevectors", "allocate_statevector_array",
     "free_statevector_array",
     # gate.h
-    "apply_gate", "apply_gate_backward", "apply_gate_to_array",
+    "apply_gate", "apply_gate_backward", "apply_gate_to_array", "apply_gate_backward_array", "apply_gate_placeholder",
     # quantum_circuit.h
     "apply_quantum_circuit", "allocate_quantum_circuit_cache", "free_quantum_circuit_cache",
     "quantum_circuit_forward", "quantum_circuit_backward", "quantum_circuit_gates_hessian_vector_product",
     # brickwall_circuit.h
     "apply_brickwall_unitary", "brickwall_unitary_forward", "brickwall_unitary_backward",
+    "brickwall_unitary_backward_hessian",
     # target.h
     "circuit_unitary_target", "circuit_unitary_target_and_gradient", "circuit_unitary_target_and_projected_gradient",
     "circuit_unitary_target_hessian_vector_product", "circuit_unitary_target_projected_hessian_vector_product",
     "brickwall_unitary_target", "brickwall_unitary_target_and_gradient",
     "brickwall_unitary_target_and_projected_gradient", "brickwall_unitary_target_hessian_vector_product",
-    "brickwall_unitary_target_projected_hessian_vector_product",
+    "brickwall_unitary_target_projected_hessian_vector_product", "brickwall_unitary_target_gradient_hessian",
+    "brickwall_unitary_target_gradient_vector_hessian_matrix",
     # matrix.h
     "zero_matrix", "identity_matrix", "scale_matrix", "conjugate_matrix", "add_matrix", "sub_matrix",
     "add_matrices", "sub_matrices", "transpose", "adjoint", "symm", "antisymm", "real_to_antisymm",
@@ -110,6 +112,7 @@ EXPORTED = [
     "rand_uint32", "rand_uint64", "randu", "randn", "crandn",
     # trust_region.h, brickwall_opt.h
     "truncated_cg_hvp", "riemannian_trust_region_optimize_hvp", "optimize_brickwall_circuit_hvp",
+    "truncated_cg_hmat", "riemannian_trust_region_optimize_hmat", "optimize_brickwall_circuit_hmat",
     # rqc_device.h
     "rqc_device_ufunc", "rqc_target_create_dense", "rqc_target_create_from_ufunc", "rqc_target_create_brickwall",
     "rqc_target_destroy", "rqc_target_nqubits", "rqc_target_download", "rqc_set_precision", "rqc_get_precision",
@@ -423,6 +426,40 @@ def brickwall_unitary_backward(vlist, perms, cache_states, dpsi_out):
     return dpsi, dv
 
 
+def apply_gate_backward_array(V, i, j, psi, dpsi_out):
+    """Gate gradients of every state of the (2^L, nstates) array psi: (nstates, 4, 4)."""
+    psi, dpsi_out = _cplx(psi), _cplx(dpsi_out)
+    L = _nq(psi.shape[0])
+    dg = np.empty((psi.shape[1], 4, 4), np.complex128)
+    a = StatevectorArray(psi.ctypes.data, L, psi.shape[1])
+    lib().apply_gate_backward_array(_p(_cplx(V)), i, j, ct.byref(a), ct.byref(_sv(dpsi_out)), _p(dg))
+    return dg
+
+
+def apply_gate_placeholder(i, j, psi):
+    """The 16 states E_rc psi (state 4r + c) as a (2^L, 16) array."""
+    psi = _cplx(psi)
+    L = _nq(psi.size)
+    out = np.empty((psi.size, 16), np.complex128)
+    lib().apply_gate_placeholder(i, j, ct.byref(_sv(psi)), ct.byref(StatevectorArray(out.ctypes.data, L, 16)))
+    return out
+
+
+def brickwall_unitary_backward_hessian(vlist, perms, cache_states, dpsi_out):
+    vlist, dpsi_out = _gates(vlist), _cplx(dpsi_out)
+    perms, rows = _perms(perms)
+    L = _nq(dpsi_out.size)
+    cache = _Cache(L, len(vlist) * (L // 2), cache_states)
+    m = 16 * len(vlist)
+    dpsi = np.empty_like(dpsi_out)
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    hess = np.empty((m, m), np.complex128)
+    _check("brickwall_unitary_backward_hessian",
+           lib().brickwall_unitary_backward_hessian(_p(vlist), len(vlist), rows, ct.byref(cache.c), ct.byref(_sv(dpsi_out)),
+                                                    ct.byref(_sv(dpsi)), _p(dv), _p(hess)))
+    return dpsi, dv, hess
+
+
 # --------------------------------------------------------------- targets
 def _f():
     return np.zeros(1, np.complex128)
@@ -532,6 +569,35 @@ def brickwall_unitary_target_projected_hessian_vector_product(target, vlist, vdi
     return f[0], g, h
 
 
+def brickwall_unitary_target_gradient_hessian(target, vlist, perms, nqubits=None):
+    """f, dvlist and the complex Hessian hess[(i*16 + x), (j*16 + y)] (16 nlayers square)."""
+    vlist, f = _gates(vlist), _f()
+    perms, rows = _perms(perms)
+    L = perms.shape[1] if nqubits is None else nqubits
+    m = 16 * len(vlist)
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    hess = np.empty((m, m), np.complex128)
+    fp, ud = _target_args(target)
+    _check("brickwall_unitary_target_gradient_hessian",
+           lib().brickwall_unitary_target_gradient_hessian(fp, ud, _p(vlist), len(vlist), L, rows, _p(f), _p(dv), _p(hess)))
+    return f[0], dv, hess
+
+
+def brickwall_unitary_target_gradient_vector_hessian_matrix(target, vlist, perms, nqubits=None):
+    """f, the projected gradient and the Riemannian Hessian matrix H (real, symmetric)."""
+    vlist, f = _gates(vlist), _f()
+    perms, rows = _perms(perms)
+    L = perms.shape[1] if nqubits is None else nqubits
+    m = 16 * len(vlist)
+    g = np.empty(m)
+    H = np.empty((m, m))
+    fp, ud = _target_args(target)
+    _check("brickwall_unitary_target_gradient_vector_hessian_matrix",
+           lib().brickwall_unitary_target_gradient_vector_hessian_matrix(fp, ud, _p(vlist), len(vlist), L, rows, _p(f), _p(g),
+                                                                        _p(H)))
+    return f[0], g, H
+
+
 # --------------------------------------------------------------- optimiser
 def rtr_default_params(n):
     """set_rtr_default_params (reference src/trust_region.h:68-76)."""
@@ -549,6 +615,31 @@ def optimize_brickwall_circuit_hvp(target, vlist_start, perms, niter, params=Non
     lib().optimize_brickwall_circuit_hvp(fp, ud, _p(vlist_start), nl, perms.shape[1], rows, ct.byref(params), niter,
                                          _p(f_iter), _p(vopt))
     return f_iter, vopt
+
+
+def optimize_brickwall_circuit_hmat(target, vlist_start, perms, niter, params=None):
+    vlist_start = _gates(vlist_start)
+    perms, rows = _perms(perms)
+    nl = len(vlist_start)
+    params = rtr_default_params(16 * nl) if params is None else params
+    f_iter = np.empty(niter + 1)
+    vopt = np.empty_like(vlist_start)
+    fp, ud = _target_args(target)
+    lib().optimize_brickwall_circuit_hmat(fp, ud, _p(vlist_start), nl, perms.shape[1], rows, ct.byref(params), niter,
+                                          _p(f_iter), _p(vopt))
+    return f_iter, vopt
+
+
+def truncated_cg_hmat(grad, hess, radius, params=None):
+    grad = np.ascontiguousarray(grad, np.float64)
+    hess = np.ascontiguousarray(hess, np.float64)
+    n = grad.size
+    params = TruncatedCgParams(2 * n, 1e-8, 1e-6) if params is None else params
+    z = np.empty(n)
+    L = lib()
+    L.truncated_cg_hmat.restype = ct.c_bool
+    on = L.truncated_cg_hmat(_p(grad), _p(hess), n, ct.c_double(radius), ct.byref(params), _p(z))
+    return z, bool(on)
 
 
 def truncated_cg_hvp(grad, hvp_fn, radius, params=None):

@@ -1218,6 +1218,54 @@ extern "C" void apply_gate_to_array(const struct mat4x4* gate, const int i, cons
 	} catch (const std::exception& e) { fatal("apply_gate_to_array", e); }
 }
 
+extern "C" void apply_gate_backward_array(const struct mat4x4* gate, const int i, const int j,
+	const struct statevector_array* RQC_RESTRICT psi, const struct statevector* RQC_RESTRICT dpsi_out, struct mat4x4* dgates)
+{
+	(void)gate;   // the gradients depend on the gate's wires only
+	try {
+		if (psi->nqubits != dpsi_out->nqubits) { throw Error(-3, "qubit counts differ"); }
+		const int L = psi->nqubits;
+		check_wires(i, j, L);
+		if (psi->nstates < 1) { return; }
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t n = (size_t)1 << L;
+		c.small.reserve(16 * (size_t)psi->nstates * sizeof(zc));
+		Staging s(c);
+		const int a = s.add(psi->data, n * psi->nstates * sizeof(zc), true, false);
+		const int d = s.add(dpsi_out->data, n * sizeof(zc), true, false);
+		s.upload();
+		{
+			LaunchTimer lt(c, RQC_KERNEL_REVERSE);
+			k_gate_backward_array<<<16 * psi->nstates, kBlock, 0, c.stream>>>(s.dev<zc>(a), s.dev<zc>(d), c.small.as<zc>(),
+				wires_of(i, j, L), (long)(n / 4), psi->nstates, 16, 0);
+		}
+		copy_out(c, dgates, c.small.ptr, 16 * (size_t)psi->nstates * sizeof(zc));
+		s.download();
+	} catch (const std::exception& e) { fatal("apply_gate_backward_array", e); }
+}
+
+extern "C" void apply_gate_placeholder(const int i, const int j, const struct statevector* psi, struct statevector_array* psi_out)
+{
+	try {
+		if (psi->nqubits != psi_out->nqubits || psi_out->nstates != 16) { throw Error(-3, "psi_out must hold 16 states of psi's size"); }
+		const int L = psi->nqubits;
+		check_wires(i, j, L);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t n = (size_t)1 << L;
+		Staging s(c);
+		const int a = s.add(psi->data, n * sizeof(zc), true, false);
+		const int o = s.add(psi_out->data, n * 16 * sizeof(zc), false, true);
+		s.upload();
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_gate_placeholder<<<nblocks((long)n * 16), kBlock, 0, c.stream>>>(s.dev<zc>(a), s.dev<zc>(o), wires_of(i, j, L), (long)n * 16);
+		}
+		s.download();
+	} catch (const std::exception& e) { fatal("apply_gate_placeholder", e); }
+}
+
 // ============================================================== C ABI: quantum_circuit.h
 int rqc::report_error(const char* where, const std::exception& e)
 {
@@ -1452,6 +1500,108 @@ extern "C" int brickwall_unitary_backward(const struct mat4x4 vlist[], int nlaye
 	} catch (const std::exception& e) { return report_error("brickwall_unitary_backward", e); }
 }
 
+// Gradient sweep storing the adjoint after every gate; then, per gate position,
+// the 16 placeholder states E_rc psi_k are carried through the later gates on
+// the GPU (16-column blocks) and contracted with the stored adjoints there
+// (reference src/brickwall_circuit.c:134-255); the contractions accumulate
+// straight into the Hessian's row block.  Diagonal blocks are symmetrised and
+// the lower blocks copied on the host.
+extern "C" int brickwall_unitary_backward_hessian(const struct mat4x4 vlist[], int nlayers, const int* perms[],
+	const struct quantum_circuit_cache* cache, const struct statevector* RQC_RESTRICT dpsi_out,
+	struct statevector* RQC_RESTRICT dpsi, struct mat4x4 dvlist[], numeric* hess)
+{
+	try {
+		const int L = dpsi->nqubits;
+		if (dpsi_out->nqubits != L || cache->nqubits != L) { throw Error(-3, "qubit counts differ"); }
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, L);
+		const int per = L / 2, ng = C.ngates, m = 16 * nlayers;
+		if (cache->ngates != ng) { throw Error(-3, "cache holds " + std::to_string(cache->ngates) + " states, circuit has " + std::to_string(ng) + " gates"); }
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t n = (size_t)1 << L;
+		const size_t bytes = n * sizeof(zc);
+		// small: gradient accumulators (32 per gate), reverse partials, Hessian m x m
+		c.small.reserve((32 * (size_t)ng + 4096 * (size_t)kWidth + (size_t)m * m) * sizeof(zc));
+		zc* acc = c.small.as<zc>();
+		zc* part = acc + 32 * (size_t)ng;
+		zc* dh = part + 4096 * (size_t)kWidth;
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_fill_zero<<<nblocks(32L * ng), kBlock, 0, c.stream>>>(acc, 32L * ng, nullptr);
+			k_fill_zero<<<nblocks((long)m * m), kBlock, 0, c.stream>>>(dh, (long)m * m, nullptr);
+		}
+		Staging s(c);
+		const int st = s.scratch(bytes * ng);          // forward states (the cache)
+		const int ad = s.scratch(bytes * ng);          // adjoint after each gate
+		const int t0 = s.scratch(bytes * 16), t1 = s.scratch(bytes * 16);
+		const int d = s.add(dpsi_out->data, bytes, true, false);
+		const int o = s.add(dpsi->data, bytes, false, true);
+		s.upload();
+		zc* S = s.dev<zc>(st);
+		zc* A = s.dev<zc>(ad);
+		for (int k = 0; k < ng; k++) { copy_in(c, S + k * n, cache->psi_list[k].data, bytes); }
+		dcopy(c, A + (size_t)(ng - 1) * n, s.dev<zc>(d), bytes);
+		for (int k = ng - 1; k >= 0; k--) {
+			zc* out = (k > 0) ? A + (size_t)(k - 1) * n : s.dev<zc>(o);
+			const zc* G = &C.gates[16 * (size_t)k];
+			launch_reverse(c, G, G, C.wi[k], C.wj[k], L, S + k * n, nullptr, A + (size_t)k * n, nullptr, out, nullptr, false,
+				acc + 32 * (size_t)k, part);
+		}
+		zc* T[2] = {s.dev<zc>(t0), s.dev<zc>(t1)};
+		for (int g = 0; g < ng; g++) {
+			const int i = g / per;
+			{
+				LaunchTimer lt(c, RQC_KERNEL_OTHER);
+				k_gate_placeholder<<<nblocks((long)n * 16), kBlock, 0, c.stream>>>(S + (size_t)g * n, T[0], wires_of(C.wi[g], C.wj[g], L),
+					(long)n * 16);
+			}
+			int p = 0;
+			for (int l = g + 1; l < ng; l++) {
+				const int j = l / per;
+				{
+					LaunchTimer lt(c, RQC_KERNEL_REVERSE);
+					k_gate_backward_array<<<256, kBlock, 0, c.stream>>>(T[p], A + (size_t)l * n,
+						dh + ((size_t)i * 16 * nlayers + j) * 16, wires_of(C.wi[l], C.wj[l], L), (long)(n / 4), 16, m, 1);
+				}
+				if (l < ng - 1) {
+					launch_forward(c, &C.gates[16 * (size_t)l], C.wi[l], C.wj[l], L, 16, T[p], T[1 - p]);
+					p = 1 - p;
+				}
+			}
+		}
+		std::vector<zc> hacc(32 * (size_t)ng), hh((size_t)m * m);
+		RQC_HIP(hipMemcpyAsync(hacc.data(), acc, hacc.size() * sizeof(zc), hipMemcpyDeviceToHost, c.stream));
+		RQC_HIP(hipMemcpyAsync(hh.data(), dh, hh.size() * sizeof(zc), hipMemcpyDeviceToHost, c.stream));
+		s.download();
+		// layer gradients summed in reverse pair order, as brickwall_unitary_backward does
+		for (int i = 0; i < nlayers; i++) {
+			zc sum[16];
+			for (int e = 0; e < 16; e++) { sum[e] = {0, 0}; }
+			for (int k = per - 1; k >= 0; k--) {
+				for (int e = 0; e < 16; e++) { sum[e] += hacc[32 * ((size_t)i * per + k) + e]; }
+			}
+			std::memcpy(dvlist[i].data, sum, sizeof(sum));
+		}
+		for (int i = 0; i < nlayers; i++) {
+			for (int x = 0; x < 16; x++) {
+				for (int y = 0; y <= x; y++) {
+					const size_t a = (size_t)(i * 16 + x) * m + i * 16 + y, b = (size_t)(i * 16 + y) * m + i * 16 + x;
+					const zc sum = hh[a] + hh[b];
+					hh[a] = sum;
+					hh[b] = sum;
+				}
+			}
+			for (int j = 0; j < i; j++) {
+				for (int x = 0; x < 16; x++) {
+					for (int y = 0; y < 16; y++) { hh[(size_t)(i * 16 + x) * m + j * 16 + y] = hh[(size_t)(j * 16 + y) * m + i * 16 + x]; }
+				}
+			}
+		}
+		std::memcpy(hess, hh.data(), hh.size() * sizeof(zc));
+		return 0;
+	} catch (const std::exception& e) { return report_error("brickwall_unitary_backward_hessian", e); }
+}
+
 // ============================================================== C ABI: target.h
 static int target_call(const char* where, const std::function<void()>& body)
 {
@@ -1580,6 +1730,75 @@ extern "C" int brickwall_unitary_target_projected_hessian_vector_product(linear_
 		put_f(fval, R);
 		std::memcpy(grad_vec, R.grad.data(), R.grad.size() * sizeof(double));
 		std::memcpy(hvp_vec, R.hvp.data(), R.hvp.size() * sizeof(double));
+	});
+}
+
+// Complex Hessian of the brickwall target (reference src/target.c:687-797).  Column
+// (j, y) is the HVP with the unit direction E_y on every gate of layer j, so the
+// 16 nlayers columns are HVP evaluations of one plan: their tables go up in one
+// copy, the evaluations are enqueued back to back into one workspace with a
+// result region each, and the results come back in one copy.  The matrix is
+// symmetric up to rounding and is symmetrised; f and dvlist come with column 0.
+static void eval_hessian(const Circuit& C0, linear_func ufunc, void* udata, int per, zc* fval, zc* dvlist, zc* hess)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	const zc* dU = resolve_target(c, ufunc, udata, C0.L);
+	Circuit C = C0;
+	C.dirs.assign(C.gates.size(), zc{0, 0});
+	const int ng = C.ngates / per, m = 16 * ng;
+	const bool unitary = gates_near_unitary(C);
+	const EvalPlan pl = plan_eval(c, C, Mode::Hvp, per, c.precision == RQC_PRECISION_MIXED, false, unitary, c.ws_limit,
+		c.work.bytes);
+	const size_t ts = align256(pl.tab_bytes), rs = align256(pl.res_bytes()), base = align256(pl.bytes);
+	std::vector<char> htab((size_t)m * ts);
+	std::vector<Circuit> cols((size_t)m, C);
+	for (int col = 0; col < m; col++) {
+		Circuit& Cc = cols[col];
+		const int jl = col / 16, y = col % 16;
+		for (int q = 0; q < per; q++) { Cc.dirs[16 * ((size_t)jl * per + q) + y] = zc{1, 0}; }
+		fill_eval_tables(pl, Cc, htab.data() + (size_t)col * ts);
+	}
+	c.work.reserve(base + (size_t)m * (ts + rs));
+	char* work = c.work.as<char>();
+	char* tabs = work + base;
+	char* ress = tabs + (size_t)m * ts;
+	RQC_HIP(hipMemcpyAsync(tabs, htab.data(), htab.size(), hipMemcpyHostToDevice, c.stream));
+	for (int col = 0; col < m; col++) {
+		enqueue_eval(c, c, pl, cols[col], dU, work, tabs + (size_t)col * ts, ress + (size_t)col * rs, nullptr, nullptr);
+	}
+	std::vector<char> hres((size_t)m * rs);
+	RQC_HIP(hipMemcpyAsync(hres.data(), ress, hres.size(), hipMemcpyDeviceToHost, c.stream));
+	RQC_HIP(hipStreamSynchronize(c.stream));
+	RQC_HIP(hipGetLastError());
+	std::vector<zc> H((size_t)m * m);
+	for (int col = 0; col < m; col++) {
+		EvalResult R;
+		unpack_result(pl, hres.data() + (size_t)col * rs, R);
+		if (col == 0) {
+			*fval = R.f;
+			std::memcpy(dvlist, R.dv.data(), R.dv.size() * sizeof(zc));
+		}
+		for (int r = 0; r < m; r++) { H[(size_t)r * m + col] = R.hz[r]; }
+	}
+	for (int a = 0; a < m; a++) {
+		for (int b = 0; b <= a; b++) {
+			const zc v = 0.5 * (H[(size_t)a * m + b] + H[(size_t)b * m + a]);
+			hess[(size_t)a * m + b] = v;
+			hess[(size_t)b * m + a] = v;
+		}
+	}
+}
+
+extern "C" int brickwall_unitary_target_gradient_hessian(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	numeric* fval, struct mat4x4 dvlist[], numeric* hess)
+{
+	return target_call("brickwall_unitary_target_gradient_hessian", [&] {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
+		zc f;
+		eval_hessian(C, ufunc, udata, nqubits / 2, &f, reinterpret_cast<zc*>(dvlist), reinterpret_cast<zc*>(hess));
+		std::memcpy(fval, &f, sizeof(zc));
 	});
 }
 

@@ -15,6 +15,7 @@
 #include "rqcopt/rng.h"
 #include "rqcopt/trust_region.h"
 #include "rqcopt/brickwall_opt.h"
+#include "rqcopt/target.h"
 
 
 
@@ -208,15 +209,18 @@ static double move_to_boundary(const double* b, const double* d, int n, double r
 	return t[1];
 }
 
-extern "C" bool truncated_cg_hvp(const double* RQC_RESTRICT x, const double* RQC_RESTRICT grad, const hessian_vector_product_func f_hvp, void* fdata,
-	const int n, const double radius, const struct truncated_cg_params* params, double* RQC_RESTRICT z)
+// Steihaug-Toint truncated CG (reference src/trust_region.c:83-141 and :159-217);
+// hv(d, out) applies the Hessian, as a matrix or through HVP evaluations
+template <typename HessApply>
+static bool truncated_cg(const double* grad, int n, double radius, const struct truncated_cg_params* params, double* z,
+	HessApply hv)
 {
 	std::vector<double> r(grad, grad + n), d(n), hd(n);
 	double rsq = ddot(n, r.data(), r.data());
 	const double stoptol = std::fmax(params->abstol, params->reltol * std::sqrt(rsq));
 	for (int i = 0; i < n; i++) { z[i] = 0; d[i] = -r[i]; }
 	for (int j = 0; j < params->maxiter; j++) {
-		f_hvp(x, fdata, d.data(), hd.data());
+		hv(d.data(), hd.data());
 		const double dhd = ddot(n, d.data(), hd.data());
 		const double t = move_to_boundary(z, d.data(), n, radius);
 		const double alpha = rsq / dhd;
@@ -235,9 +239,30 @@ extern "C" bool truncated_cg_hvp(const double* RQC_RESTRICT x, const double* RQC
 	return false;
 }
 
-extern "C" void riemannian_trust_region_optimize_hvp(const target_func f, const target_gradient_func f_deriv, const hessian_vector_product_func f_hvp, void* fdata,
-	retract_func retract_fn, void* rdata,
-	const int n, const double* x_init, const int m, const struct rtr_params* params, const int niter, double* f_iter, double* x_final)
+// y = A x for a row-major n x n matrix
+static void matvec(int n, const double* A, const double* x, double* y)
+{
+	for (int a = 0; a < n; a++) { y[a] = ddot(n, &A[(size_t)a * n], x); }
+}
+
+extern "C" bool truncated_cg_hmat(const double* grad, const double* hess, int n, double radius, const struct truncated_cg_params* params, double* z)
+{
+	return truncated_cg(grad, n, radius, params, z, [&](const double* d, double* hd) { matvec(n, hess, d, hd); });
+}
+
+extern "C" bool truncated_cg_hvp(const double* RQC_RESTRICT x, const double* RQC_RESTRICT grad, const hessian_vector_product_func f_hvp, void* fdata,
+	const int n, const double radius, const struct truncated_cg_params* params, double* RQC_RESTRICT z)
+{
+	return truncated_cg(grad, n, radius, params, z,
+		[&](const double* d, double* hd) { f_hvp(x, fdata, const_cast<double*>(d), hd); });
+}
+
+// Riemannian trust region (Absil et al. 2008, Alg. 10; reference src/trust_region.c:230-293
+// and :306-368).  derivs(x, grad) evaluates f and the gradient at x and readies the
+// Hessian at x for hv(d, out).
+template <typename Derivs, typename HessApply>
+static void trust_region(const target_func f, void* fdata, retract_func retract_fn, void* rdata, int n, const double* x_init,
+	int m, const struct rtr_params* params, int niter, double* f_iter, double* x_final, Derivs derivs, HessApply hv)
 {
 	if (!(0 <= params->rho_trust && params->rho_trust < 0.25)) {
 		fprintf(stderr, "rho_trust must lie in [0, 0.25)\n");
@@ -249,12 +274,13 @@ extern "C" void riemannian_trust_region_optimize_hvp(const target_func f, const 
 	double radius = params->radius_init;
 	if (params->g_func != nullptr) { params->g_iter[0] = params->g_func(x, params->g_data); }
 	for (int k = 0; k < niter; k++) {
-		const double fx = f_deriv(x, fdata, grad.data());
-		const bool on_boundary = truncated_cg_hvp(x, grad.data(), f_hvp, fdata, n, radius, &params->tcg_params, eta.data());
+		const double fx = derivs(x, grad.data());
+		const bool on_boundary = truncated_cg(grad.data(), n, radius, &params->tcg_params, eta.data(),
+			[&](const double* d, double* hd) { hv(x, d, hd); });
 		retract_fn(x, eta.data(), rdata, xn);
 		f_iter[k] = fx;
 		// model decrease, Eq. (7.7) of Absil et al.
-		f_hvp(x, fdata, eta.data(), heta.data());
+		hv(x, eta.data(), heta.data());
 		const double rho = (f(xn, fdata) - fx) / (ddot(n, grad.data(), eta.data()) + 0.5 * ddot(n, eta.data(), heta.data()));
 		if (rho < 0.25) {
 			radius *= 0.25;
@@ -266,6 +292,85 @@ extern "C" void riemannian_trust_region_optimize_hvp(const target_func f, const 
 	}
 	std::memcpy(x_final, x, m * sizeof(double));
 	f_iter[niter] = f(x_final, fdata);
+}
+
+extern "C" void riemannian_trust_region_optimize_hmat(target_func f, target_gradient_hessian_func f_deriv_hess, void* fdata,
+	retract_func retract_fn, void* rdata,
+	const int n, const double* x_init, const int m, const struct rtr_params* params, const int niter, double* f_iter, double* x_final)
+{
+	std::vector<double> hess((size_t)n * n);
+	trust_region(f, fdata, retract_fn, rdata, n, x_init, m, params, niter, f_iter, x_final,
+		[&](const double* x, double* grad) { return f_deriv_hess(x, fdata, grad, hess.data()); },
+		[&](const double*, const double* d, double* hd) { matvec(n, hess.data(), d, hd); });
+}
+
+extern "C" void riemannian_trust_region_optimize_hvp(const target_func f, const target_gradient_func f_deriv, const hessian_vector_product_func f_hvp, void* fdata,
+	retract_func retract_fn, void* rdata,
+	const int n, const double* x_init, const int m, const struct rtr_params* params, const int niter, double* f_iter, double* x_final)
+{
+	trust_region(f, fdata, retract_fn, rdata, n, x_init, m, params, niter, f_iter, x_final,
+		[&](const double* x, double* grad) { return f_deriv(x, fdata, grad); },
+		[&](const double* x, const double* d, double* hd) { f_hvp(x, fdata, const_cast<double*>(d), hd); });
+}
+
+// ============================================================== target.h: Riemannian Hessian matrix
+// Projection of the complex Hessian onto the tangent spaces (reference
+// src/target.c:937-994): for the upper blocks (i <= j), column k of block (i, j)
+// is the tangent basis vector k at V_j pushed through hess, conjugated (the
+// derivative convention), projected at V_i on the diagonal blocks together with
+// the curvature term 1/2 (Z dv^dagger V + V dv^dagger Z) of the gradient, and
+// written in the tangent coordinates at V_i; the lower half by symmetry.
+static void project_hessian_matrix(int nlayers, const struct mat4x4* vlist, const std::vector<zc>& dvc, const zc* hess, double* H)
+{
+
+	const int m = 16 * nlayers;
+	for (int i = 0; i < nlayers; i++) {
+		for (int j = i; j < nlayers; j++) {
+			for (int k = 0; k < 16; k++) {
+				double r[16] = {0};
+				r[k] = 1;
+				zc Zm[16], G[16];
+				rqc::m4_real_to_tangent(r, Z(&vlist[j]), Zm);
+				for (int x = 0; x < 16; x++) {
+					zc acc = {0, 0};
+					for (int y = 0; y < 16; y++) { cmac(acc, hess[(size_t)(i * 16 + x) * m + j * 16 + y], Zm[y]); }
+					G[x] = zc{acc.x, -acc.y};
+				}
+				if (i == j) {
+					zc P[16], gh[16], T[16];
+					rqc::m4_project_tangent(Z(&vlist[i]), G, P);
+					rqc::m4_adjoint(&dvc[16 * (size_t)i], gh);
+					rqc::m4_symmetric_triple(Zm, gh, Z(&vlist[i]), T);
+					for (int e = 0; e < 16; e++) { G[e] = zc{P[e].x - T[e].x, P[e].y - T[e].y}; }
+				}
+				rqc::m4_tangent_to_real(Z(&vlist[i]), G, r);
+				for (int x = 0; x < 16; x++) { H[(size_t)(i * 16 + x) * m + j * 16 + k] = r[x]; }
+			}
+		}
+	}
+	for (int a = 0; a < m; a++) {
+		for (int b = 0; b < a; b++) { H[(size_t)a * m + b] = H[(size_t)b * m + a]; }
+	}
+}
+
+extern "C" int brickwall_unitary_target_gradient_vector_hessian_matrix(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	numeric* fval, double* grad_vec, double* H)
+{
+
+	const int m = 16 * nlayers;
+	std::vector<struct mat4x4> dv((size_t)std::max(nlayers, 1));
+	std::vector<numeric> hess((size_t)m * m);
+	const int ret = brickwall_unitary_target_gradient_hessian(ufunc, udata, vlist, nlayers, nqubits, perms, fval, dv.data(), hess.data());
+	if (ret < 0) { return ret; }
+	// gradient: conjugated (derivative convention), then tangent coordinates
+	std::vector<zc> dvc(16 * (size_t)nlayers);
+	for (int i = 0; i < nlayers; i++) {
+		for (int e = 0; e < 16; e++) { dvc[16 * (size_t)i + e] = zc{dv[i].data[e].real(), -dv[i].data[e].imag()}; }
+		rqc::m4_tangent_to_real(Z(&vlist[i]), &dvc[16 * (size_t)i], &grad_vec[16 * i]);
+	}
+	project_hessian_matrix(nlayers, vlist, dvc, reinterpret_cast<const zc*>(hess.data()), H);
+	return 0;
 }
 
 // ============================================================== brickwall_opt.h
@@ -328,6 +433,19 @@ void bw_retract(const double* RQC_RESTRICT x, const double* RQC_RESTRICT eta, vo
 
 }  // namespace
 
+// wrapper of reference src/brickwall_opt.c f_deriv_hess: f, projected gradient and Hessian matrix
+static double bw_f_deriv_hess(const double* RQC_RESTRICT x, void* fdata, double* RQC_RESTRICT grad, double* RQC_RESTRICT hess)
+{
+	auto* d = static_cast<BrickwallData*>(fdata);
+	numeric f;
+	if (brickwall_unitary_target_gradient_vector_hessian_matrix(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x),
+			d->nlayers, d->nqubits, d->perms, &f, grad, hess) < 0) {
+		fprintf(stderr, "target function and derivative evaluation failed internally\n");
+		return -1;
+	}
+	return f.real();
+}
+
 extern "C" void optimize_brickwall_circuit_hvp(linear_func ufunc, void* udata,
 	const struct mat4x4 vlist_start[], const int nlayers, const int nqubits, const int* perms[],
 	struct rtr_params* params, const int niter, double* f_iter, struct mat4x4 vlist_opt[])
@@ -338,6 +456,20 @@ extern "C" void optimize_brickwall_circuit_hvp(linear_func ufunc, void* udata,
 	params->g_iter = nullptr;
 	int rdata = nlayers;
 	riemannian_trust_region_optimize_hvp(bw_f, bw_f_deriv, bw_f_hvp, &fdata, bw_retract, &rdata,
+		nlayers * num_tangent_params, reinterpret_cast<const double*>(vlist_start), nlayers * 32, params, niter, f_iter,
+		reinterpret_cast<double*>(vlist_opt));
+}
+
+extern "C" void optimize_brickwall_circuit_hmat(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist_start[], const int nlayers, const int nqubits, const int* perms[],
+	struct rtr_params* params, const int niter, double* f_iter, struct mat4x4 vlist_opt[])
+{
+	BrickwallData fdata{ufunc, udata, perms, nlayers, nqubits};
+	params->g_func = nullptr;
+	params->g_data = nullptr;
+	params->g_iter = nullptr;
+	int rdata = nlayers;
+	riemannian_trust_region_optimize_hmat(bw_f, bw_f_deriv_hess, &fdata, bw_retract, &rdata,
 		nlayers * num_tangent_params, reinterpret_cast<const double*>(vlist_start), nlayers * 32, params, niter, f_iter,
 		reinterpret_cast<double*>(vlist_opt));
 }

@@ -297,6 +297,49 @@ __global__ __launch_bounds__(kBlock) void k_fill_zero(zc* __restrict__ p, long n
 	if (t < n) { p[t] = {0, 0}; }
 }
 
+// Gate placeholder (reference src/gate.c:374-460): the 16 states E_rc psi, r, c
+// in the gate basis of (wire i, wire j), interleaved as out[a * 16 + 4 r + c]:
+// amplitude a is psi at a's index with its (i, j) bits set to c when those bits
+// equal r, else zero.  One thread per output element.
+__global__ __launch_bounds__(kBlock) void k_gate_placeholder(const zc* __restrict__ psi, zc* __restrict__ out, Wires w,
+	long n16)
+{
+	const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+	if (t >= n16) { return; }
+	const long a = t >> 4;
+	const int s = (int)(t & 15), r = s >> 2, c = s & 3;
+	const int ra = (int)((((a >> w.bi) & 1) << 1) | ((a >> w.bj) & 1));
+	if (ra != r) { out[t] = {0, 0}; return; }
+	const long src = (a & ~((1L << w.bi) | (1L << w.bj))) | ((long)(c >> 1) << w.bi) | ((long)(c & 1) << w.bj);
+	out[t] = psi[src];
+}
+
+// Gate gradients of a statevector array (reference src/gate.c:269-366):
+// entry 4 r + c of state k = sum over quads of dpsi_out[amp r] * psi[amp c, state k],
+// stored (accumulate = 0) or added (accumulate = 1) at out[k * stride + 4 r + c];
+// one workgroup per output entry, summed in a fixed order.
+__global__ __launch_bounds__(kBlock) void k_gate_backward_array(const zc* __restrict__ psi, const zc* __restrict__ dpsi,
+	zc* __restrict__ out, Wires w, long nq, int nstates, long stride, int accumulate)
+{
+	const int o = blockIdx.x;
+	const int k = o >> 4, r = (o >> 2) & 3, c = o & 3;
+	const int lo = min(w.bi, w.bj), hi = max(w.bi, w.bj);
+	zc acc = {0, 0};
+	for (long q = threadIdx.x; q < nq; q += kBlock) {
+		const long base = quad_base(q, lo, hi);
+		const long ar = base | ((long)(r >> 1) << w.bi) | ((long)(r & 1) << w.bj);
+		const long ac = base | ((long)(c >> 1) << w.bi) | ((long)(c & 1) << w.bj);
+		cmac(acc, dpsi[ar], psi[ac * nstates + k]);
+	}
+	__shared__ zc res[1];
+	block_sum_store<1>(&acc, res);
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		zc* d = out + (long)k * stride + (o & 15);
+		*d = accumulate ? *d + res[0] : res[0];
+	}
+}
+
 __global__ __launch_bounds__(kBlock) void k_add(const zc* __restrict__ a, const zc* __restrict__ b, zc* __restrict__ c, long n)
 {
 	const long t = (long)blockIdx.x * kBlock + threadIdx.x;

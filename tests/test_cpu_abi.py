@@ -17,7 +17,7 @@ import pytest
 
 import oracle as O
 import rqcopt_hpc_amd as R
-from helpers import crandn, haar_unitary
+from helpers import crandn, fixture, haar_unitary
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 INC = os.path.join(ROOT, "include", "rqcopt")
@@ -182,3 +182,18 @@ def test_gpu_entry_points_fail_loudly_without_gpu():
         R.require_gpu()
     with pytest.raises(R.RqcError):
         R.set_precision(R.PRECISION_MIXED)
+
+
+def test_truncated_cg_hmat_matches_fixture_and_oracle():
+    d = fixture("test_truncated_cg")
+    z, on = R.truncated_cg_hmat(d["grad"], d["hess"], 1.5)
+    assert np.max(np.abs(z - d["z"])) < 1e-14 and on == bool(d["on_boundary"])
+    rng = np.random.default_rng(40)
+    for radius in (0.05, 0.5, 50.0):
+        A = rng.standard_normal((20, 20))
+        H = A + A.T
+        g = rng.standard_normal(20)
+        z1, on1 = R.truncated_cg_hmat(g, H, radius)
+        z2, on2 = O.truncated_cg_hmat(g, H, radius)
+        z3, on3 = R.truncated_cg_hvp(g, lambda v: H @ v, radius)
+        assert on1 == on2 == on3 and np.max(np.abs(z1 - z2)) < 1e-12 and np.max(np.abs(z1 - z3)) < 1e-12

@@ -1,0 +1,98 @@
+"""GPU tests of the full-Hessian route: gate placeholder and array gradients
+(gate.h), the backward Hessian pass (brickwall_circuit.h), the complex Hessian
+and the Riemannian Hessian matrix (target.h), and the RTR optimiser with the
+explicit Hessian (brickwall_opt.h).
+
+The GPU builds the target Hessian from HVP columns; the oracle restates the
+reference's placeholder propagation, which the reference's stored hess / H
+fixtures pin.  Tolerance: 1e-12 relative (FP64), as for the HVP path.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import rqcopt_hpc_amd as R
+from helpers import brickwall_perms, cplx, crandn, fixture, haar_unitary, perms_of, random_perms, reldiff, scramble_matrix, stack
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    R.require_gpu()
+    R.set_precision(R.PRECISION_FP64)
+    yield
+
+
+SCR_H = R.HostUfunc(lambda x: scramble_matrix(int(np.log2(x.size))) @ x)
+
+
+def test_fixture_gradient_hessian_and_hessian_matrix():
+    d = fixture("test_brickwall_unitary_target_gradient_hessian_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    for k, nl in enumerate((4, 5)):
+        f, dv, h = R.brickwall_unitary_target_gradient_hessian(SCR_H, V[:nl], P[:nl])
+        assert abs(f - cplx(d[f"f{k}"])) < TOL
+        assert np.max(np.abs(dv - cplx(d[f"dVlist{k}"]))) < TOL
+        assert np.max(np.abs(h - cplx(d[f"hess{k}"]))) < TOL * np.max(np.abs(h))
+        assert np.array_equal(h, h.T)
+    d = fixture("test_brickwall_unitary_target_gradient_vector_hessian_matrix_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    f, g, H = R.brickwall_unitary_target_gradient_vector_hessian_matrix(SCR_H, V, P)
+    assert abs(f - cplx(d["f"])) < TOL and np.max(np.abs(g - d["grad"])) < TOL
+    assert np.max(np.abs(H - d["H"])) < TOL * np.max(np.abs(d["H"])) and np.array_equal(H, H.T)
+
+
+@pytest.mark.parametrize("L,nl", [(6, 3), (8, 2), (10, 2)])
+def test_gradient_hessian_vs_oracle(L, nl):
+    rng = np.random.default_rng(50 + L)
+    U = np.linalg.qr(crandn(rng, (1 << L, 1 << L)))[0]
+    T, Uo = R.DeviceTarget.dense(U), O.Ufunc.dense(U)
+    V = np.stack([haar_unitary(rng) for _ in range(nl)])
+    P = random_perms(rng, L, nl)
+    f, dv, h = R.brickwall_unitary_target_gradient_hessian(T, V, P)
+    f_o, dv_o, h_o = O.brickwall_unitary_target_gradient_hessian(Uo, V, P, L)
+    assert abs(f - f_o) < TOL * max(1, abs(f_o)) and reldiff(dv, dv_o) < TOL and reldiff(h, h_o) < TOL
+    f, g, H = R.brickwall_unitary_target_gradient_vector_hessian_matrix(T, V, P)
+    f_o, g_o, H_o = O.brickwall_unitary_target_gradient_vector_hessian_matrix(Uo, V, P, L)
+    assert reldiff(g, g_o) < TOL and reldiff(H, H_o) < TOL
+    # the Hessian matrix applied to a tangent vector is the projected HVP
+    vec = rng.standard_normal(16 * nl)
+    Z = np.stack([R.real_to_tangent(vec[16 * i:16 * i + 16], V[i]) for i in range(nl)])
+    _, _, hv = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
+    assert reldiff(H @ vec, hv) < 1e-11
+
+
+def test_placeholder_backward_array_and_backward_hessian_vs_oracle():
+    rng = np.random.default_rng(60)
+    L = 7
+    psi, dout = crandn(rng, 1 << L), crandn(rng, 1 << L)
+    G = crandn(rng, (4, 4))
+    for (i, j) in ((0, 1), (1, 0), (2, 6), (6, 3)):
+        assert np.array_equal(R.apply_gate_placeholder(i, j, psi), O.apply_gate_placeholder(i, j, psi))
+        arr = crandn(rng, (1 << L, 5))
+        assert reldiff(R.apply_gate_backward_array(G, i, j, arr, dout), O.apply_gate_backward_array(i, j, arr, dout)) < TOL
+    L, nl = 6, 3
+    V = np.stack([haar_unitary(rng) for _ in range(nl)])
+    P = random_perms(rng, L, nl)
+    psi, dout = crandn(rng, 1 << L), crandn(rng, 1 << L)
+    _, cache = O.brickwall_unitary_forward(V, P, psi)
+    dpsi, dv, h = R.brickwall_unitary_backward_hessian(V, P, cache, dout)
+    dpsi_o, dv_o, h_o = O.brickwall_unitary_backward_hessian(V, P, cache, dout)
+    assert reldiff(dpsi, dpsi_o) < TOL and reldiff(dv, dv_o) < TOL and reldiff(h, h_o) < TOL
+
+
+def test_hmat_optimiser_vs_oracle():
+    rng = np.random.default_rng(61)
+    L, nl = 6, 3
+    P = brickwall_perms(L, nl)
+    Vt = np.stack([haar_unitary(rng) for _ in range(nl)])
+    T = R.DeviceTarget.brickwall(Vt, P)
+    Uo = O.Ufunc.dense(T.download())
+    V0 = np.stack([haar_unitary(rng) for _ in range(nl)])
+    f_iter, vopt = R.optimize_brickwall_circuit_hmat(T, V0, P, 4)
+    f_o, v_o = O.optimize_brickwall_circuit_hmat(Uo, V0, P, L, 4)
+    assert np.max(np.abs(f_iter - f_o) / np.maximum(1, np.abs(f_o))) < 1e-9 and np.max(np.abs(vopt - v_o)) < 1e-8
+    assert f_iter[-1] < f_iter[0]

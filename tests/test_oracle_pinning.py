@@ -485,3 +485,84 @@ def test_ref_rng_sequences_bitwise():
             assert R.randn(ct.byref(rs)) == mine.randn()
             c = R.crandn(ct.byref(rs))
             assert complex(c.re, c.im) == mine.crandn()
+
+
+# ==================================================== full-Hessian route
+def test_fixture_gradient_hessian_and_hessian_matrix():
+    """The oracle's placeholder-propagation Hessian against the reference's stored hess and H."""
+    S = O.Ufunc.scramble()
+    d = fixture("test_brickwall_unitary_target_gradient_hessian_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    for k, nl in enumerate((4, 5)):
+        f, dv, h = O.brickwall_unitary_target_gradient_hessian(S, V[:nl], P[:nl], 6)
+        assert abs(f - cplx(d[f"f{k}"])) < 1e-12
+        assert np.max(np.abs(dv - cplx(d[f"dVlist{k}"]))) < 1e-12
+        assert np.max(np.abs(h - cplx(d[f"hess{k}"]))) < 1e-12
+        assert np.max(np.abs(h - h.T)) < 1e-13
+    d = fixture("test_brickwall_unitary_target_gradient_vector_hessian_matrix_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    f, g, H = O.brickwall_unitary_target_gradient_vector_hessian_matrix(S, V, P, 6)
+    assert abs(f - cplx(d["f"])) < 1e-12 and np.max(np.abs(g - d["grad"])) < 1e-13
+    assert np.max(np.abs(H - d["H"])) < 1e-13 and np.array_equal(H, H.T)
+
+
+def test_fixture_truncated_cg_hmat():
+    d = fixture("test_truncated_cg")
+    z, on = O.truncated_cg_hmat(d["grad"], d["hess"], 1.5)
+    assert np.max(np.abs(z - d["z"])) < 1e-14 and on == bool(d["on_boundary"])
+
+
+def test_hessian_columns_are_hvps_and_hmat_rtr_follows_hvp_rtr():
+    """Independent routes to the same numbers: each Hessian column is the HVP with a unit direction,
+    and RTR with the explicit Hessian takes the steps of RTR with HVPs."""
+    rng = np.random.default_rng(31)
+    L, nl = 6, 3
+    U = np.linalg.qr(crandn(rng, (1 << L, 1 << L)))[0]
+    Uo = O.Ufunc.dense(U)
+    V = np.stack([haar_unitary(rng) for _ in range(nl)])
+    P = random_perms(rng, L, nl)
+    f, dv, h = O.brickwall_unitary_target_gradient_hessian(Uo, V, P, L)
+    for col in (0, 7, 21, 47):
+        E = np.zeros((nl, 4, 4), np.complex128)
+        E[col // 16].flat[col % 16] = 1
+        _, _, hz = O.brickwall_unitary_target_hvp(Uo, V, E, P, L)
+        assert np.max(np.abs(hz.reshape(-1) - h[:, col])) < 1e-12
+    f_h, v_h = O.optimize_brickwall_circuit_hmat(Uo, V, P, L, 4)
+    f_v, v_v = O.optimize_brickwall_circuit_hvp(Uo, V, P, L, 4)
+    assert np.max(np.abs(f_h - f_v)) < 1e-9 and np.max(np.abs(v_h - v_v)) < 1e-8 and f_h[-1] < f_h[0]
+
+
+def test_placeholder_and_backward_array_closed_forms():
+    rng = np.random.default_rng(32)
+    L = 5
+    psi = crandn(rng, 1 << L)
+    dout = crandn(rng, 1 << L)
+    for (i, j) in ((0, 1), (3, 1), (4, 0)):
+        T = O.apply_gate_placeholder(i, j, psi)
+        # state 4r + c is E_rc applied on (i, j); contracting the 16 states with a gate G gives G psi
+        G = crandn(rng, (4, 4))
+        assert np.max(np.abs(T @ G.reshape(-1) - O.apply_gate(G, i, j, psi))) < 1e-13
+        # backward_array of the placeholder states: dgates[4r+c][x] = d/dG_x <dout, E_rc-propagated>
+        arr = crandn(rng, (1 << L, 3))
+        dg = O.apply_gate_backward_array(i, j, arr, dout)
+        for k in range(3):
+            _, dG = O.apply_gate_backward(G, i, j, arr[:, k].copy(), dout)
+            assert np.max(np.abs(dg[k] - dG)) < 1e-13
+
+
+@needs_ref
+def test_ref_placeholder_and_backward_array_bitwise():
+    R = _ref()
+    rng = np.random.default_rng(33)
+    L = 6
+    psi, dout = crandn(rng, 1 << L), crandn(rng, 1 << L)
+    G = crandn(rng, (4, 4))
+    for (i, j) in ((0, 1), (1, 0), (2, 5), (5, 2)):
+        out = np.empty((1 << L, 16), np.complex128)
+        R.apply_gate_placeholder(i, j, ct.byref(SV(psi.ctypes.data, L)), ct.byref(SVA(out.ctypes.data, L, 16)))
+        assert np.array_equal(out, O.apply_gate_placeholder(i, j, psi))
+        arr = crandn(rng, (1 << L, 16))
+        dg = np.empty((16, 4, 4), np.complex128)
+        R.apply_gate_backward_array(G.ctypes.data_as(ct.c_void_p), i, j, ct.byref(SVA(arr.ctypes.data, L, 16)),
+                                    ct.byref(SV(dout.ctypes.data, L)), dg.ctypes.data_as(ct.c_void_p))
+        assert np.max(np.abs(dg - O.apply_gate_backward_array(i, j, arr, dout))) <= 1e-14 * np.max(np.abs(dg))
