@@ -337,6 +337,16 @@ def optimize_brickwall_circuit_hvp(uf, vlist_start, perms, L, niter):
     return f_iter, vopt
 
 
+def optimize_quantum_circuit(uf, gates_start, wires, L, niter):
+    gates_start, wires = _cplx(gates_start), _i32(wires)
+    f_iter = np.empty(niter + 1)
+    gopt = np.empty_like(gates_start)
+    fp, ud = uf.args()
+    lib().orc_optimize_quantum_circuit(fp, ud, _p(gates_start), _i(len(gates_start)), _i(L), _p(wires), _i(niter),
+                                       _p(f_iter), _p(gopt))
+    return f_iter, gopt
+
+
 # ------------------------------------------------------------------- full-Hessian route
 def apply_gate_placeholder(i, j, psi):
     psi = _cplx(psi)

@@ -864,6 +864,64 @@ void orc_optimize_brickwall_circuit_hvp(orc_linear_func ufunc, void* udata, cons
 	free(heta); free(eta); free(grad); free(xn); free(x);
 }
 
+struct qc_data { orc_linear_func ufunc; void* udata; const int* wires; int ngates; int L; };
+
+/* wrappers, reference src/circuit_opt.c:22-79 */
+static double qc_f(const double* x, struct qc_data* dd)
+{
+	ocplx f;
+	if (orc_circuit_unitary_target(dd->ufunc, dd->udata, (const ocplx*)x, dd->ngates, dd->wires, dd->L, &f) < 0) { return -1; }
+	return creal(f);
+}
+static double qc_f_deriv(const double* x, struct qc_data* dd, double* grad)
+{
+	ocplx f;
+	if (orc_circuit_unitary_target_and_projected_gradient(dd->ufunc, dd->udata, (const ocplx*)x, dd->ngates, dd->wires, dd->L, &f, grad) < 0) { return -1; }
+	return creal(f);
+}
+static void qc_f_hvp(const double* x, void* fdata, const double* vec, double* hvp)
+{
+	struct qc_data* dd = fdata;
+	const ocplx* gates = (const ocplx*)x;
+	ocplx* dirs = xalloc((size_t)dd->ngates * 16 * sizeof(ocplx));
+	double* grad = xalloc((size_t)dd->ngates * 16 * sizeof(double));
+	for (int i = 0; i < dd->ngates; i++) { orc_real_to_tangent(&vec[16*i], &gates[16*i], &dirs[16*i]); }
+	ocplx f;
+	orc_circuit_unitary_target_projected_hessian_vector_product(dd->ufunc, dd->udata, gates, dirs, dd->ngates, dd->wires, dd->L, &f, grad, hvp);
+	free(grad); free(dirs);
+}
+
+/* reference src/circuit_opt.c:104-124 with src/trust_region.c:306-368, default parameters */
+void orc_optimize_quantum_circuit(orc_linear_func ufunc, void* udata, const ocplx* gates_start, int ngates, int L,
+	const int* wires, int niter, double* f_iter, ocplx* gates_opt)
+{
+	struct qc_data dd = { ufunc, udata, wires, ngates, L };
+	const int n = 16 * ngates, m = 32 * ngates;
+	const int maxiter = 2 * n;
+	const double abstol = 1e-8, reltol = 1e-6, rho_trust = 0.125, maxradius = 0.1;
+	double radius = 0.01;
+	double* x = xalloc(m * sizeof(double));
+	double* xn = xalloc(m * sizeof(double));
+	double* grad = xalloc(n * sizeof(double));
+	double* eta = xalloc(n * sizeof(double));
+	double* heta = xalloc(n * sizeof(double));
+	memcpy(x, gates_start, m * sizeof(double));
+	for (int k = 0; k < niter; k++) {
+		const double fx = qc_f_deriv(x, &dd, grad);
+		const bool on_boundary = orc_truncated_cg_hvp(x, grad, qc_f_hvp, &dd, n, radius, maxiter, abstol, reltol, eta);
+		for (int i = 0; i < ngates; i++) { orc_retract((const ocplx*)&x[32*i], &eta[16*i], (ocplx*)&xn[32*i]); }
+		f_iter[k] = fx;
+		qc_f_hvp(x, &dd, eta, heta);
+		const double rho = (qc_f(xn, &dd) - fx) / (ddot(n, grad, eta) + 0.5 * ddot(n, eta, heta));
+		if (rho < 0.25) { radius *= 0.25; }
+		else if (rho > 0.75 && on_boundary) { radius = fmin(2 * radius, maxradius); }
+		if (rho > rho_trust) { double* t = x; x = xn; xn = t; }
+	}
+	memcpy(gates_opt, x, m * sizeof(double));
+	f_iter[niter] = qc_f(x, &dd);
+	free(heta); free(eta); free(grad); free(xn); free(x);
+}
+
 /* ---------------------------------------------------------------------------
  * Full-Hessian route.
  */

@@ -117,6 +117,10 @@ bool orc_truncated_cg_hvp(const double* x, const double* grad, orc_hvp_func f_hv
 void orc_optimize_brickwall_circuit_hvp(orc_linear_func ufunc, void* udata, const ocplx* vlist_start, int nlayers,
 	int L, const int* perms, int niter, double* f_iter, ocplx* vlist_opt);
 
+/* RTR-HVP optimiser of a general circuit (reference src/circuit_opt.c:22-124), default parameters */
+void orc_optimize_quantum_circuit(orc_linear_func ufunc, void* udata, const ocplx* gates_start, int ngates, int L,
+	const int* wires, int niter, double* f_iter, ocplx* gates_opt);
+
 /* ---- full-Hessian route: reference src/gate.c:269-460, src/brickwall_circuit.c:134-255,
         src/target.c:687-797,937-1040, src/trust_region.c:83-141,230-293, src/brickwall_opt.c ---- */
 /* psi_out: 2^L x 16 interleaved (state 4r + c = E_rc psi on wires (i, j)) */

@@ -113,6 +113,8 @@ EXPORTED = [
     # trust_region.h, brickwall_opt.h
     "truncated_cg_hvp", "riemannian_trust_region_optimize_hvp", "optimize_brickwall_circuit_hvp",
     "truncated_cg_hmat", "riemannian_trust_region_optimize_hmat", "optimize_brickwall_circuit_hmat",
+    # circuit_opt.h
+    "optimize_quantum_circuit",
     # rqc_device.h
     "rqc_device_ufunc", "rqc_target_create_dense", "rqc_target_create_from_ufunc", "rqc_target_create_brickwall",
     "rqc_target_destroy", "rqc_target_nqubits", "rqc_target_download", "rqc_set_precision", "rqc_get_precision",
@@ -628,6 +630,18 @@ def optimize_brickwall_circuit_hmat(target, vlist_start, perms, niter, params=No
     lib().optimize_brickwall_circuit_hmat(fp, ud, _p(vlist_start), nl, perms.shape[1], rows, ct.byref(params), niter,
                                           _p(f_iter), _p(vopt))
     return f_iter, vopt
+
+
+def optimize_quantum_circuit(target, gates_start, wires, nqubits, niter, params=None):
+    gates_start, w = _gates(gates_start), _wires(wires)
+    ng = len(gates_start)
+    params = rtr_default_params(16 * ng) if params is None else params
+    f_iter = np.empty(niter + 1)
+    gopt = np.empty_like(gates_start)
+    fp, ud = _target_args(target)
+    lib().optimize_quantum_circuit(fp, ud, _p(gates_start), ng, nqubits, _p(w), ct.byref(params), niter, _p(f_iter),
+                                   _p(gopt))
+    return f_iter, gopt
 
 
 def truncated_cg_hmat(grad, hess, radius, params=None):

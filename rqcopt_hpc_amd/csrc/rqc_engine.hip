@@ -679,7 +679,10 @@ EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, boo
 	pl.gtr = reduce_grid(pl.N * pl.B);
 	pl.n_acc = 1 + 32 * (size_t)pl.G;
 	size_t npart = pl.gtr;
-	if (pl.tiles) { npart = std::max(npart, (pl.tp->partial_doubles + 1) / 2); }
+	if (pl.tiles) {
+		// outer-product partials, then the chunk sums of the first reduction stage
+		npart = std::max(npart, (pl.tp->partial_doubles + (size_t)kReduceChunks * kMaxTileGates * 256 + 1) / 2);
+	}
 	else { npart = std::max(npart, (size_t)pl.grev_pg * kWidth); }
 	pl.n_part = npart;
 	if (pl.tiles) {
@@ -768,6 +771,7 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
 	auto* kr = dual ? k_tile_reverse<T, true> : k_tile_reverse<T, false>;
 	double* mpart = reinterpret_cast<double*>(d_partial);   // grev * waves * ng * 256 doubles
+	double* cpart = mpart + tp.partial_doubles;             // chunk sums: kReduceChunks * ng * 256 doubles
 	zc* tpart = d_partial;                                  // trace partials (reused, sequential)
 	const unsigned gtr = pl.gtr;
 	const int logB = ilog2_or_neg(B);
@@ -809,8 +813,10 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 			}
 			{
 				LaunchTimer lt(s, RQC_KERNEL_OTHER);
-				k_tile_reduce<<<ng, 256, 0, s.stream>>>(mpart, (long)tp.grev * kTileWaves, ng, tp.gks[p], dual ? 1 : 0, d_acc,
-					skip);
+				const long nparts = (long)tp.grev * kTileWaves;
+				const int nch = (int)std::min<long>(kReduceChunks, nparts);
+				k_tile_reduce_chunks<<<dim3(ng, nch), 256, 0, s.stream>>>(mpart, nparts, ng, nch, cpart, skip);
+				k_tile_reduce<<<ng, 256, 0, s.stream>>>(cpart, nch, ng, tp.gks[p], dual ? 1 : 0, d_acc, skip);
 			}
 			cur = 1 - cur;
 		}

@@ -15,6 +15,7 @@
 #include "rqcopt/rng.h"
 #include "rqcopt/trust_region.h"
 #include "rqcopt/brickwall_opt.h"
+#include "rqcopt/circuit_opt.h"
 #include "rqcopt/target.h"
 
 
@@ -472,4 +473,68 @@ extern "C" void optimize_brickwall_circuit_hmat(linear_func ufunc, void* udata,
 	riemannian_trust_region_optimize_hmat(bw_f, bw_f_deriv_hess, &fdata, bw_retract, &rdata,
 		nlayers * num_tangent_params, reinterpret_cast<const double*>(vlist_start), nlayers * 32, params, niter, f_iter,
 		reinterpret_cast<double*>(vlist_opt));
+}
+
+// ============================================================== circuit_opt.h
+namespace {
+
+struct CircuitData
+{
+	linear_func ufunc;
+	void* udata;
+	const int* wires;
+	int ngates;
+	int nqubits;
+};
+
+// wrappers of reference src/circuit_opt.c:22-79
+double qc_f(const double* x, void* fdata)
+{
+	auto* d = static_cast<CircuitData*>(fdata);
+	numeric f;
+	if (circuit_unitary_target(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x), d->ngates, d->wires, d->nqubits, &f) < 0) {
+		fprintf(stderr, "target function evaluation failed internally\n");
+		return -1;
+	}
+	return f.real();
+}
+
+double qc_f_deriv(const double* RQC_RESTRICT x, void* fdata, double* RQC_RESTRICT grad)
+{
+	auto* d = static_cast<CircuitData*>(fdata);
+	numeric f;
+	if (circuit_unitary_target_and_projected_gradient(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x), d->ngates,
+			d->wires, d->nqubits, &f, grad) < 0) {
+		fprintf(stderr, "target function and derivative evaluation failed internally\n");
+		return -1;
+	}
+	return f.real();
+}
+
+void qc_f_hvp(const double* RQC_RESTRICT x, void* fdata, double* RQC_RESTRICT vec, double* RQC_RESTRICT hvp)
+{
+	auto* d = static_cast<CircuitData*>(fdata);
+	const auto* gates = reinterpret_cast<const struct mat4x4*>(x);
+	std::vector<struct mat4x4> dirs(d->ngates);
+	for (int i = 0; i < d->ngates; i++) { real_to_tangent(&vec[16 * i], &gates[i], &dirs[i]); }
+	std::vector<double> grad(16 * (size_t)d->ngates);
+	numeric f;
+	if (circuit_unitary_target_projected_hessian_vector_product(d->ufunc, d->udata, gates, dirs.data(), d->ngates, d->wires,
+			d->nqubits, &f, grad.data(), hvp) < 0) {
+		fprintf(stderr, "Hessian-vector product evaluation failed internally\n");
+	}
+}
+
+}  // namespace
+
+extern "C" void optimize_quantum_circuit(linear_func ufunc, void* udata, const struct mat4x4 gates_start[], const int ngates,
+	const int nqubits, const int wires[], struct rtr_params* params, const int niter, double* f_iter, struct mat4x4 gates_opt[])
+{
+	CircuitData fdata{ufunc, udata, wires, ngates, nqubits};
+	params->g_func = nullptr;
+	params->g_data = nullptr;
+	params->g_iter = nullptr;
+	int rdata = ngates;
+	riemannian_trust_region_optimize_hvp(qc_f, qc_f_deriv, qc_f_hvp, &fdata, bw_retract, &rdata, ngates * num_tangent_params,
+		reinterpret_cast<const double*>(gates_start), ngates * 32, params, niter, f_iter, reinterpret_cast<double*>(gates_opt));
 }

@@ -685,9 +685,26 @@ struct TileGates
 	int k[kMaxTileGates];
 };
 
-// Sum of the outer-product partials over all (workgroup, wave) parts in a fixed
-// order, decoded into 16 gradient and (DUAL) 16 HVP entries per gate and added
-// to acc[1 + 32 * gate + e].  One workgroup of 256 threads per gate of the pass.
+// First stage of the partial-sum reduction: chunk c of nchunks sums the parts
+// [c * ceil(nparts / nchunks), ...) in order, for every gate j (grid ng x nchunks);
+// out has the partials' layout with nchunks parts.
+constexpr int kReduceChunks = 16;
+
+__global__ __launch_bounds__(256) void k_tile_reduce_chunks(const double* __restrict__ partial, long nparts, int ng,
+	int nchunks, double* __restrict__ out, const int* skip)
+{
+	if (skipped(skip)) { return; }
+	const int j = blockIdx.x, c = blockIdx.y, i = threadIdx.x;
+	const long per = (nparts + nchunks - 1) / nchunks;
+	const long p0 = (long)c * per, p1 = min(nparts, p0 + per);
+	double s = 0;
+	for (long p = p0; p < p1; p++) { s += partial[(p * ng + j) * 256 + i]; }
+	out[((long)c * ng + j) * 256 + i] = s;
+}
+
+// Sum of the outer-product partials over all parts in a fixed order, decoded
+// into 16 gradient and (DUAL) 16 HVP entries per gate and added to
+// acc[1 + 32 * gate + e].  One workgroup of 256 threads per gate of the pass.
 //   dual:  grad = phi (x) psi,  hess = phi (x) dpsi + dphi (x) psi
 //   plain: grad = phi (x) psi summed over the two quads of each operand pair
 __global__ __launch_bounds__(256) void k_tile_reduce(const double* __restrict__ partial, long nparts, int ng,
