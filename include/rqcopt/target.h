@@ -22,6 +22,7 @@
 
 #include "matrix.h"
 #include "statevector.h"
+#include "rng.h"
 
 typedef int (*linear_func)(const struct statevector* RQC_RESTRICT psi, void* fdata, struct statevector* RQC_RESTRICT psi_out);
 
@@ -76,6 +77,33 @@ int brickwall_unitary_target_hessian_vector_product(linear_func ufunc, void* uda
 int brickwall_unitary_target_projected_hessian_vector_product(linear_func ufunc, void* udata,
 	const struct mat4x4 vlist[], const struct mat4x4 vdirs[], const int nlayers, const int* perms[], const int nqubits,
 	numeric* fval, double* RQC_RESTRICT grad_vec, double* RQC_RESTRICT hvp_vec);
+
+/* Stochastic-trace targets: the trace over the 2^L basis columns is estimated
+   from nsamples Haar-random states drawn from rng (in the reference's order:
+   sample by sample, crandn entries then normalisation) and scaled by
+   2^L / nsamples; rng advances as in the reference. */
+
+/* reference src/target.c:85-149 */
+int circuit_unitary_target_sampling(linear_func ufunc, void* udata,
+	const struct mat4x4 gates[], const int ngates, const int wires[], const int nqubits,
+	const long nsamples, struct rng_state* rng, numeric* fval);
+
+/* reference src/target.c:491-505 */
+int brickwall_unitary_target_sampling(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	const long nsamples, struct rng_state* rng, numeric* fval);
+
+/* reference src/target.c:807-928 */
+int brickwall_unitary_target_gradient_hessian_sampling(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	const long nsamples, struct rng_state* rng,
+	numeric* fval, struct mat4x4 dvlist[], numeric* hess);
+
+/* reference src/target.c:1050-1088 */
+int brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	const long nsamples, struct rng_state* rng,
+	numeric* fval, double* grad_vec, double* H);
 
 /* reference src/target.c:687-797: f, the gate gradients and the complex Hessian
    hess[((i*16 + x)*nlayers + j)*16 + y] ((16 nlayers)^2 numerics) */

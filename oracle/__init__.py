@@ -347,6 +347,82 @@ def optimize_quantum_circuit(uf, gates_start, wires, L, niter):
     return f_iter, gopt
 
 
+# ------------------------------------------------------------------- statevector utilities, sampled targets
+def normalize_statevector(psi):
+    out = _cplx(psi).copy()
+    lib().orc_normalize_statevector(_i(int(np.log2(out.size))), _p(out))
+    return out
+
+
+def transpose_statevector(psi, perm):
+    psi = _cplx(psi)
+    out = np.empty_like(psi)
+    lib().orc_transpose_statevector(_i(int(np.log2(psi.size))), _p(psi), _p(_i32(perm)), _p(out))
+    return out
+
+
+def haar_random_statevector(L, rng):
+    out = np.empty(1 << L, np.complex128)
+    lib().orc_haar_random_statevector(_i(L), ct.byref(rng.s), _p(out))
+    return out
+
+
+def brickwall_unitary_target_sampling(uf, vlist, perms, L, nsamples, rng):
+    vlist, perms = _cplx(vlist), _i32(perms)
+    f = _fval()
+    fp, ud = uf.args()
+    r = lib().orc_brickwall_unitary_target_sampling(fp, ud, _p(vlist), _i(len(vlist)), _i(L), _p(perms), _l(nsamples),
+                                                    ct.byref(rng.s), _p(f))
+    assert r == 0
+    return f[0]
+
+
+def circuit_unitary_target_sampling(uf, gates, wires, L, nsamples, rng):
+    gates, wires = _cplx(gates), _i32(wires)
+    f = _fval()
+    fp, ud = uf.args()
+    r = lib().orc_circuit_unitary_target_sampling(fp, ud, _p(gates), _i(len(gates)), _p(wires), _i(L), _l(nsamples),
+                                                  ct.byref(rng.s), _p(f))
+    assert r == 0
+    return f[0]
+
+
+def brickwall_unitary_target_gradient_hessian_sampling(uf, vlist, perms, L, nsamples, rng):
+    vlist, perms = _cplx(vlist), _i32(perms)
+    m = 16 * len(vlist)
+    f = _fval()
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    hess = np.empty((m, m), np.complex128)
+    fp, ud = uf.args()
+    r = lib().orc_brickwall_unitary_target_gradient_hessian_sampling(fp, ud, _p(vlist), _i(len(vlist)), _i(L), _p(perms),
+                                                                    _l(nsamples), ct.byref(rng.s), _p(f), _p(dv), _p(hess))
+    assert r == 0
+    return f[0], dv, hess
+
+
+def brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(uf, vlist, perms, L, nsamples, rng):
+    vlist, perms = _cplx(vlist), _i32(perms)
+    m = 16 * len(vlist)
+    f = _fval()
+    g = np.empty(m)
+    H = np.empty((m, m))
+    fp, ud = uf.args()
+    r = lib().orc_brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(
+        fp, ud, _p(vlist), _i(len(vlist)), _i(L), _p(perms), _l(nsamples), ct.byref(rng.s), _p(f), _p(g), _p(H))
+    assert r == 0
+    return f[0], g, H
+
+
+def optimize_brickwall_circuit_hmat_sampling(uf, vlist_start, perms, L, niter, nsamples, rng):
+    vlist_start, perms = _cplx(vlist_start), _i32(perms)
+    f_iter = np.empty(niter + 1)
+    vopt = np.empty_like(vlist_start)
+    fp, ud = uf.args()
+    lib().orc_optimize_brickwall_circuit_hmat_sampling(fp, ud, _p(vlist_start), _i(len(vlist_start)), _i(L), _p(perms),
+                                                       _l(nsamples), ct.byref(rng.s), _i(niter), _p(f_iter), _p(vopt))
+    return f_iter, vopt
+
+
 # ------------------------------------------------------------------- full-Hessian route
 def apply_gate_placeholder(i, j, psi):
     psi = _cplx(psi)

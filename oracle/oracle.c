@@ -1261,3 +1261,168 @@ ocplx orc_crandn(struct orc_rng* rng)
 	const double y = orc_randn(rng);
 	return (x + I * y) / sqrt(2.0);
 }
+
+/* ---------------------------------------------------------------------------
+ * Statevector utilities and sampled (stochastic-trace) targets.
+ */
+
+/* reference src/statevector.c:47-66 (Euclidean norm, then scaling by its inverse) */
+void orc_normalize_statevector(int L, ocplx* psi)
+{
+	const long n = 1L << L;
+	double s = 0;
+	for (long j = 0; j < n; j++) { s += creal(psi[j]) * creal(psi[j]) + cimag(psi[j]) * cimag(psi[j]); }
+	const double nrm = sqrt(s);
+	if (nrm > 0) {
+		for (long j = 0; j < n; j++) { psi[j] *= 1.0 / nrm; }
+	}
+}
+
+/* reference src/statevector.c:72-92: output bit i of the index reads input bit L-1-perm[L-1-i] */
+void orc_transpose_statevector(int L, const ocplx* psi, const int* perm, ocplx* psi_trans)
+{
+	const long n = 1L << L;
+	for (long j = 0; j < n; j++) {
+		long k = 0;
+		for (int i = 0; i < L; i++) {
+			if ((j >> i) & 1) { k |= 1L << (L - 1 - perm[L - 1 - i]); }
+		}
+		psi_trans[j] = psi[k];
+	}
+}
+
+/* reference src/statevector.c:102-127: crandn entries in order, then normalised */
+void orc_haar_random_statevector(int L, struct orc_rng* rng, ocplx* psi)
+{
+	const long n = 1L << L;
+	for (long j = 0; j < n; j++) { psi[j] = orc_crandn(rng); }
+	orc_normalize_statevector(L, psi);
+}
+
+/* reference src/target.c:85-149: sum over samples of <-conj(U psi), C psi>, scaled by 2^L / nsamples */
+int orc_circuit_unitary_target_sampling(orc_linear_func ufunc, void* udata, const ocplx* gates, int ngates, const int* wires,
+	int L, long nsamples, struct orc_rng* rng, ocplx* fval)
+{
+	const long n = 1L << L;
+	ocplx* psi = xalloc(n * sizeof(ocplx));
+	ocplx* upsi = xalloc(n * sizeof(ocplx));
+	ocplx* cpsi = xalloc(n * sizeof(ocplx));
+	ocplx f = 0;
+	int ret = 0;
+	for (long b = 0; b < nsamples; b++) {
+		orc_haar_random_statevector(L, rng, psi);
+		if (ufunc(L, psi, udata, upsi) < 0) { ret = -2; break; }
+		orc_apply_quantum_circuit(gates, ngates, wires, L, psi, cpsi);
+		for (long a = 0; a < n; a++) { f += -conj(upsi[a]) * cpsi[a]; }
+	}
+	*fval = f * ((double)n / nsamples);
+	free(cpsi); free(upsi); free(psi);
+	return ret;
+}
+
+/* reference src/target.c:491-505 */
+int orc_brickwall_unitary_target_sampling(orc_linear_func ufunc, void* udata, const ocplx* vlist, int nlayers, int L,
+	const int* perms, long nsamples, struct orc_rng* rng, ocplx* fval)
+{
+	const int ng = nlayers * (L / 2);
+	ocplx* gates = xalloc((size_t)ng * 16 * sizeof(ocplx));
+	int* wires = xalloc((size_t)ng * 2 * sizeof(int));
+	brickwall_to_sequential(L, nlayers, vlist, perms, gates, wires);
+	int ret = orc_circuit_unitary_target_sampling(ufunc, udata, gates, ng, wires, L, nsamples, rng, fval);
+	free(wires); free(gates);
+	return ret;
+}
+
+/* reference src/target.c:807-928: per sample, forward with cache, trace, backward Hessian; scaled sums */
+int orc_brickwall_unitary_target_gradient_hessian_sampling(orc_linear_func ufunc, void* udata, const ocplx* vlist,
+	int nlayers, int L, const int* perms, long nsamples, struct orc_rng* rng, ocplx* fval, ocplx* dvlist, ocplx* hess)
+{
+	const long n = 1L << L;
+	const int ng = nlayers * (L / 2), m = 16 * nlayers;
+	ocplx* psi = xalloc(n * sizeof(ocplx));
+	ocplx* upsi = xalloc(n * sizeof(ocplx));
+	ocplx* wpsi = xalloc(n * sizeof(ocplx));
+	ocplx* cache = xalloc((size_t)ng * n * sizeof(ocplx));
+	ocplx* dvu = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
+	ocplx* hu = xalloc((size_t)m * m * sizeof(ocplx));
+	for (int e = 0; e < 16 * nlayers; e++) { dvlist[e] = 0; }
+	memset(hess, 0, (size_t)m * m * sizeof(ocplx));
+	ocplx f = 0;
+	int ret = 0;
+	for (long b = 0; b < nsamples; b++) {
+		orc_haar_random_statevector(L, rng, psi);
+		if (ufunc(L, psi, udata, upsi) < 0) { ret = -2; break; }
+		for (long a = 0; a < n; a++) { upsi[a] = -conj(upsi[a]); }
+		orc_brickwall_unitary_forward(vlist, nlayers, perms, L, psi, cache, wpsi);
+		for (long a = 0; a < n; a++) { f += upsi[a] * wpsi[a]; }
+		orc_brickwall_unitary_backward_hessian(vlist, nlayers, perms, L, cache, upsi, psi, dvu, hu);
+		for (int e = 0; e < 16 * nlayers; e++) { dvlist[e] += dvu[e]; }
+		for (size_t e = 0; e < (size_t)m * m; e++) { hess[e] += hu[e]; }
+	}
+	const double scale = (double)n / nsamples;
+	*fval = f * scale;
+	for (int e = 0; e < 16 * nlayers; e++) { dvlist[e] *= scale; }
+	for (size_t e = 0; e < (size_t)m * m; e++) { hess[e] *= scale; }
+	free(hu); free(dvu); free(cache); free(wpsi); free(upsi); free(psi);
+	return ret;
+}
+
+/* reference src/target.c:1050-1088 */
+int orc_brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(orc_linear_func ufunc, void* udata,
+	const ocplx* vlist, int nlayers, int L, const int* perms, long nsamples, struct orc_rng* rng, ocplx* fval,
+	double* grad_vec, double* H)
+{
+	const int m = 16 * nlayers;
+	ocplx* dv = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
+	ocplx* hess = xalloc((size_t)m * m * sizeof(ocplx));
+	int ret = orc_brickwall_unitary_target_gradient_hessian_sampling(ufunc, udata, vlist, nlayers, L, perms, nsamples, rng,
+		fval, dv, hess);
+	if (ret == 0) {
+		for (int i = 0; i < nlayers; i++) {
+			for (int e = 0; e < 16; e++) { dv[16*i + e] = conj(dv[16*i + e]); }
+			orc_tangent_to_real(&vlist[16*i], &dv[16*i], &grad_vec[16*i]);
+		}
+		project_hessian_matrix(nlayers, vlist, dv, hess, H);
+	}
+	free(hess); free(dv);
+	return ret;
+}
+
+/* reference src/brickwall_opt.c:176-268 with src/trust_region.c:230-293: every target and
+   derivative evaluation draws fresh samples from rng; default parameters */
+void orc_optimize_brickwall_circuit_hmat_sampling(orc_linear_func ufunc, void* udata, const ocplx* vlist_start,
+	int nlayers, int L, const int* perms, long nsamples, struct orc_rng* rng, int niter, double* f_iter, ocplx* vlist_opt)
+{
+	const int n = 16 * nlayers, m = 32 * nlayers;
+	const int maxiter = 2 * n;
+	const double abstol = 1e-8, reltol = 1e-6, rho_trust = 0.125, maxradius = 0.1;
+	double radius = 0.01;
+	double* x = xalloc(m * sizeof(double));
+	double* xn = xalloc(m * sizeof(double));
+	double* grad = xalloc(n * sizeof(double));
+	double* H = xalloc((size_t)n * n * sizeof(double));
+	double* eta = xalloc(n * sizeof(double));
+	double* heta = xalloc(n * sizeof(double));
+	memcpy(x, vlist_start, m * sizeof(double));
+	for (int k = 0; k < niter; k++) {
+		ocplx fv;
+		orc_brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(ufunc, udata, (const ocplx*)x, nlayers, L, perms,
+			nsamples, rng, &fv, grad, H);
+		const double fx = creal(fv);
+		const bool on_boundary = orc_truncated_cg_hmat(grad, H, n, radius, maxiter, abstol, reltol, eta);
+		for (int i = 0; i < nlayers; i++) { orc_retract((const ocplx*)&x[32*i], &eta[16*i], (ocplx*)&xn[32*i]); }
+		f_iter[k] = fx;
+		for (int a = 0; a < n; a++) { heta[a] = ddot(n, &H[(size_t)a * n], eta); }
+		ocplx fn;
+		orc_brickwall_unitary_target_sampling(ufunc, udata, (const ocplx*)xn, nlayers, L, perms, nsamples, rng, &fn);
+		const double rho = (creal(fn) - fx) / (ddot(n, grad, eta) + 0.5 * ddot(n, eta, heta));
+		if (rho < 0.25) { radius *= 0.25; }
+		else if (rho > 0.75 && on_boundary) { radius = fmin(2 * radius, maxradius); }
+		if (rho > rho_trust) { double* t = x; x = xn; xn = t; }
+	}
+	memcpy(vlist_opt, x, m * sizeof(double));
+	ocplx ff;
+	orc_brickwall_unitary_target_sampling(ufunc, udata, (const ocplx*)x, nlayers, L, perms, nsamples, rng, &ff);
+	f_iter[niter] = creal(ff);
+	free(heta); free(eta); free(H); free(grad); free(xn); free(x);
+}

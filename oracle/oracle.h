@@ -143,6 +143,24 @@ void orc_optimize_brickwall_circuit_hmat(orc_linear_func ufunc, void* udata, con
 /* ---- RNG: reference src/rng.c, src/pcg_basic.c ---- */
 struct orc_pcg32 { uint64_t state, inc; };
 struct orc_rng { struct orc_pcg32 gen[2]; };
+
+/* ---- statevector utilities and sampled (stochastic-trace) targets: reference
+        src/statevector.c:47-127, src/target.c:85-149,491-505,807-928,1050-1088,
+        src/brickwall_opt.c:165-268 ---- */
+void orc_normalize_statevector(int L, ocplx* psi);
+void orc_transpose_statevector(int L, const ocplx* psi, const int* perm, ocplx* psi_trans);
+void orc_haar_random_statevector(int L, struct orc_rng* rng, ocplx* psi);
+int orc_circuit_unitary_target_sampling(orc_linear_func ufunc, void* udata, const ocplx* gates, int ngates, const int* wires,
+	int L, long nsamples, struct orc_rng* rng, ocplx* fval);
+int orc_brickwall_unitary_target_sampling(orc_linear_func ufunc, void* udata, const ocplx* vlist, int nlayers, int L,
+	const int* perms, long nsamples, struct orc_rng* rng, ocplx* fval);
+int orc_brickwall_unitary_target_gradient_hessian_sampling(orc_linear_func ufunc, void* udata, const ocplx* vlist,
+	int nlayers, int L, const int* perms, long nsamples, struct orc_rng* rng, ocplx* fval, ocplx* dvlist, ocplx* hess);
+int orc_brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(orc_linear_func ufunc, void* udata,
+	const ocplx* vlist, int nlayers, int L, const int* perms, long nsamples, struct orc_rng* rng, ocplx* fval,
+	double* grad_vec, double* H);
+void orc_optimize_brickwall_circuit_hmat_sampling(orc_linear_func ufunc, void* udata, const ocplx* vlist_start,
+	int nlayers, int L, const int* perms, long nsamples, struct orc_rng* rng, int niter, double* f_iter, ocplx* vlist_opt);
 void     orc_seed_rng_state(uint64_t seed, struct orc_rng* rng);
 uint32_t orc_rand_uint32(struct orc_rng* rng);
 uint64_t orc_rand_uint64(struct orc_rng* rng);

@@ -86,7 +86,7 @@ HVP_FUNC = ct.CFUNCTYPE(None, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double),
 EXPORTED = [
     # statevector.h
     "allocate_statevector", "free_statevector", "add_statevectors", "allocate_statevector_array",
-    "free_statevector_array",
+    "free_statevector_array", "normalize_statevector", "transpose_statevector", "haar_random_statevector",
     # gate.h
     "apply_gate", "apply_gate_backward", "apply_gate_to_array", "apply_gate_backward_array", "apply_gate_placeholder",
     # quantum_circuit.h
@@ -101,7 +101,9 @@ EXPORTED = [
     "brickwall_unitary_target", "brickwall_unitary_target_and_gradient",
     "brickwall_unitary_target_and_projected_gradient", "brickwall_unitary_target_hessian_vector_product",
     "brickwall_unitary_target_projected_hessian_vector_product", "brickwall_unitary_target_gradient_hessian",
-    "brickwall_unitary_target_gradient_vector_hessian_matrix",
+    "brickwall_unitary_target_gradient_vector_hessian_matrix", "circuit_unitary_target_sampling",
+    "brickwall_unitary_target_sampling", "brickwall_unitary_target_gradient_hessian_sampling",
+    "brickwall_unitary_target_gradient_vector_hessian_matrix_sampling",
     # matrix.h
     "zero_matrix", "identity_matrix", "scale_matrix", "conjugate_matrix", "add_matrix", "sub_matrix",
     "add_matrices", "sub_matrices", "transpose", "adjoint", "symm", "antisymm", "real_to_antisymm",
@@ -113,6 +115,7 @@ EXPORTED = [
     # trust_region.h, brickwall_opt.h
     "truncated_cg_hvp", "riemannian_trust_region_optimize_hvp", "optimize_brickwall_circuit_hvp",
     "truncated_cg_hmat", "riemannian_trust_region_optimize_hmat", "optimize_brickwall_circuit_hmat",
+    "optimize_brickwall_circuit_hmat_sampling",
     # circuit_opt.h
     "optimize_quantum_circuit",
     # rqc_device.h
@@ -428,6 +431,27 @@ def brickwall_unitary_backward(vlist, perms, cache_states, dpsi_out):
     return dpsi, dv
 
 
+def normalize_statevector(psi):
+    out = _cplx(psi).copy()
+    lib().normalize_statevector(ct.byref(_sv(out)))
+    return out
+
+
+def transpose_statevector(psi, perm):
+    psi = _cplx(psi)
+    perm = np.ascontiguousarray(perm, dtype=np.int32)
+    out = np.empty_like(psi)
+    lib().transpose_statevector(ct.byref(_sv(psi)), _p(perm), ct.byref(_sv(out)))
+    return out
+
+
+def haar_random_statevector(nqubits, rng):
+    """A Haar-random state drawn from rng (an Rng; advances it)."""
+    out = np.empty(1 << nqubits, np.complex128)
+    lib().haar_random_statevector(ct.byref(_sv(out)), ct.byref(rng.s))
+    return out
+
+
 def apply_gate_backward_array(V, i, j, psi, dpsi_out):
     """Gate gradients of every state of the (2^L, nstates) array psi: (nstates, 4, 4)."""
     psi, dpsi_out = _cplx(psi), _cplx(dpsi_out)
@@ -600,6 +624,67 @@ def brickwall_unitary_target_gradient_vector_hessian_matrix(target, vlist, perms
     return f[0], g, H
 
 
+def circuit_unitary_target_sampling(target, gates, wires, nqubits, nsamples, rng):
+    gates, w, f = _gates(gates), _wires(wires), _f()
+    fp, ud = _target_args(target)
+    L = lib()
+    L.circuit_unitary_target_sampling.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_int,
+                                                  ct.c_long, ct.c_void_p, ct.c_void_p]
+    _check("circuit_unitary_target_sampling", L.circuit_unitary_target_sampling(
+        fp, ud, _p(gates), len(gates), _p(w), nqubits, nsamples, ct.addressof(rng.s), _p(f)))
+    return f[0]
+
+
+def brickwall_unitary_target_sampling(target, vlist, perms, nsamples, rng, nqubits=None):
+    vlist, f = _gates(vlist), _f()
+    perms, rows = _perms(perms)
+    Lq = perms.shape[1] if nqubits is None else nqubits
+    fp, ud = _target_args(target)
+    L = lib()
+    L.brickwall_unitary_target_sampling.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p,
+                                                    ct.c_long, ct.c_void_p, ct.c_void_p]
+    _check("brickwall_unitary_target_sampling", L.brickwall_unitary_target_sampling(
+        fp, ud, _p(vlist), len(vlist), Lq, ct.cast(rows, ct.c_void_p), nsamples, ct.addressof(rng.s), _p(f)))
+    return f[0]
+
+
+def brickwall_unitary_target_gradient_hessian_sampling(target, vlist, perms, nsamples, rng, nqubits=None):
+    vlist, f = _gates(vlist), _f()
+    perms, rows = _perms(perms)
+    Lq = perms.shape[1] if nqubits is None else nqubits
+    m = 16 * len(vlist)
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    hess = np.empty((m, m), np.complex128)
+    fp, ud = _target_args(target)
+    L = lib()
+    L.brickwall_unitary_target_gradient_hessian_sampling.argtypes = [
+        ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_long, ct.c_void_p, ct.c_void_p,
+        ct.c_void_p, ct.c_void_p]
+    _check("brickwall_unitary_target_gradient_hessian_sampling", L.brickwall_unitary_target_gradient_hessian_sampling(
+        fp, ud, _p(vlist), len(vlist), Lq, ct.cast(rows, ct.c_void_p), nsamples, ct.addressof(rng.s), _p(f), _p(dv),
+        _p(hess)))
+    return f[0], dv, hess
+
+
+def brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(target, vlist, perms, nsamples, rng, nqubits=None):
+    vlist, f = _gates(vlist), _f()
+    perms, rows = _perms(perms)
+    Lq = perms.shape[1] if nqubits is None else nqubits
+    m = 16 * len(vlist)
+    g = np.empty(m)
+    H = np.empty((m, m))
+    fp, ud = _target_args(target)
+    L = lib()
+    L.brickwall_unitary_target_gradient_vector_hessian_matrix_sampling.argtypes = [
+        ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_long, ct.c_void_p, ct.c_void_p,
+        ct.c_void_p, ct.c_void_p]
+    _check("brickwall_unitary_target_gradient_vector_hessian_matrix_sampling",
+           L.brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(
+               fp, ud, _p(vlist), len(vlist), Lq, ct.cast(rows, ct.c_void_p), nsamples, ct.addressof(rng.s), _p(f), _p(g),
+               _p(H)))
+    return f[0], g, H
+
+
 # --------------------------------------------------------------- optimiser
 def rtr_default_params(n):
     """set_rtr_default_params (reference src/trust_region.h:68-76)."""
@@ -629,6 +714,24 @@ def optimize_brickwall_circuit_hmat(target, vlist_start, perms, niter, params=No
     fp, ud = _target_args(target)
     lib().optimize_brickwall_circuit_hmat(fp, ud, _p(vlist_start), nl, perms.shape[1], rows, ct.byref(params), niter,
                                           _p(f_iter), _p(vopt))
+    return f_iter, vopt
+
+
+def optimize_brickwall_circuit_hmat_sampling(target, vlist_start, perms, niter, nsamples, rng, params=None):
+    vlist_start = _gates(vlist_start)
+    perms, rows = _perms(perms)
+    nl = len(vlist_start)
+    params = rtr_default_params(16 * nl) if params is None else params
+    f_iter = np.empty(niter + 1)
+    vopt = np.empty_like(vlist_start)
+    fp, ud = _target_args(target)
+    L = lib()
+    L.optimize_brickwall_circuit_hmat_sampling.argtypes = [
+        ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_long, ct.c_void_p, ct.c_void_p,
+        ct.c_int, ct.c_void_p, ct.c_void_p]
+    L.optimize_brickwall_circuit_hmat_sampling(fp, ud, _p(vlist_start), nl, perms.shape[1], ct.cast(rows, ct.c_void_p),
+                                               nsamples, ct.addressof(rng.s), ct.addressof(params), niter, _p(f_iter),
+                                               _p(vopt))
     return f_iter, vopt
 
 

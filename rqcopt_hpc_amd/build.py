@@ -42,7 +42,8 @@ def build(verbose=True):
         _run([HIPCC] + COMMON + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     tmp = OUT + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs +
+         ["-L/opt/rocm/lib", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, OUT)
     return OUT
 

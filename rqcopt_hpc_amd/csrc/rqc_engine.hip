@@ -33,6 +33,8 @@
 #include "rqcopt/brickwall_circuit.h"
 #include "rqcopt/target.h"
 #include "rqcopt/rqc_device.h"
+#include "rqcopt/rng.h"
+#include <rocblas/rocblas.h>
 
 namespace rqc {
 
@@ -234,14 +236,14 @@ static int states_needed(Mode m, int G)
 
 // largest power-of-two column block whose nstates state arrays fit the budget
 // (0 = 85% of the free device memory plus the `held` bytes the caller would release)
-static long choose_block(long N, int nstates, size_t elem, size_t budget, size_t held)
+static long choose_block(long N, int nstates, size_t elem, size_t budget, size_t held, long Bmax = 0)
 {
 	if (budget == 0) {
 		size_t fr = 0, tot = 0;
 		RQC_HIP(hipMemGetInfo(&fr, &tot));
 		budget = (size_t)(0.85 * (double)(fr + held));
 	}
-	long B = N;
+	long B = (Bmax > 0) ? std::min(N, Bmax) : N;
 	while (B > 1 && (size_t)nstates * N * B * elem > budget) { B /= 2; }
 	if ((size_t)nstates * N * B * elem > budget) {
 		throw Error(-1, "device memory too small for " + std::to_string(nstates) + " statevectors of " +
@@ -254,7 +256,7 @@ static long choose_block(long N, int nstates, size_t elem, size_t budget, size_t
 // acc[0] = f, acc[1 + k*32 + e]: e < 16 gradient of gate k, e >= 16 its HVP
 template <typename T>
 static void run_gates(Slot& s, const EvalPlan& pl, const Circuit& C, const zc* dU, zc* d_acc, zc* d_partial, cx<T>* S,
-	const int* skip)
+	const int* skip, const cx<T>* X, const zc* W)
 {
 	const Mode mode = pl.mode;
 	const int L = C.L, G = C.ngates;
@@ -278,8 +280,14 @@ static void run_gates(Slot& s, const EvalPlan& pl, const Circuit& C, const zc* d
 	const double ab = (double)sz * sizeof(cx<T>);   // bytes of one state block
 	const bool hvp = (mode == Mode::Hvp);
 
-	for (long b0 = 0; b0 < N; b0 += B) {
-		{
+	for (long blk = 0; blk < pl.nblk; blk++) {
+		const long b0 = blk * B;
+		// the block's input columns and the trace's seed columns: basis columns and U, or samples and U X
+		const zc* seed = pl.sampled ? W + (size_t)blk * sz : dU;
+		const long ldu = pl.sampled ? B : N, u0 = pl.sampled ? 0 : b0;
+		if (pl.sampled) {
+			RQC_HIP(hipMemcpyAsync(psi(0), X + (size_t)blk * sz, sz * sizeof(cx<T>), hipMemcpyDeviceToDevice, s.stream));
+		} else {
 			LaunchTimer lt(s, RQC_KERNEL_OTHER, ab);
 			k_identity_block<T><<<nblocks(N * B), kBlock, 0, s.stream>>>(psi(0), B, logB, b0, N * B);
 		}
@@ -300,7 +308,7 @@ static void run_gates(Slot& s, const EvalPlan& pl, const Circuit& C, const zc* d
 		cx<T>* psiG = (mode == Mode::Target) ? psi(G % 2) : psi(G);
 		{
 			LaunchTimer lt(s, RQC_KERNEL_TARGET, (double)sz * sizeof(zc) + (mode == Mode::Target ? 1 : 2) * ab);
-			k_target_trace<T><<<gtr, kBlock, 0, s.stream>>>(dU, N, b0, psiG, mode == Mode::Target ? nullptr : phi[0],
+			k_target_trace<T><<<gtr, kBlock, 0, s.stream>>>(seed, ldu, u0, psiG, mode == Mode::Target ? nullptr : phi[0],
 				B, logB, N * B, d_partial, skip);
 		}
 		{
@@ -631,7 +639,7 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, bool fp32, bool project, bool unitary,
-	size_t budget, size_t held, const TileChoice* choice)
+	size_t budget, size_t held, const TileChoice* choice, long ncols)
 {
 	const int logE_c = (choice != nullptr) ? choice->logE : 0;
 	const int gmax = (choice != nullptr && choice->gmax > 0) ? std::min(choice->gmax, (int)kMaxTileGates) : tile_gmax();
@@ -646,6 +654,10 @@ EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, boo
 	pl.per = per;
 	pl.ng = C.ngates / per;
 	pl.N = 1L << C.L;
+	pl.sampled = (ncols > 0);
+	pl.ncols = pl.sampled ? ncols : pl.N;
+	long Bmax = 1;   // power-of-two cover of the columns
+	while (Bmax < pl.ncols) { Bmax <<= 1; }
 	const size_t elem = fp32 ? sizeof(cx<float>) : sizeof(cx<double>);
 	// tile-pass engine when a tile fits the circuit (and, for reverse sweeps, the gates are unitary)
 	bool tiles = dev.engine == RQC_ENGINE_TILE_PASS && C.L <= 30 && (mode == Mode::Target || unitary);
@@ -659,7 +671,7 @@ EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, boo
 		passes = plan_passes(C, ts.K, gmax);
 		const int P = (int)passes.size();
 		pl.nst = (mode == Mode::Target) ? 2 : (mode == Mode::Grad) ? P + 2 : 2 * P + 4;
-		pl.B = choose_block(pl.N, pl.nst, elem, budget, held);
+		pl.B = choose_block(pl.N, pl.nst, elem, budget, held, Bmax);
 		ts = fp32 ? tile_shape<float>(C.L, pl.B, logE_c) : tile_shape<double>(C.L, pl.B, logE_c);
 		tiles = ts.usable();
 	}
@@ -672,10 +684,11 @@ EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, boo
 		pl.tp = fp32 ? plan_tiles<float>(dev, C, pl, passes, ts) : plan_tiles<double>(dev, C, pl, passes, ts);
 	} else {
 		pl.nst = states_needed(mode, C.ngates);
-		pl.B = choose_block(pl.N, pl.nst, elem, budget, held);
+		pl.B = choose_block(pl.N, pl.nst, elem, budget, held, Bmax);
 		pl.passes = C.ngates;
 		pl.grev_pg = reduce_grid((pl.N / 4) * pl.B);
 	}
+	pl.nblk = (pl.ncols + pl.B - 1) / pl.B;
 	pl.gtr = reduce_grid(pl.N * pl.B);
 	pl.n_acc = 1 + 32 * (size_t)pl.G;
 	size_t npart = pl.gtr;
@@ -752,7 +765,7 @@ void fill_eval_tables(const EvalPlan& pl, const Circuit& C, char* host)
 // tile-pass engine: several gates per HBM pass through LDS tiles
 template <typename T>
 static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* d_partial, const cx<T>* mats,
-	const GateTerms* gterms, cx<T>* S, const int* skip)
+	const GateTerms* gterms, cx<T>* S, const int* skip, const cx<T>* X, const zc* W)
 {
 	const Mode mode = pl.mode;
 	const TilePlan& tp = *pl.tp;
@@ -778,10 +791,16 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	const double ab = (double)sz * sizeof(cx<T>);   // bytes of one state block
 	const int ns = dual ? 2 : 1;                     // state arrays per side
 
-	for (long b0 = 0; b0 < N; b0 += B) {
+	for (long blk = 0; blk < pl.nblk; blk++) {
+		const long b0 = blk * B;
+		// the block's input columns and the trace's seed columns: basis columns and U, or samples and U X
+		const zc* seed = pl.sampled ? W + (size_t)blk * sz : dU;
+		const long ldu = pl.sampled ? B : N, u0 = pl.sampled ? 0 : b0;
 		for (int p = 0; p < P; p++) {
-			const int flags = (p == 0) ? 3 : 0;
-			const cx<T>* in = (p == 0) ? nullptr : (mode == Mode::Target ? buf[(p + 1) % 2] : PS(p));
+			// pass 0 starts from the identity block (flags 1) or the sample block, with dpsi = 0 (flags 2)
+			const int flags = (p == 0) ? (pl.sampled ? 2 : 3) : 0;
+			const cx<T>* in = (p == 0) ? (pl.sampled ? X + (size_t)blk * sz : nullptr)
+			                           : (mode == Mode::Target ? buf[(p + 1) % 2] : PS(p));
 			cx<T>* out = (mode == Mode::Target) ? buf[p % 2] : PS(p + 1);
 			LaunchTimer lt(s, RQC_KERNEL_FORWARD, (double)(ns * (p > 0) + ns) * ab);
 			kf<<<tp.gfwd, kTileThreads, tp.lds_fwd, s.stream>>>(in, out, (dual && p > 0) ? DPS(p) : nullptr,
@@ -790,7 +809,7 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 		cx<T>* psiG = (mode == Mode::Target) ? buf[(P - 1) % 2] : PS(P);
 		{
 			LaunchTimer lt(s, RQC_KERNEL_TARGET, (double)sz * sizeof(zc) + (mode == Mode::Target ? 1 : 2) * ab);
-			k_target_trace<T><<<gtr, kBlock, 0, s.stream>>>(dU, N, b0, psiG, mode == Mode::Target ? nullptr : phi[0],
+			k_target_trace<T><<<gtr, kBlock, 0, s.stream>>>(seed, ldu, u0, psiG, mode == Mode::Target ? nullptr : phi[0],
 				B, logB, N * B, tpart, skip);
 		}
 		{
@@ -851,10 +870,11 @@ __global__ void k_finalize(const zc* __restrict__ acc, int ngroups, int per, con
 }
 
 void enqueue_eval(const Context& dev, Slot& s, const EvalPlan& pl, const Circuit& C, const zc* dU, char* work,
-	char* tab, char* res_dev, const char* host_tab, const int* skip)
+	char* tab, char* res_dev, const char* host_tab, const int* skip, const void* X, const zc* W)
 {
 	(void)dev;
 	if (skip != nullptr && !pl.tiles) { throw Error(-3, "skip flags need the tile-pass engine"); }
+	if (pl.sampled && (X == nullptr || W == nullptr)) { throw Error(-3, "a sampled evaluation needs its sample and seed blocks"); }
 	zc* acc = reinterpret_cast<zc*>(work + pl.off_acc);
 	zc* part = reinterpret_cast<zc*>(work + pl.off_part);
 	if (tab == nullptr) { tab = work + pl.off_tab; }
@@ -873,12 +893,14 @@ void enqueue_eval(const Context& dev, Slot& s, const EvalPlan& pl, const Circuit
 	const GateTerms* gterms = reinterpret_cast<const GateTerms*>(tab + pl.mats_bytes);
 	if (pl.fp32) {
 		cx<float>* S = reinterpret_cast<cx<float>*>(work + pl.off_states);
-		if (pl.tiles) { run_tiles<float>(s, pl, dU, acc, part, reinterpret_cast<const cx<float>*>(tab), gterms, S, skip); }
-		else { run_gates<float>(s, pl, C, dU, acc, part, S, skip); }
+		const cx<float>* Xs = static_cast<const cx<float>*>(X);
+		if (pl.tiles) { run_tiles<float>(s, pl, dU, acc, part, reinterpret_cast<const cx<float>*>(tab), gterms, S, skip, Xs, W); }
+		else { run_gates<float>(s, pl, C, dU, acc, part, S, skip, Xs, W); }
 	} else {
 		cx<double>* S = reinterpret_cast<cx<double>*>(work + pl.off_states);
-		if (pl.tiles) { run_tiles<double>(s, pl, dU, acc, part, reinterpret_cast<const cx<double>*>(tab), gterms, S, skip); }
-		else { run_gates<double>(s, pl, C, dU, acc, part, S, skip); }
+		const cx<double>* Xs = static_cast<const cx<double>*>(X);
+		if (pl.tiles) { run_tiles<double>(s, pl, dU, acc, part, reinterpret_cast<const cx<double>*>(tab), gterms, S, skip, Xs, W); }
+		else { run_gates<double>(s, pl, C, dU, acc, part, S, skip, Xs, W); }
 	}
 	LaunchTimer lt(s, RQC_KERNEL_OTHER);
 	k_finalize<<<std::max(1, (pl.ng + 63) / 64), 64, 0, s.stream>>>(acc, pl.ng, pl.per, vin, vin + 16 * (size_t)pl.ng, res,
@@ -913,7 +935,7 @@ static EvalResult eval_targets(const Circuit& C, linear_func ufunc, void* udata,
 	fill_eval_tables(pl, C, tab.data());
 	c.work.reserve(pl.bytes);
 	char* work = c.work.as<char>();
-	enqueue_eval(c, c, pl, C, dU, work, nullptr, nullptr, tab.data(), nullptr);
+	enqueue_eval(c, c, pl, C, dU, work, nullptr, nullptr, tab.data(), nullptr, nullptr, nullptr);
 	std::vector<char> res(pl.res_bytes());
 	const size_t nres = (mode == Mode::Target) ? sizeof(zc) : res.size();
 	RQC_HIP(hipMemcpyAsync(res.data(), work + pl.off_res, nres, hipMemcpyDeviceToHost, c.stream));
@@ -1159,6 +1181,76 @@ extern "C" void add_statevectors(const struct statevector* RQC_RESTRICT psi1, co
 		}
 		s.download();
 	} catch (const std::exception& e) { fatal("add_statevectors", e); }
+}
+
+extern "C" void normalize_statevector(struct statevector* RQC_RESTRICT psi)
+{
+	try {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const long n = 1L << psi->nqubits;
+		const unsigned g = reduce_grid(n);
+		c.small.reserve((1 + (size_t)g) * sizeof(zc));
+		zc* acc = c.small.as<zc>();
+		Staging s(c);
+		const int a = s.add(psi->data, n * sizeof(zc), true, true);
+		s.upload();
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_fill_zero<<<1, kBlock, 0, c.stream>>>(acc, 1, nullptr);
+			k_norm2_partials<<<g, kBlock, 0, c.stream>>>(s.dev<zc>(a), n, acc + 1);
+			k_reduce_partials<<<1, kBlock, 0, c.stream>>>(acc + 1, (int)g, 1, acc, nullptr);
+			k_scale_inv_norm<<<nblocks(n), kBlock, 0, c.stream>>>(s.dev<zc>(a), n, acc);
+		}
+		s.download();
+	} catch (const std::exception& e) { fatal("normalize_statevector", e); }
+}
+
+extern "C" void transpose_statevector(const struct statevector* RQC_RESTRICT psi, const int* perm, struct statevector* RQC_RESTRICT psi_trans)
+{
+	try {
+		const int L = psi->nqubits;
+		if (psi_trans->nqubits != L) { throw Error(-3, "qubit counts differ"); }
+		if (L < 1 || L > 32) { throw Error(-3, "transpose_statevector supports 1 .. 32 qubits"); }
+		BitMap m;
+		std::vector<int> seen(L, 0);
+		for (int i = 0; i < L; i++) {
+			if (perm[i] < 0 || perm[i] >= L || seen[perm[i]]) { throw Error(-3, "perm is not a permutation"); }
+			seen[perm[i]] = 1;
+		}
+		for (int q = 0; q < L; q++) { m.src[q] = L - 1 - perm[L - 1 - q]; }
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const long n = 1L << L;
+		Staging s(c);
+		const int a = s.add(psi->data, n * sizeof(zc), true, false);
+		const int o = s.add(psi_trans->data, n * sizeof(zc), false, true);
+		s.upload();
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_transpose_amplitudes<<<nblocks(n), kBlock, 0, c.stream>>>(s.dev<zc>(a), s.dev<zc>(o), m, L, n);
+		}
+		s.download();
+	} catch (const std::exception& e) { fatal("transpose_statevector", e); }
+}
+
+static void draw_haar_state(long n, struct rng_state* rng, zc* psi);
+
+extern "C" void haar_random_statevector(struct statevector* psi, struct rng_state* rng)
+{
+	try {
+		const long n = 1L << psi->nqubits;
+		std::vector<zc> h((size_t)n);
+		draw_haar_state(n, rng, h.data());
+		if (is_device_pointer(psi->data)) {
+			Context& c = ctx();
+			std::lock_guard<std::recursive_mutex> lk(c.mu);
+			RQC_HIP(hipMemcpyAsync(psi->data, h.data(), n * sizeof(zc), hipMemcpyHostToDevice, c.stream));
+			RQC_HIP(hipStreamSynchronize(c.stream));
+		} else {
+			std::memcpy(psi->data, h.data(), n * sizeof(zc));
+		}
+	} catch (const std::exception& e) { fatal("haar_random_statevector", e); }
 }
 
 // ============================================================== C ABI: gate.h
@@ -1739,23 +1831,145 @@ extern "C" int brickwall_unitary_target_projected_hessian_vector_product(linear_
 	});
 }
 
+// ============================================================== sampled (stochastic-trace) targets
+// The trace over the 2^L basis columns is replaced by nsamples Haar-random
+// states (reference src/target.c:85-149, :807-928): f, gradients and Hessians
+// are the sums over the samples scaled by 2^L / nsamples.  The states are drawn
+// on the host from the caller's generator in the reference's order (sample by
+// sample, entry by entry; crandn then normalisation), so the generator's
+// stream and the samples match the reference's.  The trace seeds U psi come
+// from one rocBLAS GEMM per column block for a device-resident U, or from the
+// caller's callback, one call per sample as in the reference.
+
+// psi_j = crandn, then psi /= ||psi|| (reference src/statevector.c:47-66, :102-127)
+static void draw_haar_state(long n, struct rng_state* rng, zc* psi)
+{
+	double s = 0;
+	for (long j = 0; j < n; j++) {
+		const numeric z = crandn(rng);
+		psi[j] = zc{z.real(), z.imag()};
+		s += z.real() * z.real() + z.imag() * z.imag();
+	}
+	const double nrm = std::sqrt(s);
+	if (nrm > 0) {
+		const double inv = 1.0 / nrm;
+		for (long j = 0; j < n; j++) { psi[j] = zc{psi[j].x * inv, psi[j].y * inv}; }
+	}
+}
+
+static rocblas_handle blas_handle(Context& c)
+{
+	static rocblas_handle h = nullptr;
+	if (h == nullptr) {
+		if (rocblas_create_handle(&h) != rocblas_status_success) { throw Error(-1, "rocblas_create_handle failed"); }
+	}
+	if (rocblas_set_stream(h, c.stream) != rocblas_status_success) { throw Error(-1, "rocblas_set_stream failed"); }
+	return h;
+}
+
+// device sample blocks X (sweep precision) and seeds W = U X (FP64), in the
+// plan's block layout: block k holds samples k B .. k B + B - 1 as an N x B
+// array, zero-padded past nsamples; both live in c.samples
+struct SampleBlocks
+{
+	const void* X = nullptr;
+	const zc* W = nullptr;
+};
+
+static SampleBlocks prepare_samples(Context& c, const EvalPlan& pl, linear_func ufunc, void* udata, struct rng_state* rng)
+{
+	const long N = pl.N, B = pl.B, nb = pl.nblk;
+	const size_t sz = (size_t)N * B;
+	const size_t xbytes = nb * sz * sizeof(zc);
+	const size_t x32 = pl.fp32 ? nb * sz * sizeof(cx<float>) : 0;
+	c.samples.reserve(2 * xbytes + x32);
+	zc* X64 = c.samples.as<zc>();
+	zc* W = X64 + nb * sz;
+	std::vector<zc> hx(nb * sz, zc{0, 0}), col((size_t)N);
+	std::vector<zc> hw;
+	const bool device_u = (ufunc == rqc_device_ufunc);
+	if (!device_u) {
+		if (ufunc == nullptr) { throw Error(-3, "ufunc is NULL"); }
+		hw.assign(nb * sz, zc{0, 0});
+	}
+	std::vector<zc> out((size_t)N);
+	for (long b = 0; b < pl.ncols; b++) {
+		draw_haar_state(N, rng, col.data());
+		const size_t base = (size_t)(b / B) * sz + (size_t)(b % B);
+		for (long a = 0; a < N; a++) { hx[base + (size_t)a * B] = col[a]; }
+		if (!device_u) {
+			struct statevector sin = {reinterpret_cast<numeric*>(col.data()), pl.L};
+			struct statevector sout = {reinterpret_cast<numeric*>(out.data()), pl.L};
+			const int r = ufunc(&sin, udata, &sout);
+			if (r < 0) { throw Error(-2, "call of 'ufunc' failed, return value: " + std::to_string(r)); }
+			for (long a = 0; a < N; a++) { hw[base + (size_t)a * B] = out[a]; }
+		}
+	}
+	RQC_HIP(hipMemcpyAsync(X64, hx.data(), xbytes, hipMemcpyHostToDevice, c.stream));
+	if (device_u) {
+		// W_k = U X_k: row-major N x B = (X_k^T U^T)^T, i.e. the column-major product of X_k (B x N) and U (N x N)
+		const zc* U = device_target_matrix(udata, pl.L);
+		rocblas_handle h = blas_handle(c);
+		const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0);
+		for (long k = 0; k < nb; k++) {
+			const rocblas_status st = rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)B,
+				(rocblas_int)N, (rocblas_int)N, &one, reinterpret_cast<const rocblas_double_complex*>(X64 + k * sz),
+				(rocblas_int)B, reinterpret_cast<const rocblas_double_complex*>(U), (rocblas_int)N, &zero,
+				reinterpret_cast<rocblas_double_complex*>(W + k * sz), (rocblas_int)B);
+			if (st != rocblas_status_success) { throw Error(-1, std::string("rocblas_zgemm failed: ") + rocblas_status_to_string(st)); }
+		}
+	} else {
+		RQC_HIP(hipMemcpyAsync(W, hw.data(), xbytes, hipMemcpyHostToDevice, c.stream));
+	}
+	SampleBlocks sb;
+	sb.W = W;
+	if (pl.fp32) {
+		cx<float>* X32 = reinterpret_cast<cx<float>*>(W + nb * sz);
+		k_to_float<<<nblocks((long)(nb * sz)), kBlock, 0, c.stream>>>(X64, X32, (long)(nb * sz));
+		sb.X = X32;
+	} else {
+		sb.X = X64;
+	}
+	// the host buffers are pageable and go out of scope: finish the copies here
+	RQC_HIP(hipStreamSynchronize(c.stream));
+	return sb;
+}
+
+static void scale_result(EvalResult& R, double a)
+{
+	R.f = a * R.f;
+	for (auto& z : R.dv) { z = a * z; }
+	for (auto& z : R.hz) { z = a * z; }
+	for (auto& x : R.grad) { x *= a; }
+	for (auto& x : R.hvp) { x *= a; }
+}
+
 // Complex Hessian of the brickwall target (reference src/target.c:687-797).  Column
 // (j, y) is the HVP with the unit direction E_y on every gate of layer j, so the
 // 16 nlayers columns are HVP evaluations of one plan: their tables go up in one
 // copy, the evaluations are enqueued back to back into one workspace with a
 // result region each, and the results come back in one copy.  The matrix is
 // symmetric up to rounding and is symmetrised; f and dvlist come with column 0.
-static void eval_hessian(const Circuit& C0, linear_func ufunc, void* udata, int per, zc* fval, zc* dvlist, zc* hess)
+static void eval_hessian(const Circuit& C0, linear_func ufunc, void* udata, int per, zc* fval, zc* dvlist, zc* hess,
+	long nsamples = 0, struct rng_state* rng = nullptr)
 {
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
-	const zc* dU = resolve_target(c, ufunc, udata, C0.L);
 	Circuit C = C0;
 	C.dirs.assign(C.gates.size(), zc{0, 0});
 	const int ng = C.ngates / per, m = 16 * ng;
 	const bool unitary = gates_near_unitary(C);
 	const EvalPlan pl = plan_eval(c, C, Mode::Hvp, per, c.precision == RQC_PRECISION_MIXED, false, unitary, c.ws_limit,
-		c.work.bytes);
+		c.work.bytes, nullptr, nsamples);
+	const zc* dU = nullptr;
+	SampleBlocks sb;
+	if (pl.sampled) {
+		if (rng == nullptr) { throw Error(-3, "rng is NULL"); }
+		sb = prepare_samples(c, pl, ufunc, udata, rng);
+	} else {
+		dU = resolve_target(c, ufunc, udata, C0.L);
+	}
+	const double scale = pl.sampled ? (double)pl.N / (double)nsamples : 1.0;
 	const size_t ts = align256(pl.tab_bytes), rs = align256(pl.res_bytes()), base = align256(pl.bytes);
 	std::vector<char> htab((size_t)m * ts);
 	std::vector<Circuit> cols((size_t)m, C);
@@ -1771,7 +1985,8 @@ static void eval_hessian(const Circuit& C0, linear_func ufunc, void* udata, int 
 	char* ress = tabs + (size_t)m * ts;
 	RQC_HIP(hipMemcpyAsync(tabs, htab.data(), htab.size(), hipMemcpyHostToDevice, c.stream));
 	for (int col = 0; col < m; col++) {
-		enqueue_eval(c, c, pl, cols[col], dU, work, tabs + (size_t)col * ts, ress + (size_t)col * rs, nullptr, nullptr);
+		enqueue_eval(c, c, pl, cols[col], dU, work, tabs + (size_t)col * ts, ress + (size_t)col * rs, nullptr, nullptr,
+			sb.X, sb.W);
 	}
 	std::vector<char> hres((size_t)m * rs);
 	RQC_HIP(hipMemcpyAsync(hres.data(), ress, hres.size(), hipMemcpyDeviceToHost, c.stream));
@@ -1781,6 +1996,7 @@ static void eval_hessian(const Circuit& C0, linear_func ufunc, void* udata, int 
 	for (int col = 0; col < m; col++) {
 		EvalResult R;
 		unpack_result(pl, hres.data() + (size_t)col * rs, R);
+		scale_result(R, scale);
 		if (col == 0) {
 			*fval = R.f;
 			std::memcpy(dvlist, R.dv.data(), R.dv.size() * sizeof(zc));
@@ -1804,6 +2020,63 @@ extern "C" int brickwall_unitary_target_gradient_hessian(linear_func ufunc, void
 		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
 		zc f;
 		eval_hessian(C, ufunc, udata, nqubits / 2, &f, reinterpret_cast<zc*>(dvlist), reinterpret_cast<zc*>(hess));
+		std::memcpy(fval, &f, sizeof(zc));
+	});
+}
+
+static EvalResult eval_sampled(const Circuit& C, linear_func ufunc, void* udata, Mode mode, int per, bool project, long nsamples,
+	struct rng_state* rng)
+{
+	if (nsamples < 1) { throw Error(-3, "nsamples must be positive"); }
+	if (rng == nullptr) { throw Error(-3, "rng is NULL"); }
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	const bool unitary = (mode == Mode::Target) || gates_near_unitary(C);
+	const EvalPlan pl = plan_eval(c, C, mode, per, c.precision == RQC_PRECISION_MIXED, project, unitary, c.ws_limit,
+		c.work.bytes, nullptr, nsamples);
+	const SampleBlocks sb = prepare_samples(c, pl, ufunc, udata, rng);
+	std::vector<char> tab(pl.tab_bytes);
+	fill_eval_tables(pl, C, tab.data());
+	c.work.reserve(pl.bytes);
+	char* work = c.work.as<char>();
+	enqueue_eval(c, c, pl, C, nullptr, work, nullptr, nullptr, tab.data(), nullptr, sb.X, sb.W);
+	std::vector<char> res(pl.res_bytes());
+	RQC_HIP(hipMemcpyAsync(res.data(), work + pl.off_res, res.size(), hipMemcpyDeviceToHost, c.stream));
+	RQC_HIP(hipStreamSynchronize(c.stream));
+	RQC_HIP(hipGetLastError());
+	EvalResult R;
+	unpack_result(pl, res.data(), R);
+	scale_result(R, (double)pl.N / (double)nsamples);
+	return R;
+}
+
+extern "C" int circuit_unitary_target_sampling(linear_func ufunc, void* udata, const struct mat4x4 gates[], const int ngates,
+	const int wires[], const int nqubits, const long nsamples, struct rng_state* rng, numeric* fval)
+{
+	return target_call("circuit_unitary_target_sampling", [&] {
+		Circuit C = make_circuit(gates, nullptr, ngates, wires, nqubits);
+		put_f(fval, eval_sampled(C, ufunc, udata, Mode::Target, 1, false, nsamples, rng));
+	});
+}
+
+extern "C" int brickwall_unitary_target_sampling(linear_func ufunc, void* udata, const struct mat4x4 vlist[], const int nlayers,
+	const int nqubits, const int* perms[], const long nsamples, struct rng_state* rng, numeric* fval)
+{
+	return target_call("brickwall_unitary_target_sampling", [&] {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
+		put_f(fval, eval_sampled(C, ufunc, udata, Mode::Target, nqubits / 2, false, nsamples, rng));
+	});
+}
+
+extern "C" int brickwall_unitary_target_gradient_hessian_sampling(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[], const long nsamples,
+	struct rng_state* rng, numeric* fval, struct mat4x4 dvlist[], numeric* hess)
+{
+	return target_call("brickwall_unitary_target_gradient_hessian_sampling", [&] {
+		if (nsamples < 1) { throw Error(-3, "nsamples must be positive"); }
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
+		zc f;
+		eval_hessian(C, ufunc, udata, nqubits / 2, &f, reinterpret_cast<zc*>(dvlist), reinterpret_cast<zc*>(hess), nsamples, rng);
 		std::memcpy(fval, &f, sizeof(zc));
 	});
 }

@@ -59,6 +59,9 @@ struct EvalPlan
 	bool tiles = false;    // tile-pass engine (else one gate per pass)
 	int L = 0, G = 0, per = 1, ng = 0;
 	long N = 0, B = 0;     // 2^L and the column-block width
+	bool sampled = false;  // columns are given states (stochastic trace), not the basis
+	long ncols = 0;        // columns swept (N, or the number of samples)
+	long nblk = 0;         // column blocks (the last one zero-padded when sampled)
 	int nst = 0;           // state arrays of N x B amplitudes in the workspace
 	int passes = 0;        // HBM passes per sweep direction
 	unsigned gtr = 0;      // trace-reduction grid
@@ -76,18 +79,22 @@ struct EvalPlan
 
 // `unitary`: the gates pass gates_near_unitary (the tile-pass engine is used for
 // gradient and HVP sweeps only then); `budget`: bytes for the state arrays
-// (0 = automatic, 85% of the free device memory plus `held`)
+// (0 = automatic, 85% of the free device memory plus `held`); `ncols` > 0: the
+// sweep's columns are that many given states instead of the 2^L basis columns
+// (the column block is then at most the power of two covering them)
 EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, bool fp32, bool project, bool unitary,
-	size_t budget, size_t held, const TileChoice* choice = nullptr);
+	size_t budget, size_t held, const TileChoice* choice = nullptr, long ncols = 0);
 // host image of the evaluation's table region (pl.tab_bytes bytes at `host`)
 void fill_eval_tables(const EvalPlan& pl, const Circuit& C, char* host);
 // enqueue the evaluation on slot s into workspace `work` (pl.bytes).  The table
 // region is at `tab` and the result region at `res` (NULL: their places in the
 // workspace); the table image is uploaded first when host_tab is given.  With
 // `skip`, every kernel returns at once when *skip != 0 at its start (a device
-// flag, read when the kernel runs; tile-pass plans only).
+// flag, read when the kernel runs; tile-pass plans only).  Sampled plans take
+// the input columns X (nblk blocks of N x B states in the sweep's precision,
+// the last block zero-padded) and the trace seeds W = U X (same layout, FP64).
 void enqueue_eval(const Context& dev, Slot& s, const EvalPlan& pl, const Circuit& C, const zc* dU, char* work,
-	char* tab, char* res, const char* host_tab, const int* skip);
+	char* tab, char* res, const char* host_tab, const int* skip, const void* X = nullptr, const zc* W = nullptr);
 
 struct EvalResult
 {

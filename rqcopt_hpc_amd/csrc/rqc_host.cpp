@@ -17,6 +17,7 @@
 #include "rqcopt/brickwall_opt.h"
 #include "rqcopt/circuit_opt.h"
 #include "rqcopt/target.h"
+#include "rqcopt/rng.h"
 
 
 
@@ -354,6 +355,33 @@ static void project_hessian_matrix(int nlayers, const struct mat4x4* vlist, cons
 	}
 }
 
+// gradient (conjugated, derivative convention) in tangent coordinates and the projected Hessian
+// matrix (reference src/target.c:1003-1040 and :1050-1088)
+static void gradient_vector_hessian_matrix(const struct mat4x4 vlist[], int nlayers, const std::vector<struct mat4x4>& dv,
+	const std::vector<numeric>& hess, double* grad_vec, double* H)
+{
+	std::vector<zc> dvc(16 * (size_t)nlayers);
+	for (int i = 0; i < nlayers; i++) {
+		for (int e = 0; e < 16; e++) { dvc[16 * (size_t)i + e] = zc{dv[i].data[e].real(), -dv[i].data[e].imag()}; }
+		rqc::m4_tangent_to_real(Z(&vlist[i]), &dvc[16 * (size_t)i], &grad_vec[16 * i]);
+	}
+	project_hessian_matrix(nlayers, vlist, dvc, reinterpret_cast<const zc*>(hess.data()), H);
+}
+
+extern "C" int brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[], const long nsamples,
+	struct rng_state* rng, numeric* fval, double* grad_vec, double* H)
+{
+	const int m = 16 * nlayers;
+	std::vector<struct mat4x4> dv((size_t)std::max(nlayers, 1));
+	std::vector<numeric> hess((size_t)m * m);
+	const int ret = brickwall_unitary_target_gradient_hessian_sampling(ufunc, udata, vlist, nlayers, nqubits, perms, nsamples, rng,
+		fval, dv.data(), hess.data());
+	if (ret < 0) { return ret; }
+	gradient_vector_hessian_matrix(vlist, nlayers, dv, hess, grad_vec, H);
+	return 0;
+}
+
 extern "C" int brickwall_unitary_target_gradient_vector_hessian_matrix(linear_func ufunc, void* udata,
 	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
 	numeric* fval, double* grad_vec, double* H)
@@ -364,13 +392,7 @@ extern "C" int brickwall_unitary_target_gradient_vector_hessian_matrix(linear_fu
 	std::vector<numeric> hess((size_t)m * m);
 	const int ret = brickwall_unitary_target_gradient_hessian(ufunc, udata, vlist, nlayers, nqubits, perms, fval, dv.data(), hess.data());
 	if (ret < 0) { return ret; }
-	// gradient: conjugated (derivative convention), then tangent coordinates
-	std::vector<zc> dvc(16 * (size_t)nlayers);
-	for (int i = 0; i < nlayers; i++) {
-		for (int e = 0; e < 16; e++) { dvc[16 * (size_t)i + e] = zc{dv[i].data[e].real(), -dv[i].data[e].imag()}; }
-		rqc::m4_tangent_to_real(Z(&vlist[i]), &dvc[16 * (size_t)i], &grad_vec[16 * i]);
-	}
-	project_hessian_matrix(nlayers, vlist, dvc, reinterpret_cast<const zc*>(hess.data()), H);
+	gradient_vector_hessian_matrix(vlist, nlayers, dv, hess, grad_vec, H);
 	return 0;
 }
 
@@ -537,4 +559,60 @@ extern "C" void optimize_quantum_circuit(linear_func ufunc, void* udata, const s
 	int rdata = ngates;
 	riemannian_trust_region_optimize_hvp(qc_f, qc_f_deriv, qc_f_hvp, &fdata, bw_retract, &rdata, ngates * num_tangent_params,
 		reinterpret_cast<const double*>(gates_start), ngates * 32, params, niter, f_iter, reinterpret_cast<double*>(gates_opt));
+}
+
+// ============================================================== brickwall_opt.h: sampled targets
+namespace {
+
+struct BrickwallSamplingData
+{
+	linear_func ufunc;
+	void* udata;
+	const int** perms;
+	struct rng_state* rng;
+	long nsamples;
+	int nlayers;
+	int nqubits;
+};
+
+// wrappers of reference src/brickwall_opt.c:176-215; every call draws fresh samples from rng
+double bws_f(const double* x, void* fdata)
+{
+	auto* d = static_cast<BrickwallSamplingData*>(fdata);
+	numeric f;
+	if (brickwall_unitary_target_sampling(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x), d->nlayers, d->nqubits,
+			d->perms, d->nsamples, d->rng, &f) < 0) {
+		fprintf(stderr, "target function evaluation failed internally\n");
+		return -1;
+	}
+	return f.real();
+}
+
+double bws_f_deriv_hess(const double* RQC_RESTRICT x, void* fdata, double* RQC_RESTRICT grad, double* RQC_RESTRICT hess)
+{
+	auto* d = static_cast<BrickwallSamplingData*>(fdata);
+	numeric f;
+	if (brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x),
+			d->nlayers, d->nqubits, d->perms, d->nsamples, d->rng, &f, grad, hess) < 0) {
+		fprintf(stderr, "target function and derivative evaluation failed internally\n");
+		return -1;
+	}
+	return f.real();
+}
+
+}  // namespace
+
+extern "C" void optimize_brickwall_circuit_hmat_sampling(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist_start[], const int nlayers, const int nqubits, const int* perms[],
+	const long nsamples, struct rng_state* rng,
+	struct rtr_params* params, const int niter, double* f_iter, struct mat4x4 vlist_opt[])
+{
+	BrickwallSamplingData fdata{ufunc, udata, perms, rng, nsamples, nlayers, nqubits};
+	params->g_func = nullptr;
+	params->g_data = nullptr;
+	params->g_iter = nullptr;
+	int rdata = nlayers;
+	riemannian_trust_region_optimize_hmat(bws_f, bws_f_deriv_hess, &fdata, bw_retract, &rdata,
+		nlayers * num_tangent_params, reinterpret_cast<const double*>(vlist_start), nlayers * 32, params, niter, f_iter,
+		reinterpret_cast<double*>(vlist_opt));
 }

@@ -77,6 +77,7 @@ struct Context : Slot
 	DeviceBuffer small;    // accumulators of the statevector-level API
 	DeviceBuffer vecs;     // uploaded statevectors of the statevector-level API
 	DeviceBuffer ubuf;     // U assembled from a host callback
+	DeviceBuffer samples;  // sample states and trace seeds of stochastic-trace evaluations
 
 	double prof_ms[RQC_KERNEL_KINDS] = {0, 0, 0, 0};
 	long prof_n[RQC_KERNEL_KINDS] = {0, 0, 0, 0};

@@ -340,6 +340,51 @@ __global__ __launch_bounds__(kBlock) void k_gate_backward_array(const zc* __rest
 	}
 }
 
+// partial sums of |x|^2 (real parts of the partials; fixed grid, fixed order)
+__global__ __launch_bounds__(kBlock) void k_norm2_partials(const zc* __restrict__ x, long n, zc* __restrict__ partial)
+{
+	zc acc = {0, 0};
+	for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < n; t += (long)gridDim.x * kBlock) {
+		acc.x += x[t].x * x[t].x + x[t].y * x[t].y;
+	}
+	block_sum_store<1>(&acc, partial + blockIdx.x);
+}
+
+// x /= sqrt(s[0].x) unless it is zero
+__global__ __launch_bounds__(kBlock) void k_scale_inv_norm(zc* __restrict__ x, long n, const zc* __restrict__ s)
+{
+	const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+	const double nrm = sqrt(s[0].x);
+	if (t < n && nrm > 0) {
+		const double inv = 1.0 / nrm;
+		x[t] = zc{x[t].x * inv, x[t].y * inv};
+	}
+}
+
+// qubit permutation: output bit q of the index reads input bit src[q]
+struct BitMap
+{
+	int src[32];
+};
+
+__global__ __launch_bounds__(kBlock) void k_transpose_amplitudes(const zc* __restrict__ in, zc* __restrict__ out, BitMap m,
+	int L, long n)
+{
+	const long j = (long)blockIdx.x * kBlock + threadIdx.x;
+	if (j >= n) { return; }
+	long k = 0;
+	for (int q = 0; q < L; q++) {
+		if ((j >> q) & 1) { k |= 1L << m.src[q]; }
+	}
+	out[j] = in[k];
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_float(const zc* __restrict__ x, cx<float>* __restrict__ y, long n)
+{
+	const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+	if (t < n) { y[t] = cx<float>{(float)x[t].x, (float)x[t].y}; }
+}
+
 __global__ __launch_bounds__(kBlock) void k_add(const zc* __restrict__ a, const zc* __restrict__ b, zc* __restrict__ c, long n)
 {
 	const long t = (long)blockIdx.x * kBlock + threadIdx.x;

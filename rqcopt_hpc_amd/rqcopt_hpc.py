@@ -12,15 +12,15 @@ to the GPU once per call and stays resident while the optimiser runs.  The
 brickwall optimiser uses the full Riemannian Hessian (as the reference module
 does); the general-circuit optimiser uses Hessian-vector products
 (reference src/circuit_opt.c).  The sampling variant (nsamples > 0) estimates
-the trace from Haar-random states and is not part of this library yet: it
-raises NotImplementedError.
+the trace from nsamples Haar-random states drawn from the reference's
+generator seeded with `seed`, so the samples are the reference's.
 """
 import ctypes as ct
 from collections.abc import Sequence
 
 import numpy as np
 
-from . import DeviceTarget, _p, _perms, _target_args, _wires, lib, rtr_default_params
+from . import DeviceTarget, Rng, _p, _perms, _target_args, _wires, lib, rtr_default_params
 
 
 def _check_u(U, nqubits, name):
@@ -137,8 +137,6 @@ def optimize_brickwall_circuit(L, U, Vlist_start, perms, niter, nsamples=0, seed
             P[i, j] = _read_int(perms[i][j], f"perms[{i}][{j}]")
         if sorted(P[i].tolist()) != list(range(L)):
             raise ValueError(f"'perms[{i}]' is not a valid permutation")
-    if nsamples > 0:
-        raise NotImplementedError("the sampling variant (nsamples > 0) is not part of this library yet")
     target = DeviceTarget.dense(U)
     try:
         params = rtr_default_params(16 * nlayers)
@@ -146,8 +144,17 @@ def optimize_brickwall_circuit(L, U, Vlist_start, perms, niter, nsamples=0, seed
         vopt = np.empty_like(V)
         fp, ud = _target_args(target)
         prm, rows = _perms(P)
-        lib().optimize_brickwall_circuit_hmat(fp, ud, _p(V), nlayers, L, rows, ct.byref(params), niter, _p(f_iter),
-                                              _p(vopt))
+        if nsamples == 0:
+            lib().optimize_brickwall_circuit_hmat(fp, ud, _p(V), nlayers, L, rows, ct.byref(params), niter, _p(f_iter),
+                                                  _p(vopt))
+        else:
+            rng = Rng(seed)
+            lib().optimize_brickwall_circuit_hmat_sampling.argtypes = [
+                ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_int, ct.c_void_p, ct.c_long, ct.c_void_p,
+                ct.c_void_p, ct.c_int, ct.c_void_p, ct.c_void_p]
+            lib().optimize_brickwall_circuit_hmat_sampling(fp, ud, _p(V), nlayers, L, ct.cast(rows, ct.c_void_p), nsamples,
+                                                           ct.addressof(rng.s), ct.addressof(params), niter, _p(f_iter),
+                                                           _p(vopt))
     finally:
         target.close()
     return vopt, f_iter

@@ -48,6 +48,9 @@ FIXTURES = [
     "test_real_to_tangent_cplx",
     "test_multiply_cplx",
     "test_transpose_statevector_cplx",
+    "test_brickwall_unitary_target_sampling_cplx",
+    "test_brickwall_unitary_target_gradient_hessian_sampling_cplx",
+    "test_brickwall_unitary_target_gradient_vector_hessian_matrix_sampling_cplx",
 ]
 
 

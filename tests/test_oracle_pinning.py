@@ -566,3 +566,51 @@ def test_ref_placeholder_and_backward_array_bitwise():
         R.apply_gate_backward_array(G.ctypes.data_as(ct.c_void_p), i, j, ct.byref(SVA(arr.ctypes.data, L, 16)),
                                     ct.byref(SV(dout.ctypes.data, L)), dg.ctypes.data_as(ct.c_void_p))
         assert np.max(np.abs(dg - O.apply_gate_backward_array(i, j, arr, dout))) <= 1e-14 * np.max(np.abs(dg))
+
+
+# ==================================================== statevector utilities and sampled targets
+def test_fixture_transpose_and_haar_states():
+    d = fixture("test_transpose_statevector_cplx")
+    assert np.array_equal(O.transpose_statevector(cplx(d["psi"]), d["perm"]), cplx(d["chi"]))
+    # a Haar state is the generator's next 2^L crandn values, normalised; the generator continues after them
+    r1, r2 = O.Rng(17), O.Rng(17)
+    x = O.haar_random_statevector(5, r1)
+    z = np.array([r2.crandn() for _ in range(32)])
+    assert np.max(np.abs(x - z / np.linalg.norm(z))) < 1e-15 and abs(np.linalg.norm(x) - 1) < 1e-15
+    assert r1.uint64() == r2.uint64()
+
+
+def test_sampled_targets_reference_checks():
+    """The reference's own checks (test/test_target.c:563-625, 1292-1390, 1486-1560): 10000 samples
+    approach the exact f, gradients and Hessian; the Hessian is symmetric."""
+    d = fixture("test_brickwall_unitary_target_sampling_cplx")
+    V, P, U = stack(d, "V", 3), perms_of(d, 3), cplx(d["U"])
+    Uo = O.Ufunc.dense(U)
+    f = O.brickwall_unitary_target_sampling(Uo, V, P, 8, 10000, O.Rng(677))
+    f_ref = O.brickwall_unitary_target(Uo, V, P, 8)
+    assert abs(f - f_ref) / abs(f_ref) < 0.2
+    d = fixture("test_brickwall_unitary_target_gradient_hessian_sampling_cplx")
+    V, P, U = stack(d, "V", 3), perms_of(d, 3), cplx(d["U"])
+    Uo = O.Ufunc.dense(U)
+    f, dv, h = O.brickwall_unitary_target_gradient_hessian_sampling(Uo, V, P, 6, 10000, O.Rng(823))
+    f_ref, dv_ref, h_ref = O.brickwall_unitary_target_gradient_hessian(Uo, V, P, 6)
+    assert np.max(np.abs(h - h.T)) < 1e-12
+    assert abs(f - f_ref) / abs(f_ref) < 0.2 and np.max(np.abs(dv - dv_ref)) < 0.2 * max(1, np.max(np.abs(dv_ref)))
+    assert np.max(np.abs(h - h_ref)) < 0.4 * max(1, np.max(np.abs(h_ref)))
+
+
+def test_sampled_estimate_is_the_scaled_sum_of_single_samples():
+    """Scale 2^L / nsamples and the generator stream: s samples at once equal the mean of s
+    single-sample estimates drawn in turn from the same generator."""
+    rng = np.random.default_rng(70)
+    L, nl = 6, 2
+    U = np.linalg.qr(crandn(rng, (1 << L, 1 << L)))[0]
+    Uo = O.Ufunc.dense(U)
+    V = np.stack([haar_unitary(rng) for _ in range(nl)])
+    P = random_perms(rng, L, nl)
+    f4, dv4, h4 = O.brickwall_unitary_target_gradient_hessian_sampling(Uo, V, P, L, 4, O.Rng(9))
+    r = O.Rng(9)
+    parts = [O.brickwall_unitary_target_gradient_hessian_sampling(Uo, V, P, L, 1, r) for _ in range(4)]
+    assert abs(f4 - np.mean([p[0] for p in parts])) < 1e-12
+    assert np.max(np.abs(dv4 - np.mean([p[1] for p in parts], 0))) < 1e-12
+    assert np.max(np.abs(h4 - np.mean([p[2] for p in parts], 0))) < 1e-12
