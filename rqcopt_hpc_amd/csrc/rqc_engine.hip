@@ -514,6 +514,10 @@ static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC,
 		}
 		for (int w = 0; w < kTileWaves; w++) { g.wv[w] = quad_pos(w * qpw, 0, lp[j], lq[j], a); }
 		for (int i = 0; i < 3; i++) { g.gbit[i] = (16 << i < qpw) ? quad_pos(16 << i, 0, lp[j], lq[j], a) : 0; }
+		// the kernels select among u, w, q4 by index bits (XOR-affine terms)
+		for (const int* v : {g.u, g.w, g.q4}) {
+			if (v[3] != (v[0] ^ v[1] ^ v[2])) { throw Error(-4, "tile position terms are not XOR-affine"); }
+		}
 	}
 	return a;
 }

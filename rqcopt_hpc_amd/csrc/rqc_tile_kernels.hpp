@@ -112,6 +112,36 @@ __device__ __forceinline__ T a_operand(const cx<T>* X, const cx<T>* Y, int kk)
 	return v * sign * keep;
 }
 
+// Per-lane form of a_operand, fixed for the kernel: element aoff[kk] (in T
+// units, from the start of the gate's matrices X, Y) times afac[kk] (the sign
+// of -Im above the diagonal, 0 in the zero block).  A gate's operand is then
+// one LDS read and one multiply; X of the operand pair sits `xoff` T after
+// the gate's first matrix.
+template <typename T, bool DUAL>
+struct AOperandMap
+{
+	int aoff[4];
+	T afac[4];
+};
+
+template <typename T, bool DUAL>
+__device__ __forceinline__ AOperandMap<T, DUAL> a_operand_map()
+{
+	const int lane = threadIdx.x & 63;
+	const int i = lane & 15, vo = i >> 2, ro = i & 3, ri = lane >> 4;
+	const int so = vo >> 1, po = vo & 1;
+	AOperandMap<T, DUAL> m;
+#pragma unroll
+	for (int kk = 0; kk < 4; kk++) {
+		const int si = kk >> 1, pi = kk & 1;
+		const int blk = (DUAL && so != si) ? 32 : 0;   // Y of [[X, 0], [Y, X]]
+		m.aoff[kk] = blk + 2 * (4 * ro + ri) + (po == pi ? 0 : 1);
+		const int keep = (so == si) | (DUAL & (so == 1));
+		m.afac[kk] = (T)((1 - 2 * (po < pi)) * keep);
+	}
+	return m;
+}
+
 // copy the pass's gate matrices (those listed in `which`) into LDS
 template <typename T>
 __device__ __forceinline__ void load_pass_mats(const cx<T>* __restrict__ mats, int x0, int ng, const int* which, int nw,
@@ -216,15 +246,24 @@ __device__ __forceinline__ int group_term(const GateTerms& t, int i)
 	return ((i & 1) ? t.gbit[0] : 0) ^ ((i & 2) ? t.gbit[1] : 0) ^ ((i & 4) ? t.gbit[2] : 0);
 }
 
+// v[i] for a per-lane i < 4 of four wave-uniform terms that are XOR-affine in
+// i (v[3] = v[0] ^ v[1] ^ v[2], checked when the terms are made): masks of the
+// two index bits select the differences, no per-value selects
+__device__ __forceinline__ int sel4_affine(const int (&v)[4], int i)
+{
+	const int m1 = -(i & 1), m2 = -((i >> 1) & 1);
+	return v[0] ^ ((v[1] ^ v[0]) & m1) ^ ((v[2] ^ v[0]) & m2);
+}
+
 __device__ __forceinline__ GateLanes gate_lanes(const GateTerms& t)
 {
 	const int lane = threadIdx.x & 63;
 	GateLanes g;
-	const int base = sel4(t.q4, lane & 3) ^ sel4(t.w, (lane >> 2) & 3);
+	const int base = sel4_affine(t.q4, lane & 3) ^ sel4_affine(t.w, (lane >> 2) & 3);
 #pragma unroll
 	for (int v = 0; v < 4; v++) { g.ew[v] = base ^ t.u[v]; }
-	g.ea = base ^ sel4(t.u, lane >> 4);
-	const int bk = sel4(t.q4, lane >> 4) ^ sel4(t.u, ((lane & 15) >> 1) & 3);
+	g.ea = base ^ sel4_affine(t.u, lane >> 4);
+	const int bk = sel4_affine(t.q4, lane >> 4) ^ sel4_affine(t.u, ((lane & 15) >> 1) & 3);
 #pragma unroll
 	for (int k = 0; k < 4; k++) { g.eo[k] = bk ^ t.w[k]; }
 	return g;
@@ -431,6 +470,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_forward(
 	cx<T>* lm = reinterpret_cast<cx<T>*>(lds_raw + 2 * NS * sbytes);
 	const int which[2] = {kMatG, kMatZ};
 	load_pass_mats(mats, P.x0, P.ng, which, DUAL ? 2 : 1, lm);
+	const AOperandMap<T, DUAL> amap = a_operand_map<T, DUAL>();
 	auto prefetch = [&](long tile, int b) {
 		const long rbase = deposit_bits(tile >> P.lognc, P.rmask);
 		const long c0 = (tile & cmask) << P.logC;
@@ -464,10 +504,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_forward(
 			const GateTerms gt = gterms[P.x0 + j];
 			const GateLanes gl = gate_lanes(gt);
 			const int ewv = sel4(gt.wv, wave);
-			const cx<T>* mj = lm + j * (DUAL ? 2 : 1) * 16;
+			const T* mj = reinterpret_cast<const T*>(lm + j * (DUAL ? 2 : 1) * 16);
 			T a[4];
 #pragma unroll
-			for (int kk = 0; kk < 4; kk++) { a[kk] = a_operand<T, DUAL>(mj, mj + 16, kk); }
+			for (int kk = 0; kk < 4; kk++) { a[kk] = mj[amap.aoff[kk]] * amap.afac[kk]; }
 			for (int g = 0; g < qpw; g += GS * gq) {
 				int eg[GS], eg2[GS];
 #pragma unroll
@@ -538,6 +578,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 	const int which[4] = {kMatGinv, DUAL ? kMatW : kMatGT, kMatGT, kMatZT};
 	const int which_plain[2] = {kMatGinv, kMatGT};
 	load_pass_mats(mats, P.x0, P.ng, DUAL ? which : which_plain, NM, lm);
+	const AOperandMap<T, DUAL> amap = a_operand_map<T, DUAL>();
 	auto prefetch = [&](long tile, int b) {
 		const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
 		tile_issue(state(b, 0), psi_out, base, io);
@@ -583,12 +624,12 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 			const GateLanes gl = gate_lanes(gt);
 			const int ewv = sel4(gt.wv, wave);
 			// recovery: [[G^-1, 0], [-G^-1 Z G^-1, G^-1]]; adjoint: [[G^T, 0], [Z^T, G^T]]
-			const cx<T>* mj = lm + J * NM * 16;
+			const T* mj = reinterpret_cast<const T*>(lm + J * NM * 16);
 			T au[4], aa[4];
 #pragma unroll
 			for (int kk = 0; kk < 4; kk++) {
-				au[kk] = a_operand<T, DUAL>(mj, mj + 16, kk);
-				aa[kk] = a_operand<T, DUAL>(mj + (DUAL ? 32 : 16), mj + 48, kk);
+				au[kk] = mj[amap.aoff[kk]] * amap.afac[kk];
+				aa[kk] = mj[amap.aoff[kk] + (DUAL ? 64 : 32)] * amap.afac[kk];
 			}
 			const bool adjoint = (J > 0) || !(flags & 8);
 			for (int g = 0; g < qpw; g += GS * gq) {
