@@ -117,4 +117,39 @@ int brickwall_unitary_target_gradient_vector_hessian_matrix(linear_func ufunc, v
 	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
 	numeric* fval, double* grad_vec, double* H);
 
+/* Block-encoding targets f = || P^dagger W P - H ||_F^2 / 2 (reference
+   src/target.h:97-126, src/target.c:1100-1720): W is the brickwall circuit,
+   P embeds a state of nqubits/2 qubits on the odd amplitude-index bits (bit i
+   to bit 2i + 1, the even bits |0>), and hfunc applies H to states of
+   nqubits/2 qubits (rqc_device_ufunc with a device target of nqubits/2
+   qubits keeps H on the GPU).  nqubits must be even. */
+
+/* reference src/target.c:1100-1190 */
+int brickwall_blockenc_target(linear_func hfunc, void* hdata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	double* fval);
+
+/* reference src/target.c:1202-1330 */
+int brickwall_blockenc_target_and_gradient(linear_func hfunc, void* hdata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	double* fval, struct mat4x4 dvlist[]);
+
+/* reference src/target.c:1341-1365 */
+int brickwall_blockenc_target_and_gradient_vector(linear_func hfunc, void* hdata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	double* fval, double* grad_vec);
+
+/* reference src/target.c:1380-1620: hess1 (second derivative in the gates with
+   the residual held fixed, complex symmetric) and hess2 (in the gates and
+   their conjugates, Hermitian), both (16 nlayers)^2 numerics in the layout of
+   brickwall_unitary_target_gradient_hessian */
+int brickwall_blockenc_target_gradient_hessian(linear_func hfunc, void* hdata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	double* fval, struct mat4x4 dvlist[], numeric* hess1, numeric* hess2);
+
+/* reference src/target.c:1631-1720 */
+int brickwall_blockenc_target_gradient_vector_hessian_matrix(linear_func hfunc, void* hdata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[],
+	double* fval, double* grad_vec, double* H);
+
 RQC_EXTERN_C_END

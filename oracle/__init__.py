@@ -480,6 +480,59 @@ def brickwall_unitary_target_gradient_vector_hessian_matrix(uf, vlist, perms, L)
     return f[0], g, H
 
 
+# ---------------------------------------------------------------- block-encoding targets
+def _blockenc_call(name, hf, vlist, perms, L, *outs):
+    vlist, perms = _cplx(vlist), _i32(perms)
+    f = np.empty(1)
+    fp, ud = hf.args()
+    r = getattr(lib(), name)(fp, ud, _p(vlist), _i(len(vlist)), _i(L), _p(perms), _p(f), *[_p(o) for o in outs])
+    assert r == 0, (name, r)
+    return float(f[0])
+
+
+def brickwall_blockenc_target(hf, vlist, perms, L):
+    """f = ||P^dagger W P - H||_F^2 / 2; hf applies H to L/2-qubit states."""
+    return _blockenc_call("orc_brickwall_blockenc_target", hf, vlist, perms, L)
+
+
+def brickwall_blockenc_target_and_gradient(hf, vlist, perms, L):
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    return _blockenc_call("orc_brickwall_blockenc_target_and_gradient", hf, vlist, perms, L, dv), dv
+
+
+def brickwall_blockenc_target_and_gradient_vector(hf, vlist, perms, L):
+    g = np.empty(16 * len(vlist))
+    return _blockenc_call("orc_brickwall_blockenc_target_and_gradient_vector", hf, vlist, perms, L, g), g
+
+
+def brickwall_blockenc_target_gradient_hessian(hf, vlist, perms, L):
+    m = 16 * len(vlist)
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    h1 = np.empty((m, m), np.complex128)
+    h2 = np.empty((m, m), np.complex128)
+    f = _blockenc_call("orc_brickwall_blockenc_target_gradient_hessian", hf, vlist, perms, L, dv, h1, h2)
+    return f, dv, h1, h2
+
+
+def brickwall_blockenc_target_gradient_vector_hessian_matrix(hf, vlist, perms, L):
+    m = 16 * len(vlist)
+    g = np.empty(m)
+    H = np.empty((m, m))
+    f = _blockenc_call("orc_brickwall_blockenc_target_gradient_vector_hessian_matrix", hf, vlist, perms, L, g, H)
+    return f, g, H
+
+
+def apply_brickwall_unitary_gate_placeholder(vlist, perms, l, cache):
+    """Layer l's placeholder states carried through the later gates (2^L x 16)."""
+    vlist, perms, cache = _cplx(vlist), _i32(perms), _cplx(cache)
+    L = perms.shape[1]
+    out = np.empty((1 << L, 16), np.complex128)
+    r = lib().orc_apply_brickwall_unitary_gate_placeholder(_p(vlist), _i(len(vlist)), _p(perms), _i(L), _i(l), _p(cache),
+                                                          _p(out))
+    assert r == 0
+    return out
+
+
 def truncated_cg_hmat(grad, hess, radius, maxiter=None, abstol=1e-8, reltol=1e-6):
     grad = np.ascontiguousarray(grad, np.float64)
     hess = np.ascontiguousarray(hess, np.float64)

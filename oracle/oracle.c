@@ -1070,8 +1070,11 @@ int orc_brickwall_unitary_target_gradient_hessian(orc_linear_func ufunc, void* u
 
 /* reference src/target.c:937-994: upper blocks (i <= j) of the Riemannian Hessian,
    column k of block (i, j) from the tangent basis vector k at V_j; the diagonal
-   blocks also carry the curvature term of the gradient; lower half by symmetry */
-static void project_hessian_matrix(int nlayers, const ocplx* vlist, const ocplx* dvc, const ocplx* hess, double* H)
+   blocks also carry the curvature term of the gradient; lower half by symmetry.
+   hess2 (NULL for the unitary targets) is the Hessian of the conjugated gates,
+   added unconjugated as in reference src/target.c:1665-1680. */
+static void project_hessian_matrix(int nlayers, const ocplx* vlist, const ocplx* dvc, const ocplx* hess,
+	const ocplx* hess2, double* H)
 {
 	const int m = 16 * nlayers;
 	for (int i = 0; i < nlayers; i++) {
@@ -1085,6 +1088,11 @@ static void project_hessian_matrix(int nlayers, const ocplx* vlist, const ocplx*
 					ocplx s = 0;
 					for (int y = 0; y < 16; y++) { s += hess[(size_t)(i*16 + x) * m + j*16 + y] * Z[y]; }
 					G[x] = conj(s);
+					if (hess2 != NULL) {
+						ocplx s2 = 0;
+						for (int y = 0; y < 16; y++) { s2 += hess2[(size_t)(i*16 + x) * m + j*16 + y] * Z[y]; }
+						G[x] += s2;
+					}
 				}
 				if (i == j) {
 					ocplx P[16], gh[16], T[16];
@@ -1116,7 +1124,7 @@ int orc_brickwall_unitary_target_gradient_vector_hessian_matrix(orc_linear_func 
 			for (int e = 0; e < 16; e++) { dv[16*i + e] = conj(dv[16*i + e]); }
 			orc_tangent_to_real(&vlist[16*i], &dv[16*i], &grad_vec[16*i]);
 		}
-		project_hessian_matrix(nlayers, vlist, dv, hess, H);
+		project_hessian_matrix(nlayers, vlist, dv, hess, NULL, H);
 	}
 	free(hess); free(dv);
 	return ret;
@@ -1382,7 +1390,7 @@ int orc_brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(orc_lin
 			for (int e = 0; e < 16; e++) { dv[16*i + e] = conj(dv[16*i + e]); }
 			orc_tangent_to_real(&vlist[16*i], &dv[16*i], &grad_vec[16*i]);
 		}
-		project_hessian_matrix(nlayers, vlist, dv, hess, H);
+		project_hessian_matrix(nlayers, vlist, dv, hess, NULL, H);
 	}
 	free(hess); free(dv);
 	return ret;
@@ -1425,4 +1433,233 @@ void orc_optimize_brickwall_circuit_hmat_sampling(orc_linear_func ufunc, void* u
 	orc_brickwall_unitary_target_sampling(ufunc, udata, (const ocplx*)x, nlayers, L, perms, nsamples, rng, &ff);
 	f_iter[niter] = creal(ff);
 	free(heta); free(eta); free(H); free(grad); free(xn); free(x);
+}
+
+/* ---------------------------------------------------------------------------
+ * Block-encoding targets f = || P^dagger W P - H ||_F^2 / 2 (reference
+ * src/target.c:1100-1720).  P embeds an L/2-qubit state on the odd bits of the
+ * amplitude index (bit i of b goes to bit 2i + 1), the even bits held at |0>;
+ * the Frobenius norm is summed over the 2^(L/2) unit vectors e_b.
+ */
+
+/* reference src/target.c:1146-1155: c = sum_i 2 t_i^2 with t_i = b & 2^i */
+static long blockenc_embed(long b, int half)
+{
+	long c = 0;
+	for (int i = 0; i < half; i++) {
+		const long t = b & (1L << i);
+		c += 2 * t * t;
+	}
+	return c;
+}
+
+/* H e_b into hpsi (2^(L/2) entries) and the embedded unit vector e_{c(b)} into chi */
+static int blockenc_column(orc_linear_func hfunc, void* hdata, int L, long b, ocplx* psi, ocplx* hpsi, ocplx* chi)
+{
+	const int half = L / 2;
+	const long s = 1L << half, n = 1L << L;
+	memset(psi, 0, s * sizeof(ocplx));
+	psi[b] = 1;
+	if (hfunc(half, psi, hdata, hpsi) < 0) { return -2; }
+	memset(chi, 0, n * sizeof(ocplx));
+	chi[blockenc_embed(b, half)] = 1;
+	return 0;
+}
+
+/* residual d_a = (W P e_b)[c(a)] - (H e_b)[a] (reference src/target.c:1162-1180):
+   adds |d_a|^2 / 2 to *f and, if dchi is given, writes the upstream gradient
+   conj(d_a) at c(a) of dchi, zero elsewhere */
+static void blockenc_residual(int L, const ocplx* wchi, const ocplx* hpsi, double* f, ocplx* dchi)
+{
+	const int half = L / 2;
+	const long s = 1L << half, n = 1L << L;
+	if (dchi != NULL) { memset(dchi, 0, n * sizeof(ocplx)); }
+	for (long a = 0; a < s; a++) {
+		const long c = blockenc_embed(a, half);
+		const ocplx d = wchi[c] - hpsi[a];
+		*f += 0.5 * (creal(d) * creal(d) + cimag(d) * cimag(d));
+		if (dchi != NULL) { dchi[c] = conj(d); }
+	}
+}
+
+/* reference src/target.c:1100-1190 */
+int orc_brickwall_blockenc_target(orc_linear_func hfunc, void* hdata, const ocplx* vlist, int nlayers, int L,
+	const int* perms, double* fval)
+{
+	if (L % 2 != 0) { return -3; }
+	const long s = 1L << (L / 2), n = 1L << L;
+	ocplx* psi = xalloc(s * sizeof(ocplx));
+	ocplx* hpsi = xalloc(s * sizeof(ocplx));
+	ocplx* chi = xalloc(n * sizeof(ocplx));
+	ocplx* wchi = xalloc(n * sizeof(ocplx));
+	double f = 0;
+	int ret = 0;
+	for (long b = 0; b < s; b++) {
+		if (blockenc_column(hfunc, hdata, L, b, psi, hpsi, chi) < 0) { ret = -2; break; }
+		orc_apply_brickwall_unitary(vlist, nlayers, perms, L, chi, wchi);
+		blockenc_residual(L, wchi, hpsi, &f, NULL);
+	}
+	*fval = f;
+	free(wchi); free(chi); free(hpsi); free(psi);
+	return ret;
+}
+
+/* reference src/target.c:1202-1330: per unit vector, forward with cache, the
+   residual as upstream gradient, backward; gate gradients summed over e_b */
+int orc_brickwall_blockenc_target_and_gradient(orc_linear_func hfunc, void* hdata, const ocplx* vlist, int nlayers,
+	int L, const int* perms, double* fval, ocplx* dvlist)
+{
+	if (L % 2 != 0) { return -3; }
+	const long s = 1L << (L / 2), n = 1L << L;
+	const int ng = nlayers * (L / 2);
+	ocplx* psi = xalloc(s * sizeof(ocplx));
+	ocplx* hpsi = xalloc(s * sizeof(ocplx));
+	ocplx* chi = xalloc(n * sizeof(ocplx));
+	ocplx* wchi = xalloc(n * sizeof(ocplx));
+	ocplx* cache = xalloc((size_t)ng * n * sizeof(ocplx));
+	ocplx* dvu = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
+	for (int e = 0; e < 16 * nlayers; e++) { dvlist[e] = 0; }
+	double f = 0;
+	int ret = 0;
+	for (long b = 0; b < s; b++) {
+		if (blockenc_column(hfunc, hdata, L, b, psi, hpsi, chi) < 0) { ret = -2; break; }
+		orc_brickwall_unitary_forward(vlist, nlayers, perms, L, chi, cache, wchi);
+		blockenc_residual(L, wchi, hpsi, &f, chi);
+		orc_brickwall_unitary_backward(vlist, nlayers, perms, L, cache, chi, wchi, dvu);
+		for (int e = 0; e < 16 * nlayers; e++) { dvlist[e] += dvu[e]; }
+	}
+	*fval = f;
+	free(dvu); free(cache); free(wchi); free(chi); free(hpsi); free(psi);
+	return ret;
+}
+
+/* reference src/target.c:1341-1365: conjugated gradient in tangent coordinates */
+int orc_brickwall_blockenc_target_and_gradient_vector(orc_linear_func hfunc, void* hdata, const ocplx* vlist,
+	int nlayers, int L, const int* perms, double* fval, double* grad_vec)
+{
+	ocplx* dv = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
+	const int ret = orc_brickwall_blockenc_target_and_gradient(hfunc, hdata, vlist, nlayers, L, perms, fval, dv);
+	if (ret == 0) {
+		for (int i = 0; i < nlayers; i++) {
+			for (int e = 0; e < 16; e++) { dv[16*i + e] = conj(dv[16*i + e]); }
+			orc_tangent_to_real(&vlist[16*i], &dv[16*i], &grad_vec[16*i]);
+		}
+	}
+	free(dv);
+	return ret;
+}
+
+/* reference src/brickwall_circuit.c:264-310: the placeholder states of every
+   gate of layer l (from the cached state before it), each carried through all
+   later gates, summed; psi_out is n x 16 interleaved */
+int orc_apply_brickwall_unitary_gate_placeholder(const ocplx* vlist, int nlayers, const int* perms, int L, int l,
+	const ocplx* cache, ocplx* psi_out)
+{
+	const long n = 1L << L;
+	const int per = L / 2;
+	ocplx* t[2] = { xalloc((size_t)n * 16 * sizeof(ocplx)), xalloc((size_t)n * 16 * sizeof(ocplx)) };
+	memset(psi_out, 0, (size_t)n * 16 * sizeof(ocplx));
+	for (int s2 = 0; s2 < L; s2 += 2) {
+		const int g = l * per + s2 / 2;
+		orc_apply_gate_placeholder(perms[l*L + s2], perms[l*L + s2 + 1], L, &cache[(size_t)g * n], t[0]);
+		int cur = 0;
+		for (int i = l; i < nlayers; i++) {
+			for (int j = 0; j < L; j += 2) {
+				if (i == l && j <= s2) { continue; }
+				orc_apply_gate_to_array(&vlist[16*i], perms[i*L + j], perms[i*L + j + 1], L, 16, t[cur], t[1 - cur]);
+				cur = 1 - cur;
+			}
+		}
+		for (size_t e = 0; e < (size_t)n * 16; e++) { psi_out[e] += t[cur][e]; }
+	}
+	free(t[1]); free(t[0]);
+	return 0;
+}
+
+/* reference src/target.c:1380-1620: hess1 from the backward Hessian pass with the
+   residual as upstream gradient; hess2 = sum_a conj(P^dagger J_l) (P^dagger J_k)
+   of the per-layer placeholder states (lower blocks k <= l computed, the upper
+   ones their conjugate transposes); both summed over the unit vectors e_b */
+int orc_brickwall_blockenc_target_gradient_hessian(orc_linear_func hfunc, void* hdata, const ocplx* vlist, int nlayers,
+	int L, const int* perms, double* fval, ocplx* dvlist, ocplx* hess1, ocplx* hess2)
+{
+	if (L % 2 != 0) { return -3; }
+	const int half = L / 2;
+	const long s = 1L << half, n = 1L << L;
+	const int ng = nlayers * half, m = 16 * nlayers;
+	ocplx* psi = xalloc(s * sizeof(ocplx));
+	ocplx* hpsi = xalloc(s * sizeof(ocplx));
+	ocplx* chi = xalloc(n * sizeof(ocplx));
+	ocplx* wchi = xalloc(n * sizeof(ocplx));
+	ocplx* cache = xalloc((size_t)ng * n * sizeof(ocplx));
+	ocplx* tmp = xalloc((size_t)n * 16 * sizeof(ocplx));
+	ocplx* phi = xalloc((size_t)nlayers * s * 16 * sizeof(ocplx));
+	ocplx* dvu = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
+	ocplx* hu1 = xalloc((size_t)m * m * sizeof(ocplx));
+	ocplx* hu2 = xalloc((size_t)m * m * sizeof(ocplx));
+	for (int e = 0; e < 16 * nlayers; e++) { dvlist[e] = 0; }
+	memset(hess1, 0, (size_t)m * m * sizeof(ocplx));
+	memset(hess2, 0, (size_t)m * m * sizeof(ocplx));
+	double f = 0;
+	int ret = 0;
+	for (long b = 0; b < s; b++) {
+		if (blockenc_column(hfunc, hdata, L, b, psi, hpsi, chi) < 0) { ret = -2; break; }
+		orc_brickwall_unitary_forward(vlist, nlayers, perms, L, chi, cache, wchi);
+		blockenc_residual(L, wchi, hpsi, &f, chi);
+		orc_brickwall_unitary_backward_hessian(vlist, nlayers, perms, L, cache, chi, wchi, dvu, hu1);
+		for (int l = 0; l < nlayers; l++) {
+			orc_apply_brickwall_unitary_gate_placeholder(vlist, nlayers, perms, L, l, cache, tmp);
+			for (long a = 0; a < s; a++) {
+				const long c = blockenc_embed(a, half);
+				for (int j = 0; j < 16; j++) { phi[((size_t)l * s + a) * 16 + j] = tmp[c * 16 + j]; }
+			}
+		}
+		memset(hu2, 0, (size_t)m * m * sizeof(ocplx));
+		for (int l = 0; l < nlayers; l++) {
+			for (int k = 0; k <= l; k++) {
+				for (long a = 0; a < s; a++) {
+					const ocplx* pl = &phi[((size_t)l * s + a) * 16];
+					const ocplx* pk = &phi[((size_t)k * s + a) * 16];
+					for (int i = 0; i < 16; i++) {
+						for (int j = 0; j < 16; j++) { hu2[(size_t)(l*16 + i) * m + k*16 + j] += conj(pl[i]) * pk[j]; }
+					}
+				}
+			}
+		}
+		for (int l = 0; l < nlayers; l++) {
+			for (int k = l + 1; k < nlayers; k++) {
+				for (int i = 0; i < 16; i++) {
+					for (int j = 0; j < 16; j++) {
+						hu2[(size_t)(l*16 + i) * m + k*16 + j] = conj(hu2[(size_t)(k*16 + j) * m + l*16 + i]);
+					}
+				}
+			}
+		}
+		for (int e = 0; e < 16 * nlayers; e++) { dvlist[e] += dvu[e]; }
+		for (size_t e = 0; e < (size_t)m * m; e++) { hess1[e] += hu1[e]; hess2[e] += hu2[e]; }
+	}
+	*fval = f;
+	free(hu2); free(hu1); free(dvu); free(phi); free(tmp); free(cache); free(wchi); free(chi); free(hpsi); free(psi);
+	return ret;
+}
+
+/* reference src/target.c:1631-1720 */
+int orc_brickwall_blockenc_target_gradient_vector_hessian_matrix(orc_linear_func hfunc, void* hdata, const ocplx* vlist,
+	int nlayers, int L, const int* perms, double* fval, double* grad_vec, double* H)
+{
+	const int m = 16 * nlayers;
+	ocplx* dv = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
+	ocplx* hess1 = xalloc((size_t)m * m * sizeof(ocplx));
+	ocplx* hess2 = xalloc((size_t)m * m * sizeof(ocplx));
+	const int ret = orc_brickwall_blockenc_target_gradient_hessian(hfunc, hdata, vlist, nlayers, L, perms, fval, dv, hess1,
+		hess2);
+	if (ret == 0) {
+		for (int i = 0; i < nlayers; i++) {
+			for (int e = 0; e < 16; e++) { dv[16*i + e] = conj(dv[16*i + e]); }
+			orc_tangent_to_real(&vlist[16*i], &dv[16*i], &grad_vec[16*i]);
+		}
+		project_hessian_matrix(nlayers, vlist, dv, hess1, hess2, H);
+	}
+	free(hess2); free(hess1); free(dv);
+	return ret;
 }

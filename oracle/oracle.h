@@ -168,4 +168,21 @@ double   orc_randu(struct orc_rng* rng);
 double   orc_randn(struct orc_rng* rng);
 ocplx    orc_crandn(struct orc_rng* rng);
 
+/* ---- block-encoding targets f = || P^dagger W P - H ||_F^2 / 2: reference
+        src/target.c:1100-1720, src/brickwall_circuit.c:264-310; hfunc acts on
+        L/2 qubits ---- */
+int orc_brickwall_blockenc_target(orc_linear_func hfunc, void* hdata, const ocplx* vlist, int nlayers, int L,
+	const int* perms, double* fval);
+int orc_brickwall_blockenc_target_and_gradient(orc_linear_func hfunc, void* hdata, const ocplx* vlist, int nlayers,
+	int L, const int* perms, double* fval, ocplx* dvlist);
+int orc_brickwall_blockenc_target_and_gradient_vector(orc_linear_func hfunc, void* hdata, const ocplx* vlist,
+	int nlayers, int L, const int* perms, double* fval, double* grad_vec);
+/* psi_out: 2^L x 16 interleaved */
+int orc_apply_brickwall_unitary_gate_placeholder(const ocplx* vlist, int nlayers, const int* perms, int L, int l,
+	const ocplx* cache, ocplx* psi_out);
+int orc_brickwall_blockenc_target_gradient_hessian(orc_linear_func hfunc, void* hdata, const ocplx* vlist, int nlayers,
+	int L, const int* perms, double* fval, ocplx* dvlist, ocplx* hess1, ocplx* hess2);
+int orc_brickwall_blockenc_target_gradient_vector_hessian_matrix(orc_linear_func hfunc, void* hdata, const ocplx* vlist,
+	int nlayers, int L, const int* perms, double* fval, double* grad_vec, double* H);
+
 #endif

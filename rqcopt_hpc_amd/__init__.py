@@ -104,6 +104,9 @@ EXPORTED = [
     "brickwall_unitary_target_gradient_vector_hessian_matrix", "circuit_unitary_target_sampling",
     "brickwall_unitary_target_sampling", "brickwall_unitary_target_gradient_hessian_sampling",
     "brickwall_unitary_target_gradient_vector_hessian_matrix_sampling",
+    "brickwall_blockenc_target", "brickwall_blockenc_target_and_gradient",
+    "brickwall_blockenc_target_and_gradient_vector", "brickwall_blockenc_target_gradient_hessian",
+    "brickwall_blockenc_target_gradient_vector_hessian_matrix",
     # matrix.h
     "zero_matrix", "identity_matrix", "scale_matrix", "conjugate_matrix", "add_matrix", "sub_matrix",
     "add_matrices", "sub_matrices", "transpose", "adjoint", "symm", "antisymm", "real_to_antisymm",
@@ -622,6 +625,50 @@ def brickwall_unitary_target_gradient_vector_hessian_matrix(target, vlist, perms
            lib().brickwall_unitary_target_gradient_vector_hessian_matrix(fp, ud, _p(vlist), len(vlist), L, rows, _p(f), _p(g),
                                                                         _p(H)))
     return f[0], g, H
+
+
+# ---------------------------------------------------------------- block-encoding targets
+def _blockenc(name, hop, vlist, perms, nqubits, *outs):
+    """Call a block-encoding entry point: hop applies H to states of nqubits/2 qubits."""
+    vlist, f = _gates(vlist), np.zeros(1)
+    perms, rows = _perms(perms)
+    L = perms.shape[1] if nqubits is None else nqubits
+    fp, ud = _target_args(hop)
+    _check(name, getattr(lib(), name)(fp, ud, _p(vlist), len(vlist), L, rows, _p(f), *[_p(o) for o in outs]))
+    return float(f[0])
+
+
+def brickwall_blockenc_target(hop, vlist, perms, nqubits=None):
+    """f = ||P^dagger W P - H||_F^2 / 2 of the brickwall circuit W (reference src/target.c:1100-1190)."""
+    return _blockenc("brickwall_blockenc_target", hop, vlist, perms, nqubits)
+
+
+def brickwall_blockenc_target_and_gradient(hop, vlist, perms, nqubits=None):
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    return _blockenc("brickwall_blockenc_target_and_gradient", hop, vlist, perms, nqubits, dv), dv
+
+
+def brickwall_blockenc_target_and_gradient_vector(hop, vlist, perms, nqubits=None):
+    g = np.empty(16 * len(vlist))
+    return _blockenc("brickwall_blockenc_target_and_gradient_vector", hop, vlist, perms, nqubits, g), g
+
+
+def brickwall_blockenc_target_gradient_hessian(hop, vlist, perms, nqubits=None):
+    """f, dvlist, hess1 (complex symmetric) and hess2 (Hermitian), each 16 nlayers square."""
+    m = 16 * len(vlist)
+    dv = np.empty((len(vlist), 4, 4), np.complex128)
+    h1 = np.empty((m, m), np.complex128)
+    h2 = np.empty((m, m), np.complex128)
+    f = _blockenc("brickwall_blockenc_target_gradient_hessian", hop, vlist, perms, nqubits, dv, h1, h2)
+    return f, dv, h1, h2
+
+
+def brickwall_blockenc_target_gradient_vector_hessian_matrix(hop, vlist, perms, nqubits=None):
+    """f, the projected gradient and the Riemannian Hessian matrix (real, symmetric)."""
+    m = 16 * len(vlist)
+    g = np.empty(m)
+    H = np.empty((m, m))
+    return _blockenc("brickwall_blockenc_target_gradient_vector_hessian_matrix", hop, vlist, perms, nqubits, g, H), g, H
 
 
 def circuit_unitary_target_sampling(target, gates, wires, nqubits, nsamples, rng):

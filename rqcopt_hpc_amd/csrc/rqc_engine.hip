@@ -256,7 +256,7 @@ static long choose_block(long N, int nstates, size_t elem, size_t budget, size_t
 // acc[0] = f, acc[1 + k*32 + e]: e < 16 gradient of gate k, e >= 16 its HVP
 template <typename T>
 static void run_gates(Slot& s, const EvalPlan& pl, const Circuit& C, const zc* dU, zc* d_acc, zc* d_partial, cx<T>* S,
-	const int* skip, const cx<T>* X, const zc* W)
+	const int* skip, const cx<T>* X, const zc* W, const BlockEnc* be)
 {
 	const Mode mode = pl.mode;
 	const int L = C.L, G = C.ngates;
@@ -283,7 +283,7 @@ static void run_gates(Slot& s, const EvalPlan& pl, const Circuit& C, const zc* d
 	for (long blk = 0; blk < pl.nblk; blk++) {
 		const long b0 = blk * B;
 		// the block's input columns and the trace's seed columns: basis columns and U, or samples and U X
-		const zc* seed = pl.sampled ? W + (size_t)blk * sz : dU;
+		const zc* seed = (be != nullptr) ? nullptr : pl.sampled ? W + (size_t)blk * sz : dU;
 		const long ldu = pl.sampled ? B : N, u0 = pl.sampled ? 0 : b0;
 		if (pl.sampled) {
 			RQC_HIP(hipMemcpyAsync(psi(0), X + (size_t)blk * sz, sz * sizeof(cx<T>), hipMemcpyDeviceToDevice, s.stream));
@@ -306,7 +306,12 @@ static void run_gates(Slot& s, const EvalPlan& pl, const Circuit& C, const zc* d
 			}
 		}
 		cx<T>* psiG = (mode == Mode::Target) ? psi(G % 2) : psi(G);
-		{
+		if (be != nullptr) {
+			LaunchTimer lt(s, RQC_KERNEL_TARGET, (mode == Mode::Target ? 1 : 2) * ab);
+			k_blockenc_residual<T><<<gtr, kBlock, 0, s.stream>>>(be->H, L / 2, b0, psiG,
+				mode == Mode::Target ? nullptr : phi[0], hvp ? dpsi(G) : nullptr, hvp ? be->J : nullptr, B, logB, N * B,
+				d_partial, skip);
+		} else {
 			LaunchTimer lt(s, RQC_KERNEL_TARGET, (double)sz * sizeof(zc) + (mode == Mode::Target ? 1 : 2) * ab);
 			k_target_trace<T><<<gtr, kBlock, 0, s.stream>>>(seed, ldu, u0, psiG, mode == Mode::Target ? nullptr : phi[0],
 				B, logB, N * B, d_partial, skip);
@@ -769,7 +774,7 @@ void fill_eval_tables(const EvalPlan& pl, const Circuit& C, char* host)
 // tile-pass engine: several gates per HBM pass through LDS tiles
 template <typename T>
 static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* d_partial, const cx<T>* mats,
-	const GateTerms* gterms, cx<T>* S, const int* skip, const cx<T>* X, const zc* W)
+	const GateTerms* gterms, cx<T>* S, const int* skip, const cx<T>* X, const zc* W, const BlockEnc* be)
 {
 	const Mode mode = pl.mode;
 	const TilePlan& tp = *pl.tp;
@@ -798,7 +803,7 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	for (long blk = 0; blk < pl.nblk; blk++) {
 		const long b0 = blk * B;
 		// the block's input columns and the trace's seed columns: basis columns and U, or samples and U X
-		const zc* seed = pl.sampled ? W + (size_t)blk * sz : dU;
+		const zc* seed = (be != nullptr) ? nullptr : pl.sampled ? W + (size_t)blk * sz : dU;
 		const long ldu = pl.sampled ? B : N, u0 = pl.sampled ? 0 : b0;
 		for (int p = 0; p < P; p++) {
 			// pass 0 starts from the identity block (flags 1) or the sample block, with dpsi = 0 (flags 2)
@@ -811,7 +816,12 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 				dual ? DPS(p + 1) : nullptr, tp.args[p], mats, gterms, B, b0, ntiles, flags, skip);
 		}
 		cx<T>* psiG = (mode == Mode::Target) ? buf[(P - 1) % 2] : PS(P);
-		{
+		if (be != nullptr) {
+			LaunchTimer lt(s, RQC_KERNEL_TARGET, (mode == Mode::Target ? 1 : 2) * ab);
+			k_blockenc_residual<T><<<gtr, kBlock, 0, s.stream>>>(be->H, pl.L / 2, b0, psiG,
+				mode == Mode::Target ? nullptr : phi[0], dual ? DPS(P) : nullptr, dual ? be->J : nullptr, B, logB, N * B,
+				tpart, skip);
+		} else {
 			LaunchTimer lt(s, RQC_KERNEL_TARGET, (double)sz * sizeof(zc) + (mode == Mode::Target ? 1 : 2) * ab);
 			k_target_trace<T><<<gtr, kBlock, 0, s.stream>>>(seed, ldu, u0, psiG, mode == Mode::Target ? nullptr : phi[0],
 				B, logB, N * B, tpart, skip);
@@ -874,11 +884,16 @@ __global__ void k_finalize(const zc* __restrict__ acc, int ngroups, int per, con
 }
 
 void enqueue_eval(const Context& dev, Slot& s, const EvalPlan& pl, const Circuit& C, const zc* dU, char* work,
-	char* tab, char* res_dev, const char* host_tab, const int* skip, const void* X, const zc* W)
+	char* tab, char* res_dev, const char* host_tab, const int* skip, const void* X, const zc* W, const BlockEnc* be)
 {
 	(void)dev;
 	if (skip != nullptr && !pl.tiles) { throw Error(-3, "skip flags need the tile-pass engine"); }
-	if (pl.sampled && (X == nullptr || W == nullptr)) { throw Error(-3, "a sampled evaluation needs its sample and seed blocks"); }
+	if (pl.sampled && (X == nullptr || (W == nullptr && be == nullptr))) {
+		throw Error(-3, "a sampled evaluation needs its sample and seed blocks");
+	}
+	if (be != nullptr && (!pl.sampled || be->H == nullptr || pl.ncols != (1L << (pl.L / 2)) || pl.L % 2 != 0)) {
+		throw Error(-3, "a block-encoding evaluation needs the 2^(L/2) embedded columns and H");
+	}
 	zc* acc = reinterpret_cast<zc*>(work + pl.off_acc);
 	zc* part = reinterpret_cast<zc*>(work + pl.off_part);
 	if (tab == nullptr) { tab = work + pl.off_tab; }
@@ -898,13 +913,13 @@ void enqueue_eval(const Context& dev, Slot& s, const EvalPlan& pl, const Circuit
 	if (pl.fp32) {
 		cx<float>* S = reinterpret_cast<cx<float>*>(work + pl.off_states);
 		const cx<float>* Xs = static_cast<const cx<float>*>(X);
-		if (pl.tiles) { run_tiles<float>(s, pl, dU, acc, part, reinterpret_cast<const cx<float>*>(tab), gterms, S, skip, Xs, W); }
-		else { run_gates<float>(s, pl, C, dU, acc, part, S, skip, Xs, W); }
+		if (pl.tiles) { run_tiles<float>(s, pl, dU, acc, part, reinterpret_cast<const cx<float>*>(tab), gterms, S, skip, Xs, W, be); }
+		else { run_gates<float>(s, pl, C, dU, acc, part, S, skip, Xs, W, be); }
 	} else {
 		cx<double>* S = reinterpret_cast<cx<double>*>(work + pl.off_states);
 		const cx<double>* Xs = static_cast<const cx<double>*>(X);
-		if (pl.tiles) { run_tiles<double>(s, pl, dU, acc, part, reinterpret_cast<const cx<double>*>(tab), gterms, S, skip, Xs, W); }
-		else { run_gates<double>(s, pl, C, dU, acc, part, S, skip, Xs, W); }
+		if (pl.tiles) { run_tiles<double>(s, pl, dU, acc, part, reinterpret_cast<const cx<double>*>(tab), gterms, S, skip, Xs, W, be); }
+		else { run_gates<double>(s, pl, C, dU, acc, part, S, skip, Xs, W, be); }
 	}
 	LaunchTimer lt(s, RQC_KERNEL_OTHER);
 	k_finalize<<<std::max(1, (pl.ng + 63) / 64), 64, 0, s.stream>>>(acc, pl.ng, pl.per, vin, vin + 16 * (size_t)pl.ng, res,
@@ -1948,6 +1963,88 @@ static void scale_result(EvalResult& R, double a)
 	for (auto& x : R.hvp) { x *= a; }
 }
 
+// ============================================================== block-encoding targets
+// f = ||P^dagger W P - H||_F^2 / 2 (reference src/target.c:1100-1720).  The
+// sweeps run over the 2^(L/2) embedded basis columns P e_b as a sampled plan,
+// with the residual kernel in place of the trace.  H is assembled from the
+// caller's hfunc on the unit vectors (2^(L/2) calls, as the reference makes per
+// evaluation) or read in place from a device-resident target of L/2 qubits.
+struct BlockEncInputs
+{
+	const void* X = nullptr;  // embedded columns in the plan's blocks, sweep precision
+	const zc* H = nullptr;    // 2^(L/2) x 2^(L/2), row-major
+	zc* J = nullptr;          // njac projected Jacobian columns of 2^L entries each
+};
+
+static BlockEncInputs prepare_blockenc(Context& c, const EvalPlan& pl, linear_func hfunc, void* hdata, int njac)
+{
+	if (hfunc == nullptr) { throw Error(-3, "hfunc is NULL"); }
+	const int half = pl.L / 2;
+	const long s = 1L << half, N = pl.N, B = pl.B;
+	const size_t sz = (size_t)N * B, nx = (size_t)pl.nblk * sz;
+	const size_t xbytes = align256(nx * (pl.fp32 ? sizeof(cx<float>) : sizeof(zc)));
+	const size_t hbytes = align256((size_t)s * s * sizeof(zc));
+	c.samples.reserve(xbytes + hbytes + (size_t)njac * N * sizeof(zc));
+	char* base = c.samples.as<char>();
+	const int logB = ilog2_or_neg(B);
+	{
+		LaunchTimer lt(c, RQC_KERNEL_OTHER);
+		if (pl.fp32) {
+			k_blockenc_columns<float><<<nblocks((long)nx), kBlock, 0, c.stream>>>(reinterpret_cast<cx<float>*>(base), half, B,
+				logB, (long)sz, (long)nx);
+		} else {
+			k_blockenc_columns<double><<<nblocks((long)nx), kBlock, 0, c.stream>>>(reinterpret_cast<cx<double>*>(base), half,
+				B, logB, (long)sz, (long)nx);
+		}
+	}
+	BlockEncInputs bi;
+	bi.X = base;
+	if (hfunc == rqc_device_ufunc) {
+		bi.H = device_target_matrix(hdata, half);
+	} else {
+		std::vector<zc> h((size_t)s * s), e((size_t)s), out((size_t)s);
+		for (long b = 0; b < s; b++) {
+			std::fill(e.begin(), e.end(), zc{0, 0});
+			e[b] = zc{1, 0};
+			struct statevector sin = {reinterpret_cast<numeric*>(e.data()), half};
+			struct statevector sout = {reinterpret_cast<numeric*>(out.data()), half};
+			const int r = hfunc(&sin, hdata, &sout);
+			if (r < 0) { throw Error(-2, "call of 'hfunc' failed, return value: " + std::to_string(r)); }
+			for (long a = 0; a < s; a++) { h[(size_t)a * s + b] = out[a]; }
+		}
+		zc* Hd = reinterpret_cast<zc*>(base + xbytes);
+		RQC_HIP(hipMemcpyAsync(Hd, h.data(), h.size() * sizeof(zc), hipMemcpyHostToDevice, c.stream));
+		// the host image is pageable and goes out of scope
+		RQC_HIP(hipStreamSynchronize(c.stream));
+		bi.H = Hd;
+	}
+	bi.J = (njac > 0) ? reinterpret_cast<zc*>(base + xbytes + hbytes) : nullptr;
+	return bi;
+}
+
+static EvalResult eval_blockenc(const Circuit& C, linear_func hfunc, void* hdata, Mode mode, bool project)
+{
+	Context& c = ctx();
+	std::lock_guard<std::recursive_mutex> lk(c.mu);
+	const bool unitary = (mode == Mode::Target) || gates_near_unitary(C);
+	const EvalPlan pl = plan_eval(c, C, mode, C.L / 2, c.precision == RQC_PRECISION_MIXED, project, unitary, c.ws_limit,
+		c.work.bytes, nullptr, 1L << (C.L / 2));
+	const BlockEncInputs bi = prepare_blockenc(c, pl, hfunc, hdata, 0);
+	std::vector<char> tab(pl.tab_bytes);
+	fill_eval_tables(pl, C, tab.data());
+	c.work.reserve(pl.bytes);
+	char* work = c.work.as<char>();
+	const BlockEnc be{bi.H, nullptr};
+	enqueue_eval(c, c, pl, C, nullptr, work, nullptr, nullptr, tab.data(), nullptr, bi.X, nullptr, &be);
+	std::vector<char> res(pl.res_bytes());
+	RQC_HIP(hipMemcpyAsync(res.data(), work + pl.off_res, res.size(), hipMemcpyDeviceToHost, c.stream));
+	RQC_HIP(hipStreamSynchronize(c.stream));
+	RQC_HIP(hipGetLastError());
+	EvalResult R;
+	unpack_result(pl, res.data(), R);
+	return R;
+}
+
 // Complex Hessian of the brickwall target (reference src/target.c:687-797).  Column
 // (j, y) is the HVP with the unit direction E_y on every gate of layer j, so the
 // 16 nlayers columns are HVP evaluations of one plan: their tables go up in one
@@ -1955,25 +2052,31 @@ static void scale_result(EvalResult& R, double a)
 // result region each, and the results come back in one copy.  The matrix is
 // symmetric up to rounding and is symmetrised; f and dvlist come with column 0.
 static void eval_hessian(const Circuit& C0, linear_func ufunc, void* udata, int per, zc* fval, zc* dvlist, zc* hess,
-	long nsamples = 0, struct rng_state* rng = nullptr)
+	long nsamples = 0, struct rng_state* rng = nullptr, zc* hess2 = nullptr)
 {
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
 	Circuit C = C0;
 	C.dirs.assign(C.gates.size(), zc{0, 0});
 	const int ng = C.ngates / per, m = 16 * ng;
+	// with hess2 the objective is the block encoding of the operator (ufunc, udata) on L/2 qubits
+	const bool blockenc = (hess2 != nullptr);
 	const bool unitary = gates_near_unitary(C);
 	const EvalPlan pl = plan_eval(c, C, Mode::Hvp, per, c.precision == RQC_PRECISION_MIXED, false, unitary, c.ws_limit,
-		c.work.bytes, nullptr, nsamples);
+		c.work.bytes, nullptr, blockenc ? (1L << (C.L / 2)) : nsamples);
 	const zc* dU = nullptr;
 	SampleBlocks sb;
-	if (pl.sampled) {
+	BlockEncInputs bi;
+	if (blockenc) {
+		bi = prepare_blockenc(c, pl, ufunc, udata, m);
+		sb.X = bi.X;
+	} else if (pl.sampled) {
 		if (rng == nullptr) { throw Error(-3, "rng is NULL"); }
 		sb = prepare_samples(c, pl, ufunc, udata, rng);
 	} else {
 		dU = resolve_target(c, ufunc, udata, C0.L);
 	}
-	const double scale = pl.sampled ? (double)pl.N / (double)nsamples : 1.0;
+	const double scale = (pl.sampled && !blockenc) ? (double)pl.N / (double)nsamples : 1.0;
 	const size_t ts = align256(pl.tab_bytes), rs = align256(pl.res_bytes()), base = align256(pl.bytes);
 	std::vector<char> htab((size_t)m * ts);
 	std::vector<Circuit> cols((size_t)m, C);
@@ -1983,17 +2086,28 @@ static void eval_hessian(const Circuit& C0, linear_func ufunc, void* udata, int 
 		for (int q = 0; q < per; q++) { Cc.dirs[16 * ((size_t)jl * per + q) + y] = zc{1, 0}; }
 		fill_eval_tables(pl, Cc, htab.data() + (size_t)col * ts);
 	}
-	c.work.reserve(base + (size_t)m * (ts + rs));
+	const size_t gbytes = blockenc ? (size_t)m * m * sizeof(zc) : 0;
+	c.work.reserve(base + (size_t)m * (ts + rs) + gbytes);
 	char* work = c.work.as<char>();
 	char* tabs = work + base;
 	char* ress = tabs + (size_t)m * ts;
 	RQC_HIP(hipMemcpyAsync(tabs, htab.data(), htab.size(), hipMemcpyHostToDevice, c.stream));
 	for (int col = 0; col < m; col++) {
+		const BlockEnc be{bi.H, blockenc ? bi.J + (size_t)col * pl.N : nullptr};
 		enqueue_eval(c, c, pl, cols[col], dU, work, tabs + (size_t)col * ts, ress + (size_t)col * rs, nullptr, nullptr,
-			sb.X, sb.W);
+			sb.X, sb.W, blockenc ? &be : nullptr);
 	}
 	std::vector<char> hres((size_t)m * rs);
 	RQC_HIP(hipMemcpyAsync(hres.data(), ress, hres.size(), hipMemcpyDeviceToHost, c.stream));
+	if (blockenc) {
+		// hess2: Gram matrix of the projected Jacobian columns P^dagger dW P e_b of the m unit directions
+		zc* gram = reinterpret_cast<zc*>(ress + (size_t)m * rs);
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_gram_hermitian<<<dim3(m, m), kBlock, 0, c.stream>>>(bi.J, pl.N, m, gram);
+		}
+		RQC_HIP(hipMemcpyAsync(hess2, gram, gbytes, hipMemcpyDeviceToHost, c.stream));
+	}
 	RQC_HIP(hipStreamSynchronize(c.stream));
 	RQC_HIP(hipGetLastError());
 	std::vector<zc> H((size_t)m * m);
@@ -2082,6 +2196,50 @@ extern "C" int brickwall_unitary_target_gradient_hessian_sampling(linear_func uf
 		zc f;
 		eval_hessian(C, ufunc, udata, nqubits / 2, &f, reinterpret_cast<zc*>(dvlist), reinterpret_cast<zc*>(hess), nsamples, rng);
 		std::memcpy(fval, &f, sizeof(zc));
+	});
+}
+
+extern "C" int brickwall_blockenc_target(linear_func hfunc, void* hdata, const struct mat4x4 vlist[], const int nlayers,
+	const int nqubits, const int* perms[], double* fval)
+{
+	return target_call("brickwall_blockenc_target", [&] {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
+		*fval = eval_blockenc(C, hfunc, hdata, Mode::Target, false).f.x;
+	});
+}
+
+extern "C" int brickwall_blockenc_target_and_gradient(linear_func hfunc, void* hdata, const struct mat4x4 vlist[],
+	const int nlayers, const int nqubits, const int* perms[], double* fval, struct mat4x4 dvlist[])
+{
+	return target_call("brickwall_blockenc_target_and_gradient", [&] {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
+		const EvalResult R = eval_blockenc(C, hfunc, hdata, Mode::Grad, false);
+		*fval = R.f.x;
+		std::memcpy(dvlist, R.dv.data(), R.dv.size() * sizeof(zc));
+	});
+}
+
+extern "C" int brickwall_blockenc_target_and_gradient_vector(linear_func hfunc, void* hdata, const struct mat4x4 vlist[],
+	const int nlayers, const int nqubits, const int* perms[], double* fval, double* grad_vec)
+{
+	return target_call("brickwall_blockenc_target_and_gradient_vector", [&] {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
+		const EvalResult R = eval_blockenc(C, hfunc, hdata, Mode::Grad, true);
+		*fval = R.f.x;
+		std::memcpy(grad_vec, R.grad.data(), R.grad.size() * sizeof(double));
+	});
+}
+
+extern "C" int brickwall_blockenc_target_gradient_hessian(linear_func hfunc, void* hdata, const struct mat4x4 vlist[],
+	const int nlayers, const int nqubits, const int* perms[], double* fval, struct mat4x4 dvlist[], numeric* hess1,
+	numeric* hess2)
+{
+	return target_call("brickwall_blockenc_target_gradient_hessian", [&] {
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, nqubits);
+		zc f;
+		eval_hessian(C, hfunc, hdata, nqubits / 2, &f, reinterpret_cast<zc*>(dvlist), reinterpret_cast<zc*>(hess1), 0,
+			nullptr, reinterpret_cast<zc*>(hess2));
+		*fval = f.x;
 	});
 }
 

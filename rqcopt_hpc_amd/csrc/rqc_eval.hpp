@@ -86,15 +86,29 @@ EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, boo
 	size_t budget, size_t held, const TileChoice* choice = nullptr, long ncols = 0);
 // host image of the evaluation's table region (pl.tab_bytes bytes at `host`)
 void fill_eval_tables(const EvalPlan& pl, const Circuit& C, char* host);
+// Block-encoding objective f = ||P^dagger W P - H||_F^2 / 2 in place of the
+// trace (sampled plans over the 2^(L/2) columns P e_b, with P putting bit i of
+// an index of L/2 qubits on bit 2i + 1 and the even bits |0>): the residual
+// kernel reads H, adds |d|^2 / 2 to f and seeds the adjoint with conj(d) on the
+// embedded rows.  In HVP sweeps it also stores the projected tangent
+// P^dagger dpsi of the sweep's direction at J (2^L entries, J[b 2^(L/2) + a]).
+struct BlockEnc
+{
+	const zc* H = nullptr;  // device, 2^(L/2) x 2^(L/2), row-major
+	zc* J = nullptr;        // device, or NULL
+};
+
 // enqueue the evaluation on slot s into workspace `work` (pl.bytes).  The table
 // region is at `tab` and the result region at `res` (NULL: their places in the
 // workspace); the table image is uploaded first when host_tab is given.  With
 // `skip`, every kernel returns at once when *skip != 0 at its start (a device
 // flag, read when the kernel runs; tile-pass plans only).  Sampled plans take
 // the input columns X (nblk blocks of N x B states in the sweep's precision,
-// the last block zero-padded) and the trace seeds W = U X (same layout, FP64).
+// the last block zero-padded) and the trace seeds W = U X (same layout, FP64),
+// or, with `be`, the block-encoding residual in place of the trace (W unused).
 void enqueue_eval(const Context& dev, Slot& s, const EvalPlan& pl, const Circuit& C, const zc* dU, char* work,
-	char* tab, char* res, const char* host_tab, const int* skip, const void* X = nullptr, const zc* W = nullptr);
+	char* tab, char* res, const char* host_tab, const int* skip, const void* X = nullptr, const zc* W = nullptr,
+	const BlockEnc* be = nullptr);
 
 struct EvalResult
 {

@@ -322,7 +322,10 @@ extern "C" void riemannian_trust_region_optimize_hvp(const target_func f, const 
 // derivative convention), projected at V_i on the diagonal blocks together with
 // the curvature term 1/2 (Z dv^dagger V + V dv^dagger Z) of the gradient, and
 // written in the tangent coordinates at V_i; the lower half by symmetry.
-static void project_hessian_matrix(int nlayers, const struct mat4x4* vlist, const std::vector<zc>& dvc, const zc* hess, double* H)
+// hess2 (NULL for the unitary targets) is the Hessian of the conjugated gates of
+// the block-encoding targets, added unconjugated (reference src/target.c:1665-1680).
+static void project_hessian_matrix(int nlayers, const struct mat4x4* vlist, const std::vector<zc>& dvc, const zc* hess,
+	const zc* hess2, double* H)
 {
 
 	const int m = 16 * nlayers;
@@ -337,6 +340,11 @@ static void project_hessian_matrix(int nlayers, const struct mat4x4* vlist, cons
 					zc acc = {0, 0};
 					for (int y = 0; y < 16; y++) { cmac(acc, hess[(size_t)(i * 16 + x) * m + j * 16 + y], Zm[y]); }
 					G[x] = zc{acc.x, -acc.y};
+					if (hess2 != nullptr) {
+						zc acc2 = {0, 0};
+						for (int y = 0; y < 16; y++) { cmac(acc2, hess2[(size_t)(i * 16 + x) * m + j * 16 + y], Zm[y]); }
+						G[x] = zc{G[x].x + acc2.x, G[x].y + acc2.y};
+					}
 				}
 				if (i == j) {
 					zc P[16], gh[16], T[16];
@@ -358,14 +366,15 @@ static void project_hessian_matrix(int nlayers, const struct mat4x4* vlist, cons
 // gradient (conjugated, derivative convention) in tangent coordinates and the projected Hessian
 // matrix (reference src/target.c:1003-1040 and :1050-1088)
 static void gradient_vector_hessian_matrix(const struct mat4x4 vlist[], int nlayers, const std::vector<struct mat4x4>& dv,
-	const std::vector<numeric>& hess, double* grad_vec, double* H)
+	const std::vector<numeric>& hess, double* grad_vec, double* H, const std::vector<numeric>* hess2 = nullptr)
 {
 	std::vector<zc> dvc(16 * (size_t)nlayers);
 	for (int i = 0; i < nlayers; i++) {
 		for (int e = 0; e < 16; e++) { dvc[16 * (size_t)i + e] = zc{dv[i].data[e].real(), -dv[i].data[e].imag()}; }
 		rqc::m4_tangent_to_real(Z(&vlist[i]), &dvc[16 * (size_t)i], &grad_vec[16 * i]);
 	}
-	project_hessian_matrix(nlayers, vlist, dvc, reinterpret_cast<const zc*>(hess.data()), H);
+	project_hessian_matrix(nlayers, vlist, dvc, reinterpret_cast<const zc*>(hess.data()),
+		hess2 != nullptr ? reinterpret_cast<const zc*>(hess2->data()) : nullptr, H);
 }
 
 extern "C" int brickwall_unitary_target_gradient_vector_hessian_matrix_sampling(linear_func ufunc, void* udata,
@@ -393,6 +402,21 @@ extern "C" int brickwall_unitary_target_gradient_vector_hessian_matrix(linear_fu
 	const int ret = brickwall_unitary_target_gradient_hessian(ufunc, udata, vlist, nlayers, nqubits, perms, fval, dv.data(), hess.data());
 	if (ret < 0) { return ret; }
 	gradient_vector_hessian_matrix(vlist, nlayers, dv, hess, grad_vec, H);
+	return 0;
+}
+
+// reference src/target.c:1631-1720
+extern "C" int brickwall_blockenc_target_gradient_vector_hessian_matrix(linear_func hfunc, void* hdata,
+	const struct mat4x4 vlist[], const int nlayers, const int nqubits, const int* perms[], double* fval, double* grad_vec,
+	double* H)
+{
+	const int m = 16 * nlayers;
+	std::vector<struct mat4x4> dv((size_t)std::max(nlayers, 1));
+	std::vector<numeric> hess1((size_t)m * m), hess2((size_t)m * m);
+	const int ret = brickwall_blockenc_target_gradient_hessian(hfunc, hdata, vlist, nlayers, nqubits, perms, fval, dv.data(),
+		hess1.data(), hess2.data());
+	if (ret < 0) { return ret; }
+	gradient_vector_hessian_matrix(vlist, nlayers, dv, hess1, grad_vec, H, &hess2);
 	return 0;
 }
 

@@ -263,6 +263,86 @@ __global__ __launch_bounds__(kBlock) void k_target_trace(
 	block_sum_store<1>(&acc, partial + blockIdx.x);
 }
 
+// Block-encoding residual (reference src/target.c:1162-1180) over the columns
+// b0 .. b0 + B - 1 of the embedded basis P e_b, where P puts bit i of an
+// index of L/2 qubits on bit 2i + 1 (the even bits |0>).  The embedded rows
+// c(a) are those with no even bit set: there d = psi - H[a][b], |d|^2 / 2 goes
+// to the partial sums and the adjoint seed is conj(d); all other entries, and
+// the padding columns b >= 2^(L/2), seed zero.  With J (HVP sweeps) the
+// projected tangent P^dagger dpsi of the sweep's direction is stored at
+// J[b 2^(L/2) + a].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_blockenc_residual(const zc* __restrict__ H, int half, long b0,
+	const cx<T>* __restrict__ psi, cx<T>* __restrict__ phi, const cx<T>* __restrict__ dpsi, zc* __restrict__ J, long B,
+	int logB, long nitems, zc* __restrict__ partial, const int* skip)
+{
+	if (skipped(skip)) { return; }
+	const long s = 1L << half;
+	const long even = 0x5555555555555555L & ((1L << (2 * half)) - 1);
+	double acc = 0;
+	const long stride = (long)gridDim.x * kBlock;
+	for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < nitems; t += stride) {
+		long row, c;
+		if (logB >= 0) { row = t >> logB; c = t & (B - 1); }
+		else { row = t / B; c = t - row * B; }
+		const long b = b0 + c;
+		cx<T> seed = {(T)0, (T)0};
+		if (b < s && (row & even) == 0) {
+			long a = 0;
+			for (int i = 0; i < half; i++) { a |= ((row >> (2 * i + 1)) & 1) << i; }
+			const zc h = H[a * s + b];
+			const zc p = to_z(psi[t]);
+			const double dx = p.x - h.x, dy = p.y - h.y;
+			acc += 0.5 * (dx * dx + dy * dy);
+			seed = cx<T>{(T)dx, (T)(-dy)};
+			if (J != nullptr) { J[b * s + a] = to_z(dpsi[t]); }
+		}
+		if (phi != nullptr) { phi[t] = seed; }
+	}
+	zc r = {acc, 0};
+	block_sum_store<1>(&r, partial + blockIdx.x);
+}
+
+// the embedded basis columns P e_b as blocks of N x B amplitudes (column b in
+// block b / B), zero past b = 2^(L/2) - 1; sz = N B, nitems = blocks * sz
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_blockenc_columns(cx<T>* __restrict__ X, int half, long B, int logB, long sz,
+	long nitems)
+{
+	const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+	if (t >= nitems) { return; }
+	const long blk = t / sz, r = t - blk * sz;
+	long row, c;
+	if (logB >= 0) { row = r >> logB; c = r & (B - 1); }
+	else { row = r / B; c = r - row * B; }
+	const long b = blk * B + c;
+	long e = 0;
+	for (int i = 0; i < half; i++) { e |= ((b >> i) & 1) << (2 * i + 1); }
+	X[t] = cx<T>{(b < (1L << half) && row == e) ? (T)1 : (T)0, (T)0};
+}
+
+// out[x][y] = sum_t conj(J_x[t]) J_y[t] over n entries for x >= y in a fixed
+// order, out[y][x] its conjugate and the diagonal real: an exactly Hermitian
+// Gram matrix of the m columns J_x = J + x n; one workgroup per pair
+__global__ __launch_bounds__(kBlock) void k_gram_hermitian(const zc* __restrict__ J, long n, int m, zc* __restrict__ out)
+{
+	const int x = blockIdx.x, y = blockIdx.y;
+	if (y > x) { return; }
+	const zc* jx = J + (long)x * n;
+	const zc* jy = J + (long)y * n;
+	zc acc = {0, 0};
+	for (long t = threadIdx.x; t < n; t += kBlock) { cmac(acc, zc{jx[t].x, -jx[t].y}, jy[t]); }
+	__shared__ zc res[1];
+	block_sum_store<1>(&acc, res);
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		zc v = res[0];
+		if (x == y) { v.y = 0; }
+		out[(long)x * m + y] = v;
+		out[(long)y * m + x] = zc{v.x, -v.y};
+	}
+}
+
 // acc[e] += sum_p partial[p * width + e] in a fixed order; one workgroup per e
 __global__ __launch_bounds__(kBlock) void k_reduce_partials(const zc* __restrict__ partial, int nparts, int width,
 	zc* __restrict__ acc, const int* skip)

@@ -51,6 +51,11 @@ FIXTURES = [
     "test_brickwall_unitary_target_sampling_cplx",
     "test_brickwall_unitary_target_gradient_hessian_sampling_cplx",
     "test_brickwall_unitary_target_gradient_vector_hessian_matrix_sampling_cplx",
+    "test_brickwall_blockenc_target_cplx",
+    "test_brickwall_blockenc_target_and_gradient_cplx",
+    "test_brickwall_blockenc_target_and_gradient_vector_cplx",
+    "test_brickwall_blockenc_target_gradient_hessian_cplx",
+    "test_brickwall_blockenc_target_gradient_vector_hessian_matrix_cplx",
 ]
 
 

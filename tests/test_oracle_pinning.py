@@ -614,3 +614,90 @@ def test_sampled_estimate_is_the_scaled_sum_of_single_samples():
     assert abs(f4 - np.mean([p[0] for p in parts])) < 1e-12
     assert np.max(np.abs(dv4 - np.mean([p[1] for p in parts], 0))) < 1e-12
     assert np.max(np.abs(h4 - np.mean([p[2] for p in parts], 0))) < 1e-12
+
+
+# ------------------------------------------------- block-encoding targets
+def _embedding(L):
+    """P: 2^L x 2^(L/2), column b = e_{c(b)} with bit i of b on bit 2i + 1 of c (the ancillas |0>)."""
+    s = 1 << (L // 2)
+    P = np.zeros((1 << L, s))
+    for b in range(s):
+        c = sum(1 << (2 * i + 1) for i in range(L // 2) if b >> i & 1)
+        P[c, b] = 1
+    return P
+
+
+def test_fixture_blockenc_targets():
+    """The oracle's block-encoding restatement against the reference's stored f, dVlist, grad,
+    hess1, hess2 and H (reference test/test_target.c:1578-2150; H = the scramble map on L/2 qubits)."""
+    d = fixture("test_brickwall_blockenc_target_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    for k, nl in enumerate((4, 5)):
+        assert abs(O.brickwall_blockenc_target(SCR, V[:nl], P[:nl], 8) - float(d[f"f{k}"])) < 1e-12
+    d = fixture("test_brickwall_blockenc_target_and_gradient_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    for k, nl in enumerate((4, 5)):
+        f, dv = O.brickwall_blockenc_target_and_gradient(SCR, V[:nl], P[:nl], 8)
+        assert abs(f - float(d[f"f{k}"])) < 1e-12 and np.max(np.abs(dv - cplx(d[f"dVlist{k}"]))) < 1e-12
+    d = fixture("test_brickwall_blockenc_target_and_gradient_vector_cplx")
+    V, P = stack(d, "V", 3), perms_of(d, 3)
+    f, g = O.brickwall_blockenc_target_and_gradient_vector(SCR, V, P, 6)
+    assert abs(f - float(d["f"])) < 1e-12 and np.max(np.abs(g - d["grad"])) < 1e-12
+    d = fixture("test_brickwall_blockenc_target_gradient_hessian_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    for k, nl in enumerate((4, 5)):
+        f, dv, h1, h2 = O.brickwall_blockenc_target_gradient_hessian(SCR, V[:nl], P[:nl], 6)
+        assert abs(f - float(d[f"f{k}"])) < 1e-12 and np.max(np.abs(dv - cplx(d[f"dVlist{k}"]))) < 1e-12
+        assert np.max(np.abs(h1 - cplx(d[f"hess1_{k}"]))) < 1e-12 * max(1, np.max(np.abs(h1)))
+        assert np.max(np.abs(h2 - cplx(d[f"hess2_{k}"]))) < 1e-12 * max(1, np.max(np.abs(h2)))
+        assert np.max(np.abs(h1 - h1.T)) < 1e-12 and np.array_equal(h2, h2.conj().T)
+    d = fixture("test_brickwall_blockenc_target_gradient_vector_hessian_matrix_cplx")
+    V, P = stack(d, "V", 5), perms_of(d, 5)
+    f, g, H = O.brickwall_blockenc_target_gradient_vector_hessian_matrix(SCR, V, P, 6)
+    assert abs(f - float(d["f"])) < 1e-12 and np.max(np.abs(g - d["grad"])) < 1e-12
+    assert np.max(np.abs(H - d["H"])) < 1e-12 * np.max(np.abs(d["H"])) and np.array_equal(H, H.T)
+
+
+def test_blockenc_target_closed_form():
+    """f = ||P^T W P - H||_F^2 / 2 with dense W and the explicit embedding matrix."""
+    rng = np.random.default_rng(30)
+    for L, nl in ((4, 2), (6, 3)):
+        V = crandn(rng, (nl, 4, 4))
+        P = random_perms(rng, L, nl)
+        H = crandn(rng, (1 << (L // 2), 1 << (L // 2)))
+        E = _embedding(L)
+        ref = 0.5 * np.sum(np.abs(E.T @ brickwall_matrix(V, P) @ E - H) ** 2)
+        assert abs(O.brickwall_blockenc_target(O.Ufunc.dense(H), V, P, L) - ref) < 1e-12 * ref
+
+
+def test_blockenc_gradient_and_hessians_finite_differences():
+    """f real: f(V + eD) - f(V - eD) = 2e Re sum(dv D); the derivative of dv along D is
+    hess1 D + conj(hess2 D) (hess1 from the fixed-residual second derivative, hess2 the Gram
+    matrix of the projected Jacobian), checked with D and i D so that both are pinned."""
+    rng = np.random.default_rng(31)
+    L, nl = 6, 3
+    V = crandn(rng, (nl, 4, 4))
+    P = random_perms(rng, L, nl)
+    hf = O.Ufunc.dense(crandn(rng, (8, 8)))
+    f, dv, h1, h2 = O.brickwall_blockenc_target_gradient_hessian(hf, V, P, L)
+    f2, dv2 = O.brickwall_blockenc_target_and_gradient(hf, V, P, L)
+    assert f == f2 and np.array_equal(dv, dv2)
+    eps = 1e-5
+    for D in (crandn(rng, (nl, 4, 4)), 1j * crandn(rng, (nl, 4, 4))):
+        num = (O.brickwall_blockenc_target(hf, V + eps * D, P, L) - O.brickwall_blockenc_target(hf, V - eps * D, P, L)) / (2 * eps)
+        assert abs(num - np.real(np.sum(dv * D))) < 1e-7 * max(1, abs(num))
+        for Dd in (D, 1j * D):
+            _, dp = O.brickwall_blockenc_target_and_gradient(hf, V + eps * Dd, P, L)
+            _, dm = O.brickwall_blockenc_target_and_gradient(hf, V - eps * Dd, P, L)
+            numd = ((dp - dm) / (2 * eps)).reshape(-1)
+            z = Dd.reshape(-1)
+            ref = h1 @ z + np.conj(h2 @ z)
+            assert np.max(np.abs(numd - ref)) < 1e-6 * np.max(np.abs(ref))
+    # hess2 is the Gram matrix of the projected layer Jacobians, one forward placeholder sweep per layer
+    E = _embedding(L)
+    G = np.zeros((16 * nl, 16 * nl), np.complex128)
+    for b in range(8):
+        _, cache = O.brickwall_unitary_forward(V, P, E[:, b].astype(np.complex128))
+        J = np.concatenate([E.T @ O.apply_brickwall_unitary_gate_placeholder(V, P, l, cache) for l in range(nl)], axis=1)
+        G += J.conj().T @ J
+    assert np.max(np.abs(G - h2)) < 1e-12 * np.max(np.abs(h2))
