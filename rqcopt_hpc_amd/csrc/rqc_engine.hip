@@ -712,6 +712,10 @@ EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, boo
 		pl.terms_bytes = align256((size_t)pl.G * sizeof(GateTerms));
 	}
 	pl.tab_bytes = pl.mats_bytes + pl.terms_bytes + 2 * 16 * (size_t)pl.ng * sizeof(zc);
+	if (pl.tiles && mode != Mode::Target) {
+		pl.rmats_off = align256(pl.tab_bytes);
+		pl.tab_bytes = pl.rmats_off + (size_t)pl.G * 32 * sizeof(zc);
+	}
 	pl.off_acc = 0;
 	pl.off_part = align256(pl.n_acc * sizeof(zc));
 	pl.off_tab = pl.off_part + align256(pl.n_part * sizeof(zc));
@@ -751,6 +755,12 @@ static void fill_tables(const EvalPlan& pl, const Circuit& C, char* host)
 				for (int e = 0; e < 16; e++) {
 					hm[(x * kMats + w) * 16 + e] = cx<T>{(T)m6[w][e].x, (T)m6[w][e].y};
 				}
+			}
+			if (pl.rmats_off > 0) {
+				// the reduction's maps from the adjoint at the gate input to its output: G^-T, W^T
+				zc* rm = reinterpret_cast<zc*>(host + pl.rmats_off) + 32 * x;
+				m4_transpose(m6[kMatGinv], rm);
+				m4_transpose(m6[kMatW], rm + 16);
 			}
 		}
 		std::memcpy(host + pl.mats_bytes, tp.terms.data(), tp.terms.size() * sizeof(GateTerms));
@@ -792,6 +802,7 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	const long ntiles = (N >> tp.ts.K) * (B >> tp.ts.logC);
 	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
 	auto* kr = dual ? k_tile_reverse<T, true> : k_tile_reverse<T, false>;
+	const zc* rmats = reinterpret_cast<const zc*>(reinterpret_cast<const char*>(mats) + pl.rmats_off);
 	double* mpart = reinterpret_cast<double*>(d_partial);   // grev * waves * ng * 256 doubles
 	double* cpart = mpart + tp.partial_doubles;             // chunk sums: kReduceChunks * ng * 256 doubles
 	zc* tpart = d_partial;                                  // trace partials (reused, sequential)
@@ -849,7 +860,8 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 				const long nparts = (long)tp.grev * kTileWaves;
 				const int nch = (int)std::min<long>(kReduceChunks, nparts);
 				k_tile_reduce_chunks<<<dim3(ng, nch), 256, 0, s.stream>>>(mpart, nparts, ng, nch, cpart, skip);
-				k_tile_reduce<<<ng, 256, 0, s.stream>>>(cpart, nch, ng, tp.gks[p], dual ? 1 : 0, d_acc, skip);
+				k_tile_reduce<<<ng, 256, 0, s.stream>>>(cpart, nch, ng, tp.gks[p], rmats + 32L * tp.args[p].x0, dual ? 1 : 0,
+					d_acc, skip);
 			}
 			cur = 1 - cur;
 		}

@@ -70,6 +70,7 @@ struct EvalPlan
 	// workspace layout: byte offsets from a 256-byte aligned base
 	size_t off_acc = 0, off_part = 0, off_tab = 0, off_res = 0, off_states = 0, bytes = 0;
 	size_t mats_bytes = 0, terms_bytes = 0, tab_bytes = 0;
+	size_t rmats_off = 0;  // table offset of G^-T and W^T per gate (FP64; tile-pass gradient / HVP plans)
 	int tile_K = 0, tile_logC = 0, gmax = 0;   // tile shape and gates per pass (tile-pass plans)
 	std::shared_ptr<const TilePlan> tp;
 

@@ -218,13 +218,14 @@ __device__ __forceinline__ int mfma_row(int lane, int v)
 }
 
 // per-lane LDS terms of a gate: gate application (lane: quad lane & 15,
-// amplitude lane >> 4; FP32 stores: amplitudes 0..3) and outer-product steps
-// (step t: quad 4 t + lane >> 4, amplitude of component lane & 15)
+// amplitude lane >> 4; FP32 stores: amplitudes 0..3) and the transposed
+// application's results (register v: amplitude lane & 3 of quad
+// (lane >> 4) + 4 v in FP64, 4 (lane >> 4) + v in FP32)
 struct GateLanes
 {
 	int ea;          // gate application, amplitude lane >> 4
 	int ew[4];       // amplitudes 0..3 of quad lane & 15
-	int eo[4];       // outer-product step t
+	int et[4];       // transposed results, register v
 };
 
 // (by linearity: pos(j, r) = pos(j, 0) ^ pos(0, r), pos(4 t + k, 0) =
@@ -255,6 +256,7 @@ __device__ __forceinline__ int sel4_affine(const int (&v)[4], int i)
 	return v[0] ^ ((v[1] ^ v[0]) & m1) ^ ((v[2] ^ v[0]) & m2);
 }
 
+template <typename T = double>
 __device__ __forceinline__ GateLanes gate_lanes(const GateTerms& t)
 {
 	const int lane = threadIdx.x & 63;
@@ -263,9 +265,16 @@ __device__ __forceinline__ GateLanes gate_lanes(const GateTerms& t)
 #pragma unroll
 	for (int v = 0; v < 4; v++) { g.ew[v] = base ^ t.u[v]; }
 	g.ea = base ^ sel4_affine(t.u, lane >> 4);
-	const int bk = sel4_affine(t.q4, lane >> 4) ^ sel4_affine(t.u, ((lane & 15) >> 1) & 3);
+	// quad q of a group: position q4[q & 3] ^ w[(q >> 2) & 3]
+	if constexpr (sizeof(T) == 8) {
+		const int bt = sel4_affine(t.q4, lane >> 4) ^ sel4_affine(t.u, lane & 3);
 #pragma unroll
-	for (int k = 0; k < 4; k++) { g.eo[k] = bk ^ t.w[k]; }
+		for (int v = 0; v < 4; v++) { g.et[v] = bt ^ t.w[v]; }
+	} else {
+		const int bt = sel4_affine(t.w, lane >> 4) ^ sel4_affine(t.u, lane & 3);
+#pragma unroll
+		for (int v = 0; v < 4; v++) { g.et[v] = bt ^ t.q4[v]; }
+	}
 	return g;
 }
 
@@ -320,6 +329,38 @@ __device__ __forceinline__ void gate_store(cx<T>* X, cx<T>* Y, const int* eg, co
 #pragma unroll
 			for (int v = 0; v < 4; v++) { base[2 * (e0 ^ gl.ew[v]) + (r & 1)] = o.acc[n][v]; }
 		}
+	}
+}
+
+// Transposed application D^T = S^T M^T: the same operand registers with the
+// state as the A operand and the gate as the B operand.  The result holds, in
+// lane l, component l & 15 (4 s + r: s = x.re, x.im, y.re, y.im, r the
+// amplitude) of quad (l >> 4) + 4 v (FP64) or 4 (l >> 4) + v (FP32) in register
+// v, which is the operand layout of an outer-product step over those quads.
+template <typename T, int GS>
+__device__ __forceinline__ void gate_mfma_t(const T* a, GateOperands<T, GS>& o)
+{
+#pragma unroll
+	for (int n = 0; n < GS; n++) { o.acc[n] = mfma_step(o.x[n].x, a[0], typename Vec4<T>::type{0, 0, 0, 0}); }
+#pragma unroll
+	for (int n = 0; n < GS; n++) { o.acc[n] = mfma_step(o.x[n].y, a[1], o.acc[n]); }
+#pragma unroll
+	for (int n = 0; n < GS; n++) { o.acc[n] = mfma_step(o.y[n].x, a[2], o.acc[n]); }
+#pragma unroll
+	for (int n = 0; n < GS; n++) { o.acc[n] = mfma_step(o.y[n].y, a[3], o.acc[n]); }
+}
+
+template <typename T, int GS>
+__device__ __forceinline__ void gate_store_t(cx<T>* X, cx<T>* Y, const int* eg, const int* eg2, const GateLanes& gl,
+	const GateOperands<T, GS>& o)
+{
+	const int sc = (threadIdx.x & 15) >> 2;   // x.re, x.im, y.re, y.im
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		T* base = reinterpret_cast<T*>(sc < 2 ? X : Y);
+		const int e0 = (sc < 2) ? eg[n] : eg2[n];
+#pragma unroll
+		for (int v = 0; v < 4; v++) { base[2 * (e0 ^ gl.et[v]) + (sc & 1)] = o.acc[n][v]; }
 	}
 }
 
@@ -551,11 +592,6 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 	const int lane = threadIdx.x & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 	const int qpw = (E / 4) / kTileWaves;
-	const int m = lane & 15;
-	// outer-product operands: component m = 8 s + 2 r + p of the quad (dual: s
-	// selects the primary / tangent array; plain: the quad or the one 16 after it)
-	const int osel = DUAL ? 0 : (m >> 3);
-	const int oarr = (DUAL && (m >> 3)) ? 1 : 0;
 
 	double4_t acc[kMaxTileGates];
 #pragma unroll
@@ -609,8 +645,6 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 		cx<T>* D = state(b, DUAL ? 1 : 0);
 		cx<T>* F = state(b, NS / 2);
 		cx<T>* DF = state(b, DUAL ? 3 : NS / 2);
-		const T* Fa = reinterpret_cast<const T*>(oarr ? DF : F);
-		const T* Sa = reinterpret_cast<const T*>(oarr ? D : S);
 		// body of gate J (compile-time, so that its outer products accumulate in
 		// the registers acc[J] directly); dispatched once per gate on the
 		// wave-uniform gate index
@@ -621,7 +655,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 				if (J + 1 < P.ng) { flush32(acc[J + 1]); }
 			}
 			const GateTerms gt = gterms[P.x0 + J];
-			const GateLanes gl = gate_lanes(gt);
+			const GateLanes gl = gate_lanes<T>(gt);
 			const int ewv = sel4(gt.wv, wave);
 			// recovery: [[G^-1, 0], [-G^-1 Z G^-1, G^-1]]; adjoint: [[G^T, 0], [Z^T, G^T]]
 			const T* mj = reinterpret_cast<const T*>(lm + J * NM * 16);
@@ -640,31 +674,18 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 					eg2[n] = DUAL ? eg[n] : ewv ^ group_term(gt, ((g + n * gq) >> 4) + 1);
 				}
 				// The state's recovery (uncompute) and the adjoint's update are
-				// independent: load both, run both chains, store the state; the
-				// outer products need the recovered state and the adjoint before
-				// its update (read before the adjoint's store).
+				// independent: load both, run both chains (transposed, so that the
+				// results are outer-product operands), store both; the outer
+				// products pair the adjoint at the gate input with the recovered
+				// state straight from the registers (the reduction maps
+				// phi_in (x) psi to the gate's phi_out (x) psi).
 				GateOperands<T, GS> os, oa;
 				gate_load<T, GS>(S, D, eg, eg2, gl, os);
 				gate_load<T, GS>(F, DF, eg, eg2, gl, oa);
-				T av[GS][4], bv[GS][4];
-#pragma unroll
-				for (int n = 0; n < GS; n++) {
-					const int eb = osel ? eg2[n] : eg[n];
-#pragma unroll
-					for (int t = 0; t < 4; t++) { av[n][t] = Fa[2 * (eb ^ gl.eo[t]) + (m & 1)]; }
-				}
-				gate_mfma<T, GS>(au, os);
-				gate_mfma<T, GS>(aa, oa);
-				gate_store<T, GS>(S, D, eg, eg2, gl, os);
-				// adjoints at the gate input (their pre-update values are in av)
-				if (adjoint) { gate_store<T, GS>(F, DF, eg, eg2, gl, oa); }
-				// outer products over the groups' quads, 4 quads per step
-#pragma unroll
-				for (int n = 0; n < GS; n++) {
-					const int eb = osel ? eg2[n] : eg[n];
-#pragma unroll
-					for (int t = 0; t < 4; t++) { bv[n][t] = Sa[2 * (eb ^ gl.eo[t]) + (m & 1)]; }
-				}
+				gate_mfma_t<T, GS>(au, os);
+				gate_mfma_t<T, GS>(aa, oa);
+				gate_store_t<T, GS>(S, D, eg, eg2, gl, os);
+				if (adjoint) { gate_store_t<T, GS>(F, DF, eg, eg2, gl, oa); }
 				if constexpr (sizeof(T) == 8) {
 					// FP64: group 0 accumulates onto acc[J], the others from zero
 					double4_t grp[GS];
@@ -673,7 +694,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 #pragma unroll
 					for (int t = 0; t < 4; t++) {
 #pragma unroll
-						for (int n = 0; n < GS; n++) { grp[n] = mfma_step(av[n][t], bv[n][t], grp[n]); }
+						for (int n = 0; n < GS; n++) { grp[n] = mfma_step(oa.acc[n][t], os.acc[n][t], grp[n]); }
 					}
 #pragma unroll
 					for (int n = 1; n < GS; n++) { grp[0] += grp[n]; }
@@ -684,7 +705,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 #pragma unroll
 					for (int t = 0; t < 4; t++) {
 #pragma unroll
-						for (int n = 0; n < GS; n++) { pend = mfma_step(av[n][t], bv[n][t], pend); }
+						for (int n = 0; n < GS; n++) { pend = mfma_step(oa.acc[n][t], os.acc[n][t], pend); }
 					}
 				}
 			}
@@ -746,38 +767,55 @@ __global__ __launch_bounds__(256) void k_tile_reduce_chunks(const double* __rest
 // Sum of the outer-product partials over all parts in a fixed order, decoded
 // into 16 gradient and (DUAL) 16 HVP entries per gate and added to
 // acc[1 + 32 * gate + e].  One workgroup of 256 threads per gate of the pass.
-//   dual:  grad = phi (x) psi,  hess = phi (x) dpsi + dphi (x) psi
-//   plain: grad = phi (x) psi summed over the two quads of each operand pair
+// The partials pair the adjoint at the gate input (phi_in = G^T phi,
+// dphi_in = G^T dphi + Z^T phi) with the state before the gate; component
+// 4 (2 s + p) + r is array s (primary / tangent; plain sweeps: the quad of the
+// pair), re / im p, amplitude r.  With A1 = sum phi_in (x) psi,
+// A2 = sum phi_in (x) dpsi, A3 = sum dphi_in (x) psi and Q = G^-T, the gate's
+// sums over the adjoint at its output are
+//   grad = Q A1,   hess = Q (A2 + A3) + W^T A1   (W = -G^-1 Z G^-1);
+// rm holds Q and W^T of the pass's gates (FP64, 32 entries per gate).
 __global__ __launch_bounds__(256) void k_tile_reduce(const double* __restrict__ partial, long nparts, int ng,
-	const TileGates gk, int dual, zc* __restrict__ acc, const int* skip)
+	const TileGates gk, const zc* __restrict__ rm, int dual, zc* __restrict__ acc, const int* skip)
 {
 	if (skipped(skip)) { return; }
 	__shared__ double Dm[16][16];
+	__shared__ zc A[2][16];
 	const int j = blockIdx.x, i = threadIdx.x;
 	double s = 0;
 	for (long p = 0; p < nparts; p++) { s += partial[(p * ng + j) * 256 + i]; }
 	Dm[i >> 4][i & 15] = s;
 	__syncthreads();
+	const int which = (i >> 4) & 1, r = (i >> 2) & 3, c = i & 3;
 	if (i < 32) {
-		const int hess = i >> 4, r = (i >> 2) & 3, c = i & 3;
-		auto M = [&](int sidx, int rr, int pp) { return 8 * sidx + 2 * rr + pp; };
-		// sum_{s in S} phi_r(s) * psi_c(s') over the listed block pairs
+		auto M = [&](int sidx, int rr, int pp) { return 4 * (2 * sidx + pp) + rr; };
+		// sum phi_r(s) * psi_c(s') over the listed block pairs
 		auto blk = [&](int sa, int sb, double& re, double& im) {
 			re += Dm[M(sa, r, 0)][M(sb, c, 0)] - Dm[M(sa, r, 1)][M(sb, c, 1)];
 			im += Dm[M(sa, r, 0)][M(sb, c, 1)] + Dm[M(sa, r, 1)][M(sb, c, 0)];
 		};
 		double re = 0, im = 0;
 		if (dual) {
-			if (!hess) { blk(0, 0, re, im); }
+			if (!which) { blk(0, 0, re, im); }
 			else { blk(0, 1, re, im); blk(1, 0, re, im); }
-		} else {
-			if (hess) { return; }
+		} else if (!which) {
 			blk(0, 0, re, im);
 			blk(1, 1, re, im);
 		}
-		zc* a = acc + 1 + 32L * gk.k[j] + 16 * hess + 4 * r + c;
-		a->x += re;
-		a->y += im;
+		A[which][4 * r + c] = zc{re, im};
+	}
+	__syncthreads();
+	if (i < 32 && (dual || !which)) {
+		const zc* Q = rm + 32L * j;
+		const zc* WT = Q + 16;
+		zc v = {0, 0};
+		for (int k = 0; k < 4; k++) { cmac(v, Q[4 * r + k], A[which][4 * k + c]); }
+		if (which) {
+			for (int k = 0; k < 4; k++) { cmac(v, WT[4 * r + k], A[0][4 * k + c]); }
+		}
+		zc* a = acc + 1 + 32L * gk.k[j] + 16 * which + 4 * r + c;
+		a->x += v.x;
+		a->y += v.y;
 	}
 }
 
