@@ -439,9 +439,10 @@ static std::vector<Pass> plan_passes(const Circuit& C, int K, int gmax)
 // tile bits differ) and the third least used to position 0 (the two rows of a
 // 256-byte LDS bank window differ in tile bit 0, so gates away from it read
 // their matrix-core groups without bank conflicts).  Column swizzle: a greedy
-// colouring of the pass's gate graph (tile bits as vertices) with three colours
-// h in {1, 2, 3} (FP64: column bits 0-1; FP32: bits 1-2) gives the four
-// amplitude rows of every gate distinct bank slots in the outer-product reads.
+// colouring of the pass's gate graph (tile bits as vertices) with three colours,
+// column masks h in {1, 2, 3} (FP64) or {2, 8, 10} (FP32), gives the four
+// amplitude rows of every gate distinct bank slots in the reverse pass's
+// transposed stores.
 static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC, int lognc, int x0, int elem_bytes,
 	GateTerms* gterms)
 {
@@ -489,10 +490,13 @@ static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC,
 		}
 		col[t] = (c <= 3) ? c : 1;
 	}
-	const int shift = (elem_bytes == 16) ? 0 : 1;
+	// FP32 masks {2, 8, 10}: even (the two elements of a 16-byte slot stay adjacent) and
+	// without bit 2, so that the quads of a transposed-store phase (column bit 2 apart)
+	// and their four amplitudes take distinct banks
+	static const int kFp32Masks[4] = {0, 2, 8, 10};
 	for (int t = 0; t < K; t++) {
-		const int h = (col[t] << shift) & ((1 << logC) - 1);
-		for (int i = 0; i < 3; i++) {
+		const int h = ((elem_bytes == 16) ? col[t] : kFp32Masks[col[t]]) & ((1 << logC) - 1);
+		for (int i = 0; i < 4; i++) {
 			if ((h >> i) & 1) { a.swz[i] |= 1u << t; }
 		}
 	}
@@ -629,9 +633,10 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 	const long ntiles = (pl.N >> ts.K) * (pl.B >> ts.logC);
 	// two buffers of (psi[, dpsi]) forward and (psi, [dpsi,] phi[, dphi]) reverse, then
 	// the pass's gate matrices (forward G[, Z]; reverse G^-1, [W,] G^T[, Z^T])
-	const size_t mat_bytes = (size_t)kMaxTileGates * 16 * sizeof(cx<T>);
-	tp->lds_fwd = 2 * (size_t)(dual ? 2 : 1) * (E * sizeof(cx<T>) + kLdsPadBytes) + (dual ? 2 : 1) * mat_bytes;
-	tp->lds_rev = 2 * (size_t)(dual ? 4 : 2) * (E * sizeof(cx<T>) + kLdsPadBytes) + (dual ? 4 : 2) * mat_bytes;
+	const size_t fwd_mats = (size_t)kMaxTileGates * (dual ? fwd_mat_stride<true>() : fwd_mat_stride<false>()) * sizeof(cx<T>);
+	const size_t rev_mats = (size_t)kMaxTileGates * (dual ? rev_mat_stride<true>() : rev_mat_stride<false>()) * sizeof(cx<T>);
+	tp->lds_fwd = 2 * (size_t)(dual ? 2 : 1) * (E * sizeof(cx<T>) + kLdsPadBytes) + fwd_mats;
+	tp->lds_rev = 2 * (size_t)(dual ? 4 : 2) * (E * sizeof(cx<T>) + lds_pad_rev<T>()) + rev_mats;
 	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
 	auto* kr = dual ? k_tile_reverse<T, true> : k_tile_reverse<T, false>;
 	allow_lds(kf, tp->lds_fwd);

@@ -51,6 +51,12 @@ constexpr int kMaxTileGates = 8;
 // LDS states of a tile are E elements + 64 bytes apart, so that equal positions of
 // two states fall on different banks
 constexpr int kLdsPadBytes = 64;
+// The reverse pass stores a quad's transposed results into two states with
+// one instruction.  FP64 (8-byte stores, 128-byte bank window): the 64-byte
+// pad puts the second state's amplitudes on the other half of the window.
+// FP32 (4-byte stores, 32 lanes over 256 bytes): 128 bytes.
+template <typename T>
+constexpr int lds_pad_rev() { return sizeof(T) == 8 ? 64 : 128; }
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -79,7 +85,7 @@ struct TileArgs
 	int x0;            // position of the pass's first gate in the matrix table
 	int lognc;         // log2 of the column chunks of the block (B >> logC)
 	unsigned rmask;    // row bits outside the tile (rest), deposited lowest first
-	unsigned swz[3];   // column swizzle masks over tile rows
+	unsigned swz[4];   // column swizzle masks over tile rows
 	unsigned barrier;  // bit j: the waves own different quads for gates j - 1 and j
 	int rb[16];        // row bit of tile bit t, t < K
 };
@@ -93,30 +99,21 @@ struct TileArgs
 constexpr int kMats = 6;
 enum { kMatG = 0, kMatZ = 1, kMatGinv = 2, kMatW = 3, kMatGT = 4, kMatZT = 5 };
 
-// A operand of step kk for this lane of the real 16 x 16 matrix with complex
-// blocks [[X, 0], [Y, X]] (Y = 0 without the tangent state), components
-// m = 4 v + r, v = 2 s + p: M[lane % 16][4 kk + lane / 16]
-template <typename T, bool DUAL>
-__device__ __forceinline__ T a_operand(const cx<T>* X, const cx<T>* Y, int kk)
-{
-	const int lane = threadIdx.x & 63;
-	const int i = lane & 15, vo = i >> 2, ro = i & 3, ri = lane >> 4;
-	const int so = vo >> 1, po = vo & 1, si = kk >> 1, pi = kk & 1;
-	// (branch-free: both blocks are read; sign and zero block by exact factors)
-	const cx<T> zx = X[4 * ro + ri];
-	const cx<T> zy = DUAL ? Y[4 * ro + ri] : zx;
-	const cx<T> z = (so == si) ? zx : zy;
-	const T v = (po == pi) ? z.x : z.y;
-	const T sign = (T)(1 - 2 * (po < pi));                     // -Im above the diagonal
-	const T keep = (T)((so == si) | (DUAL & (so == 1)));       // [[X, 0], [Y, X]]
-	return v * sign * keep;
-}
+// LDS layout of a pass's gate matrices: each complex 4x4 matrix is stored
+// transposed (entry (r, c) at 4 c + r), and the second matrix Y of a block pair
+// [[X, 0], [Y, X]] sits kMatPairOff entries after X.  A 16-lane phase of an
+// operand read takes one column of X and of Y (re and im): 8 consecutive doubles
+// of X and 8 of Y, which the offset puts on the other half of a 128-byte bank
+// window.  Per gate: forward X, [Y]; reverse X_rec, [Y_rec], X_adj, [Y_adj].
+constexpr int kMatPairOff = 20;
+template <bool DUAL> constexpr int fwd_mat_stride() { return DUAL ? 2 * kMatPairOff : 16; }
+template <bool DUAL> constexpr int rev_mat_stride() { return DUAL ? 4 * kMatPairOff : 2 * kMatPairOff; }
 
-// Per-lane form of a_operand, fixed for the kernel: element aoff[kk] (in T
-// units, from the start of the gate's matrices X, Y) times afac[kk] (the sign
-// of -Im above the diagonal, 0 in the zero block).  A gate's operand is then
-// one LDS read and one multiply; X of the operand pair sits `xoff` T after
-// the gate's first matrix.
+// A operand of step kk of the real 16 x 16 matrix with complex blocks
+// [[X, 0], [Y, X]] (Y = 0 without the tangent state), components m = 4 v + r,
+// v = 2 s + p: lane l takes M[l % 16][4 kk + l / 16], i.e. element aoff[kk] (in
+// T units from X, in the layout above) times afac[kk] (the sign of -Im above
+// the diagonal, 0 in the zero block).  Fixed per lane for the kernel.
 template <typename T, bool DUAL>
 struct AOperandMap
 {
@@ -134,22 +131,23 @@ __device__ __forceinline__ AOperandMap<T, DUAL> a_operand_map()
 #pragma unroll
 	for (int kk = 0; kk < 4; kk++) {
 		const int si = kk >> 1, pi = kk & 1;
-		const int blk = (DUAL && so != si) ? 32 : 0;   // Y of [[X, 0], [Y, X]]
-		m.aoff[kk] = blk + 2 * (4 * ro + ri) + (po == pi ? 0 : 1);
+		const int blk = (DUAL && so != si) ? 2 * kMatPairOff : 0;   // Y of [[X, 0], [Y, X]]
+		m.aoff[kk] = blk + 2 * (4 * ri + ro) + (po == pi ? 0 : 1);
 		const int keep = (so == si) | (DUAL & (so == 1));
 		m.afac[kk] = (T)((1 - 2 * (po < pi)) * keep);
 	}
 	return m;
 }
 
-// copy the pass's gate matrices (those listed in `which`) into LDS
+// copy the pass's gate matrices (those listed in `which`) into LDS, transposed,
+// matrix w of gate j at lm + j * stride + w * kMatPairOff
 template <typename T>
 __device__ __forceinline__ void load_pass_mats(const cx<T>* __restrict__ mats, int x0, int ng, const int* which, int nw,
-	cx<T>* lm)
+	int stride, cx<T>* lm)
 {
 	for (int e = threadIdx.x; e < ng * nw * 16; e += kTileThreads) {
 		const int j = e / (nw * 16), w = (e / 16) % nw, t = e & 15;
-		lm[e] = mats[((long)(x0 + j) * kMats + which[w]) * 16 + t];
+		lm[j * stride + w * kMatPairOff + 4 * (t & 3) + (t >> 2)] = mats[((long)(x0 + j) * kMats + which[w]) * 16 + t];
 	}
 }
 
@@ -164,7 +162,7 @@ __host__ __device__ __forceinline__ int tile_quad_row(int q, int lo, int hi)
 __host__ __device__ __forceinline__ int swz_f(int row, const TileArgs& P)
 {
 	return (__builtin_popcount(row & P.swz[0]) & 1) | ((__builtin_popcount(row & P.swz[1]) & 1) << 1) |
-	       ((__builtin_popcount(row & P.swz[2]) & 1) << 2);
+	       ((__builtin_popcount(row & P.swz[2]) & 1) << 2) | ((__builtin_popcount(row & P.swz[3]) & 1) << 3);
 }
 
 __host__ __device__ __forceinline__ int lds_elem(int row, int col, const TileArgs& P)
@@ -510,7 +508,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_forward(
 	// gate matrices G, Z of the pass after the two buffers
 	cx<T>* lm = reinterpret_cast<cx<T>*>(lds_raw + 2 * NS * sbytes);
 	const int which[2] = {kMatG, kMatZ};
-	load_pass_mats(mats, P.x0, P.ng, which, DUAL ? 2 : 1, lm);
+	load_pass_mats(mats, P.x0, P.ng, which, DUAL ? 2 : 1, fwd_mat_stride<DUAL>(), lm);
 	const AOperandMap<T, DUAL> amap = a_operand_map<T, DUAL>();
 	auto prefetch = [&](long tile, int b) {
 		const long rbase = deposit_bits(tile >> P.lognc, P.rmask);
@@ -545,7 +543,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_forward(
 			const GateTerms gt = gterms[P.x0 + j];
 			const GateLanes gl = gate_lanes(gt);
 			const int ewv = sel4(gt.wv, wave);
-			const T* mj = reinterpret_cast<const T*>(lm + j * (DUAL ? 2 : 1) * 16);
+			const T* mj = reinterpret_cast<const T*>(lm + j * fwd_mat_stride<DUAL>());
 			T a[4];
 #pragma unroll
 			for (int kk = 0; kk < 4; kk++) { a[kk] = mj[amap.aoff[kk]] * amap.afac[kk]; }
@@ -587,7 +585,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 	constexpr int GS = DUAL ? 2 : 1;
 	constexpr int gq = DUAL ? 16 : 32;
 	const int E = 1 << (P.K + P.logC);
-	const int sbytes = E * (int)sizeof(cx<T>) + kLdsPadBytes;
+	const int sbytes = E * (int)sizeof(cx<T>) + lds_pad_rev<T>();
 	auto state = [&](int b, int i) { return reinterpret_cast<cx<T>*>(lds_raw + (b * NS + i) * sbytes); };
 	const int lane = threadIdx.x & 63;
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -613,7 +611,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 	cx<T>* lm = reinterpret_cast<cx<T>*>(lds_raw + 2 * NS * sbytes);
 	const int which[4] = {kMatGinv, DUAL ? kMatW : kMatGT, kMatGT, kMatZT};
 	const int which_plain[2] = {kMatGinv, kMatGT};
-	load_pass_mats(mats, P.x0, P.ng, DUAL ? which : which_plain, NM, lm);
+	load_pass_mats(mats, P.x0, P.ng, DUAL ? which : which_plain, NM, rev_mat_stride<DUAL>(), lm);
 	const AOperandMap<T, DUAL> amap = a_operand_map<T, DUAL>();
 	auto prefetch = [&](long tile, int b) {
 		const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
@@ -658,12 +656,12 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 			const GateLanes gl = gate_lanes<T>(gt);
 			const int ewv = sel4(gt.wv, wave);
 			// recovery: [[G^-1, 0], [-G^-1 Z G^-1, G^-1]]; adjoint: [[G^T, 0], [Z^T, G^T]]
-			const T* mj = reinterpret_cast<const T*>(lm + J * NM * 16);
+			const T* mj = reinterpret_cast<const T*>(lm + J * rev_mat_stride<DUAL>());
 			T au[4], aa[4];
 #pragma unroll
 			for (int kk = 0; kk < 4; kk++) {
 				au[kk] = mj[amap.aoff[kk]] * amap.afac[kk];
-				aa[kk] = mj[amap.aoff[kk] + (DUAL ? 64 : 32)] * amap.afac[kk];
+				aa[kk] = mj[amap.aoff[kk] + 2 * kMatPairOff * (DUAL ? 2 : 1)] * amap.afac[kk];
 			}
 			const bool adjoint = (J > 0) || !(flags & 8);
 			for (int g = 0; g < qpw; g += GS * gq) {
