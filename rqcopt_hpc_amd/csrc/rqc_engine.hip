@@ -440,7 +440,7 @@ static std::vector<Pass> plan_passes(const Circuit& C, int K, int gmax)
 // 256-byte LDS bank window differ in tile bit 0, so gates away from it read
 // their matrix-core groups without bank conflicts).  Column swizzle: a greedy
 // colouring of the pass's gate graph (tile bits as vertices) with three colours,
-// column masks h in {1, 2, 3} (FP64) or {2, 8, 10} (FP32), gives the four
+// column masks h in {1, 2, 3} (FP64) or {2, 4, 6} (FP32), gives the four
 // amplitude rows of every gate distinct bank slots in the reverse pass's
 // transposed stores.
 static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC, int lognc, int x0, int elem_bytes,
@@ -490,12 +490,9 @@ static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC,
 		}
 		col[t] = (c <= 3) ? c : 1;
 	}
-	// FP32 masks {2, 8, 10}: even (the two elements of a 16-byte slot stay adjacent) and
-	// without bit 2, so that the quads of a transposed-store phase (column bit 2 apart)
-	// and their four amplitudes take distinct banks
-	static const int kFp32Masks[4] = {0, 2, 8, 10};
+	const int shift = (elem_bytes == 16) ? 0 : 1;
 	for (int t = 0; t < K; t++) {
-		const int h = ((elem_bytes == 16) ? col[t] : kFp32Masks[col[t]]) & ((1 << logC) - 1);
+		const int h = (col[t] << shift) & ((1 << logC) - 1);
 		for (int i = 0; i < 4; i++) {
 			if ((h >> i) & 1) { a.swz[i] |= 1u << t; }
 		}
@@ -518,8 +515,17 @@ static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC,
 		GateTerms& g = gterms[x0 + j];
 		for (int i = 0; i < 4; i++) {
 			g.u[i] = quad_pos(0, i, lp[j], lq[j], a);
-			g.w[i] = quad_pos(4 * i, 0, lp[j], lq[j], a);
-			g.q4[i] = quad_pos(i, 0, lp[j], lq[j], a);
+			if (elem_bytes == 16) {
+				g.w[i] = quad_pos(4 * i, 0, lp[j], lq[j], a);
+				g.q4[i] = quad_pos(i, 0, lp[j], lq[j], a);
+			} else {
+				// FP32: matrix-core row q of a group is quad q with bits 0 and 2
+				// exchanged, so that the two quads of a transposed-store phase
+				// (rows v and 4 + v) differ in column bit 0 and, with the 64-byte
+				// state padding, the phase's 16 elements take all 16 bank slots
+				g.w[i] = quad_pos((i & 1) | ((i & 2) << 2), 0, lp[j], lq[j], a);
+				g.q4[i] = quad_pos(((i & 1) << 2) | (i & 2), 0, lp[j], lq[j], a);
+			}
 		}
 		for (int w = 0; w < kTileWaves; w++) { g.wv[w] = quad_pos(w * qpw, 0, lp[j], lq[j], a); }
 		for (int i = 0; i < 3; i++) { g.gbit[i] = (16 << i < qpw) ? quad_pos(16 << i, 0, lp[j], lq[j], a) : 0; }

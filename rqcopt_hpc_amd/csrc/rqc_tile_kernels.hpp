@@ -52,11 +52,11 @@ constexpr int kMaxTileGates = 8;
 // two states fall on different banks
 constexpr int kLdsPadBytes = 64;
 // The reverse pass stores a quad's transposed results into two states with
-// one instruction.  FP64 (8-byte stores, 128-byte bank window): the 64-byte
-// pad puts the second state's amplitudes on the other half of the window.
-// FP32 (4-byte stores, 32 lanes over 256 bytes): 128 bytes.
+// one instruction (8-byte stores: 16 lanes, 4-byte stores: 32 lanes, over a
+// 128-byte bank window); the 64-byte pad puts the second state's amplitudes on
+// the other half of the window.
 template <typename T>
-constexpr int lds_pad_rev() { return sizeof(T) == 8 ? 64 : 128; }
+constexpr int lds_pad_rev() { return 64; }
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
@@ -370,7 +370,11 @@ __device__ __forceinline__ void mfma_gate_apply(cx<T>* X, cx<T>* Y, const int* e
 {
 	GateOperands<T, GS> o;
 	gate_load<T, GS>(X, Y, eg, eg2, gl, o);
+	// matrix-core work at raised wave priority, so that the other waves' LDS
+	// traffic issues in its shadow
+	__builtin_amdgcn_s_setprio(1);
 	gate_mfma<T, GS>(a, o);
+	__builtin_amdgcn_s_setprio(0);
 	gate_store<T, GS>(X, Y, eg, eg2, gl, o);
 }
 
@@ -680,6 +684,9 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 				GateOperands<T, GS> os, oa;
 				gate_load<T, GS>(S, D, eg, eg2, gl, os);
 				gate_load<T, GS>(F, DF, eg, eg2, gl, oa);
+				// matrix-core work at raised wave priority, so that the other wave's
+				// LDS traffic issues in its shadow
+				__builtin_amdgcn_s_setprio(1);
 				gate_mfma_t<T, GS>(au, os);
 				gate_mfma_t<T, GS>(aa, oa);
 				gate_store_t<T, GS>(S, D, eg, eg2, gl, os);
@@ -706,6 +713,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 						for (int n = 0; n < GS; n++) { pend = mfma_step(oa.acc[n][t], os.acc[n][t], pend); }
 					}
 				}
+				__builtin_amdgcn_s_setprio(0);
 			}
 		};
 		static_assert(kMaxTileGates == 8, "one case per gate of a pass");
