@@ -575,8 +575,18 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_forward(
 // partial[((part * ng + j) * 256) + row * 16 + col], part = block * waves + wave:
 // the 16 x 16 outer-product sums of gate j (rows: phi-side component
 // 8 s + 2 r + p, columns: psi-side component).
+// Reverse-pass buffering.  FP32 dual sweeps hold one buffer of the four tile
+// states (a workgroup stores a tile's adjoints and loads the next tile into the
+// same slots), so that 3 workgroups fit a CU and the others' gates cover the
+// loads (measured 9 % faster than two double-buffered workgroups).  FP64 dual
+// sweeps need 185 registers and would spill at 3 waves per SIMD (measured
+// slower), and the plain sweeps (two states) reach 3 waves anyway: they
+// double-buffer, the next tile streaming in while the gates run.
+template <typename T, bool DUAL> constexpr int rev_buffers() { return (DUAL && sizeof(T) == 4) ? 1 : 2; }
+template <typename T, bool DUAL> constexpr int rev_waves_per_simd() { return (DUAL && sizeof(T) == 4) ? 3 : 2; }
+
 template <typename T, bool DUAL>
-__global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
+__global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void k_tile_reverse(
 	const cx<T>* __restrict__ psi_out, const cx<T>* __restrict__ dpsi_out,
 	const cx<T>* __restrict__ phi_out, const cx<T>* __restrict__ dphi_out,
 	cx<T>* __restrict__ phi_in, cx<T>* __restrict__ dphi_in,
@@ -612,7 +622,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 	tile_io_init(P, E, B, io);
 	// gate matrices G^-1, -G^-1 Z G^-1, G^T, Z^T of the pass after the two buffers
 	constexpr int NM = DUAL ? 4 : 2;
-	cx<T>* lm = reinterpret_cast<cx<T>*>(lds_raw + 2 * NS * sbytes);
+	cx<T>* lm = reinterpret_cast<cx<T>*>(lds_raw + rev_buffers<T, DUAL>() * NS * sbytes);
 	const int which[4] = {kMatGinv, DUAL ? kMatW : kMatGT, kMatGT, kMatZT};
 	const int which_plain[2] = {kMatGinv, kMatGT};
 	load_pass_mats(mats, P.x0, P.ng, DUAL ? which : which_plain, NM, rev_mat_stride<DUAL>(), lm);
@@ -635,14 +645,8 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 		tile_store(phi_in, state(b, NS / 2), base, io);
 		if (DUAL) { tile_store(dphi_in, state(b, 3), base, io); }
 	};
-	long tile = blockIdx.x;
-	if (tile < ntiles) { prefetch(tile, 0); }
-	int b = 0;
-	for (; tile < ntiles; tile += gridDim.x, b ^= 1) {
-		__builtin_amdgcn_s_waitcnt(0);
-		__syncthreads();
-		if (tile != (long)blockIdx.x) { store(tile - gridDim.x, b ^ 1); }
-		if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, b ^ 1); }
+	// the pass's gates on the tile in buffer b
+	auto run_tile = [&](int b) {
 		cx<T>* S = state(b, 0);
 		cx<T>* D = state(b, DUAL ? 1 : 0);
 		cx<T>* F = state(b, NS / 2);
@@ -662,7 +666,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 			// recovery: [[G^-1, 0], [-G^-1 Z G^-1, G^-1]]; adjoint: [[G^T, 0], [Z^T, G^T]]
 			const T* mj = reinterpret_cast<const T*>(lm + J * rev_mat_stride<DUAL>());
 			T au[4], aa[4];
-#pragma unroll
+	#pragma unroll
 			for (int kk = 0; kk < 4; kk++) {
 				au[kk] = mj[amap.aoff[kk]] * amap.afac[kk];
 				aa[kk] = mj[amap.aoff[kk] + 2 * kMatPairOff * (DUAL ? 2 : 1)] * amap.afac[kk];
@@ -670,7 +674,7 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 			const bool adjoint = (J > 0) || !(flags & 8);
 			for (int g = 0; g < qpw; g += GS * gq) {
 				int eg[GS], eg2[GS];
-#pragma unroll
+	#pragma unroll
 				for (int n = 0; n < GS; n++) {
 					eg[n] = ewv ^ group_term(gt, (g + n * gq) >> 4);
 					eg2[n] = DUAL ? eg[n] : ewv ^ group_term(gt, ((g + n * gq) >> 4) + 1);
@@ -694,22 +698,22 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 				if constexpr (sizeof(T) == 8) {
 					// FP64: group 0 accumulates onto acc[J], the others from zero
 					double4_t grp[GS];
-#pragma unroll
+	#pragma unroll
 					for (int n = 0; n < GS; n++) { grp[n] = (n == 0) ? acc[J] : double4_t{0, 0, 0, 0}; }
-#pragma unroll
+	#pragma unroll
 					for (int t = 0; t < 4; t++) {
-#pragma unroll
+	#pragma unroll
 						for (int n = 0; n < GS; n++) { grp[n] = mfma_step(oa.acc[n][t], os.acc[n][t], grp[n]); }
 					}
-#pragma unroll
+	#pragma unroll
 					for (int n = 1; n < GS; n++) { grp[0] += grp[n]; }
 					acc[J] = grp[0];
 				} else {
 					// FP32: one chain over the gate's quads of this tile into pend,
 					// added into the FP64 accumulator by the next gate (no wait here)
-#pragma unroll
+	#pragma unroll
 					for (int t = 0; t < 4; t++) {
-#pragma unroll
+	#pragma unroll
 						for (int n = 0; n < GS; n++) { pend = mfma_step(oa.acc[n][t], os.acc[n][t], pend); }
 					}
 				}
@@ -731,10 +735,32 @@ __global__ __launch_bounds__(kTileThreads, 2) void k_tile_reverse(
 			}
 		}
 		if constexpr (sizeof(T) == 4) { flush32(acc[0]); }
-	}
-	if ((long)blockIdx.x < ntiles) {
-		__syncthreads();
-		store(tile - gridDim.x, b ^ 1);
+	};
+	long tile = blockIdx.x;
+	if (tile < ntiles) { prefetch(tile, 0); }
+	if constexpr (rev_buffers<T, DUAL>() == 2) {
+		int b = 0;
+		for (; tile < ntiles; tile += gridDim.x, b ^= 1) {
+			__builtin_amdgcn_s_waitcnt(0);
+			__syncthreads();
+			if (tile != (long)blockIdx.x) { store(tile - gridDim.x, b ^ 1); }
+			if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, b ^ 1); }
+			run_tile(b);
+		}
+		if ((long)blockIdx.x < ntiles) {
+			__syncthreads();
+			store(tile - gridDim.x, b ^ 1);
+		}
+	} else {
+		for (; tile < ntiles; tile += gridDim.x) {
+			__builtin_amdgcn_s_waitcnt(0);
+			__syncthreads();
+			run_tile(0);
+			__syncthreads();
+			// each thread stores the slots that it then refills
+			store(tile, 0);
+			if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, 0); }
+		}
 	}
 	const long part = (long)blockIdx.x * kTileWaves + (threadIdx.x >> 6);
 #pragma unroll
