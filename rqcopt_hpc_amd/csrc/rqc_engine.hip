@@ -641,7 +641,8 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 	// the pass's gate matrices (forward G[, Z]; reverse G^-1, [W,] G^T[, Z^T])
 	const size_t fwd_mats = (size_t)kMaxTileGates * (dual ? fwd_mat_stride<true>() : fwd_mat_stride<false>()) * sizeof(cx<T>);
 	const size_t rev_mats = (size_t)kMaxTileGates * (dual ? rev_mat_stride<true>() : rev_mat_stride<false>()) * sizeof(cx<T>);
-	tp->lds_fwd = 2 * (size_t)(dual ? 2 : 1) * (E * sizeof(cx<T>) + kLdsPadBytes) + fwd_mats;
+	tp->lds_fwd = (size_t)(dual ? fwd_buffers<T, true>() : fwd_buffers<T, false>()) * (dual ? 2 : 1) *
+		(E * sizeof(cx<T>) + kLdsPadBytes) + fwd_mats;
 	tp->lds_rev = (size_t)(dual ? rev_buffers<T, true>() : rev_buffers<T, false>()) * (dual ? 4 : 2) *
 		(E * sizeof(cx<T>) + lds_pad_rev<T>()) + rev_mats;
 	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
