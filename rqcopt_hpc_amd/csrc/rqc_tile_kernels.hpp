@@ -718,18 +718,12 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 				gate_store_t<T, GS>(S, D, eg, eg2, gl, os);
 				if (adjoint) { gate_store_t<T, GS>(F, DF, eg, eg2, gl, oa); }
 				if constexpr (sizeof(T) == 8) {
-					// FP64: group 0 accumulates onto acc[J], the others from zero
-					double4_t grp[GS];
-	#pragma unroll
-					for (int n = 0; n < GS; n++) { grp[n] = (n == 0) ? acc[J] : double4_t{0, 0, 0, 0}; }
-	#pragma unroll
+					// FP64: one chain over the groups' steps accumulating onto acc[J]
+#pragma unroll
 					for (int t = 0; t < 4; t++) {
-	#pragma unroll
-						for (int n = 0; n < GS; n++) { grp[n] = mfma_step(oa.acc[n][t], os.acc[n][t], grp[n]); }
+#pragma unroll
+						for (int n = 0; n < GS; n++) { acc[J] = mfma_step(oa.acc[n][t], os.acc[n][t], acc[J]); }
 					}
-	#pragma unroll
-					for (int n = 1; n < GS; n++) { grp[0] += grp[n]; }
-					acc[J] = grp[0];
 				} else {
 					// FP32: one chain over the gate's quads of this tile into pend,
 					// added into the FP64 accumulator by the next gate (no wait here)
