@@ -19,10 +19,11 @@
 //   D = sum_kk M[:, 4kk:4kk+4] X[4kk:4kk+4, :].
 // Lane l holds quad (l & 15), amplitude r = l >> 4: its B operand of step kk
 // is component v = kk of that amplitude, its A operand M[l % 16][4 kk + l / 16]
-// (the host lays the four A values of every lane out in a table).  The gate
-// thus costs four registers per lane, and each state element is read and
-// written once per gate.  Without the tangent state (plain sweeps) the second
-// half of the components is a second quad (quad + 16) and M = [[G, 0], [0, G]].
+// (read from the pass's gate matrices in LDS through a per-lane map fixed for
+// the launch).  The gate thus costs four registers per lane, and each state
+// element is read and written once per gate.  Without the tangent state (plain
+// sweeps) the second half of the components is a second quad and
+// M = [[G, 0], [0, G]].
 //
 // Reverse pass (k_tile_reverse): reloads the pass-output checkpoint and walks
 // the pass's gates backwards, recovering the state before each gate with the
@@ -30,12 +31,17 @@
 // so that LDS holds only psi, dpsi, phi, dphi whatever the number of gates:
 //   gradient / HVP sums  grad_j += phi (x) psi_j,  hess_j += phi (x) dpsi_j + dphi (x) psi_j
 //   adjoint update       phi <- G_j^T phi,  dphi <- G_j^T dphi + Z_j^T phi
-// The engine uses it only for gates that are unitary to 1e-6 (then G^-1 has
-// norm 1 and the recovery adds rounding only).  The outer-product sums run on
-// the matrix cores as 16x16x4 steps over 4 quads, in FP64 (FP64 states) or in
-// FP32 per 16-quad group added into FP64 accumulators (FP32 states).  Partial
-// sums per (workgroup, wave, gate) are reduced in a fixed order by
-// k_tile_reduce: no atomics, bitwise reproducible.
+// (phi, dphi: the adjoint at the gate's output).  The engine uses it only for
+// gates that are unitary to 1e-6 (then G^-1 has norm 1 and the recovery adds
+// rounding only).  Both updates run transposed (state as the A operand), so
+// that their results are the operands of the outer products, which thus pair
+// the adjoint at the gate's input with the recovered state; the reduction maps
+// these sums back to the output adjoint with G^-T and (-G^-1 Z G^-1)^T.  The
+// outer-product sums run on the matrix cores as 16x16x4 steps over 4 quads, in
+// FP64 (FP64 states) or in FP32 per 16-quad group added into FP64 accumulators
+// (FP32 states).  Partial sums per (workgroup, wave, gate) are reduced in a
+// fixed order by k_tile_reduce_chunks and k_tile_reduce: no atomics, bitwise
+// reproducible.
 #pragma once
 
 #include <cstdint>
