@@ -1,8 +1,8 @@
 """Convert the reference's own HDF5 test fixtures into committed .npz files.
 
-Runs only in the build container (it reads test/data/*.hdf5 of the reference
-checkout and links the image's libhdf5 from /opt/conda).  The GPU box never
-runs it: tests load the committed tests/golden/fixtures/*.npz instead.
+Runs only where the reference checkout is present (it reads test/data/*.hdf5
+there and links the image's libhdf5 from /opt/conda).  The GPU tests never run
+it: they load the committed tests/golden/fixtures/*.npz instead.
 
 Each .npz holds the datasets of one fixture under their original names
 (complex arrays keep the trailing re/im axis the fixture uses, see

@@ -1,7 +1,7 @@
 """GPU parity of librqcopt_hip against the oracle and the reference fixtures.
 
 Every call goes through the C ABI (ctypes on librqcopt_hip.so).  Tolerances:
-FP64 results match the oracle to 1e-12 relative unless a
+FP64 results match the oracle to 1e-12 relative (the required bar) unless a
 tighter reference-test tolerance applies; the mixed-precision path is checked
 against the FP64 path with the error bounds stated in each test.
 """
