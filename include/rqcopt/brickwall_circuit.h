@@ -29,4 +29,11 @@ int brickwall_unitary_backward(const struct mat4x4 vlist[], int nlayers, const i
 int brickwall_unitary_backward_hessian(const struct mat4x4 vlist[], int nlayers, const int* perms[], const struct quantum_circuit_cache* cache,
 	const struct statevector* RQC_RESTRICT dpsi_out, struct statevector* RQC_RESTRICT dpsi, struct mat4x4 dvlist[], numeric* hess);
 
+/* reference src/brickwall_circuit.c:264-310: the placeholder states of every gate of
+   layer l (from the cached states), each carried through all later gates, summed:
+   psi_out (16 states) holds the derivative of the circuit output by the 16 entries of
+   layer l's gate */
+int apply_brickwall_unitary_gate_placeholder(const struct mat4x4 vlist[], int nlayers, const int* perms[], int l,
+	const struct quantum_circuit_cache* cache, struct statevector_array* RQC_RESTRICT psi_out);
+
 RQC_EXTERN_C_END

@@ -5,16 +5,13 @@ The product is the C-ABI shared library rqcopt_hpc_amd/librqcopt_hip.so
 (HIP kernels + host driver), declared in include/rqcopt/*.h with the
 reference's function names and argument orders (gate.h, statevector.h,
 quantum_circuit.h, brickwall_circuit.h, target.h, matrix.h, rng.h,
-trust_region.h, brickwall_opt.h) plus rqc_device.h.  This module binds it
-with ctypes and offers a numpy-level mirror of the same API:
-    apply_gate, apply_gate_backward, apply_gate_to_array,
-    apply_quantum_circuit, quantum_circuit_forward, quantum_circuit_backward,
-    quantum_circuit_gates_hessian_vector_product,
-    apply_brickwall_unitary, brickwall_unitary_forward, brickwall_unitary_backward,
-    circuit_unitary_target[_and_gradient|_and_projected_gradient|
-        _hessian_vector_product|_projected_hessian_vector_product],
-    brickwall_unitary_target[...same...],
-    optimize_brickwall_circuit_hvp, matrix and RNG helpers.
+trust_region.h, brickwall_opt.h, circuit_opt.h) plus rqc_device.h,
+rqc_batch.h and rqc_server.h.  This module binds it with ctypes and offers a
+numpy-level mirror of the same API: the gate, circuit and brickwall sweeps;
+the unitary targets with gradients, HVPs and Hessians (exact and sampled);
+the block-encoding targets; the trust-region optimisers; matrix and RNG
+helpers; device-resident targets; batches (Batch) and the request server
+(Server).  rqcopt_hpc_amd.rqcopt_hpc mirrors the reference's Python module.
 Errors: a negative return code of the C ABI raises RqcError(code).  There is
 no CPU fallback: without a HIP GPU the compute entry points raise.
 """
@@ -79,6 +76,10 @@ class CDouble2(ct.Structure):
     _fields_ = [("re", ct.c_double), ("im", ct.c_double)]
 
 
+class CFloat2(ct.Structure):
+    _fields_ = [("re", ct.c_float), ("im", ct.c_float)]
+
+
 LINEAR_FUNC = ct.CFUNCTYPE(ct.c_int, ct.POINTER(Statevector), ct.c_void_p, ct.POINTER(Statevector))
 HVP_FUNC = ct.CFUNCTYPE(None, ct.c_void_p, ct.c_void_p, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double))
 
@@ -94,7 +95,7 @@ EXPORTED = [
     "quantum_circuit_forward", "quantum_circuit_backward", "quantum_circuit_gates_hessian_vector_product",
     # brickwall_circuit.h
     "apply_brickwall_unitary", "brickwall_unitary_forward", "brickwall_unitary_backward",
-    "brickwall_unitary_backward_hessian",
+    "brickwall_unitary_backward_hessian", "apply_brickwall_unitary_gate_placeholder",
     # target.h
     "circuit_unitary_target", "circuit_unitary_target_and_gradient", "circuit_unitary_target_and_projected_gradient",
     "circuit_unitary_target_hessian_vector_product", "circuit_unitary_target_projected_hessian_vector_product",
@@ -113,8 +114,9 @@ EXPORTED = [
     "antisymm_to_real", "real_to_tangent", "tangent_to_real", "project_tangent", "multiply_matrices",
     "symmetric_triple_matrix_product", "inverse_matrix", "polar_factor", "retract",
     # rng.h
-    "pcg32_srandom_r", "pcg32_random_r", "pcg32x2_srandom_r", "pcg32x2_random_r", "seed_rng_state",
-    "rand_uint32", "rand_uint64", "randu", "randn", "crandn",
+    "pcg32_srandom_r", "pcg32_random_r", "pcg32_boundedrand_r", "pcg32x2_srandom_r", "pcg32x2_random_r",
+    "pcg32x2_boundedrand_r", "seed_rng_state", "rand_uint32", "rand_uint64", "rand_interval", "rand_choice", "randu",
+    "randuf", "randn", "randnf", "crandn", "crandnf",
     # trust_region.h, brickwall_opt.h
     "truncated_cg_hvp", "riemannian_trust_region_optimize_hvp", "optimize_brickwall_circuit_hvp",
     "truncated_cg_hmat", "riemannian_trust_region_optimize_hmat", "optimize_brickwall_circuit_hmat",
@@ -152,6 +154,16 @@ def lib():
         L.rand_uint32.restype = ct.c_uint32
         L.pcg32_random_r.restype = ct.c_uint32
         L.pcg32x2_random_r.restype = ct.c_uint64
+        L.pcg32_boundedrand_r.restype = ct.c_uint32
+        L.pcg32_boundedrand_r.argtypes = [ct.c_uint32, ct.c_void_p]
+        L.pcg32x2_boundedrand_r.restype = ct.c_uint64
+        L.pcg32x2_boundedrand_r.argtypes = [ct.c_uint64, ct.c_void_p]
+        L.rand_interval.restype = ct.c_uint64
+        L.rand_interval.argtypes = [ct.c_uint64, ct.c_void_p]
+        L.rand_choice.argtypes = [ct.c_uint64, ct.c_uint64, ct.c_void_p, ct.c_void_p]
+        L.randuf.restype = ct.c_float
+        L.randnf.restype = ct.c_float
+        L.crandnf.restype = CFloat2
         L.seed_rng_state.argtypes = [ct.c_uint64, ct.POINTER(RngState)]
         L.truncated_cg_hvp.restype = ct.c_bool
         L.rqc_last_column_block.restype = ct.c_long
@@ -487,6 +499,20 @@ def brickwall_unitary_backward_hessian(vlist, perms, cache_states, dpsi_out):
            lib().brickwall_unitary_backward_hessian(_p(vlist), len(vlist), rows, ct.byref(cache.c), ct.byref(_sv(dpsi_out)),
                                                     ct.byref(_sv(dpsi)), _p(dv), _p(hess)))
     return dpsi, dv, hess
+
+
+def apply_brickwall_unitary_gate_placeholder(vlist, perms, l, cache_states):
+    """Layer l's placeholder states carried through the later gates, summed: (2^L, 16)."""
+    vlist = _gates(vlist)
+    perms, rows = _perms(perms)
+    cache_states = _cplx(cache_states)
+    L = _nq(cache_states.shape[1])
+    cache = _Cache(L, len(vlist) * (L // 2), cache_states)
+    out = np.empty((1 << L, 16), np.complex128)
+    _check("apply_brickwall_unitary_gate_placeholder",
+           lib().apply_brickwall_unitary_gate_placeholder(_p(vlist), len(vlist), rows, l, ct.byref(cache.c),
+                                                          ct.byref(StatevectorArray(out.ctypes.data, L, 16))))
+    return out
 
 
 # --------------------------------------------------------------- targets

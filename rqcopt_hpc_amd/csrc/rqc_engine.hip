@@ -1408,6 +1408,56 @@ extern "C" void apply_gate_placeholder(const int i, const int j, const struct st
 	} catch (const std::exception& e) { fatal("apply_gate_placeholder", e); }
 }
 
+// Placeholder states of every gate of layer l (from the cached state before it),
+// each carried through all later gates of the brickwall and summed (reference
+// src/brickwall_circuit.c:264-310): psi_out[a * 16 + 4 r + c] is the derivative
+// of the circuit output at amplitude a by entry (r, c) of the layer's gate.
+extern "C" int apply_brickwall_unitary_gate_placeholder(const struct mat4x4 vlist[], int nlayers, const int* perms[], int l,
+	const struct quantum_circuit_cache* cache, struct statevector_array* RQC_RESTRICT psi_out)
+{
+	try {
+		const int L = psi_out->nqubits;
+		if (cache->nqubits != L || psi_out->nstates != 16) { throw Error(-3, "psi_out must hold 16 states of the cache's size"); }
+		if (l < 0 || l >= nlayers) { throw Error(-3, "layer index out of range"); }
+		Circuit C = make_brickwall(vlist, nullptr, nlayers, perms, L);
+		const int per = L / 2, ng = C.ngates;
+		if (cache->ngates != ng) { throw Error(-3, "cache holds " + std::to_string(cache->ngates) + " states, circuit has " + std::to_string(ng) + " gates"); }
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t n = (size_t)1 << L;
+		const size_t bytes = n * sizeof(zc);
+		Staging s(c);
+		const int st = s.scratch(bytes);
+		const int t0 = s.scratch(bytes * 16), t1 = s.scratch(bytes * 16);
+		const int o = s.add(psi_out->data, bytes * 16, false, true);
+		s.upload();
+		zc* out = s.dev<zc>(o);
+		zc* T[2] = {s.dev<zc>(t0), s.dev<zc>(t1)};
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_fill_zero<<<nblocks((long)n * 16), kBlock, 0, c.stream>>>(out, (long)n * 16, nullptr);
+		}
+		for (int k = 0; k < per; k++) {
+			const int g = l * per + k;
+			copy_in(c, s.dev<zc>(st), cache->psi_list[g].data, bytes);
+			{
+				LaunchTimer lt(c, RQC_KERNEL_OTHER);
+				k_gate_placeholder<<<nblocks((long)n * 16), kBlock, 0, c.stream>>>(s.dev<zc>(st), T[0],
+					wires_of(C.wi[g], C.wj[g], L), (long)n * 16);
+			}
+			int p = 0;
+			for (int h = g + 1; h < ng; h++) {
+				launch_forward(c, &C.gates[16 * (size_t)h], C.wi[h], C.wj[h], L, 16, T[p], T[1 - p]);
+				p = 1 - p;
+			}
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_add<<<nblocks((long)n * 16), kBlock, 0, c.stream>>>(out, T[p], out, (long)n * 16);
+		}
+		s.download();
+		return 0;
+	} catch (const std::exception& e) { return report_error("apply_brickwall_unitary_gate_placeholder", e); }
+}
+
 // ============================================================== C ABI: quantum_circuit.h
 int rqc::report_error(const char* where, const std::exception& e)
 {

@@ -174,6 +174,64 @@ extern "C" numeric crandn(struct rng_state* state)
 	return numeric(x / s, y / s);
 }
 
+// rejection below (2^32 - bound) mod bound, so that r mod bound is unbiased
+// (reference src/pcg_basic.c:58-89, the PCG authors' bounded draw)
+extern "C" uint32_t pcg32_boundedrand_r(const uint32_t bound, struct pcg32_random* rng)
+{
+	const uint32_t threshold = (uint32_t)(-bound) % bound;
+	for (;;) {
+		const uint32_t r = pcg32_random_r(rng);
+		if (r >= threshold) { return r % bound; }
+	}
+}
+
+extern "C" uint64_t pcg32x2_boundedrand_r(const uint64_t bound, struct pcg32x2_random* rng)
+{
+	const uint64_t threshold = (uint64_t)(-bound) % bound;
+	for (;;) {
+		const uint64_t r = pcg32x2_random_r(rng);
+		if (r >= threshold) { return r % bound; }
+	}
+}
+
+extern "C" uint64_t rand_interval(const uint64_t bound, struct rng_state* state)
+{
+	return pcg32x2_boundedrand_r(bound, &state->pcgstate);
+}
+
+// num_samples distinct values of [0, bound) by Floyd's algorithm (reference
+// src/rng.c:58-79): draw t in [0, j) for j = bound - num_samples + 1 ..
+// bound, and take j - 1 instead when t was already drawn
+extern "C" void rand_choice(const uint64_t bound, const uint64_t num_samples, struct rng_state* state, uint64_t* ret)
+{
+	for (uint64_t i = 0; i < num_samples && i < bound; i++) {
+		const uint64_t j = bound - num_samples + 1 + i;
+		const uint64_t t = rand_interval(j, state);
+		bool drawn = false;
+		for (uint64_t k = 0; k < i && !drawn; k++) { drawn = (ret[k] == t); }
+		ret[i] = drawn ? j - 1 : t;
+	}
+}
+
+extern "C" float randuf(struct rng_state* state) { return std::ldexp((float)rand_uint32(state), -32); }
+
+// single-precision Box-Muller, u1 = 0 mapped to 1
+extern "C" float randnf(struct rng_state* state)
+{
+	float u1 = randuf(state);
+	const float u2 = randuf(state);
+	if (u1 == 0) { u1 = 1; }
+	return std::sqrt(-2 * std::log(u1)) * std::sin((float)6.2831853071795864769 * u2);
+}
+
+extern "C" std::complex<float> crandnf(struct rng_state* state)
+{
+	const float x = randnf(state);
+	const float y = randnf(state);
+	const float s = std::sqrt(2.f);
+	return std::complex<float>(x / s, y / s);
+}
+
 // ============================================================== trust_region.h
 static double ddot(int n, const double* x, const double* y)
 {

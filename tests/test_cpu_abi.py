@@ -197,3 +197,45 @@ def test_truncated_cg_hmat_matches_fixture_and_oracle():
         z2, on2 = O.truncated_cg_hmat(g, H, radius)
         z3, on3 = R.truncated_cg_hvp(g, lambda v: H @ v, radius)
         assert on1 == on2 == on3 and np.max(np.abs(z1 - z2)) < 1e-12 and np.max(np.abs(z1 - z3)) < 1e-12
+
+
+REF_SUBSET = os.path.join(os.path.dirname(O.__file__), "_ref", "libref_subset.so")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SUBSET), reason="oracle/_ref not built (needs the reference checkout)")
+def test_rng_helpers_match_reference_bitwise():
+    """Bounded draws, Floyd sampling and the single-precision normals against the reference's own
+    src/rng.c and src/pcg_basic.c compiled in oracle/_ref: same seeds, same sequences, bitwise."""
+    P, Q = R.lib(), ct.CDLL(REF_SUBSET)
+    Q.rand_interval.restype = ct.c_uint64
+    Q.rand_interval.argtypes = [ct.c_uint64, ct.c_void_p]
+    Q.rand_choice.argtypes = [ct.c_uint64, ct.c_uint64, ct.c_void_p, ct.c_void_p]
+    Q.pcg32_boundedrand_r.restype = ct.c_uint32
+    Q.pcg32_boundedrand_r.argtypes = [ct.c_uint32, ct.c_void_p]
+    Q.pcg32x2_boundedrand_r.restype = ct.c_uint64
+    Q.pcg32x2_boundedrand_r.argtypes = [ct.c_uint64, ct.c_void_p]
+    Q.randuf.restype = ct.c_float
+    Q.randnf.restype = ct.c_float
+    Q.crandnf.restype = R.CFloat2
+    Q.seed_rng_state.argtypes = [ct.c_uint64, ct.POINTER(R.RngState)]
+    for seed in (0, 3, 42, 2 ** 40 + 1):
+        a, b = R.RngState(), R.RngState()
+        P.seed_rng_state(seed, ct.byref(a))
+        Q.seed_rng_state(seed, ct.byref(b))
+        for bound in (1, 2, 3, 7, 1000, 2 ** 33 + 5, 2 ** 63 + 11):
+            assert P.rand_interval(bound, ct.byref(a)) == Q.rand_interval(bound, ct.byref(b))
+        for bound in (1, 5, 2 ** 31 + 1):
+            assert (P.pcg32_boundedrand_r(bound, ct.byref(a.gen[0])) ==
+                    Q.pcg32_boundedrand_r(bound, ct.byref(b.gen[0])))
+        assert P.pcg32x2_boundedrand_r(12345, ct.byref(a)) == Q.pcg32x2_boundedrand_r(12345, ct.byref(b))
+        for _ in range(40):
+            assert P.randuf(ct.byref(a)) == Q.randuf(ct.byref(b))
+            assert P.randnf(ct.byref(a)) == Q.randnf(ct.byref(b))
+            x, y = P.crandnf(ct.byref(a)), Q.crandnf(ct.byref(b))
+            assert (x.re, x.im) == (y.re, y.im)
+        for bound, k in ((10, 10), (100, 7), (2 ** 40, 5), (1, 1)):
+            u = np.zeros(k, np.uint64)
+            v = np.zeros(k, np.uint64)
+            P.rand_choice(bound, k, ct.byref(a), u.ctypes.data)
+            Q.rand_choice(bound, k, ct.byref(b), v.ctypes.data)
+            assert np.array_equal(u, v) and len(set(u.tolist())) == k and int(u.max()) < bound

@@ -96,3 +96,26 @@ def test_hmat_optimiser_vs_oracle():
     f_o, v_o = O.optimize_brickwall_circuit_hmat(Uo, V0, P, L, 4)
     assert np.max(np.abs(f_iter - f_o) / np.maximum(1, np.abs(f_o))) < 1e-9 and np.max(np.abs(vopt - v_o)) < 1e-8
     assert f_iter[-1] < f_iter[0]
+
+
+def test_brickwall_gate_placeholder_vs_oracle_and_finite_difference():
+    """Layer placeholders (reference src/brickwall_circuit.c:264-310) against the oracle, and as
+    the derivative of the circuit output by the layer's gate entries."""
+    rng = np.random.default_rng(62)
+    L, nl = 6, 3
+    V = np.stack([crandn(rng, (4, 4)) for _ in range(nl)])
+    P = random_perms(rng, L, nl)
+    psi = crandn(rng, 1 << L)
+    _, cache = O.brickwall_unitary_forward(V, P, psi)
+    for l in range(nl):
+        out = R.apply_brickwall_unitary_gate_placeholder(V, P, l, cache)
+        ref = O.apply_brickwall_unitary_gate_placeholder(V, P, l, cache)
+        assert reldiff(out, ref) < TOL
+        # column 4 r + c is d(W psi) / dV_l[r, c], summed over the layer's gates (central difference)
+        D = np.zeros((nl, 4, 4), np.complex128)
+        D[l, 1, 2] = 1
+        eps = 1e-6
+        num = (O.apply_brickwall_unitary(V + eps * D, P, psi) - O.apply_brickwall_unitary(V - eps * D, P, psi)) / (2 * eps)
+        assert np.max(np.abs(out[:, 6] - num)) < 1e-7 * np.max(np.abs(num))
+    with pytest.raises(R.RqcError):
+        R.apply_brickwall_unitary_gate_placeholder(V, P, nl, cache)
