@@ -49,7 +49,7 @@ RQC_EXTERN_C_BEGIN
 int rqc_device_ufunc(const struct statevector* RQC_RESTRICT psi, void* fdata, struct statevector* RQC_RESTRICT psi_out);
 
 /* U given densely (row-major, 2^nqubits x 2^nqubits; host or device pointer): copied to device memory.
-   Device targets hold 1 .. 15 qubits (U up to 17 GB); larger ones return -3. */
+   Device targets hold 1 .. 15 qubits (U up to 17 GB); larger ones return -3 (untested sizes). */
 int rqc_target_create_dense(int nqubits, const numeric* U, struct rqc_target** target);
 /* U assembled once from a host callback (2^nqubits calls) and kept on the device */
 int rqc_target_create_from_ufunc(linear_func ufunc, void* udata, int nqubits, struct rqc_target** target);
