@@ -636,6 +636,9 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 		maxng = std::max(maxng, (int)p.gates.size());
 	}
 	const int E = 1 << (ts.K + ts.logC);
+	if ((size_t)E * sizeof(cx<T>) > (size_t)kMaxSlots * 16 * kTileThreads) {
+		throw Error(-4, "tile of " + std::to_string(E) + " elements exceeds the kernels' load slots");
+	}
 	const long ntiles = (pl.N >> ts.K) * (pl.B >> ts.logC);
 	// two buffers of (psi[, dpsi]) forward and (psi, [dpsi,] phi[, dphi]) reverse, then
 	// the pass's gate matrices (forward G[, Z]; reverse G^-1, [W,] G^T[, Z^T])

@@ -391,7 +391,7 @@ __device__ __forceinline__ void mfma_gate_apply(cx<T>* X, cx<T>* Y, const int* e
 // straight from HBM into LDS (global_load_lds_dwordx4: lane-contiguous slots
 // from the wave's base), so the next tile streams in while the current one is
 // computed; the lane picks the global element whose LDS position is its slot.
-constexpr int kMaxSlots = 8;   // slots per thread and state (E * sizeof(cx<T>) <= 32 KiB)
+constexpr int kMaxSlots = 4;   // slots per thread and state (E * sizeof(cx<T>) <= 16 KiB, checked by the host)
 
 template <typename T>
 struct TileIO
@@ -496,12 +496,12 @@ __device__ __forceinline__ void tile_store(cx<T>* __restrict__ dst, const cx<T>*
 // forward pass: psi -> (gates) psi, with DUAL also dpsi.  The input is
 // column block b0 of the identity (flags & 1) or `in`; dpsi is zero (flags & 2)
 // or `din`.
-// Forward-pass buffering.  FP32: one buffer (a workgroup stores a tile and loads
-// the next into the same slots) and up to 5 workgroups per CU, whose gates cover
-// each other's tile loads (measured 3 % faster).  FP64: two buffers, the next
-// tile streaming in while the gates run (at 5 waves per SIMD it spills).
-template <typename T, bool DUAL> constexpr int fwd_buffers() { return sizeof(T) == 4 ? 1 : 2; }
-template <typename T, bool DUAL> constexpr int fwd_waves_per_simd() { return sizeof(T) == 4 ? 5 : 1; }
+// Forward-pass buffering: one buffer (a workgroup stores a tile and loads the
+// next into the same slots) and 5 (FP64) or 6 (FP32) workgroups per CU, whose
+// gates cover each other's tile loads (measured slightly faster than two
+// double-buffered workgroups fewer).
+template <typename T, bool DUAL> constexpr int fwd_buffers() { return 1; }
+template <typename T, bool DUAL> constexpr int fwd_waves_per_simd() { return sizeof(T) == 4 ? 6 : 5; }
 
 template <typename T, bool DUAL>
 __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void k_tile_forward(

@@ -60,7 +60,8 @@ def test_fixtures():
     assert np.max(np.abs(H - d["H"])) < TOL * np.max(np.abs(d["H"])) and np.array_equal(H, H.T)
 
 
-@pytest.mark.parametrize("L,nl,unitary", [(6, 3, False), (8, 4, True), (10, 3, True), (10, 2, False)])
+@pytest.mark.parametrize("L,nl,unitary", [(2, 2, True), (4, 3, False), (6, 3, False), (8, 4, True), (10, 3, True),
+                                         (10, 2, False)])
 def test_vs_oracle_both_engines_and_device_h(L, nl, unitary):
     rng = np.random.default_rng(90 + L + nl)
     s = 1 << (L // 2)
