@@ -887,12 +887,37 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 // per-group sums of per-gate gradients / HVPs and the Riemannian projection
 // (reference src/target.c:526-535 and :650-671), one thread per group.
 // res: f, dv[16 ng], hz[16 ng] (complex), grad[16 ng], hvp[16 ng] (double)
+// fcorr (FP32 sweeps with a gradient): the sweeps applied the gates rounded to
+// complex float, fl(V), so the accumulated f is f(fl(V)).  f(V) = f(fl(V)) +
+// sum_e dv[e] (V[e] - fl(V)[e]) + O(|V - fl(V)|^2), with dv = df/dV (f is
+// holomorphic in the gates for the trace targets: fcorr 1 corrects the complex
+// f; the block-encoding f is real with df = Re sum dv dV: fcorr 2 corrects its
+// real part).  Every gate of a group is the group's gate vlist[l].
 __global__ void k_finalize(const zc* __restrict__ acc, int ngroups, int per, const zc* __restrict__ vlist,
-	const zc* __restrict__ vdirs, zc* __restrict__ res, int want_grad, int project, int want_hvp, const int* skip)
+	const zc* __restrict__ vdirs, zc* __restrict__ res, int want_grad, int project, int want_hvp, int fcorr,
+	const int* skip)
 {
 	if (skipped(skip)) { return; }
 	const int l = blockIdx.x * blockDim.x + threadIdx.x;
-	if (l == 0) { res[0] = acc[0]; }
+	if (l == 0) {
+		zc f = acc[0];
+		if (want_grad && fcorr != 0) {
+			zc c = {0, 0};
+			for (int g = 0; g < ngroups; g++) {
+				for (int e = 0; e < 16; e++) {
+					const zc v = vlist[16 * g + e];
+					const zc dv = {v.x - (double)(float)v.x, v.y - (double)(float)v.y};
+					zc s = {0, 0};
+					for (int k = 0; k < per; k++) { s += acc[1 + 32L * ((long)g * per + k) + e]; }
+					c.x += s.x * dv.x - s.y * dv.y;
+					c.y += s.x * dv.y + s.y * dv.x;
+				}
+			}
+			f.x += c.x;
+			if (fcorr == 1) { f.y += c.y; }
+		}
+		res[0] = f;
+	}
 	if (!want_grad || l >= ngroups) { return; }
 	zc* dv_out = res + 1;
 	zc* hz_out = dv_out + 16L * ngroups;
@@ -951,7 +976,8 @@ void enqueue_eval(const Context& dev, Slot& s, const EvalPlan& pl, const Circuit
 	}
 	LaunchTimer lt(s, RQC_KERNEL_OTHER);
 	k_finalize<<<std::max(1, (pl.ng + 63) / 64), 64, 0, s.stream>>>(acc, pl.ng, pl.per, vin, vin + 16 * (size_t)pl.ng, res,
-		pl.mode != Mode::Target ? 1 : 0, pl.project ? 1 : 0, pl.mode == Mode::Hvp ? 1 : 0, skip);
+		pl.mode != Mode::Target ? 1 : 0, pl.project ? 1 : 0, pl.mode == Mode::Hvp ? 1 : 0,
+		pl.fp32 ? (be != nullptr ? 2 : 1) : 0, skip);
 }
 
 void unpack_result(const EvalPlan& pl, const char* res, EvalResult& R)

@@ -138,3 +138,33 @@ def test_argument_errors():
     bad = R.HostUfunc(lambda x: (_ for _ in ()).throw(ValueError("boom")))
     with pytest.raises(R.RqcError):
         R.brickwall_blockenc_target(bad, V, random_perms(rng, 6, 2))
+
+
+@pytest.mark.parametrize("L,nl,unitary", [(12, 4, True), (6, 3, False)])
+def test_mixed_gate_rounding_correction(L, nl, unitary):
+    """The FP32 sweeps apply fl(V); the evaluations with a gradient add Re sum dv (V - fl(V)) to
+    the real block-encoding f.  Mixed runs on V and fl(V) share every FP32 operation, so their
+    difference is that term alone, and the FP64 path gives its true value f(V) - f(fl(V))."""
+    rng = np.random.default_rng(97 + L)
+    s = 1 << (L // 2)
+    V = np.stack([haar_unitary(rng) if unitary else crandn(rng, (4, 4)) for _ in range(nl)])
+    V32 = (V.real.astype(np.float32) + 1j * V.imag.astype(np.float32)).astype(np.complex128)
+    P = random_perms(rng, L, nl)
+    hd = R.DeviceTarget.dense(crandn(rng, (s, s)) / np.sqrt(s))
+    f64, _ = R.brickwall_blockenc_target_and_gradient(hd, V, P)
+    f64r, _ = R.brickwall_blockenc_target_and_gradient(hd, V32, P)
+    R.set_precision(R.PRECISION_MIXED)
+    try:
+        fc, dvc = R.brickwall_blockenc_target_and_gradient(hd, V, P)
+        assert R.last_sweep()[0] == (R.ENGINE_TILE_PASS if unitary else R.ENGINE_PER_GATE)
+        fu, _ = R.brickwall_blockenc_target_and_gradient(hd, V32, P)
+    finally:
+        R.set_precision(R.PRECISION_FP64)
+    exact = f64 - f64r
+    print(f"blockenc L={L}: gate-rounding term {abs(exact):.3e}, corrected error {abs(fc - f64):.3e}", flush=True)
+    # f reaches 1e5 for the non-unitary gates: a difference of two f values carries ~eps |f|
+    rnd = 1e-14 * max(1.0, abs(f64))
+    assert abs(exact) > 1e3 * rnd
+    assert abs((fc - fu) - exact) < 1e-3 * abs(exact) + rnd
+    assert abs((fc - fu) - np.real(np.sum(dvc * (V - V32)))) < rnd
+    hd.close()

@@ -306,6 +306,52 @@ def test_mixed_precision_error_bounds():
     assert reldiff(f32[1], f64[1]) < 1e-4 and reldiff(f32[2], f64[2]) < 1e-4
 
 
+def _round32(V):
+    return (V.real.astype(np.float32) + 1j * V.imag.astype(np.float32)).astype(np.complex128)
+
+
+@pytest.mark.parametrize("L,nl,engine", [(10, 5, "tile"), (8, 4, "gate")])
+def test_mixed_gate_rounding_correction(L, nl, engine):
+    """FP32 sweeps apply the gates rounded to complex float; the evaluations with a gradient add
+    the first-order term sum dv (V - fl(V)).  Mixed runs on V and on fl(V) share every FP32
+    operation (fl(fl(V)) = fl(V)), so their difference is that term alone; the FP64 path on the
+    two gate sets gives its true value f(V) - f(fl(V)).  A dropped term, a wrong sign or a
+    conjugated gradient fails the comparison by the size of the term itself."""
+    rng = np.random.default_rng(12 + L)
+    U = np.linalg.qr(crandn(rng, (1 << L, 1 << L)))[0]
+    T = R.DeviceTarget.dense(U)
+    V = np.stack([haar_unitary(rng) for _ in range(nl)])
+    V32 = _round32(V)
+    Z = crandn(rng, (nl, 4, 4))
+    P = brickwall_perms(L, nl)
+    if engine == "gate":
+        R.set_sweep_engine(R.ENGINE_PER_GATE)
+    try:
+        f64, _ = R.brickwall_unitary_target_and_gradient(T, V, P)
+        f64r, _ = R.brickwall_unitary_target_and_gradient(T, V32, P)
+        R.set_precision(R.PRECISION_MIXED)
+        try:
+            fc, dvc = R.brickwall_unitary_target_and_gradient(T, V, P)
+            fu, _ = R.brickwall_unitary_target_and_gradient(T, V32, P)
+            hc = R.brickwall_unitary_target_hessian_vector_product(T, V, Z, P)[0]
+            hu = R.brickwall_unitary_target_hessian_vector_product(T, V32, Z, P)[0]
+            ft = R.brickwall_unitary_target(T, V, P)
+        finally:
+            R.set_precision(R.PRECISION_FP64)
+    finally:
+        R.set_sweep_engine(R.ENGINE_TILE_PASS)
+    exact = f64 - f64r
+    print(f"L={L} {engine}: gate-rounding term {abs(exact):.3e}, corrected error {abs(fc - f64):.3e}, "
+          f"target-only (uncorrected) error {abs(ft - f64):.3e}", flush=True)
+    rnd = 1e-14 * max(1.0, abs(f64))                            # rounding of a difference of two f values
+    assert abs(exact) > 1e3 * rnd                                # the term is there to be checked
+    assert abs((fc - fu) - exact) < 1e-3 * abs(exact) + rnd
+    assert abs((hc - hu) - exact) < 1e-3 * abs(exact) + rnd
+    assert abs((fc - fu) - np.sum(dvc * (V - V32))) < rnd       # the term is sum dv (V - fl(V))
+    assert abs(fc - f64) / (1 << L) < 1e-6
+    T.close()
+
+
 def test_device_target_brickwall_and_ufunc():
     rng = np.random.default_rng(10)
     L, nl = 6, 3
