@@ -601,16 +601,38 @@ static int tile_gmax()
 // position terms (all fixed by the circuit structure and the column block)
 struct TilePlan
 {
-	TileShape ts;
+	TileShape ts;                   // reverse passes (and forward, unless tsf differs)
+	TileShape tsf;                  // forward passes
 	std::vector<Pass> passes;
 	std::vector<TileArgs> args;
+	std::vector<TileArgs> args_fwd;
 	std::vector<TileGates> gks;
 	std::vector<GateTerms> terms;   // per gate, in execution order
+	std::vector<GateTerms> terms_fwd;   // forward tiles' terms (when tsf differs from ts)
+	size_t fterms_off = 0;          // byte offset of the forward terms in the table's terms region
+	long ntiles_fwd = 0;
 	std::vector<int> order;         // circuit gate index of each execution slot
 	unsigned gfwd = 0, grev = 0;
 	size_t lds_fwd = 0, lds_rev = 0;
 	size_t partial_doubles = 0;
 };
+
+// Forward-pass tile: the forward pass holds half the states of the reverse pass
+// and no outer-product accumulators, so a tile of twice the elements keeps
+// enough workgroups per CU; its HBM rows are 256 bytes ('c': twice the columns)
+// or it spans one more row bit ('k').  The passes stay those planned for the
+// reverse tile: their gates lie in the forward tile's row bits either way.
+static TileShape forward_shape(const TileShape& ts, int L, long B, int elem)
+{
+	char mode = 'c';
+	if (const char* v = getenv("RQC_FWD_TILE")) { mode = v[0]; }
+	TileShape f = ts;
+	const bool fits = (size_t)(2L << (ts.K + ts.logC)) * elem <= (size_t)kMaxSlots * 16 * kTileThreads;
+	if (!fits) { return f; }
+	if (mode == 'c' && (2L << ts.logC) <= B) { f.logC++; }
+	else if (mode == 'k' && ts.K < L) { f.K++; }
+	return f;
+}
 
 template <typename T>
 static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C, EvalPlan& pl, const std::vector<Pass>& passes,
@@ -619,12 +641,31 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 	const bool dual = (pl.mode == Mode::Hvp);
 	auto tp = std::make_shared<TilePlan>();
 	tp->ts = ts;
+	tp->tsf = forward_shape(ts, C.L, pl.B, (int)sizeof(cx<T>));
+	const bool own_fwd = (tp->tsf.K != ts.K || tp->tsf.logC != ts.logC);
 	tp->passes = passes;
 	tp->terms.resize((size_t)C.ngates);
+	if (own_fwd) {
+		tp->terms_fwd.resize((size_t)C.ngates);
+		tp->fterms_off = ((size_t)C.ngates * sizeof(GateTerms) + 255) & ~(size_t)255;
+	}
 	int x = 0, maxng = 1;
 	for (const Pass& p : passes) {
+		if (own_fwd) {
+			Pass pf = p;
+			if (tp->tsf.K > ts.K) {
+				// one more row bit: the lowest one outside the pass's bits
+				int b = 0;
+				while (std::find(pf.rb.begin(), pf.rb.end(), b) != pf.rb.end()) { b++; }
+				pf.rb.push_back(b);
+				std::sort(pf.rb.begin(), pf.rb.end());
+			}
+			tp->args_fwd.push_back(make_tile_args(C, pf, tp->tsf.K, tp->tsf.logC, ilog2_or_neg(pl.B >> tp->tsf.logC), x,
+				(int)sizeof(cx<T>), tp->terms_fwd.data()));
+		}
 		tp->args.push_back(make_tile_args(C, p, ts.K, ts.logC, ilog2_or_neg(pl.B >> ts.logC), x, (int)sizeof(cx<T>),
 			tp->terms.data()));
+		if (!own_fwd) { tp->args_fwd.push_back(tp->args.back()); }
 		TileGates tg;
 		std::memset(&tg, 0, sizeof(tg));
 		for (size_t j = 0; j < p.gates.size(); j++) {
@@ -636,16 +677,21 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 		maxng = std::max(maxng, (int)p.gates.size());
 	}
 	const int E = 1 << (ts.K + ts.logC);
-	if ((size_t)E * sizeof(cx<T>) > (size_t)kMaxSlots * 16 * kTileThreads) {
-		throw Error(-4, "tile of " + std::to_string(E) + " elements exceeds the kernels' load slots");
+	const int Ef = 1 << (tp->tsf.K + tp->tsf.logC);
+	for (int e : {E, Ef}) {
+		if ((size_t)e * sizeof(cx<T>) > (size_t)kMaxSlots * 16 * kTileThreads) {
+			throw Error(-4, "tile of " + std::to_string(e) + " elements exceeds the kernels' load slots");
+		}
 	}
+	if (tp->tsf.K > C.L || (1L << tp->tsf.logC) > pl.B) { throw Error(-4, "forward tile exceeds the column block"); }
 	const long ntiles = (pl.N >> ts.K) * (pl.B >> ts.logC);
+	tp->ntiles_fwd = (pl.N >> tp->tsf.K) * (pl.B >> tp->tsf.logC);
 	// two buffers of (psi[, dpsi]) forward and (psi, [dpsi,] phi[, dphi]) reverse, then
 	// the pass's gate matrices (forward G[, Z]; reverse G^-1, [W,] G^T[, Z^T])
 	const size_t fwd_mats = (size_t)kMaxTileGates * (dual ? fwd_mat_stride<true>() : fwd_mat_stride<false>()) * sizeof(cx<T>);
 	const size_t rev_mats = (size_t)kMaxTileGates * (dual ? rev_mat_stride<true>() : rev_mat_stride<false>()) * sizeof(cx<T>);
 	tp->lds_fwd = (size_t)(dual ? fwd_buffers<T, true>() : fwd_buffers<T, false>()) * (dual ? 2 : 1) *
-		(E * sizeof(cx<T>) + kLdsPadBytes) + fwd_mats;
+		(Ef * sizeof(cx<T>) + kLdsPadBytes) + fwd_mats;
 	tp->lds_rev = (size_t)(dual ? rev_buffers<T, true>() : rev_buffers<T, false>()) * (dual ? 4 : 2) *
 		(E * sizeof(cx<T>) + lds_pad_rev<T>()) + rev_mats;
 	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
@@ -655,7 +701,7 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 	int occ_f = 0, occ_r = 0;
 	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, reinterpret_cast<const void*>(kf), kTileThreads, tp->lds_fwd));
 	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, reinterpret_cast<const void*>(kr), kTileThreads, tp->lds_rev));
-	tp->gfwd = (unsigned)std::min<long>(ntiles, (long)std::max(1, occ_f) * dev.cu_count);
+	tp->gfwd = (unsigned)std::min<long>(tp->ntiles_fwd, (long)std::max(1, occ_f) * dev.cu_count);
 	tp->grev = (unsigned)std::max<long>(1, std::min<long>(ntiles, (long)std::max(1, occ_r) * dev.cu_count));
 	tp->partial_doubles = (pl.mode == Mode::Target) ? 0 : (size_t)tp->grev * kTileWaves * maxng * 256;
 	return tp;
@@ -726,6 +772,7 @@ EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, boo
 	if (pl.tiles) {
 		pl.mats_bytes = align256((size_t)pl.G * kMats * 16 * elem);
 		pl.terms_bytes = align256((size_t)pl.G * sizeof(GateTerms));
+		if (!pl.tp->terms_fwd.empty()) { pl.terms_bytes += pl.tp->fterms_off; }   // the forward terms follow
 	}
 	pl.tab_bytes = pl.mats_bytes + pl.terms_bytes + 2 * 16 * (size_t)pl.ng * sizeof(zc);
 	if (pl.tiles && mode != Mode::Target) {
@@ -780,6 +827,9 @@ static void fill_tables(const EvalPlan& pl, const Circuit& C, char* host)
 			}
 		}
 		std::memcpy(host + pl.mats_bytes, tp.terms.data(), tp.terms.size() * sizeof(GateTerms));
+		if (!tp.terms_fwd.empty()) {
+			std::memcpy(host + pl.mats_bytes + tp.fterms_off, tp.terms_fwd.data(), tp.terms_fwd.size() * sizeof(GateTerms));
+		}
 	}
 	// gates of each group (its first gate) and directions, for the projection
 	zc* vin = reinterpret_cast<zc*>(host + pl.mats_bytes + pl.terms_bytes);
@@ -818,6 +868,7 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	const long ntiles = (N >> tp.ts.K) * (B >> tp.ts.logC);
 	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
 	auto* kr = dual ? k_tile_reverse<T, true> : k_tile_reverse<T, false>;
+	const GateTerms* fterms = reinterpret_cast<const GateTerms*>(reinterpret_cast<const char*>(gterms) + tp.fterms_off);
 	const zc* rmats = reinterpret_cast<const zc*>(reinterpret_cast<const char*>(mats) + pl.rmats_off);
 	double* mpart = reinterpret_cast<double*>(d_partial);   // grev * waves * ng * 256 doubles
 	double* cpart = mpart + tp.partial_doubles;             // chunk sums: kReduceChunks * ng * 256 doubles
@@ -840,7 +891,7 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 			cx<T>* out = (mode == Mode::Target) ? buf[p % 2] : PS(p + 1);
 			LaunchTimer lt(s, RQC_KERNEL_FORWARD, (double)(ns * (p > 0) + ns) * ab);
 			kf<<<tp.gfwd, kTileThreads, tp.lds_fwd, s.stream>>>(in, out, (dual && p > 0) ? DPS(p) : nullptr,
-				dual ? DPS(p + 1) : nullptr, tp.args[p], mats, gterms, B, b0, ntiles, flags, skip);
+				dual ? DPS(p + 1) : nullptr, tp.args_fwd[p], mats, fterms, B, b0, tp.ntiles_fwd, flags, skip);
 		}
 		cx<T>* psiG = (mode == Mode::Target) ? buf[(P - 1) % 2] : PS(P);
 		if (be != nullptr) {
