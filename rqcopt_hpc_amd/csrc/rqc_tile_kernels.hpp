@@ -605,11 +605,12 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void
 // the adjoint at the gate input, columns: that of the state before the gate).
 // Reverse-pass buffering.  FP32 dual sweeps hold one buffer of the four tile
 // states (a workgroup stores a tile's adjoints and loads the next tile into the
-// same slots), so that 3 workgroups fit a CU and the others' gates cover the
-// loads (measured 9 % faster than two double-buffered workgroups).  FP64 dual
-// sweeps need 185 registers and would spill at 3 waves per SIMD (measured
-// slower), and the plain sweeps (two states) reach 3 waves anyway: they
-// double-buffer, the next tile streaming in while the gates run.
+// same slots), and the other workgroups' gates cover the loads: with 1024-
+// element tiles 3 workgroups fit a CU (measured 9 % faster than two double-
+// buffered ones; the kernel fits 168 registers), with the default 2048-element
+// tiles 2.  FP64 dual sweeps need 183 registers and would spill at 3 waves per
+// SIMD (measured slower); they and the plain sweeps (two states) double-buffer,
+// the next tile streaming in while the gates run.
 template <typename T, bool DUAL> constexpr int rev_buffers() { return (DUAL && sizeof(T) == 4) ? 1 : 2; }
 template <typename T, bool DUAL> constexpr int rev_waves_per_simd() { return (DUAL && sizeof(T) == 4) ? 3 : 2; }
 
