@@ -120,6 +120,10 @@ template <bool DUAL> constexpr int rev_mat_stride() { return DUAL ? 4 * kMatPair
 // v = 2 s + p: lane l takes M[l % 16][4 kk + l / 16], i.e. element aoff[kk] (in
 // T units from X, in the layout above) times afac[kk] (the sign of -Im above
 // the diagonal, 0 in the zero block).  Fixed per lane for the kernel.
+// AMP_MAJOR orders the rows m = 4 r + v instead: the FP32 result rows of lane
+// l, 4 (l >> 4) + v, are then the four components of one amplitude, which
+// the forward pass stores as two 8-byte elements (as in FP64, whose result
+// rows (l >> 4) + 4 v are that amplitude's components with the default order).
 template <typename T, bool DUAL>
 struct AOperandMap
 {
@@ -127,11 +131,12 @@ struct AOperandMap
 	T afac[4];
 };
 
-template <typename T, bool DUAL>
+template <typename T, bool DUAL, bool AMP_MAJOR = false>
 __device__ __forceinline__ AOperandMap<T, DUAL> a_operand_map()
 {
 	const int lane = threadIdx.x & 63;
-	const int i = lane & 15, vo = i >> 2, ro = i & 3, ri = lane >> 4;
+	const int i = lane & 15, ri = lane >> 4;
+	const int vo = AMP_MAJOR ? (i & 3) : (i >> 2), ro = AMP_MAJOR ? (i >> 2) : (i & 3);
 	const int so = vo >> 1, po = vo & 1;
 	AOperandMap<T, DUAL> m;
 #pragma unroll
@@ -222,13 +227,12 @@ __device__ __forceinline__ int mfma_row(int lane, int v)
 }
 
 // per-lane LDS terms of a gate: gate application (lane: quad lane & 15,
-// amplitude lane >> 4; FP32 stores: amplitudes 0..3) and the transposed
+// amplitude lane >> 4, for its loads and stores) and the transposed
 // application's results (register v: amplitude lane & 3 of quad
 // (lane >> 4) + 4 v in FP64, 4 (lane >> 4) + v in FP32)
 struct GateLanes
 {
 	int ea;          // gate application, amplitude lane >> 4
-	int ew[4];       // amplitudes 0..3 of quad lane & 15
 	int et[4];       // transposed results, register v
 };
 
@@ -266,8 +270,6 @@ __device__ __forceinline__ GateLanes gate_lanes(const GateTerms& t)
 	const int lane = threadIdx.x & 63;
 	GateLanes g;
 	const int base = sel4_affine(t.q4, lane & 3) ^ sel4_affine(t.w, (lane >> 2) & 3);
-#pragma unroll
-	for (int v = 0; v < 4; v++) { g.ew[v] = base ^ t.u[v]; }
 	g.ea = base ^ sel4_affine(t.u, lane >> 4);
 	// quad q of a group: position q4[q & 3] ^ w[(q >> 2) & 3]
 	if constexpr (sizeof(T) == 8) {
@@ -319,20 +321,12 @@ template <typename T, int GS>
 __device__ __forceinline__ void gate_store(cx<T>* X, cx<T>* Y, const int* eg, const int* eg2, const GateLanes& gl,
 	const GateOperands<T, GS>& o)
 {
-	const int r = (threadIdx.x & 63) >> 4;
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		if constexpr (sizeof(T) == 8) {
-			// register v = component v (x.re, x.im, y.re, y.im) of amplitude r
-			X[eg[n] ^ gl.ea] = cx<T>{o.acc[n][0], o.acc[n][1]};
-			Y[eg2[n] ^ gl.ea] = cx<T>{o.acc[n][2], o.acc[n][3]};
-		} else {
-			// register v = component r (x.re, x.im, y.re, y.im) of amplitude v
-			T* base = reinterpret_cast<T*>(r < 2 ? X : Y);
-			const int e0 = (r < 2) ? eg[n] : eg2[n];
-#pragma unroll
-			for (int v = 0; v < 4; v++) { base[2 * (e0 ^ gl.ew[v]) + (r & 1)] = o.acc[n][v]; }
-		}
+		// register v = component v (x.re, x.im, y.re, y.im) of amplitude lane >> 4
+		// (FP64 rows (l >> 4) + 4 v; FP32 rows 4 (l >> 4) + v with the amplitude-major operand map)
+		X[eg[n] ^ gl.ea] = cx<T>{o.acc[n][0], o.acc[n][1]};
+		Y[eg2[n] ^ gl.ea] = cx<T>{o.acc[n][2], o.acc[n][3]};
 	}
 }
 
@@ -526,7 +520,8 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void
 	cx<T>* lm = reinterpret_cast<cx<T>*>(lds_raw + fwd_buffers<T, DUAL>() * NS * sbytes);
 	const int which[2] = {kMatG, kMatZ};
 	load_pass_mats(mats, P.x0, P.ng, which, DUAL ? 2 : 1, fwd_mat_stride<DUAL>(), lm);
-	const AOperandMap<T, DUAL> amap = a_operand_map<T, DUAL>();
+	// FP32: amplitude-major rows, so that a result lane holds one amplitude (gate_store)
+	const AOperandMap<T, DUAL> amap = a_operand_map<T, DUAL, sizeof(T) == 4>();
 	auto prefetch = [&](long tile, int b) {
 		const long rbase = deposit_bits(tile >> P.lognc, P.rmask);
 		const long c0 = (tile & cmask) << P.logC;
