@@ -465,10 +465,9 @@ static TileChoice tune_batch(int n, const rqc_batch_problem* pr, int precision, 
 	std::lock_guard<std::mutex> lk(mu);
 	auto it = cache.find(key);
 	if (it != cache.end()) { return it->second; }
-	// the default tile height (2^9 FP64, 2^11 FP32), the other height, and fewer gates per pass
-	const bool mixed = (precision == RQC_PRECISION_MIXED);
-	const int base = mixed ? 11 : 9;
-	const TileChoice cands[] = {{base, 8}, {mixed ? base - 1 : base + 1, 8}, {base, 6}, {base, 4}};
+	// the default tile size (2^10 FP64, 2^11 FP32), half of it, and fewer gates per pass
+	const int base = (precision == RQC_PRECISION_MIXED) ? 11 : 10;
+	const TileChoice cands[] = {{base, 8}, {base - 1, 8}, {base, 6}, {base, 4}};
 	TileChoice best{0, 0};
 	double best_ms = 0;
 	std::vector<int> kinds(n, RQC_EVAL_HVP);

@@ -575,16 +575,16 @@ template <typename T>
 static TileShape tile_shape(int L, long B, int logE_choice = 0)
 {
 	const int elem = (int)sizeof(cx<T>);
-	// FP64 512 elements (8 KiB per state): the reverse pass double-buffers four
-	// states, 64 KiB per workgroup, two workgroups per CU.  FP32 2048 elements
-	// (16 KiB) as 64 rows of 32 columns (256-byte HBM rows): one buffer of four
-	// states, two workgroups per CU (measured 3 % faster per eval than 1024).
-	int logE = (elem == 16) ? 9 : 11;
+	// 16 KiB per state, as 64 rows of 256 bytes (FP64 16 columns, FP32 32): the
+	// reverse pass holds one buffer of four states, two workgroups per CU.
+	// Measured per L=14 eval against 8 KiB tiles of 128-byte rows: FP64 7 %
+	// faster (8 KiB double-buffered), FP32 3 %.
+	int logE = (elem == 16) ? 10 : 11;
 	if (const char* v = getenv(elem == 16 ? "RQC_TILE_LOGE_FP64" : "RQC_TILE_LOGE_FP32")) { logE = atoi(v); }
 	if (logE_choice > 0) { logE = logE_choice; }
 	logE = std::max(9, std::min(logE, elem == 16 ? 10 : 11));   // the reverse pass's states fit 160 KiB
-	// rows of 128 bytes, except the FP32 2048-element tile (256 bytes)
-	int logCdef = (elem == 16) ? 3 : (logE == 11 ? 5 : 4);
+	// rows of 256 bytes for the 16 KiB tiles, 128 bytes for the 8 KiB ones
+	int logCdef = (elem == 16) ? (logE == 10 ? 4 : 3) : (logE == 11 ? 5 : 4);
 	if (const char* v = getenv(elem == 16 ? "RQC_TILE_LOGC_FP64" : "RQC_TILE_LOGC_FP32")) {
 		logCdef = std::max(1, std::min(atoi(v), 6));
 	}

@@ -598,15 +598,16 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void
 // partial[((part * ng + j) * 256) + row * 16 + col], part = block * waves + wave:
 // the 16 x 16 outer-product sums of gate j (rows: component 4 (2 s + p) + r of
 // the adjoint at the gate input, columns: that of the state before the gate).
-// Reverse-pass buffering.  FP32 dual sweeps hold one buffer of the four tile
-// states (a workgroup stores a tile's adjoints and loads the next tile into the
-// same slots), and the other workgroups' gates cover the loads: with 1024-
-// element tiles 3 workgroups fit a CU (measured 9 % faster than two double-
-// buffered ones; the kernel fits 168 registers), with the default 2048-element
-// tiles 2.  FP64 dual sweeps need 183 registers and would spill at 3 waves per
-// SIMD (measured slower); they and the plain sweeps (two states) double-buffer,
-// the next tile streaming in while the gates run.
-template <typename T, bool DUAL> constexpr int rev_buffers() { return (DUAL && sizeof(T) == 4) ? 1 : 2; }
+// Reverse-pass buffering.  Dual sweeps hold one buffer of the four tile states
+// (a workgroup stores a tile's adjoints and loads the next tile into the same
+// slots), and the other workgroups' gates cover the loads.  With the default
+// 16 KiB tiles (FP64 64 x 16, FP32 64 x 32) two workgroups fit a CU, and each
+// wave runs two independent 32-quad steps per gate: FP64 measured 9 % faster
+// than double-buffered 512-element tiles at the same two workgroups (FP64
+// needs 183 registers, so a third wave would spill).  FP32 with 1024-element
+// tiles fits 3 workgroups (168 registers).  The plain sweeps (two states)
+// double-buffer, the next tile streaming in while the gates run.
+template <typename T, bool DUAL> constexpr int rev_buffers() { return DUAL ? 1 : 2; }
 template <typename T, bool DUAL> constexpr int rev_waves_per_simd() { return (DUAL && sizeof(T) == 4) ? 3 : 2; }
 
 template <typename T, bool DUAL>
