@@ -83,13 +83,23 @@ def cpu_baseline(L, V, Z, P, Vt, Pt, cols_per_thread, threads):
     t0 = time.perf_counter()
     with mp.get_context("spawn").Pool(threads) as pool:
         per = pool.map(_cpu_worker, jobs)
-    wall = time.perf_counter() - t0
+        wall = time.perf_counter() - t0
+        # the oracle's gate kernels timed against the reference's own (oracle/_ref), one thread
+        try:
+            calib = pool.apply(_calib_worker, (L,))
+        except Exception as e:  # the reference subset is built only where the reference is present
+            calib = {"skipped": str(e)}
     sec_per_col = float(np.mean(per)) / cols_per_thread
     eval_s = sec_per_col * N / threads
     return {"value": 1.0 / eval_s, "unit": "evals/s", "cores": threads, "kind": "port",
             "sample": f"{cols_per_thread} of {N} basis columns per thread on {threads} threads "
                       f"({wall:.1f} s wall), scaled to a full grad+HVP eval",
-            "single_thread_eval_s": sec_per_col * N}
+            "single_thread_eval_s": sec_per_col * N, "gate_kernels_vs_reference": calib}
+
+
+def _calib_worker(L):
+    from oracle.calibrate import calibrate
+    return calibrate(L, reps=5)
 
 
 def run(args):

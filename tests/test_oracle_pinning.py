@@ -701,3 +701,14 @@ def test_blockenc_gradient_and_hessians_finite_differences():
         J = np.concatenate([E.T @ O.apply_brickwall_unitary_gate_placeholder(V, P, l, cache) for l in range(nl)], axis=1)
         G += J.conj().T @ J
     assert np.max(np.abs(G - h2)) < 1e-12 * np.max(np.abs(h2))
+
+
+@needs_ref
+def test_gate_kernel_calibration_runs():
+    """The CPU baseline's calibration script times the reference's gate kernels beside the
+    oracle's on the same inputs (speed only; the outputs are bitwise equal, tested above)."""
+    from oracle.calibrate import calibrate
+    res = calibrate(L=8, reps=2)
+    for name in ("apply_gate", "apply_gate_backward"):
+        k = res["kernels"][name]
+        assert k["reference_us_per_gate"] > 0 and k["oracle_us_per_gate"] > 0 and k["reference_over_oracle"] > 0
