@@ -314,6 +314,16 @@ def brickwall_hvp_columns(uf, vlist, vdirs, perms, L, b0, b1):
     return f[0], dv, h
 
 
+def brickwall_project_grad_hvp(vlist, vdirs, dvlist, hess_vdirs):
+    """Projection step of the brickwall projected HVP applied to column sums (reference src/target.c:645-671)."""
+    vlist, vdirs = _cplx(vlist), _cplx(vdirs)
+    dv, h = np.array(dvlist, np.complex128), np.array(hess_vdirs, np.complex128)
+    g = np.empty(16 * len(vlist))
+    hv = np.empty(16 * len(vlist))
+    lib().orc_brickwall_project_grad_hvp(_p(vlist), _p(vdirs), _i(len(vlist)), _p(dv), _p(h), _p(g), _p(hv))
+    return g, hv
+
+
 def brickwall_unitary_target_projected_hvp(uf, vlist, vdirs, perms, L):
     vlist, vdirs, perms = _cplx(vlist), _cplx(vdirs), _i32(perms)
     f = _fval()

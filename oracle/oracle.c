@@ -559,11 +559,18 @@ int orc_brickwall_unitary_target_projected_hessian_vector_product(orc_linear_fun
 	ocplx* hess = xalloc((size_t)nlayers * 16 * sizeof(ocplx));
 	int ret = orc_brickwall_unitary_target_hessian_vector_product(ufunc, udata, vlist, vdirs, nlayers, perms, L, fval, dvlist, hess);
 	if (ret < 0) { free(hess); free(dvlist); return ret; }
-	for (int i = 0; i < nlayers; i++) {
-		project_grad_hvp(&vlist[16*i], &vdirs[16*i], &dvlist[16*i], &hess[16*i], &grad_vec[16*i], &hvp_vec[16*i]);
-	}
+	orc_brickwall_project_grad_hvp(vlist, vdirs, nlayers, dvlist, hess, grad_vec, hvp_vec);
 	free(hess); free(dvlist);
 	return 0;
+}
+
+/* reference src/target.c:645-671 */
+void orc_brickwall_project_grad_hvp(const ocplx* vlist, const ocplx* vdirs, int nlayers, ocplx* dvlist,
+	ocplx* hess_vdirs, double* grad_vec, double* hvp_vec)
+{
+	for (int i = 0; i < nlayers; i++) {
+		project_grad_hvp(&vlist[16*i], &vdirs[16*i], &dvlist[16*i], &hess_vdirs[16*i], &grad_vec[16*i], &hvp_vec[16*i]);
+	}
 }
 
 /* ---------------------------------------------------------------------------

@@ -92,6 +92,11 @@ int orc_ufunc_brickwall(int nqubits, const ocplx* psi, void* udata, ocplx* psi_o
 int orc_brickwall_hvp_columns(orc_linear_func ufunc, void* udata, const ocplx* vlist, const ocplx* vdirs,
 	int nlayers, const int* perms, int L, long b0, long b1, ocplx* fval, ocplx* dvlist, ocplx* hess_vdirs);
 
+/* projection step of the brickwall projected HVP (reference src/target.c:645-671), applied to
+   column sums dvlist / hess_vdirs (overwritten) that orc_brickwall_hvp_columns accumulated */
+void orc_brickwall_project_grad_hvp(const ocplx* vlist, const ocplx* vdirs, int nlayers, ocplx* dvlist,
+	ocplx* hess_vdirs, double* grad_vec, double* hvp_vec);
+
 /* ---- 4x4 algebra: reference src/matrix.c ---- */
 void orc_multiply(const ocplx* a, const ocplx* b, ocplx* c);
 void orc_adjoint(const ocplx* a, ocplx* ah);

@@ -196,8 +196,15 @@ def _cplx(a):
     return np.ascontiguousarray(a, dtype=np.complex128)
 
 
+class _ArrayPtr(ct.c_void_p):
+    """A void* argument that holds a reference to the array it points into, so that a converted
+    temporary (e.g. _p(_cplx(x)) for a float or list input) lives until the C call returns."""
+
+
 def _p(a):
-    return ct.c_void_p(a.ctypes.data)
+    p = _ArrayPtr(a.ctypes.data)
+    p._keep = a
+    return p
 
 
 def _nq(n):
@@ -209,7 +216,9 @@ def _nq(n):
 
 def _sv(a):
     a = _cplx(a)
-    return Statevector(a.ctypes.data, _nq(a.size))
+    s = Statevector(a.ctypes.data, _nq(a.size))
+    s._keep = a
+    return s
 
 
 def _gates(g):
@@ -330,7 +339,7 @@ def _target_args(target):
 def apply_gate(V, i, j, psi):
     psi = _cplx(psi)
     out = np.empty_like(psi)
-    lib().apply_gate(_p(_cplx(V)), i, j, ct.byref(_sv(psi)), ct.byref(_sv(out)))
+    lib().apply_gate(_p(_mat4(V)), i, j, ct.byref(_sv(psi)), ct.byref(_sv(out)))
     return out
 
 
@@ -338,7 +347,7 @@ def apply_gate_backward(V, i, j, psi, dpsi_out):
     psi, dpsi_out = _cplx(psi), _cplx(dpsi_out)
     dpsi = np.empty_like(psi)
     dV = np.empty((4, 4), np.complex128)
-    lib().apply_gate_backward(_p(_cplx(V)), i, j, ct.byref(_sv(psi)), ct.byref(_sv(dpsi_out)), ct.byref(_sv(dpsi)), _p(dV))
+    lib().apply_gate_backward(_p(_mat4(V)), i, j, ct.byref(_sv(psi)), ct.byref(_sv(dpsi_out)), ct.byref(_sv(dpsi)), _p(dV))
     return dpsi, dV
 
 
@@ -349,7 +358,7 @@ def apply_gate_to_array(V, i, j, psi):
     L = _nq(psi.shape[0])
     a = StatevectorArray(psi.ctypes.data, L, psi.shape[1])
     b = StatevectorArray(out.ctypes.data, L, psi.shape[1])
-    lib().apply_gate_to_array(_p(_cplx(V)), i, j, ct.byref(a), ct.byref(b))
+    lib().apply_gate_to_array(_p(_mat4(V)), i, j, ct.byref(a), ct.byref(b))
     return out
 
 
@@ -473,7 +482,7 @@ def apply_gate_backward_array(V, i, j, psi, dpsi_out):
     L = _nq(psi.shape[0])
     dg = np.empty((psi.shape[1], 4, 4), np.complex128)
     a = StatevectorArray(psi.ctypes.data, L, psi.shape[1])
-    lib().apply_gate_backward_array(_p(_cplx(V)), i, j, ct.byref(a), ct.byref(_sv(dpsi_out)), _p(dg))
+    lib().apply_gate_backward_array(_p(_mat4(V)), i, j, ct.byref(a), ct.byref(_sv(dpsi_out)), _p(dg))
     return dg
 
 
@@ -1069,27 +1078,41 @@ class Server:
 
 
 # --------------------------------------------------------------- algebra, RNG
+def _mat4(a):
+    a = _cplx(a)
+    if a.shape != (4, 4):
+        raise ValueError(f"expected a 4x4 matrix, got shape {a.shape}")
+    return a
+
+
+def _real16(r):
+    r = np.ascontiguousarray(r, np.float64)
+    if r.size != 16:
+        raise ValueError(f"expected 16 real tangent coordinates, got {r.size}")
+    return r
+
+
 def _m1(name, a):
     out = np.empty((4, 4), np.complex128)
-    getattr(lib(), name)(_p(_cplx(a)), _p(out))
+    getattr(lib(), name)(_p(_mat4(a)), _p(out))
     return out
 
 
 def multiply_matrices(a, b):
     out = np.empty((4, 4), np.complex128)
-    lib().multiply_matrices(_p(_cplx(a)), _p(_cplx(b)), _p(out))
+    lib().multiply_matrices(_p(_mat4(a)), _p(_mat4(b)), _p(out))
     return out
 
 
 def project_tangent(u, z):
     out = np.empty((4, 4), np.complex128)
-    lib().project_tangent(_p(_cplx(u)), _p(_cplx(z)), _p(out))
+    lib().project_tangent(_p(_mat4(u)), _p(_mat4(z)), _p(out))
     return out
 
 
 def symmetric_triple_matrix_product(a, b, c):
     out = np.empty((4, 4), np.complex128)
-    lib().symmetric_triple_matrix_product(_p(_cplx(a)), _p(_cplx(b)), _p(_cplx(c)), _p(out))
+    lib().symmetric_triple_matrix_product(_p(_mat4(a)), _p(_mat4(b)), _p(_mat4(c)), _p(out))
     return out
 
 
@@ -1099,25 +1122,25 @@ def polar_factor(a):
 
 def inverse_matrix(a):
     out = np.empty((4, 4), np.complex128)
-    r = lib().inverse_matrix(_p(_cplx(a)), _p(out))
+    r = lib().inverse_matrix(_p(_mat4(a)), _p(out))
     return r, out
 
 
 def real_to_tangent(r, v):
     out = np.empty((4, 4), np.complex128)
-    lib().real_to_tangent(_p(np.ascontiguousarray(r, np.float64)), _p(_cplx(v)), _p(out))
+    lib().real_to_tangent(_p(_real16(r)), _p(_mat4(v)), _p(out))
     return out
 
 
 def tangent_to_real(v, z):
     out = np.empty(16)
-    lib().tangent_to_real(_p(_cplx(v)), _p(_cplx(z)), _p(out))
+    lib().tangent_to_real(_p(_mat4(v)), _p(_mat4(z)), _p(out))
     return out
 
 
 def retract(u, eta):
     out = np.empty((4, 4), np.complex128)
-    lib().retract(_p(_cplx(u)), _p(np.ascontiguousarray(eta, np.float64)), _p(out))
+    lib().retract(_p(_mat4(u)), _p(_real16(eta)), _p(out))
     return out
 
 

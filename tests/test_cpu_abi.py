@@ -137,6 +137,33 @@ def test_matrix_functions_match_oracle():
     assert R.inverse_matrix(np.zeros((4, 4)))[0] == -1
 
 
+def test_algebra_wrappers_accept_float_and_list_inputs():
+    """Inputs that are not complex128 arrays are converted to temporaries; the temporaries must stay
+    alive until the C call has read them (two converted operands of one call must not share a buffer)."""
+    a = np.arange(16.0).reshape(4, 4)
+    b = 2.0 * np.eye(4)
+    c = np.ones((4, 4))
+    ref = O.multiply(a.astype(np.complex128), b.astype(np.complex128))
+    for x, y in ((a, b), (a.tolist(), b.tolist()), (a.astype(np.float32), b), (a.T.copy().T, b)):
+        assert np.array_equal(R.multiply_matrices(x, y), ref)
+    ac, bc, cc = (m.astype(np.complex128) for m in (a, b, c))
+    assert np.array_equal(R.symmetric_triple_matrix_product(a, b.tolist(), c),
+                          R.symmetric_triple_matrix_product(ac, bc, cc))
+    u = haar_unitary(np.random.default_rng(5))
+    r = np.arange(16.0) / 16
+    assert np.array_equal(R.project_tangent(u, a), R.project_tangent(u, ac))
+    assert np.array_equal(R.real_to_tangent(r.tolist(), u), R.real_to_tangent(r, u))
+    assert np.array_equal(R.real_to_tangent(r.astype(np.float32), u), R.real_to_tangent(r.astype(np.float32)
+                                                                                       .astype(np.float64), u))
+    assert np.array_equal(R.tangent_to_real(u, a.tolist()), R.tangent_to_real(u, ac))
+    assert np.array_equal(R.retract(u, r.tolist()), R.retract(u, r))
+    assert np.array_equal(R.polar_factor(a + np.eye(4)), R.polar_factor(ac + np.eye(4)))
+    with pytest.raises(ValueError):
+        R.multiply_matrices(np.eye(3), b)
+    with pytest.raises(ValueError):
+        R.real_to_tangent(np.zeros(15), u)
+
+
 def test_rng_matches_oracle_bitwise():
     for seed in (0, 1, 42, 123456789):
         a, b = R.Rng(seed), O.Rng(seed)
