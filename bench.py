@@ -48,9 +48,18 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_k_tile_reverse_L14_d9
 
 
 def reverse_alg_flops(L, d):
-    """Algorithmic FP64 flops of the reverse tile sweep of one eval: per gate and amplitude, the state
-    recovery (psi, dpsi: 12 complex MACs), the adjoint update (phi, dphi: 12) and the gradient / HVP
-    outer products (phi x psi, phi x dpsi + dphi x psi: 12), 8 flops per complex MAC: 288 flops."""
+    """Algorithmic FP64 flops of the reverse sweep of one grad+HVP eval: per gate and amplitude, the
+    adjoint update (phi: 4 complex MACs, dphi = G^T dphi + Z^T phi: 8) and the gradient / HVP outer
+    products (phi x psi, phi x dpsi, dphi x psi: 4 each): 24 complex MACs, 8 flops each, 192 flops.
+    Together with the forward sweep's 12 (psi: 4, dpsi: 8) these are the 36 complex MACs per
+    amplitude and gate of the eval's canonical count (144 L d N^2 flops)."""
+    return 192.0 * (d * (L // 2)) * (1 << L) ** 2
+
+
+def reverse_issued_flops(L, d):
+    """Flops the reverse tile sweep issues: the algorithmic 24 complex MACs plus the recovery of the
+    gate-input states from the pass checkpoint (psi = G^-1 psi', dpsi = G^-1 dpsi' - G^-1 Z G^-1 psi':
+    12 more), which replaces storing every gate's input state: 288 flops per amplitude and gate."""
     return 288.0 * (d * (L // 2)) * (1 << L) ** 2
 
 
@@ -177,6 +186,7 @@ def run(args):
     engine_used, passes = R.last_sweep()
     rev_flops = reverse_alg_flops(L, d) if engine == "tile_pass" else None
     achieved_tf = rev_flops / (rev_ms * 1e-3) / 1e12 if rev_flops else None
+    issued_tf = reverse_issued_flops(L, d) / (rev_ms * 1e-3) / 1e12 if rev_flops else None
     hbm_gbs = rev_bytes / (rev_ms * 1e-3) / 1e9
     traffic = None
     if engine == "tile_pass" and os.path.exists(TRAFFIC_FILE) and (L, d) == (14, 9):
@@ -207,7 +217,9 @@ def run(args):
                    "L": L, "depth": d, "gates": G, "column_block": B, "sweep_engine": engine, "device": info},
         "roofline": ({"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                       "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,
-                      "alg_flops_per_launch": rev_flops / max(rev_n, 1)}
+                      "alg_flops_per_launch": rev_flops / max(rev_n, 1),
+                      "issued": {"flops_per_launch": reverse_issued_flops(L, d) / max(rev_n, 1),
+                                 "tflops": issued_tf, "frac": issued_tf / FP64_PEAK_TFLOPS}}
                      if achieved_tf else
                      {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": hbm_gbs / HBM_PEAK_GBS, "traffic": None}) | {
@@ -228,34 +240,58 @@ def run(args):
     if line["serving"] is not None:
         line["served_p99_ms"] = {k: line["serving"][k]["offered"]["p99_ms"] for k in ("fp64", "mixed")}
     if not args.no_cpu:
-        threads = min(args.cpu_threads, os.cpu_count() or 1)
+        threads = min(args.cpu_threads, len(os.sched_getaffinity(0)))
         line["cpu_baseline"] = cpu_baseline(L, V, Z, P, Vt, Pt, args.cpu_cols, threads)
+        line["cpu_baseline"]["host_cpus"] = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+                                             "used": threads}
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
 
 
+GOLDEN = os.path.join(ROOT, "tests", "golden", "fixtures", "config12_ising_L14_d9.npz")
+
+
+def _errors(f, g, h, f0, g0, h0):
+    return (abs(f - f0), abs(f - f0) / max(1.0, abs(f0)), np.max(np.abs(g - g0)), np.max(np.abs(g - g0)) / np.max(np.abs(g0)),
+            np.max(np.abs(h - h0)), np.max(np.abs(h - h0)) / np.max(np.abs(h0)))
+
+
 def mixed_error_sweep(args, R, T, P):
-    """Mixed-precision error of f, grad and HVP against the FP64 path for seeded gate sets
-    (the FP64 path matches the oracle to 1e-12, so it stands in for it at this size)."""
+    """Errors of f, grad and HVP for seeded gate sets against the oracle's FP64 values at L=14, depth 9
+    (golden vectors written by tests/golden/make_config12.py), for the FP64 and the mixed path; at
+    other sizes, or for seeds without a golden, against the FP64 path.  Relative errors are
+    |df| / max(1, |f|) and max-entry error over max entry."""
     from rqcopt_hpc_amd.workloads import config_inputs
 
-    rows = []
+    gold = {}
+    if (args.L, args.depth) == (14, 9) and os.path.exists(GOLDEN):
+        with np.load(GOLDEN, allow_pickle=False) as z:
+            gold = {int(s): (z["f"][k], z["grad"][k], z["hvp"][k]) for k, s in enumerate(z["seeds"])}
+    rows32, rows64, vs = [], [], []
     for seed in range(1, args.mixed_seeds + 1):
         V, _, _, Z = config_inputs(args.L, args.depth, seed)
         R.set_precision(R.PRECISION_FP64)
-        f64, g64, h64 = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
+        r64 = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
         R.set_precision(R.PRECISION_MIXED)
-        f32, g32, h32 = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
-        rows.append((abs(f32 - f64), abs(f32 - f64) / abs(f64), np.max(np.abs(g32 - g64)),
-                     np.max(np.abs(g32 - g64)) / np.max(np.abs(g64)), np.max(np.abs(h32 - h64)),
-                     np.max(np.abs(h32 - h64)) / np.max(np.abs(h64))))
+        r32 = R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
+        ref = gold.get(seed)
+        vs.append("oracle" if ref is not None else "fp64_gpu")
+        if ref is None:
+            ref = r64
+        else:
+            rows64.append(_errors(*r64, *ref))
+        rows32.append(_errors(*r32, *ref))
     R.set_precision(R.PRECISION_FP64)
-    a = np.array(rows)
     names = ["f_abs", "f_rel", "grad_abs", "grad_rel", "hvp_abs", "hvp_rel"]
-    out = {"seeds": args.mixed_seeds, "max": dict(zip(names, map(float, a.max(0)))),
-           "median": dict(zip(names, map(float, np.median(a, 0))))}
-    print(f"[bench] mixed-precision errors over {args.mixed_seeds} seeds: {out}", file=sys.stderr, flush=True)
+
+    def summary(rows):
+        a = np.array(rows)
+        return {"max": dict(zip(names, map(float, a.max(0)))), "median": dict(zip(names, map(float, np.median(a, 0))))}
+
+    out = {"seeds": args.mixed_seeds, "reference": {"oracle": vs.count("oracle"), "fp64_gpu": vs.count("fp64_gpu")},
+           "mixed": summary(rows32), "fp64": summary(rows64) if rows64 else None}
+    print(f"[bench] errors over {args.mixed_seeds} seeds: {out}", file=sys.stderr, flush=True)
     return out
 
 
@@ -339,7 +375,8 @@ def main():
     ap.add_argument("--L", type=int, default=14)
     ap.add_argument("--depth", type=int, default=9)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="host threads of the CPU baseline (the GPU host allots 16 cores per GPU)")
     ap.add_argument("--cpu-cols", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--engine", choices=["tile_pass", "per_gate"], default="tile_pass",

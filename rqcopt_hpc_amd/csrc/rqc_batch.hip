@@ -259,8 +259,9 @@ static void batch_step(Batch& bt, const int* kinds, const struct mat4x4* const* 
 		if (k < 0) { continue; }
 		BatchProblem& b = bt.probs[p];
 		set_gates(b, vlists[p], k == RQC_EVAL_HVP ? vdirs[p] : nullptr);
-		if (k != RQC_EVAL_TARGET && !gates_near_unitary(b.C)) {
-			throw Error(-3, "batched gradient and HVP evaluations need gates unitary to 1e-6 (problem " + std::to_string(p) + ")");
+		if (!sweep_unitary(b.C, kind_mode(k), bt.fp32)) {
+			throw Error(-3, "batched gradient, HVP and mixed-precision target evaluations need gates unitary to 1e-6 (problem " +
+				std::to_string(p) + ")");
 		}
 		fill_eval_tables(b.plan[k], b.C, bt.h_up + bt.off_tabs + b.tab_off);
 		present[k] = true;

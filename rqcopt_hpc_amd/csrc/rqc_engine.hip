@@ -562,6 +562,11 @@ bool gates_near_unitary(const Circuit& C)
 	return true;
 }
 
+bool sweep_unitary(const Circuit& C, Mode mode, bool fp32)
+{
+	return (mode == Mode::Target && !fp32) || gates_near_unitary(C);
+}
+
 // tile shape for element type T: 2^K rows x 2^logC columns, 128-byte rows
 struct TileShape
 {
@@ -722,6 +727,7 @@ EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, boo
 	const int gmax = (choice != nullptr && choice->gmax > 0) ? std::min(choice->gmax, (int)kMaxTileGates) : tile_gmax();
 	if (per < 1 || C.ngates % per != 0) { throw Error(-3, "gate groups must divide the gate count"); }
 	if (mode == Mode::Hvp && C.dirs.size() != C.gates.size()) { throw Error(-3, "an HVP evaluation needs gate directions"); }
+	if (fp32 && mode == Mode::Target) { mode = Mode::Grad; }   // gate-rounding correction of f needs the gradient
 	EvalPlan pl;
 	pl.mode = mode;
 	pl.fp32 = fp32;
@@ -1058,7 +1064,7 @@ static EvalResult eval_targets(const Circuit& C, linear_func ufunc, void* udata,
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
 	const zc* dU = resolve_target(c, ufunc, udata, C.L);
-	const bool unitary = (mode == Mode::Target) || gates_near_unitary(C);
+	const bool unitary = sweep_unitary(C, mode, c.precision == RQC_PRECISION_MIXED);
 	const EvalPlan pl = plan_eval(c, C, mode, per, c.precision == RQC_PRECISION_MIXED, project, unitary, c.ws_limit,
 		c.work.bytes);
 	std::vector<char> tab(pl.tab_bytes);
@@ -2187,7 +2193,7 @@ static EvalResult eval_blockenc(const Circuit& C, linear_func hfunc, void* hdata
 {
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
-	const bool unitary = (mode == Mode::Target) || gates_near_unitary(C);
+	const bool unitary = sweep_unitary(C, mode, c.precision == RQC_PRECISION_MIXED);
 	const EvalPlan pl = plan_eval(c, C, mode, C.L / 2, c.precision == RQC_PRECISION_MIXED, project, unitary, c.ws_limit,
 		c.work.bytes, nullptr, 1L << (C.L / 2));
 	const BlockEncInputs bi = prepare_blockenc(c, pl, hfunc, hdata, 0);
@@ -2310,7 +2316,7 @@ static EvalResult eval_sampled(const Circuit& C, linear_func ufunc, void* udata,
 	if (rng == nullptr) { throw Error(-3, "rng is NULL"); }
 	Context& c = ctx();
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
-	const bool unitary = (mode == Mode::Target) || gates_near_unitary(C);
+	const bool unitary = sweep_unitary(C, mode, c.precision == RQC_PRECISION_MIXED);
 	const EvalPlan pl = plan_eval(c, C, mode, per, c.precision == RQC_PRECISION_MIXED, project, unitary, c.ws_limit,
 		c.work.bytes, nullptr, nsamples);
 	const SampleBlocks sb = prepare_samples(c, pl, ufunc, udata, rng);

@@ -41,6 +41,10 @@ Circuit make_circuit(const struct mat4x4* gates, const struct mat4x4* dirs, int 
 Circuit make_brickwall(const struct mat4x4* vlist, const struct mat4x4* vdirs, int nlayers, const int* const* perms, int L);
 // every gate unitary to 1e-6 (the tile-pass reverse sweep recovers states through G^-1)
 bool gates_near_unitary(const Circuit& C);
+// the `unitary` argument of plan_eval for an evaluation of kind `mode`: a plain FP64 target
+// evaluation has no reverse sweep and needs no check; every other kind, and a mixed-precision
+// target evaluation (which runs the gradient sweep, see plan_eval), needs gates_near_unitary
+bool sweep_unitary(const Circuit& C, Mode mode, bool fp32);
 
 struct TilePlan;  // tile-pass specific part of a plan (passes, pass arguments, grids)
 
@@ -78,8 +82,12 @@ struct EvalPlan
 	double state_bytes() const { return (double)nst * N * B * (fp32 ? 8.0 : 16.0); }
 };
 
+// In mixed precision (fp32) a target evaluation is planned as a gradient evaluation: the FP32
+// sweeps apply the gates rounded to complex float, and f is corrected to first order for that
+// rounding with the gate gradient (k_finalize), so that every mixed-precision f, with or without
+// a gradient, is computed the same way (a trust-region ratio compares f values of both kinds).
 // `unitary`: the gates pass gates_near_unitary (the tile-pass engine is used for
-// gradient and HVP sweeps only then); `budget`: bytes for the state arrays
+// reverse sweeps only then; see sweep_unitary); `budget`: bytes for the state arrays
 // (0 = automatic, 85% of the free device memory plus `held`); `ncols` > 0: the
 // sweep's columns are that many given states instead of the 2^L basis columns
 // (the column block is then at most the power of two covering them)

@@ -16,6 +16,7 @@
 #include "rqcopt/trust_region.h"
 #include "rqcopt/brickwall_opt.h"
 #include "rqcopt/circuit_opt.h"
+#include "rqcopt/rqc_device.h"
 #include "rqcopt/target.h"
 #include "rqcopt/rng.h"
 
@@ -481,6 +482,34 @@ extern "C" int brickwall_blockenc_target_gradient_vector_hessian_matrix(linear_f
 // ============================================================== brickwall_opt.h
 namespace {
 
+// The last (x, f, projected gradient) of a mixed-precision target evaluation.  A mixed target
+// evaluation runs the gradient sweep (its f is corrected for the FP32 rounding of the gates
+// with the gradient), so the f callback keeps the gradient, and the gradient callback that the
+// trust region makes at the same x after an accepted step returns it instead of sweeping again.
+// The values are those a new evaluation would return (evaluations are deterministic).  FP64
+// runs keep the reference's calls: a target evaluation there has no gradient.
+struct GradMemo
+{
+	std::vector<double> x, grad;
+	double f = 0;
+	bool valid = false;
+
+	bool hit(const double* xq, size_t m, double* grad_out, double& f_out) const
+	{
+		if (!valid || x.size() != m || std::memcmp(x.data(), xq, m * sizeof(double)) != 0) { return false; }
+		std::memcpy(grad_out, grad.data(), grad.size() * sizeof(double));
+		f_out = f;
+		return true;
+	}
+	void keep(const double* xq, size_t m, const double* g, size_t n, double fv)
+	{
+		x.assign(xq, xq + m);
+		grad.assign(g, g + n);
+		f = fv;
+		valid = true;
+	}
+};
+
 struct BrickwallData
 {
 	linear_func ufunc;
@@ -488,27 +517,38 @@ struct BrickwallData
 	const int** perms;
 	int nlayers;
 	int nqubits;
+	GradMemo memo;
 };
 
-// wrappers of reference src/brickwall_opt.c:22-79
-double bw_f(const double* x, void* fdata)
-{
-	auto* d = static_cast<BrickwallData*>(fdata);
-	numeric f;
-	if (brickwall_unitary_target(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x), d->nlayers, d->nqubits, d->perms, &f) < 0) {
-		fprintf(stderr, "target function evaluation failed internally\n");
-		return -1;
-	}
-	return f.real();
-}
+bool mixed_precision() { return rqc_get_precision() == RQC_PRECISION_MIXED; }
 
+// wrappers of reference src/brickwall_opt.c:22-79
 double bw_f_deriv(const double* RQC_RESTRICT x, void* fdata, double* RQC_RESTRICT grad)
 {
 	auto* d = static_cast<BrickwallData*>(fdata);
+	const size_t m = 32 * (size_t)d->nlayers, n = 16 * (size_t)d->nlayers;
+	double fm;
+	if (mixed_precision() && d->memo.hit(x, m, grad, fm)) { return fm; }
 	numeric f;
 	if (brickwall_unitary_target_and_projected_gradient(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x), d->nlayers,
 			d->nqubits, d->perms, &f, grad) < 0) {
 		fprintf(stderr, "target function and derivative evaluation failed internally\n");
+		return -1;
+	}
+	if (mixed_precision()) { d->memo.keep(x, m, grad, n, f.real()); }
+	return f.real();
+}
+
+double bw_f(const double* x, void* fdata)
+{
+	auto* d = static_cast<BrickwallData*>(fdata);
+	if (mixed_precision()) {
+		std::vector<double> grad(16 * (size_t)d->nlayers);
+		return bw_f_deriv(x, fdata, grad.data());
+	}
+	numeric f;
+	if (brickwall_unitary_target(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x), d->nlayers, d->nqubits, d->perms, &f) < 0) {
+		fprintf(stderr, "target function evaluation failed internally\n");
 		return -1;
 	}
 	return f.real();
@@ -555,7 +595,7 @@ extern "C" void optimize_brickwall_circuit_hvp(linear_func ufunc, void* udata,
 	const struct mat4x4 vlist_start[], const int nlayers, const int nqubits, const int* perms[],
 	struct rtr_params* params, const int niter, double* f_iter, struct mat4x4 vlist_opt[])
 {
-	BrickwallData fdata{ufunc, udata, perms, nlayers, nqubits};
+	BrickwallData fdata{ufunc, udata, perms, nlayers, nqubits, {}};
 	params->g_func = nullptr;
 	params->g_data = nullptr;
 	params->g_iter = nullptr;
@@ -569,7 +609,7 @@ extern "C" void optimize_brickwall_circuit_hmat(linear_func ufunc, void* udata,
 	const struct mat4x4 vlist_start[], const int nlayers, const int nqubits, const int* perms[],
 	struct rtr_params* params, const int niter, double* f_iter, struct mat4x4 vlist_opt[])
 {
-	BrickwallData fdata{ufunc, udata, perms, nlayers, nqubits};
+	BrickwallData fdata{ufunc, udata, perms, nlayers, nqubits, {}};
 	params->g_func = nullptr;
 	params->g_data = nullptr;
 	params->g_iter = nullptr;
@@ -589,27 +629,36 @@ struct CircuitData
 	const int* wires;
 	int ngates;
 	int nqubits;
+	GradMemo memo;
 };
 
-// wrappers of reference src/circuit_opt.c:22-79
-double qc_f(const double* x, void* fdata)
-{
-	auto* d = static_cast<CircuitData*>(fdata);
-	numeric f;
-	if (circuit_unitary_target(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x), d->ngates, d->wires, d->nqubits, &f) < 0) {
-		fprintf(stderr, "target function evaluation failed internally\n");
-		return -1;
-	}
-	return f.real();
-}
-
+// wrappers of reference src/circuit_opt.c:22-79 (mixed precision: as bw_f / bw_f_deriv)
 double qc_f_deriv(const double* RQC_RESTRICT x, void* fdata, double* RQC_RESTRICT grad)
 {
 	auto* d = static_cast<CircuitData*>(fdata);
+	const size_t m = 32 * (size_t)d->ngates, n = 16 * (size_t)d->ngates;
+	double fm;
+	if (mixed_precision() && d->memo.hit(x, m, grad, fm)) { return fm; }
 	numeric f;
 	if (circuit_unitary_target_and_projected_gradient(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x), d->ngates,
 			d->wires, d->nqubits, &f, grad) < 0) {
 		fprintf(stderr, "target function and derivative evaluation failed internally\n");
+		return -1;
+	}
+	if (mixed_precision()) { d->memo.keep(x, m, grad, n, f.real()); }
+	return f.real();
+}
+
+double qc_f(const double* x, void* fdata)
+{
+	auto* d = static_cast<CircuitData*>(fdata);
+	if (mixed_precision()) {
+		std::vector<double> grad(16 * (size_t)d->ngates);
+		return qc_f_deriv(x, fdata, grad.data());
+	}
+	numeric f;
+	if (circuit_unitary_target(d->ufunc, d->udata, reinterpret_cast<const struct mat4x4*>(x), d->ngates, d->wires, d->nqubits, &f) < 0) {
+		fprintf(stderr, "target function evaluation failed internally\n");
 		return -1;
 	}
 	return f.real();
@@ -634,7 +683,7 @@ void qc_f_hvp(const double* RQC_RESTRICT x, void* fdata, double* RQC_RESTRICT ve
 extern "C" void optimize_quantum_circuit(linear_func ufunc, void* udata, const struct mat4x4 gates_start[], const int ngates,
 	const int nqubits, const int wires[], struct rtr_params* params, const int niter, double* f_iter, struct mat4x4 gates_opt[])
 {
-	CircuitData fdata{ufunc, udata, wires, ngates, nqubits};
+	CircuitData fdata{ufunc, udata, wires, ngates, nqubits, {}};
 	params->g_func = nullptr;
 	params->g_data = nullptr;
 	params->g_iter = nullptr;

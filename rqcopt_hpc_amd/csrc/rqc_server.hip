@@ -425,7 +425,7 @@ extern "C" int rqc_server_submit(struct rqc_server* server, struct rqc_session* 
 				}
 			}
 		}
-		const bool unitary = (kind == RQC_EVAL_TARGET) || gates_near_unitary(r->C);
+		const bool unitary = sweep_unitary(r->C, kind == RQC_EVAL_TARGET ? Mode::Target : Mode::Grad, s.fp32);
 		r->plan = session_plan(s, kind, unitary);
 		r->fval = fval;
 		r->grad = grad;

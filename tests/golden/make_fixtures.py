@@ -8,7 +8,7 @@ Each .npz holds the datasets of one fixture under their original names
 (complex arrays keep the trailing re/im axis the fixture uses, see
 test/io_util.py:4-8 of the reference).
 
-    python tests/golden/make_fixtures.py
+    python tests/golden/make_fixtures.py [name ...]
 """
 import os
 import subprocess
@@ -18,6 +18,7 @@ import tempfile
 import numpy as np
 
 REF_DATA = "/root/reference/test/data"
+REF_EXAMPLES = "/root/reference/examples"
 HERE = os.path.dirname(os.path.abspath(__file__))
 OUT = os.path.join(HERE, "fixtures")
 
@@ -58,6 +59,19 @@ FIXTURES = [
     "test_brickwall_blockenc_target_gradient_vector_hessian_matrix_cplx",
 ]
 
+# run outputs of the reference's example programs (inputs and the 10-iteration trust-region
+# trajectories they wrote), stored as example_<file name>; they pin the optimisers end to end:
+#   ising1d_dynamics_opt.c           optimize_brickwall_circuit_hmat, L=6, 3 layers
+#   ising1d_dynamics_opt_sampling.c  optimize_brickwall_circuit_hmat_sampling, 16 samples, seed 42
+#   molecular_dynamics_opt.c         optimize_quantum_circuit (RTR-HVP), 5 qubits
+EXAMPLES = [
+    "ising1d/ising1d_dynamics_opt_n3_init",
+    "ising1d/ising1d_dynamics_opt_n3",
+    "ising1d/ising1d_dynamics_opt_sampling_n3",
+    "molecular/molecular_dynamics_opt_init",
+    "molecular/molecular_dynamics_opt",
+]
+
 
 def build_converter(workdir):
     exe = os.path.join(workdir, "h5_to_raw")
@@ -68,8 +82,8 @@ def build_converter(workdir):
     return exe
 
 
-def convert(exe, name, workdir):
-    src = os.path.join(REF_DATA, name + ".hdf5")
+def convert(exe, name, workdir, src=None):
+    src = src or os.path.join(REF_DATA, name + ".hdf5")
     d = os.path.join(workdir, name)
     os.makedirs(d, exist_ok=True)
     out = subprocess.check_output([exe, src, d], text=True)
@@ -83,16 +97,21 @@ def convert(exe, name, workdir):
     return arrays
 
 
-def main():
+def main(only=None):
+    """Convert every fixture, or only those named (test fixture names or example_<file name>)."""
     if not os.path.isdir(REF_DATA):
         sys.exit("reference fixtures not present; nothing to convert")
     os.makedirs(OUT, exist_ok=True)
+    jobs = [(name, None) for name in FIXTURES]
+    jobs += [("example_" + os.path.basename(e), os.path.join(REF_EXAMPLES, e + ".hdf5")) for e in EXAMPLES]
+    if only:
+        jobs = [j for j in jobs if j[0] in only]
     with tempfile.TemporaryDirectory() as work:
         exe = build_converter(work)
-        for name in FIXTURES:
-            arrays = convert(exe, name, work)
+        for name, src in jobs:
+            arrays = convert(exe, name, work, src)
             print(name, {k: v.shape for k, v in arrays.items()})
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

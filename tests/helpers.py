@@ -123,3 +123,28 @@ def reldiff(x, y):
     x = np.asarray(x)
     y = np.asarray(y)
     return float(np.max(np.abs(x - y)) / max(np.max(np.abs(y)), 1e-300))
+
+
+# ----------------------------------------------------------- reference example runs
+def example_ising1d():
+    """Inputs and outputs of the reference's examples/ising1d runs (L=6, 3 brickwall layers):
+    (U, Vlist_start, perms, hmat run {f_iter, Vlist}, sampled hmat run {f_iter, Vlist, f_ref})."""
+    a = fixture("example_ising1d_dynamics_opt_n3_init")
+    P = np.stack([np.asarray(a[f"perm{i}"], dtype=np.int32) for i in range(3)])
+    return cplx(a["expiH"]), cplx(a["Vlist_start"]), P, fixture("example_ising1d_dynamics_opt_n3"), \
+        fixture("example_ising1d_dynamics_opt_sampling_n3")
+
+
+def example_molecular():
+    """Inputs and outputs of the reference's examples/molecular run (5 qubits, 18 gates):
+    (U, gates_start, wires, run {f_iter, gates})."""
+    a = fixture("example_molecular_dynamics_opt_init")
+    return cplx(a["expiH"]), cplx(a["gates_start"]), np.asarray(a["wires"], dtype=np.int32), \
+        fixture("example_molecular_dynamics_opt")
+
+
+# The sampled run redraws its 16 Haar states per evaluation; its trajectory agrees to rounding
+# through iteration 7, and from iteration 8 on moves by up to ~1e-4 under perturbations of U at
+# the level of one unit in the last place (the reference applies U with BLAS zgemv, whose
+# summation order is its own), so later iterates are compared at that level.
+SAMPLED_EXACT_ITERS = 8

@@ -110,6 +110,30 @@ def test_headers_compile_as_c_and_link():
     assert abs(float(out[2]) - u[0, 0].real) < 1e-15
 
 
+REF_SRC = "/root/reference/src"
+DROPIN = [os.path.join(ROOT, "oracle", "_ref", n) for n in ("libdropin_opt.so", "libdropin_opt_refh.so")]
+
+
+def test_reference_optimiser_glue_links_against_library():
+    """The reference's own callers of the path (src/brickwall_opt.c and src/circuit_opt.c, compiled
+    unmodified where they lie, once with this library's headers and once with the reference's) link
+    against librqcopt_hip.so with --no-undefined, and every symbol they import is one the library
+    declares.  Rebuilt here when the reference checkout is present (oracle/Makefile, target dropin)."""
+    if os.path.isdir(REF_SRC):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "-B", "dropin"])
+    elif not all(os.path.exists(p) for p in DROPIN):
+        pytest.skip("reference checkout absent and the drop-in libraries were not built")
+    for so in DROPIN:
+        out = subprocess.check_output(["nm", "-D", "--undefined-only", so], text=True)
+        imported = {ln.split()[-1] for ln in out.splitlines() if ln.split() and ln.split()[0] == "U"}
+        imported = {s for s in imported if "@" not in s}   # libc symbols carry a version
+        assert imported and imported <= set(R.EXPORTED), sorted(imported - set(R.EXPORTED))
+        d = ct.CDLL(so)
+        for name in ("optimize_brickwall_circuit_hvp", "optimize_brickwall_circuit_hmat",
+                     "optimize_brickwall_circuit_hmat_sampling", "optimize_quantum_circuit"):
+            assert hasattr(d, name)
+
+
 def test_headers_compile_as_cpp():
     with tempfile.TemporaryDirectory() as d:
         src = os.path.join(d, "caller.cpp")

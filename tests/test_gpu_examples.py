@@ -1,0 +1,78 @@
+"""The reference's example runs, reproduced through the library's C ABI on the GPU.
+
+The reference ships the inputs and the 10-iteration trust-region trajectories of three example
+programs (examples/ising1d/ising1d_dynamics_opt.c, ising1d_dynamics_opt_sampling.c and
+examples/molecular/molecular_dynamics_opt.c), converted to tests/golden/fixtures/example_*.npz.
+The oracle reproduces the first two exactly and the sampled one as described in tests/helpers.py
+(tests/test_oracle_pinning.py).  Here the library's optimisers, with U resident on the GPU, must
+follow the same trajectories: f_iter within 1e-9 relative and the gates within 1e-8 (summation
+orders differ from the reference's; ten trust-region steps amplify rounding somewhat), and the
+Python module interface the same way.
+"""
+import numpy as np
+import pytest
+
+import rqcopt_hpc_amd as R
+from rqcopt_hpc_amd import rqcopt_hpc as M
+from helpers import SAMPLED_EXACT_ITERS, cplx, example_ising1d, example_molecular
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.maximum(1, np.abs(np.asarray(b))))
+
+
+def test_hmat_rtr_follows_reference_example():
+    U, V, P, run, _ = example_ising1d()
+    T = R.DeviceTarget.dense(U)
+    f_iter, vopt = R.optimize_brickwall_circuit_hmat(T, V, P, 10)
+    print("hmat f_iter dev", _rel(f_iter, run["f_iter"]), "gates", np.max(np.abs(vopt - cplx(run["Vlist"]))), flush=True)
+    assert _rel(f_iter, run["f_iter"]) < 1e-9
+    assert np.max(np.abs(vopt - cplx(run["Vlist"]))) < 1e-8
+    # the HVP trust region takes the same steps (its tCG applies the same Hessian by HVPs)
+    f_hvp, v_hvp = R.optimize_brickwall_circuit_hvp(T, V, P, 10)
+    assert _rel(f_hvp, run["f_iter"]) < 1e-9
+    assert np.max(np.abs(v_hvp - cplx(run["Vlist"]))) < 1e-8
+    T.close()
+
+
+def test_circuit_rtr_follows_reference_example():
+    U, G0, W, run = example_molecular()
+    T = R.DeviceTarget.dense(U)
+    f_iter, gopt = R.optimize_quantum_circuit(T, G0, W, 5, 10)
+    print("circuit f_iter dev", _rel(f_iter, run["f_iter"]), "gates", np.max(np.abs(gopt - cplx(run["gates"]))),
+          flush=True)
+    assert _rel(f_iter, run["f_iter"]) < 1e-9
+    assert np.max(np.abs(gopt - cplx(run["gates"]))) < 1e-8
+    T.close()
+
+
+def test_sampled_hmat_rtr_follows_reference_example():
+    U, V, P, _, run = example_ising1d()
+    T = R.DeviceTarget.dense(U)
+    f_iter, vopt = R.optimize_brickwall_circuit_hmat_sampling(T, V, P, 10, 16, R.Rng(42))
+    k = SAMPLED_EXACT_ITERS
+    print("sampled f_iter dev (exact part)", _rel(f_iter[:k], run["f_iter"][:k]), "all", _rel(f_iter, run["f_iter"]),
+          flush=True)
+    assert _rel(f_iter[:k], run["f_iter"][:k]) < 1e-9
+    assert _rel(f_iter, run["f_iter"]) < 1e-5
+    assert np.max(np.abs(vopt - cplx(run["Vlist"]))) < 1e-3
+    assert abs(R.brickwall_unitary_target(T, vopt, P).real - run["f_ref"][0]) < 1e-5
+    T.close()
+
+
+def test_python_module_follows_reference_examples():
+    """rqcopt_hpc.optimize_brickwall_circuit (full-Hessian route, as the reference module) and
+    rqcopt_hpc.optimize_quantum_circuit, called as examples/ising1d/ising1d_dynamics_opt.py and
+    examples/molecular/molecular_dynamics_opt.c call the reference."""
+    U, V, P, run, srun = example_ising1d()
+    vopt, f_iter = M.optimize_brickwall_circuit(6, U, list(V), [list(p) for p in P], 10)
+    assert _rel(f_iter, run["f_iter"]) < 1e-9
+    assert np.max(np.abs(np.asarray(vopt) - cplx(run["Vlist"]))) < 1e-8
+    vs, fs = M.optimize_brickwall_circuit(6, U, list(V), [list(p) for p in P], 10, nsamples=16, seed=42)
+    assert _rel(fs[:SAMPLED_EXACT_ITERS], srun["f_iter"][:SAMPLED_EXACT_ITERS]) < 1e-9
+    Um, G0, W, mrun = example_molecular()
+    gopt, f_iter = M.optimize_quantum_circuit(5, Um, list(G0), [tuple(w) for w in W], 10)
+    assert _rel(f_iter, mrun["f_iter"]) < 1e-9
+    assert np.max(np.abs(np.asarray(gopt) - cplx(mrun["gates"]))) < 1e-8

@@ -260,6 +260,37 @@ def test_ising_L8_d5_golden_and_rtr_trajectory():
     assert _max(vopt - d["vopt"]) < 1e-8
 
 
+@pytest.mark.parametrize("radius_init,maxradius", [(2.0, 8.0), (1.0, 4.0)])
+def test_mixed_rtr_takes_the_fp64_branches(radius_init, maxradius):
+    """L=8 Ising workload, 12 RTR-HVP iterations with a large trust radius so that some steps are
+    rejected (at iterations 0 and 2, or 0 and 11, with the oracle's evaluations).  A rejected step
+    keeps x, so f_iter repeats; mixed precision must reject exactly the steps FP64 rejects: its
+    ratio rho compares f values computed the same way (every mixed f carries the gate-rounding
+    correction)."""
+    from rqcopt_hpc_amd.workloads import expm_hermitian, ising_hamiltonian
+    d = fixture("config0_ising_L8_d5")
+    T = R.DeviceTarget.dense(expm_hermitian(ising_hamiltonian(8, 1.0, 0.75, 0.6), 0.25))
+
+    def run():
+        prm = R.rtr_default_params(16 * len(d["V"]))
+        prm.radius_init, prm.maxradius = radius_init, maxradius
+        return R.optimize_brickwall_circuit_hvp(T, d["V"], d["P"], 12, prm)
+
+    f64, v64 = run()
+    R.set_precision(R.PRECISION_MIXED)
+    try:
+        f32, v32 = run()
+    finally:
+        R.set_precision(R.PRECISION_FP64)
+    rej64 = np.abs(np.diff(f64)) < 1e-10 * np.maximum(1, np.abs(f64[1:]))
+    rej32 = np.abs(np.diff(f32)) < 1e-10 * np.maximum(1, np.abs(f32[1:]))
+    print("rejected steps:", np.nonzero(rej64)[0], np.nonzero(rej32)[0], "max |df|", np.max(np.abs(f32 - f64)), flush=True)
+    assert rej64.sum() >= 2
+    assert np.array_equal(rej64, rej32)
+    assert np.max(np.abs(f32 - f64)) / 256 < 1e-6
+    T.close()
+
+
 def test_column_blocking_and_determinism():
     rng = np.random.default_rng(8)
     L, nl = 8, 4
@@ -341,8 +372,9 @@ def test_mixed_gate_rounding_correction(L, nl, engine):
     finally:
         R.set_sweep_engine(R.ENGINE_TILE_PASS)
     exact = f64 - f64r
-    print(f"L={L} {engine}: gate-rounding term {abs(exact):.3e}, corrected error {abs(fc - f64):.3e}, "
-          f"target-only (uncorrected) error {abs(ft - f64):.3e}", flush=True)
+    print(f"L={L} {engine}: gate-rounding term {abs(exact):.3e}, corrected error {abs(fc - f64):.3e}", flush=True)
+    # a mixed target evaluation runs the gradient sweep and applies the same correction
+    assert ft == fc
     rnd = 1e-14 * max(1.0, abs(f64))                            # rounding of a difference of two f values
     assert abs(exact) > 1e3 * rnd                                # the term is there to be checked
     assert abs((fc - fu) - exact) < 1e-3 * abs(exact) + rnd

@@ -16,8 +16,9 @@ import numpy as np
 import pytest
 
 import oracle as O
-from helpers import (brickwall_matrix, brickwall_perms, circuit_matrix, cplx, crandn, fixture, gate_matrix,
-                     haar_unitary, perms_of, random_perms, random_wires, scramble_matrix, stack)
+from helpers import (SAMPLED_EXACT_ITERS, brickwall_matrix, brickwall_perms, circuit_matrix, cplx, crandn,
+                     example_ising1d, example_molecular, fixture, gate_matrix, haar_unitary, perms_of, random_perms,
+                     random_wires, scramble_matrix, stack)
 
 SCR = O.Ufunc.scramble()
 
@@ -530,6 +531,44 @@ def test_hessian_columns_are_hvps_and_hmat_rtr_follows_hvp_rtr():
     f_h, v_h = O.optimize_brickwall_circuit_hmat(Uo, V, P, L, 4)
     f_v, v_v = O.optimize_brickwall_circuit_hvp(Uo, V, P, L, 4)
     assert np.max(np.abs(f_h - f_v)) < 1e-9 and np.max(np.abs(v_h - v_v)) < 1e-8 and f_h[-1] < f_h[0]
+
+
+def _rel(a, b):
+    return np.max(np.abs(np.asarray(a) - np.asarray(b)) / np.maximum(1, np.abs(np.asarray(b))))
+
+
+def test_reference_example_hmat_rtr_trajectory():
+    """examples/ising1d/ising1d_dynamics_opt.c: optimize_brickwall_circuit_hmat with default
+    parameters, 10 iterations (reference src/trust_region.c:230-293, src/brickwall_opt.c).  The
+    stored f_iter and optimised gates pin the tCG and trust-region update rules end to end: a wrong
+    radius rule, ratio or acceptance threshold changes the trajectory."""
+    U, V, P, run, _ = example_ising1d()
+    f_iter, vopt = O.optimize_brickwall_circuit_hmat(O.Ufunc.dense(U), V, P, 6, 10)
+    assert _rel(f_iter, run["f_iter"]) < 1e-12
+    assert np.max(np.abs(vopt - cplx(run["Vlist"]))) < 1e-12
+
+
+def test_reference_example_circuit_rtr_hvp_trajectory():
+    """examples/molecular/molecular_dynamics_opt.c: optimize_quantum_circuit (RTR with HVPs, reference
+    src/circuit_opt.c:104-124, src/trust_region.c:306-368), 18 gates on 5 qubits, 10 iterations."""
+    U, G0, W, run = example_molecular()
+    f_iter, gopt = O.optimize_quantum_circuit(O.Ufunc.dense(U), G0, W, 5, 10)
+    assert _rel(f_iter, run["f_iter"]) < 1e-12
+    assert np.max(np.abs(gopt - cplx(run["gates"]))) < 1e-12
+
+
+def test_reference_example_sampled_hmat_rtr_trajectory():
+    """examples/ising1d/ising1d_dynamics_opt_sampling.c: optimize_brickwall_circuit_hmat_sampling with
+    16 Haar samples per evaluation from seed 42 (reference src/brickwall_opt.c:242-268).  Agrees to
+    rounding through the iterations that are insensitive to the order of U's sums, and within the
+    measured sensitivity after; f_ref is the exact target at the optimised gates."""
+    U, V, P, _, run = example_ising1d()
+    f_iter, vopt = O.optimize_brickwall_circuit_hmat_sampling(O.Ufunc.dense(U), V, P, 6, 10, 16, O.Rng(42))
+    k = SAMPLED_EXACT_ITERS
+    assert _rel(f_iter[:k], run["f_iter"][:k]) < 1e-12
+    assert _rel(f_iter, run["f_iter"]) < 1e-5
+    assert np.max(np.abs(vopt - cplx(run["Vlist"]))) < 1e-3
+    assert abs(O.brickwall_unitary_target(O.Ufunc.dense(U), vopt, P, 6).real - run["f_ref"][0]) < 1e-5
 
 
 def test_placeholder_and_backward_array_closed_forms():
