@@ -189,3 +189,36 @@ def test_batch_autotuned_tiles_keep_parity():
             assert abs(fa[i] - f_o) < TOL * max(1, abs(f_o)) and reldiff(ga[i], g_o) < TOL and reldiff(ha[i], h_o) < TOL
     Bd.close()
     Ba.close()
+
+
+def test_batch_workload_64_problems():
+    """The batched workload at its size: 64 problems, L in {8, 10, 12}, depth in {3, 5, 7}, XXZ
+    Trotter targets, autotuned tiles.  A sample of problems (every shape present, including the
+    largest) equals the single-call path within 1e-12 (the autotuned tile may differ from the single
+    call's default one, which changes only the rounding); the L=8 problems match the oracle at 1e-12;
+    two lock-step trust-region iterations decrease f for every problem."""
+    probs = W.batch_config(1, 64)
+    targets = [R.DeviceTarget.circuit(p["target_gates"], p["target_wires"], p["L"]) for p in probs]
+    V = [p["V0"] for p in probs]
+    Z = [W.tangent_direction(1000 + i, V[i])[1] for i in range(len(probs))]
+    shapes = sorted({(p["L"], p["depth"]) for p in probs})
+    print("shapes", shapes, flush=True)
+    B = R.Batch([(p["perms"], t) for p, t in zip(probs, targets)])
+    f, g, h = B.evaluate([R.EVAL_HVP] * len(probs), V, Z)
+    sample = [next(i for i, p in enumerate(probs) if (p["L"], p["depth"]) == s) for s in shapes]
+    for i in sample:
+        p = probs[i]
+        fs, gs, hs = R.brickwall_unitary_target_projected_hessian_vector_product(targets[i], V[i], Z[i], p["perms"])
+        print(f"problem {i} L={p['L']} d={p['depth']} tiles {B.tiles(i)}", flush=True)
+        assert abs(f[i] - fs) < TOL * max(1, abs(fs)) and reldiff(g[i], gs) < TOL and reldiff(h[i], hs) < TOL
+    for i, p in enumerate(probs):
+        if p["L"] != 8:
+            continue
+        Uo = O.Ufunc.dense(targets[i].download())
+        f_o, g_o, h_o = O.brickwall_unitary_target_projected_hvp(Uo, V[i], Z[i], p["perms"], 8)
+        assert abs(f[i] - f_o) < TOL * max(1, abs(f_o)) and reldiff(g[i], g_o) < TOL and reldiff(h[i], h_o) < TOL
+    f_iters, _ = B.optimize(V, 2)
+    assert all(fi[-1] < fi[0] for fi in f_iters)
+    B.close()
+    for t in targets:
+        t.close()

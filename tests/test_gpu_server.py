@@ -190,3 +190,51 @@ def test_poisson_load_small():
     for s in sessions:
         s.close()
     srv.close()
+
+
+def test_serving_workload_large_sessions_alone_and_in_the_mix():
+    """Sessions of the serving workload at its largest sizes (L=13 as an open-brickwall circuit and
+    L=14 as a periodic brickwall, both depth 9, FP64 and mixed) with smaller ones beside them: each
+    request's result is bitwise the same alone and interleaved with background requests."""
+    solves, targets = S.serving_workload(1, nsolves=256)
+    big = [next(i for i, s in enumerate(solves) if s["L"] == L and s["depth"] == 9) for L in (13, 14)]
+    small = [next(i for i, s in enumerate(solves) if s["L"] == L) for L in (8, 10)]
+    chosen = [solves[i] for i in big + small]
+    srv = R.Server(arena_bytes=200 * GiB, max_request_bytes=48 * GiB, nslots=4)
+    rng = np.random.default_rng(5)
+    for prec in (R.PRECISION_FP64, R.PRECISION_MIXED):
+        sessions, dev = S.open_sessions(srv, chosen, targets, prec)
+        reqs = []
+        for j in range(2):   # the two large sessions
+            V = chosen[j]["gates"]
+            for kind in (R.EVAL_TARGET, R.EVAL_GRADIENT, R.EVAL_HVP):
+                reqs.append((j, kind, V, _dirs(rng, V) if kind == R.EVAL_HVP else None))
+        alone = []
+        for j, kind, V, Z in reqs:
+            f, g, h, info = srv.wait(srv.submit(sessions[j], kind, V, Z))
+            assert info["status"] == 0
+            alone.append((f, None if g is None else g.copy(), None if h is None else h.copy()))
+        pend = []
+        for k in rng.permutation(len(reqs)):
+            j, kind, V, Z = reqs[k]
+            b = int(rng.integers(len(chosen)))
+            pend.append((None, srv.submit(sessions[b], R.EVAL_HVP, chosen[b]["gates"], _dirs(rng, chosen[b]["gates"]))))
+            pend.append((k, srv.submit(sessions[j], kind, V, Z)))
+        for k, p in pend:
+            f, g, h, info = srv.wait(p)
+            assert info["status"] == 0
+            if k is None:
+                continue
+            fa, ga, ha = alone[k]
+            print(f"prec {prec} request {k} L={chosen[reqs[k][0]]['L']} kind {reqs[k][1]}: slot {info['slot']}, "
+                  f"{info['latency_ms']:.1f} ms", flush=True)
+            assert f == fa
+            assert (g is None and ga is None) or np.array_equal(g, ga)
+            assert (h is None and ha is None) or np.array_equal(h, ha)
+        st = srv.stats()
+        assert st["failed"] == 0 and st["peak_arena_bytes"] <= st["arena_bytes"]
+        for s_ in sessions:
+            s_.close()
+        for t in dev.values():
+            t.close()
+    srv.close()
