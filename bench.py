@@ -17,7 +17,9 @@ Beside the headline line's single-circuit numbers it reports
   "serving": the request server under an open-loop Poisson stream of
     target / gradient / HVP requests from 256 solves (L in {8..14}, depth in
     {3, 5, 7, 9}, Ising or XXZ targets): p50 / p99 latency and completed
-    requests/s in FP64 and mixed, at the offered rate and at a quarter of it.
+    requests/s in FP64 and mixed, at the offered rate and at a quarter of it;
+    and under real trust-region solves of the same workload (each solve's
+    driver issuing its own requests), at several concurrencies.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--L 14] [--depth 9]
 
@@ -358,6 +360,15 @@ def run_serving(args, R):
             print(f"[bench] serving {name} {label}: {rec[label]}", file=sys.stderr, flush=True)
         rec["peak_in_flight"] = srv.stats()["peak_in_flight"]
         rec["arena_gib"] = srv.stats()["arena_bytes"] / 2**30
+        # real trust-region solves: every request comes from a solve's driver (closed loop, so the
+        # achieved rate is what the device sustains at that many concurrent solves)
+        if args.solve_seconds > 0:
+            rec["solves"] = {}
+            for conc in args.solve_concurrency:
+                res = S.run_solves(srv, solves, sessions, conc, 1, args.solve_seconds, seed=args.seed)
+                res.pop("server", None)
+                rec["solves"][str(conc)] = res
+                print(f"[bench] serving {name} solves x{conc}: {res}", file=sys.stderr, flush=True)
         for s_ in sessions:
             s_.close()
         srv.close()
@@ -391,6 +402,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving section")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="offered requests/s of the Poisson stream")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window of the Poisson stream")
+    ap.add_argument("--solve-seconds", type=float, default=6.0,
+                    help="window of the closed-loop trust-region solves through the server (0: skip)")
+    ap.add_argument("--solve-concurrency", type=int, nargs="+", default=[4, 16],
+                    help="concurrent solves of the closed-loop serving runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU column sharding is a "

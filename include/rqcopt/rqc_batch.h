@@ -44,6 +44,13 @@ enum rqc_batch_exec
 	RQC_BATCH_EAGER = 1,  /* direct launches of the active problems on the batch's streams */
 };
 
+enum rqc_batch_rtr
+{
+	RQC_BATCH_RTR_DEVICE = 0,  /* trust region on the GPU: tCG, projection, retraction and radius updates
+	                              in FP64 device kernels between graph launches (default) */
+	RQC_BATCH_RTR_HOST   = 1,  /* one host driver thread per problem (the reference's driver), in lock-step */
+};
+
 struct rqc_batch_stats
 {
 	long steps;            /* evaluation steps run */
@@ -70,12 +77,20 @@ int rqc_batch_evaluate(struct rqc_batch* batch, const int kinds[], const struct 
 	const struct mat4x4* const vdirs[], numeric fvals[], double* const grads[], double* const hvps[]);
 
 /* one Riemannian trust-region optimisation per problem (the algorithm of
-   optimize_brickwall_circuit_hvp), all run in lock-step: each step evaluates
+   optimize_brickwall_circuit_hvp, reference src/brickwall_opt.c:127-150 over
+   src/trust_region.c:159-217,306-368), all run in lock-step: each step evaluates
    every problem that has a pending target, gradient or HVP request.  params
    applies to every problem (its g_func is ignored); f_iters[p] receives
-   niter + 1 values, vlist_opt[p] the optimised gates. */
+   niter + 1 values, vlist_opt[p] the optimised gates.
+   RQC_BATCH_RTR_DEVICE (default): the driver's state (gates, gradient, tCG vectors,
+   radius) stays on the GPU; FP64 kernels run the tCG recurrences, the projection to
+   tangent directions, the polar retraction and the acceptance test, and refill the
+   gate tables, between launches of the captured evaluation graphs, so a tCG step needs
+   no host round trip (the host reads one active-problem count per chunk of steps).
+   RQC_BATCH_RTR_HOST: one host driver thread per problem. */
 int rqc_batch_optimize_brickwall_hvp(struct rqc_batch* batch, const struct mat4x4* const vlist_start[],
 	const struct rtr_params* params, int niter, double* const f_iters[], struct mat4x4* const vlist_opt[]);
+int rqc_batch_set_rtr(struct rqc_batch* batch, int mode);
 
 int rqc_batch_get_stats(const struct rqc_batch* batch, struct rqc_batch_stats* stats);
 /* column block, sweep engine and passes of problem p for an evaluation kind */

@@ -22,6 +22,7 @@
 #include "matrix.h"
 #include "rqc_device.h"
 #include "rqc_batch.h"
+#include "trust_region.h"
 
 struct rqc_server;   /* opaque */
 struct rqc_session;  /* opaque */
@@ -85,5 +86,16 @@ int rqc_server_poll(struct rqc_server* server, long ticket, struct rqc_request_i
 /* forget a finished request's record (records are otherwise kept until the server is destroyed) */
 int rqc_server_release(struct rqc_server* server, long ticket);
 int rqc_server_get_stats(const struct rqc_server* server, struct rqc_server_stats* stats);
+
+/* one trust-region optimisation (the algorithm of optimize_brickwall_circuit_hvp, reference
+   src/brickwall_opt.c:127-150, or for a general-circuit session of optimize_quantum_circuit,
+   src/circuit_opt.c:104-124: one gate per group) of a session on the calling thread, whose target,
+   gradient and HVP evaluations are requests of this server: solves on several host threads
+   share the device like any other requests.  f_iter receives niter + 1 values, vlist_opt the
+   optimised gates; records (up to max_records, may be NULL) receive the info of every request
+   in issue order and *nrecords their number. */
+int rqc_server_optimize_brickwall_hvp(struct rqc_server* server, struct rqc_session* session,
+	const struct mat4x4 vlist_start[], const struct rtr_params* params, int niter, double* f_iter,
+	struct mat4x4 vlist_opt[], struct rqc_request_info* records, int max_records, int* nrecords);
 
 RQC_EXTERN_C_END

@@ -30,6 +30,7 @@ KERNEL_FORWARD, KERNEL_REVERSE, KERNEL_TARGET, KERNEL_OTHER = range(4)
 ENGINE_TILE_PASS, ENGINE_PER_GATE = 0, 1
 EVAL_NONE, EVAL_TARGET, EVAL_GRADIENT, EVAL_HVP = -1, 0, 1, 2
 BATCH_GRAPH, BATCH_EAGER = 0, 1
+BATCH_RTR_DEVICE, BATCH_RTR_HOST = 0, 1
 
 
 class RqcError(RuntimeError):
@@ -132,11 +133,11 @@ EXPORTED = [
     # rqc_batch.h
     "rqc_batch_create", "rqc_batch_destroy", "rqc_batch_set_exec", "rqc_batch_evaluate",
     "rqc_batch_optimize_brickwall_hvp", "rqc_batch_get_stats", "rqc_batch_problem_plan", "rqc_batch_problem_tiles",
-    "rqc_set_autotune",
+    "rqc_set_autotune", "rqc_batch_set_rtr",
     # rqc_server.h
     "rqc_server_create", "rqc_server_destroy", "rqc_session_create_brickwall", "rqc_session_create_circuit",
     "rqc_session_ngroups", "rqc_session_destroy", "rqc_server_submit", "rqc_server_wait", "rqc_server_poll",
-    "rqc_server_release", "rqc_server_get_stats",
+    "rqc_server_release", "rqc_server_get_stats", "rqc_server_optimize_brickwall_hvp",
 ]
 
 
@@ -899,6 +900,11 @@ class Batch:
     def set_exec(self, mode):
         _check("rqc_batch_set_exec", lib().rqc_batch_set_exec(self.handle, mode))
 
+    def set_rtr(self, mode):
+        """BATCH_RTR_DEVICE (default): optimize() keeps the trust region's state and arithmetic on the
+        GPU; BATCH_RTR_HOST: one host driver thread per problem."""
+        _check("rqc_batch_set_rtr", lib().rqc_batch_set_rtr(self.handle, mode))
+
     @staticmethod
     def _ptrs(arrs):
         return (ct.c_void_p * len(arrs))(*[None if a is None else a.ctypes.data for a in arrs])
@@ -916,7 +922,8 @@ class Batch:
         return f, grads, hvps
 
     def optimize(self, vlist_starts, niter, params=None):
-        """Lock-step optimize_brickwall_circuit_hvp of every problem: (f_iters list, vlist_opt list)."""
+        """Lock-step optimize_brickwall_circuit_hvp of every problem: (f_iters list, vlist_opt list).
+        The trust region runs on the GPU unless set_rtr(BATCH_RTR_HOST) was called."""
         vl = [_gates(v) for v in vlist_starts]
         params = rtr_default_params(16 * self.nlayers[0]) if params is None else params
         f_iters = [np.empty(niter + 1) for _ in range(self.n)]
@@ -1059,6 +1066,25 @@ class Server:
         L = lib()
         L.rqc_server_release.argtypes = [ct.c_void_p, ct.c_long]
         _check("rqc_server_release", L.rqc_server_release(self.handle, pending.ticket))
+
+    def optimize(self, session, vlist_start, niter, params=None, max_records=4096):
+        """optimize_brickwall_circuit_hvp of a brickwall session on the calling thread, every target /
+        gradient / HVP evaluation a request of this server (ctypes releases the GIL, so solves on
+        several Python threads run concurrently): (f_iter, vlist_opt, request infos)."""
+        v = _gates(vlist_start)
+        params = rtr_default_params(16 * len(v)) if params is None else params
+        f_iter = np.empty(niter + 1)
+        vopt = np.empty_like(v)
+        recs = (_RequestInfo * max_records)()
+        nrec = ct.c_int()
+        L = lib()
+        L.rqc_server_optimize_brickwall_hvp.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int,
+                                                        ct.c_void_p, ct.c_void_p, ct.c_void_p, ct.c_int, ct.c_void_p]
+        _check("rqc_server_optimize_brickwall_hvp",
+               L.rqc_server_optimize_brickwall_hvp(self.handle, session.handle, v.ctypes.data, ct.byref(params), niter,
+                                                   f_iter.ctypes.data, vopt.ctypes.data, recs, max_records,
+                                                   ct.byref(nrec)))
+        return f_iter, vopt, [self._info(recs[i]) for i in range(min(nrec.value, max_records))]
 
     def stats(self):
         st = _ServerStats()

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "rqc_eval.hpp"
+#include "mat4.hpp"
 #include "rqcopt/matrix.h"
 #include "rqcopt/rqc_batch.h"
 #include "rqcopt/trust_region.h"
@@ -70,6 +71,18 @@ struct Batch
 	std::vector<hipEvent_t> ev_join;
 	rqc_batch_stats stats{};
 	std::mutex mu;
+	int rtr = RQC_BATCH_RTR_DEVICE;   // trust region of rqc_batch_optimize_brickwall_hvp
+
+	// device-resident trust region: one table region per problem and kind (the position
+	// terms uploaded once, the gate-dependent part refilled on the device), the slot-group
+	// maps, the fill descriptors and the graphs captured over these tables
+	char* dmem = nullptr;
+	size_t dmem_bytes = 0;
+	std::vector<size_t> dtab_off[kKinds];
+	std::vector<size_t> dgroup_off[kKinds];
+	size_t off_desc = 0, off_misc = 0;
+	hipGraphExec_t dgraph[kKinds] = {nullptr, nullptr, nullptr};
+	int* h_count = nullptr;   // pinned: active problems after a chunk of tCG steps
 
 	int* flags_dev(int k, int p) const { return reinterpret_cast<int*>(arena) + (size_t)k * probs.size() + p; }
 	char* work(const BatchProblem& b) const { return arena + off_work + b.work_off; }
@@ -100,6 +113,9 @@ static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 static void batch_release(Batch& bt)
 {
 	for (auto& g : bt.graph) { if (g != nullptr) { (void)hipGraphExecDestroy(g); g = nullptr; } }
+	for (auto& g : bt.dgraph) { if (g != nullptr) { (void)hipGraphExecDestroy(g); g = nullptr; } }
+	if (bt.dmem != nullptr) { (void)hipFree(bt.dmem); bt.dmem = nullptr; }
+	if (bt.h_count != nullptr) { (void)hipHostFree(bt.h_count); bt.h_count = nullptr; }
 	if (bt.arena != nullptr) { (void)hipFree(bt.arena); }
 	if (bt.h_up != nullptr) { (void)hipHostFree(bt.h_up); }
 	if (bt.h_down != nullptr) { (void)hipHostFree(bt.h_down); }
@@ -201,7 +217,7 @@ static void batch_init(Batch& bt, int n, const rqc_batch_problem* pr, int precis
 
 // capture the graph of evaluation kind k: fork into the branches, every
 // problem's chain on its branch with its skip flag, join
-static void capture_kind(Batch& bt, int k)
+static void capture_kind(Batch& bt, int k, bool device_tables = false)
 {
 	hipStream_t cs = bt.kstream[k].stream;
 	RQC_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
@@ -210,7 +226,8 @@ static void capture_kind(Batch& bt, int k)
 		for (auto& s : bt.branches) { RQC_HIP(hipStreamWaitEvent(s.stream, bt.ev_fork, 0)); }
 		for (size_t p = 0; p < bt.probs.size(); p++) {
 			BatchProblem& b = bt.probs[p];
-			enqueue_eval(*bt.dev, bt.branches[b.branch], b.plan[k], b.C, b.dU, bt.work(b), bt.tab(b), bt.res(b), nullptr,
+			char* tab = device_tables ? bt.dmem + bt.dtab_off[k][p] : bt.tab(b);
+			enqueue_eval(*bt.dev, bt.branches[b.branch], b.plan[k], b.C, b.dU, bt.work(b), tab, bt.res(b), nullptr,
 				bt.flags_dev(k, (int)p));
 		}
 		for (int br = 0; br < bt.nbranch; br++) {
@@ -225,7 +242,7 @@ static void capture_kind(Batch& bt, int k)
 	}
 	hipGraph_t g = nullptr;
 	RQC_HIP(hipStreamEndCapture(cs, &g));
-	const hipError_t e = hipGraphInstantiate(&bt.graph[k], g, nullptr, nullptr, 0);
+	const hipError_t e = hipGraphInstantiate(device_tables ? &bt.dgraph[k] : &bt.graph[k], g, nullptr, nullptr, 0);
 	(void)hipGraphDestroy(g);
 	RQC_HIP(e);
 }
@@ -437,6 +454,389 @@ void ls_retract(const double* RQC_RESTRICT x, const double* RQC_RESTRICT eta, vo
 
 }  // namespace
 
+// ---------------------------------------------------------------- device-resident trust region
+// The lock-step trust region with its driver on the GPU.  Each problem's state
+// (gates, gradient, tCG vectors, radius) lives in device memory and is updated
+// by one thread of its own workgroup, in the operation order of the host
+// driver (rqc_host.cpp: truncated_cg and trust_region, the reference's
+// src/trust_region.c:159-217,306-368), with contraction into fused
+// multiply-adds switched off so that the scalar recurrences round as the host's
+// do.  Between these kernels the captured evaluation graphs run on tables that
+// the device refills from the state (enqueue_fill_tables); a problem that has
+// no evaluation pending sets its skip flag, so the graphs stay the same.
+struct DevRtr
+{
+	int n = 0, nl = 0;               // tangent dimension 16 nl; layers
+	int active = 0, on_boundary = 0, tcg_iter = 0, pad = 0;
+	long evals = 0;                  // evaluations armed for this problem
+	double radius = 0, rsq = 0, stoptol = 0, fx = 0, denom = 0;
+	double* x = nullptr;             // 32 nl: the layer gates (complex, row-major)
+	double* xn = nullptr;            // the candidate
+	double* grad = nullptr;          // n
+	double* z = nullptr;             // n: the tCG iterate (eta)
+	double* r = nullptr;             // n
+	double* d = nullptr;             // n
+	zc* cur_gates = nullptr;         // 16 nl: gates of the next evaluation
+	zc* cur_dirs = nullptr;          // 16 nl: its directions
+	const char* res = nullptr;       // the problem's result region
+	double* f_iter = nullptr;        // niter + 1
+	int* flag[kKinds] = {nullptr, nullptr, nullptr};
+};
+
+__device__ __forceinline__ const double* res_grad(const DevRtr& s)
+{
+	return reinterpret_cast<const double*>(reinterpret_cast<const zc*>(s.res) + 1 + 32 * (size_t)s.nl);
+}
+__device__ __forceinline__ const double* res_hvp(const DevRtr& s) { return res_grad(s) + 16 * (size_t)s.nl; }
+__device__ __forceinline__ double res_f(const DevRtr& s) { return reinterpret_cast<const zc*>(s.res)->x; }
+
+__device__ double dot_seq(int n, const double* a, const double* b)
+{
+#pragma clang fp contract(off)
+	double t = 0;
+	for (int i = 0; i < n; i++) { t += a[i] * b[i]; }
+	return t;
+}
+
+// tangent directions Z_i = V_i A(w_i) of the layer gates V = x for a tangent vector w
+__device__ void tangent_dirs(const DevRtr& s, const double* x, const double* w)
+{
+	for (int i = 0; i < s.nl; i++) {
+		m4_real_to_tangent(w + 16 * i, reinterpret_cast<const zc*>(x) + 16 * i, s.cur_dirs + 16 * i);
+	}
+}
+
+__device__ void set_gates(const DevRtr& s, const double* x)
+{
+	const zc* v = reinterpret_cast<const zc*>(x);
+	for (int e = 0; e < 16 * s.nl; e++) { s.cur_gates[e] = v[e]; }
+}
+
+__device__ __forceinline__ void arm(DevRtr& s, int kind, bool on)
+{
+	*s.flag[kind] = on ? 0 : 1;
+	if (on) { s.evals++; }
+}
+
+// iteration start: gradient evaluation at x
+__global__ void k_rtr_begin(DevRtr* st)
+{
+	if (threadIdx.x != 0) { return; }
+	DevRtr& s = st[blockIdx.x];
+	set_gates(s, s.x);
+	arm(s, RQC_EVAL_GRADIENT, true);
+}
+
+// f(x), the gradient, and the tCG start: z = 0, r = grad, d = -r (first HVP along d)
+__global__ void k_rtr_tcg_init(DevRtr* st, int maxiter, double abstol, double reltol)
+{
+#pragma clang fp contract(off)
+	if (threadIdx.x != 0) { return; }
+	DevRtr& s = st[blockIdx.x];
+	s.fx = res_f(s);
+	const double* g = res_grad(s);
+	for (int i = 0; i < s.n; i++) { s.grad[i] = g[i]; s.r[i] = g[i]; }
+	s.rsq = dot_seq(s.n, s.r, s.r);
+	s.stoptol = fmax(abstol, reltol * sqrt(s.rsq));
+	for (int i = 0; i < s.n; i++) { s.z[i] = 0; s.d[i] = -s.r[i]; }
+	s.tcg_iter = 0;
+	s.on_boundary = 0;
+	s.active = maxiter > 0;
+	if (s.active) { tangent_dirs(s, s.x, s.d); }
+	arm(s, RQC_EVAL_HVP, s.active);
+}
+
+// t > 0 with ||b + t d|| = radius (host move_to_boundary / solve_quadratic)
+__device__ double to_boundary(int n, const double* b, const double* d, double radius)
+{
+#pragma clang fp contract(off)
+	const double dsq = dot_seq(n, d, d);
+	if (dsq == 0) { return 0; }
+	const double p = dot_seq(n, b, d) / dsq;
+	const double q = (dot_seq(n, b, b) - radius * radius) / dsq;
+	if (p == 0) { return sqrt(-q); }
+	const double a = -(p + copysign(1.0, p) * sqrt(p * p - q));
+	const double c = q / a;
+	return (a <= c) ? c : a;
+}
+
+// one tCG step with the HVP of d just evaluated
+__global__ void k_rtr_tcg_step(DevRtr* st, int maxiter)
+{
+#pragma clang fp contract(off)
+	if (threadIdx.x != 0) { return; }
+	DevRtr& s = st[blockIdx.x];
+	if (!s.active) { return; }
+	const double* hd = res_hvp(s);
+	const int n = s.n;
+	const double dhd = dot_seq(n, s.d, hd);
+	const double t = to_boundary(n, s.z, s.d, s.radius);
+	const double alpha = s.rsq / dhd;
+	if (dhd <= 0 || alpha > t) {
+		for (int i = 0; i < n; i++) { s.z[i] += t * s.d[i]; }
+		s.on_boundary = 1;
+		s.active = 0;
+	} else {
+		for (int i = 0; i < n; i++) { s.r[i] += alpha * hd[i]; }
+		for (int i = 0; i < n; i++) { s.z[i] += alpha * s.d[i]; }
+		const double rsq_next = dot_seq(n, s.r, s.r);
+		if (sqrt(rsq_next) <= s.stoptol) {
+			s.active = 0;
+		} else {
+			const double beta = rsq_next / s.rsq;
+			for (int i = 0; i < n; i++) { s.d[i] = beta * s.d[i] - s.r[i]; }
+			s.rsq = rsq_next;
+		}
+	}
+	s.tcg_iter++;
+	if (s.active && s.tcg_iter >= maxiter) { s.active = 0; }
+	if (s.active) { tangent_dirs(s, s.x, s.d); }
+	arm(s, RQC_EVAL_HVP, s.active);
+}
+
+// problems still in their tCG loop (one thread, fixed order)
+__global__ void k_rtr_count_active(const DevRtr* st, int n, int* count)
+{
+	if (threadIdx.x != 0 || blockIdx.x != 0) { return; }
+	int c = 0;
+	for (int p = 0; p < n; p++) { c += st[p].active; }
+	*count = c;
+}
+
+// candidate x_n = retract(x, eta) per layer; HVP along eta for the model decrease
+__global__ void k_rtr_candidate(DevRtr* st)
+{
+	if (threadIdx.x != 0) { return; }
+	DevRtr& s = st[blockIdx.x];
+	const zc* v = reinterpret_cast<const zc*>(s.x);
+	zc* w = reinterpret_cast<zc*>(s.xn);
+	for (int i = 0; i < s.nl; i++) { m4_retract(v + 16 * i, s.z + 16 * i, w + 16 * i); }
+	tangent_dirs(s, s.x, s.z);
+	arm(s, RQC_EVAL_HVP, true);
+}
+
+// model decrease <grad, eta> + <eta, H eta> / 2; then f at the candidate
+__global__ void k_rtr_model(DevRtr* st)
+{
+#pragma clang fp contract(off)
+	if (threadIdx.x != 0) { return; }
+	DevRtr& s = st[blockIdx.x];
+	s.denom = dot_seq(s.n, s.grad, s.z) + 0.5 * dot_seq(s.n, s.z, res_hvp(s));
+	set_gates(s, s.xn);
+	arm(s, RQC_EVAL_TARGET, true);
+}
+
+// rho, the radius update and acceptance (reference src/trust_region.c:337-361)
+__global__ void k_rtr_accept(DevRtr* st, int k, double rho_trust, double maxradius)
+{
+#pragma clang fp contract(off)
+	if (threadIdx.x != 0) { return; }
+	DevRtr& s = st[blockIdx.x];
+	const double rho = (res_f(s) - s.fx) / s.denom;
+	if (rho < 0.25) {
+		s.radius *= 0.25;
+	} else if (rho > 0.75 && s.on_boundary) {
+		s.radius = fmin(2 * s.radius, maxradius);
+	}
+	s.f_iter[k] = s.fx;
+	if (rho > rho_trust) {
+		for (int e = 0; e < 32 * s.nl; e++) { s.x[e] = s.xn[e]; }
+	}
+}
+
+__global__ void k_rtr_final_begin(DevRtr* st)
+{
+	if (threadIdx.x != 0) { return; }
+	DevRtr& s = st[blockIdx.x];
+	set_gates(s, s.x);
+	arm(s, RQC_EVAL_TARGET, true);
+}
+
+__global__ void k_rtr_final(DevRtr* st, int niter)
+{
+	if (threadIdx.x != 0) { return; }
+	DevRtr& s = st[blockIdx.x];
+	s.f_iter[niter] = res_f(s);
+}
+
+// device tables of every problem and kind (position terms from one host fill at the start
+// gates), the slot-group maps and the graphs over these tables; kept for the batch's lifetime
+static void ensure_device_tables(Batch& bt)
+{
+	if (bt.dmem != nullptr) { return; }
+	const int n = (int)bt.probs.size();
+	size_t off = 0;
+	for (int k = 0; k < kKinds; k++) {
+		bt.dtab_off[k].resize(n);
+		bt.dgroup_off[k].resize(n);
+		for (int p = 0; p < n; p++) {
+			bt.dtab_off[k][p] = off;
+			off += al256(bt.probs[p].plan[k].tab_bytes);
+			bt.dgroup_off[k][p] = off;
+			off += al256(sizeof(int) * (size_t)bt.probs[p].C.ngates);
+		}
+	}
+	bt.off_desc = off;
+	off += al256(sizeof(DeviceTableDesc) * kKinds * (size_t)n);
+	bt.off_misc = off;
+	off += 256;   // active count, singular-gate flag
+	bt.dmem_bytes = off;
+	RQC_HIP(hipMalloc(&bt.dmem, bt.dmem_bytes));
+	RQC_HIP(hipHostMalloc(reinterpret_cast<void**>(&bt.h_count), 256, hipHostMallocDefault));
+	std::vector<char> img(bt.off_desc, 0);
+	for (int k = 0; k < kKinds; k++) {
+		for (int p = 0; p < n; p++) {
+			const auto g = table_slot_groups(bt.probs[p].plan[k]);
+			std::memcpy(img.data() + bt.dgroup_off[k][p], g.data(), g.size() * sizeof(int));
+		}
+	}
+	RQC_HIP(hipMemcpy(bt.dmem, img.data(), img.size(), hipMemcpyHostToDevice));
+	for (int k = 0; k < kKinds; k++) { capture_kind(bt, k, true); }
+}
+
+static void batch_optimize_device(Batch& bt, const struct mat4x4* const* vstart, const struct rtr_params* prm, int niter,
+	double* const* f_iters, struct mat4x4* const* vopt)
+{
+	if (!(0 <= prm->rho_trust && prm->rho_trust < 0.25)) { throw Error(-3, "rho_trust must lie in [0, 0.25)"); }
+	const int n = (int)bt.probs.size();
+	ensure_device_tables(bt);
+	hipStream_t s = bt.io.stream;
+	// per-call state: gates, candidate, gradient, tCG vectors, current gates and directions, f_iter
+	std::vector<size_t> base(n);
+	size_t tot = 0;
+	for (int p = 0; p < n; p++) {
+		const size_t nl = (size_t)bt.probs[p].nlayers;
+		base[p] = tot;
+		tot += al256(sizeof(double) * (32 * nl * 2 + 16 * nl * 4 + 32 * nl * 2 + (size_t)niter + 1));
+	}
+	const size_t st_off = tot;
+	tot += al256(sizeof(DevRtr) * (size_t)n);
+	struct Pool : DeviceBuffer { ~Pool() { release(); } } pool;
+	pool.reserve(tot);
+	char* pm = pool.as<char>();
+	std::vector<DevRtr> hs(n);
+	std::vector<char> himg(st_off, 0);
+	for (int p = 0; p < n; p++) {
+		const BatchProblem& b = bt.probs[p];
+		const int nl = b.nlayers;
+		double* q = reinterpret_cast<double*>(pm + base[p]);
+		DevRtr& r = hs[p];
+		r.n = 16 * nl;
+		r.nl = nl;
+		r.radius = prm->radius_init;
+		r.x = q; q += 32 * nl;
+		r.xn = q; q += 32 * nl;
+		r.grad = q; q += 16 * nl;
+		r.z = q; q += 16 * nl;
+		r.r = q; q += 16 * nl;
+		r.d = q; q += 16 * nl;
+		r.cur_gates = reinterpret_cast<zc*>(q); q += 32 * nl;
+		r.cur_dirs = reinterpret_cast<zc*>(q); q += 32 * nl;
+		r.f_iter = q;
+		r.res = bt.res(b);
+		for (int k = 0; k < kKinds; k++) { r.flag[k] = bt.flags_dev(k, p); }
+		std::memcpy(himg.data() + base[p], vstart[p], sizeof(double) * 32 * nl);
+	}
+	DevRtr* dst = reinterpret_cast<DevRtr*>(pm + st_off);
+	RQC_HIP(hipMemcpyAsync(pm, himg.data(), himg.size(), hipMemcpyHostToDevice, s));
+	RQC_HIP(hipMemcpyAsync(dst, hs.data(), sizeof(DevRtr) * n, hipMemcpyHostToDevice, s));
+	// host fill of every table at the start gates (position terms; the rest is refilled on the device),
+	// the fill descriptors and zeroed flags
+	std::vector<DeviceTableDesc> desc((size_t)kKinds * n);
+	int* d_count = reinterpret_cast<int*>(bt.dmem + bt.off_misc);
+	int* d_bad = d_count + 1;
+	std::vector<std::vector<char>> timg((size_t)kKinds * n);
+	for (int k = 0; k < kKinds; k++) {
+		for (int p = 0; p < n; p++) {
+			BatchProblem& b = bt.probs[p];
+			const EvalPlan& pl = b.plan[k];
+			set_gates(b, vstart[p], nullptr);
+			if (k == 0 && !gates_near_unitary(b.C)) {
+				throw Error(-3, "the batched trust region needs start gates unitary to 1e-6 (problem " +
+					std::to_string(p) + ")");
+			}
+			auto& img = timg[(size_t)k * n + p];
+			img.assign(pl.tab_bytes, 0);
+			fill_eval_tables(pl, b.C, img.data());
+			RQC_HIP(hipMemcpyAsync(bt.dmem + bt.dtab_off[k][p], img.data(), img.size(), hipMemcpyHostToDevice, s));
+			DeviceTableDesc& d = desc[(size_t)k * n + p];
+			d.tab = bt.dmem + bt.dtab_off[k][p];
+			d.slot_group = reinterpret_cast<const int*>(bt.dmem + bt.dgroup_off[k][p]);
+			d.gates = hs[p].cur_gates;
+			d.dirs = hs[p].cur_dirs;
+			d.skip = bt.flags_dev(k, p);
+			d.bad = d_bad;
+			d.mats_bytes = pl.mats_bytes;
+			d.terms_bytes = pl.terms_bytes;
+			d.rmats_off = pl.rmats_off;
+			d.G = b.C.ngates;
+			d.ng = pl.ng;
+			d.mode = (int)pl.mode;
+			d.fp32 = pl.fp32 ? 1 : 0;
+		}
+	}
+	const DeviceTableDesc* d_desc = reinterpret_cast<const DeviceTableDesc*>(bt.dmem + bt.off_desc);
+	RQC_HIP(hipMemcpyAsync(const_cast<DeviceTableDesc*>(d_desc), desc.data(), sizeof(DeviceTableDesc) * desc.size(),
+		hipMemcpyHostToDevice, s));
+	RQC_HIP(hipMemsetAsync(bt.arena, 0xff, 3 * sizeof(int) * (size_t)n, s));   // every kind skipped
+	RQC_HIP(hipMemsetAsync(d_count, 0, 2 * sizeof(int), s));
+	auto graph = [&](int k) {
+		RQC_HIP(hipGraphLaunch(bt.dgraph[k], s));
+		bt.stats.graph_launches++;
+		bt.stats.steps++;
+	};
+	auto fill = [&](int k) { enqueue_fill_tables(s, d_desc + (size_t)k * n, n); };
+	const auto t0 = std::chrono::steady_clock::now();
+	const int maxiter = prm->tcg_params.maxiter;
+	constexpr int kChunk = 8;   // tCG steps between reads of the active count
+	for (int it = 0; it < niter; it++) {
+		k_rtr_begin<<<n, 64, 0, s>>>(dst);
+		fill(RQC_EVAL_GRADIENT);
+		graph(RQC_EVAL_GRADIENT);
+		k_rtr_tcg_init<<<n, 64, 0, s>>>(dst, maxiter, prm->tcg_params.abstol, prm->tcg_params.reltol);
+		fill(RQC_EVAL_HVP);
+		for (int j = 0; j < maxiter;) {
+			const int c = std::min(kChunk, maxiter - j);
+			for (int q = 0; q < c; q++) {
+				graph(RQC_EVAL_HVP);
+				k_rtr_tcg_step<<<n, 64, 0, s>>>(dst, maxiter);
+				fill(RQC_EVAL_HVP);
+			}
+			j += c;
+			k_rtr_count_active<<<1, 64, 0, s>>>(dst, n, d_count);
+			RQC_HIP(hipMemcpyAsync(bt.h_count, d_count, sizeof(int), hipMemcpyDeviceToHost, s));
+			RQC_HIP(hipStreamSynchronize(s));
+			if (bt.h_count[0] == 0) { break; }
+		}
+		k_rtr_candidate<<<n, 64, 0, s>>>(dst);
+		fill(RQC_EVAL_HVP);
+		graph(RQC_EVAL_HVP);
+		k_rtr_model<<<n, 64, 0, s>>>(dst);
+		fill(RQC_EVAL_TARGET);
+		graph(RQC_EVAL_TARGET);
+		k_rtr_accept<<<n, 64, 0, s>>>(dst, it, prm->rho_trust, prm->maxradius);
+	}
+	k_rtr_final_begin<<<n, 64, 0, s>>>(dst);
+	fill(RQC_EVAL_TARGET);
+	graph(RQC_EVAL_TARGET);
+	k_rtr_final<<<n, 64, 0, s>>>(dst, niter);
+	RQC_HIP(hipGetLastError());
+	std::vector<char> out(st_off);
+	RQC_HIP(hipMemcpyAsync(out.data(), pm, st_off, hipMemcpyDeviceToHost, s));
+	RQC_HIP(hipMemcpyAsync(hs.data(), dst, sizeof(DevRtr) * n, hipMemcpyDeviceToHost, s));
+	RQC_HIP(hipMemcpyAsync(bt.h_count, d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+	RQC_HIP(hipStreamSynchronize(s));
+	if (bt.h_count[1] != 0) { throw Error(-4, "singular gate in a batched trust-region step"); }
+	for (int p = 0; p < n; p++) {
+		const int nl = bt.probs[p].nlayers;
+		const double* q = reinterpret_cast<const double*>(out.data() + base[p]);
+		std::memcpy(vopt[p], q, sizeof(double) * 32 * nl);
+		std::memcpy(f_iters[p], q + (size_t)32 * nl * 2 + 16 * nl * 4 + 32 * nl * 2, sizeof(double) * (niter + 1));
+		bt.stats.evaluations += hs[p].evals;
+	}
+	bt.stats.device_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 }  // namespace rqc
 
 using namespace rqc;
@@ -577,6 +977,10 @@ extern "C" int rqc_batch_optimize_brickwall_hvp(struct rqc_batch* batch, const s
 		if (niter < 0) { throw Error(-3, "niter must be >= 0"); }
 		Batch& bt = batch->b;
 		std::lock_guard<std::mutex> lk(bt.mu);
+		if (bt.rtr == RQC_BATCH_RTR_DEVICE) {
+			batch_optimize_device(bt, vlist_start, params, niter, f_iters, vlist_opt);
+			return 0;
+		}
 		const int n = (int)bt.probs.size();
 		Lockstep ls;
 		ls.bt = &bt;
@@ -605,6 +1009,14 @@ extern "C" int rqc_batch_optimize_brickwall_hvp(struct rqc_batch* batch, const s
 		if (ls.failed) { throw Error(-1, "a batched evaluation step failed"); }
 		return 0;
 	} catch (const std::exception& e) { return report_error("rqc_batch_optimize_brickwall_hvp", e); }
+}
+
+extern "C" int rqc_batch_set_rtr(struct rqc_batch* batch, int mode)
+{
+	if (batch == nullptr || (mode != RQC_BATCH_RTR_DEVICE && mode != RQC_BATCH_RTR_HOST)) { return -3; }
+	std::lock_guard<std::mutex> lk(batch->b.mu);
+	batch->b.rtr = mode;
+	return 0;
 }
 
 extern "C" int rqc_batch_get_stats(const struct rqc_batch* batch, struct rqc_batch_stats* stats)

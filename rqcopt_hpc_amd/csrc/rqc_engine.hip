@@ -859,6 +859,70 @@ void fill_eval_tables(const EvalPlan& pl, const Circuit& C, char* host)
 	else { fill_tables<double>(pl, C, host); }
 }
 
+std::vector<int> table_slot_groups(const EvalPlan& pl)
+{
+	if (!pl.tiles) { throw Error(-3, "device-side tables need a tile-pass plan"); }
+	std::vector<int> g(pl.tp->order.size());
+	for (size_t x = 0; x < g.size(); x++) { g[x] = pl.tp->order[x] / pl.per; }
+	return g;
+}
+
+// The gate-dependent part of fill_tables on the device, one workgroup per
+// descriptor: the matrices of every execution slot (from its group's gate and
+// direction), the reduction maps and the groups' gates and directions.  The
+// arithmetic is fill_tables' (same mat4 routines); a singular gate sets *bad.
+template <typename T>
+__device__ void fill_slots_device(const DeviceTableDesc& d)
+{
+	const bool dual = (d.mode == (int)Mode::Hvp);
+	cx<T>* hm = reinterpret_cast<cx<T>*>(d.tab);
+	for (int x = threadIdx.x; x < d.G; x += blockDim.x) {
+		const int l = d.slot_group[x];
+		const zc* g = d.gates + 16 * (size_t)l;
+		zc m6[kMats][16];
+		for (int w = 0; w < kMats; w++) { m4_zero(m6[w]); }
+		for (int e = 0; e < 16; e++) { m6[kMatG][e] = g[e]; }
+		if (dual) { for (int e = 0; e < 16; e++) { m6[kMatZ][e] = d.dirs[16 * (size_t)l + e]; } }
+		if (d.mode != (int)Mode::Target) {
+			if (m4_inverse(g, m6[kMatGinv]) != 0) { *d.bad = 1; }
+			m4_transpose(g, m6[kMatGT]);
+			if (dual) {
+				zc t1[16];
+				m4_mul(m6[kMatGinv], m6[kMatZ], t1);
+				m4_mul(t1, m6[kMatGinv], m6[kMatW]);
+				for (int e = 0; e < 16; e++) { m6[kMatW][e] = zc{-m6[kMatW][e].x, -m6[kMatW][e].y}; }
+				m4_transpose(m6[kMatZ], m6[kMatZT]);
+			}
+		}
+		for (int w = 0; w < kMats; w++) {
+			for (int e = 0; e < 16; e++) { hm[((size_t)x * kMats + w) * 16 + e] = cx<T>{(T)m6[w][e].x, (T)m6[w][e].y}; }
+		}
+		if (d.rmats_off > 0) {
+			zc* rm = reinterpret_cast<zc*>(d.tab + d.rmats_off) + 32 * (size_t)x;
+			m4_transpose(m6[kMatGinv], rm);
+			m4_transpose(m6[kMatW], rm + 16);
+		}
+	}
+	zc* vin = reinterpret_cast<zc*>(d.tab + d.mats_bytes + d.terms_bytes);
+	for (int i = threadIdx.x; i < 16 * d.ng; i += blockDim.x) {
+		vin[i] = d.gates[i];
+		vin[16 * (size_t)d.ng + i] = dual ? d.dirs[i] : zc{0, 0};
+	}
+}
+
+__global__ __launch_bounds__(64) void k_fill_tables(const DeviceTableDesc* __restrict__ descs)
+{
+	const DeviceTableDesc& d = descs[blockIdx.x];
+	if (d.skip != nullptr && *d.skip != 0) { return; }
+	if (d.fp32) { fill_slots_device<float>(d); }
+	else { fill_slots_device<double>(d); }
+}
+
+void enqueue_fill_tables(hipStream_t s, const DeviceTableDesc* d_descs, int n)
+{
+	if (n > 0) { k_fill_tables<<<n, 64, 0, s>>>(d_descs); }
+}
+
 // tile-pass engine: several gates per HBM pass through LDS tiles
 template <typename T>
 static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* d_partial, const cx<T>* mats,

@@ -95,6 +95,26 @@ EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, boo
 	size_t budget, size_t held, const TileChoice* choice = nullptr, long ncols = 0);
 // host image of the evaluation's table region (pl.tab_bytes bytes at `host`)
 void fill_eval_tables(const EvalPlan& pl, const Circuit& C, char* host);
+
+// Device-side refill of the gate-dependent part of a tile-pass plan's table region, for
+// circuits whose gates are their groups' gates (brickwall layers: every gate of group l
+// is gates[l]); the position terms stay those of a host fill made once.  All pointers
+// are device addresses.
+struct DeviceTableDesc
+{
+	char* tab = nullptr;            // the plan's table region (pl.tab_bytes)
+	const int* slot_group = nullptr;    // group of each execution slot (table_slot_groups)
+	const zc* gates = nullptr;      // ng * 16: the groups' gates
+	const zc* dirs = nullptr;       // ng * 16: their directions (HVP plans)
+	const int* skip = nullptr;      // nothing is written when *skip != 0 (may be NULL)
+	int* bad = nullptr;             // set to 1 when a gate is singular
+	size_t mats_bytes = 0, terms_bytes = 0, rmats_off = 0;
+	int G = 0, ng = 0, mode = 0, fp32 = 0;
+};
+// group of every execution slot of a tile-pass plan
+std::vector<int> table_slot_groups(const EvalPlan& pl);
+// one workgroup per descriptor (descriptors in device memory)
+void enqueue_fill_tables(hipStream_t s, const DeviceTableDesc* d_descs, int n);
 // Block-encoding objective f = ||P^dagger W P - H||_F^2 / 2 in place of the
 // trace (sampled plans over the 2^(L/2) columns P e_b, with P putting bit i of
 // an index of L/2 qubits on bit 2i + 1 and the even bits |0>): the residual

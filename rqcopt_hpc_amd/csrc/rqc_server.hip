@@ -24,7 +24,9 @@
 #include <vector>
 
 #include "rqc_eval.hpp"
+#include "rqcopt/matrix.h"
 #include "rqcopt/rqc_server.h"
+#include "rqcopt/trust_region.h"
 
 namespace rqc {
 
@@ -501,4 +503,117 @@ extern "C" int rqc_server_get_stats(const struct rqc_server* server, struct rqc_
 	*stats = S.stats;
 	stats->peak_arena_bytes = S.alloc.peak;
 	return 0;
+}
+
+// ---------------------------------------------------------------- trust-region solves through the server
+// One optimize_brickwall_circuit_hvp (reference src/brickwall_opt.c:127-150; for a
+// general-circuit session optimize_quantum_circuit, src/circuit_opt.c:104-124, whose
+// groups are single gates) on the calling thread whose target, gradient and HVP
+// evaluations are requests of
+// the server: many solves on as many host threads share the device, their
+// requests admitted and overlapped like any others.  In mixed precision the
+// target request computes the gradient too (its f carries the gate-rounding
+// correction), and a gradient request at the same gates reuses it.
+namespace {
+
+struct SolveData
+{
+	rqc_server* server;
+	rqc_session* session;
+	int nl;
+	bool mixed;
+	rqc_request_info* rec;
+	int cap;
+	int n = 0;
+	bool failed = false;
+	std::vector<double> memo_x, memo_g;
+	double memo_f = 0;
+};
+
+double solve_request(SolveData& d, int kind, const double* x, const struct mat4x4* dirs, double* grad, double* hvp)
+{
+	numeric f;
+	long ticket = 0;
+	int r = rqc_server_submit(d.server, d.session, kind, reinterpret_cast<const struct mat4x4*>(x), dirs, &f, grad, hvp,
+		&ticket);
+	rqc_request_info info{};
+	if (r == 0) { r = rqc_server_wait(d.server, ticket, &info); }
+	if (r == 0) {
+		if (d.rec != nullptr && d.n < d.cap) { d.rec[d.n] = info; }
+		d.n++;
+		(void)rqc_server_release(d.server, ticket);
+	} else {
+		d.failed = true;
+		return std::nan("");
+	}
+	return f.real();
+}
+
+double sv_f_deriv(const double* RQC_RESTRICT x, void* fdata, double* RQC_RESTRICT grad)
+{
+	auto* d = static_cast<SolveData*>(fdata);
+	const size_t m = 32 * (size_t)d->nl, n = 16 * (size_t)d->nl;
+	if (d->mixed && d->memo_x.size() == m && std::memcmp(d->memo_x.data(), x, m * sizeof(double)) == 0) {
+		std::memcpy(grad, d->memo_g.data(), n * sizeof(double));
+		return d->memo_f;
+	}
+	const double f = solve_request(*d, RQC_EVAL_GRADIENT, x, nullptr, grad, nullptr);
+	if (d->mixed) {
+		d->memo_x.assign(x, x + m);
+		d->memo_g.assign(grad, grad + n);
+		d->memo_f = f;
+	}
+	return f;
+}
+
+double sv_f(const double* x, void* fdata)
+{
+	auto* d = static_cast<SolveData*>(fdata);
+	if (d->mixed) {
+		std::vector<double> g(16 * (size_t)d->nl);
+		return sv_f_deriv(x, fdata, g.data());
+	}
+	return solve_request(*d, RQC_EVAL_TARGET, x, nullptr, nullptr, nullptr);
+}
+
+void sv_f_hvp(const double* RQC_RESTRICT x, void* fdata, double* RQC_RESTRICT vec, double* RQC_RESTRICT hvp)
+{
+	auto* d = static_cast<SolveData*>(fdata);
+	const auto* v = reinterpret_cast<const struct mat4x4*>(x);
+	std::vector<struct mat4x4> dirs(d->nl);
+	for (int i = 0; i < d->nl; i++) { real_to_tangent(&vec[16 * i], &v[i], &dirs[i]); }
+	std::vector<double> grad(16 * (size_t)d->nl);
+	(void)solve_request(*d, RQC_EVAL_HVP, x, dirs.data(), grad.data(), hvp);
+}
+
+void sv_retract(const double* RQC_RESTRICT x, const double* RQC_RESTRICT eta, void* rdata, double* RQC_RESTRICT xs)
+{
+	const int nl = *static_cast<int*>(rdata);
+	const auto* v = reinterpret_cast<const struct mat4x4*>(x);
+	auto* w = reinterpret_cast<struct mat4x4*>(xs);
+	for (int i = 0; i < nl; i++) { retract(&v[i], &eta[16 * i], &w[i]); }
+}
+
+}  // namespace
+
+extern "C" int rqc_server_optimize_brickwall_hvp(struct rqc_server* server, struct rqc_session* session,
+	const struct mat4x4 vlist_start[], const struct rtr_params* params, int niter, double* f_iter,
+	struct mat4x4 vlist_opt[], struct rqc_request_info* records, int max_records, int* nrecords)
+{
+	try {
+		if (server == nullptr || session == nullptr || params == nullptr) { throw Error(-3, "NULL server, session or params"); }
+		if (niter < 0) { throw Error(-3, "niter must be >= 0"); }
+		if (!(0 <= params->rho_trust && params->rho_trust < 0.25)) { throw Error(-3, "rho_trust must lie in [0, 0.25)"); }
+		SolveData d{server, session, session->s.ng, session->s.fp32, records, records != nullptr ? max_records : 0};
+		struct rtr_params prm = *params;
+		prm.g_func = nullptr;
+		prm.g_data = nullptr;
+		prm.g_iter = nullptr;
+		int nl = d.nl;
+		riemannian_trust_region_optimize_hvp(sv_f, sv_f_deriv, sv_f_hvp, &d, sv_retract, &nl, nl * num_tangent_params,
+			reinterpret_cast<const double*>(vlist_start), nl * 32, &prm, niter, f_iter, reinterpret_cast<double*>(vlist_opt));
+		if (nrecords != nullptr) { *nrecords = d.n; }
+		if (d.failed) { throw Error(-1, "a request of the solve failed"); }
+		return 0;
+	} catch (const std::exception& e) { return report_error("rqc_server_optimize_brickwall_hvp", e); }
 }

@@ -190,3 +190,54 @@ def run_poisson(server, solves, sessions, rate, duration, seed, weights=(1 / 3, 
                              (("target", per_kind[EVAL_TARGET]), ("gradient", per_kind[EVAL_GRADIENT]),
                               ("hvp", per_kind[EVAL_HVP]))},
                 server=server.stats())
+
+
+def run_solves(server, solves, sessions, concurrency, niter, seconds, seed=0):
+    """Closed-loop load of real trust-region solves: `concurrency` host threads each run one solve at a
+    time (Server.optimize: every target / gradient / HVP evaluation of the solve's driver is a request),
+    taking the next solve of the workload, until `seconds` have passed.  Returns request latency
+    percentiles (submission to result on the host) and the completed request and solve rates; a
+    closed loop admits no backlog, so the achieved rate is the rate the device sustains at that
+    concurrency."""
+    import threading
+
+    from . import rtr_default_params
+    rng = np.random.default_rng(seed)
+    order = rng.permutation(len(solves))
+    lock = threading.Lock()
+    nxt = [0]
+    lat, nsolve, fdec, errors = [], [0], [0], []
+    t0 = time.perf_counter()
+
+    def worker():
+        while True:
+            with lock:
+                if time.perf_counter() - t0 > seconds:
+                    return
+                j = int(order[nxt[0] % len(order)])
+                nxt[0] += 1
+            s = solves[j]
+            try:
+                f_iter, _, recs = server.optimize(sessions[j], s["gates"], niter, rtr_default_params(16 * len(s["gates"])))
+            except Exception as e:  # reported below; a failed solve is a failed benchmark
+                errors.append(repr(e))
+                return
+            with lock:
+                lat.extend(r["latency_ms"] for r in recs)
+                nsolve[0] += 1
+                fdec[0] += int(f_iter[-1] <= f_iter[0])
+
+    ths = [threading.Thread(target=worker) for _ in range(concurrency)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    wall = time.perf_counter() - t0
+    if errors:
+        raise RuntimeError(f"{len(errors)} solves failed: {errors[0]}")
+    lat = np.array(lat)
+    return dict(concurrency=concurrency, niter=niter, seconds=wall, solves=nsolve[0], f_not_increased=fdec[0],
+                requests=int(lat.size), achieved_rate=lat.size / wall, solves_per_s=nsolve[0] / wall,
+                p50_ms=float(np.percentile(lat, 50)) if lat.size else None,
+                p99_ms=float(np.percentile(lat, 99)) if lat.size else None,
+                mean_ms=float(lat.mean()) if lat.size else None, server=server.stats())

@@ -149,6 +149,38 @@ def test_batch_lockstep_optimisation_matches_single_and_oracle():
     B.close()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_batch_device_trust_region_matches_host_driver(prec):
+    """The GPU-resident trust region (tCG recurrences, tangent directions, polar retraction and the
+    acceptance test in FP64 device kernels between graph launches) follows the host driver's
+    trajectory (one reference-style driver thread per problem over the same batch evaluations):
+    f_iter within 1e-12 relative and the gates within 1e-10 in FP64; in mixed precision the same
+    evaluations feed both drivers, so the same bounds hold.  The device driver needs no host
+    round trip per tCG step: its host syncs are one per chunk of 8 tCG steps."""
+    probs = _problems(7, [(6, 3), (8, 4), (10, 3), (6, 2), (8, 2)])
+    niter = 5
+    precision = R.PRECISION_FP64 if prec == "fp64" else R.PRECISION_MIXED
+    Bd = R.Batch([(p["P"], p["T"]) for p in probs], precision=precision)
+    Bh = R.Batch([(p["P"], p["T"]) for p in probs], precision=precision)
+    Bh.set_rtr(R.BATCH_RTR_HOST)
+    V = [p["V"] for p in probs]
+    st0 = Bd.stats()
+    fd, vd = Bd.optimize(V, niter)
+    st1 = Bd.stats()
+    fh, vh = Bh.optimize(V, niter)
+    launches = st1["graph_launches"] - st0["graph_launches"]
+    print(f"{prec}: device driver {launches} graph launches over {niter} iterations "
+          f"({launches / niter:.1f} per iteration), {st1['evaluations'] - st0['evaluations']} evaluations", flush=True)
+    for i, p in enumerate(probs):
+        ef = np.max(np.abs(fd[i] - fh[i]) / np.maximum(1, np.abs(fh[i])))
+        ev = np.max(np.abs(vd[i] - vh[i]))
+        print(f"problem {i} L={p['L']} d={p['d']}: f_iter {fd[i]} | max rel diff {ef:.2e}, gates {ev:.2e}", flush=True)
+        assert ef < 1e-12 and ev < 1e-10
+        assert fd[i][-1] < fd[i][0]
+    Bd.close()
+    Bh.close()
+
+
 def test_batch_rejects_invalid_problems():
     probs = _problems(5, [(6, 2)])
     with pytest.raises(R.RqcError):

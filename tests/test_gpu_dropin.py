@@ -6,7 +6,8 @@ the container that holds the reference checkout, oracle/Makefile target dropin; 
 itself is not read here).  Its optimize_* functions call the library's target, gradient, HVP,
 Hessian and trust-region entry points exactly as the reference calls its own, so their
 trajectories must be those of the reference's example runs and of the oracle (1e-9 in f_iter,
-1e-8 in the gates), and equal to the library's own glue.
+1e-8 in the gates; the molecular run at its measured sensitivity, tests/helpers.py), and equal to
+the library's own glue.
 """
 import ctypes as ct
 import os
@@ -15,7 +16,7 @@ import numpy as np
 import pytest
 
 import rqcopt_hpc_amd as R
-from helpers import cplx, example_ising1d, example_molecular, fixture
+from helpers import assert_molecular_trajectory, cplx, example_ising1d, example_molecular, fixture
 
 pytestmark = pytest.mark.gpu
 
@@ -78,6 +79,8 @@ def test_reference_circuit_glue_on_library():
     fp, ud = R._target_args(T)
     _dropin().optimize_quantum_circuit(fp, ud, R._p(G0), ct.c_int(len(G0)), ct.c_int(5), R._p(w), ct.byref(prm),
                                        ct.c_int(10), R._p(f_iter), R._p(gopt))
-    print("reference glue (circuit) vs reference example", _rel(f_iter, run["f_iter"]), flush=True)
-    assert _rel(f_iter, run["f_iter"]) < 1e-9 and np.max(np.abs(gopt - cplx(run["gates"]))) < 1e-8
+    # the molecular run's late iterates move under rounding-level changes (tests/helpers.py)
+    assert_molecular_trajectory(f_iter, gopt, run)
+    f_own, g_own = R.optimize_quantum_circuit(T, G0, W, 5, 10)
+    assert np.array_equal(f_iter, f_own) and np.array_equal(gopt, g_own)
     T.close()

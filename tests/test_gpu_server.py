@@ -238,3 +238,54 @@ def test_serving_workload_large_sessions_alone_and_in_the_mix():
         for t in dev.values():
             t.close()
     srv.close()
+
+
+def test_concurrent_trust_region_solves_through_the_server():
+    """Trust-region solves on several host threads, every evaluation of each solve's driver a server
+    request (rqc_server_optimize_brickwall_hvp): each solve's trajectory equals the library's own
+    optimiser on the same problem (FP64: the same plans and kernels, so the same rounding, and
+    1e-12 relative in f_iter / 1e-10 in the gates allows for a different column block), also
+    while the other solves' requests run beside it; a general-circuit session follows
+    optimize_quantum_circuit the same way."""
+    import threading
+
+    rng = np.random.default_rng(11)
+    server = R.Server(arena_bytes=8 * GiB, nslots=4)
+    specs = [(8, 3, True), (6, 4, True), (10, 3, True), (7, 2, False)]
+    solves = []
+    for i, (L, d, bw) in enumerate(specs):
+        T = _target(L, 200 + i)
+        if bw:
+            P = brickwall_perms(L, d)
+            s = server.session_brickwall(P, T)
+            V = np.stack([haar_unitary(rng) for _ in range(d)])
+            ref = R.optimize_brickwall_circuit_hvp(T, V, P, 3)
+        else:
+            wires = S.open_brickwall_wires(L, d)
+            s = server.session_circuit(wires, L, T)
+            V = np.stack([haar_unitary(rng) for _ in range(len(wires))])
+            ref = R.optimize_quantum_circuit(T, V, wires, L, 3)
+        solves.append(dict(s=s, T=T, V=V, ref=ref))
+    out = [None] * len(solves)
+
+    def run(i):
+        out[i] = server.optimize(solves[i]["s"], solves[i]["V"], 3)
+
+    ths = [threading.Thread(target=run, args=(i,)) for i in range(len(solves))]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    for i, sv in enumerate(solves):
+        assert out[i] is not None, f"solve {i} raised"
+        f_iter, vopt, recs = out[i]
+        f_ref, v_ref = sv["ref"]
+        ef = np.max(np.abs(f_iter - f_ref) / np.maximum(1, np.abs(f_ref)))
+        print(f"solve {i}: {len(recs)} requests, f_iter {f_iter}, rel diff {ef:.2e}", flush=True)
+        assert ef < 1e-12 and np.max(np.abs(vopt - v_ref)) < 1e-10
+        assert len(recs) > 3 and all(r["status"] == 0 for r in recs)
+    assert server.stats()["peak_in_flight"] >= 2
+    for sv in solves:
+        sv["s"].close()
+        sv["T"].close()
+    server.close()
