@@ -102,6 +102,21 @@ int rqc_profile_query(int kind, double* total_ms, long* launches, double* alg_by
    calibrate HBM byte counters and the roofline peak on the running device. */
 int rqc_measure_copy_bandwidth(size_t bytes, int reps, double* gbps);
 
+/* No counterpart in the reference: the same evaluation as
+   brickwall_unitary_target_projected_hessian_vector_product (target.h), for callers that
+   cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc).  It allocates and frees no device memory: its whole workspace is a
+   static device array of rqc_noalloc_scratch_bytes() bytes (256 MiB), held by each GPU the
+   library runs on until the process ends, and the column block is the widest that fits it.
+   U must be a device target (ufunc = rqc_device_ufunc, udata = rqc_target*).  A circuit whose
+   single-column plan exceeds the scratch returns -3 with a message.  Calls are serialised by
+   the library and finish before they return; the scratch keeps the last call's contents, and
+   anything that overlapped it on one GPU outside the library would race. */
+int rqc_brickwall_unitary_target_projected_hvp_noalloc(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const struct mat4x4 vdirs[], const int nlayers, const int* perms[], const int nqubits,
+	numeric* fval, double* RQC_RESTRICT grad_vec, double* RQC_RESTRICT hvp_vec);
+size_t rqc_noalloc_scratch_bytes(void);
+
 /* blocks until all work of the library's stream has finished */
 int rqc_synchronize(void);
 /* "gfx950 <name> <CUs> <HBM bytes>" into buf; returns 0 when a device is present */

@@ -130,6 +130,7 @@ EXPORTED = [
     "rqc_set_workspace_limit", "rqc_release_workspace", "rqc_last_column_block", "rqc_profile_enable",
     "rqc_profile_reset", "rqc_profile_query", "rqc_synchronize", "rqc_device_info", "rqc_measure_copy_bandwidth",
     "rqc_set_sweep_engine", "rqc_last_sweep", "rqc_target_create_circuit",
+    "rqc_brickwall_unitary_target_projected_hvp_noalloc", "rqc_noalloc_scratch_bytes",
     # rqc_batch.h
     "rqc_batch_create", "rqc_batch_destroy", "rqc_batch_set_exec", "rqc_batch_evaluate",
     "rqc_batch_optimize_brickwall_hvp", "rqc_batch_get_stats", "rqc_batch_problem_plan", "rqc_batch_problem_tiles",
@@ -168,6 +169,7 @@ def lib():
         L.seed_rng_state.argtypes = [ct.c_uint64, ct.POINTER(RngState)]
         L.truncated_cg_hvp.restype = ct.c_bool
         L.rqc_last_column_block.restype = ct.c_long
+        L.rqc_noalloc_scratch_bytes.restype = ct.c_size_t
         L.rqc_set_workspace_limit.argtypes = [ct.c_size_t]
         L.rqc_profile_query.argtypes = [ct.c_int, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long), ct.POINTER(ct.c_double)]
         _LIB = L
@@ -632,6 +634,25 @@ def brickwall_unitary_target_projected_hessian_vector_product(target, vlist, vdi
            lib().brickwall_unitary_target_projected_hessian_vector_product(fp, ud, _p(vlist), _p(vdirs), len(vlist),
                                                                            rows, L, _p(f), _p(g), _p(h)))
     return f[0], g, h
+
+
+def brickwall_unitary_target_projected_hvp_noalloc(target, vlist, vdirs, perms, nqubits=None):
+    """The projected grad+HVP evaluation with the library's static device scratch as its whole
+    workspace (no device allocation); `target` must be a DeviceTarget."""
+    vlist, vdirs, f = _gates(vlist), _gates(vdirs), _f()
+    perms, rows = _perms(perms)
+    L = perms.shape[1] if nqubits is None else nqubits
+    g = np.empty(16 * len(vlist))
+    h = np.empty(16 * len(vlist))
+    fp, ud = _target_args(target)
+    _check("rqc_brickwall_unitary_target_projected_hvp_noalloc",
+           lib().rqc_brickwall_unitary_target_projected_hvp_noalloc(fp, ud, _p(vlist), _p(vdirs), len(vlist), rows, L,
+                                                                   _p(f), _p(g), _p(h)))
+    return f[0], g, h
+
+
+def noalloc_scratch_bytes():
+    return int(lib().rqc_noalloc_scratch_bytes())
 
 
 def brickwall_unitary_target_gradient_hessian(target, vlist, perms, nqubits=None):

@@ -39,6 +39,13 @@
 namespace rqc {
 
 // ============================================================== context
+// Workspace of the no-allocation evaluation entry point: a file-scope device
+// array, allocated with the code object when it loads on a device and held
+// until the process ends.  Host code never passes its name to a kernel (on the
+// host it is a stand-in, not a device address): the context looks up its
+// device address once, with the context's device current, and keeps it.
+static __device__ __align__(256) char scratch[kScratchBytes];
+
 void DeviceBuffer::reserve(size_t n)
 {
 	if (n <= bytes) { return; }
@@ -85,6 +92,13 @@ Context& ctx()
 		const char* at = getenv("RQC_AUTOTUNE");
 		if (at != nullptr && std::string(at) == "0") { cc->autotune = false; }
 		RQC_HIP(hipStreamCreateWithFlags(&cc->stream, hipStreamNonBlocking));
+		void* sp = nullptr;
+		const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+		if (se != hipSuccess || sp == nullptr) {
+			throw Error(-1, std::string("librqcopt_hip: hipGetSymbolAddress of the scratch array failed: ") +
+				hipGetErrorString(se));
+		}
+		cc->scratch = static_cast<char*>(sp);
 		c = cc;
 	}
 	return *c;
@@ -2080,6 +2094,65 @@ extern "C" int brickwall_unitary_target_projected_hessian_vector_product(linear_
 		std::memcpy(hvp_vec, R.hvp.data(), R.hvp.size() * sizeof(double));
 	});
 }
+
+// ============================================================== no-allocation evaluation
+// The evaluation above with the context's static scratch array as its whole
+// device workspace: the column block is the widest whose plan (state arrays,
+// partial sums, tables, results) fits kScratchBytes, and no device memory is
+// allocated or freed.  U must be a device target (a host callback would need a
+// device copy of U).  Calls hold the library lock and synchronise before they
+// return, so they never overlap on the device; the scratch keeps the last
+// call's contents, and every buffer that must start at zero is cleared by the
+// evaluation's own fill kernels.
+static EvalPlan plan_in_scratch(const Context& c, const Circuit& C, Mode mode, int per, bool fp32, bool project,
+	bool unitary)
+{
+	size_t budget = kScratchBytes;
+	for (int it = 0; it < 64 && budget > 0; it++) {
+		EvalPlan pl;
+		try {
+			pl = plan_eval(c, C, mode, per, fp32, project, unitary, budget, 0);
+		} catch (const Error&) {
+			break;   // not even one column's state arrays fit
+		}
+		if (pl.bytes <= kScratchBytes) { return pl; }
+		const size_t over = pl.bytes - kScratchBytes;
+		budget = (over < budget) ? budget - over : 0;
+	}
+	throw Error(-3, "a " + std::to_string(C.L) + "-qubit circuit of " + std::to_string(C.ngates) +
+		" gates does not fit the no-allocation scratch of " + std::to_string(kScratchBytes >> 20) + " MiB");
+}
+
+extern "C" int rqc_brickwall_unitary_target_projected_hvp_noalloc(linear_func ufunc, void* udata,
+	const struct mat4x4 vlist[], const struct mat4x4 vdirs[], const int nlayers, const int* perms[], const int nqubits,
+	numeric* fval, double* RQC_RESTRICT grad_vec, double* RQC_RESTRICT hvp_vec)
+{
+	return target_call("rqc_brickwall_unitary_target_projected_hvp_noalloc", [&] {
+		if (ufunc != rqc_device_ufunc) {
+			throw Error(-3, "the no-allocation evaluation needs rqc_device_ufunc with an rqc_target* as udata");
+		}
+		Circuit C = make_brickwall(vlist, vdirs, nlayers, perms, nqubits);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const zc* dU = as_target(udata, C.L)->d_U;
+		const bool fp32 = c.precision == RQC_PRECISION_MIXED;
+		const EvalPlan pl = plan_in_scratch(c, C, Mode::Hvp, nqubits / 2, fp32, true, sweep_unitary(C, Mode::Hvp, fp32));
+		std::vector<char> tab(pl.tab_bytes);
+		fill_eval_tables(pl, C, tab.data());
+		enqueue_eval(c, c, pl, C, dU, c.scratch, nullptr, nullptr, tab.data(), nullptr, nullptr, nullptr);
+		std::vector<char> res(pl.res_bytes());
+		RQC_HIP(hipMemcpyAsync(res.data(), c.scratch + pl.off_res, res.size(), hipMemcpyDeviceToHost, c.stream));
+		RQC_HIP(hipStreamSynchronize(c.stream));
+		RQC_HIP(hipGetLastError());
+		EvalResult R;
+		unpack_result(pl, res.data(), R);
+		put_f(fval, R);
+		std::memcpy(grad_vec, R.grad.data(), R.grad.size() * sizeof(double));
+		std::memcpy(hvp_vec, R.hvp.data(), R.hvp.size() * sizeof(double));
+	});
+}
+
+extern "C" size_t rqc_noalloc_scratch_bytes(void) { return kScratchBytes; }
 
 // ============================================================== sampled (stochastic-trace) targets
 // The trace over the 2^L basis columns is replaced by nsamples Haar-random

@@ -78,6 +78,7 @@ struct Context : Slot
 	DeviceBuffer vecs;     // uploaded statevectors of the statevector-level API
 	DeviceBuffer ubuf;     // U assembled from a host callback
 	DeviceBuffer samples;  // sample states and trace seeds of stochastic-trace evaluations
+	char* scratch = nullptr;   // device address of the static no-allocation scratch array (kScratchBytes)
 
 	double prof_ms[RQC_KERNEL_KINDS] = {0, 0, 0, 0};
 	long prof_n[RQC_KERNEL_KINDS] = {0, 0, 0, 0};
@@ -88,6 +89,9 @@ struct Context : Slot
 
 // the process-wide context on the current HIP device; throws if no GPU is present
 Context& ctx();
+
+// size of the static device array that the no-allocation entry point uses as its workspace
+constexpr size_t kScratchBytes = (size_t)256 << 20;
 
 // bracket a kernel launch with timing events when the slot profiles; `bytes` is the
 // launch's algorithmic HBM traffic (arrays it must read and write once), summed per kind
