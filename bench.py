@@ -126,6 +126,8 @@ def run(args):
     V, P, vec, Z = config_inputs(L, d, args.seed)
     Vt, Pt = ising_trotter_brickwall(L, 1.0, 0.75, 0.6, 0.5, 4)
     T = R.DeviceTarget.brickwall(Vt, Pt)
+    # tile shape and gates per pass timed on the first (warm-up) evaluation of each precision
+    R.set_eval_autotune(not args.no_autotune)
 
     def one():
         return R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
@@ -139,7 +141,8 @@ def run(args):
     results = {}
     for prec, name in precs:
         R.set_precision(prec)
-        for _ in range(args.warmup):
+        # at least one untimed evaluation when the tiles are tuned on the first one
+        for _ in range(max(args.warmup, 0 if args.no_autotune else 1)):
             out = one()
         R.synchronize()
         t0 = time.perf_counter()
@@ -154,8 +157,10 @@ def run(args):
             one()
             R.profile_enable(False)
         prof = {k: R.profile_query(k) for k in range(4)}
-        results[name] = {"eval_s": dt / args.steps, "out": out, "prof": prof, "block": R.last_column_block()}
+        results[name] = {"eval_s": dt / args.steps, "out": out, "prof": prof, "block": R.last_column_block(),
+                         "tiles": R.last_tiles()}
         print(f"[bench] {name}: {dt / args.steps * 1e3:.2f} ms/eval, column block {R.last_column_block()}, "
+              f"tiles (log2 rows, log2 cols, gates per pass) {R.last_tiles()}, "
               f"kernel ms {[round(prof[k][0], 2) for k in range(4)]}, launches {[prof[k][1] for k in range(4)]}, "
               f"GB/s {[round(prof[k][2] / max(prof[k][0], 1e-9) / 1e6) for k in range(4)]}",
               file=sys.stderr, flush=True)
@@ -216,7 +221,10 @@ def run(args):
         "data": "synthetic (seeded PCG Haar gates, Ising-Trotter target formed on the GPU)",
         "copy_bandwidth_gbs": copy_gbs,
         "config": {"workload": f"brickwall grad+HVP eval, L={L}, depth {d}, periodic, Ising-Trotter U (t=0.5, 4 Strang steps)",
-                   "L": L, "depth": d, "gates": G, "column_block": B, "sweep_engine": engine, "device": info},
+                   "L": L, "depth": d, "gates": G, "column_block": B, "sweep_engine": engine, "device": info,
+                   "tiles": {"fp64": r64["tiles"], "mixed": r32["tiles"],
+                             "autotuned": not args.no_autotune,
+                             "fields": "log2 tile rows (fused qubits K), log2 tile columns, most gates per pass"}},
         "roofline": ({"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                       "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,
                       "alg_flops_per_launch": rev_flops / max(rev_n, 1),
@@ -395,6 +403,8 @@ def main():
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both",
                     help="profiling runs: time one precision only and print a short summary")
     ap.add_argument("--no-profile-pass", action="store_true", help="skip the HIP-event kernel timing pass")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="default tiles instead of timing the tile candidates on the first evaluation")
     ap.add_argument("--mixed-seeds", type=int, default=20, help="seeded gate sets of the mixed-precision error sweep")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-problems section")
     ap.add_argument("--batch-problems", type=int, default=64)

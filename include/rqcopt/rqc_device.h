@@ -76,6 +76,15 @@ int rqc_set_sweep_engine(int engine);
    structure and precision */
 int rqc_set_autotune(int on);
 
+/* tile autotuning of single evaluations (process-wide; default off, or the environment
+   variable RQC_EVAL_AUTOTUNE=1): the first target / gradient / HVP evaluation of a circuit
+   structure times the tile height (2^10 or 2^9 FP64 elements, 2^11 or 2^10 FP32, which fixes the
+   fused qubits K per pass) and the most gates per pass (8 or 6) on the evaluation itself and keeps
+   the fastest for that structure; on = 2 also forgets the choices made so far */
+int rqc_set_eval_autotune(int on);
+/* tile shape (2^rows x 2^cols amplitudes) and most gates per pass of the last evaluation */
+int rqc_last_tiles(int* tile_rows_log2, int* tile_cols_log2, int* gates_per_pass);
+
 /* device workspace: upper bound in bytes (0 = automatic, about 85% of free HBM);
    the workspace is kept between calls and released here */
 int rqc_set_workspace_limit(size_t bytes);

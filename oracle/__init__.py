@@ -125,6 +125,25 @@ def apply_gate_backward(G, i, j, psi, dpsi_out):
     return dpsi, dG
 
 
+def apply_gate_1q(V, i, psi):
+    """Single-qubit gate V (2x2) on wire i (no reference counterpart; two-qubit conventions)."""
+    psi = _cplx(psi)
+    out = np.empty_like(psi)
+    L = int(np.log2(psi.size))
+    lib().orc_apply_gate_1q(_p(_cplx(V)), _i(i), _i(L), _p(psi), _p(out))
+    return out
+
+
+def apply_gate_1q_backward(V, i, psi, dpsi_out):
+    psi = _cplx(psi)
+    dpsi_out = _cplx(dpsi_out)
+    dpsi = np.empty_like(psi)
+    dV = np.empty((2, 2), np.complex128)
+    L = int(np.log2(psi.size))
+    lib().orc_apply_gate_1q_backward(_p(_cplx(V)), _i(i), _i(L), _p(psi), _p(dpsi_out), _p(dpsi), _p(dV))
+    return dpsi, dV
+
+
 def apply_gate_to_array(G, i, j, psi):
     """psi: (2^L, nstates) interleaved array."""
     psi = _cplx(psi)

@@ -104,6 +104,43 @@ void orc_apply_gate_backward(const ocplx* gate, int i, int j, int L, const ocplx
 	}
 }
 
+/* Single-qubit gate on wire i (qubit i = bit L-1-i).  The reference has two-qubit gates
+   only; this follows its two-qubit conventions (src/gate.c:10-76 and :83-188: row-major
+   gate, dpsi = gate^T dpsi_out, dgate = sum dpsi_out (x) psi without conjugation), pair by
+   pair in ascending order.  Parity unpinned by the reference: pinned by dense Kronecker
+   products and the bilinear identity in tests/test_oracle_pinning.py. */
+void orc_apply_gate_1q(const ocplx* gate, int i, int L, const ocplx* psi, ocplx* psi_out)
+{
+	const long np = 1L << (L - 1);
+	const int b = L - 1 - i;
+	const long s = 1L << b;
+	for (long q = 0; q < np; q++) {
+		const long k = ((q >> b) << (b + 1)) | (q & (s - 1));
+		const ocplx x = psi[k], y = psi[k + s];
+		psi_out[k]     = gate[0]*x + gate[1]*y;
+		psi_out[k + s] = gate[2]*x + gate[3]*y;
+	}
+}
+
+void orc_apply_gate_1q_backward(const ocplx* gate, int i, int L, const ocplx* psi, const ocplx* dpsi_out,
+	ocplx* dpsi, ocplx* dgate)
+{
+	const long np = 1L << (L - 1);
+	const int b = L - 1 - i;
+	const long s = 1L << b;
+	for (int e = 0; e < 4; e++) { dgate[e] = 0; }
+	for (long q = 0; q < np; q++) {
+		const long k = ((q >> b) << (b + 1)) | (q & (s - 1));
+		const ocplx v[2] = { psi[k], psi[k + s] };
+		const ocplx d[2] = { dpsi_out[k], dpsi_out[k + s] };
+		for (int r = 0; r < 2; r++) {
+			for (int c = 0; c < 2; c++) { dgate[2*r + c] += d[r] * v[c]; }
+		}
+		dpsi[k]     = gate[0]*d[0] + gate[2]*d[1];
+		dpsi[k + s] = gate[1]*d[0] + gate[3]*d[1];
+	}
+}
+
 /* Gate on an interleaved statevector array, reference src/gate.c:195-262:
    the trailing stride is multiplied by nstates. */
 void orc_apply_gate_to_array(const ocplx* gate, int i, int j, int L, int nstates, const ocplx* psi, ocplx* psi_out)

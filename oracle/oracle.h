@@ -38,6 +38,10 @@ void orc_apply_gate(const ocplx* gate, int i, int j, int L, const ocplx* psi, oc
 void orc_apply_gate_backward(const ocplx* gate, int i, int j, int L, const ocplx* psi,
 	const ocplx* dpsi_out, ocplx* dpsi, ocplx* dgate);
 void orc_apply_gate_to_array(const ocplx* gate, int i, int j, int L, int nstates, const ocplx* psi, ocplx* psi_out);
+/* single-qubit gate on wire i (no reference counterpart; the two-qubit conventions) */
+void orc_apply_gate_1q(const ocplx* gate, int i, int L, const ocplx* psi, ocplx* psi_out);
+void orc_apply_gate_1q_backward(const ocplx* gate, int i, int L, const ocplx* psi, const ocplx* dpsi_out,
+	ocplx* dpsi, ocplx* dgate);
 
 /* ---- circuit level: reference src/quantum_circuit.c ---- */
 int orc_apply_quantum_circuit(const ocplx* gates, int ngates, const int* wires, int L, const ocplx* psi, ocplx* psi_out);

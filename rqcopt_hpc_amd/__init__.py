@@ -134,7 +134,11 @@ EXPORTED = [
     # rqc_batch.h
     "rqc_batch_create", "rqc_batch_destroy", "rqc_batch_set_exec", "rqc_batch_evaluate",
     "rqc_batch_optimize_brickwall_hvp", "rqc_batch_get_stats", "rqc_batch_problem_plan", "rqc_batch_problem_tiles",
-    "rqc_set_autotune", "rqc_batch_set_rtr",
+    "rqc_set_autotune", "rqc_batch_set_rtr", "rqc_set_eval_autotune", "rqc_last_tiles",
+    # rqc_gate1q.h
+    "rqc_apply_single_qubit_gate", "rqc_apply_single_qubit_gate_backward", "rqc_apply_single_qubit_gate_to_array",
+    "rqc_mixed_circuit_unitary_target", "rqc_mixed_circuit_unitary_target_and_gradient",
+    "rqc_mixed_circuit_unitary_target_hessian_vector_product",
     # rqc_server.h
     "rqc_server_create", "rqc_server_destroy", "rqc_session_create_brickwall", "rqc_session_create_circuit",
     "rqc_session_ngroups", "rqc_session_destroy", "rqc_server_submit", "rqc_server_wait", "rqc_server_poll",
@@ -566,6 +570,98 @@ def circuit_unitary_target_hessian_vector_product(target, gates, gatedirs, wires
            lib().circuit_unitary_target_hessian_vector_product(fp, ud, _p(gates), _p(dirs), len(gates), _p(w), nqubits,
                                                                _p(f), _p(dg), _p(hg)))
     return f[0], dg, hg
+
+
+# --------------------------------------------------------------- single-qubit gates and mixed circuits
+def _mat2(a):
+    a = _cplx(a)
+    if a.shape != (2, 2):
+        raise ValueError(f"expected a 2x2 matrix, got shape {a.shape}")
+    return a
+
+
+def apply_single_qubit_gate(V, i, psi):
+    """psi_out = V on wire i (2x2 V, row-major; qubit 0 the most significant bit)."""
+    psi = _cplx(psi)
+    out = np.empty_like(psi)
+    _check("rqc_apply_single_qubit_gate", lib().rqc_apply_single_qubit_gate(_p(_mat2(V)), i, ct.byref(_sv(psi)),
+                                                                            ct.byref(_sv(out))))
+    return out
+
+
+def apply_single_qubit_gate_backward(V, i, psi, dpsi_out):
+    """(dpsi = V^T dpsi_out on wire i, dV = sum dpsi_out (x) psi)."""
+    psi, dpsi_out = _cplx(psi), _cplx(dpsi_out)
+    dpsi = np.empty_like(psi)
+    dV = np.empty((2, 2), np.complex128)
+    _check("rqc_apply_single_qubit_gate_backward",
+           lib().rqc_apply_single_qubit_gate_backward(_p(_mat2(V)), i, ct.byref(_sv(psi)), ct.byref(_sv(dpsi_out)),
+                                                      ct.byref(_sv(dpsi)), _p(dV)))
+    return dpsi, dV
+
+
+def apply_single_qubit_gate_to_array(V, i, psi):
+    """psi: (2^L, nstates) interleaved statevector array."""
+    psi = _cplx(psi)
+    out = np.empty_like(psi)
+    L = _nq(psi.shape[0])
+    a = StatevectorArray(psi.ctypes.data, L, psi.shape[1])
+    b = StatevectorArray(out.ctypes.data, L, psi.shape[1])
+    _check("rqc_apply_single_qubit_gate_to_array",
+           lib().rqc_apply_single_qubit_gate_to_array(_p(_mat2(V)), i, ct.byref(a), ct.byref(b)))
+    return out
+
+
+def _mixed(gates, nq, wires):
+    nq = np.ascontiguousarray(nq, np.int32)
+    if len(gates) != nq.size:
+        raise ValueError("one qubit count per gate expected")
+    packed = np.concatenate([(_mat2(G) if q == 1 else _mat4(G)).reshape(-1) for G, q in zip(gates, nq)])
+    w = np.ascontiguousarray(np.asarray(wires).reshape(-1, 2), np.int32)
+    return packed, nq, w
+
+
+def _unpack_mixed(flat, nq):
+    out, o = [], 0
+    for q in nq:
+        m = 2 if q == 1 else 4
+        out.append(flat[o:o + m * m].reshape(m, m).copy())
+        o += m * m
+    return out
+
+
+def mixed_circuit_unitary_target(target, gates, nq, wires, nqubits):
+    """f = -Tr(U^dagger W) for a circuit of one-qubit (nq 1, 2x2, wires (i, any)) and two-qubit gates."""
+    g, nq, w = _mixed(gates, nq, wires)
+    f = _f()
+    fp, ud = _target_args(target)
+    _check("rqc_mixed_circuit_unitary_target",
+           lib().rqc_mixed_circuit_unitary_target(fp, ud, _p(g), _p(nq), _p(w), len(nq), nqubits, _p(f)))
+    return f[0]
+
+
+def mixed_circuit_unitary_target_and_gradient(target, gates, nq, wires, nqubits):
+    g, nq, w = _mixed(gates, nq, wires)
+    f = _f()
+    dg = np.empty_like(g)
+    fp, ud = _target_args(target)
+    _check("rqc_mixed_circuit_unitary_target_and_gradient",
+           lib().rqc_mixed_circuit_unitary_target_and_gradient(fp, ud, _p(g), _p(nq), _p(w), len(nq), nqubits, _p(f),
+                                                               _p(dg)))
+    return f[0], _unpack_mixed(dg, nq)
+
+
+def mixed_circuit_unitary_target_hessian_vector_product(target, gates, gatedirs, nq, wires, nqubits):
+    g, nq, w = _mixed(gates, nq, wires)
+    d, _, _ = _mixed(gatedirs, nq, wires)
+    f = _f()
+    dg = np.empty_like(g)
+    hg = np.empty_like(g)
+    fp, ud = _target_args(target)
+    _check("rqc_mixed_circuit_unitary_target_hessian_vector_product",
+           lib().rqc_mixed_circuit_unitary_target_hessian_vector_product(fp, ud, _p(g), _p(d), _p(nq), _p(w), len(nq),
+                                                                         nqubits, _p(f), _p(dg), _p(hg)))
+    return f[0], _unpack_mixed(dg, nq), _unpack_mixed(hg, nq)
 
 
 def circuit_unitary_target_projected_hessian_vector_product(target, gates, gatedirs, wires, nqubits):
@@ -1227,6 +1323,19 @@ def set_precision(mode):
 
 def set_autotune(on):
     _check("rqc_set_autotune", lib().rqc_set_autotune(1 if on else 0))
+
+
+def set_eval_autotune(on, forget=False):
+    """Tile autotuning of single evaluations: the first evaluation of each circuit structure times
+    the tile candidates and keeps the fastest (forget=True also drops the choices made so far)."""
+    _check("rqc_set_eval_autotune", lib().rqc_set_eval_autotune(2 if forget else (1 if on else 0)))
+
+
+def last_tiles():
+    """(log2 tile rows, log2 tile columns, most gates per pass) of the last evaluation."""
+    k, c, g = ct.c_int(), ct.c_int(), ct.c_int()
+    _check("rqc_last_tiles", lib().rqc_last_tiles(ct.byref(k), ct.byref(c), ct.byref(g)))
+    return k.value, c.value, g.value
 
 
 def set_sweep_engine(engine):

@@ -33,6 +33,7 @@
 #include "rqcopt/brickwall_circuit.h"
 #include "rqcopt/target.h"
 #include "rqcopt/rqc_device.h"
+#include "rqcopt/rqc_gate1q.h"
 #include "rqcopt/rng.h"
 #include <rocblas/rocblas.h>
 
@@ -91,6 +92,8 @@ Context& ctx()
 		if (eng != nullptr && std::string(eng) == "pergate") { cc->engine = RQC_ENGINE_PER_GATE; }
 		const char* at = getenv("RQC_AUTOTUNE");
 		if (at != nullptr && std::string(at) == "0") { cc->autotune = false; }
+		const char* ea = getenv("RQC_EVAL_AUTOTUNE");
+		if (ea != nullptr && std::string(ea) == "1") { cc->eval_autotune = true; }
 		RQC_HIP(hipStreamCreateWithFlags(&cc->stream, hipStreamNonBlocking));
 		void* sp = nullptr;
 		const hipError_t se = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
@@ -467,6 +470,10 @@ static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC,
 	a.lognc = lognc;
 	a.ng = (int)p.gates.size();
 	a.x0 = x0;
+	{
+		const char* w = getenv("RQC_REV_WIDE");
+		a.wide = (w == nullptr) ? 1 : atoi(w);
+	}
 	int lp[kMaxTileGates], lq[kMaxTileGates];
 	// usage of the tile's row bits by the pass's gates
 	std::vector<int> use(C.L, 0);
@@ -1098,6 +1105,9 @@ void enqueue_eval(const Context& dev, Slot& s, const EvalPlan& pl, const Circuit
 	}
 	s.last_block = pl.B;
 	s.last_engine = pl.tiles ? RQC_ENGINE_TILE_PASS : RQC_ENGINE_PER_GATE;
+	s.last_K = pl.tiles ? pl.tile_K : 0;
+	s.last_logC = pl.tiles ? pl.tile_logC : 0;
+	s.last_gmax = pl.tiles ? pl.gmax : 0;
 	s.last_passes = pl.passes;
 	{
 		LaunchTimer lt(s, RQC_KERNEL_OTHER);
@@ -1136,6 +1146,68 @@ void unpack_result(const EvalPlan& pl, const char* res, EvalResult& R)
 // dense U for a target evaluation: device-resident or assembled from the host callback
 static const zc* resolve_target(Context& c, linear_func ufunc, void* udata, int L);
 
+// Tile autotuning of single evaluations (rqc_set_eval_autotune): the first evaluation of a
+// circuit structure (qubits, wires, groups, kind, precision, projection) times the tile
+// candidates on the evaluation itself (one untimed run, then the best of two, HIP events on the
+// library's stream) and keeps the fastest for every later evaluation of that structure.  The
+// candidates are the tile height (2^10 or 2^9 FP64 elements, 2^11 or 2^10 FP32; the row bits K
+// follow from the height and the 256- or 128-byte rows) and the most gates per pass (8 or 6).
+static std::string structure_key(const Circuit& C, Mode mode, int per, bool fp32, bool project)
+{
+	std::string key = std::to_string(C.L) + "/" + std::to_string(per) + "/" + std::to_string((int)mode) + "/" +
+		std::to_string(fp32) + "/" + std::to_string(project) + ":";
+	for (int k = 0; k < C.ngates; k++) { key += std::to_string(C.wi[k]) + "," + std::to_string(C.wj[k]) + ";"; }
+	return key;
+}
+
+static std::map<std::string, TileChoice>& tuned_choices()
+{
+	static std::map<std::string, TileChoice> m;
+	return m;
+}
+
+static TileChoice tune_eval(Context& c, const Circuit& C, Mode mode, int per, bool project, bool unitary, const zc* dU)
+{
+	const bool fp32 = c.precision == RQC_PRECISION_MIXED;
+	const std::string key = structure_key(C, mode, per, fp32, project);
+	auto it = tuned_choices().find(key);
+	if (it != tuned_choices().end()) { return it->second; }
+	const int base = fp32 ? 11 : 10;
+	const TileChoice cands[] = {{base, 8}, {base - 1, 8}, {base, 6}, {base - 1, 6}};
+	TileChoice best{0, 0};
+	float best_ms = 0;
+	hipEvent_t e0, e1;
+	RQC_HIP(hipEventCreate(&e0));
+	RQC_HIP(hipEventCreate(&e1));
+	for (const TileChoice& tc : cands) {
+		EvalPlan pl;
+		try {
+			pl = plan_eval(c, C, mode, per, fp32, project, unitary, c.ws_limit, c.work.bytes, &tc);
+		} catch (const Error&) {
+			continue;   // a candidate that does not fit is skipped
+		}
+		if (!pl.tiles) { break; }   // no tile-pass plan: nothing to tune
+		std::vector<char> tab(pl.tab_bytes);
+		fill_eval_tables(pl, C, tab.data());
+		c.work.reserve(pl.bytes);
+		float ms = 0;
+		for (int r = 0; r < 3; r++) {
+			RQC_HIP(hipEventRecord(e0, c.stream));
+			enqueue_eval(c, c, pl, C, dU, c.work.as<char>(), nullptr, nullptr, tab.data(), nullptr, nullptr, nullptr);
+			RQC_HIP(hipEventRecord(e1, c.stream));
+			RQC_HIP(hipEventSynchronize(e1));
+			float t = 0;
+			RQC_HIP(hipEventElapsedTime(&t, e0, e1));
+			if (r == 1 || (r == 2 && t < ms)) { ms = t; }
+		}
+		if (best.logE == 0 || ms < best_ms) { best = tc; best_ms = ms; }
+	}
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+	tuned_choices()[key] = best;
+	return best;
+}
+
 // one evaluation on the main slot (the reference ABI's target functions)
 static EvalResult eval_targets(const Circuit& C, linear_func ufunc, void* udata, Mode mode, int per, bool project)
 {
@@ -1143,8 +1215,10 @@ static EvalResult eval_targets(const Circuit& C, linear_func ufunc, void* udata,
 	std::lock_guard<std::recursive_mutex> lk(c.mu);
 	const zc* dU = resolve_target(c, ufunc, udata, C.L);
 	const bool unitary = sweep_unitary(C, mode, c.precision == RQC_PRECISION_MIXED);
+	TileChoice tc{0, 0};
+	if (c.eval_autotune && c.engine == RQC_ENGINE_TILE_PASS) { tc = tune_eval(c, C, mode, per, project, unitary, dU); }
 	const EvalPlan pl = plan_eval(c, C, mode, per, c.precision == RQC_PRECISION_MIXED, project, unitary, c.ws_limit,
-		c.work.bytes);
+		c.work.bytes, tc.logE > 0 ? &tc : nullptr);
 	std::vector<char> tab(pl.tab_bytes);
 	fill_eval_tables(pl, C, tab.data());
 	c.work.reserve(pl.bytes);
@@ -2154,6 +2228,198 @@ extern "C" int rqc_brickwall_unitary_target_projected_hvp_noalloc(linear_func uf
 
 extern "C" size_t rqc_noalloc_scratch_bytes(void) { return kScratchBytes; }
 
+// ============================================================== single-qubit gates
+static Gate2<double> to_gate2(const numeric* g)
+{
+	Gate2<double> G;
+	std::memcpy(G.g, g, sizeof(G.g));
+	return G;
+}
+
+static void check_wire(int i, int L)
+{
+	if (L < 1 || i < 0 || i >= L) {
+		throw Error(-3, "invalid gate wire " + std::to_string(i) + " for " + std::to_string(L) + " qubits");
+	}
+}
+
+extern "C" int rqc_apply_single_qubit_gate(const numeric gate[4], int i, const struct statevector* RQC_RESTRICT psi,
+	struct statevector* RQC_RESTRICT psi_out)
+{
+	return target_call("rqc_apply_single_qubit_gate", [&] {
+		if (psi->nqubits != psi_out->nqubits) { throw Error(-3, "qubit counts differ"); }
+		const int L = psi->nqubits;
+		check_wire(i, L);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t bytes = ((size_t)1 << L) * sizeof(zc);
+		Staging st(c);
+		const int a = st.add(psi->data, bytes, true, false);
+		const int o = st.add(psi_out->data, bytes, false, true);
+		st.upload();
+		const long n = (1L << L) / 2;
+		{
+			LaunchTimer lt(c, RQC_KERNEL_FORWARD, 2.0 * (double)bytes);
+			k_gate1_forward<double><<<nblocks(n), kBlock, 0, c.stream>>>(st.dev<zc>(a), st.dev<zc>(o), to_gate2(gate),
+				L - 1 - i, 1, 0, n);
+		}
+		st.download();
+	});
+}
+
+extern "C" int rqc_apply_single_qubit_gate_to_array(const numeric gate[4], int i,
+	const struct statevector_array* RQC_RESTRICT psi, struct statevector_array* RQC_RESTRICT psi_out)
+{
+	return target_call("rqc_apply_single_qubit_gate_to_array", [&] {
+		if (psi->nqubits != psi_out->nqubits || psi->nstates != psi_out->nstates) { throw Error(-3, "array shapes differ"); }
+		const int L = psi->nqubits;
+		check_wire(i, L);
+		if (psi->nstates < 1) { return; }
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const long B = psi->nstates;
+		const size_t bytes = ((size_t)1 << L) * B * sizeof(zc);
+		Staging st(c);
+		const int a = st.add(psi->data, bytes, true, false);
+		const int o = st.add(psi_out->data, bytes, false, true);
+		st.upload();
+		const long n = (1L << L) / 2 * B;
+		{
+			LaunchTimer lt(c, RQC_KERNEL_FORWARD, 2.0 * (double)bytes);
+			k_gate1_forward<double><<<nblocks(n), kBlock, 0, c.stream>>>(st.dev<zc>(a), st.dev<zc>(o), to_gate2(gate),
+				L - 1 - i, B, ilog2_or_neg(B), n);
+		}
+		st.download();
+	});
+}
+
+extern "C" int rqc_apply_single_qubit_gate_backward(const numeric gate[4], int i, const struct statevector* RQC_RESTRICT psi,
+	const struct statevector* RQC_RESTRICT dpsi_out, struct statevector* RQC_RESTRICT dpsi, numeric dgate[4])
+{
+	return target_call("rqc_apply_single_qubit_gate_backward", [&] {
+		if (psi->nqubits != dpsi_out->nqubits || psi->nqubits != dpsi->nqubits) { throw Error(-3, "qubit counts differ"); }
+		const int L = psi->nqubits;
+		check_wire(i, L);
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const size_t bytes = ((size_t)1 << L) * sizeof(zc);
+		const long n = (1L << L) / 2;
+		const unsigned g = reduce_grid(n);
+		zc* acc = small_acc(c, 4);
+		zc* part = acc + 32;   // small_acc holds 4096 * kWidth further entries, more than 4 per workgroup
+		Staging st(c);
+		const int a = st.add(psi->data, bytes, true, false);
+		const int d = st.add(dpsi_out->data, bytes, true, false);
+		const int o = st.add(dpsi->data, bytes, false, true);
+		st.upload();
+		{
+			LaunchTimer lt(c, RQC_KERNEL_REVERSE, 3.0 * (double)bytes);
+			k_gate1_reverse<double><<<g, kBlock, 0, c.stream>>>(st.dev<zc>(a), st.dev<zc>(d), st.dev<zc>(o), to_gate2(gate),
+				L - 1 - i, 1, 0, n, part);
+		}
+		{
+			LaunchTimer lt(c, RQC_KERNEL_OTHER);
+			k_reduce_partials<<<4, kBlock, 0, c.stream>>>(part, (int)g, 4, acc, nullptr);
+		}
+		copy_out(c, dgate, acc, 4 * sizeof(zc));
+		st.download();
+	});
+}
+
+// A circuit of one- and two-qubit gates as a circuit of two-qubit gates: a one-qubit gate V
+// on wire i becomes V (x) I on wires (i, (i + 1) mod L) (first wire the more significant
+// index: (V (x) I)[2a + c][2b + d] = V[a][b] delta_cd), its direction Z (x) I.
+static Circuit mixed_circuit(const numeric* gates, const numeric* dirs, const int* nq, const int* wires, int ngates, int L,
+	std::vector<size_t>& off)
+{
+	if (ngates < 1) { throw Error(-3, "a circuit needs at least one gate"); }
+	if (L < 2) { throw Error(-3, "a circuit needs at least two qubits"); }
+	std::vector<struct mat4x4> g4((size_t)ngates), d4((size_t)ngates);
+	std::vector<int> w4(2 * (size_t)ngates);
+	off.assign((size_t)ngates + 1, 0);
+	for (int k = 0; k < ngates; k++) {
+		if (nq[k] != 1 && nq[k] != 2) { throw Error(-3, "gate " + std::to_string(k) + " acts on neither 1 nor 2 qubits"); }
+		const size_t o = off[k];
+		off[k + 1] = o + (nq[k] == 1 ? 4 : 16);
+		if (nq[k] == 2) {
+			std::memcpy(g4[k].data, gates + o, 16 * sizeof(numeric));
+			if (dirs != nullptr) { std::memcpy(d4[k].data, dirs + o, 16 * sizeof(numeric)); }
+			w4[2 * k] = wires[2 * k];
+			w4[2 * k + 1] = wires[2 * k + 1];
+			continue;
+		}
+		check_wire(wires[2 * k], L);
+		for (auto& e : g4[k].data) { e = numeric(0.0, 0.0); }
+		for (auto& e : d4[k].data) { e = numeric(0.0, 0.0); }
+		for (int a = 0; a < 2; a++) {
+			for (int b = 0; b < 2; b++) {
+				for (int cc = 0; cc < 2; cc++) {
+					g4[k].data[4 * (2 * a + cc) + 2 * b + cc] = gates[o + 2 * a + b];
+					if (dirs != nullptr) { d4[k].data[4 * (2 * a + cc) + 2 * b + cc] = dirs[o + 2 * a + b]; }
+				}
+			}
+		}
+		w4[2 * k] = wires[2 * k];
+		w4[2 * k + 1] = (wires[2 * k] + 1) % L;
+	}
+	return make_circuit(g4.data(), dirs != nullptr ? d4.data() : nullptr, ngates, w4.data(), L);
+}
+
+// per-gate sums of the two-qubit circuit back to the mixed packing: a one-qubit gate's entry
+// (a, b) is the partial trace sum_c X[2a + c][2b + c] over the partner wire
+static void unpack_mixed(const std::vector<zc>& x4, const int* nq, int ngates, const std::vector<size_t>& off, numeric* out)
+{
+	for (int k = 0; k < ngates; k++) {
+		const zc* x = &x4[16 * (size_t)k];
+		if (nq[k] == 2) {
+			std::memcpy(out + off[k], x, 16 * sizeof(zc));
+			continue;
+		}
+		for (int a = 0; a < 2; a++) {
+			for (int b = 0; b < 2; b++) {
+				const zc s = x[4 * (2 * a) + 2 * b] + x[4 * (2 * a + 1) + 2 * b + 1];
+				out[off[k] + 2 * a + b] = numeric(s.x, s.y);
+			}
+		}
+	}
+}
+
+extern "C" int rqc_mixed_circuit_unitary_target(linear_func ufunc, void* udata, const numeric* gates, const int nq[],
+	const int wires[], int ngates, int nqubits, numeric* fval)
+{
+	return target_call("rqc_mixed_circuit_unitary_target", [&] {
+		std::vector<size_t> off;
+		Circuit C = mixed_circuit(gates, nullptr, nq, wires, ngates, nqubits, off);
+		put_f(fval, eval_targets(C, ufunc, udata, Mode::Target, 1, false));
+	});
+}
+
+extern "C" int rqc_mixed_circuit_unitary_target_and_gradient(linear_func ufunc, void* udata, const numeric* gates,
+	const int nq[], const int wires[], int ngates, int nqubits, numeric* fval, numeric* dgates)
+{
+	return target_call("rqc_mixed_circuit_unitary_target_and_gradient", [&] {
+		std::vector<size_t> off;
+		Circuit C = mixed_circuit(gates, nullptr, nq, wires, ngates, nqubits, off);
+		EvalResult R = eval_targets(C, ufunc, udata, Mode::Grad, 1, false);
+		put_f(fval, R);
+		unpack_mixed(R.dv, nq, ngates, off, dgates);
+	});
+}
+
+extern "C" int rqc_mixed_circuit_unitary_target_hessian_vector_product(linear_func ufunc, void* udata, const numeric* gates,
+	const numeric* gatedirs, const int nq[], const int wires[], int ngates, int nqubits, numeric* fval, numeric* dgates,
+	numeric* hess_gatedirs)
+{
+	return target_call("rqc_mixed_circuit_unitary_target_hessian_vector_product", [&] {
+		std::vector<size_t> off;
+		Circuit C = mixed_circuit(gates, gatedirs, nq, wires, ngates, nqubits, off);
+		EvalResult R = eval_targets(C, ufunc, udata, Mode::Hvp, 1, false);
+		put_f(fval, R);
+		unpack_mixed(R.dv, nq, ngates, off, dgates);
+		unpack_mixed(R.hz, nq, ngates, off, hess_gatedirs);
+	});
+}
+
 // ============================================================== sampled (stochastic-trace) targets
 // The trace over the 2^L basis columns is replaced by nsamples Haar-random
 // states (reference src/target.c:85-149, :807-928): f, gradients and Hessians
@@ -2686,6 +2952,27 @@ extern "C" int rqc_set_sweep_engine(int engine)
 {
 	if (engine != RQC_ENGINE_TILE_PASS && engine != RQC_ENGINE_PER_GATE) { return -3; }
 	return target_call("rqc_set_sweep_engine", [&] { ctx().engine = engine; });
+}
+
+extern "C" int rqc_set_eval_autotune(int on)
+{
+	return target_call("rqc_set_eval_autotune", [&] {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		c.eval_autotune = (on != 0);
+		if (on == 2) { tuned_choices().clear(); }
+	});
+}
+
+extern "C" int rqc_last_tiles(int* tile_rows_log2, int* tile_cols_log2, int* gates_per_pass)
+{
+	return target_call("rqc_last_tiles", [&] {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		if (tile_rows_log2 != nullptr) { *tile_rows_log2 = c.last_K; }
+		if (tile_cols_log2 != nullptr) { *tile_cols_log2 = c.last_logC; }
+		if (gates_per_pass != nullptr) { *gates_per_pass = c.last_gmax; }
+	});
 }
 
 extern "C" int rqc_set_autotune(int on)

@@ -52,6 +52,7 @@ struct Slot
 	long last_block = 0;   // column-block width of the last target evaluation
 	int last_engine = -1;  // sweep engine of the last target evaluation
 	int last_passes = 0;   // HBM passes per sweep direction of the last target evaluation
+	int last_K = 0, last_logC = 0, last_gmax = 0;   // its tile shape and most gates per pass
 
 	// kernel timing (HIP events on this slot's stream)
 	bool profile = false;
@@ -72,6 +73,7 @@ struct Context : Slot
 	size_t lds_per_block = 65536;
 	size_t ws_limit = 0;  // 0 = automatic
 	bool autotune = true;  // tile autotuning of batched problems
+	bool eval_autotune = false;  // tile autotuning of single evaluations (first call per structure)
 
 	DeviceBuffer work;     // workspace of target evaluations (states, partials, tables, results)
 	DeviceBuffer small;    // accumulators of the statevector-level API

@@ -241,6 +241,75 @@ __global__ __launch_bounds__(kBlock) void k_gate_reverse(
 }
 
 // ---------------------------------------------------------------------------
+// Single-qubit gates.  A 2x2 gate V on wire i acts on the pairs of rows that
+// differ in bit b = L-1-i (the same qubit-to-bit map as the two-qubit gates).
+// Work item t = (pair, column): the pair's row with bit b clear is the pair
+// index with a zero bit inserted at b.
+template <typename T>
+struct Gate2
+{
+	cx<T> g[4];
+};
+
+__device__ __forceinline__ void pair_offsets(long t, long B, int logB, int b, long* o0, long* o1)
+{
+	long q, c;
+	if (logB >= 0) { q = t >> logB; c = t & (B - 1); }
+	else { q = t / B; c = t - q * B; }
+	const long base = ((q >> b) << (b + 1)) | (q & ((1L << b) - 1));
+	*o0 = base * B + c;
+	*o1 = (base + (1L << b)) * B + c;
+}
+
+// out = V in on wire bit b (in and out may be the same array)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gate1_forward(const cx<T>* in, cx<T>* out, Gate2<T> V, int b, long B,
+	int logB, long nitems)
+{
+	const long t = (long)blockIdx.x * kBlock + threadIdx.x;
+	if (t >= nitems) { return; }
+	long o0, o1;
+	pair_offsets(t, B, logB, b, &o0, &o1);
+	const cx<T> x0 = in[o0], x1 = in[o1];
+	cx<T> y0 = {0, 0}, y1 = {0, 0};
+	cmac(y0, V.g[0], x0);
+	cmac(y0, V.g[1], x1);
+	cmac(y1, V.g[2], x0);
+	cmac(y1, V.g[3], x1);
+	out[o0] = y0;
+	out[o1] = y1;
+}
+
+// reverse step of a single-qubit gate (phi after the gate, psi before it):
+//   dV_rc += phi_r psi_c (FP64 partial sums per workgroup, 4 entries),  phi_out = V^T phi
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gate1_reverse(const cx<T>* __restrict__ psi, const cx<T>* __restrict__ phi,
+	cx<T>* __restrict__ phi_out, Gate2<T> V, int b, long B, int logB, long nitems, zc* __restrict__ partial)
+{
+	zc ga[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+	const long stride = (long)gridDim.x * kBlock;
+	for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < nitems; t += stride) {
+		long o0, o1;
+		pair_offsets(t, B, logB, b, &o0, &o1);
+		const cx<T> x[2] = {psi[o0], psi[o1]};
+		const cx<T> f[2] = {phi[o0], phi[o1]};
+#pragma unroll
+		for (int r = 0; r < 2; r++) {
+#pragma unroll
+			for (int c = 0; c < 2; c++) { cmac(ga[2 * r + c], to_z(f[r]), to_z(x[c])); }
+		}
+		cx<T> y0 = {0, 0}, y1 = {0, 0};
+		cmac(y0, V.g[0], f[0]);
+		cmac(y0, V.g[2], f[1]);
+		cmac(y1, V.g[1], f[0]);
+		cmac(y1, V.g[3], f[1]);
+		phi_out[o0] = y0;
+		phi_out[o1] = y1;
+	}
+	block_sum_store<4>(ga, partial + (long)blockIdx.x * 4);
+}
+
+// ---------------------------------------------------------------------------
 // trace of the column block: f += sum_{a,c} -conj(U[a, b0+c]) psi[a, c]; when
 // phi is given the adjoint seed -conj(U[:, b0+c]) is also stored there
 template <typename T>

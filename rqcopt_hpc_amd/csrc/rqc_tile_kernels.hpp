@@ -93,6 +93,7 @@ struct TileArgs
 	unsigned rmask;    // row bits outside the tile (rest), deposited lowest first
 	unsigned swz[4];   // column swizzle masks over tile rows
 	unsigned barrier;  // bit j: the waves own different quads for gates j - 1 and j
+	int wide;          // FP32 dual reverse pass: four matrix-core groups per step
 	int rb[16];        // row bit of tile bit t, t < K
 };
 
@@ -608,7 +609,7 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void
 // tiles fits 3 workgroups (168 registers).  The plain sweeps (two states)
 // double-buffer, the next tile streaming in while the gates run.
 template <typename T, bool DUAL> constexpr int rev_buffers() { return DUAL ? 1 : 2; }
-template <typename T, bool DUAL> constexpr int rev_waves_per_simd() { return (DUAL && sizeof(T) == 4) ? 3 : 2; }
+template <typename T, bool DUAL> constexpr int rev_waves_per_simd() { return 2; }
 
 template <typename T, bool DUAL>
 __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void k_tile_reverse(
@@ -697,46 +698,57 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 				aa[kk] = mj[amap.aoff[kk] + 2 * kMatPairOff * (DUAL ? 2 : 1)] * amap.afac[kk];
 			}
 			const bool adjoint = (J > 0) || !(flags & 8);
-			for (int g = 0; g < qpw; g += GS * gq) {
-				int eg[GS], eg2[GS];
-	#pragma unroll
-				for (int n = 0; n < GS; n++) {
-					eg[n] = ewv ^ group_term(gt, (g + n * gq) >> 4);
-					eg2[n] = DUAL ? eg[n] : ewv ^ group_term(gt, ((g + n * gq) >> 4) + 1);
-				}
-				// The state's recovery (uncompute) and the adjoint's update are
-				// independent: load both, run both chains (transposed, so that the
-				// results are outer-product operands), store both; the outer
-				// products pair the adjoint at the gate input with the recovered
-				// state straight from the registers (the reduction maps
-				// phi_in (x) psi to the gate's phi_out (x) psi).
-				GateOperands<T, GS> os, oa;
-				gate_load<T, GS>(S, D, eg, eg2, gl, os);
-				gate_load<T, GS>(F, DF, eg, eg2, gl, oa);
-				// matrix-core work at raised wave priority, so that the other wave's
-				// LDS traffic issues in its shadow
-				__builtin_amdgcn_s_setprio(1);
-				gate_mfma_t<T, GS>(au, os);
-				gate_mfma_t<T, GS>(aa, oa);
-				gate_store_t<T, GS>(S, D, eg, eg2, gl, os);
-				if (adjoint) { gate_store_t<T, GS>(F, DF, eg, eg2, gl, oa); }
-				if constexpr (sizeof(T) == 8) {
-					// FP64: one chain over the groups' steps accumulating onto acc[J]
+			// GSX matrix-core groups of gq quads per step
+			auto steps = [&](auto gsc) {
+				constexpr int GSX = decltype(gsc)::value;
+				for (int g = 0; g < qpw; g += GSX * gq) {
+					int eg[GSX], eg2[GSX];
 #pragma unroll
-					for (int t = 0; t < 4; t++) {
+					for (int n = 0; n < GSX; n++) {
+						eg[n] = ewv ^ group_term(gt, (g + n * gq) >> 4);
+						eg2[n] = DUAL ? eg[n] : ewv ^ group_term(gt, ((g + n * gq) >> 4) + 1);
+					}
+					// The state's recovery (uncompute) and the adjoint's update are
+					// independent: load both, run both chains (transposed, so that the
+					// results are outer-product operands), store both; the outer
+					// products pair the adjoint at the gate input with the recovered
+					// state straight from the registers (the reduction maps
+					// phi_in (x) psi to the gate's phi_out (x) psi).
+					GateOperands<T, GSX> os, oa;
+					gate_load<T, GSX>(S, D, eg, eg2, gl, os);
+					gate_load<T, GSX>(F, DF, eg, eg2, gl, oa);
+					// matrix-core work at raised wave priority, so that the other wave's
+					// LDS traffic issues in its shadow
+					__builtin_amdgcn_s_setprio(1);
+					gate_mfma_t<T, GSX>(au, os);
+					gate_mfma_t<T, GSX>(aa, oa);
+					gate_store_t<T, GSX>(S, D, eg, eg2, gl, os);
+					if (adjoint) { gate_store_t<T, GSX>(F, DF, eg, eg2, gl, oa); }
+					if constexpr (sizeof(T) == 8) {
+						// FP64: one chain over the groups' steps accumulating onto acc[J]
 #pragma unroll
-						for (int n = 0; n < GS; n++) { acc[J] = mfma_step(oa.acc[n][t], os.acc[n][t], acc[J]); }
+						for (int t = 0; t < 4; t++) {
+#pragma unroll
+							for (int n = 0; n < GSX; n++) { acc[J] = mfma_step(oa.acc[n][t], os.acc[n][t], acc[J]); }
+						}
+					} else {
+						// FP32: one chain over the gate's quads of this tile into pend,
+						// added into the FP64 accumulator by the next gate (no wait here)
+#pragma unroll
+						for (int t = 0; t < 4; t++) {
+#pragma unroll
+							for (int n = 0; n < GSX; n++) { pend = mfma_step(oa.acc[n][t], os.acc[n][t], pend); }
+						}
 					}
-				} else {
-					// FP32: one chain over the gate's quads of this tile into pend,
-					// added into the FP64 accumulator by the next gate (no wait here)
-	#pragma unroll
-					for (int t = 0; t < 4; t++) {
-	#pragma unroll
-						for (int n = 0; n < GS; n++) { pend = mfma_step(oa.acc[n][t], os.acc[n][t], pend); }
-					}
+					__builtin_amdgcn_s_setprio(0);
 				}
-				__builtin_amdgcn_s_setprio(0);
+			};
+			if constexpr (DUAL && sizeof(T) == 4) {
+				// FP32 dual sweeps: four groups (64 quads) per step when the wave's quads allow
+				if (P.wide && (qpw % (4 * gq)) == 0) { steps(std::integral_constant<int, 4>{}); }
+				else { steps(std::integral_constant<int, GS>{}); }
+			} else {
+				steps(std::integral_constant<int, GS>{});
 			}
 		};
 		static_assert(kMaxTileGates == 8, "one case per gate of a pass");

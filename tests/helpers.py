@@ -71,6 +71,49 @@ def gate_matrix(G, i, j, L):
     return M
 
 
+def gate1_matrix(V, i, L):
+    """Dense matrix of a single-qubit gate V on wire i: I (x) V (x) I with wire 0 leftmost."""
+    return np.kron(np.kron(np.eye(1 << i), V), np.eye(1 << (L - 1 - i)))
+
+
+def mixed_circuit_matrix(gates, nq, wires, L):
+    """Dense matrix of a circuit of one- and two-qubit gates (gates: list of 2x2 / 4x4 arrays)."""
+    W = np.eye(1 << L, dtype=np.complex128)
+    for G, q, (i, j) in zip(gates, nq, wires):
+        W = (gate1_matrix(G, int(i), L) if q == 1 else gate_matrix(G, int(i), int(j), L)) @ W
+    return W
+
+
+def mixed_target_gradient(U, gates, nq, wires, L):
+    """f = -Tr(U^dagger W) and its holomorphic gate gradients dW/dG[a, b] for a mixed circuit, by
+    dense products: dG[a, b] = -Tr(U^dagger A_k E_ab B_k) with A_k the gates after k, B_k before."""
+    mats = [gate1_matrix(G, int(i), L) if q == 1 else gate_matrix(G, int(i), int(j), L)
+            for G, q, (i, j) in zip(gates, nq, wires)]
+    n = 1 << L
+    before = [np.eye(n, dtype=np.complex128)]
+    for M in mats:
+        before.append(M @ before[-1])
+    after = [np.eye(n, dtype=np.complex128)]
+    for M in reversed(mats):
+        after.append(after[-1] @ M)
+    after = after[::-1]   # after[k] = product of the gates k+1 .. end
+    f = -np.trace(U.conj().T @ before[-1])
+    grads = []
+    for k, (G, q, (i, j)) in enumerate(zip(gates, nq, wires)):
+        m = 2 if q == 1 else 4
+        A, B = after[k + 1], before[k]
+        X = B @ U.conj().T @ A   # -Tr(U^dag A E B) = -Tr(B U^dag A E) = -X[b, a] for E = E_ab
+        dG = np.empty((m, m), np.complex128)
+        for a in range(m):
+            for b in range(m):
+                E = np.zeros((m, m), np.complex128)
+                E[a, b] = 1
+                Em = gate1_matrix(E, int(i), L) if q == 1 else gate_matrix(E, int(i), int(j), L)
+                dG[a, b] = -np.sum(X.T * Em)
+        grads.append(dG)
+    return f, grads
+
+
 def circuit_matrix(gates, wires, L):
     n = 1 << L
     W = np.eye(n, dtype=np.complex128)

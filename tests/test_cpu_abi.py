@@ -65,6 +65,7 @@ C_CALLER = r"""
 #include "rqcopt/rqc_device.h"
 #include "rqcopt/rqc_batch.h"
 #include "rqcopt/rqc_server.h"
+#include "rqcopt/rqc_gate1q.h"
 #include <stdio.h>
 int main(void)
 {

@@ -96,3 +96,31 @@ def test_mixed_shapes_within_bounds(problem, name):
     # stated bounds: |df| / N <= 1e-6, gradient and HVP within 1e-4 relative
     assert abs(f - f64) / (1 << L) < 1e-6
     assert reldiff(dv, dv64) < 1e-4 and reldiff(hz, hz64) < 1e-4
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_eval_autotune_keeps_parity(problem, prec):
+    """Single-evaluation tile autotuning (rqc_set_eval_autotune): the first HVP evaluation of the
+    structure times the tile candidates and keeps the fastest; the tuned evaluation gives the
+    oracle's FP64 results to 1e-12 (mixed: its stated bounds), and the choice is one of the
+    candidates (2^10 / 2^9 FP64 or 2^11 / 2^10 FP32 elements per tile, 8 or 6 gates per pass) and
+    stays fixed for later evaluations of the structure."""
+    L, V, Z, P, T, ref = problem
+    f_o, dv_o, hz_o = ref["hvp"]
+    R.set_precision(R.PRECISION_FP64 if prec == "fp64" else R.PRECISION_MIXED)
+    R.set_eval_autotune(True, forget=True)
+    try:
+        f, dv, hz = R.brickwall_unitary_target_hessian_vector_product(T, V, Z, P)
+        K, logC, gmax = R.last_tiles()
+        f2, dv2, hz2 = R.brickwall_unitary_target_hessian_vector_product(T, V, Z, P)
+        assert R.last_tiles() == (K, logC, gmax)
+    finally:
+        R.set_eval_autotune(False)
+    print(f"{prec}: tuned tile 2^{K} x 2^{logC}, at most {gmax} gates per pass", flush=True)
+    base = 10 if prec == "fp64" else 11
+    assert K + logC in (base, base - 1) and gmax in (8, 6)
+    assert f == f2 and np.array_equal(dv, dv2) and np.array_equal(hz, hz2)
+    if prec == "fp64":
+        assert abs(f - f_o) < TOL * max(1, abs(f_o)) and reldiff(dv, dv_o) < TOL and reldiff(hz, hz_o) < TOL
+    else:
+        assert abs(f - f_o) / (1 << L) < 1e-6 and reldiff(dv, dv_o) < 1e-4 and reldiff(hz, hz_o) < 1e-4

@@ -751,3 +751,49 @@ def test_gate_kernel_calibration_runs():
     for name in ("apply_gate", "apply_gate_backward"):
         k = res["kernels"][name]
         assert k["reference_us_per_gate"] > 0 and k["oracle_us_per_gate"] > 0 and k["reference_over_oracle"] > 0
+
+
+# ================================================================ single-qubit gates (no reference counterpart)
+def test_single_qubit_gate_against_kronecker_products():
+    """orc_apply_gate_1q equals the dense I (x) V (x) I on every wire, and the backward map is the
+    transpose with dV = sum dpsi_out (x) psi, pinned by the bilinear identity
+    <dpsi_out, V psi> = sum dV * V = <V^T dpsi_out, psi> and by the dense Kronecker form of dV."""
+    from helpers import gate1_matrix
+    rng = np.random.default_rng(41)
+    for L in (1, 3, 6):
+        psi, dout = crandn(rng, 1 << L), crandn(rng, 1 << L)
+        for i in range(L):
+            V = crandn(rng, (2, 2))
+            M = gate1_matrix(V, i, L)
+            assert np.max(np.abs(O.apply_gate_1q(V, i, psi) - M @ psi)) < 1e-13
+            dpsi, dV = O.apply_gate_1q_backward(V, i, psi, dout)
+            assert np.max(np.abs(dpsi - M.T @ dout)) < 1e-13
+            lhs = dout @ (M @ psi)
+            assert abs(lhs - np.sum(dV * V)) < 1e-12 * max(1, abs(lhs))
+            assert abs(lhs - dpsi @ psi) < 1e-12 * max(1, abs(lhs))
+            for a in range(2):
+                for b in range(2):
+                    E = np.zeros((2, 2), np.complex128)
+                    E[a, b] = 1
+                    assert abs(dV[a, b] - dout @ (gate1_matrix(E, i, L) @ psi)) < 1e-12
+
+
+def test_mixed_circuit_dense_gradient_by_finite_differences():
+    """The dense mixed-circuit target and gradient used to check the GPU's mixed circuits: the
+    gradient matches central differences of f along random complex directions (holomorphic:
+    df = sum dG * dV)."""
+    from helpers import mixed_target_gradient
+    rng = np.random.default_rng(42)
+    L = 4
+    nq = [1, 2, 1, 2, 1]
+    wires = [(0, 0), (1, 2), (3, 3), (3, 0), (2, 2)]
+    gates = [haar_unitary(rng, 2) if q == 1 else haar_unitary(rng) for q in nq]
+    U = np.linalg.qr(crandn(rng, (1 << L, 1 << L)))[0]
+    f, grads = mixed_target_gradient(U, gates, nq, wires, L)
+    dirs = [crandn(rng, G.shape) for G in gates]
+    h = 1e-6
+    fp = mixed_target_gradient(U, [G + h * D for G, D in zip(gates, dirs)], nq, wires, L)[0]
+    fm = mixed_target_gradient(U, [G - h * D for G, D in zip(gates, dirs)], nq, wires, L)[0]
+    fd = (fp - fm) / (2 * h)
+    an = sum(np.sum(g * d) for g, d in zip(grads, dirs))
+    assert abs(fd - an) < 1e-7 * max(1, abs(an))
