@@ -666,6 +666,36 @@ static TileShape forward_shape(const TileShape& ts, int L, long B, int elem)
 	return f;
 }
 
+// tile-kernel instantiation for a tile of 2^logE elements: the dual sweeps have variants
+// with the LDS state stride fixed at compile time for the planned tile heights
+template <typename T>
+static decltype(&k_tile_forward<T, true, 0>) fwd_kernel(bool dual, int logE)
+{
+	if (!dual) { return k_tile_forward<T, false, 0>; }
+	if (sizeof(T) == 8) {
+		if (logE == 10) { return k_tile_forward<T, true, 10>; }
+		if (logE == 9) { return k_tile_forward<T, true, 9>; }
+	} else {
+		if (logE == 11) { return k_tile_forward<T, true, 11>; }
+		if (logE == 10) { return k_tile_forward<T, true, 10>; }
+	}
+	return k_tile_forward<T, true, 0>;
+}
+
+template <typename T>
+static decltype(&k_tile_reverse<T, true, 0>) rev_kernel(bool dual, int logE)
+{
+	if (!dual) { return k_tile_reverse<T, false, 0>; }
+	if (sizeof(T) == 8) {
+		if (logE == 10) { return k_tile_reverse<T, true, 10>; }
+		if (logE == 9) { return k_tile_reverse<T, true, 9>; }
+	} else {
+		if (logE == 11) { return k_tile_reverse<T, true, 11>; }
+		if (logE == 10) { return k_tile_reverse<T, true, 10>; }
+	}
+	return k_tile_reverse<T, true, 0>;
+}
+
 template <typename T>
 static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C, EvalPlan& pl, const std::vector<Pass>& passes,
 	const TileShape& ts)
@@ -726,8 +756,8 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 		(Ef * sizeof(cx<T>) + kLdsPadBytes) + fwd_mats;
 	tp->lds_rev = (size_t)(dual ? rev_buffers<T, true>() : rev_buffers<T, false>()) * (dual ? 4 : 2) *
 		(E * sizeof(cx<T>) + lds_pad_rev<T>()) + rev_mats;
-	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
-	auto* kr = dual ? k_tile_reverse<T, true> : k_tile_reverse<T, false>;
+	auto* kf = fwd_kernel<T>(dual, tp->tsf.K + tp->tsf.logC);
+	auto* kr = rev_kernel<T>(dual, ts.K + ts.logC);
 	allow_lds(kf, tp->lds_fwd);
 	allow_lds(kr, tp->lds_rev);
 	int occ_f = 0, occ_r = 0;
@@ -963,8 +993,8 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	cx<T>* phi[2] = {S + nck * sz, S + (nck + 1) * sz};
 	cx<T>* dphi[2] = {S + (nck + 2) * sz, S + (nck + 3) * sz};
 	const long ntiles = (N >> tp.ts.K) * (B >> tp.ts.logC);
-	auto* kf = dual ? k_tile_forward<T, true> : k_tile_forward<T, false>;
-	auto* kr = dual ? k_tile_reverse<T, true> : k_tile_reverse<T, false>;
+	auto* kf = fwd_kernel<T>(dual, tp.tsf.K + tp.tsf.logC);
+	auto* kr = rev_kernel<T>(dual, tp.ts.K + tp.ts.logC);
 	const GateTerms* fterms = reinterpret_cast<const GateTerms*>(reinterpret_cast<const char*>(gterms) + tp.fterms_off);
 	const zc* rmats = reinterpret_cast<const zc*>(reinterpret_cast<const char*>(mats) + pl.rmats_off);
 	double* mpart = reinterpret_cast<double*>(d_partial);   // grev * waves * ng * 256 doubles

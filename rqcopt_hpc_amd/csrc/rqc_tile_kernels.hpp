@@ -498,7 +498,10 @@ __device__ __forceinline__ void tile_store(cx<T>* __restrict__ dst, const cx<T>*
 template <typename T, bool DUAL> constexpr int fwd_buffers() { return 1; }
 template <typename T, bool DUAL> constexpr int fwd_waves_per_simd() { return sizeof(T) == 4 ? 6 : 5; }
 
-template <typename T, bool DUAL>
+// LOGE > 0: the tile holds 2^LOGE elements (checked by the host), so the LDS state stride is a
+// compile-time constant and every state's LDS base is an instruction offset: one XOR of the
+// lane's and the group's byte terms addresses the element in every state.  LOGE = 0: any tile.
+template <typename T, bool DUAL, int LOGE = 0>
 __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void k_tile_forward(
 	const cx<T>* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __restrict__ din, cx<T>* __restrict__ dout,
 	const TileArgs P, const cx<T>* __restrict__ mats, const GateTerms* __restrict__ gterms, long B, long b0, long ntiles,
@@ -509,7 +512,7 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void
 	constexpr int NS = DUAL ? 2 : 1;              // states per buffer
 	constexpr int GS = DUAL ? 2 : 1;              // matrix-core groups per step
 	constexpr int gq = DUAL ? 16 : 32;            // quads per matrix-core group
-	const int E = 1 << (P.K + P.logC);
+	const int E = (LOGE > 0) ? (1 << LOGE) : (1 << (P.K + P.logC));
 	const int sbytes = E * (int)sizeof(cx<T>) + kLdsPadBytes;
 	auto state = [&](int b, int i) { return reinterpret_cast<cx<T>*>(lds_raw + (b * NS + i) * sbytes); };
 	const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -554,14 +557,39 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void
 			T a[4];
 #pragma unroll
 			for (int kk = 0; kk < 4; kk++) { a[kk] = mj[amap.aoff[kk]] * amap.afac[kk]; }
-			for (int g = 0; g < qpw; g += GS * gq) {
-				int eg[GS], eg2[GS];
+			if constexpr (DUAL && LOGE > 0) {
+				// psi and dpsi of a group at one byte offset (the lane's XOR the group's), the
+				// states' bases as instruction offsets (b = 0: one buffer)
+				constexpr int ES = (int)sizeof(cx<T>);
+				const int eab = gl.ea * ES;
+				for (int g = 0; g < qpw; g += GS * gq) {
+					int la[GS];
+					GateOperands<T, GS> o;
 #pragma unroll
-				for (int n = 0; n < GS; n++) {
-					eg[n] = ewv ^ group_term(gt, (g + n * gq) >> 4);
-					eg2[n] = DUAL ? eg[n] : ewv ^ group_term(gt, ((g + n * gq) >> 4) + 1);
+					for (int n = 0; n < GS; n++) {
+						la[n] = eab ^ ((ewv ^ group_term(gt, (g + n * gq) >> 4)) * ES);
+						o.x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n]);
+						o.y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n] + sbytes);
+					}
+					__builtin_amdgcn_s_setprio(1);
+					gate_mfma<T, GS>(a, o);
+					__builtin_amdgcn_s_setprio(0);
+#pragma unroll
+					for (int n = 0; n < GS; n++) {
+						*reinterpret_cast<cx<T>*>(lds_raw + la[n]) = cx<T>{o.acc[n][0], o.acc[n][1]};
+						*reinterpret_cast<cx<T>*>(lds_raw + la[n] + sbytes) = cx<T>{o.acc[n][2], o.acc[n][3]};
+					}
 				}
-				mfma_gate_apply<T, GS>(S, D, eg, eg2, gl, a);
+			} else {
+				for (int g = 0; g < qpw; g += GS * gq) {
+					int eg[GS], eg2[GS];
+#pragma unroll
+					for (int n = 0; n < GS; n++) {
+						eg[n] = ewv ^ group_term(gt, (g + n * gq) >> 4);
+						eg2[n] = DUAL ? eg[n] : ewv ^ group_term(gt, ((g + n * gq) >> 4) + 1);
+					}
+					mfma_gate_apply<T, GS>(S, D, eg, eg2, gl, a);
+				}
 			}
 		}
 	};
@@ -611,7 +639,7 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void
 template <typename T, bool DUAL> constexpr int rev_buffers() { return DUAL ? 1 : 2; }
 template <typename T, bool DUAL> constexpr int rev_waves_per_simd() { return 2; }
 
-template <typename T, bool DUAL>
+template <typename T, bool DUAL, int LOGE = 0>
 __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void k_tile_reverse(
 	const cx<T>* __restrict__ psi_out, const cx<T>* __restrict__ dpsi_out,
 	const cx<T>* __restrict__ phi_out, const cx<T>* __restrict__ dphi_out,
@@ -624,7 +652,7 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 	constexpr int NS = DUAL ? 4 : 2;              // states per buffer: psi, [dpsi], phi, [dphi]
 	constexpr int GS = DUAL ? 2 : 1;
 	constexpr int gq = DUAL ? 16 : 32;
-	const int E = 1 << (P.K + P.logC);
+	const int E = (LOGE > 0) ? (1 << LOGE) : (1 << (P.K + P.logC));
 	const int sbytes = E * (int)sizeof(cx<T>) + lds_pad_rev<T>();
 	auto state = [&](int b, int i) { return reinterpret_cast<cx<T>*>(lds_raw + (b * NS + i) * sbytes); };
 	const int lane = threadIdx.x & 63;
@@ -653,6 +681,9 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 	const int which_plain[2] = {kMatGinv, kMatGT};
 	load_pass_mats(mats, P.x0, P.ng, DUAL ? which : which_plain, NM, rev_mat_stride<DUAL>(), lm);
 	const AOperandMap<T, DUAL> amap = a_operand_map<T, DUAL>();
+	// transposed results: lane l holds component (l & 15) >> 2 = x.re, x.im, y.re, y.im
+	const int sc_lane = (threadIdx.x & 15) >> 2;
+	const int ybo = (sc_lane < 2) ? 0 : sbytes;
 	auto prefetch = [&](long tile, int b) {
 		const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
 		tile_issue(state(b, 0), psi_out, base, io);
@@ -698,6 +729,13 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 				aa[kk] = mj[amap.aoff[kk] + 2 * kMatPairOff * (DUAL ? 2 : 1)] * amap.afac[kk];
 			}
 			const bool adjoint = (J > 0) || !(flags & 8);
+			// byte terms of the gate (LOGE > 0): the lane's load offset and, per result
+			// register, its transposed-store offset (element term, re / im)
+			constexpr int ES = (int)sizeof(cx<T>);
+			const int eab = gl.ea * ES;
+			int etb[4];
+#pragma unroll
+			for (int v = 0; v < 4; v++) { etb[v] = gl.et[v] * ES + (sc_lane & 1) * (int)sizeof(T); }
 			// GSX matrix-core groups of gq quads per step
 			auto steps = [&](auto gsc) {
 				constexpr int GSX = decltype(gsc)::value;
@@ -715,15 +753,43 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 					// state straight from the registers (the reduction maps
 					// phi_in (x) psi to the gate's phi_out (x) psi).
 					GateOperands<T, GSX> os, oa;
-					gate_load<T, GSX>(S, D, eg, eg2, gl, os);
-					gate_load<T, GSX>(F, DF, eg, eg2, gl, oa);
+					if constexpr (DUAL && LOGE > 0) {
+						// the four states (psi, dpsi, phi, dphi) of a group at one byte offset,
+						// their bases as instruction offsets (b = 0: one buffer)
+						int la[GSX];
+#pragma unroll
+						for (int n = 0; n < GSX; n++) {
+							la[n] = eab ^ (eg[n] * ES);
+							os.x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n]);
+							os.y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n] + sbytes);
+							oa.x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n] + 2 * sbytes);
+							oa.y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n] + 3 * sbytes);
+						}
+					} else {
+						gate_load<T, GSX>(S, D, eg, eg2, gl, os);
+						gate_load<T, GSX>(F, DF, eg, eg2, gl, oa);
+					}
 					// matrix-core work at raised wave priority, so that the other wave's
 					// LDS traffic issues in its shadow
 					__builtin_amdgcn_s_setprio(1);
 					gate_mfma_t<T, GSX>(au, os);
 					gate_mfma_t<T, GSX>(aa, oa);
-					gate_store_t<T, GSX>(S, D, eg, eg2, gl, os);
-					if (adjoint) { gate_store_t<T, GSX>(F, DF, eg, eg2, gl, oa); }
+					if constexpr (DUAL && LOGE > 0) {
+						// component l & 15 of the transposed results: state psi / dpsi (phi /
+						// dphi) by the lane's ybo, re / im in etb; one address for both pairs
+#pragma unroll
+						for (int n = 0; n < GSX; n++) {
+#pragma unroll
+							for (int v = 0; v < 4; v++) {
+								const int sa = (etb[v] ^ (eg[n] * ES)) + ybo;
+								*reinterpret_cast<T*>(lds_raw + sa) = os.acc[n][v];
+								if (adjoint) { *reinterpret_cast<T*>(lds_raw + sa + 2 * sbytes) = oa.acc[n][v]; }
+							}
+						}
+					} else {
+						gate_store_t<T, GSX>(S, D, eg, eg2, gl, os);
+						if (adjoint) { gate_store_t<T, GSX>(F, DF, eg, eg2, gl, oa); }
+					}
 					if constexpr (sizeof(T) == 8) {
 						// FP64: one chain over the groups' steps accumulating onto acc[J]
 #pragma unroll
