@@ -672,7 +672,7 @@ template <typename T>
 static decltype(&k_tile_forward<T, true, 0>) fwd_kernel(bool dual, int logE)
 {
 	if (!dual) { return k_tile_forward<T, false, 0>; }
-	if (sizeof(T) == 8) {
+	if constexpr (sizeof(T) == 8) {
 		if (logE == 10) { return k_tile_forward<T, true, 10>; }
 		if (logE == 9) { return k_tile_forward<T, true, 9>; }
 	} else {
@@ -686,7 +686,7 @@ template <typename T>
 static decltype(&k_tile_reverse<T, true, 0>) rev_kernel(bool dual, int logE)
 {
 	if (!dual) { return k_tile_reverse<T, false, 0>; }
-	if (sizeof(T) == 8) {
+	if constexpr (sizeof(T) == 8) {
 		if (logE == 10) { return k_tile_reverse<T, true, 10>; }
 		if (logE == 9) { return k_tile_reverse<T, true, 9>; }
 	} else {

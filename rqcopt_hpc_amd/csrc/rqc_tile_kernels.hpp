@@ -709,8 +709,7 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 		cx<T>* F = state(b, NS / 2);
 		cx<T>* DF = state(b, DUAL ? 3 : NS / 2);
 		// body of gate J (compile-time, so that its outer products accumulate in
-		// the registers acc[J] directly); dispatched once per gate on the
-		// wave-uniform gate index
+		// the registers acc[J] directly)
 		auto gate_body = [&](auto jc) {
 			constexpr int J = decltype(jc)::value;
 			if constexpr (sizeof(T) == 4 && J + 1 < kMaxTileGates) {
@@ -736,101 +735,152 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 			int etb[4];
 #pragma unroll
 			for (int v = 0; v < 4; v++) { etb[v] = gl.et[v] * ES + (sc_lane & 1) * (int)sizeof(T); }
+			// the outer products of a step's transposed results: phi_in (x) psi,
+			// mapped by the reduction to the gate's phi_out (x) psi
+			auto outer = [&](auto gsc, const auto& os, const auto& oa) {
+				constexpr int GSX = decltype(gsc)::value;
+				if constexpr (sizeof(T) == 8) {
+					// FP64: one chain over the groups' steps accumulating onto acc[J]
+#pragma unroll
+					for (int t = 0; t < 4; t++) {
+#pragma unroll
+						for (int n = 0; n < GSX; n++) { acc[J] = mfma_step(oa.acc[n][t], os.acc[n][t], acc[J]); }
+					}
+				} else {
+					// FP32: one chain over the gate's quads of this tile into pend,
+					// added into the FP64 accumulator by the next gate (no wait here)
+#pragma unroll
+					for (int t = 0; t < 4; t++) {
+#pragma unroll
+						for (int n = 0; n < GSX; n++) { pend = mfma_step(oa.acc[n][t], os.acc[n][t], pend); }
+					}
+				}
+			};
 			// GSX matrix-core groups of gq quads per step
 			auto steps = [&](auto gsc) {
 				constexpr int GSX = decltype(gsc)::value;
-				for (int g = 0; g < qpw; g += GSX * gq) {
-					int eg[GSX], eg2[GSX];
-#pragma unroll
-					for (int n = 0; n < GSX; n++) {
-						eg[n] = ewv ^ group_term(gt, (g + n * gq) >> 4);
-						eg2[n] = DUAL ? eg[n] : ewv ^ group_term(gt, ((g + n * gq) >> 4) + 1);
-					}
-					// The state's recovery (uncompute) and the adjoint's update are
-					// independent: load both, run both chains (transposed, so that the
-					// results are outer-product operands), store both; the outer
-					// products pair the adjoint at the gate input with the recovered
-					// state straight from the registers (the reduction maps
-					// phi_in (x) psi to the gate's phi_out (x) psi).
-					GateOperands<T, GSX> os, oa;
-					if constexpr (DUAL && LOGE > 0) {
-						// the four states (psi, dpsi, phi, dphi) of a group at one byte offset,
-						// their bases as instruction offsets (b = 0: one buffer)
-						int la[GSX];
+				if constexpr (DUAL && LOGE > 0) {
+					// The four states (psi, dpsi, phi, dphi) of a group at one byte offset, their
+					// bases as instruction offsets (b = 0: one buffer).  Software-pipelined over
+					// the gate's NST steps: step s + 1's operands are loaded after step s's
+					// stores (a wave's LDS accesses complete in issue order, and the steps'
+					// quads are disjoint) and before step s's outer products, whose
+					// matrix-core time covers the loads' latency.
+					constexpr int NST = ((1 << LOGE) / 4 / kTileWaves) / (GSX * gq);
+					static_assert(NST >= 1, "a wave owns at least one step of quads");
+					GateOperands<T, GSX> os[NST], oa[NST];
+					int egb[NST][GSX];
+					auto load_step = [&](auto sc) {
+						constexpr int S = decltype(sc)::value;
 #pragma unroll
 						for (int n = 0; n < GSX; n++) {
-							la[n] = eab ^ (eg[n] * ES);
-							os.x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n]);
-							os.y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n] + sbytes);
-							oa.x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n] + 2 * sbytes);
-							oa.y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n] + 3 * sbytes);
+							egb[S][n] = (ewv ^ group_term(gt, (S * GSX * gq + n * gq) >> 4)) * ES;
+							const int la = eab ^ egb[S][n];
+							os[S].x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la);
+							os[S].y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la + sbytes);
+							oa[S].x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la + 2 * sbytes);
+							oa[S].y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la + 3 * sbytes);
 						}
-					} else {
-						gate_load<T, GSX>(S, D, eg, eg2, gl, os);
-						gate_load<T, GSX>(F, DF, eg, eg2, gl, oa);
-					}
-					// matrix-core work at raised wave priority, so that the other wave's
-					// LDS traffic issues in its shadow
-					__builtin_amdgcn_s_setprio(1);
-					gate_mfma_t<T, GSX>(au, os);
-					gate_mfma_t<T, GSX>(aa, oa);
-					if constexpr (DUAL && LOGE > 0) {
+					};
+					auto step = [&](auto sc) {
+						constexpr int S = decltype(sc)::value;
+						// matrix-core work at raised wave priority, so that the other wave's
+						// LDS traffic issues in its shadow
+						__builtin_amdgcn_s_setprio(1);
+						gate_mfma_t<T, GSX>(au, os[S]);
+						gate_mfma_t<T, GSX>(aa, oa[S]);
 						// component l & 15 of the transposed results: state psi / dpsi (phi /
 						// dphi) by the lane's ybo, re / im in etb; one address for both pairs
+						int sa[GSX][4];
 #pragma unroll
 						for (int n = 0; n < GSX; n++) {
 #pragma unroll
 							for (int v = 0; v < 4; v++) {
-								const int sa = (etb[v] ^ (eg[n] * ES)) + ybo;
-								*reinterpret_cast<T*>(lds_raw + sa) = os.acc[n][v];
-								if (adjoint) { *reinterpret_cast<T*>(lds_raw + sa + 2 * sbytes) = oa.acc[n][v]; }
+								sa[n][v] = (etb[v] ^ egb[S][n]) + ybo;
+								*reinterpret_cast<T*>(lds_raw + sa[n][v]) = os[S].acc[n][v];
 							}
 						}
-					} else {
+						if (adjoint) {
+#pragma unroll
+							for (int n = 0; n < GSX; n++) {
+#pragma unroll
+								for (int v = 0; v < 4; v++) {
+									*reinterpret_cast<T*>(lds_raw + sa[n][v] + 2 * sbytes) = oa[S].acc[n][v];
+								}
+							}
+						}
+						if constexpr (S + 1 < NST) { load_step(std::integral_constant<int, S + 1>{}); }
+						// (a scheduling fence: the stores and loads issue before the outer
+						// products, which would otherwise hold them back behind the busy
+						// matrix core)
+						__builtin_amdgcn_sched_barrier(0);
+						outer(gsc, os[S], oa[S]);
+						__builtin_amdgcn_s_setprio(0);
+					};
+					load_step(std::integral_constant<int, 0>{});
+					step(std::integral_constant<int, 0>{});
+					if constexpr (NST > 1) { step(std::integral_constant<int, 1>{}); }
+					if constexpr (NST > 2) { step(std::integral_constant<int, 2>{}); }
+					if constexpr (NST > 3) { step(std::integral_constant<int, 3>{}); }
+					static_assert(NST <= 4, "at most four steps per gate and wave");
+				} else {
+					for (int g = 0; g < qpw; g += GSX * gq) {
+						int eg[GSX], eg2[GSX];
+#pragma unroll
+						for (int n = 0; n < GSX; n++) {
+							eg[n] = ewv ^ group_term(gt, (g + n * gq) >> 4);
+							eg2[n] = DUAL ? eg[n] : ewv ^ group_term(gt, ((g + n * gq) >> 4) + 1);
+						}
+						// The state's recovery (uncompute) and the adjoint's update are
+						// independent: load both, run both chains (transposed, so that the
+						// results are outer-product operands), store both; the outer
+						// products pair the adjoint at the gate input with the recovered
+						// state straight from the registers.
+						GateOperands<T, GSX> os, oa;
+						gate_load<T, GSX>(S, D, eg, eg2, gl, os);
+						gate_load<T, GSX>(F, DF, eg, eg2, gl, oa);
+						__builtin_amdgcn_s_setprio(1);
+						gate_mfma_t<T, GSX>(au, os);
+						gate_mfma_t<T, GSX>(aa, oa);
 						gate_store_t<T, GSX>(S, D, eg, eg2, gl, os);
 						if (adjoint) { gate_store_t<T, GSX>(F, DF, eg, eg2, gl, oa); }
+						outer(gsc, os, oa);
+						__builtin_amdgcn_s_setprio(0);
 					}
-					if constexpr (sizeof(T) == 8) {
-						// FP64: one chain over the groups' steps accumulating onto acc[J]
-#pragma unroll
-						for (int t = 0; t < 4; t++) {
-#pragma unroll
-							for (int n = 0; n < GSX; n++) { acc[J] = mfma_step(oa.acc[n][t], os.acc[n][t], acc[J]); }
-						}
-					} else {
-						// FP32: one chain over the gate's quads of this tile into pend,
-						// added into the FP64 accumulator by the next gate (no wait here)
-#pragma unroll
-						for (int t = 0; t < 4; t++) {
-#pragma unroll
-							for (int n = 0; n < GSX; n++) { pend = mfma_step(oa.acc[n][t], os.acc[n][t], pend); }
-						}
-					}
-					__builtin_amdgcn_s_setprio(0);
 				}
 			};
 			if constexpr (DUAL && sizeof(T) == 4) {
 				// FP32 dual sweeps: four groups (64 quads) per step when the wave's quads allow
-				if (P.wide && (qpw % (4 * gq)) == 0) { steps(std::integral_constant<int, 4>{}); }
-				else { steps(std::integral_constant<int, GS>{}); }
+				constexpr bool wide_fits = (LOGE == 0) || (((1 << LOGE) / 4 / kTileWaves) % (4 * gq)) == 0;
+				if constexpr (wide_fits) {
+					if (P.wide && (qpw % (4 * gq)) == 0) { steps(std::integral_constant<int, 4>{}); }
+					else { steps(std::integral_constant<int, GS>{}); }
+				} else {
+					steps(std::integral_constant<int, GS>{});
+				}
 			} else {
 				steps(std::integral_constant<int, GS>{});
 			}
 		};
-		static_assert(kMaxTileGates == 8, "one case per gate of a pass");
-		for (int j = P.ng - 1; j >= 0; j--) {
-			if (j + 1 < P.ng && ((P.barrier >> (j + 1)) & 1)) { __syncthreads(); }
-			switch (j) {
-			case 0: gate_body(std::integral_constant<int, 0>{}); break;
-			case 1: gate_body(std::integral_constant<int, 1>{}); break;
-			case 2: gate_body(std::integral_constant<int, 2>{}); break;
-			case 3: gate_body(std::integral_constant<int, 3>{}); break;
-			case 4: gate_body(std::integral_constant<int, 4>{}); break;
-			case 5: gate_body(std::integral_constant<int, 5>{}); break;
-			case 6: gate_body(std::integral_constant<int, 6>{}); break;
-			default: gate_body(std::integral_constant<int, 7>{}); break;
+		// The pass's gates, last first, as straight-line code under wave-uniform
+		// guards: each accumulator acc[J] stays in its registers (a loop over a switch
+		// on the gate index makes the compiler move every accumulator on each gate).
+		auto gate_at = [&](auto jc) {
+			constexpr int J = decltype(jc)::value;
+			if (J < P.ng) {
+				if (J + 1 < P.ng && ((P.barrier >> (J + 1)) & 1)) { __syncthreads(); }
+				gate_body(jc);
 			}
-		}
+		};
+		static_assert(kMaxTileGates == 8, "one guarded body per gate of a pass");
+		gate_at(std::integral_constant<int, 7>{});
+		gate_at(std::integral_constant<int, 6>{});
+		gate_at(std::integral_constant<int, 5>{});
+		gate_at(std::integral_constant<int, 4>{});
+		gate_at(std::integral_constant<int, 3>{});
+		gate_at(std::integral_constant<int, 2>{});
+		gate_at(std::integral_constant<int, 1>{});
+		gate_at(std::integral_constant<int, 0>{});
 		if constexpr (sizeof(T) == 4) { flush32(acc[0]); }
 	};
 	long tile = blockIdx.x;
