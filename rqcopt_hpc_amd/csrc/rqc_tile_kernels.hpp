@@ -492,17 +492,22 @@ __device__ __forceinline__ void tile_store(cx<T>* __restrict__ dst, const cx<T>*
 // column block b0 of the identity (flags & 1) or `in`; dpsi is zero (flags & 2)
 // or `din`.
 // Forward-pass buffering: one buffer (a workgroup stores a tile and loads the
-// next into the same slots) and 5 (FP64) or 6 (FP32) workgroups per CU, whose
-// gates cover each other's tile loads (measured slightly faster than two
-// double-buffered workgroups fewer).
+// next into the same slots) and several workgroups per CU, whose gates cover
+// each other's tile loads (measured slightly faster than two double-buffered
+// workgroups fewer): four for the 16 KiB states of the planned dual tiles (LDS
+// holds no more), which leaves 128 registers for the pipelined steps, else 5
+// (FP64) or 6 (FP32).
 template <typename T, bool DUAL> constexpr int fwd_buffers() { return 1; }
-template <typename T, bool DUAL> constexpr int fwd_waves_per_simd() { return sizeof(T) == 4 ? 6 : 5; }
+template <typename T, bool DUAL, int LOGE = 0> constexpr int fwd_waves_per_simd()
+{
+	return (DUAL && LOGE > 0 && (1 << LOGE) * (int)sizeof(cx<T>) >= 16384) ? 4 : (sizeof(T) == 4 ? 6 : 5);
+}
 
 // LOGE > 0: the tile holds 2^LOGE elements (checked by the host), so the LDS state stride is a
 // compile-time constant and every state's LDS base is an instruction offset: one XOR of the
 // lane's and the group's byte terms addresses the element in every state.  LOGE = 0: any tile.
 template <typename T, bool DUAL, int LOGE = 0>
-__global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void k_tile_forward(
+__global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())) void k_tile_forward(
 	const cx<T>* __restrict__ in, cx<T>* __restrict__ out, const cx<T>* __restrict__ din, cx<T>* __restrict__ dout,
 	const TileArgs P, const cx<T>* __restrict__ mats, const GateTerms* __restrict__ gterms, long B, long b0, long ntiles,
 	int flags, const int* skip)
@@ -548,8 +553,7 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void
 	auto run_tile = [&](int b) {
 		cx<T>* S = state(b, 0);
 		cx<T>* D = state(b, DUAL ? 1 : 0);
-		for (int j = 0; j < P.ng; j++) {
-			if (j > 0 && ((P.barrier >> j) & 1)) { __syncthreads(); }
+		auto gate = [&](int j) {
 			const GateTerms gt = gterms[P.x0 + j];
 			const GateLanes gl = gate_lanes(gt);
 			const int ewv = sel4(gt.wv, wave);
@@ -559,27 +563,44 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void
 			for (int kk = 0; kk < 4; kk++) { a[kk] = mj[amap.aoff[kk]] * amap.afac[kk]; }
 			if constexpr (DUAL && LOGE > 0) {
 				// psi and dpsi of a group at one byte offset (the lane's XOR the group's), the
-				// states' bases as instruction offsets (b = 0: one buffer)
+				// states' bases as instruction offsets (b = 0: one buffer).  Software-pipelined
+				// over the gate's NST steps: step s + 1's operands are loaded before step s's
+				// matrix-core chain (the steps' quads are disjoint), so that the chain covers
+				// the loads' latency.
 				constexpr int ES = (int)sizeof(cx<T>);
+				constexpr int NST = ((1 << LOGE) / 4 / kTileWaves) / (GS * gq);
+				static_assert(NST >= 1 && NST <= 4, "one to four steps per gate and wave");
 				const int eab = gl.ea * ES;
-				for (int g = 0; g < qpw; g += GS * gq) {
-					int la[GS];
-					GateOperands<T, GS> o;
+				GateOperands<T, GS> o[NST];
+				int la[NST][GS];
+				auto load_step = [&](auto sc) {
+					constexpr int SI = decltype(sc)::value;
 #pragma unroll
 					for (int n = 0; n < GS; n++) {
-						la[n] = eab ^ ((ewv ^ group_term(gt, (g + n * gq) >> 4)) * ES);
-						o.x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n]);
-						o.y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[n] + sbytes);
+						la[SI][n] = eab ^ ((ewv ^ group_term(gt, (SI * GS * gq + n * gq) >> 4)) * ES);
+						o[SI].x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[SI][n]);
+						o[SI].y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[SI][n] + sbytes);
 					}
+				};
+				auto step = [&](auto sc) {
+					constexpr int SI = decltype(sc)::value;
+					if constexpr (SI + 1 < NST) { load_step(std::integral_constant<int, SI + 1>{}); }
+					// (a scheduling fence: those loads issue before the chain)
+					__builtin_amdgcn_sched_barrier(0);
 					__builtin_amdgcn_s_setprio(1);
-					gate_mfma<T, GS>(a, o);
+					gate_mfma<T, GS>(a, o[SI]);
 					__builtin_amdgcn_s_setprio(0);
 #pragma unroll
 					for (int n = 0; n < GS; n++) {
-						*reinterpret_cast<cx<T>*>(lds_raw + la[n]) = cx<T>{o.acc[n][0], o.acc[n][1]};
-						*reinterpret_cast<cx<T>*>(lds_raw + la[n] + sbytes) = cx<T>{o.acc[n][2], o.acc[n][3]};
+						*reinterpret_cast<cx<T>*>(lds_raw + la[SI][n]) = cx<T>{o[SI].acc[n][0], o[SI].acc[n][1]};
+						*reinterpret_cast<cx<T>*>(lds_raw + la[SI][n] + sbytes) = cx<T>{o[SI].acc[n][2], o[SI].acc[n][3]};
 					}
-				}
+				};
+				load_step(std::integral_constant<int, 0>{});
+				step(std::integral_constant<int, 0>{});
+				if constexpr (NST > 1) { step(std::integral_constant<int, 1>{}); }
+				if constexpr (NST > 2) { step(std::integral_constant<int, 2>{}); }
+				if constexpr (NST > 3) { step(std::integral_constant<int, 3>{}); }
 			} else {
 				for (int g = 0; g < qpw; g += GS * gq) {
 					int eg[GS], eg2[GS];
@@ -590,6 +611,14 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL>())) void
 					}
 					mfma_gate_apply<T, GS>(S, D, eg, eg2, gl, a);
 				}
+			}
+		};
+		// the pass's gates as straight-line code under wave-uniform guards
+#pragma unroll
+		for (int j = 0; j < kMaxTileGates; j++) {
+			if (j < P.ng) {
+				if (j > 0 && ((P.barrier >> j) & 1)) { __syncthreads(); }
+				gate(j);
 			}
 		}
 	};
