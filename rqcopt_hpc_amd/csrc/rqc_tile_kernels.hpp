@@ -525,7 +525,7 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 	const long cmask = (1L << P.lognc) - 1;
 	TileIO<T> io;
 	tile_io_init(P, E, B, io);
-	// gate matrices G, Z of the pass after the two buffers
+	// gate matrices G, Z of the pass after the buffers
 	cx<T>* lm = reinterpret_cast<cx<T>*>(lds_raw + fwd_buffers<T, DUAL>() * NS * sbytes);
 	const int which[2] = {kMatG, kMatZ};
 	load_pass_mats(mats, P.x0, P.ng, which, DUAL ? 2 : 1, fwd_mat_stride<DUAL>(), lm);
@@ -550,34 +550,59 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 		if (DUAL) { tile_store(dout, state(b, 1), base, io); }
 	};
 	// the pass's gates on the tile in buffer b
+	// Dual sweeps with a planned tile: the gates' position terms are fixed over the
+	// launch's tiles, so they are formed once: the lane's byte term (eaj) and the
+	// wave's byte terms (sgj: the wave's range, then the group bits).
+	constexpr int ES = (int)sizeof(cx<T>);
+	constexpr bool FIXED = DUAL && LOGE > 0;
+	int eaj[kMaxTileGates];
+	int sgj[kMaxTileGates][4];
+#pragma unroll
+	for (int j = 0; j < kMaxTileGates; j++) {
+		eaj[j] = 0;
+		sgj[j][0] = sgj[j][1] = sgj[j][2] = sgj[j][3] = 0;
+		if (FIXED && j < P.ng) {
+			const GateTerms gt = gterms[P.x0 + j];
+			eaj[j] = gate_lanes(gt).ea * ES;
+			sgj[j][0] = sel4(gt.wv, wave) * ES;
+			sgj[j][1] = gt.gbit[0] * ES;
+			sgj[j][2] = gt.gbit[1] * ES;
+			sgj[j][3] = gt.gbit[2] * ES;
+		}
+	}
+	// the pass's gates on the tile in buffer b
 	auto run_tile = [&](int b) {
 		cx<T>* S = state(b, 0);
 		cx<T>* D = state(b, DUAL ? 1 : 0);
-		auto gate = [&](int j) {
-			const GateTerms gt = gterms[P.x0 + j];
-			const GateLanes gl = gate_lanes(gt);
-			const int ewv = sel4(gt.wv, wave);
+		auto gate = [&](auto jc) {
+			constexpr int j = decltype(jc)::value;
 			const T* mj = reinterpret_cast<const T*>(lm + j * fwd_mat_stride<DUAL>());
 			T a[4];
 #pragma unroll
 			for (int kk = 0; kk < 4; kk++) { a[kk] = mj[amap.aoff[kk]] * amap.afac[kk]; }
-			if constexpr (DUAL && LOGE > 0) {
+			if constexpr (FIXED) {
 				// psi and dpsi of a group at one byte offset (the lane's XOR the group's), the
 				// states' bases as instruction offsets (b = 0: one buffer).  Software-pipelined
 				// over the gate's NST steps: step s + 1's operands are loaded before step s's
 				// matrix-core chain (the steps' quads are disjoint), so that the chain covers
 				// the loads' latency.
-				constexpr int ES = (int)sizeof(cx<T>);
 				constexpr int NST = ((1 << LOGE) / 4 / kTileWaves) / (GS * gq);
 				static_assert(NST >= 1 && NST <= 4, "one to four steps per gate and wave");
-				const int eab = gl.ea * ES;
 				GateOperands<T, GS> o[NST];
 				int la[NST][GS];
+				// (an empty asm statement makes the lane term opaque here, so that the
+				// compiler forms the groups' offsets per tile instead of holding every
+				// gate's offsets in registers across the tile loop)
+				int ea = eaj[j];
+				asm volatile("" : "+v"(ea));
 				auto load_step = [&](auto sc) {
 					constexpr int SI = decltype(sc)::value;
 #pragma unroll
 					for (int n = 0; n < GS; n++) {
-						la[SI][n] = eab ^ ((ewv ^ group_term(gt, (SI * GS * gq + n * gq) >> 4)) * ES);
+						const int gi = (SI * GS * gq + n * gq) >> 4;   // group index in the wave's range
+						const int gb = sgj[j][0] ^ ((gi & 1) ? sgj[j][1] : 0) ^ ((gi & 2) ? sgj[j][2] : 0) ^
+						               ((gi & 4) ? sgj[j][3] : 0);
+						la[SI][n] = ea ^ gb;
 						o[SI].x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[SI][n]);
 						o[SI].y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[SI][n] + sbytes);
 					}
@@ -602,6 +627,9 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 				if constexpr (NST > 2) { step(std::integral_constant<int, 2>{}); }
 				if constexpr (NST > 3) { step(std::integral_constant<int, 3>{}); }
 			} else {
+				const GateTerms gt = gterms[P.x0 + j];
+				const GateLanes gl = gate_lanes(gt);
+				const int ewv = sel4(gt.wv, wave);
 				for (int g = 0; g < qpw; g += GS * gq) {
 					int eg[GS], eg2[GS];
 #pragma unroll
@@ -614,13 +642,22 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 			}
 		};
 		// the pass's gates as straight-line code under wave-uniform guards
-#pragma unroll
-		for (int j = 0; j < kMaxTileGates; j++) {
+		auto gate_at = [&](auto jc) {
+			constexpr int j = decltype(jc)::value;
 			if (j < P.ng) {
 				if (j > 0 && ((P.barrier >> j) & 1)) { __syncthreads(); }
-				gate(j);
+				gate(jc);
 			}
-		}
+		};
+		static_assert(kMaxTileGates == 8, "one guarded body per gate of a pass");
+		gate_at(std::integral_constant<int, 0>{});
+		gate_at(std::integral_constant<int, 1>{});
+		gate_at(std::integral_constant<int, 2>{});
+		gate_at(std::integral_constant<int, 3>{});
+		gate_at(std::integral_constant<int, 4>{});
+		gate_at(std::integral_constant<int, 5>{});
+		gate_at(std::integral_constant<int, 6>{});
+		gate_at(std::integral_constant<int, 7>{});
 	};
 	long tile = blockIdx.x;
 	if (tile < ntiles) { prefetch(tile, 0); }
@@ -751,7 +788,7 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 			// recovery: [[G^-1, 0], [-G^-1 Z G^-1, G^-1]]; adjoint: [[G^T, 0], [Z^T, G^T]]
 			const T* mj = reinterpret_cast<const T*>(lm + J * rev_mat_stride<DUAL>());
 			T au[4], aa[4];
-	#pragma unroll
+#pragma unroll
 			for (int kk = 0; kk < 4; kk++) {
 				au[kk] = mj[amap.aoff[kk]] * amap.afac[kk];
 				aa[kk] = mj[amap.aoff[kk] + 2 * kMatPairOff * (DUAL ? 2 : 1)] * amap.afac[kk];
