@@ -265,6 +265,21 @@ __device__ __forceinline__ int sel4_affine(const int (&v)[4], int i)
 	return v[0] ^ ((v[1] ^ v[0]) & m1) ^ ((v[2] ^ v[0]) & m2);
 }
 
+// transposed results: et[v] = bt ^ tv(v), the lane's term bt and the register's
+// wave-uniform term tv(v) (quad q of a group: position q4[q & 3] ^ w[(q >> 2) & 3])
+template <typename T>
+__device__ __forceinline__ int transposed_lane_term(const GateTerms& t)
+{
+	const int lane = threadIdx.x & 63;
+	return (sizeof(T) == 8 ? sel4_affine(t.q4, lane >> 4) : sel4_affine(t.w, lane >> 4)) ^ sel4_affine(t.u, lane & 3);
+}
+
+template <typename T>
+__device__ __forceinline__ int transposed_reg_term(const GateTerms& t, int v)
+{
+	return (sizeof(T) == 8) ? t.w[v] : t.q4[v];
+}
+
 template <typename T = double>
 __device__ __forceinline__ GateLanes gate_lanes(const GateTerms& t)
 {
@@ -272,16 +287,9 @@ __device__ __forceinline__ GateLanes gate_lanes(const GateTerms& t)
 	GateLanes g;
 	const int base = sel4_affine(t.q4, lane & 3) ^ sel4_affine(t.w, (lane >> 2) & 3);
 	g.ea = base ^ sel4_affine(t.u, lane >> 4);
-	// quad q of a group: position q4[q & 3] ^ w[(q >> 2) & 3]
-	if constexpr (sizeof(T) == 8) {
-		const int bt = sel4_affine(t.q4, lane >> 4) ^ sel4_affine(t.u, lane & 3);
+	const int bt = transposed_lane_term<T>(t);
 #pragma unroll
-		for (int v = 0; v < 4; v++) { g.et[v] = bt ^ t.w[v]; }
-	} else {
-		const int bt = sel4_affine(t.w, lane >> 4) ^ sel4_affine(t.u, lane & 3);
-#pragma unroll
-		for (int v = 0; v < 4; v++) { g.et[v] = bt ^ t.q4[v]; }
-	}
+	for (int v = 0; v < 4; v++) { g.et[v] = bt ^ transposed_reg_term<T>(t, v); }
 	return g;
 }
 
@@ -768,6 +776,33 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 		tile_store(phi_in, state(b, NS / 2), base, io);
 		if (DUAL) { tile_store(dphi_in, state(b, 3), base, io); }
 	};
+	// FP32 dual sweeps with a planned tile: the gates' position terms are fixed over
+	// the launch's tiles, so they are formed once: the lane's load term (eabj) and
+	// transposed-store term (btbj, with the re / im byte of its component), and
+	// the wave's terms (sgj: the wave's range, the group bits, then the result
+	// registers' terms).  (FP64 forms them per gate: its accumulators leave no room
+	// for the held terms, measured 3 % slower with them.)
+	constexpr int ES = (int)sizeof(cx<T>);
+	constexpr bool FIXED = DUAL && LOGE > 0 && sizeof(T) == 4;
+	int eabj[kMaxTileGates], btbj[kMaxTileGates];
+	int sgj[kMaxTileGates][8];
+#pragma unroll
+	for (int j = 0; j < kMaxTileGates; j++) {
+		eabj[j] = btbj[j] = 0;
+#pragma unroll
+		for (int i = 0; i < 8; i++) { sgj[j][i] = 0; }
+		if (FIXED && j < P.ng) {
+			const GateTerms gt = gterms[P.x0 + j];
+			eabj[j] = gate_lanes<T>(gt).ea * ES;
+			btbj[j] = (transposed_lane_term<T>(gt) * ES) | ((sc_lane & 1) * (int)sizeof(T));
+			sgj[j][0] = sel4(gt.wv, wave) * ES;
+			sgj[j][1] = gt.gbit[0] * ES;
+			sgj[j][2] = gt.gbit[1] * ES;
+			sgj[j][3] = gt.gbit[2] * ES;
+#pragma unroll
+			for (int v = 0; v < 4; v++) { sgj[j][4 + v] = transposed_reg_term<T>(gt, v) * ES; }
+		}
+	}
 	// the pass's gates on the tile in buffer b
 	auto run_tile = [&](int b) {
 		cx<T>* S = state(b, 0);
@@ -782,9 +817,6 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 				// FP32: the previous gate (J + 1) left its tile sums in pend
 				if (J + 1 < P.ng) { flush32(acc[J + 1]); }
 			}
-			const GateTerms gt = gterms[P.x0 + J];
-			const GateLanes gl = gate_lanes<T>(gt);
-			const int ewv = sel4(gt.wv, wave);
 			// recovery: [[G^-1, 0], [-G^-1 Z G^-1, G^-1]]; adjoint: [[G^T, 0], [Z^T, G^T]]
 			const T* mj = reinterpret_cast<const T*>(lm + J * rev_mat_stride<DUAL>());
 			T au[4], aa[4];
@@ -794,13 +826,6 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 				aa[kk] = mj[amap.aoff[kk] + 2 * kMatPairOff * (DUAL ? 2 : 1)] * amap.afac[kk];
 			}
 			const bool adjoint = (J > 0) || !(flags & 8);
-			// byte terms of the gate (LOGE > 0): the lane's load offset and, per result
-			// register, its transposed-store offset (element term, re / im)
-			constexpr int ES = (int)sizeof(cx<T>);
-			const int eab = gl.ea * ES;
-			int etb[4];
-#pragma unroll
-			for (int v = 0; v < 4; v++) { etb[v] = gl.et[v] * ES + (sc_lane & 1) * (int)sizeof(T); }
 			// the outer products of a step's transposed results: phi_in (x) psi,
 			// mapped by the reduction to the gate's phi_out (x) psi
 			auto outer = [&](auto gsc, const auto& os, const auto& oa) {
@@ -836,11 +861,33 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 					static_assert(NST >= 1, "a wave owns at least one step of quads");
 					GateOperands<T, GSX> os[NST], oa[NST];
 					int egb[NST][GSX];
+					// (an empty asm statement makes the lane terms opaque here, so that the
+					// compiler forms the groups' offsets per tile instead of holding every
+					// gate's offsets in registers across the tile loop)
+					int eab, btb, sg[8];
+					if constexpr (FIXED) {
+						eab = eabj[J];
+						btb = btbj[J];
+						asm volatile("" : "+v"(eab), "+v"(btb));
+#pragma unroll
+						for (int i = 0; i < 8; i++) { sg[i] = sgj[J][i]; }
+					} else {
+						const GateTerms gt = gterms[P.x0 + J];
+						eab = gate_lanes<T>(gt).ea * ES;
+						btb = (transposed_lane_term<T>(gt) * ES) | ((sc_lane & 1) * (int)sizeof(T));
+						sg[0] = sel4(gt.wv, wave) * ES;
+						sg[1] = gt.gbit[0] * ES;
+						sg[2] = gt.gbit[1] * ES;
+						sg[3] = gt.gbit[2] * ES;
+#pragma unroll
+						for (int v = 0; v < 4; v++) { sg[4 + v] = transposed_reg_term<T>(gt, v) * ES; }
+					}
 					auto load_step = [&](auto sc) {
 						constexpr int S = decltype(sc)::value;
 #pragma unroll
 						for (int n = 0; n < GSX; n++) {
-							egb[S][n] = (ewv ^ group_term(gt, (S * GSX * gq + n * gq) >> 4)) * ES;
+							const int gi = (S * GSX * gq + n * gq) >> 4;   // group index in the wave's range
+							egb[S][n] = sg[0] ^ ((gi & 1) ? sg[1] : 0) ^ ((gi & 2) ? sg[2] : 0) ^ ((gi & 4) ? sg[3] : 0);
 							const int la = eab ^ egb[S][n];
 							os[S].x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la);
 							os[S].y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la + sbytes);
@@ -862,7 +909,7 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 						for (int n = 0; n < GSX; n++) {
 #pragma unroll
 							for (int v = 0; v < 4; v++) {
-								sa[n][v] = (etb[v] ^ egb[S][n]) + ybo;
+								sa[n][v] = (btb ^ (sg[4 + v] ^ egb[S][n])) + ybo;
 								*reinterpret_cast<T*>(lds_raw + sa[n][v]) = os[S].acc[n][v];
 							}
 						}
@@ -890,6 +937,9 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 					if constexpr (NST > 3) { step(std::integral_constant<int, 3>{}); }
 					static_assert(NST <= 4, "at most four steps per gate and wave");
 				} else {
+					const GateTerms gt = gterms[P.x0 + J];
+					const GateLanes gl = gate_lanes<T>(gt);
+					const int ewv = sel4(gt.wv, wave);
 					for (int g = 0; g < qpw; g += GSX * gq) {
 						int eg[GSX], eg2[GSX];
 #pragma unroll
