@@ -512,9 +512,13 @@ static TileArgs make_tile_args(const Circuit& C, const Pass& p, int K, int logC,
 		col[t] = (c <= 3) ? c : 1;
 	}
 	const int shift = (elem_bytes == 16) ? 0 : 1;
+	// FP32: an 8-byte LDS read serves 32 lanes over 256 bytes (32 elements), i.e. two
+	// amplitudes of a 16-quad group, whose rows differ in one tile bit; column bit 4 in every
+	// mask (bit 4 of f = parity of the row) puts them in different halves of the window
+	const int bit4 = (elem_bytes == 8 && logC >= 5) ? 16 : 0;
 	for (int t = 0; t < K; t++) {
-		const int h = (col[t] << shift) & ((1 << logC) - 1);
-		for (int i = 0; i < 4; i++) {
+		const int h = ((col[t] << shift) | bit4) & ((1 << logC) - 1);
+		for (int i = 0; i < 5; i++) {
 			if ((h >> i) & 1) { a.swz[i] |= 1u << t; }
 		}
 	}

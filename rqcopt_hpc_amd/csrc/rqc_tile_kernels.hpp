@@ -91,7 +91,7 @@ struct TileArgs
 	int x0;            // position of the pass's first gate in the matrix table
 	int lognc;         // log2 of the column chunks of the block (B >> logC)
 	unsigned rmask;    // row bits outside the tile (rest), deposited lowest first
-	unsigned swz[4];   // column swizzle masks over tile rows
+	unsigned swz[5];   // column swizzle masks over tile rows (bit i of the column term)
 	unsigned barrier;  // bit j: the waves own different quads for gates j - 1 and j
 	int wide;          // FP32 dual reverse pass: four matrix-core groups per step
 	int rb[16];        // row bit of tile bit t, t < K
@@ -174,7 +174,8 @@ __host__ __device__ __forceinline__ int tile_quad_row(int q, int lo, int hi)
 __host__ __device__ __forceinline__ int swz_f(int row, const TileArgs& P)
 {
 	return (__builtin_popcount(row & P.swz[0]) & 1) | ((__builtin_popcount(row & P.swz[1]) & 1) << 1) |
-	       ((__builtin_popcount(row & P.swz[2]) & 1) << 2) | ((__builtin_popcount(row & P.swz[3]) & 1) << 3);
+	       ((__builtin_popcount(row & P.swz[2]) & 1) << 2) | ((__builtin_popcount(row & P.swz[3]) & 1) << 3) |
+	       ((__builtin_popcount(row & P.swz[4]) & 1) << 4);
 }
 
 __host__ __device__ __forceinline__ int lds_elem(int row, int col, const TileArgs& P)
