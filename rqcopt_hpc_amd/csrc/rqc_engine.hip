@@ -687,9 +687,16 @@ static decltype(&k_tile_forward<T, true, 0>) fwd_kernel(bool dual, int logE)
 }
 
 template <typename T>
-static decltype(&k_tile_reverse<T, true, 0>) rev_kernel(bool dual, int logE)
+static decltype(&k_tile_reverse<T, true, 0>) rev_kernel(bool dual, int logE, bool seed = false)
 {
 	if (!dual) { return k_tile_reverse<T, false, 0>; }
+	if constexpr (sizeof(T) == 8) {
+		if (seed) {
+			if (logE == 10) { return k_tile_reverse<T, true, 10, true>; }
+			if (logE == 9) { return k_tile_reverse<T, true, 9, true>; }
+			return k_tile_reverse<T, true, 0, true>;
+		}
+	}
 	if constexpr (sizeof(T) == 8) {
 		if (logE == 10) { return k_tile_reverse<T, true, 10>; }
 		if (logE == 9) { return k_tile_reverse<T, true, 9>; }
@@ -764,6 +771,7 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 	auto* kr = rev_kernel<T>(dual, ts.K + ts.logC);
 	allow_lds(kf, tp->lds_fwd);
 	allow_lds(kr, tp->lds_rev);
+	allow_lds(rev_kernel<T>(dual, ts.K + ts.logC, true), tp->lds_rev);
 	int occ_f = 0, occ_r = 0;
 	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, reinterpret_cast<const void*>(kf), kTileThreads, tp->lds_fwd));
 	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, reinterpret_cast<const void*>(kr), kTileThreads, tp->lds_rev));
@@ -831,8 +839,10 @@ EvalPlan plan_eval(const Context& dev, const Circuit& C, Mode mode, int per, boo
 	pl.n_acc = 1 + 32 * (size_t)pl.G;
 	size_t npart = pl.gtr;
 	if (pl.tiles) {
-		// outer-product partials, then the chunk sums of the first reduction stage
-		npart = std::max(npart, (pl.tp->partial_doubles + (size_t)kReduceChunks * kMaxTileGates * 256 + 1) / 2);
+		// outer-product partials, the chunk sums of the first reduction stage, then the trace
+		// partials of a first reverse pass that takes the target's trace
+		npart = std::max(npart, (pl.tp->partial_doubles + (size_t)kReduceChunks * kMaxTileGates * 256 + 1) / 2 +
+			(size_t)pl.tp->grev * kTileWaves);
 	}
 	else { npart = std::max(npart, (size_t)pl.grev_pg * kWidth); }
 	pl.n_part = npart;
@@ -999,6 +1009,7 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	const long ntiles = (N >> tp.ts.K) * (B >> tp.ts.logC);
 	auto* kf = fwd_kernel<T>(dual, tp.tsf.K + tp.tsf.logC);
 	auto* kr = rev_kernel<T>(dual, tp.ts.K + tp.ts.logC);
+	auto* krs = rev_kernel<T>(dual, tp.ts.K + tp.ts.logC, true);
 	const GateTerms* fterms = reinterpret_cast<const GateTerms*>(reinterpret_cast<const char*>(gterms) + tp.fterms_off);
 	const zc* rmats = reinterpret_cast<const zc*>(reinterpret_cast<const char*>(mats) + pl.rmats_off);
 	double* mpart = reinterpret_cast<double*>(d_partial);   // grev * waves * ng * 256 doubles
@@ -1008,6 +1019,11 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	const int logB = ilog2_or_neg(B);
 	const double ab = (double)sz * sizeof(cx<T>);   // bytes of one state block
 	const int ns = dual ? 2 : 1;                     // state arrays per side
+	// FP64 HVP sweeps over the whole of a dense target in one block: the first reverse pass
+	// reads U's block as its output adjoint, forms the seed -conj(U) in LDS and takes the
+	// trace on the way (no separate target pass over U, psi and the seed)
+	const bool fuse_trace = sizeof(T) == 8 && dual && be == nullptr && !pl.sampled && B == N && P >= 1;
+	double* ttr = cpart + (size_t)kReduceChunks * kMaxTileGates * 256;   // its per-wave trace partials
 
 	for (long blk = 0; blk < pl.nblk; blk++) {
 		const long b0 = blk * B;
@@ -1030,12 +1046,12 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 			k_blockenc_residual<T><<<gtr, kBlock, 0, s.stream>>>(be->H, pl.L / 2, b0, psiG,
 				mode == Mode::Target ? nullptr : phi[0], dual ? DPS(P) : nullptr, dual ? be->J : nullptr, B, logB, N * B,
 				tpart, skip);
-		} else {
+		} else if (!fuse_trace) {
 			LaunchTimer lt(s, RQC_KERNEL_TARGET, (double)sz * sizeof(zc) + (mode == Mode::Target ? 1 : 2) * ab);
 			k_target_trace<T><<<gtr, kBlock, 0, s.stream>>>(seed, ldu, u0, psiG, mode == Mode::Target ? nullptr : phi[0],
 				B, logB, N * B, tpart, skip);
 		}
-		{
+		if (!fuse_trace) {
 			LaunchTimer lt(s, RQC_KERNEL_OTHER);
 			k_reduce_partials<<<1, kBlock, 0, s.stream>>>(tpart, (int)gtr, 1, d_acc, skip);
 		}
@@ -1045,13 +1061,19 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 			int flags = 0;
 			if (p == P - 1) { flags |= 4; }
 			if (p == 0) { flags |= 8; }
+			const bool seed_pass = fuse_trace && p == P - 1;
 			const int ng = (int)tp.passes[p].gates.size();
 			{
 				const int reads = dual ? 3 + (p < P - 1) : 2;
 				LaunchTimer lt(s, RQC_KERNEL_REVERSE, (double)(reads + ns * (p > 0)) * ab);
-				kr<<<tp.grev, kTileThreads, tp.lds_rev, s.stream>>>(PS(p + 1), dual ? DPS(p + 1) : nullptr, phi[cur],
-					dual ? dphi[cur] : nullptr, phi[1 - cur], dual ? dphi[1 - cur] : nullptr, tp.args[p], mats, gterms, B,
-					ntiles, flags, mpart, skip);
+				(seed_pass ? krs : kr)<<<tp.grev, kTileThreads, tp.lds_rev, s.stream>>>(PS(p + 1), dual ? DPS(p + 1) : nullptr,
+					seed_pass ? reinterpret_cast<const cx<T>*>(seed) : phi[cur], dual ? dphi[cur] : nullptr, phi[1 - cur],
+					dual ? dphi[1 - cur] : nullptr, tp.args[p], mats, gterms, B, ntiles, flags, mpart, ttr, skip);
+			}
+			if (seed_pass) {
+				LaunchTimer lt(s, RQC_KERNEL_OTHER);
+				k_reduce_partials<<<1, kBlock, 0, s.stream>>>(reinterpret_cast<const zc*>(ttr),
+					(int)(tp.grev * kTileWaves), 1, d_acc, skip);
 			}
 			{
 				LaunchTimer lt(s, RQC_KERNEL_OTHER);

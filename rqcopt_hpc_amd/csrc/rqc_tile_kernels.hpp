@@ -699,6 +699,10 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 // reverse pass.  Inputs: psi (and dpsi) at the pass output (checkpoint), phi
 // (and dphi) at the pass output; outputs phi (and dphi) at the pass input.
 // flags: 4 = dphi at the pass output is zero, 8 = do not write phi at the pass input.
+// SEED (FP64 dual sweeps, the circuit's last pass): phi_out is the target U's column block;
+// the seed -conj(U) is formed in LDS and the trace sum(-conj(U) psi) goes to
+// trace[2 (block * waves + wave)] (a separate instantiation, so that the other passes run
+// exactly the kernel without it: the extra loop code measured 8 % slower on every pass).
 // partial[((part * ng + j) * 256) + row * 16 + col], part = block * waves + wave:
 // the 16 x 16 outer-product sums of gate j (rows: component 4 (2 s + p) + r of
 // the adjoint at the gate input, columns: that of the state before the gate).
@@ -714,13 +718,13 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 template <typename T, bool DUAL> constexpr int rev_buffers() { return DUAL ? 1 : 2; }
 template <typename T, bool DUAL> constexpr int rev_waves_per_simd() { return 2; }
 
-template <typename T, bool DUAL, int LOGE = 0>
+template <typename T, bool DUAL, int LOGE = 0, bool SEED = false>
 __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void k_tile_reverse(
 	const cx<T>* __restrict__ psi_out, const cx<T>* __restrict__ dpsi_out,
 	const cx<T>* __restrict__ phi_out, const cx<T>* __restrict__ dphi_out,
 	cx<T>* __restrict__ phi_in, cx<T>* __restrict__ dphi_in,
 	const TileArgs P, const cx<T>* __restrict__ mats, const GateTerms* __restrict__ gterms, long B, long ntiles, int flags,
-	double* __restrict__ partial, const int* skip)
+	double* __restrict__ partial, double* __restrict__ trace, const int* skip)
 {
 	if (skipped(skip)) { return; }
 	extern __shared__ __align__(16) char lds_raw[];
@@ -1016,14 +1020,51 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 			store(tile - gridDim.x, b ^ 1);
 		}
 	} else {
+		// SEED: phi_out holds the target U's tile; the adjoint seed is -conj(U), formed in
+		// place, and the target's trace sum(-conj(U) psi) is taken over the tile on the way
+		// (per thread, then per wave)
+		static_assert(!SEED || (sizeof(T) == 8 && DUAL), "the seed pass is an FP64 dual sweep");
+		double tr_re = 0, tr_im = 0;
 		for (; tile < ntiles; tile += gridDim.x) {
 			__builtin_amdgcn_s_waitcnt(0);
 			__syncthreads();
+			if constexpr (SEED) {
+				{
+#pragma unroll
+					for (int k = 0; k < kMaxSlots; k++) {
+						if (k < io.nk) {
+							const int si = threadIdx.x + k * kTileThreads;   // one FP64 element per slot
+							cx<T>* ph = state(0, NS / 2) + si;
+							const cx<T> u = *ph;
+							const cx<T> ps = state(0, 0)[si];
+							const cx<T> sd = {-u.x, u.y};
+							*ph = sd;
+							tr_re += sd.x * ps.x - sd.y * ps.y;
+							tr_im += sd.x * ps.y + sd.y * ps.x;
+						}
+					}
+					__syncthreads();
+				}
+			}
 			run_tile(0);
 			__syncthreads();
 			// each thread stores the slots that it then refills
 			store(tile, 0);
 			if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, 0); }
+		}
+		if constexpr (SEED) {
+			{
+#pragma unroll
+				for (int o = 32; o > 0; o >>= 1) {
+					tr_re += __shfl_xor(tr_re, o);
+					tr_im += __shfl_xor(tr_im, o);
+				}
+				if (lane == 0) {
+					const long w = (long)blockIdx.x * kTileWaves + (threadIdx.x >> 6);
+					trace[2 * w] = tr_re;
+					trace[2 * w + 1] = tr_im;
+				}
+			}
 		}
 	}
 	const long part = (long)blockIdx.x * kTileWaves + (threadIdx.x >> 6);
