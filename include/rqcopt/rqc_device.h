@@ -111,6 +111,11 @@ int rqc_profile_query(int kind, double* total_ms, long* launches, double* alg_by
    calibrate HBM byte counters and the roofline peak on the running device. */
 int rqc_measure_copy_bandwidth(size_t bytes, int reps, double* gbps);
 
+/* the same measurement with a copy kernel tuned for HBM bandwidth (four 16-byte elements per
+   thread in flight, non-temporal loads and stores): the device's streaming ceiling, against
+   which the tile passes' HBM rates are compared */
+int rqc_measure_stream_bandwidth(size_t bytes, int reps, double* gbps);
+
 /* No counterpart in the reference: the same evaluation as
    brickwall_unitary_target_projected_hessian_vector_product (target.h), for callers that
    cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses

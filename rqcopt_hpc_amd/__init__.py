@@ -129,7 +129,7 @@ EXPORTED = [
     "rqc_target_destroy", "rqc_target_nqubits", "rqc_target_download", "rqc_set_precision", "rqc_get_precision",
     "rqc_set_workspace_limit", "rqc_release_workspace", "rqc_last_column_block", "rqc_profile_enable",
     "rqc_profile_reset", "rqc_profile_query", "rqc_synchronize", "rqc_device_info", "rqc_measure_copy_bandwidth",
-    "rqc_set_sweep_engine", "rqc_last_sweep", "rqc_target_create_circuit",
+    "rqc_measure_stream_bandwidth", "rqc_set_sweep_engine", "rqc_last_sweep", "rqc_target_create_circuit",
     "rqc_brickwall_unitary_target_projected_hvp_noalloc", "rqc_noalloc_scratch_bytes",
     # rqc_batch.h
     "rqc_batch_create", "rqc_batch_destroy", "rqc_batch_set_exec", "rqc_batch_evaluate",
@@ -1387,6 +1387,16 @@ def measure_copy_bandwidth(nbytes=1 << 30, reps=10):
     L.rqc_measure_copy_bandwidth.argtypes = [ct.c_size_t, ct.c_int, ct.POINTER(ct.c_double)]
     g = ct.c_double()
     _check("rqc_measure_copy_bandwidth", L.rqc_measure_copy_bandwidth(nbytes, reps, ct.byref(g)))
+    return g.value
+
+
+def measure_stream_bandwidth(nbytes=1 << 30, reps=10):
+    """GB/s of a device-to-device copy tuned for HBM bandwidth (read + write bytes): the
+    streaming ceiling of the device."""
+    L = lib()
+    L.rqc_measure_stream_bandwidth.argtypes = [ct.c_size_t, ct.c_int, ct.POINTER(ct.c_double)]
+    g = ct.c_double()
+    _check("rqc_measure_stream_bandwidth", L.rqc_measure_stream_bandwidth(nbytes, reps, ct.byref(g)))
     return g.value
 
 

@@ -766,7 +766,7 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 	tp->lds_fwd = (size_t)(dual ? fwd_buffers<T, true>() : fwd_buffers<T, false>()) * (dual ? 2 : 1) *
 		(Ef * sizeof(cx<T>) + kLdsPadBytes) + fwd_mats;
 	tp->lds_rev = (size_t)(dual ? rev_buffers<T, true>() : rev_buffers<T, false>()) * (dual ? 4 : 2) *
-		(E * sizeof(cx<T>) + lds_pad_rev<T>()) + rev_mats;
+		(E * sizeof(cx<T>) + lds_pad_rev<T>()) + rev_mats + kRevTraceBytes;
 	auto* kf = fwd_kernel<T>(dual, tp->tsf.K + tp->tsf.logC);
 	auto* kr = rev_kernel<T>(dual, ts.K + ts.logC);
 	allow_lds(kf, tp->lds_fwd);
@@ -3125,6 +3125,35 @@ extern "C" int rqc_measure_copy_bandwidth(size_t bytes, int reps, double* gbps)
 		k_copy16<<<grid, kBlock, 0, c.stream>>>(a, b, n);  // warm-up
 		RQC_HIP(hipEventRecord(e0, c.stream));
 		for (int r = 0; r < reps; r++) { k_copy16<<<grid, kBlock, 0, c.stream>>>(a, b, n); }
+		RQC_HIP(hipEventRecord(e1, c.stream));
+		RQC_HIP(hipEventSynchronize(e1));
+		float ms = 0;
+		RQC_HIP(hipEventElapsedTime(&ms, e0, e1));
+		(void)hipEventDestroy(e0);
+		(void)hipEventDestroy(e1);
+		*gbps = 2.0 * (double)n * 16 * reps / (ms * 1e-3) / 1e9;
+	});
+}
+
+extern "C" int rqc_measure_stream_bandwidth(size_t bytes, int reps, double* gbps)
+{
+	return target_call("rqc_measure_stream_bandwidth", [&] {
+		if (reps < 1 || bytes < 16) { throw Error(-3, "need reps >= 1 and bytes >= 16"); }
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		const long n = (long)(bytes / 16);
+		c.work.reserve(2 * (size_t)n * 16);
+		zc* a = c.work.as<zc>();
+		zc* b = a + n;
+		// 256 blocks per CU of four 4 KiB chunks each (measured best of 1-256 per CU)
+		const unsigned grid = (unsigned)std::max<long>(1, std::min<long>((n + 4 * kBlock - 1) / (4 * kBlock), 256L * c.cu_count));
+		k_fill_zero<<<nblocks(n), kBlock, 0, c.stream>>>(a, n, nullptr);
+		hipEvent_t e0, e1;
+		RQC_HIP(hipEventCreate(&e0));
+		RQC_HIP(hipEventCreate(&e1));
+		k_copy_stream<<<grid, kBlock, 0, c.stream>>>(a, b, n);  // warm-up
+		RQC_HIP(hipEventRecord(e0, c.stream));
+		for (int r = 0; r < reps; r++) { k_copy_stream<<<grid, kBlock, 0, c.stream>>>(a, b, n); }
 		RQC_HIP(hipEventRecord(e1, c.stream));
 		RQC_HIP(hipEventSynchronize(e1));
 		float ms = 0;

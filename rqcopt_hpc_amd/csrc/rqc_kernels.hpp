@@ -557,6 +557,33 @@ __global__ __launch_bounds__(kBlock) void k_copy16(const zc* __restrict__ src, z
 	for (long t = (long)blockIdx.x * kBlock + threadIdx.x; t < n; t += stride) { dst[t] = src[t]; }
 }
 
+// streaming copy tuned for HBM bandwidth: four 16-byte elements per thread in flight
+// (block-strided, so that each load instruction is coalesced), non-temporal loads and stores
+__global__ __launch_bounds__(kBlock) void k_copy_stream(const zc* __restrict__ src, zc* __restrict__ dst, long n)
+{
+	constexpr int U = 4;
+	const long step = (long)gridDim.x * kBlock * U;
+	for (long t0 = (long)blockIdx.x * kBlock * U + threadIdx.x; t0 < n; t0 += step) {
+		zc v[U];
+#pragma unroll
+		for (int k = 0; k < U; k++) {
+			const long i = t0 + (long)k * kBlock;
+			if (i < n) {
+				v[k].x = __builtin_nontemporal_load(&src[i].x);
+				v[k].y = __builtin_nontemporal_load(&src[i].y);
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < U; k++) {
+			const long i = t0 + (long)k * kBlock;
+			if (i < n) {
+				__builtin_nontemporal_store(v[k].x, &dst[i].x);
+				__builtin_nontemporal_store(v[k].y, &dst[i].y);
+			}
+		}
+	}
+}
+
 // copy column block X (n x B, column fastest) into columns [b0, b0 + B) of the row-major n x n U
 __global__ __launch_bounds__(kBlock) void k_store_columns(const zc* __restrict__ X, zc* __restrict__ U, long n, long B, long b0)
 {

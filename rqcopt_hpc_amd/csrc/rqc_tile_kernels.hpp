@@ -115,6 +115,8 @@ enum { kMatG = 0, kMatZ = 1, kMatGinv = 2, kMatW = 3, kMatGT = 4, kMatZT = 5 };
 constexpr int kMatPairOff = 20;
 template <bool DUAL> constexpr int fwd_mat_stride() { return DUAL ? 2 * kMatPairOff : 16; }
 template <bool DUAL> constexpr int rev_mat_stride() { return DUAL ? 4 * kMatPairOff : 2 * kMatPairOff; }
+// after the reverse pass's gate matrices: a running trace sum (two doubles) per wave
+constexpr int kRevTraceBytes = kTileWaves * 16;
 
 // A operand of step kk of the real 16 x 16 matrix with complex blocks
 // [[X, 0], [Y, X]] (Y = 0 without the tangent state), components m = 4 v + r,
@@ -1022,29 +1024,43 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 	} else {
 		// SEED: phi_out holds the target U's tile; the adjoint seed is -conj(U), formed in
 		// place, and the target's trace sum(-conj(U) psi) is taken over the tile on the way
-		// (per thread, then per wave)
+		// (per thread and wave, then added to the wave's running sum)
 		static_assert(!SEED || (sizeof(T) == 8 && DUAL), "the seed pass is an FP64 dual sweep");
-		double tr_re = 0, tr_im = 0;
+		// (the running trace sum of a wave sits in LDS after the gate matrices, so that no
+		// register is held across the gates for it)
+		double* tracc = reinterpret_cast<double*>(lds_raw + rev_buffers<T, DUAL>() * NS * sbytes +
+			kMaxTileGates * rev_mat_stride<DUAL>() * (int)sizeof(cx<T>)) + 2 * (threadIdx.x >> 6);
+		if constexpr (SEED) {
+			if (lane == 0) { tracc[0] = 0; tracc[1] = 0; }
+		}
 		for (; tile < ntiles; tile += gridDim.x) {
 			__builtin_amdgcn_s_waitcnt(0);
 			__syncthreads();
 			if constexpr (SEED) {
-				{
+				double tr_re = 0, tr_im = 0;
 #pragma unroll
-					for (int k = 0; k < kMaxSlots; k++) {
-						if (k < io.nk) {
-							const int si = threadIdx.x + k * kTileThreads;   // one FP64 element per slot
-							cx<T>* ph = state(0, NS / 2) + si;
-							const cx<T> u = *ph;
-							const cx<T> ps = state(0, 0)[si];
-							const cx<T> sd = {-u.x, u.y};
-							*ph = sd;
-							tr_re += sd.x * ps.x - sd.y * ps.y;
-							tr_im += sd.x * ps.y + sd.y * ps.x;
-						}
+				for (int k = 0; k < kMaxSlots; k++) {
+					if (k < io.nk) {
+						const int si = threadIdx.x + k * kTileThreads;   // one FP64 element per slot
+						cx<T>* ph = state(0, NS / 2) + si;
+						const cx<T> u = *ph;
+						const cx<T> ps = state(0, 0)[si];
+						const cx<T> sd = {-u.x, u.y};
+						*ph = sd;
+						tr_re += sd.x * ps.x - sd.y * ps.y;
+						tr_im += sd.x * ps.y + sd.y * ps.x;
 					}
-					__syncthreads();
 				}
+#pragma unroll
+				for (int o = 32; o > 0; o >>= 1) {
+					tr_re += __shfl_xor(tr_re, o);
+					tr_im += __shfl_xor(tr_im, o);
+				}
+				if (lane == 0) {
+					tracc[0] += tr_re;
+					tracc[1] += tr_im;
+				}
+				__syncthreads();
 			}
 			run_tile(0);
 			__syncthreads();
@@ -1053,17 +1069,10 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 			if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, 0); }
 		}
 		if constexpr (SEED) {
-			{
-#pragma unroll
-				for (int o = 32; o > 0; o >>= 1) {
-					tr_re += __shfl_xor(tr_re, o);
-					tr_im += __shfl_xor(tr_im, o);
-				}
-				if (lane == 0) {
-					const long w = (long)blockIdx.x * kTileWaves + (threadIdx.x >> 6);
-					trace[2 * w] = tr_re;
-					trace[2 * w + 1] = tr_im;
-				}
+			if (lane == 0) {
+				const long w = (long)blockIdx.x * kTileWaves + (threadIdx.x >> 6);
+				trace[2 * w] = tracc[0];
+				trace[2 * w + 1] = tracc[1];
 			}
 		}
 	}
