@@ -20,6 +20,7 @@
 #include <functional>
 #include <map>
 #include <string>
+#include <set>
 #include <vector>
 
 #include "rqc_internal.hpp"
@@ -372,15 +373,20 @@ struct Pass
 	std::vector<int> rb;     // tile row bits, ascending
 };
 
-// Greedy pass planning over the circuit's dependency order: gates on disjoint
-// wires commute, so a pass may take any gate whose predecessors on its two
-// wires are done, as long as its wires lie in the pass's K tile qubits.
-// Candidate tile sets: every window of K consecutive wires (cyclic) and the
-// wires of the ready gates in circuit order; the candidate that takes the
-// most gates (at most gmax) wins, ties to the earlier candidate.  Tiles are
-// padded with the lowest unused row bits.  Any topological order of the gates
-// gives the same states at each gate up to rounding, so the sweeps of the
-// passes compute the reference's per-gate quantities.
+// Pass planning over the circuit's dependency order: gates on disjoint wires
+// commute, so a pass may take any gate whose predecessors on its two wires are
+// done, as long as its wires lie in the pass's K tile qubits.  Candidate tile
+// sets for a pass: every window of K consecutive wires (cyclic) and the wires of
+// the ready gates in circuit order; a candidate takes its ready gates in circuit
+// order, at most gmax.  A beam search over these choices keeps, after each pass,
+// the kPlanBeam states with the most gates done (ties to the earlier state and
+// candidate), and the first plan to finish has the fewest passes found; width 1
+// is the greedy plan (the candidate that takes the most gates).  At L = 14,
+// depth 9, K = 6 and 8 gates the beam finds 9 passes where the greedy plan
+// needs 10 (its last passes take 3 and 1 gates).  Tiles are padded with the
+// lowest unused row bits.  Any topological order of the gates gives the same
+// states at each gate up to rounding, so the sweeps of the passes compute the
+// reference's per-gate quantities.
 static std::vector<Pass> plan_passes(const Circuit& C, int K, int gmax)
 {
 	const int L = C.L, G = C.ngates;
@@ -392,52 +398,96 @@ static std::vector<Pass> plan_passes(const Circuit& C, int K, int gmax)
 		last[C.wi[k]] = k;
 		last[C.wj[k]] = k;
 	}
-	std::vector<char> done(G, 0), taken(G, 0);
-	int ndone = 0;
-	auto ok = [&](int k) { return (prev_i[k] < 0 || done[prev_i[k]] || taken[prev_i[k]]) &&
-	                              (prev_j[k] < 0 || done[prev_j[k]] || taken[prev_j[k]]); };
-	auto take = [&](unsigned mask, std::vector<int>* out) {
-		std::fill(taken.begin(), taken.end(), 0);
-		int n = 0;
-		for (int k = 0; k < G && n < gmax; k++) {
-			if (done[k] || (bitmask(k) & ~mask) != 0 || !ok(k)) { continue; }
-			taken[k] = 1;
-			n++;
-			if (out != nullptr) { out->push_back(k); }
-		}
-		return n;
+	const int beam_width = (G <= 256) ? 32 : (G <= 2048) ? 4 : 1;
+	std::vector<char> taken(G, 0);
+	auto ok = [&](const std::vector<char>& done, int k) {
+		return (prev_i[k] < 0 || done[prev_i[k]] || taken[prev_i[k]]) &&
+		       (prev_j[k] < 0 || done[prev_j[k]] || taken[prev_j[k]]);
 	};
-	std::vector<Pass> out;
-	while (ndone < G) {
+	// the ready gates of the tile set `mask`, in circuit order, at most gmax
+	auto take = [&](const std::vector<char>& done, unsigned mask, std::vector<int>& out) {
+		std::fill(taken.begin(), taken.end(), 0);
+		out.clear();
+		for (int k = 0; k < G && (int)out.size() < gmax; k++) {
+			if (done[k] || (bitmask(k) & ~mask) != 0 || !ok(done, k)) { continue; }
+			taken[k] = 1;
+			out.push_back(k);
+		}
+		std::fill(taken.begin(), taken.end(), 0);
+	};
+	auto candidates = [&](const std::vector<char>& done) {
 		std::vector<unsigned> cands;
 		if (K >= L) {
 			cands.push_back((L >= 32) ? ~0u : ((1u << L) - 1));
-		} else {
-			for (int s = 0; s < L; s++) {
-				unsigned m = 0;
-				for (int t = 0; t < K; t++) { m |= 1u << (L - 1 - (s + t) % L); }
-				cands.push_back(m);
-			}
-			std::fill(taken.begin(), taken.end(), 0);
+			return cands;
+		}
+		for (int s = 0; s < L; s++) {
 			unsigned m = 0;
-			for (int k = 0; k < G; k++) {
-				if (done[k] || !ok(k)) { continue; }
-				const unsigned m2 = m | bitmask(k);
-				if (__builtin_popcount(m2) <= K) { m = m2; }
-			}
+			for (int t = 0; t < K; t++) { m |= 1u << (L - 1 - (s + t) % L); }
 			cands.push_back(m);
 		}
-		int best = -1, bestn = 0;
-		for (size_t ci = 0; ci < cands.size(); ci++) {
-			const int n = take(cands[ci], nullptr);
-			if (n > bestn) { bestn = n; best = (int)ci; }
+		unsigned m = 0;
+		for (int k = 0; k < G; k++) {
+			if (done[k] || !ok(done, k)) { continue; }
+			const unsigned m2 = m | bitmask(k);
+			if (__builtin_popcount(m2) <= K) { m = m2; }
 		}
-		if (best < 0) { throw Error(-4, "pass planning made no progress"); }
+		cands.push_back(m);
+		return cands;
+	};
+	struct State
+	{
+		std::vector<char> done;
+		int ndone;
+		std::vector<std::vector<int>> passes;
+	};
+	std::vector<State> beam(1);
+	beam[0].done.assign(G, 0);
+	beam[0].ndone = 0;
+	std::vector<std::vector<int>> plan;
+	if (G == 0) { beam.clear(); }
+	std::vector<int> lst;
+	while (!beam.empty()) {
+		std::vector<State> next;
+		std::set<std::vector<char>> seen;
+		bool finished = false;
+		for (const State& st : beam) {
+			// expansions in candidate order, the one taking the most gates first
+			std::vector<std::pair<int, std::vector<int>>> exps;
+			for (unsigned m : candidates(st.done)) {
+				take(st.done, m, lst);
+				if (!lst.empty()) { exps.emplace_back((int)lst.size(), lst); }
+			}
+			std::stable_sort(exps.begin(), exps.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+			for (auto& e : exps) {
+				State t;
+				t.done = st.done;
+				for (int k : e.second) { t.done[k] = 1; }
+				if (!seen.insert(t.done).second) { continue; }
+				t.ndone = st.ndone + e.first;
+				t.passes = st.passes;
+				t.passes.push_back(std::move(e.second));
+				if (t.ndone == G) {
+					plan = std::move(t.passes);
+					finished = true;
+					break;
+				}
+				next.push_back(std::move(t));
+			}
+			if (finished) { break; }
+		}
+		if (finished) { break; }
+		if (next.empty()) { throw Error(-4, "pass planning made no progress"); }
+		std::stable_sort(next.begin(), next.end(), [](const State& a, const State& b) { return a.ndone > b.ndone; });
+		if ((int)next.size() > beam_width) { next.resize(beam_width); }
+		beam = std::move(next);
+	}
+	std::vector<Pass> out;
+	for (auto& gl : plan) {
 		Pass p;
-		take(cands[best], &p.gates);
+		p.gates = gl;
 		unsigned used = 0;
-		for (int k : p.gates) { done[k] = 1; ndone++; used |= bitmask(k); }
-		std::fill(taken.begin(), taken.end(), 0);
+		for (int k : p.gates) { used |= bitmask(k); }
 		for (int b = 0; b < L; b++) {
 			if (used & (1u << b)) { p.rb.push_back(b); }
 		}
