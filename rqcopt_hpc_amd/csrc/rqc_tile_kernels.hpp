@@ -374,6 +374,147 @@ __device__ __forceinline__ void gate_store_t(cx<T>* X, cx<T>* Y, const int* eg, 
 	}
 }
 
+// ---------------------------------------------------------------------------
+// FP64 gate applications on 4 x 4 blocks.  The 4x4x4 FP64 matrix-core
+// instruction computes four independent 4 x 4 x 4 products, block b from lanes
+// 16 k + 4 b + i (A row i, column k), 16 k + 4 b + j (B row k, column j) into
+// lane 16 i + 4 b + j, at the 16x16x4 instruction's rate per product.  A
+// complex 4 x 4 gate on the states' real and imaginary parts is then 4 such
+// products, and the dual matrix [[X, 0], [Y, X]] 12, where the real 16 x 16 form
+// costs the time of 16 (its zero block included); the outer products likewise
+// need 12 instead of 16.  Signs come from the instruction's operand negation.
+__device__ __forceinline__ double mfma44(double a, double b, double c)
+{
+	return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+}
+
+// -a b + c (the last argument of the FP64 instruction negates its operands: 1 = A)
+__device__ __forceinline__ double mfma44_na(double a, double b, double c)
+{
+	return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 1);
+}
+
+// Entry (l & 3, l >> 4) of a block pair [[X, 0], [Y, X]] held by lane l (the
+// pair's matrices in the transposed LDS layout, entry (r, c) at 4 c + r): the A
+// operand of the direct application and the B operand of the transposed one.
+struct Frag44
+{
+	double xr, xi, yr, yi;
+};
+
+__device__ __forceinline__ Frag44 frag44(const cx<double>* m)
+{
+	const int lane = threadIdx.x & 63;
+	const int fi = 4 * (lane >> 4) + (lane & 3);
+	const cx<double> x = m[fi], y = m[fi + kMatPairOff];
+	return Frag44{x.x, x.y, y.x, y.y};
+}
+
+// Direct application to GS groups of 16 quads in the gate layout (lane l:
+// amplitude l >> 4 of quad l & 15; x = primary, y = tangent state): the results
+// land in the same lanes, acc[n] = (x.re, x.im, y.re, y.im).
+template <int GS>
+__device__ __forceinline__ void gate_mfma44(const Frag44& f, GateOperands<double, GS>& o)
+{
+	double d[GS][4];
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][0] = mfma44(f.xr, o.x[n].x, 0.0);
+		d[n][1] = mfma44(f.xi, o.x[n].x, 0.0);
+		d[n][2] = mfma44(f.yr, o.x[n].x, 0.0);
+		d[n][3] = mfma44(f.yi, o.x[n].x, 0.0);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][0] = mfma44_na(f.xi, o.x[n].y, d[n][0]);
+		d[n][1] = mfma44(f.xr, o.x[n].y, d[n][1]);
+		d[n][2] = mfma44_na(f.yi, o.x[n].y, d[n][2]);
+		d[n][3] = mfma44(f.yr, o.x[n].y, d[n][3]);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][2] = mfma44(f.xr, o.y[n].x, d[n][2]);
+		d[n][3] = mfma44(f.xi, o.y[n].x, d[n][3]);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][2] = mfma44_na(f.xi, o.y[n].y, d[n][2]);
+		d[n][3] = mfma44(f.xr, o.y[n].y, d[n][3]);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) { o.acc[n] = double4_t{d[n][0], d[n][1], d[n][2], d[n][3]}; }
+}
+
+// Transposed application (the state as the A operand, the gate as B): the
+// results of lane l are amplitude l & 3 of quad 4 ((l >> 2) & 3) + (l >> 4) of
+// the group, which is the operand layout of the outer products below.
+template <int GS>
+__device__ __forceinline__ void gate_mfma44_t(const Frag44& f, GateOperands<double, GS>& o)
+{
+	double d[GS][4];
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][0] = mfma44(o.x[n].x, f.xr, 0.0);
+		d[n][1] = mfma44(o.x[n].x, f.xi, 0.0);
+		d[n][2] = mfma44(o.x[n].x, f.yr, 0.0);
+		d[n][3] = mfma44(o.x[n].x, f.yi, 0.0);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][0] = mfma44_na(o.x[n].y, f.xi, d[n][0]);
+		d[n][1] = mfma44(o.x[n].y, f.xr, d[n][1]);
+		d[n][2] = mfma44_na(o.x[n].y, f.yi, d[n][2]);
+		d[n][3] = mfma44(o.x[n].y, f.yr, d[n][3]);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][2] = mfma44(o.y[n].x, f.xr, d[n][2]);
+		d[n][3] = mfma44(o.y[n].x, f.xi, d[n][3]);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][2] = mfma44_na(o.y[n].y, f.xi, d[n][2]);
+		d[n][3] = mfma44(o.y[n].y, f.xr, d[n][3]);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) { o.acc[n] = double4_t{d[n][0], d[n][1], d[n][2], d[n][3]}; }
+}
+
+// Outer-product sums of GS groups' transposed results (a: adjoint, s: state)
+// onto acc = (A1.re, A1.im, A2.re, A2.im), A1 = sum a (x) s and
+// A2 = sum a (x) ds + da (x) s (complex, no conjugation): lane 16 r + 4 b + c
+// holds entry (r, c) summed over the quads of block b.
+template <int GS>
+__device__ __forceinline__ void outer44(const GateOperands<double, GS>& s, const GateOperands<double, GS>& a,
+	double4_t& acc)
+{
+	double c0 = acc[0], c1 = acc[1], c2 = acc[2], c3 = acc[3];
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		c0 = mfma44(a.acc[n][0], s.acc[n][0], c0);
+		c1 = mfma44(a.acc[n][0], s.acc[n][1], c1);
+		c2 = mfma44(a.acc[n][0], s.acc[n][2], c2);
+		c3 = mfma44(a.acc[n][0], s.acc[n][3], c3);
+		c0 = mfma44_na(a.acc[n][1], s.acc[n][1], c0);
+		c1 = mfma44(a.acc[n][1], s.acc[n][0], c1);
+		c2 = mfma44_na(a.acc[n][1], s.acc[n][3], c2);
+		c3 = mfma44(a.acc[n][1], s.acc[n][2], c3);
+		c2 = mfma44(a.acc[n][2], s.acc[n][0], c2);
+		c3 = mfma44(a.acc[n][2], s.acc[n][1], c3);
+		c2 = mfma44_na(a.acc[n][3], s.acc[n][1], c2);
+		c3 = mfma44(a.acc[n][3], s.acc[n][0], c3);
+	}
+	acc = double4_t{c0, c1, c2, c3};
+}
+
+// LDS position term of lane l's transposed result (amplitude l & 3 of group
+// quad 4 ((l >> 2) & 3) + (l >> 4))
+__device__ __forceinline__ int transposed44_lane_term(const GateTerms& t)
+{
+	const int lane = threadIdx.x & 63;
+	return sel4_affine(t.q4, lane >> 4) ^ sel4_affine(t.w, (lane >> 2) & 3) ^ sel4_affine(t.u, lane & 3);
+}
+
 // all loads are issued before the matrix-core chains and all stores after, so
 // the groups' chains overlap
 template <typename T, int GS>
@@ -588,9 +729,16 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 		auto gate = [&](auto jc) {
 			constexpr int j = decltype(jc)::value;
 			const T* mj = reinterpret_cast<const T*>(lm + j * fwd_mat_stride<DUAL>());
+			// FP64 dual sweeps: 4 x 4 block products (F44); else the real 16 x 16 form
+			constexpr bool F44 = FIXED && sizeof(T) == 8;
 			T a[4];
+			Frag44 f44;
+			if constexpr (F44) {
+				f44 = frag44(reinterpret_cast<const cx<double>*>(mj));
+			} else {
 #pragma unroll
-			for (int kk = 0; kk < 4; kk++) { a[kk] = mj[amap.aoff[kk]] * amap.afac[kk]; }
+				for (int kk = 0; kk < 4; kk++) { a[kk] = mj[amap.aoff[kk]] * amap.afac[kk]; }
+			}
 			if constexpr (FIXED) {
 				// psi and dpsi of a group at one byte offset (the lane's XOR the group's), the
 				// states' bases as instruction offsets (b = 0: one buffer).  Software-pipelined
@@ -624,7 +772,8 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 					// (a scheduling fence: those loads issue before the chain)
 					__builtin_amdgcn_sched_barrier(0);
 					__builtin_amdgcn_s_setprio(1);
-					gate_mfma<T, GS>(a, o[SI]);
+					if constexpr (F44) { gate_mfma44<GS>(f44, o[SI]); }
+					else { gate_mfma<T, GS>(a, o[SI]); }
 					__builtin_amdgcn_s_setprio(0);
 #pragma unroll
 					for (int n = 0; n < GS; n++) {
@@ -791,6 +940,9 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 	// for the held terms, measured 3 % slower with them.)
 	constexpr int ES = (int)sizeof(cx<T>);
 	constexpr bool FIXED = DUAL && LOGE > 0 && sizeof(T) == 4;
+	// FP64 dual sweeps with a planned tile: 4 x 4 block products (gate_mfma44_t, outer44);
+	// acc[J] then holds (A1.re, A1.im, A2.re, A2.im) of the lane's block entry
+	constexpr bool F44 = DUAL && LOGE > 0 && sizeof(T) == 8;
 	int eabj[kMaxTileGates], btbj[kMaxTileGates];
 	int sgj[kMaxTileGates][8];
 #pragma unroll
@@ -827,10 +979,16 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 			// recovery: [[G^-1, 0], [-G^-1 Z G^-1, G^-1]]; adjoint: [[G^T, 0], [Z^T, G^T]]
 			const T* mj = reinterpret_cast<const T*>(lm + J * rev_mat_stride<DUAL>());
 			T au[4], aa[4];
+			Frag44 fu, fa;
+			if constexpr (F44) {
+				fu = frag44(reinterpret_cast<const cx<double>*>(mj));
+				fa = frag44(reinterpret_cast<const cx<double>*>(mj) + 2 * kMatPairOff);
+			} else {
 #pragma unroll
-			for (int kk = 0; kk < 4; kk++) {
-				au[kk] = mj[amap.aoff[kk]] * amap.afac[kk];
-				aa[kk] = mj[amap.aoff[kk] + 2 * kMatPairOff * (DUAL ? 2 : 1)] * amap.afac[kk];
+				for (int kk = 0; kk < 4; kk++) {
+					au[kk] = mj[amap.aoff[kk]] * amap.afac[kk];
+					aa[kk] = mj[amap.aoff[kk] + 2 * kMatPairOff * (DUAL ? 2 : 1)] * amap.afac[kk];
+				}
 			}
 			const bool adjoint = (J > 0) || !(flags & 8);
 			// the outer products of a step's transposed results: phi_in (x) psi,
@@ -881,7 +1039,8 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 					} else {
 						const GateTerms gt = gterms[P.x0 + J];
 						eab = gate_lanes<T>(gt).ea * ES;
-						btb = (transposed_lane_term<T>(gt) * ES) | ((sc_lane & 1) * (int)sizeof(T));
+						btb = F44 ? transposed44_lane_term(gt) * ES
+						          : (transposed_lane_term<T>(gt) * ES) | ((sc_lane & 1) * (int)sizeof(T));
 						sg[0] = sel4(gt.wv, wave) * ES;
 						sg[1] = gt.gbit[0] * ES;
 						sg[2] = gt.gbit[1] * ES;
@@ -907,6 +1066,33 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 						// matrix-core work at raised wave priority, so that the other wave's
 						// LDS traffic issues in its shadow
 						__builtin_amdgcn_s_setprio(1);
+						if constexpr (F44) {
+							// the transposed results of lane l: one amplitude of each state
+							gate_mfma44_t<GSX>(fu, os[S]);
+							gate_mfma44_t<GSX>(fa, oa[S]);
+							int st[GSX];
+#pragma unroll
+							for (int n = 0; n < GSX; n++) {
+								st[n] = btb ^ egb[S][n];
+								*reinterpret_cast<cx<double>*>(lds_raw + st[n]) = cx<double>{os[S].acc[n][0], os[S].acc[n][1]};
+								*reinterpret_cast<cx<double>*>(lds_raw + st[n] + sbytes) =
+									cx<double>{os[S].acc[n][2], os[S].acc[n][3]};
+							}
+							if (adjoint) {
+#pragma unroll
+								for (int n = 0; n < GSX; n++) {
+									*reinterpret_cast<cx<double>*>(lds_raw + st[n] + 2 * sbytes) =
+										cx<double>{oa[S].acc[n][0], oa[S].acc[n][1]};
+									*reinterpret_cast<cx<double>*>(lds_raw + st[n] + 3 * sbytes) =
+										cx<double>{oa[S].acc[n][2], oa[S].acc[n][3]};
+								}
+							}
+							if constexpr (S + 1 < NST) { load_step(std::integral_constant<int, S + 1>{}); }
+							__builtin_amdgcn_sched_barrier(0);
+							outer44<GSX>(os[S], oa[S], acc[J]);
+							__builtin_amdgcn_s_setprio(0);
+							return;
+						}
 						gate_mfma_t<T, GSX>(au, os[S]);
 						gate_mfma_t<T, GSX>(aa, oa[S]);
 						// component l & 15 of the transposed results: state psi / dpsi (phi /
@@ -1081,8 +1267,25 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 	for (int j = 0; j < kMaxTileGates; j++) {
 		if (j < P.ng) {
 			double* d = partial + (part * P.ng + j) * 256;
+			if constexpr (F44) {
+				// the four blocks' sums (lane bits 2, 3) into the 16 x 16 layout: A1 and A2 as
+				// the (re, re), (re, im) blocks of phi (x) psi and phi (x) dpsi, zeros elsewhere
+				double4_t v = acc[j];
 #pragma unroll
-			for (int v = 0; v < 4; v++) { d[mfma_row<T>(lane, v) * 16 + (lane & 15)] = acc[j][v]; }
+				for (int q = 0; q < 4; q++) {
+					v[q] += __shfl_xor(v[q], 4);
+					v[q] += __shfl_xor(v[q], 8);
+				}
+				if (((lane >> 2) & 3) == 0) {
+#pragma unroll
+					for (int q = 0; q < 4; q++) { d[(lane >> 4) * 16 + 4 * q + (lane & 3)] = v[q]; }
+				}
+#pragma unroll
+				for (int z = 1; z < 4; z++) { d[64 * z + lane] = 0.0; }
+			} else {
+#pragma unroll
+				for (int v = 0; v < 4; v++) { d[mfma_row<T>(lane, v) * 16 + (lane & 15)] = acc[j][v]; }
+			}
 		}
 	}
 }
