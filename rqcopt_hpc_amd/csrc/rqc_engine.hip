@@ -618,6 +618,11 @@ static void allow_lds(KernelT* fn, size_t bytes)
 	if (bytes > 65536) {
 		RQC_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 	}
+	// the tile kernels address their dynamic LDS by offset from 0, which holds while they
+	// declare no static LDS
+	hipFuncAttributes fa;
+	RQC_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(fn)));
+	if (fa.sharedSizeBytes != 0) { throw Error(-4, "tile kernel with static LDS"); }
 }
 
 // gates whose inverse is their adjoint up to 1e-6 (the reverse tile pass

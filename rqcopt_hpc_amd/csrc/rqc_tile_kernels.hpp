@@ -515,6 +515,25 @@ __device__ __forceinline__ int transposed44_lane_term(const GateTerms& t)
 	return sel4_affine(t.q4, lane >> 4) ^ sel4_affine(t.w, (lane >> 2) & 3) ^ sel4_affine(t.u, lane & 3);
 }
 
+// LDS accesses by byte offset into the kernel's dynamic LDS.  The tile kernels
+// declare no static LDS, so their dynamic LDS starts at offset 0 (the host
+// checks this when it plans the passes) and an offset is the LDS address: the
+// state strides fold into the instructions' offset fields and no base is added.
+// (a 16-byte vector type, so that each access is one 16-byte instruction)
+typedef double lds_d2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) lds_d2 lds_d2_as3;
+
+__device__ __forceinline__ cx<double> lds_load_cxd(unsigned off)
+{
+	const lds_d2 v = *reinterpret_cast<const lds_d2_as3*>(off);
+	return cx<double>{v.x, v.y};
+}
+
+__device__ __forceinline__ void lds_store_cxd(unsigned off, cx<double> v)
+{
+	*reinterpret_cast<lds_d2_as3*>(off) = lds_d2{v.x, v.y};
+}
+
 // all loads are issued before the matrix-core chains and all stores after, so
 // the groups' chains overlap
 template <typename T, int GS>
@@ -932,16 +951,16 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 		tile_store(phi_in, state(b, NS / 2), base, io);
 		if (DUAL) { tile_store(dphi_in, state(b, 3), base, io); }
 	};
-	// FP32 dual sweeps with a planned tile: the gates' position terms are fixed over
-	// the launch's tiles, so they are formed once: the lane's load term (eabj) and
-	// transposed-store term (btbj, with the re / im byte of its component), and
-	// the wave's terms (sgj: the wave's range, the group bits, then the result
-	// registers' terms).  (FP64 forms them per gate: its accumulators leave no room
-	// for the held terms, measured 3 % slower with them.)
+	// Dual sweeps with a planned tile: the gates' position terms are fixed over the
+	// launch's tiles, so they are formed once: the lane's load term (eabj) and
+	// transposed-store term (btbj; FP32: with the re / im byte of its component), and
+	// the wave's terms (sgj: the wave's range, the group bits, then FP32's result
+	// registers' terms).
 	constexpr int ES = (int)sizeof(cx<T>);
-	constexpr bool FIXED = DUAL && LOGE > 0 && sizeof(T) == 4;
+	constexpr bool FIXED = DUAL && LOGE > 0;
 	// FP64 dual sweeps with a planned tile: 4 x 4 block products (gate_mfma44_t, outer44);
-	// acc[J] then holds (A1.re, A1.im, A2.re, A2.im) of the lane's block entry
+	// acc[J] then holds (A1.re, A1.im, A2.re, A2.im) of the lane's block entry, btbj the
+	// lane's term of its transposed results (one amplitude of each state)
 	constexpr bool F44 = DUAL && LOGE > 0 && sizeof(T) == 8;
 	int eabj[kMaxTileGates], btbj[kMaxTileGates];
 	int sgj[kMaxTileGates][8];
@@ -953,7 +972,8 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 		if (FIXED && j < P.ng) {
 			const GateTerms gt = gterms[P.x0 + j];
 			eabj[j] = gate_lanes<T>(gt).ea * ES;
-			btbj[j] = (transposed_lane_term<T>(gt) * ES) | ((sc_lane & 1) * (int)sizeof(T));
+			btbj[j] = F44 ? transposed44_lane_term(gt) * ES
+			              : (transposed_lane_term<T>(gt) * ES) | ((sc_lane & 1) * (int)sizeof(T));
 			sgj[j][0] = sel4(gt.wv, wave) * ES;
 			sgj[j][1] = gt.gbit[0] * ES;
 			sgj[j][2] = gt.gbit[1] * ES;
@@ -1030,24 +1050,14 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 					// compiler forms the groups' offsets per tile instead of holding every
 					// gate's offsets in registers across the tile loop)
 					int eab, btb, sg[8];
-					if constexpr (FIXED) {
-						eab = eabj[J];
-						btb = btbj[J];
-						asm volatile("" : "+v"(eab), "+v"(btb));
+					eab = eabj[J];
+					btb = btbj[J];
+					asm volatile("" : "+v"(eab), "+v"(btb));
+					// (16-byte element terms: whole-element LDS accesses with the state
+					// strides in the instructions' offset fields)
+					if constexpr (F44) { __builtin_assume(((eab | btb) & 15) == 0); }
 #pragma unroll
-						for (int i = 0; i < 8; i++) { sg[i] = sgj[J][i]; }
-					} else {
-						const GateTerms gt = gterms[P.x0 + J];
-						eab = gate_lanes<T>(gt).ea * ES;
-						btb = F44 ? transposed44_lane_term(gt) * ES
-						          : (transposed_lane_term<T>(gt) * ES) | ((sc_lane & 1) * (int)sizeof(T));
-						sg[0] = sel4(gt.wv, wave) * ES;
-						sg[1] = gt.gbit[0] * ES;
-						sg[2] = gt.gbit[1] * ES;
-						sg[3] = gt.gbit[2] * ES;
-#pragma unroll
-						for (int v = 0; v < 4; v++) { sg[4 + v] = transposed_reg_term<T>(gt, v) * ES; }
-					}
+					for (int i = 0; i < 8; i++) { sg[i] = sgj[J][i]; }
 					auto load_step = [&](auto sc) {
 						constexpr int S = decltype(sc)::value;
 #pragma unroll
@@ -1055,10 +1065,17 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 							const int gi = (S * GSX * gq + n * gq) >> 4;   // group index in the wave's range
 							egb[S][n] = sg[0] ^ ((gi & 1) ? sg[1] : 0) ^ ((gi & 2) ? sg[2] : 0) ^ ((gi & 4) ? sg[3] : 0);
 							const int la = eab ^ egb[S][n];
-							os[S].x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la);
-							os[S].y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la + sbytes);
-							oa[S].x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la + 2 * sbytes);
-							oa[S].y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la + 3 * sbytes);
+							if constexpr (F44) {
+								os[S].x[n] = lds_load_cxd((unsigned)la);
+								os[S].y[n] = lds_load_cxd((unsigned)la + sbytes);
+								oa[S].x[n] = lds_load_cxd((unsigned)la + 2 * sbytes);
+								oa[S].y[n] = lds_load_cxd((unsigned)la + 3 * sbytes);
+							} else {
+								os[S].x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la);
+								os[S].y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la + sbytes);
+								oa[S].x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la + 2 * sbytes);
+								oa[S].y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la + 3 * sbytes);
+							}
 						}
 					};
 					auto step = [&](auto sc) {
@@ -1070,21 +1087,18 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 							// the transposed results of lane l: one amplitude of each state
 							gate_mfma44_t<GSX>(fu, os[S]);
 							gate_mfma44_t<GSX>(fa, oa[S]);
-							int st[GSX];
+							unsigned st[GSX];
 #pragma unroll
 							for (int n = 0; n < GSX; n++) {
-								st[n] = btb ^ egb[S][n];
-								*reinterpret_cast<cx<double>*>(lds_raw + st[n]) = cx<double>{os[S].acc[n][0], os[S].acc[n][1]};
-								*reinterpret_cast<cx<double>*>(lds_raw + st[n] + sbytes) =
-									cx<double>{os[S].acc[n][2], os[S].acc[n][3]};
+								st[n] = (unsigned)(btb ^ egb[S][n]);
+								lds_store_cxd(st[n], cx<double>{os[S].acc[n][0], os[S].acc[n][1]});
+								lds_store_cxd(st[n] + sbytes, cx<double>{os[S].acc[n][2], os[S].acc[n][3]});
 							}
 							if (adjoint) {
 #pragma unroll
 								for (int n = 0; n < GSX; n++) {
-									*reinterpret_cast<cx<double>*>(lds_raw + st[n] + 2 * sbytes) =
-										cx<double>{oa[S].acc[n][0], oa[S].acc[n][1]};
-									*reinterpret_cast<cx<double>*>(lds_raw + st[n] + 3 * sbytes) =
-										cx<double>{oa[S].acc[n][2], oa[S].acc[n][3]};
+									lds_store_cxd(st[n] + 2 * sbytes, cx<double>{oa[S].acc[n][0], oa[S].acc[n][1]});
+									lds_store_cxd(st[n] + 3 * sbytes, cx<double>{oa[S].acc[n][2], oa[S].acc[n][3]});
 								}
 							}
 							if constexpr (S + 1 < NST) { load_step(std::integral_constant<int, S + 1>{}); }
