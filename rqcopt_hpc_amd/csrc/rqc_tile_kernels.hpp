@@ -1087,12 +1087,16 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 							// the transposed results of lane l: one amplitude of each state
 							gate_mfma44_t<GSX>(fu, os[S]);
 							gate_mfma44_t<GSX>(fa, oa[S]);
+							// (the states recovered by the pass's first gate, J = 0, are its
+							// outer products' operands only: no gate reads them from LDS)
 							unsigned st[GSX];
 #pragma unroll
 							for (int n = 0; n < GSX; n++) {
 								st[n] = (unsigned)(btb ^ egb[S][n]);
-								lds_store_cxd(st[n], cx<double>{os[S].acc[n][0], os[S].acc[n][1]});
-								lds_store_cxd(st[n] + sbytes, cx<double>{os[S].acc[n][2], os[S].acc[n][3]});
+								if constexpr (J > 0) {
+									lds_store_cxd(st[n], cx<double>{os[S].acc[n][0], os[S].acc[n][1]});
+									lds_store_cxd(st[n] + sbytes, cx<double>{os[S].acc[n][2], os[S].acc[n][3]});
+								}
 							}
 							if (adjoint) {
 #pragma unroll
