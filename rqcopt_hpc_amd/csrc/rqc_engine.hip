@@ -705,6 +705,7 @@ struct TilePlan
 	std::vector<int> order;         // circuit gate index of each execution slot
 	unsigned gfwd = 0, grev = 0;
 	size_t lds_fwd = 0, lds_rev = 0;
+	size_t lds_rev_seed = 0;        // the seed pass: also the fused forward pass's G, Z
 	size_t partial_doubles = 0;
 };
 
@@ -826,7 +827,8 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 	auto* kr = rev_kernel<T>(dual, ts.K + ts.logC);
 	allow_lds(kf, tp->lds_fwd);
 	allow_lds(kr, tp->lds_rev);
-	allow_lds(rev_kernel<T>(dual, ts.K + ts.logC, true), tp->lds_rev);
+	tp->lds_rev_seed = tp->lds_rev + (size_t)kMaxTileGates * fwd_mat_stride<true>() * sizeof(cx<T>);
+	allow_lds(rev_kernel<T>(dual, ts.K + ts.logC, true), tp->lds_rev_seed);
 	int occ_f = 0, occ_r = 0;
 	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_f, reinterpret_cast<const void*>(kf), kTileThreads, tp->lds_fwd));
 	RQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_r, reinterpret_cast<const void*>(kr), kTileThreads, tp->lds_rev));
@@ -1079,13 +1081,19 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	// trace on the way (no separate target pass over U, psi and the seed)
 	const bool fuse_trace = sizeof(T) == 8 && dual && be == nullptr && !pl.sampled && B == N && P >= 1;
 	double* ttr = cpart + (size_t)kReduceChunks * kMaxTileGates * 256;   // its per-wave trace partials
+	// and, with the planned FP64 tiles (one tile for both directions), it also runs the circuit's
+	// last forward pass on its tiles first: the circuit's output states never go through HBM
+	const int logEr = tp.ts.K + tp.ts.logC;
+	const bool fuse_fwd = fuse_trace && P >= 2 && tp.tsf.K == tp.ts.K && tp.tsf.logC == tp.ts.logC &&
+		(logEr == 9 || logEr == 10);
+	const int Pf = fuse_fwd ? P - 1 : P;   // forward launches
 
 	for (long blk = 0; blk < pl.nblk; blk++) {
 		const long b0 = blk * B;
 		// the block's input columns and the trace's seed columns: basis columns and U, or samples and U X
 		const zc* seed = (be != nullptr) ? nullptr : pl.sampled ? W + (size_t)blk * sz : dU;
 		const long ldu = pl.sampled ? B : N, u0 = pl.sampled ? 0 : b0;
-		for (int p = 0; p < P; p++) {
+		for (int p = 0; p < Pf; p++) {
 			// pass 0 starts from the identity block (flags 1) or the sample block, with dpsi = 0 (flags 2)
 			const int flags = (p == 0) ? (pl.sampled ? 2 : 3) : 0;
 			const cx<T>* in = (p == 0) ? (pl.sampled ? X + (size_t)blk * sz : nullptr)
@@ -1117,11 +1125,14 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 			if (p == P - 1) { flags |= 4; }
 			if (p == 0) { flags |= 8; }
 			const bool seed_pass = fuse_trace && p == P - 1;
+			const bool with_fwd = seed_pass && fuse_fwd;   // psi, dpsi at the pass input (flags 16)
+			if (with_fwd) { flags |= 16; }
 			const int ng = (int)tp.passes[p].gates.size();
 			{
 				const int reads = dual ? 3 + (p < P - 1) : 2;
 				LaunchTimer lt(s, RQC_KERNEL_REVERSE, (double)(reads + ns * (p > 0)) * ab);
-				(seed_pass ? krs : kr)<<<tp.grev, kTileThreads, tp.lds_rev, s.stream>>>(PS(p + 1), dual ? DPS(p + 1) : nullptr,
+				(seed_pass ? krs : kr)<<<tp.grev, kTileThreads, seed_pass ? tp.lds_rev_seed : tp.lds_rev, s.stream>>>(
+					with_fwd ? PS(p) : PS(p + 1), dual ? (with_fwd ? DPS(p) : DPS(p + 1)) : nullptr,
 					seed_pass ? reinterpret_cast<const cx<T>*>(seed) : phi[cur], dual ? dphi[cur] : nullptr, phi[1 - cur],
 					dual ? dphi[1 - cur] : nullptr, tp.args[p], mats, gterms, B, ntiles, flags, mpart, ttr, skip);
 			}

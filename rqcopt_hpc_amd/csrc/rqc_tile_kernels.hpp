@@ -868,7 +868,9 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 // ---------------------------------------------------------------------------
 // reverse pass.  Inputs: psi (and dpsi) at the pass output (checkpoint), phi
 // (and dphi) at the pass output; outputs phi (and dphi) at the pass input.
-// flags: 4 = dphi at the pass output is zero, 8 = do not write phi at the pass input.
+// flags: 4 = dphi at the pass output is zero, 8 = do not write phi at the pass input,
+// 16 (SEED, FP64 planned tiles) = psi and dpsi are given at the pass input and the pass's
+// gates are applied to them first (G, Z after the trace sums in LDS).
 // SEED (FP64 dual sweeps, the circuit's last pass): phi_out is the target U's column block;
 // the seed -conj(U) is formed in LDS and the trace sum(-conj(U) psi) goes to
 // trace[2 (block * waves + wave)] (a separate instantiation, so that the other passes run
@@ -1237,9 +1239,82 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 		if constexpr (SEED) {
 			if (lane == 0) { tracc[0] = 0; tracc[1] = 0; }
 		}
+		// SEED with flags & 16 (FP64 planned tiles): psi_out and dpsi_out hold the states at
+		// the pass input, and the pass's gates are applied to them here first, so that the
+		// circuit's output states never go through HBM (its last forward pass, fused)
+		cx<T>* lmf = reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(tracc - 2 * (threadIdx.x >> 6)) + kRevTraceBytes);
+		if constexpr (SEED && F44) {
+			if (flags & 16) {
+				const int whichf[2] = {kMatG, kMatZ};
+				load_pass_mats(mats, P.x0, P.ng, whichf, 2, fwd_mat_stride<true>(), lmf);
+			}
+		}
+		auto forward_gates = [&]() {
+			if constexpr (SEED && F44) {
+				constexpr int GSF = 2, gqf = 16;
+				constexpr int NSTF = ((1 << LOGE) / 4 / kTileWaves) / (GSF * gqf);
+				static_assert(NSTF >= 1 && NSTF <= 4, "one to four steps per gate and wave");
+				auto fgate = [&](auto jc) {
+					constexpr int j = decltype(jc)::value;
+					if (j < P.ng) {
+						if (j > 0 && ((P.barrier >> j) & 1)) { __syncthreads(); }
+						const Frag44 f = frag44(reinterpret_cast<const cx<double>*>(lmf + j * fwd_mat_stride<true>()));
+						int ea = eabj[j];
+						asm volatile("" : "+v"(ea));
+						__builtin_assume((ea & 15) == 0);
+						GateOperands<double, GSF> o[NSTF];
+						unsigned la[NSTF][GSF];
+						auto load_step = [&](auto sc) {
+							constexpr int SI = decltype(sc)::value;
+#pragma unroll
+							for (int n = 0; n < GSF; n++) {
+								const int gi = (SI * GSF * gqf + n * gqf) >> 4;
+								const int gb = sgj[j][0] ^ ((gi & 1) ? sgj[j][1] : 0) ^ ((gi & 2) ? sgj[j][2] : 0) ^
+								               ((gi & 4) ? sgj[j][3] : 0);
+								la[SI][n] = (unsigned)(ea ^ gb);
+								o[SI].x[n] = lds_load_cxd(la[SI][n]);
+								o[SI].y[n] = lds_load_cxd(la[SI][n] + sbytes);
+							}
+						};
+						auto step = [&](auto sc) {
+							constexpr int SI = decltype(sc)::value;
+							if constexpr (SI + 1 < NSTF) { load_step(std::integral_constant<int, SI + 1>{}); }
+							__builtin_amdgcn_sched_barrier(0);
+							__builtin_amdgcn_s_setprio(1);
+							gate_mfma44<GSF>(f, o[SI]);
+							__builtin_amdgcn_s_setprio(0);
+#pragma unroll
+							for (int n = 0; n < GSF; n++) {
+								lds_store_cxd(la[SI][n], cx<double>{o[SI].acc[n][0], o[SI].acc[n][1]});
+								lds_store_cxd(la[SI][n] + sbytes, cx<double>{o[SI].acc[n][2], o[SI].acc[n][3]});
+							}
+						};
+						load_step(std::integral_constant<int, 0>{});
+						step(std::integral_constant<int, 0>{});
+						if constexpr (NSTF > 1) { step(std::integral_constant<int, 1>{}); }
+						if constexpr (NSTF > 2) { step(std::integral_constant<int, 2>{}); }
+						if constexpr (NSTF > 3) { step(std::integral_constant<int, 3>{}); }
+					}
+				};
+				fgate(std::integral_constant<int, 0>{});
+				fgate(std::integral_constant<int, 1>{});
+				fgate(std::integral_constant<int, 2>{});
+				fgate(std::integral_constant<int, 3>{});
+				fgate(std::integral_constant<int, 4>{});
+				fgate(std::integral_constant<int, 5>{});
+				fgate(std::integral_constant<int, 6>{});
+				fgate(std::integral_constant<int, 7>{});
+			}
+		};
 		for (; tile < ntiles; tile += gridDim.x) {
 			__builtin_amdgcn_s_waitcnt(0);
 			__syncthreads();
+			if constexpr (SEED && F44) {
+				if (flags & 16) {
+					forward_gates();
+					__syncthreads();
+				}
+			}
 			if constexpr (SEED) {
 				double tr_re = 0, tr_im = 0;
 #pragma unroll
