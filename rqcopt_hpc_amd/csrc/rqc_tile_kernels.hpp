@@ -507,6 +507,81 @@ __device__ __forceinline__ void outer44(const GateOperands<double, GS>& s, const
 	acc = double4_t{c0, c1, c2, c3};
 }
 
+// Plain sweeps (no tangent state): x and y are two groups of quads, both
+// multiplied by the block pair's X (8 block products instead of the real
+// 16 x 16 form's 16); Frag44 from frag44x, whose y part is unused.
+__device__ __forceinline__ Frag44 frag44x(const cx<double>* m)
+{
+	const int lane = threadIdx.x & 63;
+	const cx<double> x = m[4 * (lane >> 4) + (lane & 3)];
+	return Frag44{x.x, x.y, 0.0, 0.0};
+}
+
+template <int GS>
+__device__ __forceinline__ void gate_mfma44_pair(const Frag44& f, GateOperands<double, GS>& o)
+{
+	double d[GS][4];
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][0] = mfma44(f.xr, o.x[n].x, 0.0);
+		d[n][1] = mfma44(f.xi, o.x[n].x, 0.0);
+		d[n][2] = mfma44(f.xr, o.y[n].x, 0.0);
+		d[n][3] = mfma44(f.xi, o.y[n].x, 0.0);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][0] = mfma44_na(f.xi, o.x[n].y, d[n][0]);
+		d[n][1] = mfma44(f.xr, o.x[n].y, d[n][1]);
+		d[n][2] = mfma44_na(f.xi, o.y[n].y, d[n][2]);
+		d[n][3] = mfma44(f.xr, o.y[n].y, d[n][3]);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) { o.acc[n] = double4_t{d[n][0], d[n][1], d[n][2], d[n][3]}; }
+}
+
+template <int GS>
+__device__ __forceinline__ void gate_mfma44_pair_t(const Frag44& f, GateOperands<double, GS>& o)
+{
+	double d[GS][4];
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][0] = mfma44(o.x[n].x, f.xr, 0.0);
+		d[n][1] = mfma44(o.x[n].x, f.xi, 0.0);
+		d[n][2] = mfma44(o.y[n].x, f.xr, 0.0);
+		d[n][3] = mfma44(o.y[n].x, f.xi, 0.0);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][0] = mfma44_na(o.x[n].y, f.xi, d[n][0]);
+		d[n][1] = mfma44(o.x[n].y, f.xr, d[n][1]);
+		d[n][2] = mfma44_na(o.y[n].y, f.xi, d[n][2]);
+		d[n][3] = mfma44(o.y[n].y, f.xr, d[n][3]);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) { o.acc[n] = double4_t{d[n][0], d[n][1], d[n][2], d[n][3]}; }
+}
+
+// outer-product sums of the plain sweeps: acc = (A1.re, A1.im, -, -), A1 summed
+// over both groups of quads
+template <int GS>
+__device__ __forceinline__ void outer44_pair(const GateOperands<double, GS>& s, const GateOperands<double, GS>& a,
+	double4_t& acc)
+{
+	double c0 = acc[0], c1 = acc[1];
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		c0 = mfma44(a.acc[n][0], s.acc[n][0], c0);
+		c1 = mfma44(a.acc[n][0], s.acc[n][1], c1);
+		c0 = mfma44_na(a.acc[n][1], s.acc[n][1], c0);
+		c1 = mfma44(a.acc[n][1], s.acc[n][0], c1);
+		c0 = mfma44(a.acc[n][2], s.acc[n][2], c0);
+		c1 = mfma44(a.acc[n][2], s.acc[n][3], c1);
+		c0 = mfma44_na(a.acc[n][3], s.acc[n][3], c0);
+		c1 = mfma44(a.acc[n][3], s.acc[n][2], c1);
+	}
+	acc = double4_t{c0, c1, acc[2], acc[3]};
+}
+
 // LDS position term of lane l's transposed result (amplitude l & 3 of group
 // quad 4 ((l >> 2) & 3) + (l >> 4))
 __device__ __forceinline__ int transposed44_lane_term(const GateTerms& t)
@@ -532,6 +607,21 @@ __device__ __forceinline__ cx<double> lds_load_cxd(unsigned off)
 __device__ __forceinline__ void lds_store_cxd(unsigned off, cx<double> v)
 {
 	*reinterpret_cast<lds_d2_as3*>(off) = lds_d2{v.x, v.y};
+}
+
+// FP64 on 4 x 4 blocks (DUAL: the pair [[G, 0], [Z, G]] on (psi, dpsi); plain: G
+// on two groups of quads)
+template <int GS, bool DUAL>
+__device__ __forceinline__ void mfma_gate_apply44(cx<double>* X, cx<double>* Y, const int* eg, const int* eg2,
+	const GateLanes& gl, const Frag44& f)
+{
+	GateOperands<double, GS> o;
+	gate_load<double, GS>(X, Y, eg, eg2, gl, o);
+	__builtin_amdgcn_s_setprio(1);
+	if constexpr (DUAL) { gate_mfma44<GS>(f, o); }
+	else { gate_mfma44_pair<GS>(f, o); }
+	__builtin_amdgcn_s_setprio(0);
+	gate_store<double, GS>(X, Y, eg, eg2, gl, o);
 }
 
 // all loads are issued before the matrix-core chains and all stores after, so
@@ -748,12 +838,12 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 		auto gate = [&](auto jc) {
 			constexpr int j = decltype(jc)::value;
 			const T* mj = reinterpret_cast<const T*>(lm + j * fwd_mat_stride<DUAL>());
-			// FP64 dual sweeps: 4 x 4 block products (F44); else the real 16 x 16 form
+			// FP64: 4 x 4 block products (F44: the planned dual tiles); FP32: the real 16 x 16 form
 			constexpr bool F44 = FIXED && sizeof(T) == 8;
 			T a[4];
 			Frag44 f44;
-			if constexpr (F44) {
-				f44 = frag44(reinterpret_cast<const cx<double>*>(mj));
+			if constexpr (sizeof(T) == 8) {
+				f44 = DUAL ? frag44(reinterpret_cast<const cx<double>*>(mj)) : frag44x(reinterpret_cast<const cx<double>*>(mj));
 			} else {
 #pragma unroll
 				for (int kk = 0; kk < 4; kk++) { a[kk] = mj[amap.aoff[kk]] * amap.afac[kk]; }
@@ -816,7 +906,12 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 						eg[n] = ewv ^ group_term(gt, (g + n * gq) >> 4);
 						eg2[n] = DUAL ? eg[n] : ewv ^ group_term(gt, ((g + n * gq) >> 4) + 1);
 					}
-					mfma_gate_apply<T, GS>(S, D, eg, eg2, gl, a);
+					if constexpr (sizeof(T) == 8) {
+						mfma_gate_apply44<GS, DUAL>(reinterpret_cast<cx<double>*>(S), reinterpret_cast<cx<double>*>(D), eg, eg2,
+							gl, f44);
+					} else {
+						mfma_gate_apply<T, GS>(S, D, eg, eg2, gl, a);
+					}
 				}
 			}
 		};
@@ -960,9 +1055,10 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 	// registers' terms).
 	constexpr int ES = (int)sizeof(cx<T>);
 	constexpr bool FIXED = DUAL && LOGE > 0;
-	// FP64 dual sweeps with a planned tile: 4 x 4 block products (gate_mfma44_t, outer44);
-	// acc[J] then holds (A1.re, A1.im, A2.re, A2.im) of the lane's block entry, btbj the
-	// lane's term of its transposed results (one amplitude of each state)
+	// FP64 sweeps run on 4 x 4 block products (gate_mfma44_t, outer44; plain sweeps the
+	// _pair forms): acc[J] holds (A1.re, A1.im[, A2.re, A2.im]) of the lane's block entry.
+	// F44: the dual sweeps with a planned tile, btbj the lane's term of its transposed
+	// results (one amplitude of each state)
 	constexpr bool F44 = DUAL && LOGE > 0 && sizeof(T) == 8;
 	int eabj[kMaxTileGates], btbj[kMaxTileGates];
 	int sgj[kMaxTileGates][8];
@@ -1002,9 +1098,12 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 			const T* mj = reinterpret_cast<const T*>(lm + J * rev_mat_stride<DUAL>());
 			T au[4], aa[4];
 			Frag44 fu, fa;
-			if constexpr (F44) {
-				fu = frag44(reinterpret_cast<const cx<double>*>(mj));
-				fa = frag44(reinterpret_cast<const cx<double>*>(mj) + 2 * kMatPairOff);
+			if constexpr (sizeof(T) == 8) {
+				// recovery G^-1 [and W], adjoint G^T [and Z^T]: pairs after 2 kMatPairOff, plain X
+				// matrices after kMatPairOff
+				const cx<double>* m = reinterpret_cast<const cx<double>*>(mj);
+				fu = DUAL ? frag44(m) : frag44x(m);
+				fa = DUAL ? frag44(m + 2 * kMatPairOff) : frag44x(m + kMatPairOff);
 			} else {
 #pragma unroll
 				for (int kk = 0; kk < 4; kk++) {
@@ -1152,6 +1251,7 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 				} else {
 					const GateTerms gt = gterms[P.x0 + J];
 					const GateLanes gl = gate_lanes<T>(gt);
+					const int es44 = (sizeof(T) == 8) ? transposed44_lane_term(gt) : 0;
 					const int ewv = sel4(gt.wv, wave);
 					for (int g = 0; g < qpw; g += GSX * gq) {
 						int eg[GSX], eg2[GSX];
@@ -1169,11 +1269,35 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 						gate_load<T, GSX>(S, D, eg, eg2, gl, os);
 						gate_load<T, GSX>(F, DF, eg, eg2, gl, oa);
 						__builtin_amdgcn_s_setprio(1);
-						gate_mfma_t<T, GSX>(au, os);
-						gate_mfma_t<T, GSX>(aa, oa);
-						gate_store_t<T, GSX>(S, D, eg, eg2, gl, os);
-						if (adjoint) { gate_store_t<T, GSX>(F, DF, eg, eg2, gl, oa); }
-						outer(gsc, os, oa);
+						if constexpr (sizeof(T) == 8) {
+							// 4 x 4 block products; lane l's results: one amplitude of each state
+							if constexpr (DUAL) {
+								gate_mfma44_t<GSX>(fu, os);
+								gate_mfma44_t<GSX>(fa, oa);
+							} else {
+								gate_mfma44_pair_t<GSX>(fu, os);
+								gate_mfma44_pair_t<GSX>(fa, oa);
+							}
+#pragma unroll
+							for (int n = 0; n < GSX; n++) {
+								if constexpr (J > 0) {
+									S[eg[n] ^ es44] = cx<T>{os.acc[n][0], os.acc[n][1]};
+									D[eg2[n] ^ es44] = cx<T>{os.acc[n][2], os.acc[n][3]};
+								}
+								if (adjoint) {
+									F[eg[n] ^ es44] = cx<T>{oa.acc[n][0], oa.acc[n][1]};
+									DF[eg2[n] ^ es44] = cx<T>{oa.acc[n][2], oa.acc[n][3]};
+								}
+							}
+							if constexpr (DUAL) { outer44<GSX>(os, oa, acc[J]); }
+							else { outer44_pair<GSX>(os, oa, acc[J]); }
+						} else {
+							gate_mfma_t<T, GSX>(au, os);
+							gate_mfma_t<T, GSX>(aa, oa);
+							gate_store_t<T, GSX>(S, D, eg, eg2, gl, os);
+							if (adjoint) { gate_store_t<T, GSX>(F, DF, eg, eg2, gl, oa); }
+							outer(gsc, os, oa);
+						}
 						__builtin_amdgcn_s_setprio(0);
 					}
 				}
@@ -1360,7 +1484,7 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 	for (int j = 0; j < kMaxTileGates; j++) {
 		if (j < P.ng) {
 			double* d = partial + (part * P.ng + j) * 256;
-			if constexpr (F44) {
+			if constexpr (sizeof(T) == 8) {
 				// the four blocks' sums (lane bits 2, 3) into the 16 x 16 layout: A1 and A2 as
 				// the (re, re), (re, im) blocks of phi (x) psi and phi (x) dpsi, zeros elsewhere
 				double4_t v = acc[j];
