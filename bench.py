@@ -42,7 +42,9 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 REVERSE_KERNEL = {
     "tile_pass": "k_tile_reverse<double,dual> (reverse tile pass: up to 8 gates per HBM pass; state recovery, "
-                 "adjoint update and gradient/HVP outer products on the FP64 matrix cores)",
+                 "adjoint update and gradient/HVP outer products on the FP64 matrix cores as 4x4x4 block "
+                 "products; the launch of the circuit's last pass also runs that pass's forward gates, "
+                 "which the algorithmic count leaves out)",
     "per_gate": "k_gate_reverse<double,dual> (per-gate reverse sweep with gradient/HVP sums)",
 }
 # measured HBM traffic of the reverse kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated)
