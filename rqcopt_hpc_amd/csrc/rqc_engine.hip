@@ -820,7 +820,7 @@ static std::shared_ptr<TilePlan> plan_tiles(const Context& dev, const Circuit& C
 	const size_t fwd_mats = (size_t)kMaxTileGates * (dual ? fwd_mat_stride<true>() : fwd_mat_stride<false>()) * sizeof(cx<T>);
 	const size_t rev_mats = (size_t)kMaxTileGates * (dual ? rev_mat_stride<true>() : rev_mat_stride<false>()) * sizeof(cx<T>);
 	tp->lds_fwd = (size_t)(dual ? fwd_buffers<T, true>() : fwd_buffers<T, false>()) * (dual ? 2 : 1) *
-		(Ef * sizeof(cx<T>) + kLdsPadBytes) + fwd_mats;
+		(Ef * sizeof(cx<T>) + kLdsPadBytes) + fwd_mats + kFwdVoteBytes;
 	tp->lds_rev = (size_t)(dual ? rev_buffers<T, true>() : rev_buffers<T, false>()) * (dual ? 4 : 2) *
 		(E * sizeof(cx<T>) + lds_pad_rev<T>()) + rev_mats + kRevTraceBytes;
 	auto* kf = fwd_kernel<T>(dual, tp->tsf.K + tp->tsf.logC);

@@ -117,6 +117,8 @@ template <bool DUAL> constexpr int fwd_mat_stride() { return DUAL ? 2 * kMatPair
 template <bool DUAL> constexpr int rev_mat_stride() { return DUAL ? 4 * kMatPairOff : 2 * kMatPairOff; }
 // after the reverse pass's gate matrices: a running trace sum (two doubles) per wave
 constexpr int kRevTraceBytes = kTileWaves * 16;
+// after the forward pass's gate matrices: one word per wave (pass 0's zero-tile vote)
+constexpr int kFwdVoteBytes = kTileWaves * 4;
 
 // A operand of step kk of the real 16 x 16 matrix with complex blocks
 // [[X, 0], [Y, X]] (Y = 0 without the tangent state), components m = 4 v + r,
@@ -710,20 +712,25 @@ __device__ __forceinline__ void tile_fill(cx<T>* S, const TileIO<T>& io, cx<T> v
 }
 
 // column block b0 of the identity: element (row, col) = [row == b0 + c0 + col]
+// (returns whether the thread's slots hold a nonzero element)
 template <typename T>
-__device__ __forceinline__ void tile_identity(cx<T>* S, const TileIO<T>& io, long rbase, long c0, long b0)
+__device__ __forceinline__ int tile_identity(cx<T>* S, const TileIO<T>& io, long rbase, long c0, long b0)
 {
 	constexpr int epl = TileIO<T>::epl;
+	int nz = 0;
 #pragma unroll
 	for (int k = 0; k < kMaxSlots; k++) {
 		if (k < io.nk) {
 			const long row = rbase | (long)io.trow[k];
 #pragma unroll
 			for (int i = 0; i < epl; i++) {
-				S[(threadIdx.x + k * kTileThreads) * epl + i] = cx<T>{(row == b0 + c0 + io.col[k] + i) ? (T)1 : (T)0, (T)0};
+				const bool one = row == b0 + c0 + io.col[k] + i;
+				nz |= one;
+				S[(threadIdx.x + k * kTileThreads) * epl + i] = cx<T>{one ? (T)1 : (T)0, (T)0};
 			}
 		}
 	}
+	return nz;
 }
 
 template <typename T>
@@ -792,10 +799,16 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 	load_pass_mats(mats, P.x0, P.ng, which, DUAL ? 2 : 1, fwd_mat_stride<DUAL>(), lm);
 	// FP32: amplitude-major rows, so that a result lane holds one amplitude (gate_store)
 	const AOperandMap<T, DUAL> amap = a_operand_map<T, DUAL, sizeof(T) == 4>();
+	// pass 0 (flags & 1): whether the thread's slots of the prefetched identity tile hold a
+	// nonzero; most identity tiles are zero, and a zero tile's output is zero (its gates skipped)
+	int nz = 1;
+	static_assert(kTileWaves == 4, "one vote word per wave");
+	volatile int* tnz = reinterpret_cast<volatile int*>(reinterpret_cast<char*>(lm) +
+		kMaxTileGates * fwd_mat_stride<DUAL>() * (int)sizeof(cx<T>));
 	auto prefetch = [&](long tile, int b) {
 		const long rbase = deposit_bits(tile >> P.lognc, P.rmask);
 		const long c0 = (tile & cmask) << P.logC;
-		if (flags & 1) { tile_identity(state(b, 0), io, rbase, c0, b0); }
+		if (flags & 1) { nz = tile_identity(state(b, 0), io, rbase, c0, b0); }
 		else { tile_issue(state(b, 0), in, rbase * B + c0, io); }
 		if (DUAL) {
 			if (flags & 2) { tile_fill(state(b, 1), io, cx<T>{0, 0}); }
@@ -951,8 +964,17 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 	} else {
 		for (; tile < ntiles; tile += gridDim.x) {
 			__builtin_amdgcn_s_waitcnt(0);
-			__syncthreads();
-			run_tile(0);
+			if (flags & 1) {
+				// each wave's vote in LDS after the gate matrices (read by all before the
+				// barrier that ends the tile, written again only after it)
+				const int wnz = __any(nz);   // (a vote of all the wave's lanes)
+				if ((threadIdx.x & 63) == 0) { tnz[wave] = wnz ? 1 : 0; }
+				__syncthreads();
+				if (tnz[0] | tnz[1] | tnz[2] | tnz[3]) { run_tile(0); }
+			} else {
+				__syncthreads();
+				run_tile(0);
+			}
 			__syncthreads();
 			store(tile, 0);
 			if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, 0); }
