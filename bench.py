@@ -43,8 +43,8 @@ FP64_PEAK_TFLOPS = 78.6
 REVERSE_KERNEL = {
     "tile_pass": "k_tile_reverse<double,dual> (reverse tile pass: up to 8 gates per HBM pass; state recovery, "
                  "adjoint update and gradient/HVP outer products on the FP64 matrix cores as 4x4x4 block "
-                 "products; the launch of the circuit's last pass also runs that pass's forward gates, "
-                 "which the algorithmic count leaves out)",
+                 "products; the launch of the circuit's last pass also runs that pass's forward gates, and "
+                 "the circuit's first pass skips the tiles that hold no identity element)",
     "per_gate": "k_gate_reverse<double,dual> (per-gate reverse sweep with gradient/HVP sums)",
 }
 # measured HBM traffic of the reverse kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE, copy-kernel calibrated)
@@ -58,13 +58,6 @@ def reverse_alg_flops(L, d):
     Together with the forward sweep's 12 (psi: 4, dpsi: 8) these are the 36 complex MACs per
     amplitude and gate of the eval's canonical count (144 L d N^2 flops)."""
     return 192.0 * (d * (L // 2)) * (1 << L) ** 2
-
-
-def reverse_issued_flops(L, d):
-    """Flops the reverse tile sweep issues: the algorithmic 24 complex MACs plus the recovery of the
-    gate-input states from the pass checkpoint (psi = G^-1 psi', dpsi = G^-1 dpsi' - G^-1 Z G^-1 psi':
-    12 more), which replaces storing every gate's input state: 288 flops per amplitude and gate."""
-    return 288.0 * (d * (L // 2)) * (1 << L) ** 2
 
 
 def canonical_bytes(L, d, s):
@@ -160,7 +153,9 @@ def run(args):
             one()
             R.profile_enable(False)
         prof = {k: R.profile_query(k) for k in range(4)}
-        results[name] = {"eval_s": dt / args.steps, "out": out, "prof": prof, "block": R.last_column_block(),
+        flops = {k: R.profile_query_flops(k) for k in range(4)}
+        results[name] = {"eval_s": dt / args.steps, "out": out, "prof": prof, "flops": flops,
+                         "block": R.last_column_block(),
                          "tiles": R.last_tiles()}
         print(f"[bench] {name}: {dt / args.steps * 1e3:.2f} ms/eval, column block {R.last_column_block()}, "
               f"tiles (log2 rows, log2 cols, gates per pass) {R.last_tiles()}, "
@@ -194,9 +189,12 @@ def run(args):
     # arrays read + written once, summed by the library per launch) is reported beside it
     rev_ms, rev_n, rev_bytes = r64["prof"][R.KERNEL_REVERSE]
     engine_used, passes = R.last_sweep()
-    rev_flops = reverse_alg_flops(L, d) if engine == "tile_pass" else None
+    # the library's count per launch: 192 flops (issued: 288) per amplitude and gate of the tiles
+    # a launch processes (the first pass skips the tiles without an identity element, whose
+    # terms vanish), plus the forward pass the last launch carries (96)
+    rev_flops, rev_issued = r64["flops"][R.KERNEL_REVERSE] if engine == "tile_pass" else (None, None)
     achieved_tf = rev_flops / (rev_ms * 1e-3) / 1e12 if rev_flops else None
-    issued_tf = reverse_issued_flops(L, d) / (rev_ms * 1e-3) / 1e12 if rev_flops else None
+    issued_tf = rev_issued / (rev_ms * 1e-3) / 1e12 if rev_flops else None
     hbm_gbs = rev_bytes / (rev_ms * 1e-3) / 1e9
     traffic = None
     if engine == "tile_pass" and os.path.exists(TRAFFIC_FILE) and (L, d) == (14, 9):
@@ -232,7 +230,8 @@ def run(args):
         "roofline": ({"bound": "mfma", "achieved": achieved_tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                       "frac": achieved_tf / FP64_PEAK_TFLOPS, "traffic": traffic,
                       "alg_flops_per_launch": rev_flops / max(rev_n, 1),
-                      "issued": {"flops_per_launch": reverse_issued_flops(L, d) / max(rev_n, 1),
+                      "alg_flops_full_sweep": reverse_alg_flops(L, d),
+                      "issued": {"flops_per_launch": rev_issued / max(rev_n, 1),
                                  "tflops": issued_tf, "frac": issued_tf / FP64_PEAK_TFLOPS}}
                      if achieved_tf else
                      {"bound": "hbm", "achieved": hbm_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

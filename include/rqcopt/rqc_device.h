@@ -104,6 +104,12 @@ int rqc_last_sweep(int* engine, int* passes);
 void rqc_profile_enable(int on);
 void rqc_profile_reset(void);
 int rqc_profile_query(int kind, double* total_ms, long* launches, double* alg_bytes);
+/* algorithmic flops of the timed tile-pass launches of one kind (per amplitude and gate of the
+   tiles a launch processes: forward 96, reverse 192 for the adjoint update and the outer
+   products, plain sweeps a third of these; a reverse launch that carries a forward pass adds
+   that pass's), and the flops they issue (the reverse passes' state recovery besides: 96 per
+   amplitude and gate, plain 32).  No reference counterpart: the library's own measurement. */
+int rqc_profile_query_flops(int kind, double* alg_flops, double* issued_flops);
 
 /* device-to-device copy of `bytes` bytes by a plain streaming kernel (reads
    `bytes`, writes `bytes`), timed with HIP events over `reps` launches; gives the

@@ -128,7 +128,7 @@ EXPORTED = [
     "rqc_device_ufunc", "rqc_target_create_dense", "rqc_target_create_from_ufunc", "rqc_target_create_brickwall",
     "rqc_target_destroy", "rqc_target_nqubits", "rqc_target_download", "rqc_set_precision", "rqc_get_precision",
     "rqc_set_workspace_limit", "rqc_release_workspace", "rqc_last_column_block", "rqc_profile_enable",
-    "rqc_profile_reset", "rqc_profile_query", "rqc_synchronize", "rqc_device_info", "rqc_measure_copy_bandwidth",
+    "rqc_profile_reset", "rqc_profile_query", "rqc_profile_query_flops", "rqc_synchronize", "rqc_device_info", "rqc_measure_copy_bandwidth",
     "rqc_measure_stream_bandwidth", "rqc_set_sweep_engine", "rqc_last_sweep", "rqc_target_create_circuit",
     "rqc_brickwall_unitary_target_projected_hvp_noalloc", "rqc_noalloc_scratch_bytes",
     # rqc_batch.h
@@ -176,6 +176,7 @@ def lib():
         L.rqc_noalloc_scratch_bytes.restype = ct.c_size_t
         L.rqc_set_workspace_limit.argtypes = [ct.c_size_t]
         L.rqc_profile_query.argtypes = [ct.c_int, ct.POINTER(ct.c_double), ct.POINTER(ct.c_long), ct.POINTER(ct.c_double)]
+        L.rqc_profile_query_flops.argtypes = [ct.c_int, ct.POINTER(ct.c_double), ct.POINTER(ct.c_double)]
         _LIB = L
     return _LIB
 
@@ -1379,6 +1380,13 @@ def profile_query(kind):
     ms, n, b = ct.c_double(), ct.c_long(), ct.c_double()
     _check("rqc_profile_query", lib().rqc_profile_query(kind, ct.byref(ms), ct.byref(n), ct.byref(b)))
     return ms.value, n.value, b.value
+
+
+def profile_query_flops(kind):
+    """(algorithmic flops, issued flops) of the timed tile-pass launches of one kernel kind."""
+    a, i = ct.c_double(), ct.c_double()
+    _check("rqc_profile_query_flops", lib().rqc_profile_query_flops(kind, ct.byref(a), ct.byref(i)))
+    return a.value, i.value
 
 
 def measure_copy_bandwidth(nbytes=1 << 30, reps=10):

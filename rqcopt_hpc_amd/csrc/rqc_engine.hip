@@ -108,7 +108,7 @@ Context& ctx()
 	return *c;
 }
 
-LaunchTimer::LaunchTimer(Slot& sl, int kind, double bytes) : s(sl)
+LaunchTimer::LaunchTimer(Slot& sl, int kind, double bytes, double flops, double issued) : s(sl)
 {
 	if (!s.profile) { return; }
 	if (s.events_used == s.events.size()) {
@@ -120,6 +120,8 @@ LaunchTimer::LaunchTimer(Slot& sl, int kind, double bytes) : s(sl)
 	ev = &s.events[s.events_used++];
 	ev->kind = kind;
 	ev->bytes = bytes;
+	ev->flops = flops;
+	ev->issued = issued;
 	RQC_HIP(hipEventRecord(ev->a, s.stream));
 }
 
@@ -138,6 +140,8 @@ void profile_flush(Context& c)
 		c.prof_ms[c.events[k].kind] += ms;
 		c.prof_n[c.events[k].kind] += 1;
 		c.prof_bytes[c.events[k].kind] += c.events[k].bytes;
+		c.prof_flops[c.events[k].kind] += c.events[k].flops;
+		c.prof_issued[c.events[k].kind] += c.events[k].issued;
 	}
 	c.events_used = 0;
 }
@@ -1046,6 +1050,15 @@ void enqueue_fill_tables(hipStream_t s, const DeviceTableDesc* d_descs, int n)
 }
 
 // tile-pass engine: several gates per HBM pass through LDS tiles
+// tiles of a pass's geometry that hold an element of the identity columns b0 + j of a column
+// block of width B (any b0): per column chunk, one value of the row bits outside the tile above
+// the chunk's low log2 C bits, and every value of those among the low bits
+static long active_tiles(const TileArgs& a, long B)
+{
+	const unsigned low = a.rmask & ((1u << a.logC) - 1);
+	return (B >> a.logC) << __builtin_popcount(low);
+}
+
 template <typename T>
 static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* d_partial, const cx<T>* mats,
 	const GateTerms* gterms, cx<T>* S, const int* skip, const cx<T>* X, const zc* W, const BlockEnc* be)
@@ -1064,6 +1077,7 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	cx<T>* phi[2] = {S + nck * sz, S + (nck + 1) * sz};
 	cx<T>* dphi[2] = {S + (nck + 2) * sz, S + (nck + 3) * sz};
 	const long ntiles = (N >> tp.ts.K) * (B >> tp.ts.logC);
+	const double E = (double)(1L << (tp.ts.K + tp.ts.logC)), Ef = (double)(1L << (tp.tsf.K + tp.tsf.logC));
 	auto* kf = fwd_kernel<T>(dual, tp.tsf.K + tp.tsf.logC);
 	auto* kr = rev_kernel<T>(dual, tp.ts.K + tp.ts.logC);
 	auto* krs = rev_kernel<T>(dual, tp.ts.K + tp.ts.logC, true);
@@ -1099,7 +1113,10 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 			const cx<T>* in = (p == 0) ? (pl.sampled ? X + (size_t)blk * sz : nullptr)
 			                           : (mode == Mode::Target ? buf[(p + 1) % 2] : PS(p));
 			cx<T>* out = (mode == Mode::Target) ? buf[p % 2] : PS(p + 1);
-			LaunchTimer lt(s, RQC_KERNEL_FORWARD, (double)(ns * (p > 0) + ns) * ab);
+			// pass 0 applies its gates to the tiles that hold an identity element only
+			const double fwd_amps = (double)(flags & 1 ? active_tiles(tp.args_fwd[p], pl.B) : tp.ntiles_fwd) * Ef;
+			const double fwd_flops = (dual ? 96.0 : 32.0) * (double)tp.passes[p].gates.size() * fwd_amps;
+			LaunchTimer lt(s, RQC_KERNEL_FORWARD, (double)(ns * (p > 0) + ns) * ab, fwd_flops, fwd_flops);
 			kf<<<tp.gfwd, kTileThreads, tp.lds_fwd, s.stream>>>(in, out, (dual && p > 0) ? DPS(p) : nullptr,
 				dual ? DPS(p + 1) : nullptr, tp.args_fwd[p], mats, fterms, B, b0, tp.ntiles_fwd, flags, skip);
 		}
@@ -1124,17 +1141,28 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 			int flags = 0;
 			if (p == P - 1) { flags |= 4; }
 			if (p == 0) { flags |= 8; }
+			// the first pass from identity columns: only tiles with an identity element contribute
+			if (p == 0 && !pl.sampled) { flags |= 32; }
 			const bool seed_pass = fuse_trace && p == P - 1;
 			const bool with_fwd = seed_pass && fuse_fwd;   // psi, dpsi at the pass input (flags 16)
 			if (with_fwd) { flags |= 16; }
 			const int ng = (int)tp.passes[p].gates.size();
 			{
 				const int reads = dual ? 3 + (p < P - 1) : 2;
-				LaunchTimer lt(s, RQC_KERNEL_REVERSE, (double)(reads + ns * (p > 0)) * ab);
+				// algorithmic work of the launch: per amplitude and gate of the tiles it processes,
+				// the adjoint update and the outer products (dual 24 complex MACs, plain 8), the
+				// recovery besides (dual 12, plain 4), and a fused forward pass (12)
+				const double frac = (flags & 32) ? (double)active_tiles(tp.args[p], pl.B) / (double)ntiles : 1.0;
+				const double amps = frac * (double)ntiles * E;
+				const double rflops = (dual ? 192.0 : 64.0) * ng * amps;
+				const double rissued = (dual ? 288.0 : 96.0) * ng * amps;
+				const double fflops = with_fwd ? 96.0 * ng * (double)ntiles * E : 0.0;
+				LaunchTimer lt(s, RQC_KERNEL_REVERSE, (double)(reads * frac + ns * (p > 0)) * ab, rflops + fflops,
+					rissued + fflops);
 				(seed_pass ? krs : kr)<<<tp.grev, kTileThreads, seed_pass ? tp.lds_rev_seed : tp.lds_rev, s.stream>>>(
 					with_fwd ? PS(p) : PS(p + 1), dual ? (with_fwd ? DPS(p) : DPS(p + 1)) : nullptr,
 					seed_pass ? reinterpret_cast<const cx<T>*>(seed) : phi[cur], dual ? dphi[cur] : nullptr, phi[1 - cur],
-					dual ? dphi[1 - cur] : nullptr, tp.args[p], mats, gterms, B, ntiles, flags, mpart, ttr, skip);
+					dual ? dphi[1 - cur] : nullptr, tp.args[p], mats, gterms, B, b0, ntiles, flags, mpart, ttr, skip);
 			}
 			if (seed_pass) {
 				LaunchTimer lt(s, RQC_KERNEL_OTHER);
@@ -3156,7 +3184,13 @@ extern "C" void rqc_profile_reset(void)
 		Context& c = ctx();
 		std::lock_guard<std::recursive_mutex> lk(c.mu);
 		profile_flush(c);
-		for (int k = 0; k < RQC_KERNEL_KINDS; k++) { c.prof_ms[k] = 0; c.prof_n[k] = 0; c.prof_bytes[k] = 0; }
+		for (int k = 0; k < RQC_KERNEL_KINDS; k++) {
+			c.prof_ms[k] = 0;
+			c.prof_n[k] = 0;
+			c.prof_bytes[k] = 0;
+			c.prof_flops[k] = 0;
+			c.prof_issued[k] = 0;
+		}
 	} catch (const std::exception& e) { fatal("rqc_profile_reset", e); }
 }
 
@@ -3170,6 +3204,18 @@ extern "C" int rqc_profile_query(int kind, double* total_ms, long* launches, dou
 		if (total_ms != nullptr) { *total_ms = c.prof_ms[kind]; }
 		if (launches != nullptr) { *launches = c.prof_n[kind]; }
 		if (alg_bytes != nullptr) { *alg_bytes = c.prof_bytes[kind]; }
+	});
+}
+
+extern "C" int rqc_profile_query_flops(int kind, double* alg_flops, double* issued_flops)
+{
+	if (kind < 0 || kind >= RQC_KERNEL_KINDS) { return -3; }
+	return target_call("rqc_profile_query_flops", [&] {
+		Context& c = ctx();
+		std::lock_guard<std::recursive_mutex> lk(c.mu);
+		profile_flush(c);
+		if (alg_flops != nullptr) { *alg_flops = c.prof_flops[kind]; }
+		if (issued_flops != nullptr) { *issued_flops = c.prof_issued[kind]; }
 	});
 }
 

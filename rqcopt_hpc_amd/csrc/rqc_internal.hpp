@@ -56,7 +56,7 @@ struct Slot
 
 	// kernel timing (HIP events on this slot's stream)
 	bool profile = false;
-	struct Ev { hipEvent_t a, b; int kind; double bytes; };
+	struct Ev { hipEvent_t a, b; int kind; double bytes, flops, issued; };
 	std::vector<Ev> events;
 	size_t events_used = 0;
 };
@@ -85,6 +85,8 @@ struct Context : Slot
 	double prof_ms[RQC_KERNEL_KINDS] = {0, 0, 0, 0};
 	long prof_n[RQC_KERNEL_KINDS] = {0, 0, 0, 0};
 	double prof_bytes[RQC_KERNEL_KINDS] = {0, 0, 0, 0};   // algorithmic HBM bytes of the timed launches
+	double prof_flops[RQC_KERNEL_KINDS] = {0, 0, 0, 0};   // their algorithmic flops (tile passes)
+	double prof_issued[RQC_KERNEL_KINDS] = {0, 0, 0, 0};  // and the flops they issue (with the recovery)
 
 	std::recursive_mutex mu;
 };
@@ -101,7 +103,7 @@ struct LaunchTimer
 {
 	Slot& s;
 	Slot::Ev* ev = nullptr;
-	LaunchTimer(Slot& sl, int kind, double bytes = 0);
+	LaunchTimer(Slot& sl, int kind, double bytes = 0, double flops = 0, double issued = 0);
 	~LaunchTimer();
 };
 

@@ -987,7 +987,8 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 // (and dphi) at the pass output; outputs phi (and dphi) at the pass input.
 // flags: 4 = dphi at the pass output is zero, 8 = do not write phi at the pass input,
 // 16 (SEED, FP64 planned tiles) = psi and dpsi are given at the pass input and the pass's
-// gates are applied to them first (G, Z after the trace sums in LDS).
+// gates are applied to them first (G, Z after the trace sums in LDS), 32 (with 8) = the
+// sweep started from identity columns b0 + j: tiles without an identity element are skipped.
 // SEED (FP64 dual sweeps, the circuit's last pass): phi_out is the target U's column block;
 // the seed -conj(U) is formed in LDS and the trace sum(-conj(U) psi) goes to
 // trace[2 (block * waves + wave)] (a separate instantiation, so that the other passes run
@@ -1012,8 +1013,8 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 	const cx<T>* __restrict__ psi_out, const cx<T>* __restrict__ dpsi_out,
 	const cx<T>* __restrict__ phi_out, const cx<T>* __restrict__ dphi_out,
 	cx<T>* __restrict__ phi_in, cx<T>* __restrict__ dphi_in,
-	const TileArgs P, const cx<T>* __restrict__ mats, const GateTerms* __restrict__ gterms, long B, long ntiles, int flags,
-	double* __restrict__ partial, double* __restrict__ trace, const int* skip)
+	const TileArgs P, const cx<T>* __restrict__ mats, const GateTerms* __restrict__ gterms, long B, long b0, long ntiles,
+	int flags, double* __restrict__ partial, double* __restrict__ trace, const int* skip)
 {
 	if (skipped(skip)) { return; }
 	extern __shared__ __align__(16) char lds_raw[];
@@ -1358,20 +1359,41 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 		gate_at(std::integral_constant<int, 0>{});
 		if constexpr (sizeof(T) == 4) { flush32(acc[0]); }
 	};
-	long tile = blockIdx.x;
+	// flags & 32 (the sweep's first pass, from identity columns b0 + j, no adjoint written):
+	// psi and dpsi at the pass output are zero on every tile without an identity element (the
+	// pass's gates act inside a tile), so such tiles add nothing to the sums and are skipped,
+	// loads included.  Tile t holds element (b0 + c0 + i, b0 + c0 + i) for some column i < C
+	// iff the column chunk's row bits outside the tile equal the tile's (its low log2 C bits
+	// range over every value).
+	auto tile_nonzero = [&](long t) {
+		if (!(flags & 32)) { return true; }
+		const long rbase = deposit_bits(t >> P.lognc, P.rmask);
+		const long hi = ~((1L << P.logC) - 1);
+		return ((b0 + ((t & cmask) << P.logC)) & (long)P.rmask & hi) == (rbase & hi);
+	};
+	auto next_tile = [&](long t) {
+		while (t < ntiles && !tile_nonzero(t)) { t += gridDim.x; }
+		return t;
+	};
+	long tile = next_tile(blockIdx.x);
 	if (tile < ntiles) { prefetch(tile, 0); }
 	if constexpr (rev_buffers<T, DUAL>() == 2) {
 		int b = 0;
-		for (; tile < ntiles; tile += gridDim.x, b ^= 1) {
+		long prev = -1;
+		while (tile < ntiles) {
 			__builtin_amdgcn_s_waitcnt(0);
 			__syncthreads();
-			if (tile != (long)blockIdx.x) { store(tile - gridDim.x, b ^ 1); }
-			if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, b ^ 1); }
+			if (prev >= 0) { store(prev, b ^ 1); }
+			const long next = next_tile(tile + gridDim.x);
+			if (next < ntiles) { prefetch(next, b ^ 1); }
 			run_tile(b);
+			prev = tile;
+			tile = next;
+			b ^= 1;
 		}
-		if ((long)blockIdx.x < ntiles) {
+		if (prev >= 0) {
 			__syncthreads();
-			store(tile - gridDim.x, b ^ 1);
+			store(prev, b ^ 1);
 		}
 	} else {
 		// SEED: phi_out holds the target U's tile; the adjoint seed is -conj(U), formed in
@@ -1452,7 +1474,7 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 				fgate(std::integral_constant<int, 7>{});
 			}
 		};
-		for (; tile < ntiles; tile += gridDim.x) {
+		while (tile < ntiles) {
 			__builtin_amdgcn_s_waitcnt(0);
 			__syncthreads();
 			if constexpr (SEED && F44) {
@@ -1491,7 +1513,9 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 			__syncthreads();
 			// each thread stores the slots that it then refills
 			store(tile, 0);
-			if (tile + gridDim.x < ntiles) { prefetch(tile + gridDim.x, 0); }
+			const long next = next_tile(tile + gridDim.x);
+			if (next < ntiles) { prefetch(next, 0); }
+			tile = next;
 		}
 		if constexpr (SEED) {
 			if (lane == 0) {

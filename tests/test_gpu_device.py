@@ -1,7 +1,9 @@
 """GPU tests of the device measurement helpers (include/rqcopt/rqc_device.h; no reference
 counterpart): the plain and the tuned streaming copy report a bandwidth between zero and the
-MI355X's 8 TB/s HBM peak, and reject invalid arguments with the reference's -3.
+MI355X's 8 TB/s HBM peak, and reject invalid arguments with the reference's -3; the launch
+profile's flop counts follow the tile passes' per-amplitude work.
 """
+import numpy as np
 import pytest
 
 import rqcopt_hpc_amd as R
@@ -33,3 +35,39 @@ def test_bandwidth_rejects_invalid_arguments():
             fn(1 << 20, 0)
         with pytest.raises(R.RqcError):
             fn(8, 1)
+
+
+def test_profile_flops_of_an_hvp_eval():
+    """One L=10 depth-5 HVP eval: the reverse launches count 192 flops per amplitude and gate of
+    the tiles they process (288 issued, with the recovery), the forward launches 96.  The first
+    reverse pass skips the tiles without an identity element and the last reverse launch carries
+    the last forward pass, so the reverse counts lie between bounds fixed by the sweep."""
+    from rqcopt_hpc_amd.workloads import config_inputs
+    L, nl = 10, 5
+    V, P, vec, Z = config_inputs(L, nl, seed=3)
+    rng = np.random.default_rng(3)
+    N = 1 << L
+    U = np.linalg.qr(rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))[0]
+    T = R.DeviceTarget.dense(U)
+    R.set_precision(R.PRECISION_FP64)
+    R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)   # plan and warm up
+    R.profile_reset()
+    R.profile_enable(True)
+    try:
+        R.brickwall_unitary_target_projected_hessian_vector_product(T, V, Z, P)
+    finally:
+        R.profile_enable(False)
+    engine, passes = R.last_sweep()
+    G = nl * (L // 2)
+    amp_gates = float(G) * N * N
+    rev_alg, rev_issued = R.profile_query_flops(R.KERNEL_REVERSE)
+    fwd_alg, fwd_issued = R.profile_query_flops(R.KERNEL_FORWARD)
+    print(f"passes {passes}: reverse {rev_alg:.4g} / {rev_issued:.4g}, forward {fwd_alg:.4g}", flush=True)
+    assert fwd_alg == fwd_issued
+    assert 0 < fwd_alg <= 96 * amp_gates
+    # every reverse gate at least on its tiles with identity elements, at most on all of them
+    assert 192 * amp_gates / 16 < rev_alg <= 192 * amp_gates + 96 * amp_gates
+    # issued = algorithmic + 96 per processed reverse amplitude-gate (the fused forward has none)
+    rev_only = (rev_issued - rev_alg) / 96.0
+    assert rev_only * 192 <= rev_alg + 1
+    assert 1.0 < rev_issued / rev_alg <= 1.5
