@@ -5,25 +5,25 @@
 // (gates on disjoint wires commute), whose wires all lie in a set Q of K
 // "tile" qubits.  A tile is the 2^K x C sub-array of a column block
 // X[a * B + c] (column fastest) obtained by fixing the other L - K row bits
-// ("rest") and a chunk of C consecutive columns; C * sizeof(element) = 128
-// bytes, so every tile row is one fully used 128-byte segment.  A workgroup
-// loads a tile into LDS, applies all gates of the pass there and writes it
-// back: one HBM read and one write per pass instead of per gate.
+// ("rest") and a chunk of C consecutive columns (at the planned tiles, rows of
+// 256 bytes).  A workgroup loads a tile into LDS, applies all gates of the pass
+// there and writes it back: one HBM read and one write per pass instead of per
+// gate.
 //
 // Gate application on the matrix cores.  A dual quad (psi and dpsi on the four
-// amplitudes of a gate) has 16 real components m = 4 v + r, v = 2 s + p
-// (s: primary / tangent state, p: re / im, r: amplitude).  The update
+// amplitudes of a gate) has 16 real components; the update
 //   psi' = G psi,  dpsi' = G dpsi + Z psi
-// is a real 16 x 16 matrix M (complex blocks [[G, 0], [Z, G]]) applied to the
-// 16 x 16 array of 16 quads, i.e. four chained 16x16x4 steps
-//   D = sum_kk M[:, 4kk:4kk+4] X[4kk:4kk+4, :].
-// Lane l holds quad (l & 15), amplitude r = l >> 4: its B operand of step kk
-// is component v = kk of that amplitude, its A operand M[l % 16][4 kk + l / 16]
-// (read from the pass's gate matrices in LDS through a per-lane map fixed for
-// the launch).  The gate thus costs four registers per lane, and each state
-// element is read and written once per gate.  Without the tangent state (plain
-// sweeps) the second half of the components is a second quad and
-// M = [[G, 0], [0, G]].
+// is a real 16 x 16 matrix with complex blocks [[G, 0], [Z, G]].  Lane l holds
+// amplitude l >> 4 of quad l & 15 of a 16-quad group.
+// - FP64: 4 x 4 block products (v_mfma_f64_4x4x4_4b_f64, see gate_mfma44): each
+//   complex 4 x 4 gate is 4 real block products, the dual matrix 12, with no
+//   zero block; plain sweeps (no tangent state) apply G to two groups of quads.
+// - FP32: the real 16 x 16 form, four chained 16x16x4 steps
+//     D = sum_kk M[:, 4kk:4kk+4] X[4kk:4kk+4, :],
+//   lane l's A operand M[l % 16][4 kk + l / 16] read from the pass's gate
+//   matrices in LDS through a per-lane map fixed for the launch; plain sweeps
+//   take M = [[G, 0], [0, G]] on two quads.
+// Each state element is read and written once per gate.
 //
 // Reverse pass (k_tile_reverse): reloads the pass-output checkpoint and walks
 // the pass's gates backwards, recovering the state before each gate with the
@@ -37,11 +37,11 @@
 // that their results are the operands of the outer products, which thus pair
 // the adjoint at the gate's input with the recovered state; the reduction maps
 // these sums back to the output adjoint with G^-T and (-G^-1 Z G^-1)^T.  The
-// outer-product sums run on the matrix cores as 16x16x4 steps over 4 quads, in
-// FP64 (FP64 states) or in FP32 per 16-quad group added into FP64 accumulators
-// (FP32 states).  Partial sums per (workgroup, wave, gate) are reduced in a
-// fixed order by k_tile_reduce_chunks and k_tile_reduce: no atomics, bitwise
-// reproducible.
+// outer-product sums run on the matrix cores: FP64 as 4 x 4 block products into
+// four FP64 accumulators per gate, FP32 as 16x16x4 steps chained in FP32 over a
+// gate's quads of the tile and added into FP64 accumulators.  Partial sums per
+// (workgroup, wave, gate) are reduced in a fixed order by k_tile_reduce_chunks
+// and k_tile_reduce: no atomics, bitwise reproducible.
 #pragma once
 
 #include <cstdint>
