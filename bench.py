@@ -43,7 +43,7 @@ FP64_PEAK_TFLOPS = 78.6
 REVERSE_KERNEL = {
     "tile_pass": "k_tile_reverse<double,dual> (reverse tile pass: up to 8 gates per HBM pass; state recovery, "
                  "adjoint update and gradient/HVP outer products on the FP64 matrix cores as 4x4x4 block "
-                 "products; the launch of the circuit's last pass also runs that pass's forward gates, and "
+                 "products, the gate applications in the three-real-product complex form; the launch of the circuit's last pass also runs that pass's forward gates, and "
                  "the circuit's first pass skips the tiles that hold no identity element)",
     "per_gate": "k_gate_reverse<double,dual> (per-gate reverse sweep with gradient/HVP sums)",
 }
@@ -189,9 +189,10 @@ def run(args):
     # arrays read + written once, summed by the library per launch) is reported beside it
     rev_ms, rev_n, rev_bytes = r64["prof"][R.KERNEL_REVERSE]
     engine_used, passes = R.last_sweep()
-    # the library's count per launch: 192 flops (issued: 288) per amplitude and gate of the tiles
+    # the library's count per launch: 192 flops (issued on the matrix cores: 240, recovery
+    # included, gate applications in the three-product form) per amplitude and gate of the tiles
     # a launch processes (the first pass skips the tiles without an identity element, whose
-    # terms vanish), plus the forward pass the last launch carries (96)
+    # terms vanish), plus the forward pass the last launch carries (96; issued 72)
     rev_flops, rev_issued = r64["flops"][R.KERNEL_REVERSE] if engine == "tile_pass" else (None, None)
     achieved_tf = rev_flops / (rev_ms * 1e-3) / 1e12 if rev_flops else None
     issued_tf = rev_issued / (rev_ms * 1e-3) / 1e12 if rev_flops else None

@@ -1078,6 +1078,11 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	cx<T>* dphi[2] = {S + (nck + 2) * sz, S + (nck + 3) * sz};
 	const long ntiles = (N >> tp.ts.K) * (B >> tp.ts.logC);
 	const double E = (double)(1L << (tp.ts.K + tp.ts.logC)), Ef = (double)(1L << (tp.tsf.K + tp.tsf.logC));
+	// matrix-core flops issued per amplitude and gate: a gate application (FP64 4 x 4 blocks, the
+	// dual pairs in the three-product form: 9 block products per 16 quads; FP32 the real 16 x 16
+	// form) and the outer-product sums
+	const double app_issued = (sizeof(T) == 8) ? (dual ? 72.0 : 32.0) : (dual ? 128.0 : 64.0);
+	const double outer_issued = (sizeof(T) == 8) ? (dual ? 96.0 : 32.0) : (dual ? 128.0 : 64.0);
 	auto* kf = fwd_kernel<T>(dual, tp.tsf.K + tp.tsf.logC);
 	auto* kr = rev_kernel<T>(dual, tp.ts.K + tp.ts.logC);
 	auto* krs = rev_kernel<T>(dual, tp.ts.K + tp.ts.logC, true);
@@ -1116,7 +1121,8 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 			// pass 0 applies its gates to the tiles that hold an identity element only
 			const double fwd_amps = (double)(flags & 1 ? active_tiles(tp.args_fwd[p], pl.B) : tp.ntiles_fwd) * Ef;
 			const double fwd_flops = (dual ? 96.0 : 32.0) * (double)tp.passes[p].gates.size() * fwd_amps;
-			LaunchTimer lt(s, RQC_KERNEL_FORWARD, (double)(ns * (p > 0) + ns) * ab, fwd_flops, fwd_flops);
+			LaunchTimer lt(s, RQC_KERNEL_FORWARD, (double)(ns * (p > 0) + ns) * ab, fwd_flops,
+				fwd_flops * app_issued / (dual ? 96.0 : 32.0));
 			kf<<<tp.gfwd, kTileThreads, tp.lds_fwd, s.stream>>>(in, out, (dual && p > 0) ? DPS(p) : nullptr,
 				dual ? DPS(p + 1) : nullptr, tp.args_fwd[p], mats, fterms, B, b0, tp.ntiles_fwd, flags, skip);
 		}
@@ -1150,15 +1156,16 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 			{
 				const int reads = dual ? 3 + (p < P - 1) : 2;
 				// algorithmic work of the launch: per amplitude and gate of the tiles it processes,
-				// the adjoint update and the outer products (dual 24 complex MACs, plain 8), the
-				// recovery besides (dual 12, plain 4), and a fused forward pass (12)
+				// the adjoint update and the outer products (dual 24 complex MACs, plain 8), and a
+				// fused forward pass (12); issued besides: the recovery, in the forms above
 				const double frac = (flags & 32) ? (double)active_tiles(tp.args[p], pl.B) / (double)ntiles : 1.0;
 				const double amps = frac * (double)ntiles * E;
 				const double rflops = (dual ? 192.0 : 64.0) * ng * amps;
-				const double rissued = (dual ? 288.0 : 96.0) * ng * amps;
+				const double rissued = (2.0 * app_issued + outer_issued) * ng * amps;
 				const double fflops = with_fwd ? 96.0 * ng * (double)ntiles * E : 0.0;
+				const double fissued = with_fwd ? app_issued * ng * (double)ntiles * E : 0.0;
 				LaunchTimer lt(s, RQC_KERNEL_REVERSE, (double)(reads * frac + ns * (p > 0)) * ab, rflops + fflops,
-					rissued + fflops);
+					rissued + fissued);
 				(seed_pass ? krs : kr)<<<tp.grev, kTileThreads, seed_pass ? tp.lds_rev_seed : tp.lds_rev, s.stream>>>(
 					with_fwd ? PS(p) : PS(p + 1), dual ? (with_fwd ? DPS(p) : DPS(p + 1)) : nullptr,
 					seed_pass ? reinterpret_cast<const cx<T>*>(seed) : phi[cur], dual ? dphi[cur] : nullptr, phi[1 - cur],

@@ -398,10 +398,13 @@ __device__ __forceinline__ double mfma44_na(double a, double b, double c)
 
 // Entry (l & 3, l >> 4) of a block pair [[X, 0], [Y, X]] held by lane l (the
 // pair's matrices in the transposed LDS layout, entry (r, c) at 4 c + r): the A
-// operand of the direct application and the B operand of the transposed one.
+// operand of the direct application and the B operand of the transposed one.  Real and
+// imaginary parts, and for the dual sweeps the sums xa = Re X + Im X, xb = Im X - Re X (ya,
+// yb of Y) of the three-product form below.
 struct Frag44
 {
 	double xr, xi, yr, yi;
+	double xa, xb, ya, yb;
 };
 
 __device__ __forceinline__ Frag44 frag44(const cx<double>* m)
@@ -409,8 +412,16 @@ __device__ __forceinline__ Frag44 frag44(const cx<double>* m)
 	const int lane = threadIdx.x & 63;
 	const int fi = 4 * (lane >> 4) + (lane & 3);
 	const cx<double> x = m[fi], y = m[fi + kMatPairOff];
-	return Frag44{x.x, x.y, y.x, y.y};
+	return Frag44{x.x, x.y, y.x, y.y, x.x + x.y, x.y - x.x, y.x + y.y, y.y - y.x};
 }
+
+// The dual sweeps' complex block products take three real products each (Gauss's form):
+// for M = Mr + i Mi and s = sr + i si,
+//   t = Mr (sr + si),  Re(M s) = t - (Mr + Mi) si,  Im(M s) = t + (Mi - Mr) sr,
+// and the same with the operands' order swapped (the transposed applications).  A dual gate
+// pair, x' = X x and y' = Y x + X y, is 9 block products instead of 12; the sums sr + si cost
+// one vector add per state and group.  (The outer-product sums below keep four products: their
+// long sums then round as the direct complex products do.)
 
 // Direct application to GS groups of 16 quads in the gate layout (lane l:
 // amplitude l >> 4 of quad l & 15; x = primary, y = tangent state): the results
@@ -418,30 +429,32 @@ __device__ __forceinline__ Frag44 frag44(const cx<double>* m)
 template <int GS>
 __device__ __forceinline__ void gate_mfma44(const Frag44& f, GateOperands<double, GS>& o)
 {
-	double d[GS][4];
+	double t[GS][2], d[GS][4];
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][0] = mfma44(f.xr, o.x[n].x, 0.0);
-		d[n][1] = mfma44(f.xi, o.x[n].x, 0.0);
-		d[n][2] = mfma44(f.yr, o.x[n].x, 0.0);
-		d[n][3] = mfma44(f.yi, o.x[n].x, 0.0);
+		const double xs = o.x[n].x + o.x[n].y;
+		t[n][0] = mfma44(f.xr, xs, 0.0);
+		t[n][1] = mfma44(f.yr, xs, 0.0);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][0] = mfma44_na(f.xi, o.x[n].y, d[n][0]);
-		d[n][1] = mfma44(f.xr, o.x[n].y, d[n][1]);
-		d[n][2] = mfma44_na(f.yi, o.x[n].y, d[n][2]);
-		d[n][3] = mfma44(f.yr, o.x[n].y, d[n][3]);
+		const double ys = o.y[n].x + o.y[n].y;
+		t[n][1] = mfma44(f.xr, ys, t[n][1]);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][2] = mfma44(f.xr, o.y[n].x, d[n][2]);
-		d[n][3] = mfma44(f.xi, o.y[n].x, d[n][3]);
+		d[n][0] = mfma44_na(f.xa, o.x[n].y, t[n][0]);
+		d[n][1] = mfma44(f.xb, o.x[n].x, t[n][0]);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][2] = mfma44_na(f.xi, o.y[n].y, d[n][2]);
-		d[n][3] = mfma44(f.xr, o.y[n].y, d[n][3]);
+		d[n][2] = mfma44_na(f.ya, o.x[n].y, t[n][1]);
+		d[n][3] = mfma44(f.yb, o.x[n].x, t[n][1]);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][2] = mfma44_na(f.xa, o.y[n].y, d[n][2]);
+		d[n][3] = mfma44(f.xb, o.y[n].x, d[n][3]);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) { o.acc[n] = double4_t{d[n][0], d[n][1], d[n][2], d[n][3]}; }
@@ -453,30 +466,32 @@ __device__ __forceinline__ void gate_mfma44(const Frag44& f, GateOperands<double
 template <int GS>
 __device__ __forceinline__ void gate_mfma44_t(const Frag44& f, GateOperands<double, GS>& o)
 {
-	double d[GS][4];
+	double t[GS][2], d[GS][4];
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][0] = mfma44(o.x[n].x, f.xr, 0.0);
-		d[n][1] = mfma44(o.x[n].x, f.xi, 0.0);
-		d[n][2] = mfma44(o.x[n].x, f.yr, 0.0);
-		d[n][3] = mfma44(o.x[n].x, f.yi, 0.0);
+		const double xs = o.x[n].x + o.x[n].y;
+		t[n][0] = mfma44(xs, f.xr, 0.0);
+		t[n][1] = mfma44(xs, f.yr, 0.0);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][0] = mfma44_na(o.x[n].y, f.xi, d[n][0]);
-		d[n][1] = mfma44(o.x[n].y, f.xr, d[n][1]);
-		d[n][2] = mfma44_na(o.x[n].y, f.yi, d[n][2]);
-		d[n][3] = mfma44(o.x[n].y, f.yr, d[n][3]);
+		const double ys = o.y[n].x + o.y[n].y;
+		t[n][1] = mfma44(ys, f.xr, t[n][1]);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][2] = mfma44(o.y[n].x, f.xr, d[n][2]);
-		d[n][3] = mfma44(o.y[n].x, f.xi, d[n][3]);
+		d[n][0] = mfma44_na(o.x[n].y, f.xa, t[n][0]);
+		d[n][1] = mfma44(o.x[n].x, f.xb, t[n][0]);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][2] = mfma44_na(o.y[n].y, f.xi, d[n][2]);
-		d[n][3] = mfma44(o.y[n].y, f.xr, d[n][3]);
+		d[n][2] = mfma44_na(o.x[n].y, f.ya, t[n][1]);
+		d[n][3] = mfma44(o.x[n].x, f.yb, t[n][1]);
+	}
+#pragma unroll
+	for (int n = 0; n < GS; n++) {
+		d[n][2] = mfma44_na(o.y[n].y, f.xa, d[n][2]);
+		d[n][3] = mfma44(o.y[n].x, f.xb, d[n][3]);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) { o.acc[n] = double4_t{d[n][0], d[n][1], d[n][2], d[n][3]}; }
@@ -516,7 +531,7 @@ __device__ __forceinline__ Frag44 frag44x(const cx<double>* m)
 {
 	const int lane = threadIdx.x & 63;
 	const cx<double> x = m[4 * (lane >> 4) + (lane & 3)];
-	return Frag44{x.x, x.y, 0.0, 0.0};
+	return Frag44{x.x, x.y, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 }
 
 template <int GS>

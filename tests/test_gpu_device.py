@@ -39,9 +39,12 @@ def test_bandwidth_rejects_invalid_arguments():
 
 def test_profile_flops_of_an_hvp_eval():
     """One L=10 depth-5 HVP eval: the reverse launches count 192 flops per amplitude and gate of
-    the tiles they process (288 issued, with the recovery), the forward launches 96.  The first
-    reverse pass skips the tiles without an identity element and the last reverse launch carries
-    the last forward pass, so the reverse counts lie between bounds fixed by the sweep."""
+    the tiles they process, the forward launches 96 (complex MACs at 8 flops).  Issued on the
+    FP64 matrix cores: a dual gate application takes 9 block products per 16 quads (72 flops per
+    amplitude), the outer products 12 (96), so a reverse amplitude-gate issues 240 with the
+    recovery and a forward one 72.  The first reverse pass skips the tiles without an identity
+    element and the last reverse launch carries the last forward pass, so the reverse counts lie
+    between bounds fixed by the sweep."""
     from rqcopt_hpc_amd.workloads import config_inputs
     L, nl = 10, 5
     V, P, vec, Z = config_inputs(L, nl, seed=3)
@@ -63,11 +66,11 @@ def test_profile_flops_of_an_hvp_eval():
     rev_alg, rev_issued = R.profile_query_flops(R.KERNEL_REVERSE)
     fwd_alg, fwd_issued = R.profile_query_flops(R.KERNEL_FORWARD)
     print(f"passes {passes}: reverse {rev_alg:.4g} / {rev_issued:.4g}, forward {fwd_alg:.4g}", flush=True)
-    assert fwd_alg == fwd_issued
+    assert abs(fwd_issued - 0.75 * fwd_alg) <= 1e-12 * fwd_alg
     assert 0 < fwd_alg <= 96 * amp_gates
+    # reverse: alg = 192 r + 96 f, issued = 240 r + 72 f (r reverse, f fused forward amplitude-gates)
+    r = (rev_issued - 0.75 * rev_alg) / 96.0
+    f = (rev_alg - 192.0 * r) / 96.0
     # every reverse gate at least on its tiles with identity elements, at most on all of them
-    assert 192 * amp_gates / 16 < rev_alg <= 192 * amp_gates + 96 * amp_gates
-    # issued = algorithmic + 96 per processed reverse amplitude-gate (the fused forward has none)
-    rev_only = (rev_issued - rev_alg) / 96.0
-    assert rev_only * 192 <= rev_alg + 1
-    assert 1.0 < rev_issued / rev_alg <= 1.5
+    assert amp_gates / 16 < r <= amp_gates * (1 + 1e-12)
+    assert -1e-9 * amp_gates <= f <= amp_gates
