@@ -1078,10 +1078,10 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	cx<T>* dphi[2] = {S + (nck + 2) * sz, S + (nck + 3) * sz};
 	const long ntiles = (N >> tp.ts.K) * (B >> tp.ts.logC);
 	const double E = (double)(1L << (tp.ts.K + tp.ts.logC)), Ef = (double)(1L << (tp.tsf.K + tp.tsf.logC));
-	// matrix-core flops issued per amplitude and gate: a gate application (FP64 4 x 4 blocks, the
-	// dual pairs in the three-product form: 9 block products per 16 quads; FP32 the real 16 x 16
+	// matrix-core flops issued per amplitude and gate: a gate application (FP64 4 x 4 blocks in the
+	// three-product form: dual pairs 9 block products per 16 quads, plain 3; FP32 the real 16 x 16
 	// form) and the outer-product sums
-	const double app_issued = (sizeof(T) == 8) ? (dual ? 72.0 : 32.0) : (dual ? 128.0 : 64.0);
+	const double app_issued = (sizeof(T) == 8) ? (dual ? 72.0 : 24.0) : (dual ? 128.0 : 64.0);
 	const double outer_issued = (sizeof(T) == 8) ? (dual ? 96.0 : 32.0) : (dual ? 128.0 : 64.0);
 	auto* kf = fwd_kernel<T>(dual, tp.tsf.K + tp.tsf.logC);
 	auto* kr = rev_kernel<T>(dual, tp.ts.K + tp.ts.logC);

@@ -525,32 +525,30 @@ __device__ __forceinline__ void outer44(const GateOperands<double, GS>& s, const
 }
 
 // Plain sweeps (no tangent state): x and y are two groups of quads, both
-// multiplied by the block pair's X (8 block products instead of the real
-// 16 x 16 form's 16); Frag44 from frag44x, whose y part is unused.
+// multiplied by the block pair's X, in the three-product form (6 block products
+// instead of the real 16 x 16 form's 16); Frag44 from frag44x, whose y part is unused.
 __device__ __forceinline__ Frag44 frag44x(const cx<double>* m)
 {
 	const int lane = threadIdx.x & 63;
 	const cx<double> x = m[4 * (lane >> 4) + (lane & 3)];
-	return Frag44{x.x, x.y, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+	return Frag44{x.x, x.y, 0.0, 0.0, x.x + x.y, x.y - x.x, 0.0, 0.0};
 }
 
 template <int GS>
 __device__ __forceinline__ void gate_mfma44_pair(const Frag44& f, GateOperands<double, GS>& o)
 {
-	double d[GS][4];
+	double t[GS][2], d[GS][4];
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][0] = mfma44(f.xr, o.x[n].x, 0.0);
-		d[n][1] = mfma44(f.xi, o.x[n].x, 0.0);
-		d[n][2] = mfma44(f.xr, o.y[n].x, 0.0);
-		d[n][3] = mfma44(f.xi, o.y[n].x, 0.0);
+		t[n][0] = mfma44(f.xr, o.x[n].x + o.x[n].y, 0.0);
+		t[n][1] = mfma44(f.xr, o.y[n].x + o.y[n].y, 0.0);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][0] = mfma44_na(f.xi, o.x[n].y, d[n][0]);
-		d[n][1] = mfma44(f.xr, o.x[n].y, d[n][1]);
-		d[n][2] = mfma44_na(f.xi, o.y[n].y, d[n][2]);
-		d[n][3] = mfma44(f.xr, o.y[n].y, d[n][3]);
+		d[n][0] = mfma44_na(f.xa, o.x[n].y, t[n][0]);
+		d[n][1] = mfma44(f.xb, o.x[n].x, t[n][0]);
+		d[n][2] = mfma44_na(f.xa, o.y[n].y, t[n][1]);
+		d[n][3] = mfma44(f.xb, o.y[n].x, t[n][1]);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) { o.acc[n] = double4_t{d[n][0], d[n][1], d[n][2], d[n][3]}; }
@@ -559,20 +557,18 @@ __device__ __forceinline__ void gate_mfma44_pair(const Frag44& f, GateOperands<d
 template <int GS>
 __device__ __forceinline__ void gate_mfma44_pair_t(const Frag44& f, GateOperands<double, GS>& o)
 {
-	double d[GS][4];
+	double t[GS][2], d[GS][4];
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][0] = mfma44(o.x[n].x, f.xr, 0.0);
-		d[n][1] = mfma44(o.x[n].x, f.xi, 0.0);
-		d[n][2] = mfma44(o.y[n].x, f.xr, 0.0);
-		d[n][3] = mfma44(o.y[n].x, f.xi, 0.0);
+		t[n][0] = mfma44(o.x[n].x + o.x[n].y, f.xr, 0.0);
+		t[n][1] = mfma44(o.y[n].x + o.y[n].y, f.xr, 0.0);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) {
-		d[n][0] = mfma44_na(o.x[n].y, f.xi, d[n][0]);
-		d[n][1] = mfma44(o.x[n].y, f.xr, d[n][1]);
-		d[n][2] = mfma44_na(o.y[n].y, f.xi, d[n][2]);
-		d[n][3] = mfma44(o.y[n].y, f.xr, d[n][3]);
+		d[n][0] = mfma44_na(o.x[n].y, f.xa, t[n][0]);
+		d[n][1] = mfma44(o.x[n].x, f.xb, t[n][0]);
+		d[n][2] = mfma44_na(o.y[n].y, f.xa, t[n][1]);
+		d[n][3] = mfma44(o.y[n].x, f.xb, t[n][1]);
 	}
 #pragma unroll
 	for (int n = 0; n < GS; n++) { o.acc[n] = double4_t{d[n][0], d[n][1], d[n][2], d[n][3]}; }
