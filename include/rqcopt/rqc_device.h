@@ -107,8 +107,10 @@ int rqc_profile_query(int kind, double* total_ms, long* launches, double* alg_by
 /* algorithmic flops of the timed tile-pass launches of one kind (per amplitude and gate of the
    tiles a launch processes: forward 96, reverse 192 for the adjoint update and the outer
    products, plain sweeps a third of these; a reverse launch that carries a forward pass adds
-   that pass's), and the flops they issue (the reverse passes' state recovery besides: 96 per
-   amplitude and gate, plain 32).  No reference counterpart: the library's own measurement. */
+   that pass's), and the flops the matrix cores issue for them: the reverse passes' state
+   recovery besides, in the forms the kernels use (FP64 dual: forward 72, reverse 240, with three
+   real products per complex gate product; FP32 the real 16 x 16 form: forward 128, reverse 384).
+   No reference counterpart: the library's own measurement. */
 int rqc_profile_query_flops(int kind, double* alg_flops, double* issued_flops);
 
 /* device-to-device copy of `bytes` bytes by a plain streaming kernel (reads
