@@ -750,12 +750,15 @@ template <typename T>
 static decltype(&k_tile_reverse<T, true, 0>) rev_kernel(bool dual, int logE, bool seed = false)
 {
 	if (!dual) { return k_tile_reverse<T, false, 0>; }
-	if constexpr (sizeof(T) == 8) {
-		if (seed) {
+	if (seed) {
+		if constexpr (sizeof(T) == 8) {
 			if (logE == 10) { return k_tile_reverse<T, true, 10, true>; }
 			if (logE == 9) { return k_tile_reverse<T, true, 9, true>; }
-			return k_tile_reverse<T, true, 0, true>;
+		} else {
+			if (logE == 11) { return k_tile_reverse<T, true, 11, true>; }
+			if (logE == 10) { return k_tile_reverse<T, true, 10, true>; }
 		}
+		return k_tile_reverse<T, true, 0, true>;
 	}
 	if constexpr (sizeof(T) == 8) {
 		if (logE == 10) { return k_tile_reverse<T, true, 10>; }
@@ -1095,16 +1098,17 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 	const int logB = ilog2_or_neg(B);
 	const double ab = (double)sz * sizeof(cx<T>);   // bytes of one state block
 	const int ns = dual ? 2 : 1;                     // state arrays per side
-	// FP64 HVP sweeps over the whole of a dense target in one block: the first reverse pass
-	// reads U's block as its output adjoint, forms the seed -conj(U) in LDS and takes the
-	// trace on the way (no separate target pass over U, psi and the seed)
-	const bool fuse_trace = sizeof(T) == 8 && dual && be == nullptr && !pl.sampled && B == N && P >= 1;
+	// HVP sweeps over the whole of a dense target in one block: the first reverse pass reads
+	// U's block as its output adjoint (FP64: into LDS; FP32: from HBM into registers, U staying
+	// FP64), forms the seed -conj(U) in LDS and takes the trace on the way (no separate target
+	// pass over U, psi and the seed)
+	const bool fuse_trace = dual && be == nullptr && !pl.sampled && B == N && P >= 1;
 	double* ttr = cpart + (size_t)kReduceChunks * kMaxTileGates * 256;   // its per-wave trace partials
-	// and, with the planned FP64 tiles (one tile for both directions), it also runs the circuit's
+	// and, with the planned tiles (one tile for both directions), it also runs the circuit's
 	// last forward pass on its tiles first: the circuit's output states never go through HBM
 	const int logEr = tp.ts.K + tp.ts.logC;
-	const bool fuse_fwd = fuse_trace && P >= 2 && tp.tsf.K == tp.ts.K && tp.tsf.logC == tp.ts.logC &&
-		(logEr == 9 || logEr == 10);
+	const bool planned_tile = (sizeof(T) == 8) ? (logEr == 9 || logEr == 10) : (logEr == 10 || logEr == 11);
+	const bool fuse_fwd = fuse_trace && P >= 2 && tp.tsf.K == tp.ts.K && tp.tsf.logC == tp.ts.logC && planned_tile;
 	const int Pf = fuse_fwd ? P - 1 : P;   // forward launches
 
 	for (long blk = 0; blk < pl.nblk; blk++) {
@@ -1154,7 +1158,8 @@ static void run_tiles(Slot& s, const EvalPlan& pl, const zc* dU, zc* d_acc, zc* 
 			if (with_fwd) { flags |= 16; }
 			const int ng = (int)tp.passes[p].gates.size();
 			{
-				const int reads = dual ? 3 + (p < P - 1) : 2;
+				// (the FP32 seed pass reads U in FP64: twice an FP32 array)
+				const int reads = (dual ? 3 + (p < P - 1) : 2) + ((seed_pass && sizeof(T) == 4) ? 1 : 0);
 				// algorithmic work of the launch: per amplitude and gate of the tiles it processes,
 				// the adjoint update and the outer products (dual 24 complex MACs, plain 8), and a
 				// fused forward pass (12); issued besides: the recovery, in the forms above

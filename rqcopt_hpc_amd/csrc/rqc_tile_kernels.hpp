@@ -1067,7 +1067,8 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 	auto prefetch = [&](long tile, int b) {
 		const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
 		tile_issue(state(b, 0), psi_out, base, io);
-		tile_issue(state(b, NS / 2), phi_out, base, io);
+		// (the FP32 seed pass reads the FP64 target's tile itself, in its seed step)
+		if constexpr (!(SEED && sizeof(T) == 4)) { tile_issue(state(b, NS / 2), phi_out, base, io); }
 		if (DUAL) {
 			tile_issue(state(b, 1), dpsi_out, base, io);
 			if (flags & 4) { tile_fill(state(b, 3), io, cx<T>{0, 0}); }
@@ -1410,7 +1411,7 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 		// SEED: phi_out holds the target U's tile; the adjoint seed is -conj(U), formed in
 		// place, and the target's trace sum(-conj(U) psi) is taken over the tile on the way
 		// (per thread and wave, then added to the wave's running sum)
-		static_assert(!SEED || (sizeof(T) == 8 && DUAL), "the seed pass is an FP64 dual sweep");
+		static_assert(!SEED || DUAL, "the seed pass is a dual sweep");
 		// (the running trace sum of a wave sits in LDS after the gate matrices, so that no
 		// register is held across the gates for it)
 		double* tracc = reinterpret_cast<double*>(lds_raw + rev_buffers<T, DUAL>() * NS * sbytes +
@@ -1422,14 +1423,17 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 		// the pass input, and the pass's gates are applied to them here first, so that the
 		// circuit's output states never go through HBM (its last forward pass, fused)
 		cx<T>* lmf = reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(tracc - 2 * (threadIdx.x >> 6)) + kRevTraceBytes);
-		if constexpr (SEED && F44) {
+		if constexpr (SEED && FIXED) {
 			if (flags & 16) {
 				const int whichf[2] = {kMatG, kMatZ};
 				load_pass_mats(mats, P.x0, P.ng, whichf, 2, fwd_mat_stride<true>(), lmf);
 			}
 		}
 		auto forward_gates = [&]() {
-			if constexpr (SEED && F44) {
+			if constexpr (SEED && FIXED) {
+				// the forward pass's steps: FP64 the 4 x 4 blocks, FP32 the real 16 x 16 form with
+				// amplitude-major rows (the forward kernel's gate code, so that the states are the
+				// ones a forward launch would write)
 				constexpr int GSF = 2, gqf = 16;
 				constexpr int NSTF = ((1 << LOGE) / 4 / kTileWaves) / (GSF * gqf);
 				static_assert(NSTF >= 1 && NSTF <= 4, "one to four steps per gate and wave");
@@ -1437,11 +1441,20 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 					constexpr int j = decltype(jc)::value;
 					if (j < P.ng) {
 						if (j > 0 && ((P.barrier >> j) & 1)) { __syncthreads(); }
-						const Frag44 f = frag44(reinterpret_cast<const cx<double>*>(lmf + j * fwd_mat_stride<true>()));
+						Frag44 f;
+						T af[4];
+						if constexpr (F44) {
+							f = frag44(reinterpret_cast<const cx<double>*>(lmf + j * fwd_mat_stride<true>()));
+						} else {
+							const AOperandMap<T, true> amf = a_operand_map<T, true, true>();
+							const T* mjf = reinterpret_cast<const T*>(lmf + j * fwd_mat_stride<true>());
+#pragma unroll
+							for (int kk = 0; kk < 4; kk++) { af[kk] = mjf[amf.aoff[kk]] * amf.afac[kk]; }
+						}
 						int ea = eabj[j];
 						asm volatile("" : "+v"(ea));
-						__builtin_assume((ea & 15) == 0);
-						GateOperands<double, GSF> o[NSTF];
+						if constexpr (F44) { __builtin_assume((ea & 15) == 0); }
+						GateOperands<T, GSF> o[NSTF];
 						unsigned la[NSTF][GSF];
 						auto load_step = [&](auto sc) {
 							constexpr int SI = decltype(sc)::value;
@@ -1451,8 +1464,13 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 								const int gb = sgj[j][0] ^ ((gi & 1) ? sgj[j][1] : 0) ^ ((gi & 2) ? sgj[j][2] : 0) ^
 								               ((gi & 4) ? sgj[j][3] : 0);
 								la[SI][n] = (unsigned)(ea ^ gb);
-								o[SI].x[n] = lds_load_cxd(la[SI][n]);
-								o[SI].y[n] = lds_load_cxd(la[SI][n] + sbytes);
+								if constexpr (F44) {
+									o[SI].x[n] = lds_load_cxd(la[SI][n]);
+									o[SI].y[n] = lds_load_cxd(la[SI][n] + sbytes);
+								} else {
+									o[SI].x[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[SI][n]);
+									o[SI].y[n] = *reinterpret_cast<const cx<T>*>(lds_raw + la[SI][n] + sbytes);
+								}
 							}
 						};
 						auto step = [&](auto sc) {
@@ -1460,12 +1478,19 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 							if constexpr (SI + 1 < NSTF) { load_step(std::integral_constant<int, SI + 1>{}); }
 							__builtin_amdgcn_sched_barrier(0);
 							__builtin_amdgcn_s_setprio(1);
-							gate_mfma44<GSF>(f, o[SI]);
+							if constexpr (F44) { gate_mfma44<GSF>(f, o[SI]); }
+							else { gate_mfma<T, GSF>(af, o[SI]); }
 							__builtin_amdgcn_s_setprio(0);
 #pragma unroll
 							for (int n = 0; n < GSF; n++) {
-								lds_store_cxd(la[SI][n], cx<double>{o[SI].acc[n][0], o[SI].acc[n][1]});
-								lds_store_cxd(la[SI][n] + sbytes, cx<double>{o[SI].acc[n][2], o[SI].acc[n][3]});
+								if constexpr (F44) {
+									lds_store_cxd(la[SI][n], cx<double>{o[SI].acc[n][0], o[SI].acc[n][1]});
+									lds_store_cxd(la[SI][n] + sbytes, cx<double>{o[SI].acc[n][2], o[SI].acc[n][3]});
+								} else {
+									*reinterpret_cast<cx<T>*>(lds_raw + la[SI][n]) = cx<T>{o[SI].acc[n][0], o[SI].acc[n][1]};
+									*reinterpret_cast<cx<T>*>(lds_raw + la[SI][n] + sbytes) =
+										cx<T>{o[SI].acc[n][2], o[SI].acc[n][3]};
+								}
 							}
 						};
 						load_step(std::integral_constant<int, 0>{});
@@ -1488,10 +1513,25 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 		while (tile < ntiles) {
 			__builtin_amdgcn_s_waitcnt(0);
 			__syncthreads();
-			if constexpr (SEED && F44) {
+			if constexpr (SEED && FIXED) {
 				if (flags & 16) {
 					forward_gates();
 					__syncthreads();
+				}
+			}
+			// FP32 seed pass: the target's FP64 elements of the thread's slots, from HBM (held
+			// across the forward gates they would take registers the gates need)
+			constexpr int EPL = TileIO<T>::epl;
+			cx<double> useed[(SEED && sizeof(T) == 4) ? kMaxSlots * EPL : 1];
+			if constexpr (SEED && sizeof(T) == 4) {
+				const cx<double>* Ud = reinterpret_cast<const cx<double>*>(phi_out);
+				const long base = deposit_bits(tile >> P.lognc, P.rmask) * B + ((tile & cmask) << P.logC);
+#pragma unroll
+				for (int k = 0; k < kMaxSlots; k++) {
+					if (k < io.nk) {
+#pragma unroll
+						for (int i = 0; i < EPL; i++) { useed[k * EPL + i] = Ud[base + io.goff[k] + i]; }
+					}
 				}
 			}
 			if constexpr (SEED) {
@@ -1499,14 +1539,19 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 #pragma unroll
 				for (int k = 0; k < kMaxSlots; k++) {
 					if (k < io.nk) {
-						const int si = threadIdx.x + k * kTileThreads;   // one FP64 element per slot
-						cx<T>* ph = state(0, NS / 2) + si;
-						const cx<T> u = *ph;
-						const cx<T> ps = state(0, 0)[si];
-						const cx<T> sd = {-u.x, u.y};
-						*ph = sd;
-						tr_re += sd.x * ps.x - sd.y * ps.y;
-						tr_im += sd.x * ps.y + sd.y * ps.x;
+#pragma unroll
+						for (int i = 0; i < EPL; i++) {
+							const int e = (threadIdx.x + k * kTileThreads) * EPL + i;   // the slot's elements
+							cx<T>* ph = state(0, NS / 2) + e;
+							cx<double> u;
+							if constexpr (sizeof(T) == 4) { u = useed[k * EPL + i]; }
+							else { u = cx<double>{(double)ph->x, (double)ph->y}; }
+							const cx<double> ps = {(double)state(0, 0)[e].x, (double)state(0, 0)[e].y};
+							const cx<double> sd = {-u.x, u.y};
+							*ph = cx<T>{(T)sd.x, (T)sd.y};
+							tr_re += sd.x * ps.x - sd.y * ps.y;
+							tr_im += sd.x * ps.y + sd.y * ps.x;
+						}
 					}
 				}
 #pragma unroll
