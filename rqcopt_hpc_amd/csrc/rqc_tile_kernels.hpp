@@ -997,10 +997,11 @@ __global__ __launch_bounds__(kTileThreads, (fwd_waves_per_simd<T, DUAL, LOGE>())
 // reverse pass.  Inputs: psi (and dpsi) at the pass output (checkpoint), phi
 // (and dphi) at the pass output; outputs phi (and dphi) at the pass input.
 // flags: 4 = dphi at the pass output is zero, 8 = do not write phi at the pass input,
-// 16 (SEED, FP64 planned tiles) = psi and dpsi are given at the pass input and the pass's
+// 16 (SEED, planned tiles) = psi and dpsi are given at the pass input and the pass's
 // gates are applied to them first (G, Z after the trace sums in LDS), 32 (with 8) = the
 // sweep started from identity columns b0 + j: tiles without an identity element are skipped.
-// SEED (FP64 dual sweeps, the circuit's last pass): phi_out is the target U's column block;
+// SEED (dual sweeps, the circuit's last pass): phi_out is the target U's column block (FP64 in
+// both precisions; FP32 reads it from HBM in the seed step, FP64 loads it as the adjoint tile);
 // the seed -conj(U) is formed in LDS and the trace sum(-conj(U) psi) goes to
 // trace[2 (block * waves + wave)] (a separate instantiation, so that the other passes run
 // exactly the kernel without it: the extra loop code measured 8 % slower on every pass).
@@ -1419,7 +1420,7 @@ __global__ __launch_bounds__(kTileThreads, (rev_waves_per_simd<T, DUAL>())) void
 		if constexpr (SEED) {
 			if (lane == 0) { tracc[0] = 0; tracc[1] = 0; }
 		}
-		// SEED with flags & 16 (FP64 planned tiles): psi_out and dpsi_out hold the states at
+		// SEED with flags & 16 (planned tiles): psi_out and dpsi_out hold the states at
 		// the pass input, and the pass's gates are applied to them here first, so that the
 		// circuit's output states never go through HBM (its last forward pass, fused)
 		cx<T>* lmf = reinterpret_cast<cx<T>*>(reinterpret_cast<char*>(tracc - 2 * (threadIdx.x >> 6)) + kRevTraceBytes);
